@@ -1,0 +1,122 @@
+/* wbmhd: the ideal-MHD step on one AMD MI355X (gfx950), C ABI.
+ *
+ * Every entry point takes device pointers, 64-bit extents, doubles and a HIP
+ * stream handle (hipStream_t passed as void*), enqueues its kernels on that
+ * stream and returns 0, or a negative code: WB_EINVAL for arguments it rejects,
+ * -(hipError_t) for a HIP launch error.  Nothing is allocated or freed: the caller
+ * owns every buffer, work arrays included.  Calls are asynchronous; the caller
+ * synchronises the stream before it reads a result on the host.
+ *
+ * Arrays are C order with the last index contiguous, one plane per variable.  A
+ * haloed 2D block of nx x ny cells is X = nx+4 by Y = ny+4 (halo 2, carafe
+ * mhd.py:128); a 3D block X x Y x Z likewise.
+ */
+#ifndef WBMHD_H
+#define WBMHD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WB_EINVAL (-100000)
+
+/* ---- 2D per-kernel twins: same arguments as the reference's kernels ------ */
+
+/* Replaces kernels/mhd_primitive/mhd_primitive_pyccel.py:23-40 (mhd_primitive).
+ * In: the 6 conserved planes (X, Y) and az (X+1, Y+1).  Out: prim (8, X, Y),
+ * bxf (X+1, Y), byf (X, Y+1), ecc (X, Y) and *bad (device int32), the number of
+ * cells with a non-positive density or pressure or a non-finite variable. */
+int wb_primitive2d(const double* rho, const double* mx, const double* my, const double* mz,
+                   const double* energy, const double* bz, const double* az, int64_t X, int64_t Y,
+                   double dx, double dy, double b0x, double b0y, double gamma, double* prim,
+                   double* bxf, double* byf, double* ecc, int32_t* bad, void* stream);
+
+/* Replaces kernels/mhd_sweep/mhd_sweep_pyccel.py:254-261 (mhd_sweep).  limiter:
+ * 0 minmod, 1 vanleer, 2 mc, 3 none (kernels/mhd_sweep/__init__.py:13).
+ * bn: (X+1, Y) for axis 0, (X, Y+1) for axis 1.  flux: (7, X-3, Y) or (7, X, Y-3). */
+int wb_sweep2d(const double* prim, const double* bn, int64_t X, int64_t Y, int axis, int limiter,
+               double gamma, double* flux, void* stream);
+
+/* Replaces kernels/ct_contact/ct_contact_pyccel.py:26-33 (ct_contact).  fx_*:
+ * (X-3, Y), gy_*: (X, Y-3), ecc: (X, Y); ez: (X-3, Y-3) = (nx+1, ny+1). */
+int wb_ct_contact(const double* fx_by, const double* gy_bx, const double* fx_rho,
+                  const double* gy_rho, const double* ecc, int64_t X, int64_t Y, double* ez,
+                  void* stream);
+
+/* Replaces kernels/mhd_rate/mhd_rate_pyccel.py:30-45 (mhd_rate).  out: device
+ * double[2] = largest rate, smallest density of the block's own cells. */
+int wb_rate2d(const double* rho, const double* mx, const double* my, const double* mz,
+              const double* energy, const double* bz, const double* az, int64_t X, int64_t Y,
+              int halo, double dx, double dy, double b0x, double b0y, double gamma, double* out,
+              void* stream);
+
+/* ---- 2D stage pieces (MHD2D._fill_halos / _patch_fluxes / _advance) ------ */
+
+/* State buffer of a 2D patch: 6 cell planes (X*Y each) then az ((X+1)*(Y+1)).
+ * Replaces the halo exchange of one patch covering its level plus the domain
+ * faces, carafe mhd.py:744-777.  periodic_mask / wall_mask: bit a for axis a
+ * (a non-periodic axis without its wall bit is an outflow axis). */
+int wb_mhd2d_fill_halos(double* state, int64_t nx, int64_t ny, int periodic_mask, int wall_mask,
+                        void* stream);
+
+/* Wall faces of a patch, mhd.py:873-886: side_mask bits 1 x-lower, 2 x-upper,
+ * 4 y-lower, 8 y-upper zero flux rows 0,2,3,4,6 and the wall row of ez. */
+int wb_mhd2d_walls(double* fx, double* fy, double* ez, int64_t nx, int64_t ny, int side_mask,
+                   void* stream);
+
+/* The conservative update of patchwork's FluxRegister.update as called at
+ * mhd.py:911-922 (u - (dt/dx dFx + dt/dy dFy); A_z - dt E_z) from state `in` into
+ * `out`; heun != 0 fuses the average of mhd.py:983-988: out = (u' + out) * 0.5. */
+int wb_mhd2d_update(const double* in, double* out, const double* fx, const double* fy,
+                    const double* ez, int64_t nx, int64_t ny, double dt, double dx, double dy,
+                    int heun, void* stream);
+
+/* ---- 3D stage pieces (MHD3D) ------------------------------------------- */
+
+/* Number of doubles of a 3D state buffer: 5 cell planes, then A_x (X, Y+1, Z+1),
+ * A_y (X+1, Y, Z+1), A_z (X+1, Y+1, Z). */
+int64_t wb_mhd3d_state_size(int64_t nx, int64_t ny, int64_t nz);
+
+/* Replaces MHD3D._fill_halos, mhd3d.py:448-473 (periodic or outflow axes). */
+int wb_mhd3d_fill_halos(double* state, int64_t nx, int64_t ny, int64_t nz, int periodic_mask,
+                        void* stream);
+
+/* Replaces MHD3D._primitive, mhd3d.py:475-509 (+ _face_fields/_cell_fields
+ * :361-386): prim (8, X, Y, Z), face fields fbx (X+1,Y,Z), fby (X,Y+1,Z),
+ * fbz (X,Y,Z+1); *bad counts the bad inner cells (the caller raises). */
+int wb_primitive3d(const double* state, int64_t nx, int64_t ny, int64_t nz, double dx, double dy,
+                   double dz, double b0x, double b0y, double b0z, double gamma, double* prim,
+                   double* fbx, double* fby, double* fbz, int32_t* bad, void* stream);
+
+/* Replaces MHD3D._sweep, mhd3d.py:511-528 (mhd_sweep in the cyclic frame):
+ * flux (7, ...) in the physical layout with n-3 faces along `axis`. */
+int wb_sweep3d(const double* prim, const double* bn, int64_t nx, int64_t ny, int64_t nz, int axis,
+               int limiter, double gamma, double* flux, void* stream);
+
+/* Replaces one edge direction c of MHD3D._edge_field, mhd3d.py:530-559:
+ * flux_a / flux_b are the sweeps along c+1 and c+2; e gets E_c on the block's
+ * c-edges in the physical layout (n_c along c, n+1 along the other two). */
+int wb_edge3d(const double* prim, const double* flux_a, const double* flux_b, int64_t nx,
+              int64_t ny, int64_t nz, int c, double* e, void* stream);
+
+/* The update of mhd3d.py:653-663 (register.update x5, A_c -= dt E_c) from `in`
+ * into `out`, heun as in wb_mhd2d_update (mhd3d.py:742-747). */
+int wb_mhd3d_update(const double* in, double* out, const double* fx, const double* fy,
+                    const double* fz, const double* ex, const double* ey, const double* ez,
+                    int64_t nx, int64_t ny, int64_t nz, double dt, double dx, double dy, double dz,
+                    int heun, void* stream);
+
+/* Replaces the rate of MHD3D.stable_step, mhd3d.py:674-724: out = device
+ * double[2], largest rate and smallest density. */
+int wb_rate3d(const double* state, int64_t nx, int64_t ny, int64_t nz, double dx, double dy,
+              double dz, double b0x, double b0y, double b0z, double gamma, double* out,
+              void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WBMHD_H */

@@ -1,0 +1,88 @@
+"""Write the golden vectors of the two kernels that no closed form pins.
+
+The HLLD sweep's star and double-star branches with the limiters and the
+first-order fallback, and the CT-contact upwind weights, are not fixed by any
+closed form or invariant that the reference's tests hold.  This script runs the
+reference's own scalar kernels -- ``kernels/mhd_sweep/mhd_sweep_pyccel.py`` and
+``kernels/ct_contact/ct_contact_pyccel.py``, which import only ``math`` and
+``numpy`` -- as plain Python on the seeded blocks of the reference's kernel test
+(``tests/unit/test_mhd_kernels.py:17-33,54-76``) and stores inputs and outputs in
+``golden.npz`` next to this file.  Run it only where the reference tree exists:
+
+    python tests/golden/make_golden.py /root/reference
+"""
+
+from __future__ import annotations
+
+import importlib.util
+import sys
+from pathlib import Path
+
+import numpy as np
+
+SHAPE = (13, 9)
+GAMMA = 5.0 / 3.0
+
+
+def _load(path: Path, name: str):
+    spec = importlib.util.spec_from_file_location(name, path)
+    module = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(module)
+    return module
+
+
+def block(seed=1):
+    """The seeded block of test_mhd_kernels.py:17-33."""
+    rng = np.random.default_rng(seed)
+    nx, ny = SHAPE
+    prim = rng.normal(size=(8, nx, ny))
+    prim[0] = rng.uniform(0.5, 2.0, SHAPE)
+    prim[4] = rng.uniform(0.1, 1.0, SHAPE)
+    prim[1, :, 6] += 30.0
+    prim[2, 8, :] -= 30.0
+    prim[4, 5, 3] = 1.0e-3
+    bxf = rng.normal(size=(nx + 1, ny))
+    byf = rng.normal(size=(nx, ny + 1))
+    bxf[4, 2] = byf[2, 4] = 0.0
+    return prim, bxf, byf
+
+
+def ct_inputs(seed=2):
+    """The seeded fluxes of test_mhd_kernels.py:66-76."""
+    rng = np.random.default_rng(seed)
+    nx, ny = SHAPE
+    fx = rng.normal(size=(7, nx - 3, ny))
+    fy = rng.normal(size=(7, nx, ny - 3))
+    fx[0, 3, 4] = fy[0, 2, 5] = 0.0
+    ecc = rng.normal(size=SHAPE)
+    return fx, fy, ecc
+
+
+def main(reference: Path) -> None:
+    kernels = reference / "src" / "carafe" / "kernels"
+    sweep = _load(kernels / "mhd_sweep" / "mhd_sweep_pyccel.py", "ref_mhd_sweep")
+    ct = _load(kernels / "ct_contact" / "ct_contact_pyccel.py", "ref_ct_contact")
+    out: dict[str, np.ndarray] = {}
+    nx, ny = SHAPE
+    for seed in (1, 7):
+        prim, bxf, byf = block(seed)
+        out[f"prim_{seed}"] = prim
+        out[f"bxf_{seed}"] = bxf
+        out[f"byf_{seed}"] = byf
+        for limiter in range(4):
+            for axis in (0, 1):
+                shape = (7, nx - 3, ny) if axis == 0 else (7, nx, ny - 3)
+                flux = np.full(shape, np.nan)
+                sweep.mhd_sweep(prim, (bxf, byf)[axis], axis, limiter, GAMMA, flux)
+                out[f"flux_{seed}_{axis}_{limiter}"] = flux
+    fx, fy, ecc = ct_inputs()
+    ez = np.full((nx - 3, ny - 3), np.nan)
+    ct.ct_contact(fx[5], fy[5], fx[0], fy[0], ecc, ez)
+    out.update(ct_fx=fx, ct_fy=fy, ct_ecc=ecc, ct_ez=ez)
+    target = Path(__file__).resolve().parent / "golden.npz"
+    np.savez_compressed(target, **out)
+    print(f"wrote {target} ({len(out)} arrays)")
+
+
+if __name__ == "__main__":
+    main(Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference"))

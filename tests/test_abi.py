@@ -1,0 +1,74 @@
+"""The C ABI: the library loads and exports every entry point the header declares
+(no compute calls, so this runs without a GPU)."""
+
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+from wellbalanced_amd import _lib
+
+HEADER = Path(__file__).resolve().parent.parent / "include" / "wbmhd.h"
+
+
+def _declared():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t)\s+(wb_\w+)\s*\(", text, re.M)))
+
+
+def test_header_declares_the_entry_points():
+    names = _declared()
+    assert "wb_sweep2d" in names and "wb_primitive3d" in names
+    assert set(names) == set(_lib.SIGNATURES), "the ctypes table and the header disagree"
+
+
+def test_library_exports_every_declared_symbol():
+    if not _lib.LIBRARY.exists():
+        pytest.skip("libwbmhd.so is not built")
+    lib = ctypes.CDLL(str(_lib.LIBRARY))
+    for name in _declared():
+        assert hasattr(lib, name), name
+
+
+def test_invalid_arguments_are_rejected_without_a_launch():
+    if not _lib.LIBRARY.exists():
+        pytest.skip("libwbmhd.so is not built")
+    lib = _lib.load()
+    assert lib.wb_mhd3d_state_size(1, 4, 4) == _lib.WB_EINVAL
+    assert lib.wb_mhd3d_state_size(4, 4, 4) == 5 * 8**3 + 3 * 8 * 9 * 9
+
+
+def test_the_product_fails_loudly_without_a_gpu():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from wellbalanced_amd import MHD2D
+    from wellbalanced_amd.hierarchy import AMRHierarchy
+
+    with pytest.raises(RuntimeError, match="GPU"):
+        MHD2D(AMRHierarchy((8, 8), periodic=(True, True)))
+
+
+def test_validation_messages_follow_the_reference():
+    from wellbalanced_amd import MHD2D
+    from wellbalanced_amd.hierarchy import AMRHierarchy
+
+    with pytest.raises(ValueError, match="2D"):
+        MHD2D(AMRHierarchy((8,)))
+    with pytest.raises(NotImplementedError, match="Cartesian"):
+        MHD2D(AMRHierarchy((4, 4), ((0.0, 1.0), (0.0, 1.0)), geometry="cylindrical"))
+    with pytest.raises(ValueError, match="limiter"):
+        MHD2D(AMRHierarchy((4, 4)), limiter="superbee")
+    with pytest.raises(ValueError, match="cfl"):
+        MHD2D(AMRHierarchy((4, 4)), cfl=1.5)
+    with pytest.raises(ValueError, match="gamma"):
+        MHD2D(AMRHierarchy((4, 4)), gamma=1.0)
+    periodic = AMRHierarchy((4, 4), periodic=(True, False))
+    with pytest.raises(ValueError, match="periodic"):
+        MHD2D(periodic, boundaries={"x": "wall"})
+    with pytest.raises(ValueError, match="boundaries"):
+        MHD2D(AMRHierarchy((4, 4)), boundaries={"z": "wall"})
+    with pytest.raises(ValueError, match="mean field"):
+        MHD2D(AMRHierarchy((4, 4)), boundaries={"y": "wall"}, mean_field=(0.0, 1.0))

@@ -1,0 +1,81 @@
+"""ctypes binding of ``libwbmhd.so`` (the C ABI of ``include/wbmhd.h``).
+
+The library is loaded from the package folder.  There is no CPU fallback: if it
+is missing or a GPU is not available, the product raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from ctypes import c_double, c_int, c_int64, c_void_p
+from pathlib import Path
+
+LIBRARY = Path(__file__).resolve().parent / "libwbmhd.so"
+WB_EINVAL = -100000
+
+P = c_void_p
+I64 = c_int64
+D = c_double
+I = c_int
+
+#: Argument types of every exported entry point (``include/wbmhd.h``).
+SIGNATURES: dict[str, tuple[type, list]] = {
+    "wb_primitive2d": (I, [P] * 7 + [I64, I64] + [D] * 5 + [P] * 5 + [P]),
+    "wb_sweep2d": (I, [P, P, I64, I64, I, I, D, P, P]),
+    "wb_ct_contact": (I, [P] * 5 + [I64, I64, P, P]),
+    "wb_rate2d": (I, [P] * 7 + [I64, I64, I] + [D] * 5 + [P, P]),
+    "wb_mhd2d_fill_halos": (I, [P, I64, I64, I, I, P]),
+    "wb_mhd2d_walls": (I, [P, P, P, I64, I64, I, P]),
+    "wb_mhd2d_update": (I, [P, P, P, P, P, I64, I64, D, D, D, I, P]),
+    "wb_mhd3d_state_size": (I64, [I64, I64, I64]),
+    "wb_mhd3d_fill_halos": (I, [P, I64, I64, I64, I, P]),
+    "wb_primitive3d": (I, [P, I64, I64, I64] + [D] * 7 + [P] * 5 + [P]),
+    "wb_sweep3d": (I, [P, P, I64, I64, I64, I, I, D, P, P]),
+    "wb_edge3d": (I, [P, P, P, I64, I64, I64, I, P, P]),
+    "wb_mhd3d_update": (I, [P, P] + [P] * 6 + [I64] * 3 + [D] * 4 + [I, P]),
+    "wb_rate3d": (I, [P, I64, I64, I64] + [D] * 7 + [P, P]),
+}
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load ``libwbmhd.so`` once and set the argument types; raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIBRARY.exists():
+        raise RuntimeError(
+            f"{LIBRARY} is not built: run `python -m wellbalanced_amd.build` "
+            "(the HIP path has no CPU fallback)"
+        )
+    lib = ctypes.CDLL(str(LIBRARY))
+    for name, (restype, argtypes) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = restype
+        fn.argtypes = argtypes
+    _lib = lib
+    return lib
+
+
+def call(name: str, *args) -> int:
+    """Call entry point ``name`` and raise ``RuntimeError`` on a negative return."""
+    rc = getattr(load(), name)(*args)
+    if rc < 0 and SIGNATURES[name][0] is I:
+        what = "invalid arguments" if rc == WB_EINVAL else f"HIP error {-rc}"
+        raise RuntimeError(f"{name} failed: {what}")
+    return rc
+
+
+def ptr(tensor) -> int:
+    """Device address of a contiguous torch tensor."""
+    if not tensor.is_contiguous():
+        raise ValueError("wbmhd needs contiguous tensors")
+    return tensor.data_ptr()
+
+
+def stream_handle(device=None) -> int:
+    """The raw HIP stream of torch's current stream on ``device``."""
+    import torch
+
+    return torch.cuda.current_stream(device).cuda_stream

@@ -1,0 +1,332 @@
+"""``MHD2D``: the reference's 2.5D ideal-MHD model with its step on the GPU.
+
+Same constructor, problem set-ups, ``set_state``, ``stable_step``, ``step``,
+``run`` and diagnostics as carafe ``mhd.py:386-1226``.  The state of the patch
+lives in HBM as one FP64 buffer (6 haloed cell planes, then the haloed nodal
+``A_z``); each stage of Heun's method runs as HIP kernels through
+``libwbmhd.so`` (``include/wbmhd.h``).  Two state buffers ping-pong: stage 1 reads
+``U^n`` and writes ``U^1``, stage 2 reads ``U^1`` and writes the Heun average over
+``U^n`` in place, so no start copy is made (``mhd.py:972-988``).
+
+``precision="fp64"`` is the parity path; ``"mixed"`` computes the reconstruction
+and the HLLD flux in FP32 and keeps the primitive conversion, CT, face field,
+update and sums in FP64.
+"""
+
+from __future__ import annotations
+
+from collections.abc import Callable
+from typing import Any
+
+import numpy as np
+import torch
+
+from . import _lib, initial
+from .hierarchy import AMRHierarchy, Patch
+
+HALO = 2
+CONSERVED = ("density", "momentum_x", "momentum_y", "momentum_z", "energy", "bz")
+BOUNDARIES = ("outflow", "wall")
+LIMITER_CODES = ("minmod", "vanleer", "mc", "none")
+PRECISIONS = ("fp64", "mixed")
+
+Value = float | np.ndarray | Callable[..., Any]
+
+
+def _device(device) -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("wellbalanced_amd needs a GPU (HIP device); there is no CPU fallback")
+    dev = torch.device("cuda") if device is None else torch.device(device)
+    if dev.type != "cuda":
+        raise ValueError("the state lives on a HIP device ('cuda' in PyTorch-ROCm)")
+    if dev.index is None:
+        dev = torch.device("cuda", torch.cuda.current_device())
+    return dev
+
+
+class MHD2D:
+    """2.5D ideal MHD on one uniform patch of an `AMRHierarchy` (see the module)."""
+
+    def __init__(self, hierarchy: AMRHierarchy, *, gamma: float = 5.0 / 3.0,
+                 mean_field: tuple[float, float] = (0.0, 0.0), resistivity: float = 0.0,
+                 limiter: str = "vanleer", cfl: float = 0.4,
+                 conduction: tuple[float, float] = (0.0, 0.0),
+                 gravity: tuple[float, float] = (0.0, 0.0),
+                 boundaries: dict[str, str] | None = None, prefix: str = "mhd_",
+                 precision: str = "fp64", device=None) -> None:
+        # validation as mhd.py:425-455
+        if hierarchy.ndim != 2:
+            raise ValueError("MHD2D needs a 2D hierarchy")
+        if hierarchy.cylindrical or hierarchy.graded:
+            raise NotImplementedError(
+                "MHD2D is Cartesian on uniform levels: no r–z metric, no graded axes"
+            )
+        if limiter not in LIMITER_CODES:
+            raise ValueError(f"limiter is one of {sorted(LIMITER_CODES)}")
+        if not 0.0 < cfl <= 1.0:
+            raise ValueError("cfl must be in (0, 1]")
+        if gamma <= 1.0:
+            raise ValueError("gamma must be larger than 1")
+        if not resistivity >= 0.0:
+            raise ValueError("resistivity must be non-negative")
+        if not min(conduction) >= 0.0:
+            raise ValueError("the conductivities must be non-negative")
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision is one of {PRECISIONS}")
+        kinds = {"x": "outflow", "y": "outflow"}
+        for axis_name, kind in (boundaries or {}).items():
+            if axis_name not in kinds or kind not in BOUNDARIES:
+                raise ValueError(
+                    f"boundaries maps 'x' or 'y' to one of {BOUNDARIES}, "
+                    f"got {axis_name!r}: {kind!r}"
+                )
+            axis = "xy".index(axis_name)
+            if hierarchy.periodic[axis]:
+                raise ValueError(f"the hierarchy is periodic along {axis_name}")
+            if kind == "wall" and mean_field[axis] != 0.0:
+                raise ValueError(f"a wall normal to {axis_name} needs a zero mean field along it")
+            kinds[axis_name] = kind
+        if resistivity or max(conduction) > 0.0 or any(gravity):
+            raise NotImplementedError(
+                "the resistive, conductive and gravity terms are outside this ideal-MHD path"
+            )
+        if len(hierarchy.patches) != 1:
+            raise NotImplementedError("refined patches run through wellbalanced_amd.amr.AMRMHD2D")
+        self.hierarchy = hierarchy
+        self.gamma = float(gamma)
+        self.mean_field = (float(mean_field[0]), float(mean_field[1]))
+        self.resistivity = 0.0
+        self.limiter = limiter
+        self.cfl = float(cfl)
+        self.conduction = (0.0, 0.0)
+        self.gravity = (0.0, 0.0)
+        self.walls = tuple(kinds[name] == "wall" for name in "xy")
+        self.periodic = tuple(hierarchy.periodic)
+        self.names = {name: prefix + name for name in CONSERVED}
+        self.potential = prefix + "az"
+        self.precision = precision
+        self.device = _device(device)
+        self.patch: Patch = hierarchy.patches[0]
+        self.nx, self.ny = self.patch.shape
+        self.dx, self.dy = hierarchy.spacing(0)
+        X, Y = self.nx + 2 * HALO, self.ny + 2 * HALO
+        self.X, self.Y = X, Y
+        self._cells = 6 * X * Y
+        size = self._cells + (X + 1) * (Y + 1)
+        opts = dict(dtype=torch.float64, device=self.device)
+        self._buf = [torch.zeros(size, **opts), torch.zeros(size, **opts)]
+        self._prim = torch.empty(8 * X * Y, **opts)
+        self._bxf = torch.empty((X + 1) * Y, **opts)
+        self._byf = torch.empty(X * (Y + 1), **opts)
+        self._ecc = torch.empty(X * Y, **opts)
+        self._fx = torch.empty(7 * (X - 3) * Y, **opts)
+        self._fy = torch.empty(7 * X * (Y - 3), **opts)
+        self._ez = torch.empty((self.nx + 1) * (self.ny + 1), **opts)
+        self._bad = torch.zeros(2, dtype=torch.int32, device=self.device)
+        self._rate = torch.zeros(2, **opts)
+        self.check_interval = 1
+        self.time = 0.0
+        self.steps = 0
+        self.period = float("nan")
+        self._ready = False
+
+    # -- construction of common cases (mhd.py:486-665) --------------------------
+    @classmethod
+    def from_problem(cls, problem, **options: Any) -> MHD2D:
+        """A model of a `wellbalanced_amd.initial.Problem` with its state set."""
+        model = cls(problem.hierarchy, **{**problem.options, **options})
+        model.set_state(**problem.state)
+        model.period = problem.period
+        return model
+
+    @classmethod
+    def alfven_wave(cls, cells: int, *, amplitude: float = 0.1, **options: Any):
+        """The oblique circularly polarised Alfvén wave (mhd.py:486-530) and its exact state."""
+        model = cls.from_problem(initial.alfven_wave(cells, amplitude=amplitude, **options))
+        return model, model.dataset()
+
+    @classmethod
+    def orszag_tang(cls, cells: int, **options: Any) -> MHD2D:
+        """The Orszag–Tang vortex on the periodic unit square (mhd.py:532-559)."""
+        return cls.from_problem(initial.orszag_tang(cells, **options))
+
+    @classmethod
+    def rotor(cls, cells: int, **options: Any) -> MHD2D:
+        """The MHD rotor (mhd.py:561-592)."""
+        return cls.from_problem(initial.rotor(cells, **options))
+
+    @classmethod
+    def blast_wave(cls, cells: int, **options: Any) -> MHD2D:
+        """The magnetised blast wave (mhd.py:594-620)."""
+        return cls.from_problem(initial.blast_wave(cells, **options))
+
+    # -- state ---------------------------------------------------------------------
+    def _views(self, buf: torch.Tensor):
+        cells = buf[: self._cells].view(6, self.X, self.Y)
+        az = buf[self._cells :].view(self.X + 1, self.Y + 1)
+        return cells, az
+
+    @property
+    def state(self) -> torch.Tensor:
+        """The current FP64 state buffer (6 haloed cell planes then haloed A_z)."""
+        return self._buf[0]
+
+    def set_state(self, density: Value, velocity, pressure: Value, *,
+                  vector_potential: Value = 0.0, bz: Value = 0.0) -> None:
+        """Set the state (mhd.py:683-737): functions of the cell centres / nodes or values."""
+        cons, az = initial.host_state_2d(
+            self.hierarchy, self.patch, self.gamma, self.mean_field, density, velocity, pressure,
+            vector_potential=vector_potential, bz=bz,
+        )
+        self.load_arrays(cons, az)
+
+    def load_arrays(self, cons: np.ndarray, az: np.ndarray) -> None:
+        """Upload haloed host arrays ``cons (6, X, Y)`` and ``az (X+1, Y+1)`` as the state."""
+        flat = np.concatenate([np.asarray(cons, dtype=np.float64).ravel(),
+                               np.asarray(az, dtype=np.float64).ravel()])
+        self._buf[0].copy_(torch.from_numpy(flat))
+        self.time = 0.0
+        self.steps = 0
+        self._ready = True
+
+    def arrays(self) -> tuple[np.ndarray, np.ndarray]:
+        """The haloed state on the host: ``cons (6, X, Y)``, ``az (X+1, Y+1)``."""
+        cells, az = self._views(self._buf[0])
+        return cells.cpu().numpy(), az.cpu().numpy()
+
+    def _check_ready(self) -> None:
+        if not self._ready:
+            raise RuntimeError("set the state first (set_state)")
+
+    # -- the step --------------------------------------------------------------------
+    def _wall_mask(self) -> int:
+        mask = 0
+        if self.walls[0]:
+            mask |= 3
+        if self.walls[1]:
+            mask |= 12
+        return mask
+
+    def _stage(self, src: torch.Tensor, dst: torch.Tensor, dt: float, heun: bool, slot: int,
+               stream: int) -> None:
+        X, Y, nx, ny = self.X, self.Y, self.nx, self.ny
+        call, ptr = _lib.call, _lib.ptr
+        pmask = int(self.periodic[0]) | (int(self.periodic[1]) << 1)
+        wmask = int(self.walls[0]) | (int(self.walls[1]) << 1)
+        call("wb_mhd2d_fill_halos", ptr(src), nx, ny, pmask, wmask, stream)
+        base = src.data_ptr()
+        plane = X * Y * 8
+        planes = [base + k * plane for k in range(6)]
+        az = base + self._cells * 8
+        bad = self._bad.data_ptr() + 4 * slot
+        call("wb_primitive2d", *planes, az, X, Y, self.dx, self.dy, *self.mean_field,
+             self.gamma, ptr(self._prim), ptr(self._bxf), ptr(self._byf), ptr(self._ecc),
+             bad, stream)
+        code = LIMITER_CODES.index(self.limiter)
+        call("wb_sweep2d", ptr(self._prim), ptr(self._bxf), X, Y, 0, code, self.gamma,
+             ptr(self._fx), stream)
+        call("wb_sweep2d", ptr(self._prim), ptr(self._byf), X, Y, 1, code, self.gamma,
+             ptr(self._fy), stream)
+        fxp, fyp = self._fx.data_ptr(), self._fy.data_ptr()
+        sx, sy = (X - 3) * Y * 8, X * (Y - 3) * 8
+        call("wb_ct_contact", fxp + 5 * sx, fyp + 5 * sy, fxp, fyp, ptr(self._ecc), X, Y,
+             ptr(self._ez), stream)
+        if self._wall_mask():
+            call("wb_mhd2d_walls", fxp, fyp, ptr(self._ez), nx, ny, self._wall_mask(), stream)
+        call("wb_mhd2d_update", ptr(src), ptr(dst), fxp, fyp, ptr(self._ez), nx, ny, dt,
+             self.dx, self.dy, int(heun), stream)
+
+    def stable_step(self) -> float:
+        """cfl / (largest fast-wave rate) (mhd.py:930-963); synchronises."""
+        self._check_ready()
+        cells, _ = self._views(self._buf[0])
+        base = self._buf[0].data_ptr()
+        plane = self.X * self.Y * 8
+        stream = _lib.stream_handle(self.device)
+        _lib.call("wb_rate2d", *[base + k * plane for k in range(6)], base + self._cells * 8,
+                  self.X, self.Y, HALO, self.dx, self.dy, *self.mean_field, self.gamma,
+                  _lib.ptr(self._rate), stream)
+        largest = float(self._rate[0].item())
+        return self.cfl * (1.0 / largest)
+
+    def step(self, dt: float | None = None) -> float:
+        """Advance by one Heun (SSP-RK2) step (mhd.py:965-994); return the step taken."""
+        self._check_ready()
+        if dt is None:
+            dt = self.stable_step()
+        dt = float(dt)
+        stream = _lib.stream_handle(self.device)
+        a, b = self._buf
+        self._stage(a, b, dt, False, 0, stream)
+        self._stage(b, a, dt, True, 1, stream)
+        self.time += dt
+        self.steps += 1
+        if self.check_interval and self.steps % self.check_interval == 0:
+            self.check()
+        return dt
+
+    def check(self) -> None:
+        """Raise FloatingPointError if a stage met a bad cell (mhd.py:852-857); synchronises."""
+        bad = self._bad.cpu()
+        if int(bad.sum()):
+            raise FloatingPointError(
+                f"non-positive density or pressure on the level-0 patch {self.patch.box} "
+                f"at t = {self.time}"
+            )
+
+    def run(self, duration: float, *, max_steps: int = 1_000_000) -> int:
+        """Advance by ``duration``, the last step shortened (mhd.py:996-1008)."""
+        self._check_ready()
+        if not np.isfinite(duration) or duration < 0.0:
+            raise ValueError("duration must be a finite, non-negative time")
+        end = self.time + duration
+        taken = 0
+        while self.time < end * (1.0 - 1e-14) and taken < max_steps:
+            dt = min(self.stable_step(), end - self.time)
+            self.step(dt)
+            taken += 1
+        return taken
+
+    # -- diagnostics (mhd.py:1059-1226) -------------------------------------------------
+    def totals(self) -> dict[str, float]:
+        """Integrals of the conserved fields (FP64 sums on the device)."""
+        self._check_ready()
+        cells, _ = self._views(self._buf[0])
+        inner = cells[:, HALO : HALO + self.nx, HALO : HALO + self.ny]
+        sums = inner.sum(dim=(1, 2)) * (self.dx * self.dy)
+        return {name: float(v) for name, v in zip(CONSERVED, sums.cpu().tolist())}
+
+    def divergence(self) -> float:
+        """Largest |div B| times the cell size (mhd.py:1114-1131)."""
+        self._check_ready()
+        _, az = self._views(self._buf[0])
+        az = az[HALO : HALO + self.nx + 1, HALO : HALO + self.ny + 1]
+        bxf = (az[:, 1:] - az[:, :-1]) / self.dy + self.mean_field[0]
+        byf = -(az[1:, :] - az[:-1, :]) / self.dx + self.mean_field[1]
+        div = (bxf[1:] - bxf[:-1]) / self.dx + (byf[:, 1:] - byf[:, :-1]) / self.dy
+        return float(div.abs().max()) * min(self.dx, self.dy)
+
+    def dataset(self, level: int = 0) -> dict[str, np.ndarray]:
+        """Primitive variables over the cell centres (mhd.py:1177-1226), as plain arrays."""
+        self._check_ready()
+        if level != 0:
+            raise ValueError("a uniform model has level 0 only")
+        cons, az = self.arrays()
+        h = HALO
+        c = cons[:, h : h + self.nx, h : h + self.ny]
+        a = az[h : h + self.nx + 1, h : h + self.ny + 1]
+        bxf = (a[:, 1:] - a[:, :-1]) / self.dy + self.mean_field[0]
+        byf = -(a[1:, :] - a[:-1, :]) / self.dx + self.mean_field[1]
+        bx, by = 0.5 * (bxf[:-1] + bxf[1:]), 0.5 * (byf[:, :-1] + byf[:, 1:])
+        rho = c[0]
+        u, v, w = c[1] / rho, c[2] / rho, c[3] / rho
+        b2 = bx**2 + by**2 + c[5] ** 2
+        p = (self.gamma - 1.0) * (c[4] - 0.5 * rho * (u**2 + v**2 + w**2) - 0.5 * b2)
+        x, y = self.hierarchy.cell_centers(self.patch)
+        return {
+            "x": x[:, 0], "y": y[0, :],
+            "density": rho, "velocity_x": u, "velocity_y": v, "velocity_z": w,
+            "pressure": p, "magnetic_field_x": bx, "magnetic_field_y": by,
+            "magnetic_field_z": c[5], "magnetic_pressure": 0.5 * b2,
+            "time": self.time,
+        }
