@@ -1,0 +1,178 @@
+"""Benchmark of the ideal-MHD step on one MI355X.
+
+Workload: 3D Alfvénic turbulence on a periodic 256^3 grid (rho = 1, p = 1,
+gamma = 5/3, B0 = (1, 0, 0), seeded velocity and field perturbations of rms 0.1
+over modes 1 <= |n| <= 4), minmod, CFL 0.3.  A step is one full SSP-RK2 (Heun)
+step including its CFL reduction; inputs are resident in HBM.  The script times
+the FP64 path and the mixed-precision path, reports the L1 density error of the
+mixed path against the FP64 path after the timed steps, the roofline of the
+stage kernel measured with HIP events on the stream it runs on, and a CPU
+baseline (the NumPy oracle on one core, bounded sample of the same workload).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells 256]
+
+Prints one JSON line.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+STATE_BYTES_3D = 64  # 5 FP64 cell fields + 3 FP64 edge-potential components per cell
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=40)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--cells", type=int, default=256)
+    p.add_argument("--fp64-kernels", default="fused", choices=("split", "fused"))
+    p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--no-cpu", action="store_true")
+    return p.parse_args()
+
+
+def time_model(model, steps, warmup):
+    """Warm up, then time ``steps`` full steps; returns (seconds, stage-kernel ms per launch)."""
+    import torch
+
+    for _ in range(warmup):
+        model.advance(1)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    events = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        marks = model.advance(1, events=True)
+        events.extend(marks)
+    torch.cuda.synchronize()
+    seconds = time.perf_counter() - t0
+    stage_ms = [a.elapsed_time(b) for a, b in events]
+    del stream
+    return seconds, sum(stage_ms) / max(1, len(stage_ms))
+
+
+def cpu_baseline(cells_sample: int, budget: float):
+    """The NumPy oracle step (one core) on a turbulence box of ``cells_sample``^3."""
+    from tests.helpers import oracle3d
+    from wellbalanced_amd import initial
+
+    problem = initial.turbulence_3d(cells_sample, seed=0)
+    oracle = oracle3d(problem)
+    oracle.step()  # warm-up
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        oracle.step()
+        steps += 1
+        elapsed = time.perf_counter() - t0
+        if elapsed >= budget or steps >= 50:
+            break
+    return cells_sample**3 * steps / elapsed, steps, elapsed
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                          "(the multi-GPU sharding is a design in DESIGN.md, not built)"}))
+        sys.exit(2)
+    import torch
+
+    from wellbalanced_amd import MHD3D, initial
+
+    torch.cuda.set_device(0)
+    n = args.cells
+    problem = initial.turbulence_3d(n, seed=0)
+    cells = n**3
+
+    fp64 = MHD3D.from_problem(problem, kernels=args.fp64_kernels, precision="fp64")
+    mixed_kernels = "fused"
+    try:
+        mixed = MHD3D.from_problem(problem, kernels=mixed_kernels, precision="mixed")
+    except (NotImplementedError, ValueError):
+        mixed = None
+    fp64.check_interval = 0
+    sec64, stage64 = time_model(fp64, args.steps, args.warmup)
+    rate64 = cells * args.steps / sec64
+    result_model, sec, stage_ms, precision = fp64, sec64, stage64, "fp64"
+    l1 = linf = 0.0
+    if mixed is not None:
+        mixed.check_interval = 0
+        secm, stagem = time_model(mixed, args.steps, args.warmup)
+        result_model, sec, stage_ms, precision = mixed, secm, stagem, "mixed"
+        fp64.check()
+        mixed.check()
+        a = fp64.dataset()["density"]
+        b = mixed.dataset()["density"]
+        import numpy as np
+
+        l1 = float(np.mean(np.abs(a - b)))
+        linf = float(np.max(np.abs(a - b)))
+    else:
+        fp64.check()
+    value = cells * args.steps / sec
+    ms_per_step = 1e3 * sec / args.steps
+    # stage kernel roofline: algorithmic bytes per launch = 2.5 S per cell (stage 1
+    # reads U^n, writes U^1: 2S; stage 2 reads U^1 and U^n, writes U^n+1: 3S)
+    bytes_per_launch = 2.5 * STATE_BYTES_3D * cells
+    achieved = bytes_per_launch / (stage_ms * 1e-3) / 1e9
+    out = {
+        "metric": "zone-updates/s on 1 MI355X, mixed vs FP64 at stated L1 error; % HBM peak",
+        "value": value,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-flux/f64-state" if precision == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"3D Alfvenic turbulence {n}^3 periodic, minmod, CFL 0.3, SSP-RK2",
+            "cells": cells,
+            "precision": precision,
+            "fp64_kernels": args.fp64_kernels,
+        },
+        "fp64_zone_updates_per_s": rate64,
+        "speedup_vs_fp64": value / rate64,
+        "l1_density_vs_fp64": l1,
+        "linf_density_vs_fp64": linf,
+        "hbm_fraction_algorithmic": value * 5 * STATE_BYTES_3D / (HBM_PEAK_GBS * 1e9),
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "stage (both Heun stages)",
+            "stage_ms": stage_ms,
+        },
+    }
+    if not args.no_cpu:
+        sample = 32
+        cpu_rate, steps, elapsed = cpu_baseline(sample, args.cpu_seconds)
+        out["cpu_baseline"] = {
+            "value": cpu_rate,
+            "unit": "zone-updates/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"NumPy FP64 oracle step, turbulence {sample}^3, {steps} steps in {elapsed:.1f} s",
+        }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

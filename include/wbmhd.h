@@ -69,10 +69,34 @@ int wb_mhd2d_walls(double* fx, double* fy, double* ez, int64_t nx, int64_t ny, i
 
 /* The conservative update of patchwork's FluxRegister.update as called at
  * mhd.py:911-922 (u - (dt/dx dFx + dt/dy dFy); A_z - dt E_z) from state `in` into
- * `out`; heun != 0 fuses the average of mhd.py:983-988: out = (u' + out) * 0.5. */
+ * `out`; heun != 0 fuses the average of mhd.py:983-988: out = (u' + out) * 0.5.
+ * dt is a device scalar (see wb_stable_dt). */
 int wb_mhd2d_update(const double* in, double* out, const double* fx, const double* fy,
-                    const double* ez, int64_t nx, int64_t ny, double dt, double dx, double dy,
-                    int heun, void* stream);
+                    const double* ez, int64_t nx, int64_t ny, const double* dt, double dx,
+                    double dy, int heun, void* stream);
+
+/* One whole forward-Euler stage of MHD2D._advance (mhd.py:889-927: primitive,
+ * both sweeps, CT-contact E_z, the wall faces, the update; with heun the average
+ * of mhd.py:983-988) in a single launch from the haloed state `in` (halos filled)
+ * into `out`.  FP64: the same bits as the split kernels above.  wall_mask bit 0:
+ * x walls, bit 1: y walls.  nxs: cells along x per workgroup; tile: cells along
+ * y per workgroup (62, 126 or 254). */
+int wb_mhd2d_stage(const double* in, double* out, int64_t nx, int64_t ny, double dx, double dy,
+                   double b0x, double b0y, double gamma, int limiter, const double* dt, int heun,
+                   int wall_mask, int32_t* bad, int nxs, int tile, void* stream);
+
+/* The same 2D stage with the reconstruction and the HLLD flux in FP32. */
+int wb_mhd2d_stage_mixed(const double* in, double* out, int64_t nx, int64_t ny, double dx,
+                         double dy, double b0x, double b0y, double gamma, int limiter,
+                         const double* dt, int heun, int wall_mask, int32_t* bad, int nxs,
+                         int tile, void* stream);
+
+/* The stable step on the device, mhd.py:954,963 (3D: mhd3d.py:722-724):
+ * *dt = cfl * (1 / rate[0]) from the output of wb_rate2d/wb_rate3d, shortened to
+ * `remaining` when 0 < remaining < dt (mhd.py:1004-1007); if time is not NULL,
+ * *time += *dt.  No host synchronisation. */
+int wb_stable_dt(const double* rate, double cfl, double remaining, double* dt, double* time,
+                 void* stream);
 
 /* ---- 3D stage pieces (MHD3D) ------------------------------------------- */
 
@@ -103,11 +127,28 @@ int wb_edge3d(const double* prim, const double* flux_a, const double* flux_b, in
               int64_t ny, int64_t nz, int c, double* e, void* stream);
 
 /* The update of mhd3d.py:653-663 (register.update x5, A_c -= dt E_c) from `in`
- * into `out`, heun as in wb_mhd2d_update (mhd3d.py:742-747). */
+ * into `out`, heun as in wb_mhd2d_update (mhd3d.py:742-747); dt on the device. */
 int wb_mhd3d_update(const double* in, double* out, const double* fx, const double* fy,
                     const double* fz, const double* ex, const double* ey, const double* ez,
-                    int64_t nx, int64_t ny, int64_t nz, double dt, double dx, double dy, double dz,
-                    int heun, void* stream);
+                    int64_t nx, int64_t ny, int64_t nz, const double* dt, double dx, double dy,
+                    double dz, int heun, void* stream);
+
+/* One whole forward-Euler stage of MHD3D._advance (mhd3d.py:635-671: primitive,
+ * the three sweeps, the CT-contact edge field, the update; with heun the average
+ * of mhd3d.py:742-747) in a single launch, from the haloed state `in` into `out`
+ * (halos of `in` filled).  FP64: the same bits as the split kernels above.
+ * nxs: cells along x per workgroup segment.  *bad (device) counts bad cells. */
+int wb_mhd3d_stage(const double* in, double* out, int64_t nx, int64_t ny, int64_t nz, double dx,
+                   double dy, double dz, double b0x, double b0y, double b0z, double gamma,
+                   int limiter, const double* dt, int heun, int32_t* bad, int nxs, void* stream);
+
+/* The same stage with the reconstruction and the HLLD flux in FP32; the primitive
+ * conversion (E -> p), the cell electric field, the CT average, the potential
+ * and the update stay FP64. */
+int wb_mhd3d_stage_mixed(const double* in, double* out, int64_t nx, int64_t ny, int64_t nz,
+                         double dx, double dy, double dz, double b0x, double b0y, double b0z,
+                         double gamma, int limiter, const double* dt, int heun, int32_t* bad,
+                         int nxs, void* stream);
 
 /* Replaces the rate of MHD3D.stable_step, mhd3d.py:674-724: out = device
  * double[2], largest rate and smallest density. */

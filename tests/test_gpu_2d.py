@@ -227,3 +227,54 @@ def test_needs_a_state_first(gpu):
         model.set_state(1.0, (0.0, 0.0, 0.0), -1.0)
     with pytest.raises(ValueError, match="three"):
         model.set_state(1.0, (0.0, 0.0), 1.0)
+
+
+@pytest.mark.parametrize("tile,nxs", [(126, 32), (62, 5), (254, 64)])
+@pytest.mark.parametrize("limiter", ["minmod", "vanleer", "mc", "none"])
+def test_fused_fp64_stage_is_bitwise_the_split_path(gpu, tile, nxs, limiter):
+    for problem in (initial.orszag_tang(40, limiter=limiter),
+                    initial.harris_sheet(70, 36, length=3.2, noise=1e-2, seed=1, limiter=limiter)):
+        split = MHD2D.from_problem(problem)
+        fused = MHD2D.from_problem(problem, kernels="fused")
+        fused.tile, fused.nxs = tile, nxs
+        for _ in range(4):
+            dt = split.stable_step()
+            split.step(dt)
+            fused.step(dt)
+        a, aa = split.arrays()
+        b, bb = fused.arrays()
+        np.testing.assert_array_equal(b[:, 2:-2, 2:-2], a[:, 2:-2, 2:-2])
+        np.testing.assert_array_equal(bb[2:-2, 2:-2], aa[2:-2, 2:-2])
+
+
+def test_fused_outflow_and_walls_match_the_oracle(gpu):
+    h = AMRHierarchy((37, 29), ((0.0, 1.0), (0.0, 1.0)))
+    problem = initial.Problem(h, {"limiter": "mc", "boundaries": {"x": "wall"}, "mean_field": (0.0, 0.1)}, dict(
+        density=lambda x, y: 1.0 + 0.3 * np.exp(-((x - 0.4) ** 2 + (y - 0.6) ** 2) / 0.01),
+        velocity=(0.3, -0.2, 0.1), pressure=1.0,
+        vector_potential=lambda x, y: 0.05 * np.sin(3 * x) * np.cos(2 * y), bz=0.2))
+    model = MHD2D.from_problem(problem, kernels="fused")
+    model.nxs = 8
+    oracle = oracle2d(problem)
+    for _ in range(6):
+        dt = oracle.stable_step()
+        model.step(dt)
+        oracle.step(dt)
+    _compare(model, oracle)
+
+
+def test_mixed_2d_is_close_to_fp64(gpu):
+    problem = initial.orszag_tang(128, limiter="minmod")
+    fp64 = MHD2D.from_problem(problem, kernels="fused")
+    mixed = MHD2D.from_problem(problem, kernels="fused", precision="mixed")
+    before = mixed.totals()
+    for _ in range(20):
+        dt = fp64.stable_step()
+        fp64.step(dt)
+        mixed.step(dt)
+    l1 = float(np.mean(np.abs(fp64.dataset()["density"] - mixed.dataset()["density"])))
+    assert 0.0 < l1 < 1e-5
+    assert mixed.divergence() < 1e-14
+    after = mixed.totals()
+    for name in before:
+        assert abs(after[name] - before[name]) < 1e-13

@@ -90,3 +90,82 @@ def test_turbulence_256_conserves_and_keeps_div_b_zero(gpu):
     for name in before:
         assert abs(after[name] - before[name]) < 1e-12 * max(1.0, abs(before[name]))
     assert model.divergence() < 1e-13
+
+
+@pytest.mark.parametrize("shape,nxs", [((12, 12, 12), 32), ((31, 17, 29), 4), ((16, 30, 15), 7)])
+@pytest.mark.parametrize("limiter", ["minmod", "vanleer", "mc", "none"])
+def test_fused_fp64_stage_is_bitwise_the_split_path(gpu, shape, nxs, limiter):
+    """The one-launch FP64 stage gives the same bits as the split kernels (and so as
+    the oracle), on uneven tiles and segments."""
+    from wellbalanced_amd.hierarchy import AMRHierarchy
+
+    h = AMRHierarchy(shape, ((0.0, 1.0), (0.0, 0.7), (0.0, 1.3)), periodic=(True, True, True))
+    base = initial.turbulence_3d(8, seed=2)
+    rng = np.random.default_rng(9)
+    problem = initial.Problem(h, {"mean_field": (0.6, -0.3, 0.2), "limiter": limiter}, dict(
+        density=lambda x, y, z: 1.0 + 0.3 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * z),
+        velocity=lambda x, y, z: (0.2 * np.sin(2 * np.pi * y), 0.1 * np.cos(2 * np.pi * z + 1), 0.3 * np.sin(2 * np.pi * x)),
+        pressure=lambda x, y, z: 0.8 + 0.1 * np.cos(2 * np.pi * y),
+        vector_potential=(lambda x, y, z: 0.02 * np.sin(2 * np.pi * z), lambda x, y, z: 0.03 * np.cos(2 * np.pi * x),
+                          lambda x, y, z: 0.01 * np.sin(2 * np.pi * (x + y)))))
+    del base, rng
+    split = MHD3D.from_problem(problem, kernels="split")
+    fused = MHD3D.from_problem(problem, kernels="fused")
+    fused.nxs = nxs
+    for _ in range(3):
+        dt = split.stable_step()
+        split.step(dt)
+        fused.step(dt)
+    a, pa = split.arrays()
+    b, pb = fused.arrays()
+    h2 = 2
+    inner = (slice(None),) + tuple(slice(h2, h2 + n) for n in shape)
+    np.testing.assert_array_equal(b[inner], a[inner])
+    for c in range(3):
+        idx = tuple(slice(h2, h2 + shape[k] + (k != c)) for k in range(3))
+        np.testing.assert_array_equal(pb[c][idx], pa[c][idx])
+
+
+def test_fused_turbulence_matches_the_oracle(gpu):
+    problem = initial.turbulence_3d(20, seed=6)
+    model = MHD3D.from_problem(problem, kernels="fused")
+    oracle = oracle3d(problem)
+    for _ in range(3):
+        dt = oracle.stable_step()
+        model.step(dt)
+        oracle.step(dt)
+    _compare(model, oracle)
+
+
+def test_mixed_precision_is_close_and_keeps_div_b_and_conservation(gpu):
+    """FP32 reconstruction/HLLD with FP64 CT and update: the density stays within
+    1e-5 (L1) of the FP64 path, div B at FP64 round-off, totals conserved."""
+    problem = initial.turbulence_3d(32, seed=3)
+    fp64 = MHD3D.from_problem(problem, kernels="fused")
+    mixed = MHD3D.from_problem(problem, kernels="fused", precision="mixed")
+    before = mixed.totals()
+    for _ in range(10):
+        dt = fp64.stable_step()
+        fp64.step(dt)
+        mixed.step(dt)
+    a = fp64.dataset()["density"]
+    b = mixed.dataset()["density"]
+    l1 = float(np.mean(np.abs(a - b)))
+    assert 0.0 < l1 < 1e-5
+    assert mixed.divergence() < 1e-13
+    after = mixed.totals()
+    for name in before:
+        assert abs(after[name] - before[name]) < 1e-12 * max(1.0, abs(before[name]))
+
+
+def test_advance_keeps_dt_on_the_device(gpu):
+    """advance() computes dt on the device: the same steps as step() with the host dt."""
+    problem = initial.turbulence_3d(16, seed=4)
+    a = MHD3D.from_problem(problem, kernels="fused")
+    b = MHD3D.from_problem(problem, kernels="fused")
+    times = []
+    for _ in range(3):
+        times.append(a.step())
+    b.advance(3)
+    assert b.time == a.time == sum([0.0] + times) or abs(b.time - a.time) == 0.0
+    np.testing.assert_array_equal(a.arrays()[0], b.arrays()[0])

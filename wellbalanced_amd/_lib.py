@@ -26,13 +26,18 @@ SIGNATURES: dict[str, tuple[type, list]] = {
     "wb_rate2d": (I, [P] * 7 + [I64, I64, I] + [D] * 5 + [P, P]),
     "wb_mhd2d_fill_halos": (I, [P, I64, I64, I, I, P]),
     "wb_mhd2d_walls": (I, [P, P, P, I64, I64, I, P]),
-    "wb_mhd2d_update": (I, [P, P, P, P, P, I64, I64, D, D, D, I, P]),
+    "wb_mhd2d_update": (I, [P, P, P, P, P, I64, I64, P, D, D, I, P]),
+    "wb_stable_dt": (I, [P, D, D, P, P, P]),
+    "wb_mhd2d_stage": (I, [P, P, I64, I64] + [D] * 5 + [I, P, I, I, P, I, I, P]),
+    "wb_mhd2d_stage_mixed": (I, [P, P, I64, I64] + [D] * 5 + [I, P, I, I, P, I, I, P]),
     "wb_mhd3d_state_size": (I64, [I64, I64, I64]),
     "wb_mhd3d_fill_halos": (I, [P, I64, I64, I64, I, P]),
     "wb_primitive3d": (I, [P, I64, I64, I64] + [D] * 7 + [P] * 5 + [P]),
     "wb_sweep3d": (I, [P, P, I64, I64, I64, I, I, D, P, P]),
     "wb_edge3d": (I, [P, P, P, I64, I64, I64, I, P, P]),
-    "wb_mhd3d_update": (I, [P, P] + [P] * 6 + [I64] * 3 + [D] * 4 + [I, P]),
+    "wb_mhd3d_update": (I, [P, P] + [P] * 6 + [I64] * 3 + [P] + [D] * 3 + [I, P]),
+    "wb_mhd3d_stage": (I, [P, P, I64, I64, I64] + [D] * 7 + [I, P, I, P, I, P]),
+    "wb_mhd3d_stage_mixed": (I, [P, P, I64, I64, I64] + [D] * 7 + [I, P, I, P, I, P]),
     "wb_rate3d": (I, [P, I64, I64, I64] + [D] * 7 + [P, P]),
 }
 

@@ -1,0 +1,227 @@
+// The fused 2D stage kernel: one launch runs a whole forward-Euler stage of
+// MHD2D (primitive variables, MUSCL + HLLD along x and y, the CT-contact nodal
+// E_z, the wall faces, the update of the 6 cell fields and of A_z, with the Heun
+// average fused into stage 2).
+//
+// A workgroup owns TY cells along y (the contiguous axis) and marches along x
+// over NXS cells.  Thread t maps to the face column ly = t - 1 in [-1, TY]; it
+// keeps the x-sweep's four planes of its column in registers, the current
+// primitive row sits in LDS for the y-sweep, and the y-face fluxes and the cell
+// E_z of the last two rows sit in LDS rings for the CT-contact average.
+// P is the arithmetic of the reconstruction and the HLLD flux; the rest is FP64.
+#pragma once
+
+#include "mhd_device.hpp"
+#include "wbmhd_common.hpp"
+
+namespace wb {
+
+template <int TY>
+struct Row {
+  static constexpr int FY = TY + 2;  // face columns
+  static constexpr int PY = TY + 4;  // primitive columns
+  static constexpr int THREADS = ((FY + 63) / 64) * 64;
+};
+
+template <typename P, int TY>
+struct Stage2Shared {
+  using T = Row<TY>;
+  P prim[2][8][T::PY];
+  double ecc[3][T::FY];  // cell E_z of rows f-1, f, f+1
+  P fx[2][T::FY];     // x-face f: rows 0 (mass) and 5 (B_y)
+  P fy[2][7][T::FY];  // y-faces of rows f-1 and f
+};
+
+struct Stage2Args {
+  const double* in;
+  double* out;
+  const double* dt;
+  int* bad;
+  int nx, ny, X, Y;
+  double dx, dy, b0x, b0y, gamma;
+  int limiter, heun, nxs;
+  int walls;  // bit 0: x walls, bit 1: y walls
+};
+
+template <typename P, int TY>
+__global__ void __launch_bounds__((Row<TY>::THREADS)) k_stage2d(Stage2Args a) {
+  using T = Row<TY>;
+  __shared__ Stage2Shared<P, TY> sh;
+  const int tid = threadIdx.x;
+  const int X = a.X, Y = a.Y, YN = Y + 1;
+  const int jy0 = 2 + blockIdx.x * TY;
+  const int ix0 = 2 + blockIdx.y * a.nxs;
+  const int tyv = min(TY, a.ny + 2 - jy0);
+  const int nxv = min(a.nxs, a.nx + 2 - ix0);
+  const bool last_y = jy0 + TY >= a.ny + 2, last_x = ix0 + a.nxs >= a.nx + 2;
+  const bool fthread = tid < T::FY;
+  const int ly = tid - 1, gy = jy0 + ly;
+  const size_t PL = (size_t)X * Y;
+  const double* az = a.in + 6 * PL;
+  const double dt = *a.dt;
+  const double dtdx = dt / a.dx, dtdy = dt / a.dy;
+  const bool xwall = a.walls & 1, ywall = a.walls & 2;
+
+  P q[7][4];
+  P fprev[6], fcur[6];
+  double dyy[6];
+#pragma unroll
+  for (int c = 0; c < 7; ++c)
+#pragma unroll
+    for (int s = 0; s < 4; ++s) q[c][s] = P(0);
+#pragma unroll
+  for (int r = 0; r < 6; ++r) { fprev[r] = fcur[r] = P(0); dyy[r] = 0.0; }
+
+  auto AZ = [&](int i, int j) -> double { return az[(size_t)i * YN + j]; };
+  // mhd_primitive_pyccel.py:51-86 for row `pl`
+  auto primitive_row = [&](int pl, int buf) {
+    const int i = min(max(pl, 0), X - 1);
+    for (int idx = tid; idx < T::PY; idx += T::THREADS) {
+      const int j = min(max(jy0 + idx - 2, 0), Y - 1);
+      const size_t c = (size_t)i * Y + j;
+      const double bx0 = (AZ(i, j + 1) - AZ(i, j)) / a.dy + a.b0x;
+      const double bx1 = (AZ(i + 1, j + 1) - AZ(i + 1, j)) / a.dy + a.b0x;
+      const double by0 = -(AZ(i + 1, j) - AZ(i, j)) / a.dx + a.b0y;
+      const double by1 = -(AZ(i + 1, j + 1) - AZ(i, j + 1)) / a.dx + a.b0y;
+      const double r = a.in[c];
+      const double u = a.in[PL + c] / r, v = a.in[2 * PL + c] / r, w = a.in[3 * PL + c] / r;
+      const double bx = 0.5 * (bx0 + bx1);
+      const double by = 0.5 * (by0 + by1);
+      const double b = a.in[5 * PL + c];
+      const double p = (a.gamma - 1.0) * (a.in[4 * PL + c] - 0.5 * r * (u * u + v * v + w * w) -
+                                          0.5 * (bx * bx + by * by + b * b));
+      sh.prim[buf][0][idx] = P(r);
+      sh.prim[buf][1][idx] = P(u);
+      sh.prim[buf][2][idx] = P(v);
+      sh.prim[buf][3][idx] = P(w);
+      sh.prim[buf][4][idx] = P(p);
+      sh.prim[buf][5][idx] = P(bx);
+      sh.prim[buf][6][idx] = P(by);
+      sh.prim[buf][7][idx] = P(b);
+      if (idx >= 1 && idx <= T::FY) sh.ecc[pl % 3][idx - 1] = bx * v - by * u;
+      if (idx >= 2 && idx < 2 + tyv && pl >= ix0 && pl < ix0 + nxv) {
+        const bool is_bad = !(r > 0.0 && p > 0.0) ||
+                            !(finite(r) && finite(u) && finite(v) && finite(w) && finite(p) &&
+                              finite(bx) && finite(by) && finite(b));
+        if (is_bad) atomicAdd(a.bad, 1);
+      }
+    }
+  };
+  auto push_column = [&](int buf) {
+    if (!fthread) return;
+    constexpr int comp[7] = {0, 1, 2, 3, 4, 6, 7};
+#pragma unroll
+    for (int c = 0; c < 7; ++c) {
+      q[c][0] = q[c][1];
+      q[c][1] = q[c][2];
+      q[c][2] = q[c][3];
+      q[c][3] = sh.prim[buf][comp[c]][tid + 1];
+    }
+  };
+
+  primitive_row(ix0 - 2, (ix0 - 2) & 1);
+  __syncthreads();
+  push_column((ix0 - 2) & 1);
+  __syncthreads();
+  primitive_row(ix0 - 1, (ix0 - 1) & 1);
+  __syncthreads();
+  push_column((ix0 - 1) & 1);
+
+  constexpr int XR[6] = {0, 1, 2, 3, 4, 6};  // x-sweep rows of the 6 fields (mhd.py:134)
+  constexpr int YR[6] = {0, 2, 1, 3, 4, 6};
+  for (int f = ix0 - 1; f <= ix0 + nxv; ++f) {
+    __syncthreads();
+    primitive_row(f + 1, (f + 1) & 1);
+    __syncthreads();
+    push_column((f + 1) & 1);
+    const int cur = f & 1;
+    if (fthread) {
+      const int jj = min(max(gy, 0), Y - 1);
+      if (f >= ix0) {  // x-face f of the column
+        P L[7], R[7], fl[7];
+        reconstruct<P>(q, a.limiter, L, R);
+        const P bn = P((AZ(f, jj + 1) - AZ(f, jj)) / a.dy + a.b0x);
+        hlld<P>(L, R, bn, P(a.gamma), fl);
+        // walls: nothing but the pressure crosses (mhd.py:873-886), after the CT
+        const bool wall = xwall && (f == 2 || f == a.nx + 2);
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+          fprev[r] = fcur[r];
+          const int row = XR[r];
+          fcur[r] = (wall && row != 1 && row != 5) ? P(0) : fl[row];
+        }
+        sh.fx[0][tid] = fl[0];
+        sh.fx[1][tid] = fl[5];
+      }
+      if (tid >= 1) {  // y-face below the column's cell: cells ly-2 .. ly+1
+        constexpr int comp[7] = {0, 2, 1, 3, 4, 5, 7};
+        P qq[7][4];
+#pragma unroll
+        for (int c = 0; c < 7; ++c)
+#pragma unroll
+          for (int s = 0; s < 4; ++s) qq[c][s] = sh.prim[cur][comp[c]][tid + 1 + s - 2];
+        P L[7], R[7], fl[7];
+        reconstruct<P>(qq, a.limiter, L, R);
+        const int ii = min(max(f, 0), X - 1);
+        const P bn = P(-(AZ(ii + 1, jj) - AZ(ii, jj)) / a.dx + a.b0y);
+        hlld<P>(L, R, bn, P(a.gamma), fl);
+#pragma unroll
+        for (int r = 0; r < 7; ++r) sh.fy[cur][r][tid] = fl[r];
+      }
+    }
+    __syncthreads();
+    if (fthread) {
+      const int prv = cur ^ 1;
+      const bool in_y = ly >= 0 && (ly < tyv || (ly == tyv && last_y));
+      const bool cell_y = ly >= 0 && ly < tyv;
+      // E_z at (x-node f, y-node ly) (ct_contact_pyccel.py:35-61), walls zero it
+      const bool node_x = f >= ix0 && (f < ix0 + nxv || (f == ix0 + nxv && last_x));
+      if (node_x && in_y) {
+        double e = ct_node(double(sh.fx[1][tid - 1]), double(sh.fx[1][tid]),
+                           double(sh.fy[prv][5][tid]), double(sh.fy[cur][5][tid]),
+                           double(sh.fx[0][tid - 1]), double(sh.fx[0][tid]),
+                           double(sh.fy[prv][0][tid]), double(sh.fy[cur][0][tid]),
+                           sh.ecc[(f + 2) % 3][tid - 1], sh.ecc[f % 3][tid - 1],
+                           sh.ecc[(f + 2) % 3][tid], sh.ecc[f % 3][tid]);
+        if ((xwall && (f == 2 || f == a.nx + 2)) || (ywall && (gy == 2 || gy == a.ny + 2))) e = 0.0;
+        const size_t o = 6 * PL + (size_t)f * YN + gy;
+        double v = a.in[o] - dt * e;
+        if (a.heun) v = (v + a.out[o]) * 0.5;
+        a.out[o] = v;
+      }
+      if (cell_y) {
+        if (f - 1 >= ix0 && f - 1 < ix0 + nxv) {
+          const size_t c = (size_t)(f - 1) * Y + gy;
+#pragma unroll
+          for (int r = 0; r < 6; ++r) {
+            const double dfx = double(fcur[r]) - double(fprev[r]);
+            double v = a.in[r * PL + c] - (dtdx * dfx + dyy[r]);
+            if (a.heun) v = (v + a.out[r * PL + c]) * 0.5;
+            a.out[r * PL + c] = v;
+          }
+        }
+        if (f >= ix0 && f < ix0 + nxv) {
+          const bool wlo = ywall && gy == 2, whi = ywall && gy + 1 == a.ny + 2;
+#pragma unroll
+          for (int r = 0; r < 6; ++r) {
+            const int row = YR[r];
+            const bool zeroable = row != 1 && row != 5;
+            const double hi = (whi && zeroable) ? 0.0 : double(sh.fy[cur][row][tid + 1]);
+            const double lo = (wlo && zeroable) ? 0.0 : double(sh.fy[cur][row][tid]);
+            dyy[r] = dtdy * (hi - lo);
+          }
+        }
+      }
+    }
+  }
+}
+
+template <typename P, int TY>
+inline int launch_stage2d(const Stage2Args& a, hipStream_t s) {
+  const int tiles = (a.ny + TY - 1) / TY;
+  const int segs = (a.nx + a.nxs - 1) / a.nxs;
+  hipLaunchKernelGGL((k_stage2d<P, TY>), dim3(tiles, segs), dim3(Row<TY>::THREADS), 0, s, a);
+  return wb_last_error();
+}
+
+}  // namespace wb

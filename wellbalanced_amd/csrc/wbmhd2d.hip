@@ -61,7 +61,7 @@ __global__ void k_primitive2d(const double* __restrict__ rho, const double* __re
 // mhd_sweep_pyccel.py:254-319.  One thread per face (k, line); axis 0 writes
 // flux (7, X-3, Y), axis 1 flux (7, X, Y-3).
 template <int AXIS>
-__global__ void k_sweep2d(const double* __restrict__ prim, const double* __restrict__ bn, int X,
+__global__ void __launch_bounds__(128) k_sweep2d(const double* __restrict__ prim, const double* __restrict__ bn, int X,
                           int Y, int limiter, double gamma, double* __restrict__ flux) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int row = blockIdx.y;
@@ -259,8 +259,10 @@ __global__ void k_walls2d(double* __restrict__ fx, double* __restrict__ fy, doub
 // out = (u' + out) * 0.5 (mhd.py:983-988) reading out's own previous value.
 __global__ void k_update2d(const double* __restrict__ in, double* __restrict__ out,
                            const double* __restrict__ fx, const double* __restrict__ fy,
-                           const double* __restrict__ ez, int nx, int ny, double dtdx,
-                           double dtdy, double dt, int heun) {
+                           const double* __restrict__ ez, int nx, int ny, double dx, double dy,
+                           const double* __restrict__ dtp, int heun) {
+  const double dt = *dtp;
+  const double dtdx = dt / dx, dtdy = dt / dy;
   constexpr int XR[6] = {0, 1, 2, 3, 4, 6};
   constexpr int YR[6] = {0, 2, 1, 3, 4, 6};
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -375,12 +377,12 @@ int wb_mhd2d_walls(double* fx, double* fy, double* ez, int64_t nx, int64_t ny, i
 }
 
 int wb_mhd2d_update(const double* in, double* out, const double* fx, const double* fy,
-                    const double* ez, int64_t nx, int64_t ny, double dt, double dx, double dy,
-                    int heun, void* stream) {
+                    const double* ez, int64_t nx, int64_t ny, const double* dt, double dx,
+                    double dy, int heun, void* stream) {
   if (nx < 1 || ny < 1) return WB_EINVAL;
   dim3 grid((unsigned)((ny + 1 + 255) / 256), (unsigned)(nx + 1));
   hipLaunchKernelGGL(k_update2d, grid, dim3(256), 0, (hipStream_t)stream, in, out, fx, fy, ez,
-                     (int)nx, (int)ny, dt / dx, dt / dy, dt, heun);
+                     (int)nx, (int)ny, dx, dy, dt, heun);
   return wb_last_error();
 }
 

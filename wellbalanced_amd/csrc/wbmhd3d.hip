@@ -108,7 +108,7 @@ __global__ void k_primitive3d(const double* __restrict__ state, Dims3 g, double 
 // every transverse cell, in the physical layout (7, ...) with n_S - 3 faces along
 // S.  Rows: rho, m_S, m_b, m_c, E, B_b, B_c for the cyclic frame (S, b, c).
 template <int S>
-__global__ void k_sweep3d(const double* __restrict__ prim, const double* __restrict__ bn, Dims3 g,
+__global__ void __launch_bounds__(128) k_sweep3d(const double* __restrict__ prim, const double* __restrict__ bn, Dims3 g,
                           int limiter, double gamma, double* __restrict__ flux) {
   constexpr int B = (S + 1) % 3, C = (S + 2) % 3;
   constexpr int comp[7] = {0, 1 + S, 1 + B, 1 + C, 4, 5 + B, 5 + C};
@@ -194,7 +194,10 @@ __global__ void k_update3d(const double* __restrict__ in, double* __restrict__ o
                            const double* __restrict__ fx, const double* __restrict__ fy,
                            const double* __restrict__ fz, const double* __restrict__ ex,
                            const double* __restrict__ ey, const double* __restrict__ ez, Dims3 g,
-                           double dtdx, double dtdy, double dtdz, double dt, int heun) {
+                           double dx, double dy, double dz, const double* __restrict__ dtp,
+                           int heun) {
+  const double dt = *dtp;
+  const double dtdx = dt / dx, dtdy = dt / dy, dtdz = dt / dz;
   const int nx = g.X - 4, ny = g.Y - 4, nz = g.Z - 4, h = HALO;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int a = blockIdx.y;
@@ -400,14 +403,14 @@ int wb_edge3d(const double* prim, const double* flux_a, const double* flux_b, in
 
 int wb_mhd3d_update(const double* in, double* out, const double* fx, const double* fy,
                     const double* fz, const double* ex, const double* ey, const double* ez,
-                    int64_t nx, int64_t ny, int64_t nz, double dt, double dx, double dy, double dz,
-                    int heun, void* stream) {
+                    int64_t nx, int64_t ny, int64_t nz, const double* dt, double dx, double dy,
+                    double dz, int heun, void* stream) {
   if (!dims_ok(nx, ny, nz)) return WB_EINVAL;
   Dims3 g{(int)nx + 4, (int)ny + 4, (int)nz + 4};
   const int64_t inner = (ny + 1) * (nz + 1);
   dim3 grid((unsigned)((inner + 255) / 256), (unsigned)(nx + 1));
   hipLaunchKernelGGL(k_update3d, grid, dim3(256), 0, (hipStream_t)stream, in, out, fx, fy, fz, ex,
-                     ey, ez, g, dt / dx, dt / dy, dt / dz, dt, heun);
+                     ey, ez, g, dx, dy, dz, dt, heun);
   return wb_last_error();
 }
 

@@ -1,0 +1,23 @@
+// The fused 3D stage with the reconstruction and the HLLD flux in FP32 and the
+// primitive conversion, the cell electric field, the CT average, the potential
+// and the update in FP64.
+#include "stage3d.hpp"
+
+using namespace wb;
+
+extern "C" __attribute__((visibility("hidden"))) int wb_fill_stage_args3d(void* args, const double* in, double* out, int64_t nx,
+                                    int64_t ny, int64_t nz, double dx, double dy, double dz,
+                                    double b0x, double b0y, double b0z, double gamma, int limiter,
+                                    const double* dt, int heun, int32_t* bad, int nxs);
+
+extern "C" int wb_mhd3d_stage_mixed(const double* in, double* out, int64_t nx, int64_t ny,
+                                    int64_t nz, double dx, double dy, double dz, double b0x,
+                                    double b0y, double b0z, double gamma, int limiter,
+                                    const double* dt, int heun, int32_t* bad, int nxs,
+                                    void* stream) {
+  StageArgs a;
+  const int rc = wb_fill_stage_args3d(&a, in, out, nx, ny, nz, dx, dy, dz, b0x, b0y, b0z, gamma,
+                                      limiter, dt, heun, bad, nxs);
+  if (rc) return rc;
+  return launch_stage3d<float, 14, 14>(a, (hipStream_t)stream);
+}

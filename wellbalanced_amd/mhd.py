@@ -29,6 +29,7 @@ CONSERVED = ("density", "momentum_x", "momentum_y", "momentum_z", "energy", "bz"
 BOUNDARIES = ("outflow", "wall")
 LIMITER_CODES = ("minmod", "vanleer", "mc", "none")
 PRECISIONS = ("fp64", "mixed")
+KERNELS = ("split", "fused")
 
 Value = float | np.ndarray | Callable[..., Any]
 
@@ -53,7 +54,7 @@ class MHD2D:
                  conduction: tuple[float, float] = (0.0, 0.0),
                  gravity: tuple[float, float] = (0.0, 0.0),
                  boundaries: dict[str, str] | None = None, prefix: str = "mhd_",
-                 precision: str = "fp64", device=None) -> None:
+                 precision: str = "fp64", kernels: str = "split", device=None) -> None:
         # validation as mhd.py:425-455
         if hierarchy.ndim != 2:
             raise ValueError("MHD2D needs a 2D hierarchy")
@@ -73,6 +74,10 @@ class MHD2D:
             raise ValueError("the conductivities must be non-negative")
         if precision not in PRECISIONS:
             raise ValueError(f"precision is one of {PRECISIONS}")
+        if kernels not in KERNELS:
+            raise ValueError(f"kernels is one of {KERNELS}")
+        if kernels == "split" and precision != "fp64":
+            raise ValueError("the split kernels are the FP64 parity path; mixed needs kernels='fused'")
         kinds = {"x": "outflow", "y": "outflow"}
         for axis_name, kind in (boundaries or {}).items():
             if axis_name not in kinds or kind not in BOUNDARIES:
@@ -105,6 +110,9 @@ class MHD2D:
         self.names = {name: prefix + name for name in CONSERVED}
         self.potential = prefix + "az"
         self.precision = precision
+        self.kernels = kernels
+        self.nxs = 32
+        self.tile = 126
         self.device = _device(device)
         self.patch: Patch = hierarchy.patches[0]
         self.nx, self.ny = self.patch.shape
@@ -124,8 +132,9 @@ class MHD2D:
         self._ez = torch.empty((self.nx + 1) * (self.ny + 1), **opts)
         self._bad = torch.zeros(2, dtype=torch.int32, device=self.device)
         self._rate = torch.zeros(2, **opts)
+        self._dt = torch.zeros(1, **opts)
+        self._t = torch.zeros(1, **opts)
         self.check_interval = 1
-        self.time = 0.0
         self.steps = 0
         self.period = float("nan")
         self._ready = False
@@ -189,6 +198,15 @@ class MHD2D:
         self.steps = 0
         self._ready = True
 
+    @property
+    def time(self) -> float:
+        """The model time (kept on the device; reading it synchronises)."""
+        return float(self._t.item())
+
+    @time.setter
+    def time(self, value: float) -> None:
+        self._t.fill_(float(value))
+
     def arrays(self) -> tuple[np.ndarray, np.ndarray]:
         """The haloed state on the host: ``cons (6, X, Y)``, ``az (X+1, Y+1)``."""
         cells, az = self._views(self._buf[0])
@@ -207,7 +225,7 @@ class MHD2D:
             mask |= 12
         return mask
 
-    def _stage(self, src: torch.Tensor, dst: torch.Tensor, dt: float, heun: bool, slot: int,
+    def _stage(self, src: torch.Tensor, dst: torch.Tensor, heun: bool, slot: int,
                stream: int) -> None:
         X, Y, nx, ny = self.X, self.Y, self.nx, self.ny
         call, ptr = _lib.call, _lib.ptr
@@ -233,21 +251,49 @@ class MHD2D:
              ptr(self._ez), stream)
         if self._wall_mask():
             call("wb_mhd2d_walls", fxp, fyp, ptr(self._ez), nx, ny, self._wall_mask(), stream)
-        call("wb_mhd2d_update", ptr(src), ptr(dst), fxp, fyp, ptr(self._ez), nx, ny, dt,
-             self.dx, self.dy, int(heun), stream)
+        call("wb_mhd2d_update", ptr(src), ptr(dst), fxp, fyp, ptr(self._ez), nx, ny,
+             ptr(self._dt), self.dx, self.dy, int(heun), stream)
+
+    def _rate_kernel(self, stream: int) -> None:
+        base = self._buf[0].data_ptr()
+        plane = self.X * self.Y * 8
+        _lib.call("wb_rate2d", *[base + k * plane for k in range(6)], base + self._cells * 8,
+                  self.X, self.Y, HALO, self.dx, self.dy, *self.mean_field, self.gamma,
+                  _lib.ptr(self._rate), stream)
 
     def stable_step(self) -> float:
         """cfl / (largest fast-wave rate) (mhd.py:930-963); synchronises."""
         self._check_ready()
-        cells, _ = self._views(self._buf[0])
-        base = self._buf[0].data_ptr()
-        plane = self.X * self.Y * 8
-        stream = _lib.stream_handle(self.device)
-        _lib.call("wb_rate2d", *[base + k * plane for k in range(6)], base + self._cells * 8,
-                  self.X, self.Y, HALO, self.dx, self.dy, *self.mean_field, self.gamma,
-                  _lib.ptr(self._rate), stream)
+        self._rate_kernel(_lib.stream_handle(self.device))
         largest = float(self._rate[0].item())
         return self.cfl * (1.0 / largest)
+
+    def _stage_fused(self, src: torch.Tensor, dst: torch.Tensor, heun: bool, slot: int,
+                     stream: int) -> None:
+        pmask = int(self.periodic[0]) | (int(self.periodic[1]) << 1)
+        wmask = int(self.walls[0]) | (int(self.walls[1]) << 1)
+        _lib.call("wb_mhd2d_fill_halos", _lib.ptr(src), self.nx, self.ny, pmask, wmask, stream)
+        name = "wb_mhd2d_stage" if self.precision == "fp64" else "wb_mhd2d_stage_mixed"
+        _lib.call(name, _lib.ptr(src), _lib.ptr(dst), self.nx, self.ny, self.dx, self.dy,
+                  *self.mean_field, self.gamma, LIMITER_CODES.index(self.limiter),
+                  _lib.ptr(self._dt), int(heun), wmask, self._bad.data_ptr() + 4 * slot,
+                  int(self.nxs), int(self.tile), stream)
+
+    def _heun(self, stream: int, events=None) -> None:
+        a, b = self._buf
+        stage = self._stage if self.kernels == "split" else self._stage_fused
+        if events is None:
+            stage(a, b, False, 0, stream)
+            stage(b, a, True, 1, stream)
+            return
+        marks = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        marks[0].record()
+        stage(a, b, False, 0, stream)
+        marks[1].record()
+        marks[2].record()
+        stage(b, a, True, 1, stream)
+        marks[3].record()
+        events.extend([(marks[0], marks[1]), (marks[2], marks[3])])
 
     def step(self, dt: float | None = None) -> float:
         """Advance by one Heun (SSP-RK2) step (mhd.py:965-994); return the step taken."""
@@ -256,14 +302,28 @@ class MHD2D:
             dt = self.stable_step()
         dt = float(dt)
         stream = _lib.stream_handle(self.device)
-        a, b = self._buf
-        self._stage(a, b, dt, False, 0, stream)
-        self._stage(b, a, dt, True, 1, stream)
-        self.time += dt
+        self._dt.fill_(dt)
+        self._heun(stream)
+        self._t.add_(dt)
         self.steps += 1
         if self.check_interval and self.steps % self.check_interval == 0:
             self.check()
         return dt
+
+    def advance(self, steps: int = 1, events: bool = False):
+        """``steps`` steps at the stable step, with dt and the time kept on the device:
+        no host synchronisation (call `check` to test for bad cells).  With ``events``
+        returns the HIP event pairs bracketing each stage launch."""
+        self._check_ready()
+        stream = _lib.stream_handle(self.device)
+        marks = [] if events else None
+        for _ in range(steps):
+            self._rate_kernel(stream)
+            _lib.call("wb_stable_dt", _lib.ptr(self._rate), self.cfl, 0.0, _lib.ptr(self._dt),
+                      _lib.ptr(self._t), stream)
+            self._heun(stream, marks)
+            self.steps += 1
+        return marks
 
     def check(self) -> None:
         """Raise FloatingPointError if a stage met a bad cell (mhd.py:852-857); synchronises."""

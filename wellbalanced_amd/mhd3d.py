@@ -118,8 +118,10 @@ class MHD3D:
         self._work: dict[str, torch.Tensor] = {}
         self._bad = torch.zeros(2, dtype=torch.int32, device=self.device)
         self._rate = torch.zeros(2, **opts)
+        self._dt = torch.zeros(1, **opts)
+        self._t = torch.zeros(1, **opts)
         self.check_interval = 1
-        self.time = 0.0
+        self.nxs = 32
         self.steps = 0
         self.period = float("nan")
         self._ready = False
@@ -193,6 +195,15 @@ class MHD3D:
         self.steps = 0
         self._ready = True
 
+    @property
+    def time(self) -> float:
+        """The model time (kept on the device; reading it synchronises)."""
+        return float(self._t.item())
+
+    @time.setter
+    def time(self, value: float) -> None:
+        self._t.fill_(float(value))
+
     def arrays(self):
         cells, pots = self._views(self._buf[0])
         return cells.cpu().numpy(), [p.cpu().numpy() for p in pots]
@@ -223,7 +234,7 @@ class MHD3D:
         self._work = w
         return w
 
-    def _stage_split(self, src, dst, dt, heun, slot, stream):
+    def _stage_split(self, src, dst, heun, slot, stream):
         nx, ny, nz = self.n
         call, ptr = _lib.call, _lib.ptr
         w = self._workspace()
@@ -241,40 +252,77 @@ class MHD3D:
             call("wb_edge3d", ptr(w["prim"]), ptr(w[f"s{a}"]), ptr(w[f"s{b}"]), nx, ny, nz, c,
                  ptr(w[f"e{c}"]), stream)
         call("wb_mhd3d_update", ptr(src), ptr(dst), ptr(w["s0"]), ptr(w["s1"]), ptr(w["s2"]),
-             ptr(w["e0"]), ptr(w["e1"]), ptr(w["e2"]), nx, ny, nz, dt, *self.spacing,
+             ptr(w["e0"]), ptr(w["e1"]), ptr(w["e2"]), nx, ny, nz, ptr(self._dt), *self.spacing,
              int(heun), stream)
+
+    def _stage_fused(self, src, dst, heun, slot, stream):
+        nx, ny, nz = self.n
+        pmask = sum(int(p) << a for a, p in enumerate(self.periodic))
+        _lib.call("wb_mhd3d_fill_halos", _lib.ptr(src), nx, ny, nz, pmask, stream)
+        name = "wb_mhd3d_stage" if self.precision == "fp64" else "wb_mhd3d_stage_mixed"
+        _lib.call(name, _lib.ptr(src), _lib.ptr(dst), nx, ny, nz, *self.spacing, *self.mean_field,
+                  self.gamma, LIMITER_CODES.index(self.limiter), _lib.ptr(self._dt), int(heun),
+                  self._bad.data_ptr() + 4 * slot, int(self.nxs), stream)
+
+    def _rate_kernel(self, stream: int) -> None:
+        nx, ny, nz = self.n
+        _lib.call("wb_rate3d", _lib.ptr(self._buf[0]), nx, ny, nz, *self.spacing,
+                  *self.mean_field, self.gamma, _lib.ptr(self._rate), stream)
 
     def stable_step(self) -> float:
         """cfl / (largest summed fast-wave rate) (mhd3d.py:674-724); synchronises."""
         self._check_ready()
-        nx, ny, nz = self.n
-        stream = _lib.stream_handle(self.device)
-        _lib.call("wb_rate3d", _lib.ptr(self._buf[0]), nx, ny, nz, *self.spacing,
-                  *self.mean_field, self.gamma, _lib.ptr(self._rate), stream)
+        self._rate_kernel(_lib.stream_handle(self.device))
         largest = float(self._rate[0].item())
         return self.cfl * (1.0 / largest)
 
+    def _heun(self, stream: int, events=None) -> None:
+        import torch
+
+        a, b = self._buf
+        stage = self._stage_split if self.kernels == "split" else self._stage_fused
+        if events is None:
+            stage(a, b, False, 0, stream)
+            stage(b, a, True, 1, stream)
+            return
+        marks = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        marks[0].record()
+        stage(a, b, False, 0, stream)
+        marks[1].record()
+        marks[2].record()
+        stage(b, a, True, 1, stream)
+        marks[3].record()
+        events.extend([(marks[0], marks[1]), (marks[2], marks[3])])
+
     def step(self, dt: float | None = None) -> float:
-        """Advance by one Heun (SSP-RK2) step (mhd3d.py:726-754)."""
+        """Advance by one Heun (SSP-RK2) step (mhd3d.py:726-754); return the step taken."""
         self._check_ready()
         if dt is None:
             dt = self.stable_step()
         dt = float(dt)
         stream = _lib.stream_handle(self.device)
-        a, b = self._buf
-        stage = self._stage_split if self.kernels == "split" else self._stage_fused
-        stage(a, b, dt, False, 0, stream)
-        stage(b, a, dt, True, 1, stream)
-        self.time += dt
+        self._dt.fill_(dt)
+        self._heun(stream)
+        self._t.add_(dt)
         self.steps += 1
         if self.check_interval and self.steps % self.check_interval == 0:
             self.check()
         return dt
 
-    def _stage_fused(self, src, dst, dt, heun, slot, stream):
-        from . import fused
-
-        fused.stage3d(self, src, dst, dt, heun, slot, stream)
+    def advance(self, steps: int = 1, events: bool = False):
+        """``steps`` steps at the stable step with dt and the time kept on the device (no
+        host synchronisation; `check` tests for bad cells).  With ``events`` returns the
+        HIP event pairs bracketing each stage launch (halo fill included)."""
+        self._check_ready()
+        stream = _lib.stream_handle(self.device)
+        marks = [] if events else None
+        for _ in range(steps):
+            self._rate_kernel(stream)
+            _lib.call("wb_stable_dt", _lib.ptr(self._rate), self.cfl, 0.0, _lib.ptr(self._dt),
+                      _lib.ptr(self._t), stream)
+            self._heun(stream, marks)
+            self.steps += 1
+        return marks
 
     def check(self) -> None:
         bad = self._bad.cpu()
