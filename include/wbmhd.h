@@ -156,6 +156,62 @@ int wb_rate3d(const double* state, int64_t nx, int64_t ny, int64_t nz, double dx
               double dz, double b0x, double b0y, double b0z, double gamma, double* out,
               void* stream);
 
+/* ---- 2D patch hierarchy (refined, periodic) ------------------------------------ */
+/* A hierarchy lives in one FP64 buffer, patch after patch (each a 2D patch state as
+ * above).  `patches` is an array of {int64 offset; int32 nx, ny; double dx, dy} per
+ * patch; the int64 tables are offsets into the buffer built on the host from the
+ * geometry (wellbalanced_amd/amr_maps.py).  Replaces patchwork's exchange_halos,
+ * FluxRegister.reflux and restrict as MHD2D calls them (mhd.py:744-777,889-994). */
+
+/* Same-level halo copies: ncells ghost cells (6 fields, plane sizes per entry) and
+ * nnodes A_z nodes (ghosts and shared boundary nodes from their canonical owner). */
+int wb_amr2d_copy(double* buf, const long long* cdst, const long long* csrc, const long long* cpd,
+                  const long long* cps, int ncells, const long long* ndst, const long long* nsrc,
+                  int nnodes, void* stream);
+
+/* Halos from the level below: bilinear cells (upper-cell weights wx, wy) and A_z
+ * nodes by the coarse node or the 4-point midpoint rule per axis (kind nkx, nky). */
+int wb_amr2d_interp(double* buf, const long long* dst, const long long* pd, const long long* c00,
+                    const long long* cy, const long long* pc, const double* wx, const double* wy,
+                    int ncells, const long long* ndst, const long long* nbase,
+                    const long long* nsx, const long long* nsy, const long long* nkx,
+                    const long long* nky, int nnodes, void* stream);
+
+/* Covered coarse cells := mean of their four fine children (rows of 5 int64). */
+int wb_amr2d_average(double* buf, const long long* rows, int n, void* stream);
+
+/* Coarse A_z nodes := the fine node on them (rows of 2 int64), mhd.py:981,991. */
+int wb_amr2d_restrict(double* buf, const long long* rows, int n, void* stream);
+
+/* The CFL rate over every patch cell (mhd.py:930-963): out = device double[2]. */
+int wb_amr2d_rate(const double* buf, const void* patches, int npatches, int maxcells, double b0x,
+                  double b0y, double gamma, double* out, void* stream);
+
+/* One stage of every patch in ONE persistent launch: `workgroups` workgroups take
+ * (patch, tile, segment) items (int32 triples) from *counter until `nitems` are done
+ * (the counter is zeroed first).  FP64 parity path; _mixed: FP32 flux. */
+int wb_amr2d_stage(const double* in, double* out, const void* patches, const void* items,
+                   int nitems, int32_t* counter, double b0x, double b0y, double gamma, int limiter,
+                   const double* dt, int heun, int32_t* bad, int nxs, int tile, int workgroups,
+                   void* stream);
+int wb_amr2d_stage_mixed(const double* in, double* out, const void* patches, const void* items,
+                         int nitems, int32_t* counter, double b0x, double b0y, double gamma,
+                         int limiter, const double* dt, int heun, int32_t* bad, int nxs, int tile,
+                         int workgroups, void* stream);
+
+/* Refluxing (FluxRegister.reflux, mhd.py:989-990) after a stage: for each group
+ * (first row, count) of rows (13 int64: coarse patch, cell, axis, side, face, and
+ * the two fine faces), the coarse flux the cell was updated with is replaced by the
+ * mean of the fine fluxes, recomputed from the stage input `src` (heun: half of the
+ * correction, as the stage's average). */
+int wb_amr2d_reflux(const double* src, double* dst, const void* patches, const long long* rows,
+                    const int32_t* groups, int ngroups, double b0x, double b0y, double gamma,
+                    int limiter, const double* dt, int heun, void* stream);
+int wb_amr2d_reflux_mixed(const double* src, double* dst, const void* patches,
+                          const long long* rows, const int32_t* groups, int ngroups, double b0x,
+                          double b0y, double gamma, int limiter, const double* dt, int heun,
+                          void* stream);
+
 #ifdef __cplusplus
 }
 #endif

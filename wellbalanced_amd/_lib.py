@@ -39,6 +39,15 @@ SIGNATURES: dict[str, tuple[type, list]] = {
     "wb_mhd3d_stage": (I, [P, P, I64, I64, I64] + [D] * 7 + [I, P, I, P, I, P]),
     "wb_mhd3d_stage_mixed": (I, [P, P, I64, I64, I64] + [D] * 7 + [I, P, I, P, I, P]),
     "wb_rate3d": (I, [P, I64, I64, I64] + [D] * 7 + [P, P]),
+    "wb_amr2d_copy": (I, [P] * 5 + [I] + [P, P, I, P]),
+    "wb_amr2d_interp": (I, [P] * 8 + [I] + [P] * 6 + [I, P]),
+    "wb_amr2d_average": (I, [P, P, I, P]),
+    "wb_amr2d_restrict": (I, [P, P, I, P]),
+    "wb_amr2d_rate": (I, [P, P, I, I, D, D, D, P, P]),
+    "wb_amr2d_stage": (I, [P, P, P, P, I, P, D, D, D, I, P, I, P, I, I, I, P]),
+    "wb_amr2d_stage_mixed": (I, [P, P, P, P, I, P, D, D, D, I, P, I, P, I, I, I, P]),
+    "wb_amr2d_reflux": (I, [P, P, P, P, P, I, D, D, D, I, P, I, P]),
+    "wb_amr2d_reflux_mixed": (I, [P, P, P, P, P, I, D, D, D, I, P, I, P]),
 }
 
 _lib = None

@@ -44,13 +44,13 @@ struct Stage2Args {
 };
 
 template <typename P, int TY>
-__global__ void __launch_bounds__((Row<TY>::THREADS)) k_stage2d(Stage2Args a) {
+__device__ __forceinline__ void stage2d_body(const Stage2Args& a, int tile, int seg,
+                                             Stage2Shared<P, TY>& sh) {
   using T = Row<TY>;
-  __shared__ Stage2Shared<P, TY> sh;
   const int tid = threadIdx.x;
   const int X = a.X, Y = a.Y, YN = Y + 1;
-  const int jy0 = 2 + blockIdx.x * TY;
-  const int ix0 = 2 + blockIdx.y * a.nxs;
+  const int jy0 = 2 + tile * TY;
+  const int ix0 = 2 + seg * a.nxs;
   const int tyv = min(TY, a.ny + 2 - jy0);
   const int nxv = min(a.nxs, a.nx + 2 - ix0);
   const bool last_y = jy0 + TY >= a.ny + 2, last_x = ix0 + a.nxs >= a.nx + 2;
@@ -213,6 +213,135 @@ __global__ void __launch_bounds__((Row<TY>::THREADS)) k_stage2d(Stage2Args a) {
         }
       }
     }
+  }
+}
+
+template <typename P, int TY>
+__global__ void __launch_bounds__((Row<TY>::THREADS)) k_stage2d(Stage2Args a) {
+  __shared__ Stage2Shared<P, TY> sh;
+  stage2d_body<P, TY>(a, blockIdx.x, blockIdx.y, sh);
+}
+
+
+// All tiles of all patches in one persistent launch: each workgroup takes the next
+// work item from a counter until the list is exhausted (the counter is zeroed by
+// k_zero_i32 before the launch).
+template <typename P, int TY>
+__global__ void __launch_bounds__((Row<TY>::THREADS))
+k_stage2d_batched(Stage2Args base, const PatchDesc* __restrict__ patches,
+                  const WorkItem* __restrict__ items, int nitems, int* __restrict__ counter,
+                  double* __restrict__ out_base) {
+  __shared__ Stage2Shared<P, TY> sh;
+  __shared__ int next;
+  for (;;) {
+    if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+    __syncthreads();
+    const int it = next;
+    __syncthreads();
+    if (it >= nitems) break;
+    const WorkItem w = items[it];
+    const PatchDesc pd = patches[w.patch];
+    Stage2Args a = base;
+    a.in = base.in + pd.off;
+    a.out = out_base + pd.off;
+    a.nx = pd.nx;
+    a.ny = pd.ny;
+    a.X = pd.nx + 4;
+    a.Y = pd.ny + 4;
+    a.dx = pd.dx;
+    a.dy = pd.dy;
+    stage2d_body<P, TY>(a, w.tile, w.seg, sh);
+    __syncthreads();
+  }
+}
+
+// The flux of one face of a patch from its (ghost-filled) stage input, computed as
+// the stage kernels compute it: primitive variables in FP64, reconstruction and
+// HLLD in P.  axis 0: x-face `fa` (low face of cell fa) at row fb; axis 1: y-face.
+template <typename P>
+__device__ void face_flux2d(const double* in, int X, int Y, double dx, double dy, double b0x,
+                            double b0y, double gamma, int limiter, int axis, int fa, int fb,
+                            double* out7) {
+  const size_t PL = (size_t)X * Y;
+  const double* az = in + 6 * PL;
+  const int YN = Y + 1;
+  auto AZ = [&](int i, int j) -> double { return az[(size_t)i * YN + j]; };
+  P q[7][4];
+  for (int s = 0; s < 4; ++s) {
+    const int i = axis == 0 ? fa - 2 + s : fa;
+    const int j = axis == 0 ? fb : fb - 2 + s;
+    const size_t c = (size_t)i * Y + j;
+    const double bx0 = (AZ(i, j + 1) - AZ(i, j)) / dy + b0x;
+    const double bx1 = (AZ(i + 1, j + 1) - AZ(i + 1, j)) / dy + b0x;
+    const double by0 = -(AZ(i + 1, j) - AZ(i, j)) / dx + b0y;
+    const double by1 = -(AZ(i + 1, j + 1) - AZ(i, j + 1)) / dx + b0y;
+    const double r = in[c];
+    const double u = in[PL + c] / r, v = in[2 * PL + c] / r, w = in[3 * PL + c] / r;
+    const double bx = 0.5 * (bx0 + bx1);
+    const double by = 0.5 * (by0 + by1);
+    const double b = in[5 * PL + c];
+    const double p = (gamma - 1.0) * (in[4 * PL + c] - 0.5 * r * (u * u + v * v + w * w) -
+                                      0.5 * (bx * bx + by * by + b * b));
+    const double pr[8] = {r, u, v, w, p, bx, by, b};
+    const int comp0[7] = {0, 1, 2, 3, 4, 6, 7};
+    const int comp1[7] = {0, 2, 1, 3, 4, 5, 7};
+    for (int c7 = 0; c7 < 7; ++c7) q[c7][s] = P(pr[axis == 0 ? comp0[c7] : comp1[c7]]);
+  }
+  const double bn = axis == 0 ? (AZ(fa, fb + 1) - AZ(fa, fb)) / dy + b0x
+                              : -(AZ(fa + 1, fb) - AZ(fa, fb)) / dx + b0y;
+  P L[7], R[7], f[7];
+  reconstruct<P>(q, limiter, L, R);
+  hlld<P>(L, R, P(bn), P(gamma), f);
+  for (int c = 0; c < 7; ++c) out7[c] = double(f[c]);
+}
+
+// Refluxing (patchwork FluxRegister.reflux, mhd.py:989-990): per uncovered coarse
+// cell next to finer patches, the coarse face fluxes it was updated with are
+// replaced by the mean of the two fine face fluxes.  rows (13 int64 each):
+// q, ca, cb, axis, side, face_a, face_b, p0, fa0, fb0, p1, fa1, fb1; groups[g] =
+// (first row, row count) of coarse cell g.  Deterministic: one thread per cell.
+template <typename P>
+__global__ void __launch_bounds__(64) k_reflux2d(const double* __restrict__ src, double* __restrict__ dst,
+                           const PatchDesc* __restrict__ patches, const long long* __restrict__ rows,
+                           const int* __restrict__ groups, int ngroups, double b0x, double b0y,
+                           double gamma, int limiter, const double* __restrict__ dtp, int heun) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ngroups) return;
+  constexpr int XR[6] = {0, 1, 2, 3, 4, 6};
+  constexpr int YR[6] = {0, 2, 1, 3, 4, 6};
+  const double dt = *dtp;
+  const int first = groups[2 * g], count = groups[2 * g + 1];
+  double delta[6] = {0, 0, 0, 0, 0, 0};
+  long long q = 0, ca = 0, cb = 0;
+  for (int k = 0; k < count; ++k) {
+    const long long* r = rows + 13 * (size_t)(first + k);
+    q = r[0];
+    ca = r[1];
+    cb = r[2];
+    const int axis = (int)r[3], side = (int)r[4];
+    const PatchDesc cq = patches[q];
+    const double dtd = axis == 0 ? dt / cq.dx : dt / cq.dy;
+    double fc[7], f0[7], f1[7];
+    face_flux2d<P>(src + cq.off, cq.nx + 4, cq.ny + 4, cq.dx, cq.dy, b0x, b0y, gamma, limiter,
+                   axis, (int)r[5], (int)r[6], fc);
+    const PatchDesc p0 = patches[r[7]], p1 = patches[r[10]];
+    face_flux2d<P>(src + p0.off, p0.nx + 4, p0.ny + 4, p0.dx, p0.dy, b0x, b0y, gamma, limiter,
+                   axis, (int)r[8], (int)r[9], f0);
+    face_flux2d<P>(src + p1.off, p1.nx + 4, p1.ny + 4, p1.dx, p1.dy, b0x, b0y, gamma, limiter,
+                   axis, (int)r[11], (int)r[12], f1);
+    for (int f = 0; f < 6; ++f) {
+      const int row = axis == 0 ? XR[f] : YR[f];
+      const double mean = 0.5 * (f0[row] + f1[row]);
+      const double d = side == 0 ? dtd * (fc[row] - mean) : dtd * (mean - fc[row]);
+      delta[f] = delta[f] + d;
+    }
+  }
+  const PatchDesc cq = patches[q];
+  const size_t PL = (size_t)(cq.nx + 4) * (cq.ny + 4);
+  const size_t c = cq.off + (size_t)ca * (cq.ny + 4) + cb;
+  for (int f = 0; f < 6; ++f) {
+    const double d = heun ? 0.5 * delta[f] : delta[f];
+    dst[c + f * PL] = dst[c + f * PL] + d;
   }
 }
 

@@ -22,3 +22,40 @@ extern "C" int wb_mhd2d_stage_mixed(const double* in, double* out, int64_t nx, i
   if (tile == 254) return launch_stage2d<float, 254>(a, s);
   return launch_stage2d<float, 126>(a, s);
 }
+
+// All patches of a periodic 2D hierarchy in one persistent launch per stage, and the
+// refluxing of the coarse cells next to finer patches.
+extern "C" int wb_amr2d_stage_mixed(const double* in, double* out, const void* patches,
+                                    const void* items, int nitems, int32_t* counter, double b0x,
+                                    double b0y, double gamma, int limiter, const double* dt,
+                                    int heun, int32_t* bad, int nxs, int tile, int workgroups,
+                                    void* stream) {
+  Stage2Args a;
+  const int rc = wb_fill_stage_args2d(&a, in, out, 2, 2, 1.0, 1.0, b0x, b0y, gamma, limiter, dt,
+                                      heun, 0, bad, nxs);
+  if (rc) return rc;
+  if (nitems <= 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, counter, 1);
+  const PatchDesc* pd = (const PatchDesc*)patches;
+  const WorkItem* wi = (const WorkItem*)items;
+  const unsigned grid = (unsigned)(workgroups > 0 ? workgroups : 1);
+  if (tile == 62)
+    hipLaunchKernelGGL((k_stage2d_batched<float, 62>), dim3(grid), dim3(Row<62>::THREADS), 0, s, a,
+                       pd, wi, nitems, (int*)counter, out);
+  else
+    hipLaunchKernelGGL((k_stage2d_batched<float, 126>), dim3(grid), dim3(Row<126>::THREADS), 0, s,
+                       a, pd, wi, nitems, (int*)counter, out);
+  return wb_last_error();
+}
+
+extern "C" int wb_amr2d_reflux_mixed(const double* src, double* dst, const void* patches,
+                                     const long long* rows, const int32_t* groups, int ngroups,
+                                     double b0x, double b0y, double gamma, int limiter,
+                                     const double* dt, int heun, void* stream) {
+  if (ngroups <= 0) return 0;
+  hipLaunchKernelGGL((k_reflux2d<float>), dim3((unsigned)((ngroups + 63) / 64)), dim3(64), 0,
+                     (hipStream_t)stream, src, dst, (const PatchDesc*)patches, rows,
+                     (const int*)groups, ngroups, b0x, b0y, gamma, limiter, dt, heun);
+  return wb_last_error();
+}

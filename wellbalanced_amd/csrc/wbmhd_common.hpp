@@ -14,6 +14,16 @@ static inline int wb_last_error() {
 
 namespace wb {
 
+// A patch of a hierarchy buffer and a unit of work (one tile x segment of a patch).
+struct PatchDesc {
+  long long off;  // first double of the patch's state
+  int nx, ny;
+  double dx, dy;
+};
+struct WorkItem {
+  int patch, tile, seg;
+};
+
 // A double as an unsigned key whose integer order is the order of the doubles
 // (sign-magnitude to two's-complement style flip), so atomicMin/atomicMax on the
 // keys reduce doubles exactly and in any order.
