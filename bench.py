@@ -39,6 +39,12 @@ def parse():
     p.add_argument("--fp64-kernels", default="fused", choices=("split", "fused"))
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-tune", action="store_true")
+    p.add_argument("--nxs", type=int, nargs=2, default=None, metavar=("FP64", "MIXED"),
+                   help="cells per workgroup along x (skips the autotuner)")
+    p.add_argument("--waves", type=int, default=None, help="waves/SIMD of the mixed stage")
+    p.add_argument("--calibrate", action="store_true",
+                   help="also run one device copy of 1 GiB (the traffic counters' reference)")
     return p.parse_args()
 
 
@@ -75,7 +81,7 @@ def cpu_baseline(cells_sample: int, budget: float):
         oracle.step()
         steps += 1
         elapsed = time.perf_counter() - t0
-        if elapsed >= budget or steps >= 50:
+        if elapsed >= budget or steps >= 100:
             break
     return cells_sample**3 * steps / elapsed, steps, elapsed
 
@@ -101,6 +107,25 @@ def main():
         mixed = MHD3D.from_problem(problem, kernels=mixed_kernels, precision="mixed")
     except (NotImplementedError, ValueError):
         mixed = None
+    tuned = {}
+    if args.nxs is not None:
+        fp64.nxs = args.nxs[0]
+        if mixed is not None:
+            mixed.nxs = args.nxs[1]
+            mixed.waves = args.waves or 2
+        tuned = {"fp64": {"nxs": fp64.nxs}, "mixed": {"nxs": args.nxs[1], "waves": args.waves or 2}}
+    elif not args.no_tune:
+        from wellbalanced_amd import autotune
+
+        tuned["fp64"] = autotune.tune(fp64, steps=2)
+        if mixed is not None:
+            tuned["mixed"] = autotune.tune(mixed, steps=2)
+    if args.calibrate:
+        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     fp64.check_interval = 0
     sec64, stage64 = time_model(fp64, args.steps, args.warmup)
     rate64 = cells * args.steps / sec64
@@ -144,12 +169,14 @@ def main():
             "cells": cells,
             "precision": precision,
             "fp64_kernels": args.fp64_kernels,
+            "launch": tuned,
         },
         "fp64_zone_updates_per_s": rate64,
         "speedup_vs_fp64": value / rate64,
         "l1_density_vs_fp64": l1,
         "linf_density_vs_fp64": linf,
         "hbm_fraction_algorithmic": value * 5 * STATE_BYTES_3D / (HBM_PEAK_GBS * 1e9),
+        "fp64_stage_ms": stage64,
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,
@@ -162,7 +189,7 @@ def main():
         },
     }
     if not args.no_cpu:
-        sample = 32
+        sample = 48
         cpu_rate, steps, elapsed = cpu_baseline(sample, args.cpu_seconds)
         out["cpu_baseline"] = {
             "value": cpu_rate,

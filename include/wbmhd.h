@@ -144,11 +144,12 @@ int wb_mhd3d_stage(const double* in, double* out, int64_t nx, int64_t ny, int64_
 
 /* The same stage with the reconstruction and the HLLD flux in FP32; the primitive
  * conversion (E -> p), the cell electric field, the CT average, the potential
- * and the update stay FP64. */
+ * and the update stay FP64.  waves: 2 or 3 waves per SIMD for the register
+ * allocation (a launch-shape choice; the results do not depend on it). */
 int wb_mhd3d_stage_mixed(const double* in, double* out, int64_t nx, int64_t ny, int64_t nz,
                          double dx, double dy, double dz, double b0x, double b0y, double b0z,
                          double gamma, int limiter, const double* dt, int heun, int32_t* bad,
-                         int nxs, void* stream);
+                         int nxs, int waves, void* stream);
 
 /* Replaces the rate of MHD3D.stable_step, mhd3d.py:674-724: out = device
  * double[2], largest rate and smallest density. */

@@ -181,3 +181,22 @@ def test_gpu_blast_hierarchy_conserves_mixed(gpu):
     rho_a = a[lay.offset : lay.offset + lay.plane].reshape(lay.X, lay.Y)[2:-2, 2:-2]
     rho_b = b[lay.offset : lay.offset + lay.plane].reshape(lay.X, lay.Y)[2:-2, 2:-2]
     assert float(np.mean(np.abs(rho_a - rho_b))) < 1e-5
+
+
+@pytest.mark.gpu
+def test_gpu_autotuned_launch_shape_keeps_the_bits(gpu):
+    from wellbalanced_amd import autotune
+    from wellbalanced_amd.amr import AMRMHD2D
+
+    h = _refined(3)
+    a = AMRMHD2D(h, limiter="minmod")
+    b = AMRMHD2D(h, limiter="minmod")
+    a.set_state(**_smooth_state())
+    b.set_state(**_smooth_state())
+    choice = autotune.tune(b, steps=1)
+    assert set(choice) == {"tile", "nxs", "workgroups"}
+    for _ in range(2):
+        dt = a.stable_step()
+        a.step(dt)
+        b.step(dt)
+    np.testing.assert_array_equal(a.buffer(), b.buffer())

@@ -37,7 +37,7 @@ SIGNATURES: dict[str, tuple[type, list]] = {
     "wb_edge3d": (I, [P, P, P, I64, I64, I64, I, P, P]),
     "wb_mhd3d_update": (I, [P, P] + [P] * 6 + [I64] * 3 + [P] + [D] * 3 + [I, P]),
     "wb_mhd3d_stage": (I, [P, P, I64, I64, I64] + [D] * 7 + [I, P, I, P, I, P]),
-    "wb_mhd3d_stage_mixed": (I, [P, P, I64, I64, I64] + [D] * 7 + [I, P, I, P, I, P]),
+    "wb_mhd3d_stage_mixed": (I, [P, P, I64, I64, I64] + [D] * 7 + [I, P, I, P, I, I, P]),
     "wb_rate3d": (I, [P, I64, I64, I64] + [D] * 7 + [P, P]),
     "wb_amr2d_copy": (I, [P] * 5 + [I] + [P, P, I, P]),
     "wb_amr2d_interp": (I, [P] * 8 + [I] + [P] * 6 + [I, P]),

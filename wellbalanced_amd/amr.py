@@ -75,6 +75,19 @@ class AMRMHD2D:
         self.steps = 0
         self._ready = False
 
+    def set_tiling(self, tile: int | None = None, nxs: int | None = None,
+                   workgroups: int | None = None) -> None:
+        """Change the launch shape (no effect on the results) and rebuild the work list."""
+        if tile is not None:
+            if tile not in (62, 126):
+                raise ValueError("tile is 62 or 126 cells")
+            self.tile = tile
+        if nxs is not None:
+            self.nxs = int(nxs)
+        if workgroups is not None:
+            self.workgroups = int(workgroups)
+        self._upload_tables()
+
     # -- tables on the device ------------------------------------------------------------
     def _dev(self, a, dtype=torch.int64):
         return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=self.device)

@@ -1,0 +1,76 @@
+"""Autotuner of the fused stage's launch shape.
+
+The fused stages have free parameters that do not change the results (the FP64
+path gives the same bits for any of them): the cells per workgroup along the
+contiguous axis (``tile``: 62, 126 or 254 in 2D -> 64, 128 or 256 threads) and the
+cells each workgroup marches along x (``nxs``), and for the refined hierarchy the
+number of persistent workgroups.  `tune` times each candidate on the model's own
+state for a few steps (state restored afterwards) and keeps the fastest; the choice
+is cached per (model kind, shape, precision) for the process.
+"""
+
+from __future__ import annotations
+
+import itertools
+import time
+
+import torch
+
+_CACHE: dict[tuple, dict] = {}
+
+DEFAULTS_2D = {"tile": [62, 126, 254], "nxs": [8, 16, 32, 64]}
+DEFAULTS_3D = {"nxs": [8, 16, 32, 64]}
+DEFAULTS_3D_MIXED = {"nxs": [8, 16, 32, 64], "waves": [2, 3]}
+DEFAULTS_AMR = {"tile": [62, 126], "nxs": [8, 16, 32], "workgroups": [512, 1024, 2048]}
+
+
+def _key(model) -> tuple:
+    shape = getattr(model, "n", None) or (getattr(model, "nx", None), getattr(model, "ny", None))
+    if hasattr(model, "layouts"):
+        shape = tuple((l.nx, l.ny) for l in model.layouts)
+    return (type(model).__name__, tuple(shape) if not isinstance(shape, tuple) else shape,
+            model.precision)
+
+
+def _apply(model, choice: dict) -> None:
+    if hasattr(model, "set_tiling"):
+        model.set_tiling(**choice)
+    else:
+        for k, v in choice.items():
+            setattr(model, k, v)
+
+
+def tune(model, space: dict | None = None, steps: int = 3) -> dict:
+    """Pick the fastest launch shape for ``model``; returns and applies it."""
+    key = _key(model)
+    if key in _CACHE:
+        _apply(model, _CACHE[key])
+        return _CACHE[key]
+    if space is None:
+        if hasattr(model, "layouts"):
+            space = DEFAULTS_AMR
+        elif hasattr(model, "n"):
+            space = DEFAULTS_3D if model.precision == "fp64" else DEFAULTS_3D_MIXED
+        else:
+            space = DEFAULTS_2D
+    saved = [b.clone() for b in model._buf], model._t.clone(), model.steps
+    best, best_t = None, float("inf")
+    names = list(space)
+    for values in itertools.product(*(space[n] for n in names)):
+        choice = dict(zip(names, values))
+        _apply(model, choice)
+        model.advance(1)
+        torch.cuda.synchronize(model.device)
+        t0 = time.perf_counter()
+        model.advance(steps)
+        torch.cuda.synchronize(model.device)
+        elapsed = time.perf_counter() - t0
+        if elapsed < best_t:
+            best, best_t = choice, elapsed
+        for b, s in zip(model._buf, saved[0]):
+            b.copy_(s)
+        model._t.copy_(saved[1])
+        model.steps = saved[2]
+    _apply(model, best)
+    _CACHE[key] = best
+    return best

@@ -46,13 +46,15 @@ struct Tile {
 template <typename P, int TY, int TZ>
 struct StageShared {
   using T = Tile<TY, TZ>;
-  P prim[2][8][T::NP];  // primitive planes f and f+1
-  double ex[2][T::NF];  // cell E_x of planes f, f+1
-  double ey[3][T::NF];  // cell E_y of planes f-1, f, f+1
+  P prim[8][T::NP];      // the primitive plane f (the y/z sweeps read it)
+  double ex[2][T::NF];   // cell E_x of planes f, f+1
+  double ey[3][T::NF];   // cell E_y of planes f-1, f, f+1
   double ez[3][T::NF];
-  P fx[3][T::NF];       // x-face f: rows 0 (mass), 5 (B_y), 6 (B_z)
-  P fy[2][7][T::NF];    // y-faces of planes f-1, f (face below the column's cell)
-  P fz[2][7][T::NF];
+  P fx[3][T::NF];        // x-face f: rows 0 (mass), 5 (B_y), 6 (B_z)
+  P fy[7][T::NF];        // y-faces of plane f (face below the column's cell)
+  P fz[7][T::NF];
+  P fyc[2][2][T::NF];    // rows 0 and 6 (B_x) of the y-faces of planes f-1, f
+  P fzc[2][2][T::NF];    // rows 0 and 5 (B_x) of the z-faces of planes f-1, f
 };
 
 struct StageArgs {
@@ -85,10 +87,53 @@ struct EdgePot {
   }
 };
 
-template <typename P, int TY, int TZ>
-__global__ void __launch_bounds__((Tile<TY, TZ>::THREADS))
+// The primitive variables of one cell (mhd3d.py:475-509).  FAST (the mixed path)
+// multiplies by the reciprocal spacings and divides once for 1/rho; otherwise the
+// divisions of the reference are kept for bit-for-bit parity.
+template <bool FAST>
+struct CellPrim {
+  double r, u, v, w, p, bx, by, bz;
+  __device__ __forceinline__ void compute(const StageArgs& a, const EdgePot& A, size_t PL, int i,
+                                          int j, int k, double rdx, double rdy, double rdz) {
+    const size_t c = cidx(a.g, i, j, k);
+    if (FAST) {
+      const double bx0 = (A.AZ(i, j + 1, k) - A.AZ(i, j, k)) * rdy - (A.AY(i, j, k + 1) - A.AY(i, j, k)) * rdz;
+      const double bx1 = (A.AZ(i + 1, j + 1, k) - A.AZ(i + 1, j, k)) * rdy - (A.AY(i + 1, j, k + 1) - A.AY(i + 1, j, k)) * rdz;
+      const double by0 = (A.AX(i, j, k + 1) - A.AX(i, j, k)) * rdz - (A.AZ(i + 1, j, k) - A.AZ(i, j, k)) * rdx;
+      const double by1 = (A.AX(i, j + 1, k + 1) - A.AX(i, j + 1, k)) * rdz - (A.AZ(i + 1, j + 1, k) - A.AZ(i, j + 1, k)) * rdx;
+      const double bz0 = (A.AY(i + 1, j, k) - A.AY(i, j, k)) * rdx - (A.AX(i, j + 1, k) - A.AX(i, j, k)) * rdy;
+      const double bz1 = (A.AY(i + 1, j, k + 1) - A.AY(i, j, k + 1)) * rdx - (A.AX(i, j + 1, k + 1) - A.AX(i, j, k + 1)) * rdy;
+      bx = 0.5 * (bx0 + bx1) + a.b0x;
+      by = 0.5 * (by0 + by1) + a.b0y;
+      bz = 0.5 * (bz0 + bz1) + a.b0z;
+      r = a.in[c];
+      const double ri = 1.0 / r;
+      u = a.in[PL + c] * ri;
+      v = a.in[2 * PL + c] * ri;
+      w = a.in[3 * PL + c] * ri;
+    } else {
+      bx = 0.5 * (A.bx(i, j, k, a.dy, a.dz, a.b0x) + A.bx(i + 1, j, k, a.dy, a.dz, a.b0x));
+      by = 0.5 * (A.by(i, j, k, a.dz, a.dx, a.b0y) + A.by(i, j + 1, k, a.dz, a.dx, a.b0y));
+      bz = 0.5 * (A.bz(i, j, k, a.dx, a.dy, a.b0z) + A.bz(i, j, k + 1, a.dx, a.dy, a.b0z));
+      r = a.in[c];
+      u = a.in[PL + c] / r;
+      v = a.in[2 * PL + c] / r;
+      w = a.in[3 * PL + c] / r;
+    }
+    p = (a.gamma - 1.0) * (a.in[4 * PL + c] - 0.5 * r * (u * u + v * v + w * w) -
+                           0.5 * (bx * bx + by * by + bz * bz));
+  }
+  __device__ __forceinline__ bool bad() const {
+    return !(finite(r) && finite(u) && finite(v) && finite(w) && finite(p) && finite(bx) &&
+             finite(by) && finite(bz) && r > 0.0 && p > 0.0);
+  }
+};
+
+template <typename P, int TY, int TZ, int WAVES>
+__global__ void __launch_bounds__((Tile<TY, TZ>::THREADS), WAVES)
 k_stage3d(StageArgs a) {
   using T = Tile<TY, TZ>;
+  constexpr bool FAST = sizeof(P) == 4;
   __shared__ StageShared<P, TY, TZ> sh;
   const Grid3 g = a.g;
   const int tid = threadIdx.x;
@@ -105,103 +150,100 @@ k_stage3d(StageArgs a) {
   const int fa = tid / T::FZ, fb = tid % T::FZ;
   const int ly = fa - 1, lz = fb - 1;
   const int gy = jy0 + ly, gz = kz0 + lz;
+  const int cgy = min(max(gy, 0), g.Y - 1), cgz = min(max(gz, 0), g.Z - 1);
   const EdgePot A{a.in + a.off_ax, a.in + a.off_ay, a.in + a.off_az, g.Y, g.Z};
   const size_t PL = (size_t)g.X * g.Y * g.Z;
   const double dt = *a.dt;
   const double dtdx = dt / a.dx, dtdy = dt / a.dy, dtdz = dt / a.dz;
+  const double rdx = 1.0 / a.dx, rdy = 1.0 / a.dy, rdz = 1.0 / a.dz;
 
   // register state of the column
   P q[7][4];                       // x-frame primitives of planes f-2 .. f+1
+  P own[8];                        // the column's primitives of plane f+1, until LDS is free
   P fprev[5], fcur[5];             // x-face fluxes (rows 0-4) of faces f-1 and f
-  double dyz[5];                   // dt/dy dFy + ... partial terms of the pending cell
   double dyy[5], dzz[5];
 #pragma unroll
   for (int c = 0; c < 7; ++c)
 #pragma unroll
     for (int s = 0; s < 4; ++s) q[c][s] = P(0);
 #pragma unroll
-  for (int r = 0; r < 5; ++r) { fprev[r] = fcur[r] = P(0); dyy[r] = dzz[r] = dyz[r] = 0.0; }
+  for (int r = 0; r < 5; ++r) { fprev[r] = fcur[r] = P(0); dyy[r] = dzz[r] = 0.0; }
+#pragma unroll
+  for (int r = 0; r < 8; ++r) own[r] = P(0);
 
-  // -- primitive plane `pl` into LDS buffer `buf`, cell E into the rings ----------
-  auto primitive_plane = [&](int pl, int buf, bool to_lds) {
-    const int pc = min(max(pl, 0), g.X - 1);  // clamp keeps every load in bounds
-    for (int idx = tid; idx < T::NP; idx += T::THREADS) {
-      const int py = idx / T::PZ, pz = idx % T::PZ;
-      const int j = min(max(jy0 + py - 2, 0), g.Y - 1), k = min(max(kz0 + pz - 2, 0), g.Z - 1);
-      const size_t c = cidx(g, pc, j, k);
-      const double bx = 0.5 * (A.bx(pc, j, k, a.dy, a.dz, a.b0x) + A.bx(pc + 1, j, k, a.dy, a.dz, a.b0x));
-      const double byv = 0.5 * (A.by(pc, j, k, a.dz, a.dx, a.b0y) + A.by(pc, j + 1, k, a.dz, a.dx, a.b0y));
-      const double bzv = 0.5 * (A.bz(pc, j, k, a.dx, a.dy, a.b0z) + A.bz(pc, j, k + 1, a.dx, a.dy, a.b0z));
-      const double r = a.in[c];
-      const double u = a.in[PL + c] / r, v = a.in[2 * PL + c] / r, w = a.in[3 * PL + c] / r;
-      const double p = (a.gamma - 1.0) * (a.in[4 * PL + c] - 0.5 * r * (u * u + v * v + w * w) -
-                                          0.5 * (bx * bx + byv * byv + bzv * bzv));
-      if (to_lds) {
-        sh.prim[buf][0][idx] = P(r);
-        sh.prim[buf][1][idx] = P(u);
-        sh.prim[buf][2][idx] = P(v);
-        sh.prim[buf][3][idx] = P(w);
-        sh.prim[buf][4][idx] = P(p);
-        sh.prim[buf][5][idx] = P(bx);
-        sh.prim[buf][6][idx] = P(byv);
-        sh.prim[buf][7][idx] = P(bzv);
-      }
-      // the face-column region [-1, T] holds the cell electric field
-      const int fy_ = py - 1, fz_ = pz - 1;
-      if (fy_ >= 0 && fy_ < T::FY && fz_ >= 0 && fz_ < T::FZ) {
-        const int f = fy_ * T::FZ + fz_;
-        sh.ex[pl & 1][f] = byv * w - bzv * v;
-        sh.ey[(pl + 3) % 3][f] = bzv * u - bx * w;
-        sh.ez[(pl + 3) % 3][f] = bx * v - byv * u;
-      }
-      // bad cells: only the tile's own interior cells of the segment's planes
-      if (py >= 2 && py < 2 + tyv && pz >= 2 && pz < 2 + tzv && pl >= ix0 && pl < ix0 + nxv) {
-        const bool ok = finite(r) && finite(u) && finite(v) && finite(w) && finite(p) &&
-                        finite(bx) && finite(byv) && finite(bzv) && r > 0.0 && p > 0.0;
-        if (!ok) atomicAdd(a.bad, 1);
-      }
-    }
-  };
-  // the x-frame primitives of the own column, pushed into the register ring
-  auto push_column = [&](int buf) {
+  // the own column of plane `pl`: registers (ring + own), cell E, bad cells
+  auto own_column = [&](int pl) {
     if (!fthread) return;
-    const int idx = (fa + 1) * T::PZ + (fb + 1);
+    const int pc = min(max(pl, 0), g.X - 1);
+    CellPrim<FAST> c;
+    c.compute(a, A, PL, pc, cgy, cgz, rdx, rdy, rdz);
+    own[0] = P(c.r); own[1] = P(c.u); own[2] = P(c.v); own[3] = P(c.w);
+    own[4] = P(c.p); own[5] = P(c.bx); own[6] = P(c.by); own[7] = P(c.bz);
     constexpr int comp[7] = {0, 1, 2, 3, 4, 6, 7};
 #pragma unroll
-    for (int c = 0; c < 7; ++c) {
-      q[c][0] = q[c][1];
-      q[c][1] = q[c][2];
-      q[c][2] = q[c][3];
-      q[c][3] = sh.prim[buf][comp[c]][idx];
+    for (int k = 0; k < 7; ++k) {
+      q[k][0] = q[k][1];
+      q[k][1] = q[k][2];
+      q[k][2] = q[k][3];
+      q[k][3] = own[comp[k]];
+    }
+    sh.ex[pl & 1][tid] = c.by * c.w - c.bz * c.v;
+    sh.ey[pl % 3][tid] = c.bz * c.u - c.bx * c.w;
+    sh.ez[pl % 3][tid] = c.bx * c.v - c.by * c.u;
+    if (ly >= 0 && ly < tyv && lz >= 0 && lz < tzv && pl >= ix0 && pl < ix0 + nxv && c.bad())
+      atomicAdd(a.bad, 1);
+  };
+  // plane `pl` into LDS: the own column from registers, the outer ring computed here
+  auto store_plane = [&](int pl) {
+    if (fthread) {
+      const int idx = (fa + 1) * T::PZ + (fb + 1);
+#pragma unroll
+      for (int r = 0; r < 8; ++r) sh.prim[r][idx] = own[r];
+    }
+    constexpr int NH = T::NP - T::NF;  // the outermost ring of primitive columns
+    const int pc = min(max(pl, 0), g.X - 1);
+    for (int hidx = tid; hidx < NH; hidx += T::THREADS) {
+      int py, pz;
+      if (hidx < T::PZ) { py = 0; pz = hidx; }
+      else if (hidx < 2 * T::PZ) { py = T::PY - 1; pz = hidx - T::PZ; }
+      else { const int e = hidx - 2 * T::PZ; py = 1 + e / 2; pz = (e & 1) ? T::PZ - 1 : 0; }
+      const int j = min(max(jy0 + py - 2, 0), g.Y - 1), k = min(max(kz0 + pz - 2, 0), g.Z - 1);
+      CellPrim<FAST> c;
+      c.compute(a, A, PL, pc, j, k, rdx, rdy, rdz);
+      const int idx = py * T::PZ + pz;
+      sh.prim[0][idx] = P(c.r); sh.prim[1][idx] = P(c.u); sh.prim[2][idx] = P(c.v);
+      sh.prim[3][idx] = P(c.w); sh.prim[4][idx] = P(c.p); sh.prim[5][idx] = P(c.bx);
+      sh.prim[6][idx] = P(c.by); sh.prim[7][idx] = P(c.bz);
     }
   };
+  auto face_bx = [&](int i) -> double {
+    if (FAST) return (A.AZ(i, cgy + 1, cgz) - A.AZ(i, cgy, cgz)) * rdy - (A.AY(i, cgy, cgz + 1) - A.AY(i, cgy, cgz)) * rdz + a.b0x;
+    return A.bx(i, cgy, cgz, a.dy, a.dz, a.b0x);
+  };
+  auto face_by = [&](int i) -> double {
+    if (FAST) return (A.AX(i, cgy, cgz + 1) - A.AX(i, cgy, cgz)) * rdz - (A.AZ(i + 1, cgy, cgz) - A.AZ(i, cgy, cgz)) * rdx + a.b0y;
+    return A.by(i, cgy, cgz, a.dz, a.dx, a.b0y);
+  };
+  auto face_bz = [&](int i) -> double {
+    if (FAST) return (A.AY(i + 1, cgy, cgz) - A.AY(i, cgy, cgz)) * rdx - (A.AX(i, cgy + 1, cgz) - A.AX(i, cgy, cgz)) * rdy + a.b0z;
+    return A.bz(i, cgy, cgz, a.dx, a.dy, a.b0z);
+  };
 
-  // prologue: planes ix0-2 (ring only) and ix0-1 (ring + LDS)
-  {
-    const int pl = ix0 - 2;
-    primitive_plane(pl, pl & 1, true);
-    __syncthreads();
-    push_column(pl & 1);
-    __syncthreads();
-    primitive_plane(pl + 1, (pl + 1) & 1, true);
-    __syncthreads();
-    push_column((pl + 1) & 1);
-  }
+  // prologue: planes ix0-2 (ring) and ix0-1 (ring + LDS)
+  own_column(ix0 - 2);
+  own_column(ix0 - 1);
+  store_plane(ix0 - 1);
+  __syncthreads();
 
   for (int f = ix0 - 1; f <= ix0 + nxv; ++f) {
-    __syncthreads();
-    // 1. primitive plane f+1
-    primitive_plane(f + 1, (f + 1) & 1, true);
-    __syncthreads();
-    push_column((f + 1) & 1);
-    const int cur = f & 1;
+    const int cur = f & 1, prv = cur ^ 1;
+    // a-b. the own column of plane f+1 into the ring; c. x-face f from the ring
+    own_column(f + 1);
     if (fthread) {
-      // 3. x-face f of the column (lower face of cell f): planes f-2 .. f+1
       if (f >= ix0) {
         P L[7], R[7], fl[7];
         reconstruct<P>(q, a.limiter, L, R);
-        const P bn = P(A.bx(f, min(max(gy, 0), g.Y - 1), min(max(gz, 0), g.Z - 1), a.dy, a.dz, a.b0x));
-        hlld<P>(L, R, bn, P(a.gamma), fl);
+        hlld<P>(L, R, P(face_bx(f)), P(a.gamma), fl);
 #pragma unroll
         for (int r = 0; r < 5; ++r) {
           fprev[r] = fcur[r];
@@ -211,57 +253,56 @@ k_stage3d(StageArgs a) {
         sh.fx[1][tid] = fl[5];
         sh.fx[2][tid] = fl[6];
       }
-      // 4. y- and z-faces of plane f below the column's cell
-      {
-        const int pidx = (fa + 1) * T::PZ + (fb + 1);
-        // y-face between cells (ly-1) and ly: cells ly-2 .. ly+1 in frame (y, z, x)
-        if (fa >= 1) {
-          constexpr int comp[7] = {0, 2, 3, 1, 4, 7, 5};
-          P qq[7][4];
+      // d. y- and z-faces of plane f below the column's cell, from the LDS plane
+      const int pidx = (fa + 1) * T::PZ + (fb + 1);
+      const int fi = min(max(f, 0), g.X - 1);
+      if (fa >= 1) {
+        constexpr int comp[7] = {0, 2, 3, 1, 4, 7, 5};
+        P qq[7][4];
 #pragma unroll
-          for (int c = 0; c < 7; ++c)
+        for (int c = 0; c < 7; ++c)
 #pragma unroll
-            for (int s = 0; s < 4; ++s) qq[c][s] = sh.prim[cur][comp[c]][pidx + (s - 2) * T::PZ];
-          P L[7], R[7], fl[7];
-          reconstruct<P>(qq, a.limiter, L, R);
-          const int jj = min(max(gy, 0), g.Y - 1), kk = min(max(gz, 0), g.Z - 1);
-          const P bn = P(A.by(min(max(f, 0), g.X - 1), jj, kk, a.dz, a.dx, a.b0y));
-          hlld<P>(L, R, bn, P(a.gamma), fl);
+          for (int s = 0; s < 4; ++s) qq[c][s] = sh.prim[comp[c]][pidx + (s - 2) * T::PZ];
+        P L[7], R[7], fl[7];
+        reconstruct<P>(qq, a.limiter, L, R);
+        hlld<P>(L, R, P(face_by(fi)), P(a.gamma), fl);
 #pragma unroll
-          for (int r = 0; r < 7; ++r) sh.fy[cur][r][tid] = fl[r];
-        }
-        if (fb >= 1) {
-          constexpr int comp[7] = {0, 3, 1, 2, 4, 5, 6};
-          P qq[7][4];
+        for (int r = 0; r < 7; ++r) sh.fy[r][tid] = fl[r];
+        sh.fyc[cur][0][tid] = fl[0];
+        sh.fyc[cur][1][tid] = fl[6];
+      }
+      if (fb >= 1) {
+        constexpr int comp[7] = {0, 3, 1, 2, 4, 5, 6};
+        P qq[7][4];
 #pragma unroll
-          for (int c = 0; c < 7; ++c)
+        for (int c = 0; c < 7; ++c)
 #pragma unroll
-            for (int s = 0; s < 4; ++s) qq[c][s] = sh.prim[cur][comp[c]][pidx + (s - 2)];
-          P L[7], R[7], fl[7];
-          reconstruct<P>(qq, a.limiter, L, R);
-          const int jj = min(max(gy, 0), g.Y - 1), kk = min(max(gz, 0), g.Z - 1);
-          const P bn = P(A.bz(min(max(f, 0), g.X - 1), jj, kk, a.dx, a.dy, a.b0z));
-          hlld<P>(L, R, bn, P(a.gamma), fl);
+          for (int s = 0; s < 4; ++s) qq[c][s] = sh.prim[comp[c]][pidx + (s - 2)];
+        P L[7], R[7], fl[7];
+        reconstruct<P>(qq, a.limiter, L, R);
+        hlld<P>(L, R, P(face_bz(fi)), P(a.gamma), fl);
 #pragma unroll
-          for (int r = 0; r < 7; ++r) sh.fz[cur][r][tid] = fl[r];
-        }
+        for (int r = 0; r < 7; ++r) sh.fz[r][tid] = fl[r];
+        sh.fzc[cur][0][tid] = fl[0];
+        sh.fzc[cur][1][tid] = fl[5];
       }
     }
     __syncthreads();
+    // e. the LDS plane is free: plane f+1 goes in for the next iteration
+    store_plane(f + 1);
     if (fthread) {
-      const int prv = cur ^ 1;
-      const int e0 = (f + 2) % 3, e1 = (f + 3) % 3;  // planes f-1 and f of the E rings
+      const int e0 = (f + 2) % 3, e1 = f % 3;  // planes f-1 and f of the E rings
       const bool in_y = ly >= 0 && (ly < tyv || (ly == tyv && last_y));
       const bool in_z = lz >= 0 && (lz < tzv || (lz == tzv && last_z));
       const bool cell_y = ly >= 0 && ly < tyv, cell_z = lz >= 0 && lz < tzv;
       const int tym = tid - T::FZ, tzm = tid - 1, tymz = tid - T::FZ - 1;
-      // 6a. E_x on the x-cell plane f at (y-node ly, z-node lz): plane (a=y, b=z)
-      if (f >= ix0 && f < ix0 + nxv && in_y && in_z && ly <= TY && lz <= TZ) {
+      // f1. E_x on the x-cell plane f at (y-node ly, z-node lz): plane (a=y, b=z)
+      if (f >= ix0 && f < ix0 + nxv && in_y && in_z) {
         const double e = ct_node(
-            double(sh.fy[cur][5][tzm]), double(sh.fy[cur][5][tid]),
-            double(sh.fz[cur][6][tym]), double(sh.fz[cur][6][tid]),
-            double(sh.fy[cur][0][tzm]), double(sh.fy[cur][0][tid]),
-            double(sh.fz[cur][0][tym]), double(sh.fz[cur][0][tid]),
+            double(sh.fy[5][tzm]), double(sh.fy[5][tid]),
+            double(sh.fz[6][tym]), double(sh.fz[6][tid]),
+            double(sh.fy[0][tzm]), double(sh.fy[0][tid]),
+            double(sh.fz[0][tym]), double(sh.fz[0][tid]),
             sh.ex[f & 1][tymz], sh.ex[f & 1][tzm], sh.ex[f & 1][tym], sh.ex[f & 1][tid]);
         const size_t o = a.off_ax + ((size_t)f * (g.Y + 1) + gy) * (g.Z + 1) + gz;
         double v = a.in[o] - dt * e;
@@ -269,12 +310,12 @@ k_stage3d(StageArgs a) {
         a.out[o] = v;
       }
       const bool node_x = f >= ix0 && (f < ix0 + nxv || (f == ix0 + nxv && last_x));
-      // 6b. E_y at (x-node f, y-cell ly, z-node lz): plane (a=z, b=x)
+      // f2. E_y at (x-node f, y-cell ly, z-node lz): plane (a=z, b=x)
       if (node_x && cell_y && in_z) {
         const double e = ct_node(
-            double(sh.fz[prv][5][tid]), double(sh.fz[cur][5][tid]),
+            double(sh.fzc[prv][1][tid]), double(sh.fzc[cur][1][tid]),
             double(sh.fx[2][tzm]), double(sh.fx[2][tid]),
-            double(sh.fz[prv][0][tid]), double(sh.fz[cur][0][tid]),
+            double(sh.fzc[prv][0][tid]), double(sh.fzc[cur][0][tid]),
             double(sh.fx[0][tzm]), double(sh.fx[0][tid]),
             sh.ey[e0][tzm], sh.ey[e0][tid], sh.ey[e1][tzm], sh.ey[e1][tid]);
         const size_t o = a.off_ay + ((size_t)f * g.Y + gy) * (g.Z + 1) + gz;
@@ -282,20 +323,20 @@ k_stage3d(StageArgs a) {
         if (a.heun) v = (v + a.out[o]) * 0.5;
         a.out[o] = v;
       }
-      // 6c. E_z at (x-node f, y-node ly, z-cell lz): plane (a=x, b=y)
+      // f3. E_z at (x-node f, y-node ly, z-cell lz): plane (a=x, b=y)
       if (node_x && in_y && cell_z) {
         const double e = ct_node(
             double(sh.fx[1][tym]), double(sh.fx[1][tid]),
-            double(sh.fy[prv][6][tid]), double(sh.fy[cur][6][tid]),
+            double(sh.fyc[prv][1][tid]), double(sh.fyc[cur][1][tid]),
             double(sh.fx[0][tym]), double(sh.fx[0][tid]),
-            double(sh.fy[prv][0][tid]), double(sh.fy[cur][0][tid]),
+            double(sh.fyc[prv][0][tid]), double(sh.fyc[cur][0][tid]),
             sh.ez[e0][tym], sh.ez[e1][tym], sh.ez[e0][tid], sh.ez[e1][tid]);
         const size_t o = a.off_az + ((size_t)f * (g.Y + 1) + gy) * g.Z + gz;
         double v = a.in[o] - dt * e;
         if (a.heun) v = (v + a.out[o]) * 0.5;
         a.out[o] = v;
       }
-      // 7. update of cell (f-1, ly, lz): x-faces f-1 and f, y/z terms of plane f-1
+      // g. update of cell (f-1, ly, lz): x-faces f-1 and f, y/z terms of plane f-1
       if (cell_y && cell_z) {
         if (f - 1 >= ix0 && f - 1 < ix0 + nxv) {
           const size_t c = cidx(g, f - 1, gy, gz);
@@ -315,21 +356,24 @@ k_stage3d(StageArgs a) {
           constexpr int zrow[5] = {0, 2, 3, 1, 4};
 #pragma unroll
           for (int r = 0; r < 5; ++r) {
-            dyy[r] = dtdy * (double(sh.fy[cur][yrow[r]][typ]) - double(sh.fy[cur][yrow[r]][tid]));
-            dzz[r] = dtdz * (double(sh.fz[cur][zrow[r]][tzp]) - double(sh.fz[cur][zrow[r]][tid]));
+            dyy[r] = dtdy * (double(sh.fy[yrow[r]][typ]) - double(sh.fy[yrow[r]][tid]));
+            dzz[r] = dtdz * (double(sh.fz[zrow[r]][tzp]) - double(sh.fz[zrow[r]][tid]));
           }
         }
       }
     }
+    __syncthreads();
   }
 }
 
-template <typename P, int TY, int TZ>
+// WAVES: the waves per SIMD the register allocation must allow (2 or 3 for the
+// mixed path: 3 trades a few spills for occupancy; the autotuner picks).
+template <typename P, int TY, int TZ, int WAVES>
 inline int launch_stage3d(const StageArgs& a, hipStream_t s) {
   using T = Tile<TY, TZ>;
   const int tiles = ((a.g.ny + TY - 1) / TY) * ((a.g.nz + TZ - 1) / TZ);
   const int segs = (a.g.nx + a.nxs - 1) / a.nxs;
-  hipLaunchKernelGGL((k_stage3d<P, TY, TZ>), dim3(tiles, segs), dim3(T::THREADS), 0, s, a);
+  hipLaunchKernelGGL((k_stage3d<P, TY, TZ, WAVES>), dim3(tiles, segs), dim3(T::THREADS), 0, s, a);
   return wb_last_error();
 }
 

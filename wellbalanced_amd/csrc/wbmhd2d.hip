@@ -121,20 +121,20 @@ __global__ void k_ct2d(const double* __restrict__ fx_by, const double* __restric
 // mhd_rate_pyccel.py:30-83: the largest fast-wave rate and the smallest density
 // of the block's own cells, reduced into ordered-integer keys (order-independent,
 // so exact).
-__global__ void k_rate2d(const double* __restrict__ rho, const double* __restrict__ mx,
+__global__ void __launch_bounds__(256) k_rate2d(const double* __restrict__ rho, const double* __restrict__ mx,
                          const double* __restrict__ my, const double* __restrict__ mz,
                          const double* __restrict__ en, const double* __restrict__ bz,
                          const double* __restrict__ az, int X, int Y, int h, double dx,
                          double dy, double b0x, double b0y, double gamma,
                          unsigned long long* __restrict__ keys) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  const int a = blockIdx.y;
   const int nx = X - 2 * h, ny = Y - 2 * h;
-  double rate = 0.0, r = __builtin_inf();
-  if (a < nx && b < ny) {
+  const unsigned total = (unsigned)nx * ny;
+  double rate = 0.0, rmin = __builtin_inf();
+  for (unsigned id = blockIdx.x * blockDim.x + threadIdx.x; id < total; id += gridDim.x * blockDim.x) {
+    const int b = id % ny, a = id / ny;
     const int i = a + h, j = b + h, YN = Y + 1;
     const size_t c = (size_t)i * Y + j;
-    r = rho[c];
+    const double r = rho[c];
     const double u = mx[c] / r, v = my[c] / r, w = mz[c] / r;
     const double bx = 0.5 * ((az[(size_t)i * YN + j + 1] - az[(size_t)i * YN + j]) / dy + b0x +
                              ((az[(size_t)(i + 1) * YN + j + 1] - az[(size_t)(i + 1) * YN + j]) / dy + b0x));
@@ -143,10 +143,12 @@ __global__ void k_rate2d(const double* __restrict__ rho, const double* __restric
     const double bzc = bz[c];
     const double p = (gamma - 1.0) * (en[c] - 0.5 * r * (u * u + v * v + w * w) -
                                       0.5 * (bx * bx + by * by + bzc * bzc));
-    rate = (__builtin_fabs(u) + fast_speed_rate(r, p, bx, by, bzc, gamma)) / dx +
-           (__builtin_fabs(v) + fast_speed_rate(r, p, by, bx, bzc, gamma)) / dy;
+    const double rc = (__builtin_fabs(u) + fast_speed_rate(r, p, bx, by, bzc, gamma)) / dx +
+                      (__builtin_fabs(v) + fast_speed_rate(r, p, by, bx, bzc, gamma)) / dy;
+    if (rc > rate) rate = rc;
+    if (r < rmin) rmin = r;
   }
-  block_reduce_rate_rho(rate, r, keys);
+  block_reduce_rate_rho_once(rate, rmin, keys);
 }
 
 // Halo pass along one axis for the 6 cell planes and the A_z node plane.
@@ -342,7 +344,10 @@ int wb_rate2d(const double* rho, const double* mx, const double* my, const doubl
   hipStream_t s = (hipStream_t)stream;
   unsigned long long* keys = (unsigned long long*)out;
   hipLaunchKernelGGL(k_rate_init, dim3(1), dim3(64), 0, s, keys);
-  dim3 grid((unsigned)((Y - 2 * halo + 255) / 256), (unsigned)(X - 2 * halo));
+  const int64_t total = (X - 2 * halo) * (Y - 2 * halo);
+  if (total >= ((int64_t)1 << 31)) return WB_EINVAL;
+  const int64_t want = (total + 255) / 256;
+  dim3 grid((unsigned)(want < 2048 ? want : 2048));
   hipLaunchKernelGGL(k_rate2d, grid, dim3(256), 0, s, rho, mx, my, mz, energy, bz, az, (int)X,
                      (int)Y, halo, dx, dy, b0x, b0y, gamma, keys);
   hipLaunchKernelGGL(k_rate_finish, dim3(1), dim3(64), 0, s, keys, out);

@@ -247,31 +247,33 @@ __global__ void k_update3d(const double* __restrict__ in, double* __restrict__ o
 
 // mhd3d.py:674-724: the largest sum over axes of (|u_a| + c_f,a) / dx_a and the
 // smallest density of the block's own cells.
-__global__ void k_rate3d(const double* __restrict__ state, Dims3 g, double dx, double dy, double dz,
-                         double b0x, double b0y, double b0z, double gamma,
-                         unsigned long long* __restrict__ keys) {
+__global__ void __launch_bounds__(256) k_rate3d(const double* __restrict__ state, Dims3 g,
+                                                double dx, double dy, double dz, double b0x,
+                                                double b0y, double b0z, double gamma,
+                                                unsigned long long* __restrict__ keys) {
   const int nx = g.X - 4, ny = g.Y - 4, nz = g.Z - 4, h = HALO;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int a = blockIdx.y;
-  const int c = t % nz, b = t / nz;
-  double rate = 0.0, r = __builtin_inf();
-  if (a < nx && b < ny) {
-    const Pot A = pot_of(state, g);
+  const unsigned total = (unsigned)nx * ny * nz;
+  const Pot A = pot_of(state, g);
+  const size_t P = g.cells();
+  double rate = 0.0, rmin = __builtin_inf();
+  for (unsigned id = blockIdx.x * blockDim.x + threadIdx.x; id < total; id += gridDim.x * blockDim.x) {
+    const int c = id % nz, t = id / nz, b = t % ny, a = t / ny;
     const int i = a + h, j = b + h, k = c + h;
     const size_t cell = ((size_t)i * g.Y + j) * g.Z + k;
-    const size_t P = g.cells();
     const double bx = 0.5 * (A.bx(i, j, k, dy, dz, b0x) + A.bx(i + 1, j, k, dy, dz, b0x));
     const double by = 0.5 * (A.by(i, j, k, dz, dx, b0y) + A.by(i, j + 1, k, dz, dx, b0y));
     const double bz = 0.5 * (A.bz(i, j, k, dx, dy, b0z) + A.bz(i, j, k + 1, dx, dy, b0z));
-    r = state[cell];
+    const double r = state[cell];
     const double u = state[P + cell] / r, v = state[2 * P + cell] / r, w = state[3 * P + cell] / r;
     const double p = (gamma - 1.0) * (state[4 * P + cell] - 0.5 * r * (u * u + v * v + w * w) -
                                       0.5 * (bx * bx + by * by + bz * bz));
-    rate = (__builtin_fabs(u) + fast_speed_rate(r, p, bx, by, bz, gamma)) / dx +
-           (__builtin_fabs(v) + fast_speed_rate(r, p, by, bz, bx, gamma)) / dy +
-           (__builtin_fabs(w) + fast_speed_rate(r, p, bz, bx, by, gamma)) / dz;
+    const double rc = (__builtin_fabs(u) + fast_speed_rate(r, p, bx, by, bz, gamma)) / dx +
+                      (__builtin_fabs(v) + fast_speed_rate(r, p, by, bz, bx, gamma)) / dy +
+                      (__builtin_fabs(w) + fast_speed_rate(r, p, bz, bx, by, gamma)) / dz;
+    if (rc > rate) rate = rc;
+    if (r < rmin) rmin = r;
   }
-  block_reduce_rate_rho(rate, r, keys);
+  block_reduce_rate_rho_once(rate, rmin, keys);
 }
 
 // One halo pass along `axis` over the 8 arrays of the state.  kind 0 periodic,
@@ -422,8 +424,10 @@ int wb_rate3d(const double* state, int64_t nx, int64_t ny, int64_t nz, double dx
   Dims3 g{(int)nx + 4, (int)ny + 4, (int)nz + 4};
   unsigned long long* keys = (unsigned long long*)out;
   hipLaunchKernelGGL(k_rate_init, dim3(1), dim3(64), 0, s, keys);
-  const int64_t inner = ny * nz;
-  dim3 grid((unsigned)((inner + 255) / 256), (unsigned)nx);
+  const int64_t total = nx * ny * nz;
+  if (total >= ((int64_t)1 << 31)) return WB_EINVAL;
+  const int64_t want = (total + 255) / 256;
+  dim3 grid((unsigned)(want < 2048 ? want : 2048));
   hipLaunchKernelGGL(k_rate3d, grid, dim3(256), 0, s, state, g, dx, dy, dz, b0x, b0y, b0z, gamma,
                      keys);
   hipLaunchKernelGGL(k_rate_finish, dim3(1), dim3(64), 0, s, keys, out);

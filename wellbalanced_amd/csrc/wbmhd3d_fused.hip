@@ -59,5 +59,5 @@ extern "C" int wb_mhd3d_stage(const double* in, double* out, int64_t nx, int64_t
   const int rc = wb_fill_stage_args3d(&a, in, out, nx, ny, nz, dx, dy, dz, b0x, b0y, b0z, gamma,
                                       limiter, dt, heun, bad, nxs);
   if (rc) return rc;
-  return launch_stage3d<double, 14, 14>(a, (hipStream_t)stream);
+  return launch_stage3d<double, 14, 14, 1>(a, (hipStream_t)stream);
 }

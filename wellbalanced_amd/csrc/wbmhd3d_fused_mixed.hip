@@ -14,10 +14,11 @@ extern "C" int wb_mhd3d_stage_mixed(const double* in, double* out, int64_t nx, i
                                     int64_t nz, double dx, double dy, double dz, double b0x,
                                     double b0y, double b0z, double gamma, int limiter,
                                     const double* dt, int heun, int32_t* bad, int nxs,
-                                    void* stream) {
+                                    int waves, void* stream) {
   StageArgs a;
   const int rc = wb_fill_stage_args3d(&a, in, out, nx, ny, nz, dx, dy, dz, b0x, b0y, b0z, gamma,
                                       limiter, dt, heun, bad, nxs);
   if (rc) return rc;
-  return launch_stage3d<float, 14, 14>(a, (hipStream_t)stream);
+  if (waves == 3) return launch_stage3d<float, 14, 14, 3>(a, (hipStream_t)stream);
+  return launch_stage3d<float, 14, 14, 2>(a, (hipStream_t)stream);
 }

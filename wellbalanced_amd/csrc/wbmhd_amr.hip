@@ -111,7 +111,7 @@ __global__ void k_rate_amr(const double* __restrict__ buf, const PatchDesc* __re
     rate = (__builtin_fabs(u) + fast_speed_rate(r, p, bx, by, bzc, gamma)) / dx +
            (__builtin_fabs(v) + fast_speed_rate(r, p, by, bx, bzc, gamma)) / dy;
   }
-  block_reduce_rate_rho(rate, r, keys);
+  block_reduce_rate_rho_once(rate, r, keys);
 }
 
 inline dim3 blocks(int n) { return dim3((unsigned)((n + 255) / 256)); }

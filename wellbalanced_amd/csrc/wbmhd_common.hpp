@@ -79,4 +79,36 @@ __device__ __forceinline__ void block_reduce_rate_rho(double rate, double r,
   }
 }
 
+// Block-wide max rate / min density: wavefront shuffles, the waves combined in
+// LDS, then ONE atomic pair per workgroup (blockDim a multiple of 64, <= 1024; every
+// thread of the block must call it).
+__device__ __forceinline__ void block_reduce_rate_rho_once(double rate, double r,
+                                                           unsigned long long* keys) {
+  __shared__ double s_rate[16], s_rho[16];
+  if (!(rate > 0.0)) rate = 0.0;
+  if (!(r == r)) r = __builtin_inf();
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(rate, off, 64);
+    const double p = __shfl_xor(r, off, 64);
+    rate = o > rate ? o : rate;
+    r = p < r ? p : r;
+  }
+  const int wave = threadIdx.x >> 6, waves = (blockDim.x + 63) >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    s_rate[wave] = rate;
+    s_rho[wave] = r;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = s_rate[0], b = s_rho[0];
+    for (int w = 1; w < waves; ++w) {
+      a = s_rate[w] > a ? s_rate[w] : a;
+      b = s_rho[w] < b ? s_rho[w] : b;
+    }
+    atomicMax(&keys[0], order_key(a));
+    atomicMin(&keys[1], order_key(b));
+  }
+}
+
 }  // namespace wb
