@@ -157,6 +157,65 @@ int wb_rate3d(const double* state, int64_t nx, int64_t ny, int64_t nz, double dx
               double dz, double b0x, double b0y, double b0z, double gamma, double* out,
               void* stream);
 
+/* ---- refinement estimator ---------------------------------------------------------- */
+
+/* The second-derivative estimator of Löhner (1987) in FLASH's normalised form,
+ * max over rho and p, on every interior cell of a haloed 3D state (halos filled):
+ * out (nx, ny, nz); scratch: 2 * X*Y*Z doubles (rho and p of the haloed block).
+ * The reference's own estimator is |J| h (mhd3d.py:771-801); this one labels the
+ * learned indicator and is its fallback.  eps: FLASH's filter (0.01). */
+int wb_lohner3d(const double* state, int64_t nx, int64_t ny, int64_t nz, double dx, double dy,
+                double dz, double b0x, double b0y, double b0z, double gamma, double eps,
+                double* scratch, double* out, void* stream);
+
+/* ---- learned refinement indicator (3D CNN, BF16 MFMA, FP32 accumulation) -------- */
+/* Three 3x3x3 periodic convolutions 8 -> 16 -> 16 -> 1 with ReLU, a logit per cell;
+ * activations bf16 channel-last (voxel-major, z fastest), nz a multiple of 16.
+ * `params` are FP32 masters, per layer W[27][16][16] (tap, ci, co; absent channels
+ * zero) then bias[16] (3 x 6928 floats); `kw` the BF16 kernel weights made from
+ * them by wb_cnn_prep (31104 uint16).  The reference has no learned indicator: its
+ * only estimator is current_tags (mhd3d.py:771-801); these stand beside
+ * wb_lohner3d, whose flags are the labels. */
+int wb_cnn_prep(const float* params, uint16_t* kw, void* stream);
+
+/* Per-channel mean and 1/std of the 8 primitives of a haloed 3D state (norm: 16
+ * floats); partial: workgroups x 16 doubles. */
+int wb_cnn_moments(const double* state, int64_t nx, int64_t ny, int64_t nz, double dx, double dy,
+                   double dz, double b0x, double b0y, double b0z, double gamma, double* partial,
+                   int workgroups, float* norm, void* stream);
+
+/* Network input: the normalised primitives, bf16 (nx*ny*nz x 8). */
+int wb_cnn_input(const double* state, int64_t nx, int64_t ny, int64_t nz, double dx, double dy,
+                 double dz, double b0x, double b0y, double b0z, double gamma, const float* norm,
+                 uint16_t* out, void* stream);
+
+/* Forward: a1, a2 (n x 16 bf16) and the logits (n floats). */
+int wb_cnn_forward(const uint16_t* x, const float* params, const uint16_t* kw, int64_t nx,
+                   int64_t ny, int64_t nz, uint16_t* a1, uint16_t* a2, float* logit,
+                   int workgroups, void* stream);
+
+/* Backward of the mean BCE-with-logits against `label` (n floats, 0/1): the
+ * gradient of every parameter into grad (params' layout) and the summed loss into
+ * *loss; g3, g2, g1: n x 16 bf16 scratch; partial: workgroups x 6928 floats. */
+int wb_cnn_backward(const uint16_t* x, const uint16_t* a1, const uint16_t* a2, const float* logit,
+                    const float* label, const uint16_t* kw, int64_t nx, int64_t ny, int64_t nz,
+                    uint16_t* g3, uint16_t* g2, uint16_t* g1, float* partial, int workgroups,
+                    float* grad, float* loss, void* stream);
+
+/* One Adam step (Kingma and Ba 2015) of n FP32 parameters; step counts from 1. */
+int wb_cnn_adam(float* p, const float* g, float* m, float* v, int n, float lr, float b1, float b2,
+                float eps, int step, void* stream);
+
+/* Single layers, for tests: conv (mode 0: bf16 relu(acc + bias) into out16; 1:
+ * acc + bias of channel 0 into logit; 2: bf16 acc * (mask > 0), the backward-data
+ * pass with kernel weights K'[t][ci][co] = W[26-t][ci][co]) and the weight
+ * gradient of one layer (27 x 16 x 16 + 16 floats into out). */
+int wb_cnn_conv(const uint16_t* in, int cin, const uint16_t* wk, const float* bias, int64_t nx,
+                int64_t ny, int64_t nz, int mode, const uint16_t* mask, uint16_t* out16,
+                float* logit, int workgroups, void* stream);
+int wb_cnn_wgrad(const uint16_t* x, int cin, const uint16_t* g16, int64_t nx, int64_t ny,
+                 int64_t nz, float* partial, int workgroups, float* out, void* stream);
+
 /* ---- 2D patch hierarchy (refined, periodic) ------------------------------------ */
 /* A hierarchy lives in one FP64 buffer, patch after patch (each a 2D patch state as
  * above).  `patches` is an array of {int64 offset; int32 nx, ny; double dx, dy} per

@@ -7,7 +7,7 @@ is missing or a GPU is not available, the product raises.
 from __future__ import annotations
 
 import ctypes
-from ctypes import c_double, c_int, c_int64, c_void_p
+from ctypes import c_double, c_float, c_int, c_int64, c_void_p
 from pathlib import Path
 
 LIBRARY = Path(__file__).resolve().parent / "libwbmhd.so"
@@ -17,6 +17,7 @@ P = c_void_p
 I64 = c_int64
 D = c_double
 I = c_int
+F = c_float
 
 #: Argument types of every exported entry point (``include/wbmhd.h``).
 SIGNATURES: dict[str, tuple[type, list]] = {
@@ -39,6 +40,15 @@ SIGNATURES: dict[str, tuple[type, list]] = {
     "wb_mhd3d_stage": (I, [P, P, I64, I64, I64] + [D] * 7 + [I, P, I, P, I, P]),
     "wb_mhd3d_stage_mixed": (I, [P, P, I64, I64, I64] + [D] * 7 + [I, P, I, P, I, I, P]),
     "wb_rate3d": (I, [P, I64, I64, I64] + [D] * 7 + [P, P]),
+    "wb_lohner3d": (I, [P, I64, I64, I64] + [D] * 8 + [P, P, P]),
+    "wb_cnn_prep": (I, [P, P, P]),
+    "wb_cnn_moments": (I, [P, I64, I64, I64] + [D] * 7 + [P, I, P, P]),
+    "wb_cnn_input": (I, [P, I64, I64, I64] + [D] * 7 + [P, P, P]),
+    "wb_cnn_forward": (I, [P, P, P, I64, I64, I64, P, P, P, I, P]),
+    "wb_cnn_backward": (I, [P] * 6 + [I64, I64, I64] + [P, P, P, P, I, P, P, P]),
+    "wb_cnn_adam": (I, [P, P, P, P, I, F, F, F, F, I, P]),
+    "wb_cnn_conv": (I, [P, I, P, P, I64, I64, I64, I, P, P, P, I, P]),
+    "wb_cnn_wgrad": (I, [P, I, P, I64, I64, I64, P, I, P, P]),
     "wb_amr2d_copy": (I, [P] * 5 + [I] + [P, P, I, P]),
     "wb_amr2d_interp": (I, [P] * 8 + [I] + [P] * 6 + [I, P]),
     "wb_amr2d_average": (I, [P, P, I, P]),
