@@ -226,3 +226,33 @@ def test_gpu_training_lowers_the_loss(gpu):
     assert losses[-1] < 0.5 * losses[0], (losses[0], losses[-1])
     pred = net.forward(xin).cpu().numpy() > 0
     assert (pred == (y > 0)).mean() > 0.9
+
+
+@pytest.mark.gpu
+def test_gpu_learned_indicator_gain(gpu):
+    """What the network gains over the plain estimator: trained on a blast in
+    turbulence (64^3) to predict the Löhner flags 8 steps ahead, on later snapshots it
+    never saw it recovers more of those future flags than the current plain flags do,
+    at the same number of flagged cells.  Measured: 0.80-0.83 against 0.68-0.75."""
+    from wellbalanced_amd import MHD3D, cnn, indicator, initial
+
+    model = MHD3D.from_problem(initial.blast_turbulence_3d(64, seed=0))
+    net = cnn.Indicator(model.n, device=gpu, lr=3e-3, seed=0)
+    model.advance(4)
+    net.fit_normalisation(model)
+    samples = indicator.lookahead_samples(model, net, snapshots=14, every=4, lead=8)
+    train, test = samples[:9], samples[9:]
+    losses = net.fit([(x, (later > 0.8).float().reshape(-1).contiguous())
+                      for _, x, _, later in train], 200)
+    assert losses[-1] < 0.05 * losses[0]
+    gains = []
+    for _, x, now, later in test:
+        target = later > 0.8
+        count = int((now > 0.8).sum())
+        learned = indicator.recall_at_count(net.forward(x), target, count)
+        plain = indicator.recall_at_count(now, target, count)
+        assert learned > plain, (learned, plain)
+        gains.append(learned - plain)
+    assert np.mean(gains) >= 0.05, gains
+    # the fallback: without a network the flags are the estimator's
+    assert bool((indicator.refine(model) == (indicator.lohner(model) > 0.8)).all())

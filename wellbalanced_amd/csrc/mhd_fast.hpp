@@ -1,0 +1,215 @@
+// FP32 reconstruction and HLLD flux of the mixed-precision path, written for the
+// CDNA issue model rather than for bit parity: no branches (each region of the
+// HLLD fan is chosen by selects, so a wavefront whose faces fall in different
+// regions runs one instruction stream), one reciprocal per distinct denominator
+// (14 transcendental instructions per face instead of 34 divisions and square
+// roots), and the slope limiter fixed at compile time.  The algebra is that of
+// the parity path (mhd_device.hpp, after mhd_sweep_pyccel.py:25-307) with the
+// divisions regrouped; the FP64 CT, update and E -> p conversion are unchanged.
+#pragma once
+
+#include <type_traits>
+
+#include "mhd_device.hpp"
+
+namespace wb {
+namespace fast {
+
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float sqrtf_(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float sel(bool c, float a, float b) { return c ? a : b; }
+
+// Limited slope (mhd_sweep_pyccel.py:25-48) for the compile-time limiter LIM.
+template <int LIM>
+__device__ __forceinline__ float slope(float a, float b) {
+  if constexpr (LIM == 3) {
+    return 0.5f * (a + b);
+  } else if constexpr (LIM == 0) {
+    const float m = sel(__builtin_fabsf(a) < __builtin_fabsf(b), a, b);
+    return sel(a * b > 0.0f, m, 0.0f);
+  } else if constexpr (LIM == 1) {
+    const float s = 2.0f * (a * b) * rcp(a + b);
+    return sel(a * b > 0.0f, s, 0.0f);
+  } else {
+    const float central = 0.5f * (a + b);
+    const float limit = 2.0f * sel(__builtin_fabsf(a) < __builtin_fabsf(b), a, b);
+    const float size = __builtin_fminf(__builtin_fabsf(limit), __builtin_fabsf(central));
+    const float s = sel(central > 0.0f, size, sel(central < 0.0f, -size, 0.0f));
+    return sel(a * b > 0.0f, s, 0.0f);
+  }
+}
+
+// Face states from four cells q[c][0..3] along the normal; first order where a
+// density or pressure would not be positive (mhd_sweep_pyccel.py:290-307).
+template <int LIM>
+__device__ __forceinline__ void reconstruct(const float (*q)[4], float* L, float* R) {
+#pragma unroll
+  for (int c = 0; c < 7; ++c) {
+    const float d0 = q[c][1] - q[c][0], d1 = q[c][2] - q[c][1], d2 = q[c][3] - q[c][2];
+    L[c] = q[c][1] + 0.5f * slope<LIM>(d0, d1);
+    R[c] = q[c][2] - 0.5f * slope<LIM>(d1, d2);
+  }
+  const bool first = L[0] <= 0.0f || L[4] <= 0.0f || R[0] <= 0.0f || R[4] <= 0.0f;
+#pragma unroll
+  for (int c = 0; c < 7; ++c) {
+    L[c] = sel(first, q[c][1], L[c]);
+    R[c] = sel(first, q[c][2], R[c]);
+  }
+}
+
+// One side of the face: conserved state u, flux f, total pressure and fast speed.
+struct Side {
+  float u[7], f[7], pt, cf;
+};
+
+__device__ __forceinline__ void side(const float* w, float bn, float gamma, float inv_gm1, Side& s) {
+  const float rho = w[0], vn = w[1], vt1 = w[2], vt2 = w[3], p = w[4], bt1 = w[5], bt2 = w[6];
+  const float bb = bn * bn + bt1 * bt1 + bt2 * bt2;
+  const float magnetic = 0.5f * bb;
+  const float energy = p * inv_gm1 + 0.5f * rho * (vn * vn + vt1 * vt1 + vt2 * vt2) + magnetic;
+  const float pt = p + magnetic;
+  const float vb = vn * bn + vt1 * bt1 + vt2 * bt2;
+  const float mn = rho * vn;
+  s.u[0] = rho;
+  s.u[1] = mn;
+  s.u[2] = rho * vt1;
+  s.u[3] = rho * vt2;
+  s.u[4] = energy;
+  s.u[5] = bt1;
+  s.u[6] = bt2;
+  s.f[0] = mn;
+  s.f[1] = mn * vn + pt - bn * bn;
+  s.f[2] = mn * vt1 - bn * bt1;
+  s.f[3] = mn * vt2 - bn * bt2;
+  s.f[4] = (energy + pt) * vn - bn * vb;
+  s.f[5] = bt1 * vn - bn * vt1;
+  s.f[6] = bt2 * vn - bn * vt2;
+  s.pt = pt;
+  // fast magnetosonic speed (mhd_sweep_pyccel.py:51-61)
+  const float irho = rcp(rho);
+  const float a2 = gamma * p * irho, b2 = bb * irho;
+  const float disc = (a2 + b2) * (a2 + b2) - 4.0f * a2 * bn * bn * irho;
+  const float root = sqrtf_(__builtin_fmaxf(disc, 0.0f));
+  s.cf = sqrtf_(0.5f * (a2 + b2 + root));
+}
+
+// Star state beside the contact (mhd_sweep_pyccel.py:101-145): us, and
+// vt = (vt1, vt2, bt1, bt2).
+__device__ __forceinline__ void star(const float* w, const Side& sd, float bn, float s, float d,
+                                     float sm, float pts, float* us, float* vt) {
+  const float vn = w[1];
+  const float inv = rcp(s - sm);
+  const float rho_s = d * inv;
+  const float den = d * (s - sm) - bn * bn;
+  const bool regular = __builtin_fabsf(den) >= float(DEGENERATE) * (sd.pt + bn * bn);
+  const float iden = rcp(den);
+  const float cv = bn * (sm - vn) * iden;
+  const float cb = (d * (s - vn) - bn * bn) * iden;
+  const float vt1 = sel(regular, w[2] - w[5] * cv, w[2]);
+  const float vt2 = sel(regular, w[3] - w[6] * cv, w[3]);
+  const float bt1 = sel(regular, w[5] * cb, w[5]);
+  const float bt2 = sel(regular, w[6] * cb, w[6]);
+  const float vb = vn * bn + w[2] * w[5] + w[3] * w[6];
+  const float vb_s = sm * bn + vt1 * bt1 + vt2 * bt2;
+  us[0] = rho_s;
+  us[1] = rho_s * sm;
+  us[2] = rho_s * vt1;
+  us[3] = rho_s * vt2;
+  us[4] = ((s - vn) * sd.u[4] - sd.pt * vn + pts * sm + bn * (vb - vb_s)) * inv;
+  us[5] = bt1;
+  us[6] = bt2;
+  vt[0] = vt1;
+  vt[1] = vt2;
+  vt[2] = bt1;
+  vt[3] = bt2;
+}
+
+// The HLLD flux (Miyoshi and Kusano 2005) of normal-frame states L, R.
+__device__ __forceinline__ void hlld(const float* L, const float* R, float bn, float gamma,
+                                     float inv_gm1, float* flux) {
+  Side sl_, sr_;
+  side(L, bn, gamma, inv_gm1, sl_);
+  side(R, bn, gamma, inv_gm1, sr_);
+  const float vl = L[1], vr = R[1];
+  const float cf = __builtin_fmaxf(sl_.cf, sr_.cf);
+  const float sl = __builtin_fminf(vl, vr) - cf;
+  const float sr = __builtin_fmaxf(vl, vr) + cf;
+  const float dl = (sl - vl) * L[0];
+  const float dr = (sr - vr) * R[0];
+  const float idd = rcp(dr - dl);
+  const float sm = (dr * vr - dl * vl - sr_.pt + sl_.pt) * idd;
+  const float pts = (dr * sl_.pt - dl * sr_.pt + dl * dr * (vr - vl)) * idd;
+  float usl[7], usr[7], wl[4], wr[4];
+  star(L, sl_, bn, sl, dl, sm, pts, usl, wl);
+  star(R, sr_, bn, sr, dr, sm, pts, usr, wr);
+  const float isql = rsq(usl[0]), isqr = rsq(usr[0]);
+  const float sql = usl[0] * isql, sqr = usr[0] * isqr;
+  const float sign = sel(bn > 0.0f, 1.0f, sel(bn < 0.0f, -1.0f, 0.0f));
+  const float isum = rcp(sql + sqr);
+  const float vt2_1 = (sql * wl[0] + sqr * wr[0] + (wr[2] - wl[2]) * sign) * isum;
+  const float vt2_2 = (sql * wl[1] + sqr * wr[1] + (wr[3] - wl[3]) * sign) * isum;
+  const float bt2_1 = (sql * wr[2] + sqr * wl[2] + sql * sqr * (wr[0] - wl[0]) * sign) * isum;
+  const float bt2_2 = (sql * wr[3] + sqr * wl[3] + sql * sqr * (wr[1] - wl[1]) * sign) * isum;
+  const float vb2 = sm * bn + vt2_1 * bt2_1 + vt2_2 * bt2_2;
+  const float abn = __builtin_fabsf(bn);
+  const float ssl = sm - abn * isql, ssr = sm + abn * isqr;
+  // the side of the contact: the star and double-star states of that side only
+  const bool left = sm >= 0.0f;
+  const float s = sel(left, sl, sr);
+  const float ss = sel(left, __builtin_fminf(ssl, 0.0f), __builtin_fmaxf(ssr, 0.0f));
+  const float sq = sel(left, -sql, sqr);
+  const float w0 = sel(left, wl[0], wr[0]), w1 = sel(left, wl[1], wr[1]);
+  const float w2 = sel(left, wl[2], wr[2]), w3 = sel(left, wl[3], wr[3]);
+  const float vb_s = sm * bn + w0 * w2 + w1 * w3;
+  float us[7], uu[7], ff[7];
+#pragma unroll
+  for (int c = 0; c < 7; ++c) {
+    us[c] = sel(left, usl[c], usr[c]);
+    uu[c] = sel(left, sl_.u[c], sr_.u[c]);
+    ff[c] = sel(left, sl_.f[c], sr_.f[c]);
+  }
+  const float rho_s = us[0];
+  const float udd[7] = {rho_s, rho_s * sm, rho_s * vt2_1, rho_s * vt2_2,
+                        us[4] + sq * (vb_s - vb2) * sign, bt2_1, bt2_2};
+  const bool supl = sl >= 0.0f, supr = sr < 0.0f;
+#pragma unroll
+  for (int c = 0; c < 7; ++c) {
+    const float mid = ff[c] + s * (us[c] - uu[c]) + ss * (udd[c] - us[c]);
+    flux[c] = sel(supl, sl_.f[c], sel(supr, sr_.f[c], mid));
+  }
+}
+
+}  // namespace fast
+}  // namespace wb
+
+namespace wb {
+
+// The face flux of either path: the parity reconstruction and HLLD (runtime
+// limiter) for LIM < 0 or FP64, the branch-free FP32 forms for LIM >= 0.
+template <typename P, int LIM>
+__device__ __forceinline__ void face_solve(const P (*q)[4], int limiter, P bn, P gamma, P inv_gm1,
+                                           P* flux) {
+  P L[7], R[7];
+  if constexpr (sizeof(P) == 4 && LIM >= 0) {
+    fast::reconstruct<LIM>(q, L, R);
+    fast::hlld(L, R, bn, gamma, inv_gm1, flux);
+  } else {
+    reconstruct<P>(q, limiter, L, R);
+    hlld<P>(L, R, bn, gamma, flux);
+  }
+}
+
+// Calls f(std::integral_constant<int, LIM>) for the runtime limiter code 0..3.
+template <typename F>
+inline int with_limiter(int limiter, F&& f) {
+  switch (limiter) {
+    case 0: return f(std::integral_constant<int, 0>());
+    case 1: return f(std::integral_constant<int, 1>());
+    case 2: return f(std::integral_constant<int, 2>());
+    case 3: return f(std::integral_constant<int, 3>());
+    default: return -100000;
+  }
+}
+
+}  // namespace wb

@@ -12,6 +12,7 @@
 #pragma once
 
 #include "mhd_device.hpp"
+#include "mhd_fast.hpp"
 #include "wbmhd_common.hpp"
 
 namespace wb {
@@ -43,7 +44,9 @@ struct Stage2Args {
   int walls;  // bit 0: x walls, bit 1: y walls
 };
 
-template <typename P, int TY>
+// LIM: the compile-time slope limiter of the branch-free FP32 flux (mixed path),
+// or -1 for the parity path's runtime limiter.
+template <typename P, int TY, int LIM>
 __device__ __forceinline__ void stage2d_body(const Stage2Args& a, int tile, int seg,
                                              Stage2Shared<P, TY>& sh) {
   using T = Row<TY>;
@@ -138,10 +141,9 @@ __device__ __forceinline__ void stage2d_body(const Stage2Args& a, int tile, int 
     if (fthread) {
       const int jj = min(max(gy, 0), Y - 1);
       if (f >= ix0) {  // x-face f of the column
-        P L[7], R[7], fl[7];
-        reconstruct<P>(q, a.limiter, L, R);
+        P fl[7];
         const P bn = P((AZ(f, jj + 1) - AZ(f, jj)) / a.dy + a.b0x);
-        hlld<P>(L, R, bn, P(a.gamma), fl);
+        face_solve<P, LIM>(q, a.limiter, bn, P(a.gamma), P(1.0 / (a.gamma - 1.0)), fl);
         // walls: nothing but the pressure crosses (mhd.py:873-886), after the CT
         const bool wall = xwall && (f == 2 || f == a.nx + 2);
 #pragma unroll
@@ -160,11 +162,10 @@ __device__ __forceinline__ void stage2d_body(const Stage2Args& a, int tile, int 
         for (int c = 0; c < 7; ++c)
 #pragma unroll
           for (int s = 0; s < 4; ++s) qq[c][s] = sh.prim[cur][comp[c]][tid + 1 + s - 2];
-        P L[7], R[7], fl[7];
-        reconstruct<P>(qq, a.limiter, L, R);
+        P fl[7];
         const int ii = min(max(f, 0), X - 1);
         const P bn = P(-(AZ(ii + 1, jj) - AZ(ii, jj)) / a.dx + a.b0y);
-        hlld<P>(L, R, bn, P(a.gamma), fl);
+        face_solve<P, LIM>(qq, a.limiter, bn, P(a.gamma), P(1.0 / (a.gamma - 1.0)), fl);
 #pragma unroll
         for (int r = 0; r < 7; ++r) sh.fy[cur][r][tid] = fl[r];
       }
@@ -216,17 +217,17 @@ __device__ __forceinline__ void stage2d_body(const Stage2Args& a, int tile, int 
   }
 }
 
-template <typename P, int TY>
+template <typename P, int TY, int LIM>
 __global__ void __launch_bounds__((Row<TY>::THREADS)) k_stage2d(Stage2Args a) {
   __shared__ Stage2Shared<P, TY> sh;
-  stage2d_body<P, TY>(a, blockIdx.x, blockIdx.y, sh);
+  stage2d_body<P, TY, LIM>(a, blockIdx.x, blockIdx.y, sh);
 }
 
 
 // All tiles of all patches in one persistent launch: each workgroup takes the next
 // work item from a counter until the list is exhausted (the counter is zeroed by
 // k_zero_i32 before the launch).
-template <typename P, int TY>
+template <typename P, int TY, int LIM>
 __global__ void __launch_bounds__((Row<TY>::THREADS))
 k_stage2d_batched(Stage2Args base, const PatchDesc* __restrict__ patches,
                   const WorkItem* __restrict__ items, int nitems, int* __restrict__ counter,
@@ -250,7 +251,7 @@ k_stage2d_batched(Stage2Args base, const PatchDesc* __restrict__ patches,
     a.Y = pd.ny + 4;
     a.dx = pd.dx;
     a.dy = pd.dy;
-    stage2d_body<P, TY>(a, w.tile, w.seg, sh);
+    stage2d_body<P, TY, LIM>(a, w.tile, w.seg, sh);
     __syncthreads();
   }
 }
@@ -258,7 +259,7 @@ k_stage2d_batched(Stage2Args base, const PatchDesc* __restrict__ patches,
 // The flux of one face of a patch from its (ghost-filled) stage input, computed as
 // the stage kernels compute it: primitive variables in FP64, reconstruction and
 // HLLD in P.  axis 0: x-face `fa` (low face of cell fa) at row fb; axis 1: y-face.
-template <typename P>
+template <typename P, int LIM>
 __device__ void face_flux2d(const double* in, int X, int Y, double dx, double dy, double b0x,
                             double b0y, double gamma, int limiter, int axis, int fa, int fb,
                             double* out7) {
@@ -289,9 +290,8 @@ __device__ void face_flux2d(const double* in, int X, int Y, double dx, double dy
   }
   const double bn = axis == 0 ? (AZ(fa, fb + 1) - AZ(fa, fb)) / dy + b0x
                               : -(AZ(fa + 1, fb) - AZ(fa, fb)) / dx + b0y;
-  P L[7], R[7], f[7];
-  reconstruct<P>(q, limiter, L, R);
-  hlld<P>(L, R, P(bn), P(gamma), f);
+  P f[7];
+  face_solve<P, LIM>(q, limiter, P(bn), P(gamma), P(1.0 / (gamma - 1.0)), f);
   for (int c = 0; c < 7; ++c) out7[c] = double(f[c]);
 }
 
@@ -300,7 +300,7 @@ __device__ void face_flux2d(const double* in, int X, int Y, double dx, double dy
 // replaced by the mean of the two fine face fluxes.  rows (13 int64 each):
 // q, ca, cb, axis, side, face_a, face_b, p0, fa0, fb0, p1, fa1, fb1; groups[g] =
 // (first row, row count) of coarse cell g.  Deterministic: one thread per cell.
-template <typename P>
+template <typename P, int LIM>
 __global__ void __launch_bounds__(64) k_reflux2d(const double* __restrict__ src, double* __restrict__ dst,
                            const PatchDesc* __restrict__ patches, const long long* __restrict__ rows,
                            const int* __restrict__ groups, int ngroups, double b0x, double b0y,
@@ -322,12 +322,12 @@ __global__ void __launch_bounds__(64) k_reflux2d(const double* __restrict__ src,
     const PatchDesc cq = patches[q];
     const double dtd = axis == 0 ? dt / cq.dx : dt / cq.dy;
     double fc[7], f0[7], f1[7];
-    face_flux2d<P>(src + cq.off, cq.nx + 4, cq.ny + 4, cq.dx, cq.dy, b0x, b0y, gamma, limiter,
+    face_flux2d<P, LIM>(src + cq.off, cq.nx + 4, cq.ny + 4, cq.dx, cq.dy, b0x, b0y, gamma, limiter,
                    axis, (int)r[5], (int)r[6], fc);
     const PatchDesc p0 = patches[r[7]], p1 = patches[r[10]];
-    face_flux2d<P>(src + p0.off, p0.nx + 4, p0.ny + 4, p0.dx, p0.dy, b0x, b0y, gamma, limiter,
+    face_flux2d<P, LIM>(src + p0.off, p0.nx + 4, p0.ny + 4, p0.dx, p0.dy, b0x, b0y, gamma, limiter,
                    axis, (int)r[8], (int)r[9], f0);
-    face_flux2d<P>(src + p1.off, p1.nx + 4, p1.ny + 4, p1.dx, p1.dy, b0x, b0y, gamma, limiter,
+    face_flux2d<P, LIM>(src + p1.off, p1.nx + 4, p1.ny + 4, p1.dx, p1.dy, b0x, b0y, gamma, limiter,
                    axis, (int)r[11], (int)r[12], f1);
     for (int f = 0; f < 6; ++f) {
       const int row = axis == 0 ? XR[f] : YR[f];
@@ -345,11 +345,11 @@ __global__ void __launch_bounds__(64) k_reflux2d(const double* __restrict__ src,
   }
 }
 
-template <typename P, int TY>
+template <typename P, int TY, int LIM = -1>
 inline int launch_stage2d(const Stage2Args& a, hipStream_t s) {
   const int tiles = (a.ny + TY - 1) / TY;
   const int segs = (a.nx + a.nxs - 1) / a.nxs;
-  hipLaunchKernelGGL((k_stage2d<P, TY>), dim3(tiles, segs), dim3(Row<TY>::THREADS), 0, s, a);
+  hipLaunchKernelGGL((k_stage2d<P, TY, LIM>), dim3(tiles, segs), dim3(Row<TY>::THREADS), 0, s, a);
   return wb_last_error();
 }
 

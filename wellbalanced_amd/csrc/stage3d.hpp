@@ -21,6 +21,7 @@
 #pragma once
 
 #include "mhd_device.hpp"
+#include "mhd_fast.hpp"
 #include "wbmhd_common.hpp"
 
 namespace wb {
@@ -129,7 +130,9 @@ struct CellPrim {
   }
 };
 
-template <typename P, int TY, int TZ, int WAVES>
+// LIM: the slope limiter fixed at compile time (the mixed path's branch-free
+// FP32 flux), or -1 for the parity path's runtime limiter.
+template <typename P, int TY, int TZ, int WAVES, int LIM>
 __global__ void __launch_bounds__((Tile<TY, TZ>::THREADS), WAVES)
 k_stage3d(StageArgs a) {
   using T = Tile<TY, TZ>;
@@ -156,6 +159,7 @@ k_stage3d(StageArgs a) {
   const double dt = *a.dt;
   const double dtdx = dt / a.dx, dtdy = dt / a.dy, dtdz = dt / a.dz;
   const double rdx = 1.0 / a.dx, rdy = 1.0 / a.dy, rdz = 1.0 / a.dz;
+  const P gam = P(a.gamma), inv_gm1 = P(1.0 / (a.gamma - 1.0));
 
   // register state of the column
   P q[7][4];                       // x-frame primitives of planes f-2 .. f+1
@@ -241,9 +245,8 @@ k_stage3d(StageArgs a) {
     own_column(f + 1);
     if (fthread) {
       if (f >= ix0) {
-        P L[7], R[7], fl[7];
-        reconstruct<P>(q, a.limiter, L, R);
-        hlld<P>(L, R, P(face_bx(f)), P(a.gamma), fl);
+        P fl[7];
+        face_solve<P, LIM>(q, a.limiter, P(face_bx(f)), gam, inv_gm1, fl);
 #pragma unroll
         for (int r = 0; r < 5; ++r) {
           fprev[r] = fcur[r];
@@ -263,9 +266,8 @@ k_stage3d(StageArgs a) {
         for (int c = 0; c < 7; ++c)
 #pragma unroll
           for (int s = 0; s < 4; ++s) qq[c][s] = sh.prim[comp[c]][pidx + (s - 2) * T::PZ];
-        P L[7], R[7], fl[7];
-        reconstruct<P>(qq, a.limiter, L, R);
-        hlld<P>(L, R, P(face_by(fi)), P(a.gamma), fl);
+        P fl[7];
+        face_solve<P, LIM>(qq, a.limiter, P(face_by(fi)), gam, inv_gm1, fl);
 #pragma unroll
         for (int r = 0; r < 7; ++r) sh.fy[r][tid] = fl[r];
         sh.fyc[cur][0][tid] = fl[0];
@@ -278,9 +280,8 @@ k_stage3d(StageArgs a) {
         for (int c = 0; c < 7; ++c)
 #pragma unroll
           for (int s = 0; s < 4; ++s) qq[c][s] = sh.prim[comp[c]][pidx + (s - 2)];
-        P L[7], R[7], fl[7];
-        reconstruct<P>(qq, a.limiter, L, R);
-        hlld<P>(L, R, P(face_bz(fi)), P(a.gamma), fl);
+        P fl[7];
+        face_solve<P, LIM>(qq, a.limiter, P(face_bz(fi)), gam, inv_gm1, fl);
 #pragma unroll
         for (int r = 0; r < 7; ++r) sh.fz[r][tid] = fl[r];
         sh.fzc[cur][0][tid] = fl[0];
@@ -368,12 +369,12 @@ k_stage3d(StageArgs a) {
 
 // WAVES: the waves per SIMD the register allocation must allow (2 or 3 for the
 // mixed path: 3 trades a few spills for occupancy; the autotuner picks).
-template <typename P, int TY, int TZ, int WAVES>
+template <typename P, int TY, int TZ, int WAVES, int LIM = -1>
 inline int launch_stage3d(const StageArgs& a, hipStream_t s) {
   using T = Tile<TY, TZ>;
   const int tiles = ((a.g.ny + TY - 1) / TY) * ((a.g.nz + TZ - 1) / TZ);
   const int segs = (a.g.nx + a.nxs - 1) / a.nxs;
-  hipLaunchKernelGGL((k_stage3d<P, TY, TZ, WAVES>), dim3(tiles, segs), dim3(T::THREADS), 0, s, a);
+  hipLaunchKernelGGL((k_stage3d<P, TY, TZ, WAVES, LIM>), dim3(tiles, segs), dim3(T::THREADS), 0, s, a);
   return wb_last_error();
 }
 

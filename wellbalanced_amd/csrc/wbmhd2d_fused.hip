@@ -64,10 +64,10 @@ extern "C" int wb_amr2d_stage(const double* in, double* out, const void* patches
   const WorkItem* wi = (const WorkItem*)items;
   const unsigned grid = (unsigned)(workgroups > 0 ? workgroups : 1);
   if (tile == 62)
-    hipLaunchKernelGGL((k_stage2d_batched<double, 62>), dim3(grid), dim3(Row<62>::THREADS), 0, s, a,
+    hipLaunchKernelGGL((k_stage2d_batched<double, 62, -1>), dim3(grid), dim3(Row<62>::THREADS), 0, s, a,
                        pd, wi, nitems, (int*)counter, out);
   else
-    hipLaunchKernelGGL((k_stage2d_batched<double, 126>), dim3(grid), dim3(Row<126>::THREADS), 0, s,
+    hipLaunchKernelGGL((k_stage2d_batched<double, 126, -1>), dim3(grid), dim3(Row<126>::THREADS), 0, s,
                        a, pd, wi, nitems, (int*)counter, out);
   return wb_last_error();
 }
@@ -77,7 +77,7 @@ extern "C" int wb_amr2d_reflux(const double* src, double* dst, const void* patch
                                      double b0x, double b0y, double gamma, int limiter,
                                      const double* dt, int heun, void* stream) {
   if (ngroups <= 0) return 0;
-  hipLaunchKernelGGL((k_reflux2d<double>), dim3((unsigned)((ngroups + 63) / 64)), dim3(64), 0,
+  hipLaunchKernelGGL((k_reflux2d<double, -1>), dim3((unsigned)((ngroups + 63) / 64)), dim3(64), 0,
                      (hipStream_t)stream, src, dst, (const PatchDesc*)patches, rows,
                      (const int*)groups, ngroups, b0x, b0y, gamma, limiter, dt, heun);
   return wb_last_error();

@@ -18,9 +18,12 @@ extern "C" int wb_mhd2d_stage_mixed(const double* in, double* out, int64_t nx, i
                                       heun, wall_mask, bad, nxs);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
-  if (tile == 62) return launch_stage2d<float, 62>(a, s);
-  if (tile == 254) return launch_stage2d<float, 254>(a, s);
-  return launch_stage2d<float, 126>(a, s);
+  return with_limiter(limiter, [&](auto lim) {
+    constexpr int L = decltype(lim)::value;
+    if (tile == 62) return launch_stage2d<float, 62, L>(a, s);
+    if (tile == 254) return launch_stage2d<float, 254, L>(a, s);
+    return launch_stage2d<float, 126, L>(a, s);
+  });
 }
 
 // All patches of a periodic 2D hierarchy in one persistent launch per stage, and the
@@ -40,13 +43,16 @@ extern "C" int wb_amr2d_stage_mixed(const double* in, double* out, const void* p
   const PatchDesc* pd = (const PatchDesc*)patches;
   const WorkItem* wi = (const WorkItem*)items;
   const unsigned grid = (unsigned)(workgroups > 0 ? workgroups : 1);
-  if (tile == 62)
-    hipLaunchKernelGGL((k_stage2d_batched<float, 62>), dim3(grid), dim3(Row<62>::THREADS), 0, s, a,
-                       pd, wi, nitems, (int*)counter, out);
-  else
-    hipLaunchKernelGGL((k_stage2d_batched<float, 126>), dim3(grid), dim3(Row<126>::THREADS), 0, s,
-                       a, pd, wi, nitems, (int*)counter, out);
-  return wb_last_error();
+  return with_limiter(limiter, [&](auto lim) {
+    constexpr int L = decltype(lim)::value;
+    if (tile == 62)
+      hipLaunchKernelGGL((k_stage2d_batched<float, 62, L>), dim3(grid), dim3(Row<62>::THREADS), 0,
+                         s, a, pd, wi, nitems, (int*)counter, out);
+    else
+      hipLaunchKernelGGL((k_stage2d_batched<float, 126, L>), dim3(grid), dim3(Row<126>::THREADS),
+                         0, s, a, pd, wi, nitems, (int*)counter, out);
+    return wb_last_error();
+  });
 }
 
 extern "C" int wb_amr2d_reflux_mixed(const double* src, double* dst, const void* patches,
@@ -54,8 +60,11 @@ extern "C" int wb_amr2d_reflux_mixed(const double* src, double* dst, const void*
                                      double b0x, double b0y, double gamma, int limiter,
                                      const double* dt, int heun, void* stream) {
   if (ngroups <= 0) return 0;
-  hipLaunchKernelGGL((k_reflux2d<float>), dim3((unsigned)((ngroups + 63) / 64)), dim3(64), 0,
-                     (hipStream_t)stream, src, dst, (const PatchDesc*)patches, rows,
-                     (const int*)groups, ngroups, b0x, b0y, gamma, limiter, dt, heun);
-  return wb_last_error();
+  return with_limiter(limiter, [&](auto lim) {
+    constexpr int L = decltype(lim)::value;
+    hipLaunchKernelGGL((k_reflux2d<float, L>), dim3((unsigned)((ngroups + 63) / 64)), dim3(64), 0,
+                       (hipStream_t)stream, src, dst, (const PatchDesc*)patches, rows,
+                       (const int*)groups, ngroups, b0x, b0y, gamma, limiter, dt, heun);
+    return wb_last_error();
+  });
 }

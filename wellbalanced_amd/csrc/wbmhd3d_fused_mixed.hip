@@ -19,6 +19,10 @@ extern "C" int wb_mhd3d_stage_mixed(const double* in, double* out, int64_t nx, i
   const int rc = wb_fill_stage_args3d(&a, in, out, nx, ny, nz, dx, dy, dz, b0x, b0y, b0z, gamma,
                                       limiter, dt, heun, bad, nxs);
   if (rc) return rc;
-  if (waves == 3) return launch_stage3d<float, 14, 14, 3>(a, (hipStream_t)stream);
-  return launch_stage3d<float, 14, 14, 2>(a, (hipStream_t)stream);
+  hipStream_t s = (hipStream_t)stream;
+  return with_limiter(limiter, [&](auto lim) {
+    constexpr int L = decltype(lim)::value;
+    if (waves == 3) return launch_stage3d<float, 14, 14, 3, L>(a, s);
+    return launch_stage3d<float, 14, 14, 2, L>(a, s);
+  });
 }

@@ -1,10 +1,14 @@
 """Refinement indicators of the 3D step: the Löhner estimator and the learned CNN.
 
-The reference tags cells with |J| times the cell size (``mhd3d.py:771-801``,
-`MHD3D.current_tags` here).  This module adds the second-derivative estimator of
-Löhner (1987) in FLASH's normalised form (``wb_lohner3d``): max over density and
-pressure, flag where it exceeds 0.8.  It is the label source of the learned
-indicator (`wellbalanced_amd.cnn`) and its fallback.
+The reference tags cells with |J| times the cell size (``mhd3d.py:771-801``).  This
+module adds the second-derivative estimator of Löhner (1987) in FLASH's normalised
+form (``wb_lohner3d``): max over density and pressure, flag where it exceeds 0.8.
+It is the label source of the learned indicator (`wellbalanced_amd.cnn`) and its
+fallback.
+
+The learned indicator is trained to look ahead: from the state at step s it
+predicts the Löhner flags of step s + lead, so that a hierarchy regridded every
+``lead`` steps still covers the features that move in between.
 """
 
 from __future__ import annotations
@@ -41,3 +45,33 @@ def block_flags(cell_flags: torch.Tensor, block: int = 16) -> torch.Tensor:
     f = cell_flags[: nx * block, : ny * block, : nz * block]
     f = f.reshape(nx, block, ny, block, nz, block)
     return f.any(dim=5).any(dim=3).any(dim=1)
+
+
+def refine(model, net=None, cutoff: float = CUTOFF) -> torch.Tensor:
+    """Cell flags of ``model``: the learned indicator's (logit > 0) when ``net`` is
+    given, else the Löhner estimator's."""
+    if net is None:
+        return flags(lohner(model), cutoff)
+    return net.predict(model) > 0.0
+
+
+def lookahead_samples(model, net, snapshots: int, every: int, lead: int, cutoff: float = CUTOFF):
+    """Advance ``model`` and collect ``snapshots`` triples (step, features at the step,
+    Löhner estimate at the step, Löhner estimate ``lead`` steps later), one every
+    ``every`` steps, the first after ``every`` steps."""
+    wanted = {every * (i + 1) for i in range(snapshots)}
+    need = wanted | {s + lead for s in wanted}
+    feats, ests = {}, {}
+    for s in range(1, max(need) + 1):
+        model.advance(1)
+        if s in wanted:
+            feats[s] = net.features(model)
+        if s in need:
+            ests[s] = lohner(model)
+    return [(s, feats[s], ests[s], ests[s + lead]) for s in sorted(wanted)]
+
+
+def recall_at_count(score: torch.Tensor, target: torch.Tensor, count: int) -> float:
+    """The fraction of ``target`` cells among the ``count`` highest-scoring cells."""
+    idx = torch.topk(score.reshape(-1), count).indices
+    return float(target.reshape(-1)[idx].float().sum() / target.float().sum())
