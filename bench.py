@@ -28,6 +28,23 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 STATE_BYTES_3D = 64  # 5 FP64 cell fields + 3 FP64 edge-potential components per cell
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "stage3d_turbulence256_traffic.md")
+
+
+def profiled_traffic(precision: str, nxs: int, waves: int, cells: int):
+    """HBM bytes per stage launch measured by the FETCH_SIZE / WRITE_SIZE runs of the
+    committed profile (copy-calibrated), if it covers this workload and launch."""
+    if cells != 256**3 or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    want = f"{precision} nxs={nxs}" + (f" waves={waves}" if precision == "mixed" else "")
+    with open(TRAFFIC_PROFILE) as fh:
+        for line in fh:
+            if line.startswith("traffic_bytes_per_launch:"):
+                body = line.split(":", 1)[1].strip()
+                spec, value = body.rsplit(" ", 1)
+                if spec == want:
+                    return float(value)
+    return None
 
 
 def parse():
@@ -151,6 +168,9 @@ def main():
     # reads U^n, writes U^1: 2S; stage 2 reads U^1 and U^n, writes U^n+1: 3S)
     bytes_per_launch = 2.5 * STATE_BYTES_3D * cells
     achieved = bytes_per_launch / (stage_ms * 1e-3) / 1e9
+    launch = tuned.get(precision, {})
+    traffic = profiled_traffic(precision, launch.get("nxs", result_model.nxs),
+                               launch.get("waves", getattr(result_model, "waves", 0)), cells)
     out = {
         "metric": "zone-updates/s on 1 MI355X, mixed vs FP64 at stated L1 error; % HBM peak",
         "value": value,
@@ -183,7 +203,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": (os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic is not None
+                               else None),
             "kernel": "stage (both Heun stages)",
             "stage_ms": stage_ms,
         },

@@ -132,6 +132,11 @@ struct CellPrim {
 
 // LIM: the slope limiter fixed at compile time (the mixed path's branch-free
 // FP32 flux), or -1 for the parity path's runtime limiter.
+//
+// Workgroup order: the hardware hands workgroup w to XCD w % 8, each XCD with its
+// own L2.  The workgroups are renumbered so that XCD x runs a contiguous run of
+// tiles (z fastest, then y, then the x segment): tiles running together on one
+// XCD are neighbours, and the halo lines they share are fetched into that L2 once.
 template <typename P, int TY, int TZ, int WAVES, int LIM>
 __global__ void __launch_bounds__((Tile<TY, TZ>::THREADS), WAVES)
 k_stage3d(StageArgs a) {
@@ -140,10 +145,19 @@ k_stage3d(StageArgs a) {
   __shared__ StageShared<P, TY, TZ> sh;
   const Grid3 g = a.g;
   const int tid = threadIdx.x;
-  const int tiles_z = (g.nz + TZ - 1) / TZ;
-  const int by = blockIdx.x / tiles_z, bz = blockIdx.x % tiles_z;
-  const int jy0 = 2 + by * TY, kz0 = 2 + bz * TZ;       // first interior cell of the tile
-  const int ix0 = 2 + blockIdx.y * a.nxs;               // first plane of the segment
+  const int tiles_z = (g.nz + TZ - 1) / TZ, tiles_y = (g.ny + TY - 1) / TY;
+  int tz_, ty_, seg_;
+  {
+    const int n = gridDim.x, xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
+    const int per = n >> 3, rem = n & 7;
+    const int logical = xcd * per + min(xcd, rem) + k;
+    tz_ = logical % tiles_z;
+    const int rest = logical / tiles_z;
+    ty_ = rest % tiles_y;
+    seg_ = rest / tiles_y;
+  }
+  const int jy0 = 2 + ty_ * TY, kz0 = 2 + tz_ * TZ;       // first interior cell of the tile
+  const int ix0 = 2 + seg_ * a.nxs;                       // first plane of the segment
   const int tyv = min(TY, g.ny + 2 - jy0), tzv = min(TZ, g.nz + 2 - kz0);
   const int nxv = min(a.nxs, g.nx + 2 - ix0);
   const bool last_y = jy0 + TY >= g.ny + 2, last_z = kz0 + TZ >= g.nz + 2;
@@ -374,7 +388,7 @@ inline int launch_stage3d(const StageArgs& a, hipStream_t s) {
   using T = Tile<TY, TZ>;
   const int tiles = ((a.g.ny + TY - 1) / TY) * ((a.g.nz + TZ - 1) / TZ);
   const int segs = (a.g.nx + a.nxs - 1) / a.nxs;
-  hipLaunchKernelGGL((k_stage3d<P, TY, TZ, WAVES, LIM>), dim3(tiles, segs), dim3(T::THREADS), 0, s, a);
+  hipLaunchKernelGGL((k_stage3d<P, TY, TZ, WAVES, LIM>), dim3(tiles * segs), dim3(T::THREADS), 0, s, a);
   return wb_last_error();
 }
 
