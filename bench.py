@@ -1,6 +1,6 @@
 """Benchmark of the ideal-MHD step on one MI355X.
 
-Workload: 3D Alfvénic turbulence on a periodic 256^3 grid (rho = 1, p = 1,
+Default workload: 3D Alfvénic turbulence on a periodic 256^3 grid (rho = 1, p = 1,
 gamma = 5/3, B0 = (1, 0, 0), seeded velocity and field perturbations of rms 0.1
 over modes 1 <= |n| <= 4), minmod, CFL 0.3.  A step is one full SSP-RK2 (Heun)
 step including its CFL reduction; inputs are resident in HBM.  The script times
@@ -10,6 +10,7 @@ stage kernel measured with HIP events on the stream it runs on, and a CPU
 baseline (the NumPy oracle on one core, bounded sample of the same workload).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells 256]
+    python bench.py --workload ot2d|blast2d-amr|harris2d   (the other workloads)
 
 Prints one JSON line.
 """
@@ -62,6 +63,8 @@ def parse():
     p.add_argument("--waves", type=int, default=None, help="waves/SIMD of the mixed stage")
     p.add_argument("--calibrate", action="store_true",
                    help="also run one device copy of 1 GiB (the traffic counters' reference)")
+    p.add_argument("--workload", default="turbulence3d",
+                   choices=("turbulence3d", "ot2d", "blast2d-amr", "harris2d"))
     return p.parse_args()
 
 
@@ -109,6 +112,25 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(the multi-GPU sharding is a design in DESIGN.md, not built)"}))
         sys.exit(2)
+    runs = {"turbulence3d": run_turbulence3d, "ot2d": run_ot2d, "blast2d-amr": run_blast2d_amr,
+            "harris2d": run_harris2d}
+    print(json.dumps(runs[args.workload](args)), flush=True)
+
+
+def host_info():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu": model}
+
+
+def run_turbulence3d(args):
     import torch
 
     from wellbalanced_amd import MHD3D, initial
@@ -219,8 +241,176 @@ def main():
             "cores": 1,
             "kind": "port",
             "sample": f"NumPy FP64 oracle step, turbulence {sample}^3, {steps} steps in {elapsed:.1f} s",
+            "host": host_info(),
         }
-    print(json.dumps(out), flush=True)
+    return out
+
+
+STATE_BYTES_2D = 56  # 6 FP64 cell fields + 1 FP64 node (A_z) per cell
+
+
+def _line(metric_value, rate64, steps, args, sec, workload, cells, precision, l1, linf, stage_ms,
+          state_bytes, extra):
+    """The JSON line of a non-default workload (same contract as the default one)."""
+    achieved = 2.5 * state_bytes * cells / (stage_ms * 1e-3) / 1e9 if stage_ms else None
+    out = {
+        "metric": "zone-updates/s on 1 MI355X, mixed vs FP64 at stated L1 error; % HBM peak",
+        "value": metric_value,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * sec / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-flux/f64-state" if precision == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": workload, "cells": cells, "precision": precision, **extra},
+        "fp64_zone_updates_per_s": rate64,
+        "speedup_vs_fp64": metric_value / rate64,
+        "l1_density_vs_fp64": l1,
+        "linf_density_vs_fp64": linf,
+        "hbm_fraction_algorithmic": metric_value * 5 * state_bytes / (HBM_PEAK_GBS * 1e9),
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+            "kernel": "stage (both Heun stages)", "stage_ms": stage_ms,
+        },
+    }
+    return out
+
+
+def _pair_2d(problem, args, graph_steps=0):
+    """FP64 and mixed fused 2D models, tuned and timed; returns the line's pieces."""
+    import numpy as np
+    import torch
+
+    from wellbalanced_amd import MHD2D, autotune
+    from wellbalanced_amd.graph import StepGraph
+
+    res = {}
+    models = {}
+    for precision in ("fp64", "mixed"):
+        m = MHD2D.from_problem(problem, kernels="fused", precision=precision)
+        launch = autotune.tune(m, steps=2) if not args.no_tune else {}
+        m.check_interval = 0
+        _, stage_ms = time_model(m, 3, 1)
+        if graph_steps:
+            graph = StepGraph(m, steps=graph_steps)
+            reps = max(1, args.steps // graph_steps)
+            graph.replay(1)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            graph.replay(reps)
+            torch.cuda.synchronize()
+            sec, steps = time.perf_counter() - t0, reps * graph_steps
+        else:
+            for _ in range(args.warmup):
+                m.advance(1)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            m.advance(args.steps)
+            torch.cuda.synchronize()
+            sec, steps = time.perf_counter() - t0, args.steps
+        m.check()
+        res[precision] = (m.nx * m.ny * steps / sec, sec, steps, stage_ms, launch)
+        models[precision] = m
+    a = models["fp64"].dataset()["density"]
+    b = models["mixed"].dataset()["density"]
+    l1, linf = float(np.mean(np.abs(a - b))), float(np.max(np.abs(a - b)))
+    div = max(models["fp64"].divergence(), models["mixed"].divergence())
+    return res, l1, linf, div, models
+
+
+def run_ot2d(args):
+    from wellbalanced_amd import initial
+
+    n = 256
+    problem = initial.orszag_tang(n, limiter="minmod")
+    res, l1, linf, div, models = _pair_2d(problem, args)
+    rate, sec, steps, stage_ms, launch = res["mixed"]
+    return _line(rate, res["fp64"][0], steps, args, sec,
+                 f"2D Orszag-Tang {n}^2 periodic, minmod, CFL 0.4, SSP-RK2", n * n, "mixed",
+                 l1, linf, stage_ms, STATE_BYTES_2D,
+                 {"launch": {"fp64": res["fp64"][4], "mixed": launch}, "divergence": div,
+                  "time": models["mixed"].time})
+
+
+def run_harris2d(args):
+    from wellbalanced_amd import initial
+
+    problem = initial.harris_sheet(2048, 1024)
+    res, l1, linf, div, models = _pair_2d(problem, args, graph_steps=10)
+    rate, sec, steps, stage_ms, launch = res["mixed"]
+    return _line(rate, res["fp64"][0], steps, args, sec,
+                 "2D Harris sheet 2048x1024, walls in y, minmod, CFL 0.4, SSP-RK2, "
+                 "steps replayed from HIP graphs of 10 steps", 2048 * 1024, "mixed", l1, linf,
+                 stage_ms, STATE_BYTES_2D,
+                 {"launch": {"fp64": res["fp64"][4], "mixed": launch}, "divergence": div,
+                  "time": models["mixed"].time})
+
+
+def _leaf_l1(a_model, b_model):
+    """Volume-weighted mean and max |rho_a - rho_b| over the leaf cells of two
+    hierarchy models on the same geometry (the domain is the unit square)."""
+    import numpy as np
+
+    from wellbalanced_amd.amr_maps import HALO
+
+    buf_a, buf_b = a_model.buffer(), b_model.buffer()
+    maps = a_model.maps
+    covered = np.zeros(maps.total, bool)
+    for L in range(a_model.nlev - 1):
+        rows = maps.average[L]
+        if len(rows):
+            covered[rows[:, 0]] = True
+    total, area, worst = 0.0, 0.0, 0.0
+    for lay in maps.layouts:
+        dx, dy = a_model.hierarchy.spacing(lay.level)
+        i, j = np.meshgrid(np.arange(lay.nx) + HALO, np.arange(lay.ny) + HALO, indexing="ij")
+        offs = lay.cell(i, j)[~covered[lay.cell(i, j)]]
+        d = np.abs(buf_a[offs] - buf_b[offs])
+        total += float(d.sum()) * dx * dy
+        area += offs.size * dx * dy
+        worst = max(worst, float(d.max()) if d.size else 0.0)
+    return total / area, worst
+
+
+def run_blast2d_amr(args):
+    import torch
+
+    from wellbalanced_amd import autotune
+    from wellbalanced_amd.amr import AMRMHD2D, blast_hierarchy, blast_state
+
+    h = blast_hierarchy(512, 3)
+    res, models = {}, {}
+    for precision in ("fp64", "mixed"):
+        m = AMRMHD2D(h, limiter="minmod", mean_field=(2**-0.5, 2**-0.5), precision=precision)
+        m.set_state(**blast_state())
+        launch = autotune.tune(m, steps=2) if not args.no_tune else {}
+        _, stage_ms = time_model(m, 3, 1)
+        for _ in range(args.warmup):
+            m.advance(1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        m.advance(args.steps)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        m.check()
+        res[precision] = (m.cells * args.steps / sec, sec, stage_ms, launch)
+        models[precision] = m
+    l1, linf = _leaf_l1(models["fp64"], models["mixed"])
+    m = models["mixed"]
+    leaf = int(m.cells - sum(len(m.maps.average[L]) for L in range(m.nlev - 1)))
+    rate, sec, stage_ms, launch = res["mixed"]
+    return _line(rate, res["fp64"][0], args.steps, args, sec,
+                 "2D blast on a 3-level hierarchy (512^2 base, patches of 16..128 cells), "
+                 "minmod, CFL 0.4, SSP-RK2, one persistent launch per stage", m.cells, "mixed",
+                 l1, linf, stage_ms, STATE_BYTES_2D,
+                 {"launch": {"fp64": res["fp64"][3], "mixed": launch}, "patches": len(m.layouts),
+                  "leaf_cells": leaf, "divergence": m.divergence(),
+                  "composite_divergence": m.composite_divergence()})
 
 
 if __name__ == "__main__":

@@ -157,6 +157,18 @@ int wb_rate3d(const double* state, int64_t nx, int64_t ny, int64_t nz, double dx
               double dz, double b0x, double b0y, double b0z, double gamma, double* out,
               void* stream);
 
+/* ---- HIP graphs of whole steps -------------------------------------------------- */
+/* The step of the reference (MHD2D.step mhd.py:965-994, MHD3D.step mhd3d.py:726-754)
+ * enqueues only this library's kernels with dt, the time and the counters on the
+ * device, so k steps are captured by bracketing their calls on a non-default stream:
+ * wb_graph_capture_begin, the step's entry points, wb_graph_capture_end (which
+ * instantiates the graph into *graph).  wb_graph_launch replays it `times` times on
+ * `stream`.  The graph keeps the device pointers it was captured with. */
+int wb_graph_capture_begin(void* stream);
+int wb_graph_capture_end(void* stream, void** graph);
+int wb_graph_launch(void* graph, int times, void* stream);
+int wb_graph_destroy(void* graph);
+
 /* ---- refinement estimator ---------------------------------------------------------- */
 
 /* The second-derivative estimator of Löhner (1987) in FLASH's normalised form,
@@ -167,6 +179,18 @@ int wb_rate3d(const double* state, int64_t nx, int64_t ny, int64_t nz, double dx
 int wb_lohner3d(const double* state, int64_t nx, int64_t ny, int64_t nz, double dx, double dy,
                 double dz, double b0x, double b0y, double b0z, double gamma, double eps,
                 double* scratch, double* out, void* stream);
+
+/* The reference's estimator, current_tags: out[i, j] = 1 where the mean of |J_z| at
+ * the cell's four nodes times max(dx, dy) exceeds threshold (mhd.py:1011-1041,
+ * _current mhd.py:826-831), from the face fields of wb_primitive2d (halos filled);
+ * out: nx*ny bytes. */
+int wb_current_tags2d(const double* bxf, const double* byf, int64_t nx, int64_t ny, double dx,
+                      double dy, double threshold, uint8_t* out, void* stream);
+
+/* 3D (mhd3d.py:771-801): |J| of the cell field's curl by central differences times
+ * max(dx, dy, dz) > threshold, from prim (8, X, Y, Z) of wb_primitive3d. */
+int wb_current_tags3d(const double* prim, int64_t nx, int64_t ny, int64_t nz, double dx, double dy,
+                      double dz, double threshold, uint8_t* out, void* stream);
 
 /* ---- learned refinement indicator (3D CNN, BF16 MFMA, FP32 accumulation) -------- */
 /* Three 3x3x3 periodic convolutions 8 -> 16 -> 16 -> 1 with ReLU, a logit per cell;

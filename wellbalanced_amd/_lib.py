@@ -41,6 +41,12 @@ SIGNATURES: dict[str, tuple[type, list]] = {
     "wb_mhd3d_stage_mixed": (I, [P, P, I64, I64, I64] + [D] * 7 + [I, P, I, P, I, I, P]),
     "wb_rate3d": (I, [P, I64, I64, I64] + [D] * 7 + [P, P]),
     "wb_lohner3d": (I, [P, I64, I64, I64] + [D] * 8 + [P, P, P]),
+    "wb_graph_capture_begin": (I, [P]),
+    "wb_graph_capture_end": (I, [P, P]),
+    "wb_graph_launch": (I, [P, I, P]),
+    "wb_graph_destroy": (I, [P]),
+    "wb_current_tags2d": (I, [P, P, I64, I64, D, D, D, P, P]),
+    "wb_current_tags3d": (I, [P, I64, I64, I64, D, D, D, D, P, P]),
     "wb_cnn_prep": (I, [P, P, P]),
     "wb_cnn_moments": (I, [P, I64, I64, I64] + [D] * 7 + [P, I, P, P]),
     "wb_cnn_input": (I, [P, I64, I64, I64] + [D] * 7 + [P, P, P]),

@@ -249,6 +249,38 @@ class AMRMHD2D:
             self.steps += 1
         return marks
 
+    def current_tags(self, threshold: float, levels: int) -> dict[int, np.ndarray]:
+        """Tag the cells where |J_z| times the cell size exceeds ``threshold`` on levels
+        ``0 .. levels - 1`` that exist, patch by patch (``mhd.py:1011-1041``)."""
+        stream = _lib.stream_handle(self.device)
+        buf = self._buf[0]
+        self._fill(buf, stream)
+        tags = {L: np.zeros(self.hierarchy.level_shape(L), dtype=bool)
+                for L in range(min(levels, self.nlev))}
+        big = max(lay.X * lay.Y for lay in self.layouts)
+        opts = dict(dtype=torch.float64, device=self.device)
+        prim = torch.empty(8 * big, **opts)
+        bxf = torch.empty(big + max(lay.Y for lay in self.layouts), **opts)
+        byf = torch.empty(big + max(lay.X for lay in self.layouts), **opts)
+        ecc = torch.empty(big, **opts)
+        out = torch.empty(big, dtype=torch.uint8, device=self.device)
+        for lay in self.layouts:
+            if lay.level not in tags:
+                continue
+            dx, dy = self.hierarchy.spacing(lay.level)
+            base = buf.data_ptr() + 8 * lay.offset
+            plane = 8 * lay.plane
+            _lib.call("wb_primitive2d", *[base + k * plane for k in range(6)], base + 6 * plane,
+                      lay.X, lay.Y, dx, dy, *self.mean_field, self.gamma, _lib.ptr(prim),
+                      _lib.ptr(bxf), _lib.ptr(byf), _lib.ptr(ecc), self._bad.data_ptr() + 4,
+                      stream)
+            _lib.call("wb_current_tags2d", _lib.ptr(bxf), _lib.ptr(byf), lay.nx, lay.ny, dx, dy,
+                      float(threshold), _lib.ptr(out), stream)
+            cell = out[: lay.nx * lay.ny].view(lay.nx, lay.ny).cpu().numpy().astype(bool)
+            (i0, i1), (j0, j1) = lay.box
+            tags[lay.level][i0:i1, j0:j1] |= cell
+        return tags
+
     def check(self) -> None:
         if int(self._bad.cpu().sum()):
             raise FloatingPointError(f"non-positive density or pressure at t = {self.time}")

@@ -94,6 +94,14 @@ __device__ __forceinline__ void side(const float* w, float bn, float gamma, floa
   s.cf = sqrtf_(0.5f * (a2 + b2 + root));
 }
 
+// Below this fraction of p_T + B_n^2 the FP32 star denominator is treated as zero.
+// The reference's guard (1e-12, mhd_sweep_pyccel.py:127) is below FP32 resolution:
+// den = d (s - s_M) - B_n^2 is a cancellation whose FP32 rounding leaves noise of
+// about 1e-6 (p_T + B_n^2) -- where c_f ~ c_A with a weak tangential field, a
+// noise-sized den passing a 1e-12 guard scales B_t by num / den ~ 1e4.  Treating
+// |den| < 1e-5 (p_T + B_n^2) as degenerate keeps the tangential state instead.
+constexpr float DEGENERATE32 = 1.0e-5f;
+
 // Star state beside the contact (mhd_sweep_pyccel.py:101-145): us, and
 // vt = (vt1, vt2, bt1, bt2).
 __device__ __forceinline__ void star(const float* w, const Side& sd, float bn, float s, float d,
@@ -102,7 +110,7 @@ __device__ __forceinline__ void star(const float* w, const Side& sd, float bn, f
   const float inv = rcp(s - sm);
   const float rho_s = d * inv;
   const float den = d * (s - sm) - bn * bn;
-  const bool regular = __builtin_fabsf(den) >= float(DEGENERATE) * (sd.pt + bn * bn);
+  const bool regular = __builtin_fabsf(den) >= DEGENERATE32 * (sd.pt + bn * bn);
   const float iden = rcp(den);
   const float cv = bn * (sm - vn) * iden;
   const float cb = (d * (s - vn) - bn * bn) * iden;

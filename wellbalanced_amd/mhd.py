@@ -325,6 +325,29 @@ class MHD2D:
             self.steps += 1
         return marks
 
+    def current_tags(self, threshold: float, levels: int = 1) -> dict[int, np.ndarray]:
+        """Tag the cells where |J_z| times the cell size exceeds ``threshold``
+        (``mhd.py:1011-1041``): the node current from the face fields, averaged over a
+        cell's four nodes.  The uniform grid is level 0."""
+        self._check_ready()
+        if levels < 1:
+            return {}
+        stream = _lib.stream_handle(self.device)
+        X, Y, nx, ny = self.X, self.Y, self.nx, self.ny
+        src = self._buf[0]
+        pmask = int(self.periodic[0]) | (int(self.periodic[1]) << 1)
+        wmask = int(self.walls[0]) | (int(self.walls[1]) << 1)
+        _lib.call("wb_mhd2d_fill_halos", _lib.ptr(src), nx, ny, pmask, wmask, stream)
+        base, plane = src.data_ptr(), X * Y * 8
+        _lib.call("wb_primitive2d", *[base + k * plane for k in range(6)], base + self._cells * 8,
+                  X, Y, self.dx, self.dy, *self.mean_field, self.gamma, _lib.ptr(self._prim),
+                  _lib.ptr(self._bxf), _lib.ptr(self._byf), _lib.ptr(self._ecc),
+                  self._bad.data_ptr() + 4, stream)
+        out = torch.empty(nx * ny, dtype=torch.uint8, device=self.device)
+        _lib.call("wb_current_tags2d", _lib.ptr(self._bxf), _lib.ptr(self._byf), nx, ny, self.dx,
+                  self.dy, float(threshold), _lib.ptr(out), stream)
+        return {0: out.view(nx, ny).cpu().numpy().astype(bool)}
+
     def check(self) -> None:
         """Raise FloatingPointError if a stage met a bad cell (mhd.py:852-857); synchronises."""
         bad = self._bad.cpu()

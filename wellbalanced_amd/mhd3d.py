@@ -337,6 +337,27 @@ class MHD3D:
             self.steps += 1
         return marks
 
+    def current_tags(self, threshold: float, levels: int = 1) -> dict[int, np.ndarray]:
+        """Tag the cells where |J| (the cell field's curl by central differences) times
+        the cell size exceeds ``threshold`` (``mhd3d.py:771-801``).  The uniform grid
+        is level 0."""
+        self._check_ready()
+        if levels < 1:
+            return {}
+        stream = _lib.stream_handle(self.device)
+        nx, ny, nz = self.n
+        w = self._workspace()
+        src = self._buf[0]
+        pmask = sum(int(p) << a for a, p in enumerate(self.periodic))
+        _lib.call("wb_mhd3d_fill_halos", _lib.ptr(src), nx, ny, nz, pmask, stream)
+        _lib.call("wb_primitive3d", _lib.ptr(src), nx, ny, nz, *self.spacing, *self.mean_field,
+                  self.gamma, _lib.ptr(w["prim"]), _lib.ptr(w["f0"]), _lib.ptr(w["f1"]),
+                  _lib.ptr(w["f2"]), self._bad.data_ptr() + 4, stream)
+        out = torch.empty(nx * ny * nz, dtype=torch.uint8, device=self.device)
+        _lib.call("wb_current_tags3d", _lib.ptr(w["prim"]), nx, ny, nz, *self.spacing,
+                  float(threshold), _lib.ptr(out), stream)
+        return {0: out.view(nx, ny, nz).cpu().numpy().astype(bool)}
+
     def check(self) -> None:
         bad = self._bad.cpu()
         if int(bad.sum()):
