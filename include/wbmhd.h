@@ -267,9 +267,11 @@ int wb_amr2d_average(double* buf, const long long* rows, int n, void* stream);
 /* Coarse A_z nodes := the fine node on them (rows of 2 int64), mhd.py:981,991. */
 int wb_amr2d_restrict(double* buf, const long long* rows, int n, void* stream);
 
-/* The CFL rate over every patch cell (mhd.py:930-963): out = device double[2]. */
-int wb_amr2d_rate(const double* buf, const void* patches, int npatches, int maxcells, double b0x,
-                  double b0y, double gamma, double* out, void* stream);
+/* The CFL rate over every patch cell (mhd.py:930-963): out = device double[2].
+ * chunks: int32 pairs (patch, first cell) cutting every patch's cells into runs of
+ * 256 (built on the host with the other tables). */
+int wb_amr2d_rate(const double* buf, const void* patches, const int32_t* chunks, int nchunks,
+                  double b0x, double b0y, double gamma, double* out, void* stream);
 
 /* One stage of every patch in ONE persistent launch: `workgroups` workgroups take
  * (patch, tile, segment) items (int32 triples) from *counter until `nitems` are done

@@ -59,7 +59,7 @@ SIGNATURES: dict[str, tuple[type, list]] = {
     "wb_amr2d_interp": (I, [P] * 8 + [I] + [P] * 6 + [I, P]),
     "wb_amr2d_average": (I, [P, P, I, P]),
     "wb_amr2d_restrict": (I, [P, P, I, P]),
-    "wb_amr2d_rate": (I, [P, P, I, I, D, D, D, P, P]),
+    "wb_amr2d_rate": (I, [P, P, P, I, D, D, D, P, P]),
     "wb_amr2d_stage": (I, [P, P, P, P, I, P, D, D, D, I, P, I, P, I, I, I, P]),
     "wb_amr2d_stage_mixed": (I, [P, P, P, P, I, P, D, D, D, I, P, I, P, I, I, I, P]),
     "wb_amr2d_reflux": (I, [P, P, P, P, P, I, D, D, D, I, P, I, P]),

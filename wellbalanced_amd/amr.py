@@ -135,6 +135,11 @@ class AMRMHD2D:
         self._patches = torch.from_numpy(patches.view(np.uint8).copy()).to(self.device)
         self._items = torch.from_numpy(items.view(np.uint8).copy()).to(self.device)
         self.nitems = len(items)
+        # the CFL reduction walks every patch's cells in chunks of 256
+        chunks = [(k, c) for k, lay in enumerate(self.layouts)
+                  for c in range(0, lay.nx * lay.ny, 256)]
+        self._chunks = self._dev(np.array(chunks, np.int32).reshape(-1), torch.int32)
+        self.nchunks = len(chunks)
 
     # -- state ---------------------------------------------------------------------------
     def set_state(self, density, velocity, pressure, *, vector_potential=0.0, bz=0.0) -> None:
@@ -201,8 +206,8 @@ class AMRMHD2D:
 
     def _rate_kernel(self, stream: int) -> None:
         _lib.call("wb_amr2d_rate", _lib.ptr(self._buf[0]), _lib.ptr(self._patches),
-                  len(self.layouts), max(l.nx * l.ny for l in self.layouts), *self.mean_field,
-                  self.gamma, _lib.ptr(self._rate), stream)
+                  _lib.ptr(self._chunks), self.nchunks, *self.mean_field, self.gamma,
+                  _lib.ptr(self._rate), stream)
 
     def stable_step(self) -> float:
         self._rate_kernel(_lib.stream_handle(self.device))

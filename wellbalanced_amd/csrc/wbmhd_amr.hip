@@ -86,30 +86,39 @@ __global__ void k_restrict(double* __restrict__ buf, const long long* __restrict
 }
 
 // The CFL rate over the own cells of every patch (mhd_rate_pyccel.py:30-83 per patch,
-// the largest over all): blockIdx.y is the patch.
-__global__ void k_rate_amr(const double* __restrict__ buf, const PatchDesc* __restrict__ patches,
-                           double b0x, double b0y, double gamma, unsigned long long* keys) {
-  const PatchDesc pd = patches[blockIdx.y];
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+// the largest over all).  The cells are cut into chunks of 256 of one patch (chunks:
+// int32 pairs patch, first cell, built on the host); a fixed grid walks the chunks
+// and each workgroup ends with one atomic pair.
+__global__ void __launch_bounds__(256) k_rate_amr(const double* __restrict__ buf,
+                                                  const PatchDesc* __restrict__ patches,
+                                                  const int* __restrict__ chunks, int nchunks,
+                                                  double b0x, double b0y, double gamma,
+                                                  unsigned long long* keys) {
   double rate = 0.0, r = __builtin_inf();
-  if (t < pd.nx * pd.ny) {
+  for (int ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    const PatchDesc pd = patches[chunks[2 * ch]];
+    const int t = chunks[2 * ch + 1] + (int)threadIdx.x;
+    if (t >= pd.nx * pd.ny) continue;
     const int X = pd.nx + 4, Y = pd.ny + 4, YN = Y + 1;
     const int i = t / pd.ny + 2, j = t % pd.ny + 2;
     const double* in = buf + pd.off;
     const double* az = in + 6 * (size_t)X * Y;
     const double dx = pd.dx, dy = pd.dy;
     const size_t c = (size_t)i * Y + j, P = (size_t)X * Y;
-    r = in[c];
-    const double u = in[P + c] / r, v = in[2 * P + c] / r, w = in[3 * P + c] / r;
+    const double rc = in[c];
+    const double u = in[P + c] / rc, v = in[2 * P + c] / rc, w = in[3 * P + c] / rc;
     const double bx = 0.5 * ((az[(size_t)i * YN + j + 1] - az[(size_t)i * YN + j]) / dy + b0x +
                              ((az[(size_t)(i + 1) * YN + j + 1] - az[(size_t)(i + 1) * YN + j]) / dy + b0x));
     const double by = 0.5 * (-(az[(size_t)(i + 1) * YN + j] - az[(size_t)i * YN + j]) / dx + b0y +
                              (-(az[(size_t)(i + 1) * YN + j + 1] - az[(size_t)i * YN + j + 1]) / dx + b0y));
     const double bzc = in[5 * P + c];
-    const double p = (gamma - 1.0) * (in[4 * P + c] - 0.5 * r * (u * u + v * v + w * w) -
+    const double p = (gamma - 1.0) * (in[4 * P + c] - 0.5 * rc * (u * u + v * v + w * w) -
                                       0.5 * (bx * bx + by * by + bzc * bzc));
-    rate = (__builtin_fabs(u) + fast_speed_rate(r, p, bx, by, bzc, gamma)) / dx +
-           (__builtin_fabs(v) + fast_speed_rate(r, p, by, bx, bzc, gamma)) / dy;
+    const double cr = (__builtin_fabs(u) + fast_speed_rate(rc, p, bx, by, bzc, gamma)) / dx +
+                      (__builtin_fabs(v) + fast_speed_rate(rc, p, by, bx, bzc, gamma)) / dy;
+    // NaN rates and densities drop out, as in the single-patch reduction
+    rate = cr > rate ? cr : rate;
+    r = rc < r ? rc : r;
   }
   block_reduce_rate_rho_once(rate, r, keys);
 }
@@ -150,14 +159,15 @@ int wb_amr2d_restrict(double* buf, const long long* rows, int n, void* stream) {
   return wb_last_error();
 }
 
-int wb_amr2d_rate(const double* buf, const void* patches, int npatches, int maxcells, double b0x,
-                  double b0y, double gamma, double* out, void* stream) {
-  if (npatches <= 0 || maxcells <= 0) return WB_EINVAL;
+int wb_amr2d_rate(const double* buf, const void* patches, const int32_t* chunks, int nchunks,
+                  double b0x, double b0y, double gamma, double* out, void* stream) {
+  if (nchunks < 1) return WB_EINVAL;
   hipStream_t s = (hipStream_t)stream;
   unsigned long long* keys = (unsigned long long*)out;
   hipLaunchKernelGGL(k_rate_init, dim3(1), dim3(64), 0, s, keys);
-  hipLaunchKernelGGL(k_rate_amr, dim3((unsigned)((maxcells + 255) / 256), (unsigned)npatches),
-                     dim3(256), 0, s, buf, (const PatchDesc*)patches, b0x, b0y, gamma, keys);
+  const unsigned grid = (unsigned)(nchunks < 1024 ? nchunks : 1024);
+  hipLaunchKernelGGL(k_rate_amr, dim3(grid), dim3(256), 0, s, buf, (const PatchDesc*)patches,
+                     (const int*)chunks, nchunks, b0x, b0y, gamma, keys);
   hipLaunchKernelGGL(k_rate_finish, dim3(1), dim3(64), 0, s, keys, out);
   return wb_last_error();
 }
