@@ -298,6 +298,49 @@ int wb_amr2d_reflux_mixed(const double* src, double* dst, const void* patches,
                           double b0y, double gamma, int limiter, const double* dt, int heun,
                           void* stream);
 
+/* ---- 3D hierarchy of equal blocks (refined, periodic) ---------------------------- */
+/* Every level is a set of blocks of 16^3 cells with a halo of 2, each stored as the
+ * uniform 3D state of a 16^3 grid, back to back in one buffer; level L+1 refines
+ * level L by 2 as an octree.  `blocks`: per block {int64 offset; int32 level, bx, by,
+ * bz; double dx, dy, dz}; `lookup`: per level the block id at every block coordinate
+ * (-1: none), the levels back to back; `levels`: {int32 nlev; int32 nb[6]; int32
+ * base[6]} (blocks per axis and first lookup entry of each level).  Replaces
+ * patchwork's exchange_halos / restrict / FluxRegister.reflux as MHD3D.step calls
+ * them (mhd3d.py:448-473,635-671,726-754), all blocks of a stage in one launch. */
+
+/* Ghost cells and edges of every block: copies from the block of the same level
+ * holding them, else interpolation from the level below. */
+int wb_amr3d_fill(double* buf, const void* blocks, int nblocks, const int32_t* lookup,
+                  const void* levels, void* stream);
+
+/* One level's coarsening: the covered cells of each refined block := the mean of its
+ * 8 children, and its edges := the mean of the two fine edges along them (every coarse
+ * copy); children: rows of 9 int32 (the block, then its 8 children, x slowest). */
+int wb_amr3d_coarsen(double* buf, const void* blocks, const int32_t* children, int nrefined,
+                     const int32_t* lookup, const void* levels, void* stream);
+
+/* The CFL rate over every block cell (mhd3d.py:674-724): out = device double[2]. */
+int wb_amr3d_rate(const double* buf, const void* blocks, int nblocks, double b0x, double b0y,
+                  double b0z, double gamma, double* out, void* stream);
+
+/* One stage of every block (FP64 parity path / FP32 flux). */
+int wb_amr3d_stage(const double* in, double* out, const void* blocks, int nblocks, double b0x,
+                   double b0y, double b0z, double gamma, int limiter, const double* dt, int heun,
+                   int32_t* bad, void* stream);
+int wb_amr3d_stage_mixed(const double* in, double* out, const void* blocks, int nblocks,
+                         double b0x, double b0y, double b0z, double gamma, int limiter,
+                         const double* dt, int heun, int32_t* bad, void* stream);
+
+/* Refluxing of one direction (axis, side 0: coarse cells at the fine blocks' low
+ * faces, 1: at their high faces) after a stage; items: {int32 fine, coarse, ci, cj, ck,
+ * pad} per coarse-fine block face.  Run the six directions in turn. */
+int wb_amr3d_reflux(const double* src, double* dst, const void* blocks, const void* items,
+                    int nitems, int axis, int side, double b0x, double b0y, double b0z,
+                    double gamma, int limiter, const double* dt, int heun, void* stream);
+int wb_amr3d_reflux_mixed(const double* src, double* dst, const void* blocks, const void* items,
+                          int nitems, int axis, int side, double b0x, double b0y, double b0z,
+                          double gamma, int limiter, const double* dt, int heun, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

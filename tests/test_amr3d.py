@@ -1,0 +1,153 @@
+"""The refined 3D hierarchy of equal blocks: the oracle's invariants (CPU) and the
+GPU path against the oracle (FP64: bit for bit)."""
+
+import numpy as np
+import pytest
+
+from oracle.amr3d import OracleAMR3D
+from oracle.mhd3d import Oracle3D
+from wellbalanced_amd import amr3d_maps, initial
+from wellbalanced_amd.amr3d_maps import Hierarchy3D
+from wellbalanced_amd.hierarchy import AMRHierarchy
+
+TWO_PI = 2.0 * np.pi
+B0 = (0.3, 0.2, 0.1)
+
+
+def _state():
+    return dict(
+        density=lambda x, y, z: 1.0 + 0.2 * np.sin(TWO_PI * x) * np.sin(TWO_PI * y) * np.cos(TWO_PI * z),
+        velocity=lambda x, y, z: (0.3 + 0.1 * np.cos(TWO_PI * y), 0.2 + 0.0 * x, 0.1 * np.sin(TWO_PI * x)),
+        pressure=lambda x, y, z: 1.0 + 0.1 * np.cos(TWO_PI * z) + 0.05 * np.sin(TWO_PI * x),
+        vector_potential=(lambda x, y, z: 0.05 * np.sin(TWO_PI * y) * np.cos(TWO_PI * z),
+                          lambda x, y, z: 0.05 * np.sin(TWO_PI * z),
+                          lambda x, y, z: 0.05 * np.sin(TWO_PI * x) + 0.02 * np.cos(TWO_PI * y)),
+    )
+
+
+def _blast_state():
+    """A pressure bump off the block corners, smooth enough for 2 steps."""
+    return dict(
+        density=1.0,
+        velocity=(0.1, -0.05, 0.02),
+        pressure=lambda x, y, z: 1.0 + 4.0 * np.exp(-((x - 0.3) ** 2 + (y - 0.35) ** 2 + (z - 0.3) ** 2) / 0.01),
+        vector_potential=(lambda x, y, z: 0.0 * x,
+                          lambda x, y, z: 0.1 * np.sin(TWO_PI * z),
+                          lambda x, y, z: 0.1 * np.sin(TWO_PI * x)),
+    )
+
+
+def _oracle(h, state, limiter="minmod"):
+    t = amr3d_maps.build(h)
+    buf = amr3d_maps.pack_state(h, t, 5.0 / 3.0, B0, state)
+    o = OracleAMR3D(h, t, buf, mean_field=B0, limiter=limiter)
+    o.prepare()
+    return t, o
+
+
+def _two_levels():
+    h = Hierarchy3D(32)
+    h.refine(0, [(0, 0, 0)])
+    return h
+
+
+def _three_levels():
+    h = Hierarchy3D(32)
+    h.refine(0, [(0, 0, 0)])
+    h.refine(1, [(1, 1, 1)])
+    return h
+
+
+def test_nesting_adds_the_parent_neighbourhood():
+    h = _three_levels()
+    assert len(h.levels) == 3 and len(h.levels[2]) == 8
+    for b in h.levels[2]:
+        parent = tuple(v // 2 for v in b)
+        for d in np.ndindex(3, 3, 3):
+            q = tuple((parent[a] + d[a] - 1) % h.nb(1) for a in range(3))
+            assert q in h.levels[1]
+
+
+def test_a_refined_level_covering_the_domain_is_the_uniform_run():
+    """All 8 level-1 blocks over the periodic domain evolve exactly as one uniform
+    32^3 grid: same-level ghost cells and shared edges are exact copies."""
+    h = Hierarchy3D(16)
+    h.refine(0, [(0, 0, 0)])
+    t, amr = _oracle(h, _state())
+    uh = AMRHierarchy((32, 32, 32), ((0.0, 1.0),) * 3, periodic=(True, True, True))
+    cons, pots = initial.host_state_3d(uh, uh.patches[0], 5.0 / 3.0, B0, **_state())
+    uniform = Oracle3D(cons, pots, uh.spacing(0), mean_field=B0, limiter="minmod")
+    for _ in range(2):
+        dt = uniform.stable_step()
+        uniform.step(dt)
+        amr.step(dt)
+    for k, row in enumerate(t.blocks):
+        if row["level"] != 1:
+            continue
+        cells, bp = amr.views(amr.buf, k)
+        o = [16 * int(row[a]) for a in ("bx", "by", "bz")]
+        want = uniform.interior()[:, o[0] : o[0] + 16, o[1] : o[1] + 16, o[2] : o[2] + 16]
+        np.testing.assert_array_equal(cells[:, 2:18, 2:18, 2:18], want)
+        for c in range(3):
+            sl = tuple(slice(2, 18 + (a != c)) for a in range(3))
+            usl = tuple(slice(2 + o[a], 18 + o[a] + (a != c)) for a in range(3))
+            np.testing.assert_array_equal(bp[c][sl], uniform.pot[c][usl])
+
+
+@pytest.mark.parametrize("levels", [2, 3])
+def test_refined_blocks_conserve_and_keep_div_b_zero(levels):
+    h = _two_levels() if levels == 2 else _three_levels()
+    t, amr = _oracle(h, _blast_state())
+    before = amr.leaf_totals()
+    for _ in range(2):
+        amr.step()
+    after = amr.leaf_totals()
+    np.testing.assert_allclose(after, before, rtol=1e-13, atol=1e-14)
+    for k in range(len(t.blocks)):
+        o = Oracle3D(*amr.views(amr.buf, k), amr.spacing(k), mean_field=B0)
+        assert o.divergence() < 1e-13
+    # the coarse edges of refined blocks are the mean of the fine edges along them
+    for row in t.children:
+        _, cp = amr.views(amr.buf, row[0])
+        kids = [amr.views(amr.buf, k)[1] for k in row[1:]]
+        np.testing.assert_array_equal(cp[0][2, 2, 2], 0.5 * (kids[0][0][2, 2, 2] + kids[0][0][3, 2, 2]))
+
+
+# -- GPU ----------------------------------------------------------------------------------
+@pytest.mark.gpu
+def test_gpu_hierarchy_3d_is_bitwise_the_oracle(gpu):
+    from wellbalanced_amd.amr3d import AMRMHD3D
+
+    h = _two_levels()
+    t, amr = _oracle(h, _blast_state())
+    model = AMRMHD3D(h, mean_field=B0, limiter="minmod")
+    model.set_state(**_blast_state())
+    np.testing.assert_array_equal(model.buffer(), amr.buf)
+    for _ in range(2):
+        dt = amr.stable_step()
+        assert model.stable_step() == dt
+        model.step(dt)
+        amr.step(dt)
+    got = model.buffer()
+    for k in range(len(t.blocks)):
+        gc, gp = amr.views(got, k)
+        wc, wp = amr.views(amr.buf, k)
+        np.testing.assert_array_equal(gc[:, 2:18, 2:18, 2:18], wc[:, 2:18, 2:18, 2:18])
+        for c in range(3):
+            sl = tuple(slice(2, 18 + (a != c)) for a in range(3))
+            np.testing.assert_array_equal(gp[c][sl], wp[c][sl])
+
+
+@pytest.mark.gpu
+def test_gpu_three_level_mixed_conserves(gpu):
+    from wellbalanced_amd.amr3d import AMRMHD3D
+
+    h = _three_levels()
+    model = AMRMHD3D(h, mean_field=B0, limiter="minmod", precision="mixed")
+    model.set_state(**_blast_state())
+    before = model.leaf_totals()
+    model.advance(4)
+    model.check()
+    np.testing.assert_allclose(model.leaf_totals(), before, rtol=1e-12, atol=1e-13)
+    assert model.divergence() < 1e-13
+    assert model.composite_divergence() < 1e-12

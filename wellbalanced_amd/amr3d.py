@@ -1,0 +1,249 @@
+"""The ideal-MHD step on a refined 3D hierarchy of equal blocks (periodic).
+
+`AMRMHD3D` runs ``MHD3D.step`` (``mhd3d.py:726-754``) on every block of every level
+of a `wellbalanced_amd.amr3d_maps.Hierarchy3D`: one time step for all levels, the
+stage of every block in one launch, refluxing at coarse-fine faces, the covered
+cells averaged down and the potential restricted (the mean of the fine edges along
+a coarse edge, ``mhd3d.py:31-36``) so that div B is zero on the composite mesh.
+The ghost cells of a block come from the blocks of its level, or by interpolation
+from the level below (``wb_amr3d_fill``).  ``precision="mixed"`` runs the FP32
+reconstruction/HLLD stage.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib, amr3d_maps
+from .amr3d_maps import HALO, Hierarchy3D, block_views
+from .mhd import LIMITER_CODES
+
+PRECISIONS = ("fp64", "mixed")
+_LEVELS = np.dtype([("nlev", np.int32), ("nb", np.int32, 6), ("base", np.int32, 6)])
+
+
+class AMRMHD3D:
+    def __init__(self, hierarchy: Hierarchy3D, *, gamma: float = 5.0 / 3.0,
+                 mean_field=(0.0, 0.0, 0.0), limiter: str = "minmod", cfl: float = 0.3,
+                 precision: str = "fp64", device=None) -> None:
+        if limiter not in LIMITER_CODES:
+            raise ValueError(f"limiter is one of {LIMITER_CODES}")
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision is one of {PRECISIONS}")
+        if len(hierarchy.levels) > 6:
+            raise ValueError("at most 6 levels")
+        _lib.load()
+        self.hierarchy = hierarchy
+        self.gamma = float(gamma)
+        self.mean_field = tuple(float(b) for b in mean_field)
+        self.limiter = limiter
+        self.cfl = float(cfl)
+        self.precision = precision
+        self.device = torch.device(device or "cuda")
+        self.tables = amr3d_maps.build(hierarchy)
+        self.nlev = len(hierarchy.levels)
+        self.nblocks = len(self.tables.blocks)
+        opts = dict(dtype=torch.float64, device=self.device)
+        self._buf = [torch.zeros(self.tables.total, **opts), torch.zeros(self.tables.total, **opts)]
+        self._rate = torch.zeros(2, **opts)
+        self._dt = torch.zeros(1, **opts)
+        self._t = torch.zeros(1, **opts)
+        self._bad = torch.zeros(2, dtype=torch.int32, device=self.device)
+        self.steps = 0
+        self.check_interval = 1
+        self._upload_tables()
+
+    # -- tables --------------------------------------------------------------------------
+    def _bytes(self, a: np.ndarray) -> torch.Tensor:
+        return torch.from_numpy(np.ascontiguousarray(a).view(np.uint8).copy()).to(self.device)
+
+    def _upload_tables(self) -> None:
+        t = self.tables
+        self._blocks = self._bytes(t.blocks)
+        self._lookup = torch.as_tensor(t.lookup, dtype=torch.int32, device=self.device)
+        lv = np.zeros(1, _LEVELS)
+        lv["nlev"] = self.nlev
+        lv["nb"][0, : self.nlev] = t.level_nb
+        lv["base"][0, : self.nlev] = t.level_base
+        self._levels = self._bytes(lv)
+        # the refined blocks of each level (coarsened finest level first)
+        level_of = t.blocks["level"]
+        self._children = []
+        for L in range(self.nlev - 1):
+            rows = t.children[level_of[t.children[:, 0]] == L] if len(t.children) else t.children
+            self._children.append((torch.as_tensor(rows.reshape(-1), dtype=torch.int32,
+                                                   device=self.device), len(rows)))
+        self._reflux = [(self._bytes(items) if len(items) else self._blocks, len(items))
+                        for items in t.reflux]
+
+    # -- state ---------------------------------------------------------------------------
+    def set_state(self, density, velocity, pressure, *, vector_potential=(0.0, 0.0, 0.0)) -> None:
+        """Every block from the state functions on its own level, then the covered
+        cells and edges coarsened and the ghosts filled."""
+        state = dict(density=density, velocity=velocity, pressure=pressure,
+                     vector_potential=vector_potential)
+        buf = amr3d_maps.pack_state(self.hierarchy, self.tables, self.gamma, self.mean_field, state)
+        self.load_buffer(buf)
+
+    def load_buffer(self, buf: np.ndarray) -> None:
+        self._buf[0].copy_(torch.from_numpy(np.asarray(buf, np.float64)))
+        stream = _lib.stream_handle(self.device)
+        for L in range(self.nlev - 2, -1, -1):
+            self._coarsen(self._buf[0], L, stream)
+        self._fill(self._buf[0], stream)
+        self._t.zero_()
+        self.steps = 0
+
+    def buffer(self) -> np.ndarray:
+        return self._buf[0].cpu().numpy()
+
+    @property
+    def time(self) -> float:
+        return float(self._t.item())
+
+    @property
+    def cells(self) -> int:
+        """Cells of every block of every level (the zone-updates of one step)."""
+        return self.nblocks * self.hierarchy.block**3
+
+    # -- the step ------------------------------------------------------------------------
+    def _fill(self, buf: torch.Tensor, stream: int) -> None:
+        _lib.call("wb_amr3d_fill", _lib.ptr(buf), _lib.ptr(self._blocks), self.nblocks,
+                  _lib.ptr(self._lookup), _lib.ptr(self._levels), stream)
+
+    def _coarsen(self, buf: torch.Tensor, L: int, stream: int) -> None:
+        rows, n = self._children[L]
+        if n:
+            _lib.call("wb_amr3d_coarsen", _lib.ptr(buf), _lib.ptr(self._blocks), _lib.ptr(rows), n,
+                      _lib.ptr(self._lookup), _lib.ptr(self._levels), stream)
+
+    def _stage(self, src, dst, heun: bool, slot: int, stream: int, marks=None) -> None:
+        self._fill(src, stream)
+        suffix = "" if self.precision == "fp64" else "_mixed"
+        code = LIMITER_CODES.index(self.limiter)
+        if marks is not None:
+            marks[0].record()
+        _lib.call("wb_amr3d_stage" + suffix, _lib.ptr(src), _lib.ptr(dst), _lib.ptr(self._blocks),
+                  self.nblocks, *self.mean_field, self.gamma, code, _lib.ptr(self._dt), int(heun),
+                  self._bad.data_ptr() + 4 * slot, stream)
+        if marks is not None:
+            marks[1].record()
+        for d, (items, n) in enumerate(self._reflux):
+            if n:
+                _lib.call("wb_amr3d_reflux" + suffix, _lib.ptr(src), _lib.ptr(dst),
+                          _lib.ptr(self._blocks), _lib.ptr(items), n, d // 2, d % 2,
+                          *self.mean_field, self.gamma, code, _lib.ptr(self._dt), int(heun), stream)
+        for L in range(self.nlev - 2, -1, -1):
+            self._coarsen(dst, L, stream)
+
+    def _heun(self, stream: int, events=None):
+        marks = None
+        if events is not None:
+            marks = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        self._stage(self._buf[0], self._buf[1], False, 0, stream, marks[:2] if marks else None)
+        self._stage(self._buf[1], self._buf[0], True, 1, stream, marks[2:] if marks else None)
+        if events is not None:
+            events.append((marks[0], marks[1]))
+            events.append((marks[2], marks[3]))
+
+    def _rate_kernel(self, stream: int) -> None:
+        _lib.call("wb_amr3d_rate", _lib.ptr(self._buf[0]), _lib.ptr(self._blocks), self.nblocks,
+                  *self.mean_field, self.gamma, _lib.ptr(self._rate), stream)
+
+    def stable_step(self) -> float:
+        """``cfl`` over the largest fast-wave rate of any block cell (mhd3d.py:674-724)."""
+        self._rate_kernel(_lib.stream_handle(self.device))
+        return self.cfl * (1.0 / float(self._rate[0].item()))
+
+    def step(self, dt: float | None = None) -> float:
+        if dt is None:
+            dt = self.stable_step()
+        dt = float(dt)
+        self._dt.fill_(dt)
+        self._heun(_lib.stream_handle(self.device))
+        self._t.add_(dt)
+        self.steps += 1
+        if self.check_interval and self.steps % self.check_interval == 0:
+            self.check()
+        return dt
+
+    def advance(self, steps: int = 1, events: bool = False):
+        """``steps`` steps at the stable step with dt and the time on the device."""
+        stream = _lib.stream_handle(self.device)
+        marks = [] if events else None
+        for _ in range(steps):
+            self._rate_kernel(stream)
+            _lib.call("wb_stable_dt", _lib.ptr(self._rate), self.cfl, 0.0, _lib.ptr(self._dt),
+                      _lib.ptr(self._t), stream)
+            self._heun(stream, marks)
+            self.steps += 1
+        return marks
+
+    def check(self) -> None:
+        if int(self._bad.cpu().sum()):
+            raise FloatingPointError(f"non-positive density or pressure at t = {self.time}")
+
+    # -- diagnostics ---------------------------------------------------------------------
+    def _covered(self) -> np.ndarray:
+        """Per block: True where a finer block covers it."""
+        t = self.tables
+        cov = np.zeros(self.nblocks, bool)
+        if len(t.children):
+            cov[t.children[:, 0]] = True
+        return cov
+
+    def leaf_totals(self) -> np.ndarray:
+        """Integrals of the 5 conserved fields over the leaf cells."""
+        buf = self.buffer()
+        totals = np.zeros(5)
+        cov = self._covered()
+        h = HALO
+        for k, row in enumerate(self.tables.blocks):
+            if cov[k]:
+                continue
+            cells, _ = block_views(buf, int(row["off"]), self.hierarchy.block)
+            vol = float(row["dx"] * row["dy"] * row["dz"])
+            totals += cells[:, h:-h, h:-h, h:-h].sum(axis=(1, 2, 3)) * vol
+        return totals
+
+    def divergence(self) -> float:
+        """Largest |div B| times the cell size over the cells of every block."""
+        from .mhd3d import face_fields
+
+        buf = self.buffer()
+        largest = 0.0
+        h = HALO
+        for row in self.tables.blocks:
+            _, pots = block_views(buf, int(row["off"]), self.hierarchy.block)
+            spacing = (float(row["dx"]), float(row["dy"]), float(row["dz"]))
+            local = [p[tuple(slice(h, p.shape[a] - h) for a in range(3))] for p in pots]
+            faces = face_fields(local, spacing, self.mean_field)
+            div = sum(np.diff(f, axis=a) / spacing[a] for a, f in enumerate(faces))
+            largest = max(largest, float(np.abs(div).max()) * min(spacing))
+        return largest
+
+    def composite_divergence(self) -> float:
+        """Largest mismatch of a coarse edge of a refined block and the mean of the fine
+        edges along it, relative to the largest fine potential difference: zero after
+        every stage (the flux through a coarse face is the sum of the fine fluxes)."""
+        buf = self.buffer()
+        t = self.tables
+        b = self.hierarchy.block
+        worst, scale = 0.0, 0.0
+        for row in t.children:
+            _, cp = block_views(buf, int(t.blocks[row[0]]["off"]), b)
+            kids = [block_views(buf, int(t.blocks[k]["off"]), b)[1] for k in row[1:]]
+            for c in range(3):
+                ext = [b if a == c else b + 1 for a in range(3)]
+                for q in np.ndindex(*ext):
+                    vals = []
+                    for s in (0, 1):
+                        f = [2 * q[a] + (s if a == c else 0) for a in range(3)]
+                        oct_ = [1 if f[a] >= 2 * b else f[a] // b for a in range(3)]
+                        loc = tuple(f[a] - b * oct_[a] + 2 for a in range(3))
+                        vals.append(kids[(oct_[0] * 2 + oct_[1]) * 2 + oct_[2]][c][loc])
+                    coarse = cp[c][tuple(v + 2 for v in q)]
+                    worst = max(worst, abs(coarse - 0.5 * (vals[0] + vals[1])))
+                    scale = max(scale, abs(vals[0] - vals[1]))
+        return worst / scale if scale else worst

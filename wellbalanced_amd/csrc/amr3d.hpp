@@ -1,0 +1,181 @@
+// The refined 3D hierarchy of equal blocks: device structures, the block-batched
+// stage and the refluxing (templated on the flux arithmetic, instantiated by the
+// FP64 and the mixed translation units).
+//
+// A block holds B^3 cells (B = 16) with a halo of 2 as the uniform 3D state stores
+// a grid: 5 cell planes of 20^3, then A_x (20, 21, 21), A_y (21, 20, 21), A_z
+// (21, 21, 20).  Level L + 1 refines level L by 2; lookup tables give the block at
+// every block coordinate of a level.  The domain is periodic.
+#pragma once
+
+#include "stage3d.hpp"
+
+namespace wb {
+namespace amr3 {
+
+constexpr int B = 16;            // cells per block side
+constexpr int X = B + 4;         // haloed cells per side
+constexpr int XN = X + 1;        // haloed nodes per side
+constexpr int NCELL = X * X * X;  // one cell field of a block
+constexpr int NEDGE = X * XN * XN;  // one potential component of a block
+constexpr int OFF_A = 5 * NCELL;    // first potential double of a block
+
+struct BlockDesc {
+  long long off;  // first double of the block in the buffer
+  int level, bx, by, bz;
+  double dx, dy, dz;
+};
+
+struct Levels {
+  int nlev;
+  int nb[6];    // blocks per axis
+  int base[6];  // first entry of the level in the lookup table
+};
+
+struct RefluxItem {
+  int fine, coarse, ci, cj, ck, pad;
+};
+
+// Offsets inside a block: cell field f at (i, j, k); potential component c at
+// (i, j, k) in its own extents (A_x (X, XN, XN), A_y (XN, X, XN), A_z (XN, XN, X)).
+__device__ __forceinline__ int cell_at(int f, int i, int j, int k) {
+  return f * NCELL + (i * X + j) * X + k;
+}
+__device__ __forceinline__ int edge_at(int c, int i, int j, int k) {
+  if (c == 0) return OFF_A + (i * XN + j) * XN + k;
+  if (c == 1) return OFF_A + NEDGE + (i * X + j) * XN + k;
+  return OFF_A + 2 * NEDGE + (i * XN + j) * X + k;
+}
+
+__device__ __forceinline__ int wrap(int g, int n) {
+  g %= n;
+  return g < 0 ? g + n : g;
+}
+
+// The block of `level` holding level cell/node coordinates (g0, g1, g2) canonically
+// (the block whose interior range [0, B) contains each coordinate), or -1.
+__device__ __forceinline__ int block_of(const int* lookup, const Levels& lv, int level, int g0,
+                                        int g1, int g2) {
+  const int nb = lv.nb[level];
+  return lookup[lv.base[level] + ((g0 / B) * nb + g1 / B) * nb + g2 / B];
+}
+
+__device__ __forceinline__ StageArgs block_args(const StageArgs& base, const BlockDesc& d) {
+  StageArgs a = base;
+  a.in = base.in + d.off;
+  a.out = base.out + d.off;
+  a.dx = d.dx;
+  a.dy = d.dy;
+  a.dz = d.dz;
+  return a;
+}
+
+// One stage of every block: one workgroup per block, a 16 x 16 tile marching over
+// the block's 16 planes.
+template <typename P, int WAVES, int LIM>
+__global__ void __launch_bounds__((Tile<B, B>::THREADS), WAVES)
+k_stage3d_blocks(StageArgs base, const BlockDesc* __restrict__ blocks) {
+  __shared__ StageShared<P, B, B> sh;
+  const BlockDesc d = blocks[blockIdx.x];
+  StageArgs a = base;
+  a.in = base.in + d.off;
+  a.out = base.out + d.off;
+  a.dx = d.dx;
+  a.dy = d.dy;
+  a.dz = d.dz;
+  stage3d_body<P, B, B, LIM>(a, 0, 0, 0, sh);
+}
+
+// The flux through the low `axis` face of haloed cell (i, j, k) of the block state
+// `a.in`, computed as the stage kernel computes it: the four cells along the normal
+// by CellPrim, the frame rows of the sweep, the face field of the potential, then
+// the face solve of the path (rows of the normal frame, as double).
+template <typename P, int LIM>
+__device__ void face_flux3d(const StageArgs& a, int axis, int i, int j, int k, double out[7]) {
+  constexpr bool FAST = sizeof(P) == 4;
+  const Grid3 g = a.g;
+  const EdgePot A{a.in + a.off_ax, a.in + a.off_ay, a.in + a.off_az, g.Y, g.Z};
+  const size_t PL = (size_t)g.X * g.Y * g.Z;
+  const double rdx = 1.0 / a.dx, rdy = 1.0 / a.dy, rdz = 1.0 / a.dz;
+  constexpr int cx[7] = {0, 1, 2, 3, 4, 6, 7};
+  constexpr int cy[7] = {0, 2, 3, 1, 4, 7, 5};
+  constexpr int cz[7] = {0, 3, 1, 2, 4, 5, 6};
+  P q[7][4];
+  for (int s = 0; s < 4; ++s) {
+    const int ii = i + (axis == 0 ? s - 2 : 0), jj = j + (axis == 1 ? s - 2 : 0),
+              kk = k + (axis == 2 ? s - 2 : 0);
+    CellPrim<FAST> c;
+    c.compute(a, A, PL, ii, jj, kk, rdx, rdy, rdz);
+    const P pr[8] = {P(c.r), P(c.u), P(c.v), P(c.w), P(c.p), P(c.bx), P(c.by), P(c.bz)};
+    for (int r = 0; r < 7; ++r) q[r][s] = pr[axis == 0 ? cx[r] : (axis == 1 ? cy[r] : cz[r])];
+  }
+  double bn;
+  if (axis == 0) {
+    bn = FAST ? (A.AZ(i, j + 1, k) - A.AZ(i, j, k)) * rdy - (A.AY(i, j, k + 1) - A.AY(i, j, k)) * rdz + a.b0x
+              : A.bx(i, j, k, a.dy, a.dz, a.b0x);
+  } else if (axis == 1) {
+    bn = FAST ? (A.AX(i, j, k + 1) - A.AX(i, j, k)) * rdz - (A.AZ(i + 1, j, k) - A.AZ(i, j, k)) * rdx + a.b0y
+              : A.by(i, j, k, a.dz, a.dx, a.b0y);
+  } else {
+    bn = FAST ? (A.AY(i + 1, j, k) - A.AY(i, j, k)) * rdx - (A.AX(i, j + 1, k) - A.AX(i, j, k)) * rdy + a.b0z
+              : A.bz(i, j, k, a.dx, a.dy, a.b0z);
+  }
+  P fl[7];
+  face_solve<P, LIM>(q, a.limiter, P(bn), P(a.gamma), P(1.0 / (a.gamma - 1.0)), fl);
+  for (int r = 0; r < 7; ++r) out[r] = double(fl[r]);
+}
+
+// Refluxing of one direction (axis, side) after a stage: per coarse-fine block face,
+// the 8 x 8 coarse cells across it replace the flux of their face on the fine side
+// by the mean of the four fine fluxes, both recomputed from the stage input `src`
+// (heun: half of the correction, as the stage's average).  side 0: the coarse cells
+// at the fine block's low face (their +axis face); side 1: at its high face.  One
+// coarse cell is refluxed at most once per direction, so the six directions run in
+// turn without atomics (deterministic).
+template <typename P, int LIM>
+__global__ void __launch_bounds__(64) k_reflux3d(StageArgs base, const BlockDesc* __restrict__ blocks,
+                                                 const RefluxItem* __restrict__ items, int nitems,
+                                                 int axis, int side, double* __restrict__ dst) {
+  const int it = blockIdx.x;
+  if (it >= nitems) return;
+  const RefluxItem item = items[it];
+  const BlockDesc fd = blocks[item.fine], cd = blocks[item.coarse];
+  const StageArgs fa = block_args(base, fd), ca = block_args(base, cd);
+  const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+  const int u = threadIdx.x / 8, v = threadIdx.x % 8;
+  // the coarse cell (haloed local) and the face of it on the fine side
+  int c[3] = {item.ci + 2, item.cj + 2, item.ck + 2};
+  c[a1] += u;
+  c[a2] += v;
+  int cf[3] = {c[0], c[1], c[2]};
+  if (side == 0) cf[axis] += 1;  // its +axis face is the low face of the next cell
+  double fc[7];
+  face_flux3d<P, LIM>(ca, axis, cf[0], cf[1], cf[2], fc);
+  // the four fine faces: the fine block's low face (side 0) or high face (side 1)
+  double fm[7] = {0, 0, 0, 0, 0, 0, 0};
+  double ff[2][2][7];
+  for (int p = 0; p < 2; ++p)
+    for (int q = 0; q < 2; ++q) {
+      int f[3];
+      f[axis] = side == 0 ? 2 : 2 + B;
+      f[a1] = 2 + 2 * u + p;
+      f[a2] = 2 + 2 * v + q;
+      face_flux3d<P, LIM>(fa, axis, f[0], f[1], f[2], ff[p][q]);
+    }
+  for (int r = 0; r < 7; ++r) fm[r] = ((ff[0][0][r] + ff[0][1][r]) + (ff[1][0][r] + ff[1][1][r])) * 0.25;
+  const double dt = *base.dt;
+  const double dtd = axis == 0 ? dt / cd.dx : (axis == 1 ? dt / cd.dy : dt / cd.dz);
+  constexpr int xrow[5] = {0, 1, 2, 3, 4};
+  constexpr int yrow[5] = {0, 3, 1, 2, 4};
+  constexpr int zrow[5] = {0, 2, 3, 1, 4};
+  double* out = dst + cd.off;
+  for (int fld = 0; fld < 5; ++fld) {
+    const int row = axis == 0 ? xrow[fld] : (axis == 1 ? yrow[fld] : zrow[fld]);
+    const double d = side == 0 ? dtd * (fc[row] - fm[row]) : dtd * (fm[row] - fc[row]);
+    const int o = cell_at(fld, c[0], c[1], c[2]);
+    out[o] = out[o] + (base.heun ? 0.5 * d : d);
+  }
+}
+
+}  // namespace amr3
+}  // namespace wb

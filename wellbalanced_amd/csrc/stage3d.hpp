@@ -130,32 +130,17 @@ struct CellPrim {
   }
 };
 
-// LIM: the slope limiter fixed at compile time (the mixed path's branch-free
-// FP32 flux), or -1 for the parity path's runtime limiter.
-//
-// Workgroup order: the hardware hands workgroup w to XCD w % 8, each XCD with its
-// own L2.  The workgroups are renumbered so that XCD x runs a contiguous run of
-// tiles (z fastest, then y, then the x segment): tiles running together on one
-// XCD are neighbours, and the halo lines they share are fetched into that L2 once.
-template <typename P, int TY, int TZ, int WAVES, int LIM>
-__global__ void __launch_bounds__((Tile<TY, TZ>::THREADS), WAVES)
-k_stage3d(StageArgs a) {
+// One stage of the tile (ty_, tz_) and x segment seg_ of the grid a.g (the tiles
+// and segments of a launch together cover the grid).  LIM: the slope limiter fixed
+// at compile time (the mixed path's branch-free FP32 flux), or -1 for the parity
+// path's runtime limiter.
+template <typename P, int TY, int TZ, int LIM>
+__device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz_, int seg_,
+                                             StageShared<P, TY, TZ>& sh) {
   using T = Tile<TY, TZ>;
   constexpr bool FAST = sizeof(P) == 4;
-  __shared__ StageShared<P, TY, TZ> sh;
   const Grid3 g = a.g;
   const int tid = threadIdx.x;
-  const int tiles_z = (g.nz + TZ - 1) / TZ, tiles_y = (g.ny + TY - 1) / TY;
-  int tz_, ty_, seg_;
-  {
-    const int n = gridDim.x, xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
-    const int per = n >> 3, rem = n & 7;
-    const int logical = xcd * per + min(xcd, rem) + k;
-    tz_ = logical % tiles_z;
-    const int rest = logical / tiles_z;
-    ty_ = rest % tiles_y;
-    seg_ = rest / tiles_y;
-  }
   const int jy0 = 2 + ty_ * TY, kz0 = 2 + tz_ * TZ;       // first interior cell of the tile
   const int ix0 = 2 + seg_ * a.nxs;                       // first plane of the segment
   const int tyv = min(TY, g.ny + 2 - jy0), tzv = min(TZ, g.nz + 2 - kz0);
@@ -380,6 +365,26 @@ k_stage3d(StageArgs a) {
     __syncthreads();
   }
 }
+
+// The stage over a uniform grid, one workgroup per (tile, segment).  Workgroup
+// order: the hardware hands workgroup w to XCD w % 8, each XCD with its own L2.
+// The workgroups are renumbered so that XCD x runs a contiguous run of tiles (z
+// fastest, then y, then the x segment): tiles running together on one XCD are
+// neighbours, and the halo lines they share are fetched into that L2 once.
+template <typename P, int TY, int TZ, int WAVES, int LIM>
+__global__ void __launch_bounds__((Tile<TY, TZ>::THREADS), WAVES)
+k_stage3d(StageArgs a) {
+  __shared__ StageShared<P, TY, TZ> sh;
+  const Grid3 g = a.g;
+  const int tiles_z = (g.nz + TZ - 1) / TZ, tiles_y = (g.ny + TY - 1) / TY;
+  const int n = gridDim.x, xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
+  const int per = n >> 3, rem = n & 7;
+  const int logical = xcd * per + min(xcd, rem) + k;
+  const int tz = logical % tiles_z;
+  const int rest = logical / tiles_z;
+  stage3d_body<P, TY, TZ, LIM>(a, rest % tiles_y, tz, rest / tiles_y, sh);
+}
+
 
 // WAVES: the waves per SIMD the register allocation must allow (2 or 3 for the
 // mixed path: 3 trades a few spills for occupancy; the autotuner picks).

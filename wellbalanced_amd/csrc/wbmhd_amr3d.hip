@@ -1,0 +1,344 @@
+// The refined 3D hierarchy: ghost fill, average-down, restriction of the potential,
+// the CFL rate over every block, and the FP64 stage and refluxing (the parity path,
+// built with -ffp-contract=off).
+#include "amr3d.hpp"
+
+using namespace wb;
+using namespace wb::amr3;
+
+namespace {
+
+// The block of `level` whose copy of edge (component c) at level coordinates g is
+// the master: the block holding it in its own range [0, B) on every axis first, then
+// the blocks holding it on their upper boundary (local index B on a node axis where
+// g is a multiple of B), in a fixed order; -1 if the level has none.  *loc gets the
+// master's local (haloed) coordinates.
+__device__ int edge_master(const int* lookup, const Levels& lv, int level, int c, const int g[3],
+                           int loc[3]) {
+  const int nb = lv.nb[level];
+  for (int s = 0; s < 8; ++s) {
+    int blk[3];
+    bool ok = true;
+    for (int a = 0; a < 3; ++a) {
+      const int sa = (s >> (2 - a)) & 1;
+      if (sa && (a == c || g[a] % B != 0)) ok = false;
+      blk[a] = sa ? wrap(g[a] / B - 1, nb) : g[a] / B;
+      loc[a] = sa ? B + 2 : g[a] % B + 2;
+    }
+    if (!ok) continue;
+    const int b = lookup[lv.base[level] + (blk[0] * nb + blk[1]) * nb + blk[2]];
+    if (b >= 0) return b;
+  }
+  return -1;
+}
+
+// Ghosts of every block (one workgroup per block).  A ghost cell takes the value of
+// the block of the same level holding it; an edge outside the block's own range
+// [2, 18) takes the value of its master copy on the same level (a block keeps the
+// edges it is the master of).  Without a block of the same level, cells are
+// interpolated from the level below trilinearly (weights 1/4, 3/4 per axis) and edges
+// per axis by the same weights along the cell axis and, across, the coarse node
+// (even fine nodes) or the 4-point rule (-1, 9, 9, -1)/16 (odd).  Only values a fill
+// never writes are read, so one launch fills every level.
+__global__ void __launch_bounds__(256) k_fill3d(double* __restrict__ buf,
+                                                const BlockDesc* __restrict__ blocks,
+                                                const int* __restrict__ lookup, Levels lv) {
+  const BlockDesc d = blocks[blockIdx.x];
+  const int L = d.level;
+  const int n = lv.nb[L] * B;
+  const int bo[3] = {d.bx * B, d.by * B, d.bz * B};
+  double* me = buf + d.off;
+  // cells
+  for (int t = threadIdx.x; t < NCELL; t += blockDim.x) {
+    const int p[3] = {t / (X * X), (t / X) % X, t % X};
+    if (p[0] >= 2 && p[0] < 2 + B && p[1] >= 2 && p[1] < 2 + B && p[2] >= 2 && p[2] < 2 + B)
+      continue;
+    int g[3];
+    for (int a = 0; a < 3; ++a) g[a] = wrap(bo[a] + p[a] - 2, n);
+    const int owner = block_of(lookup, lv, L, g[0], g[1], g[2]);
+    if (owner >= 0) {
+      const double* src = buf + blocks[owner].off;
+      const int o = cell_at(0, g[0] % B + 2, g[1] % B + 2, g[2] % B + 2);
+      const int m = cell_at(0, p[0], p[1], p[2]);
+      for (int f = 0; f < 5; ++f) me[m + f * NCELL] = src[o + f * NCELL];
+      continue;
+    }
+    const int nc = lv.nb[L - 1] * B;
+    int c0[3];
+    double w[3][2];
+    for (int a = 0; a < 3; ++a) {
+      const int m = g[a] >> 1;
+      const double tt = (g[a] & 1) ? 0.25 : 0.75;
+      c0[a] = (g[a] & 1) ? m : m - 1;
+      w[a][0] = 1.0 - tt;
+      w[a][1] = tt;
+    }
+    for (int f = 0; f < 5; ++f) {
+      double acc = 0.0;
+      for (int x = 0; x < 2; ++x)
+        for (int y = 0; y < 2; ++y)
+          for (int z = 0; z < 2; ++z) {
+            const int G0 = wrap(c0[0] + x, nc), G1 = wrap(c0[1] + y, nc), G2 = wrap(c0[2] + z, nc);
+            const int cb = block_of(lookup, lv, L - 1, G0, G1, G2);
+            if (cb < 0) continue;  // outside the level below: excluded by the nesting rule
+            const double v = buf[blocks[cb].off + cell_at(f, G0 % B + 2, G1 % B + 2, G2 % B + 2)];
+            acc = acc + ((w[0][x] * w[1][y]) * w[2][z]) * v;
+          }
+      me[cell_at(f, p[0], p[1], p[2])] = acc;
+    }
+  }
+  // edges of component c: axis c is the cell axis, the others node axes
+  for (int c = 0; c < 3; ++c) {
+    const int e0 = c == 0 ? X : XN, e1 = c == 1 ? X : XN, e2 = c == 2 ? X : XN;
+    for (int t = threadIdx.x; t < e0 * e1 * e2; t += blockDim.x) {
+      const int p[3] = {t / (e1 * e2), (t / e2) % e1, t % e2};
+      if (p[0] >= 2 && p[0] < 2 + B && p[1] >= 2 && p[1] < 2 + B && p[2] >= 2 && p[2] < 2 + B)
+        continue;
+      int g[3], loc[3];
+      for (int a = 0; a < 3; ++a) g[a] = wrap(bo[a] + p[a] - 2, n);
+      const int master = edge_master(lookup, lv, L, c, g, loc);
+      if (master >= 0) {
+        if (master == (int)blockIdx.x && loc[0] == p[0] && loc[1] == p[1] && loc[2] == p[2])
+          continue;  // this block's copy is the master
+        me[edge_at(c, p[0], p[1], p[2])] = buf[blocks[master].off + edge_at(c, loc[0], loc[1], loc[2])];
+        continue;
+      }
+      const int nc = lv.nb[L - 1] * B;
+      int c0[3], nt[3];
+      double w[3][4];
+      for (int a = 0; a < 3; ++a) {
+        const int m = g[a] >> 1;
+        if (a == c) {  // cell axis
+          const double tt = (g[a] & 1) ? 0.25 : 0.75;
+          c0[a] = (g[a] & 1) ? m : m - 1;
+          nt[a] = 2;
+          w[a][0] = 1.0 - tt;
+          w[a][1] = tt;
+        } else if ((g[a] & 1) == 0) {  // a coarse node
+          c0[a] = m;
+          nt[a] = 1;
+          w[a][0] = 1.0;
+        } else {  // the midpoint of coarse nodes m and m + 1
+          c0[a] = m - 1;
+          nt[a] = 4;
+          w[a][0] = -0.0625;
+          w[a][1] = 0.5625;
+          w[a][2] = 0.5625;
+          w[a][3] = -0.0625;
+        }
+      }
+      double acc = 0.0;
+      for (int x = 0; x < nt[0]; ++x)
+        for (int y = 0; y < nt[1]; ++y)
+          for (int z = 0; z < nt[2]; ++z) {
+            const int G[3] = {wrap(c0[0] + x, nc), wrap(c0[1] + y, nc), wrap(c0[2] + z, nc)};
+            int cl[3];
+            const int cb = edge_master(lookup, lv, L - 1, c, G, cl);
+            if (cb < 0) continue;  // outside the level below: excluded by the nesting rule
+            const double v = buf[blocks[cb].off + edge_at(c, cl[0], cl[1], cl[2])];
+            acc = acc + ((w[0][x] * w[1][y]) * w[2][z]) * v;
+          }
+      me[edge_at(c, p[0], p[1], p[2])] = acc;
+    }
+  }
+}
+
+// Covered cells of a refined block := the mean of their 8 children (one workgroup
+// per refined block; children[9]: the block, then its children with x slowest).
+__global__ void __launch_bounds__(256) k_average3d(double* __restrict__ buf,
+                                                   const BlockDesc* __restrict__ blocks,
+                                                   const int* __restrict__ children) {
+  const int* row = children + 9 * blockIdx.x;
+  double* parent = buf + blocks[row[0]].off;
+  for (int t = threadIdx.x; t < B * B * B; t += blockDim.x) {
+    const int I = t / (B * B), J = (t / B) % B, K = t % B;
+    const int oct = ((I / 8) * 2 + J / 8) * 2 + K / 8;
+    const double* kid = buf + blocks[row[1 + oct]].off;
+    const int fi = 2 * (I % 8) + 2, fj = 2 * (J % 8) + 2, fk = 2 * (K % 8) + 2;
+    for (int f = 0; f < 5; ++f) {
+      const double s0 = (kid[cell_at(f, fi, fj, fk)] + kid[cell_at(f, fi, fj, fk + 1)]) +
+                        (kid[cell_at(f, fi, fj + 1, fk)] + kid[cell_at(f, fi, fj + 1, fk + 1)]);
+      const double s1 = (kid[cell_at(f, fi + 1, fj, fk)] + kid[cell_at(f, fi + 1, fj, fk + 1)]) +
+                        (kid[cell_at(f, fi + 1, fj + 1, fk)] + kid[cell_at(f, fi + 1, fj + 1, fk + 1)]);
+      parent[cell_at(f, I + 2, J + 2, K + 2)] = (s0 + s1) * 0.125;
+    }
+  }
+}
+
+// Coarse edges of a refined block := the mean of the two fine edges along them
+// (mhd3d.py:31-36), written to every block of the coarse level holding the edge.
+__global__ void __launch_bounds__(256) k_restrict3d(double* __restrict__ buf,
+                                                    const BlockDesc* __restrict__ blocks,
+                                                    const int* __restrict__ children,
+                                                    const int* __restrict__ lookup, Levels lv) {
+  const int* row = children + 9 * blockIdx.x;
+  const BlockDesc pd = blocks[row[0]];
+  const int L = pd.level;
+  const int n = lv.nb[L] * B;
+  const int bo[3] = {pd.bx * B, pd.by * B, pd.bz * B};
+  for (int c = 0; c < 3; ++c) {
+    const int e0 = c == 0 ? B : B + 1, e1 = c == 1 ? B : B + 1, e2 = c == 2 ? B : B + 1;
+    for (int t = threadIdx.x; t < e0 * e1 * e2; t += blockDim.x) {
+      const int q[3] = {t / (e1 * e2), (t / e2) % e1, t % e2};  // 0..B (nodes) / B-1 (cells)
+      // the two fine edges: fine coordinates within the block's footprint (0..2B)
+      double v[2];
+      for (int s = 0; s < 2; ++s) {
+        int oct[3], loc[3];
+        for (int a = 0; a < 3; ++a) {
+          const int fcoord = 2 * q[a] + (a == c ? s : 0);
+          oct[a] = fcoord >= 2 * B ? 1 : fcoord / B;
+          loc[a] = fcoord - B * oct[a] + 2;
+        }
+        const double* kid = buf + blocks[row[1 + (oct[0] * 2 + oct[1]) * 2 + oct[2]]].off;
+        v[s] = kid[edge_at(c, loc[0], loc[1], loc[2])];
+      }
+      const double value = 0.5 * (v[0] + v[1]);
+      // every coarse block holding edge (bo + q): each node coordinate on a block
+      // boundary is held with local index 2 (the block above) and 2 + B (below)
+      int g[3];
+      for (int a = 0; a < 3; ++a) g[a] = wrap(bo[a] + q[a], n);
+      for (int s0 = 0; s0 < 2; ++s0)
+        for (int s1 = 0; s1 < 2; ++s1)
+          for (int s2 = 0; s2 < 2; ++s2) {
+            const int sh[3] = {s0, s1, s2};
+            int blk[3], loc[3];
+            bool ok = true;
+            for (int a = 0; a < 3; ++a) {
+              if (sh[a] == 0) {
+                blk[a] = g[a] / B;
+                loc[a] = g[a] % B + 2;
+              } else {  // the copy on the upper boundary of the block below
+                ok = ok && a != c && g[a] % B == 0;
+                blk[a] = wrap(g[a] / B - 1, lv.nb[L]);
+                loc[a] = B + 2;
+              }
+            }
+            if (!ok) continue;
+            const int b = lookup[lv.base[L] + (blk[0] * lv.nb[L] + blk[1]) * lv.nb[L] + blk[2]];
+            if (b >= 0) buf[blocks[b].off + edge_at(c, loc[0], loc[1], loc[2])] = value;
+          }
+    }
+  }
+}
+
+// The CFL rate (mhd3d.py:674-724) over the cells of every block: a fixed grid walks
+// chunks of 256 cells (16 per block), one atomic pair per workgroup.
+__global__ void __launch_bounds__(256) k_rate3d_blocks(const double* __restrict__ buf,
+                                                       const BlockDesc* __restrict__ blocks,
+                                                       int nblocks, double b0x, double b0y,
+                                                       double b0z, double gamma,
+                                                       unsigned long long* keys) {
+  double rate = 0.0, rmin = __builtin_inf();
+  const int nchunks = nblocks * (B * B * B / 256);
+  for (int ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    const BlockDesc d = blocks[ch / (B * B * B / 256)];
+    const int t = (ch % (B * B * B / 256)) * 256 + (int)threadIdx.x;
+    const int i = t / (B * B) + 2, j = (t / B) % B + 2, k = t % B + 2;
+    const double* in = buf + d.off;
+    const EdgePot A{in + OFF_A, in + OFF_A + NEDGE, in + OFF_A + 2 * NEDGE, X, X};
+    const double dx = d.dx, dy = d.dy, dz = d.dz;
+    const double bx = 0.5 * (A.bx(i, j, k, dy, dz, b0x) + A.bx(i + 1, j, k, dy, dz, b0x));
+    const double by = 0.5 * (A.by(i, j, k, dz, dx, b0y) + A.by(i, j + 1, k, dz, dx, b0y));
+    const double bz = 0.5 * (A.bz(i, j, k, dx, dy, b0z) + A.bz(i, j, k + 1, dx, dy, b0z));
+    const int c = cell_at(0, i, j, k);
+    const double r = in[c];
+    const double u = in[c + NCELL] / r, v = in[c + 2 * NCELL] / r, w = in[c + 3 * NCELL] / r;
+    const double p = (gamma - 1.0) * (in[c + 4 * NCELL] - 0.5 * r * (u * u + v * v + w * w) -
+                                      0.5 * (bx * bx + by * by + bz * bz));
+    const double rc = (__builtin_fabs(u) + fast_speed_rate(r, p, bx, by, bz, gamma)) / dx +
+                      (__builtin_fabs(v) + fast_speed_rate(r, p, by, bz, bx, gamma)) / dy +
+                      (__builtin_fabs(w) + fast_speed_rate(r, p, bz, bx, by, gamma)) / dz;
+    if (rc > rate) rate = rc;
+    if (r < rmin) rmin = r;
+  }
+  block_reduce_rate_rho_once(rate, rmin, keys);
+}
+
+bool levels_ok(const Levels& lv) { return lv.nlev >= 1 && lv.nlev <= 6; }
+
+}  // namespace
+
+extern "C" {
+
+// Hidden: the stage arguments of a block (shared with the mixed unit).
+__attribute__((visibility("hidden"))) void wb_amr3d_base_args(
+    void* args, const double* in, double* out, double b0x, double b0y, double b0z, double gamma,
+    int limiter, const double* dt, int heun, int32_t* bad) {
+  StageArgs& a = *(StageArgs*)args;
+  a.in = in;
+  a.out = out;
+  a.dt = dt;
+  a.bad = (int*)bad;
+  a.g = Grid3{X, X, X, B, B, B};
+  a.dx = a.dy = a.dz = 1.0;
+  a.b0x = b0x;
+  a.b0y = b0y;
+  a.b0z = b0z;
+  a.gamma = gamma;
+  a.limiter = limiter;
+  a.heun = heun;
+  a.nxs = B;
+  a.off_ax = OFF_A;
+  a.off_ay = OFF_A + NEDGE;
+  a.off_az = OFF_A + 2 * NEDGE;
+}
+
+int wb_amr3d_fill(double* buf, const void* blocks, int nblocks, const int32_t* lookup,
+                  const void* levels, void* stream) {
+  const Levels lv = *(const Levels*)levels;
+  if (nblocks < 1 || !levels_ok(lv)) return WB_EINVAL;
+  hipLaunchKernelGGL(k_fill3d, dim3(nblocks), dim3(256), 0, (hipStream_t)stream, buf,
+                     (const BlockDesc*)blocks, (const int*)lookup, lv);
+  return wb_last_error();
+}
+
+int wb_amr3d_coarsen(double* buf, const void* blocks, const int32_t* children, int nrefined,
+                     const int32_t* lookup, const void* levels, void* stream) {
+  const Levels lv = *(const Levels*)levels;
+  if (nrefined < 0 || !levels_ok(lv)) return WB_EINVAL;
+  if (nrefined == 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_average3d, dim3(nrefined), dim3(256), 0, s, buf, (const BlockDesc*)blocks,
+                     (const int*)children);
+  hipLaunchKernelGGL(k_restrict3d, dim3(nrefined), dim3(256), 0, s, buf, (const BlockDesc*)blocks,
+                     (const int*)children, (const int*)lookup, lv);
+  return wb_last_error();
+}
+
+int wb_amr3d_rate(const double* buf, const void* blocks, int nblocks, double b0x, double b0y,
+                  double b0z, double gamma, double* out, void* stream) {
+  if (nblocks < 1) return WB_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  unsigned long long* keys = (unsigned long long*)out;
+  hipLaunchKernelGGL(k_rate_init, dim3(1), dim3(64), 0, s, keys);
+  const int nchunks = nblocks * (B * B * B / 256);
+  hipLaunchKernelGGL(k_rate3d_blocks, dim3((unsigned)(nchunks < 1024 ? nchunks : 1024)), dim3(256),
+                     0, s, buf, (const BlockDesc*)blocks, nblocks, b0x, b0y, b0z, gamma, keys);
+  hipLaunchKernelGGL(k_rate_finish, dim3(1), dim3(64), 0, s, keys, out);
+  return wb_last_error();
+}
+
+int wb_amr3d_stage(const double* in, double* out, const void* blocks, int nblocks, double b0x,
+                   double b0y, double b0z, double gamma, int limiter, const double* dt, int heun,
+                   int32_t* bad, void* stream) {
+  if (nblocks < 1 || limiter < 0 || limiter > 3) return WB_EINVAL;
+  StageArgs a;
+  wb_amr3d_base_args(&a, in, out, b0x, b0y, b0z, gamma, limiter, dt, heun, bad);
+  hipLaunchKernelGGL((k_stage3d_blocks<double, 1, -1>), dim3(nblocks), dim3(Tile<B, B>::THREADS),
+                     0, (hipStream_t)stream, a, (const BlockDesc*)blocks);
+  return wb_last_error();
+}
+
+int wb_amr3d_reflux(const double* src, double* dst, const void* blocks, const void* items,
+                    int nitems, int axis, int side, double b0x, double b0y, double b0z,
+                    double gamma, int limiter, const double* dt, int heun, void* stream) {
+  if (nitems < 0 || axis < 0 || axis > 2 || side < 0 || side > 1) return WB_EINVAL;
+  if (nitems == 0) return 0;
+  StageArgs a;
+  wb_amr3d_base_args(&a, src, dst, b0x, b0y, b0z, gamma, limiter, dt, heun, nullptr);
+  hipLaunchKernelGGL((k_reflux3d<double, -1>), dim3(nitems), dim3(64), 0, (hipStream_t)stream, a,
+                     (const BlockDesc*)blocks, (const RefluxItem*)items, nitems, axis, side, dst);
+  return wb_last_error();
+}
+
+}  // extern "C"

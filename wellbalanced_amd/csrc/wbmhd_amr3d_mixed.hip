@@ -1,0 +1,45 @@
+// The refined 3D hierarchy's stage and refluxing with the FP32 reconstruction and
+// HLLD (the mixed path, built with -ffp-contract=fast).
+#include "amr3d.hpp"
+
+using namespace wb;
+using namespace wb::amr3;
+
+extern "C" __attribute__((visibility("hidden"))) void wb_amr3d_base_args(
+    void* args, const double* in, double* out, double b0x, double b0y, double b0z, double gamma,
+    int limiter, const double* dt, int heun, int32_t* bad);
+
+extern "C" {
+
+int wb_amr3d_stage_mixed(const double* in, double* out, const void* blocks, int nblocks,
+                         double b0x, double b0y, double b0z, double gamma, int limiter,
+                         const double* dt, int heun, int32_t* bad, void* stream) {
+  if (nblocks < 1) return WB_EINVAL;
+  StageArgs a;
+  wb_amr3d_base_args(&a, in, out, b0x, b0y, b0z, gamma, limiter, dt, heun, bad);
+  hipStream_t s = (hipStream_t)stream;
+  return with_limiter(limiter, [&](auto lim) {
+    constexpr int L = decltype(lim)::value;
+    hipLaunchKernelGGL((k_stage3d_blocks<float, 3, L>), dim3(nblocks), dim3(Tile<B, B>::THREADS), 0,
+                       s, a, (const BlockDesc*)blocks);
+    return wb_last_error();
+  });
+}
+
+int wb_amr3d_reflux_mixed(const double* src, double* dst, const void* blocks, const void* items,
+                          int nitems, int axis, int side, double b0x, double b0y, double b0z,
+                          double gamma, int limiter, const double* dt, int heun, void* stream) {
+  if (nitems < 0 || axis < 0 || axis > 2 || side < 0 || side > 1) return WB_EINVAL;
+  if (nitems == 0) return 0;
+  StageArgs a;
+  wb_amr3d_base_args(&a, src, dst, b0x, b0y, b0z, gamma, limiter, dt, heun, nullptr);
+  hipStream_t s = (hipStream_t)stream;
+  return with_limiter(limiter, [&](auto lim) {
+    constexpr int L = decltype(lim)::value;
+    hipLaunchKernelGGL((k_reflux3d<float, L>), dim3(nitems), dim3(64), 0, s, a,
+                       (const BlockDesc*)blocks, (const RefluxItem*)items, nitems, axis, side, dst);
+    return wb_last_error();
+  });
+}
+
+}  // extern "C"
