@@ -64,7 +64,7 @@ def parse():
     p.add_argument("--calibrate", action="store_true",
                    help="also run one device copy of 1 GiB (the traffic counters' reference)")
     p.add_argument("--workload", default="turbulence3d",
-                   choices=("turbulence3d", "ot2d", "blast2d-amr", "harris2d"))
+                   choices=("turbulence3d", "ot2d", "blast2d-amr", "harris2d", "blast3d-amr"))
     return p.parse_args()
 
 
@@ -113,7 +113,7 @@ def main():
                           "(the multi-GPU sharding is a design in DESIGN.md, not built)"}))
         sys.exit(2)
     runs = {"turbulence3d": run_turbulence3d, "ot2d": run_ot2d, "blast2d-amr": run_blast2d_amr,
-            "harris2d": run_harris2d}
+            "harris2d": run_harris2d, "blast3d-amr": run_blast3d_amr}
     print(json.dumps(runs[args.workload](args)), flush=True)
 
 
@@ -411,6 +411,121 @@ def run_blast2d_amr(args):
                  {"launch": {"fp64": res["fp64"][3], "mixed": launch}, "patches": len(m.layouts),
                   "leaf_cells": leaf, "divergence": m.divergence(),
                   "composite_divergence": m.composite_divergence()})
+
+
+
+def _footprint(h, level: int = 1):
+    """Level-0 cells covered by the blocks of ``level``."""
+    import numpy as np
+
+    n0 = h.level_shape(0)[0]
+    mask = np.zeros((n0,) * 3, bool)
+    side = h.block // 2**level
+    for b in h.levels[level] if len(h.levels) > level else ():
+        sl = tuple(slice(v * side, (v + 1) * side) for v in b)
+        mask[sl] = True
+    return mask
+
+
+def _hierarchy_from_flags(n0, flags1, flags2):
+    """Level 1 over the base blocks holding a flagged cell, level 2 over the level-1
+    blocks holding a strongly flagged base cell (nesting added by the hierarchy)."""
+    import numpy as np
+
+    from wellbalanced_amd.amr3d_maps import Hierarchy3D
+
+    h = Hierarchy3D(n0)
+    h.refine_flagged(0, flags1)
+    if len(h.levels) > 1 and flags2.any():
+        cells = np.argwhere(flags2)
+        parents = {tuple(int(v) for v in c // 8) for c in cells}
+        parents = [p for p in parents if p in h.levels[1]]
+        if parents:
+            h.refine(1, parents)
+    return h
+
+
+def run_blast3d_amr(args):
+    """The learned-indicator refined step: a blast in Alfvenic turbulence on a 128^3
+    base with 16^3 blocks and 3 levels, refined by the CNN indicator (trained on the
+    GPU on the Loehner flags 8 steps ahead) and, for comparison, by the plain Loehner
+    flags; the hierarchies start from the uniform run at step 12."""
+    import numpy as np
+    import torch
+
+    from wellbalanced_amd import MHD3D, cnn, indicator, initial
+    from wellbalanced_amd.amr3d import AMRMHD3D
+
+    n0, every, lead, start_step = 128, 4, 8, 12
+    problem = initial.blast_turbulence_3d(n0, seed=0)
+    # train the indicator on a uniform run: features at steps 4..40, flags 8 steps later
+    uni = MHD3D.from_problem(problem, kernels="fused", precision="mixed")
+    uni.check_interval = 0
+    net = cnn.Indicator(uni.n, lr=3e-3, seed=0)
+    uni.advance(every)
+    net.fit_normalisation(uni)
+    samples = indicator.lookahead_samples(uni, net, snapshots=10, every=every, lead=lead)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = net.fit([(x, (later > 0.8).float().reshape(-1).contiguous())
+                      for _, x, _, later in samples], 150)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    del uni, samples
+    # the start state and the flags each indicator raises there; the truth 8 steps on
+    start = MHD3D.from_problem(problem, kernels="fused", precision="fp64")
+    start.check_interval = 0
+    start.advance(start_step)
+    est = indicator.lohner(start)
+    logit = net.predict(start)
+    later = MHD3D.from_problem(problem, kernels="fused", precision="fp64")
+    later.check_interval = 0
+    later.advance(start_step + lead)
+    truth = (indicator.lohner(later) > indicator.CUTOFF).cpu().numpy()
+    del later
+    est_np, logit_np = est.cpu().numpy(), logit.cpu().numpy()
+    hierarchies = {
+        "learned": _hierarchy_from_flags(n0, logit_np > 0.0, logit_np > 3.0),
+        "plain": _hierarchy_from_flags(n0, est_np > indicator.CUTOFF, est_np > 0.95),
+    }
+    coverage = {k: [float(_footprint(h, L)[truth].mean()) for L in (1, 2)]
+                for k, h in hierarchies.items()}
+    # the refined step on the learned hierarchy, mixed and FP64
+    h = hierarchies["learned"]
+    res, models = {}, {}
+    from wellbalanced_amd import autotune
+
+    launch = {}
+    for precision in ("fp64", "mixed"):
+        m = AMRMHD3D.from_uniform(start, h, precision=precision)
+        m.check_interval = 0
+        launch[precision] = autotune.tune(m, steps=2) if not args.no_tune else {}
+        _, stage_ms = time_model(m, 2, 1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        m.advance(args.steps)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        m.check()
+        res[precision] = (m.cells * args.steps / sec, sec, stage_ms)
+        models[precision] = m
+    a, b = models["fp64"].level_state(0)[0][0], models["mixed"].level_state(0)[0][0]
+    d = np.abs(a - b)[2:-2, 2:-2, 2:-2]
+    m = models["mixed"]
+    rate, sec, stage_ms = res["mixed"]
+    return _line(rate, res["fp64"][0], args.steps, args, sec,
+                 "3D blast in Alfvenic turbulence, 128^3 base, 3 levels of 16^3 blocks refined "
+                 "by the learned indicator, minmod, CFL 0.3, SSP-RK2", m.cells, "mixed",
+                 float(d.mean()), float(d.max()), stage_ms, STATE_BYTES_3D,
+                 {"blocks_per_level": [len(s) for s in h.levels], "launch": launch,
+                  "leaf_cells": h.leaf_cells(), "divergence": m.divergence(),
+                  "indicator": {"train_seconds": train_s, "loss_first": losses[0],
+                                "loss_last": losses[-1],
+                                "flags_8_steps_on_covered_by_levels_1_2": coverage,
+                                "flagged_now": {"learned": int((logit_np > 0).sum()),
+                                                "plain": int((est_np > indicator.CUTOFF).sum())},
+                                "blocks_plain": [len(s) for s in hierarchies["plain"].levels]},
+                  "l1_note": "level-0 (averaged-down) density, mixed vs FP64"})
 
 
 if __name__ == "__main__":

@@ -313,6 +313,12 @@ int wb_amr2d_reflux_mixed(const double* src, double* dst, const void* patches,
 int wb_amr3d_fill(double* buf, const void* blocks, int nblocks, const int32_t* lookup,
                   const void* levels, void* stream);
 
+/* New blocks (ids, all of one level >= 1): every cell and edge, interior included, by
+ * the fill's interpolation from the level below (regrid: mhd3d.py regrid's
+ * interpolation of the new fine patches). */
+int wb_amr3d_prolong(double* buf, const void* blocks, const int32_t* ids, int nids,
+                     const int32_t* lookup, const void* levels, void* stream);
+
 /* One level's coarsening: the covered cells of each refined block := the mean of its
  * 8 children, and its edges := the mean of the two fine edges along them (every coarse
  * copy); children: rows of 9 int32 (the block, then its 8 children, x slowest). */
@@ -323,13 +329,14 @@ int wb_amr3d_coarsen(double* buf, const void* blocks, const int32_t* children, i
 int wb_amr3d_rate(const double* buf, const void* blocks, int nblocks, double b0x, double b0y,
                   double b0z, double gamma, double* out, void* stream);
 
-/* One stage of every block (FP64 parity path / FP32 flux). */
+/* One stage of every block (FP64 parity path / FP32 flux; waves: 2 or 3 per SIMD,
+ * the register budget of the mixed kernel). */
 int wb_amr3d_stage(const double* in, double* out, const void* blocks, int nblocks, double b0x,
                    double b0y, double b0z, double gamma, int limiter, const double* dt, int heun,
                    int32_t* bad, void* stream);
 int wb_amr3d_stage_mixed(const double* in, double* out, const void* blocks, int nblocks,
                          double b0x, double b0y, double b0z, double gamma, int limiter,
-                         const double* dt, int heun, int32_t* bad, void* stream);
+                         const double* dt, int heun, int32_t* bad, int waves, void* stream);
 
 /* Refluxing of one direction (axis, side 0: coarse cells at the fine blocks' low
  * faces, 1: at their high faces) after a stage; items: {int32 fine, coarse, ci, cj, ck,

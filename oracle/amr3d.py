@@ -139,9 +139,9 @@ class OracleAMR3D:
                 q, gq = p[:, todo], g[:, todo]
                 nc = self.t.level_nb[L - 1] * B
                 N = q.shape[1]
-                # per axis: (offsets, weights) lists of 4 (padded)
-                offs = np.zeros((3, 4, N), np.int64)
-                wts = np.zeros((3, 4, N))
+                # per axis: (offsets, weights) lists of 2 (padded)
+                offs = np.zeros((3, 2, N), np.int64)
+                wts = np.zeros((3, 2, N))
                 ntap = np.zeros((3, N), np.int64)
                 for a in range(3):
                     m = gq[a] >> 1
@@ -153,14 +153,10 @@ class OracleAMR3D:
                         wts[a, 0], wts[a, 1] = 1.0 - tt, tt
                         ntap[a] = 2
                     else:
-                        offs[a, 0] = np.where(odd, m - 1, m)
-                        for s in range(1, 4):
-                            offs[a, s] = m - 1 + s
-                        wts[a, 0] = np.where(odd, -0.0625, 1.0)
-                        wts[a, 1] = np.where(odd, 0.5625, 0.0)
-                        wts[a, 2] = np.where(odd, 0.5625, 0.0)
-                        wts[a, 3] = np.where(odd, -0.0625, 0.0)
-                        ntap[a] = np.where(odd, 4, 1)
+                        offs[a, 0], offs[a, 1] = m, m + 1
+                        wts[a, 0] = np.where(odd, 0.5, 1.0)
+                        wts[a, 1] = np.where(odd, 0.5, 0.0)
+                        ntap[a] = np.where(odd, 2, 1)
                 # taps in the kernel's order, for each position its own tap counts;
                 # positions group by their (nt0, nt1, nt2) pattern
                 for pattern in {tuple(v) for v in ntap.T}:

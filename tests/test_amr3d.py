@@ -151,3 +151,31 @@ def test_gpu_three_level_mixed_conserves(gpu):
     np.testing.assert_allclose(model.leaf_totals(), before, rtol=1e-12, atol=1e-13)
     assert model.divergence() < 1e-13
     assert model.composite_divergence() < 1e-12
+
+
+@pytest.mark.gpu
+def test_gpu_hierarchy_from_a_uniform_run(gpu):
+    """A hierarchy started from a uniform run: the base blocks that are not refined
+    keep the uniform state bit for bit, every block is divergence-free, and it steps."""
+    from wellbalanced_amd import MHD3D
+    from wellbalanced_amd.amr3d import AMRMHD3D
+
+    problem = initial.blast_turbulence_3d(32, seed=2)
+    uni = MHD3D.from_problem(problem)
+    uni.advance(2)
+    h = _two_levels()
+    amr = AMRMHD3D.from_uniform(uni, h, precision="mixed")
+    cons, _ = uni.arrays()
+    got, _ = amr.level_state(0)
+    refined = {tuple(int(v) for v in row[["bx", "by", "bz"]]) for row in amr.tables.blocks[amr.tables.children[:, 0]]}
+    for b in h.levels[0]:
+        if b in refined:
+            continue
+        sl = (slice(None),) + tuple(slice(2 + 16 * v, 18 + 16 * v) for v in b)
+        np.testing.assert_array_equal(got[sl], cons[sl])
+    assert amr.divergence() < 1e-13
+    assert amr.composite_divergence() < 1e-12
+    before = amr.leaf_totals()
+    amr.advance(2)
+    amr.check()
+    np.testing.assert_allclose(amr.leaf_totals(), before, rtol=1e-12, atol=1e-13)

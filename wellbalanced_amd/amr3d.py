@@ -52,6 +52,7 @@ class AMRMHD3D:
         self._bad = torch.zeros(2, dtype=torch.int32, device=self.device)
         self.steps = 0
         self.check_interval = 1
+        self.waves = 2  # registers of the mixed block stage (2 or 3 waves per SIMD)
         self._upload_tables()
 
     # -- tables --------------------------------------------------------------------------
@@ -98,6 +99,68 @@ class AMRMHD3D:
     def buffer(self) -> np.ndarray:
         return self._buf[0].cpu().numpy()
 
+    @classmethod
+    def from_uniform(cls, model, hierarchy: Hierarchy3D, **options) -> "AMRMHD3D":
+        """A hierarchy started from the state of a uniform `MHD3D` model of the base
+        resolution: the level-0 blocks copy it, every finer block is interpolated from
+        the level below (as a regrid fills new patches), then the covered cells and
+        edges are coarsened and the ghosts filled."""
+        opts = dict(gamma=model.gamma, mean_field=model.mean_field, limiter=model.limiter,
+                    cfl=model.cfl, device=model.device)
+        opts.update(options)
+        amr = cls(hierarchy, **opts)
+        if tuple(model.n) != hierarchy.level_shape(0):
+            raise ValueError("the uniform model must have the base resolution")
+        cons, pots = model.arrays()
+        buf = np.zeros(amr.tables.total)
+        b, h = hierarchy.block, HALO
+        for row in amr.tables.blocks:
+            if row["level"] != 0:
+                continue
+            cells, views = block_views(buf, int(row["off"]), b)
+            o = [int(row[a]) * b for a in ("bx", "by", "bz")]
+            cells[:, h:-h, h:-h, h:-h] = cons[:, h + o[0] : h + o[0] + b, h + o[1] : h + o[1] + b,
+                                             h + o[2] : h + o[2] + b]
+            for c in range(3):
+                sl = tuple(slice(h, h + b + (a != c)) for a in range(3))
+                src = tuple(slice(h + o[a], h + o[a] + b + (a != c)) for a in range(3))
+                views[c][sl] = pots[c][src]
+        amr._buf[0].copy_(torch.from_numpy(buf))
+        stream = _lib.stream_handle(amr.device)
+        amr._fill(amr._buf[0], stream)
+        levels = amr.tables.blocks["level"]
+        for L in range(1, amr.nlev):
+            ids = torch.as_tensor(np.flatnonzero(levels == L).astype(np.int32), device=amr.device)
+            _lib.call("wb_amr3d_prolong", _lib.ptr(amr._buf[0]), _lib.ptr(amr._blocks),
+                      _lib.ptr(ids), len(ids), _lib.ptr(amr._lookup), _lib.ptr(amr._levels), stream)
+        for L in range(amr.nlev - 2, -1, -1):
+            amr._coarsen(amr._buf[0], L, stream)
+        amr._fill(amr._buf[0], stream)
+        amr._t.fill_(model.time)
+        return amr
+
+    def level_state(self, level: int = 0):
+        """The state of a level that blocks cover entirely (level 0) as uniform haloed
+        arrays ``cons (5, n+4, n+4, n+4)`` and the three potentials (halos zero)."""
+        n = self.hierarchy.level_shape(level)[0]
+        b, h = self.hierarchy.block, HALO
+        buf = self.buffer()
+        X = n + 4
+        cons = np.zeros((5, X, X, X))
+        pots = [np.zeros((X, X + 1, X + 1)), np.zeros((X + 1, X, X + 1)), np.zeros((X + 1, X + 1, X))]
+        for row in self.tables.blocks:
+            if row["level"] != level:
+                continue
+            cells, views = block_views(buf, int(row["off"]), b)
+            o = [int(row[a]) * b for a in ("bx", "by", "bz")]
+            cons[:, h + o[0] : h + o[0] + b, h + o[1] : h + o[1] + b, h + o[2] : h + o[2] + b] = \
+                cells[:, h:-h, h:-h, h:-h]
+            for c in range(3):
+                sl = tuple(slice(h, h + b + (a != c)) for a in range(3))
+                dst = tuple(slice(h + o[a], h + o[a] + b + (a != c)) for a in range(3))
+                pots[c][dst] = views[c][sl]
+        return cons, pots
+
     @property
     def time(self) -> float:
         return float(self._t.item())
@@ -124,9 +187,10 @@ class AMRMHD3D:
         code = LIMITER_CODES.index(self.limiter)
         if marks is not None:
             marks[0].record()
+        extra = () if self.precision == "fp64" else (int(self.waves),)
         _lib.call("wb_amr3d_stage" + suffix, _lib.ptr(src), _lib.ptr(dst), _lib.ptr(self._blocks),
                   self.nblocks, *self.mean_field, self.gamma, code, _lib.ptr(self._dt), int(heun),
-                  self._bad.data_ptr() + 4 * slot, stream)
+                  self._bad.data_ptr() + 4 * slot, *extra, stream)
         if marks is not None:
             marks[1].record()
         for d, (items, n) in enumerate(self._reflux):

@@ -22,12 +22,15 @@ DEFAULTS_2D = {"tile": [62, 126, 254], "nxs": [8, 16, 32, 64]}
 DEFAULTS_3D = {"nxs": [8, 16, 32, 64]}
 DEFAULTS_3D_MIXED = {"nxs": [8, 16, 32, 64], "waves": [2, 3]}
 DEFAULTS_AMR = {"tile": [62, 126], "nxs": [8, 16, 32], "workgroups": [512, 1024, 2048]}
+DEFAULTS_AMR3D_MIXED = {"waves": [2, 3]}
 
 
 def _key(model) -> tuple:
     shape = getattr(model, "n", None) or (getattr(model, "nx", None), getattr(model, "ny", None))
     if hasattr(model, "layouts"):
         shape = tuple((l.nx, l.ny) for l in model.layouts)
+    if hasattr(model, "nblocks"):
+        shape = (model.nblocks,)
     return (type(model).__name__, tuple(shape) if not isinstance(shape, tuple) else shape,
             model.precision)
 
@@ -47,12 +50,17 @@ def tune(model, space: dict | None = None, steps: int = 3) -> dict:
         _apply(model, _CACHE[key])
         return _CACHE[key]
     if space is None:
-        if hasattr(model, "layouts"):
+        if hasattr(model, "nblocks"):
+            space = DEFAULTS_AMR3D_MIXED if model.precision == "mixed" else {}
+        elif hasattr(model, "layouts"):
             space = DEFAULTS_AMR
         elif hasattr(model, "n"):
             space = DEFAULTS_3D if model.precision == "fp64" else DEFAULTS_3D_MIXED
         else:
             space = DEFAULTS_2D
+    if not space:
+        _CACHE[key] = {}
+        return {}
     saved = [b.clone() for b in model._buf], model._t.clone(), model.steps
     best, best_t = None, float("inf")
     names = list(space)

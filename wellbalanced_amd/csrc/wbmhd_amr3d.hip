@@ -32,30 +32,72 @@ __device__ int edge_master(const int* lookup, const Levels& lv, int level, int c
   return -1;
 }
 
+// The t-th position of the ghost shell of an array of extents e (the positions with a
+// coordinate outside the block's own range [2, 2 + B)): the slab of the low and high
+// x layers, then the y layers inside it, then the z layers; shell_size counts them.
+__device__ __forceinline__ int shell_size(int e0, int e1, int e2) {
+  return (e0 - B) * e1 * e2 + B * (e1 - B) * e2 + B * B * (e2 - B);
+}
+__device__ __forceinline__ void shell_at(int t, int e0, int e1, int e2, int p[3]) {
+  const int n0 = (e0 - B) * e1 * e2, n1 = B * (e1 - B) * e2;
+  if (t < n0) {
+    const int o = t / (e1 * e2);
+    p[0] = o < 2 ? o : o + B;
+    p[1] = (t / e2) % e1;
+    p[2] = t % e2;
+  } else if (t < n0 + n1) {
+    t -= n0;
+    const int w = (e1 - B) * e2;
+    p[0] = 2 + t / w;
+    const int r = t % w, o = r / e2;
+    p[1] = o < 2 ? o : o + B;
+    p[2] = r % e2;
+  } else {
+    t -= n0 + n1;
+    const int w = B * (e2 - B);
+    p[0] = 2 + t / w;
+    const int r = t % w, o = r % (e2 - B);
+    p[1] = 2 + r / (e2 - B);
+    p[2] = o < 2 ? o : o + B;
+  }
+}
+
 // Ghosts of every block (one workgroup per block).  A ghost cell takes the value of
 // the block of the same level holding it; an edge outside the block's own range
 // [2, 18) takes the value of its master copy on the same level (a block keeps the
 // edges it is the master of).  Without a block of the same level, cells are
 // interpolated from the level below trilinearly (weights 1/4, 3/4 per axis) and edges
 // per axis by the same weights along the cell axis and, across, the coarse node
-// (even fine nodes) or the 4-point rule (-1, 9, 9, -1)/16 (odd).  Only values a fill
+// (even fine nodes) or the mean of the two coarse nodes (odd).  Only values a fill
 // never writes are read, so one launch fills every level.
+//
+// With `ids` (prolongation of new blocks) the listed blocks take every cell and edge,
+// interior included, by interpolation from the level below.
 __global__ void __launch_bounds__(256) k_fill3d(double* __restrict__ buf,
                                                 const BlockDesc* __restrict__ blocks,
-                                                const int* __restrict__ lookup, Levels lv) {
-  const BlockDesc d = blocks[blockIdx.x];
+                                                const int* __restrict__ lookup, Levels lv,
+                                                const int* __restrict__ ids) {
+  const int self = ids ? ids[blockIdx.x] : (int)blockIdx.x;
+  const bool all = ids != nullptr;
+  const BlockDesc d = blocks[self];
   const int L = d.level;
   const int n = lv.nb[L] * B;
   const int bo[3] = {d.bx * B, d.by * B, d.bz * B};
   double* me = buf + d.off;
   // cells
-  for (int t = threadIdx.x; t < NCELL; t += blockDim.x) {
-    const int p[3] = {t / (X * X), (t / X) % X, t % X};
-    if (p[0] >= 2 && p[0] < 2 + B && p[1] >= 2 && p[1] < 2 + B && p[2] >= 2 && p[2] < 2 + B)
-      continue;
+  const int ncell = all ? NCELL : shell_size(X, X, X);
+  for (int t = threadIdx.x; t < ncell; t += blockDim.x) {
+    int p[3];
+    if (all) {
+      p[0] = t / (X * X);
+      p[1] = (t / X) % X;
+      p[2] = t % X;
+    } else {
+      shell_at(t, X, X, X, p);
+    }
     int g[3];
     for (int a = 0; a < 3; ++a) g[a] = wrap(bo[a] + p[a] - 2, n);
-    const int owner = block_of(lookup, lv, L, g[0], g[1], g[2]);
+    const int owner = all ? -1 : block_of(lookup, lv, L, g[0], g[1], g[2]);
     if (owner >= 0) {
       const double* src = buf + blocks[owner].off;
       const int o = cell_at(0, g[0] % B + 2, g[1] % B + 2, g[2] % B + 2);
@@ -90,22 +132,28 @@ __global__ void __launch_bounds__(256) k_fill3d(double* __restrict__ buf,
   // edges of component c: axis c is the cell axis, the others node axes
   for (int c = 0; c < 3; ++c) {
     const int e0 = c == 0 ? X : XN, e1 = c == 1 ? X : XN, e2 = c == 2 ? X : XN;
-    for (int t = threadIdx.x; t < e0 * e1 * e2; t += blockDim.x) {
-      const int p[3] = {t / (e1 * e2), (t / e2) % e1, t % e2};
-      if (p[0] >= 2 && p[0] < 2 + B && p[1] >= 2 && p[1] < 2 + B && p[2] >= 2 && p[2] < 2 + B)
-        continue;
+    const int nedge = all ? e0 * e1 * e2 : shell_size(e0, e1, e2);
+    for (int t = threadIdx.x; t < nedge; t += blockDim.x) {
+      int p[3];
+      if (all) {
+        p[0] = t / (e1 * e2);
+        p[1] = (t / e2) % e1;
+        p[2] = t % e2;
+      } else {
+        shell_at(t, e0, e1, e2, p);
+      }
       int g[3], loc[3];
       for (int a = 0; a < 3; ++a) g[a] = wrap(bo[a] + p[a] - 2, n);
-      const int master = edge_master(lookup, lv, L, c, g, loc);
+      const int master = all ? -1 : edge_master(lookup, lv, L, c, g, loc);
       if (master >= 0) {
-        if (master == (int)blockIdx.x && loc[0] == p[0] && loc[1] == p[1] && loc[2] == p[2])
+        if (master == self && loc[0] == p[0] && loc[1] == p[1] && loc[2] == p[2])
           continue;  // this block's copy is the master
         me[edge_at(c, p[0], p[1], p[2])] = buf[blocks[master].off + edge_at(c, loc[0], loc[1], loc[2])];
         continue;
       }
       const int nc = lv.nb[L - 1] * B;
       int c0[3], nt[3];
-      double w[3][4];
+      double w[3][2];
       for (int a = 0; a < 3; ++a) {
         const int m = g[a] >> 1;
         if (a == c) {  // cell axis
@@ -119,12 +167,10 @@ __global__ void __launch_bounds__(256) k_fill3d(double* __restrict__ buf,
           nt[a] = 1;
           w[a][0] = 1.0;
         } else {  // the midpoint of coarse nodes m and m + 1
-          c0[a] = m - 1;
-          nt[a] = 4;
-          w[a][0] = -0.0625;
-          w[a][1] = 0.5625;
-          w[a][2] = 0.5625;
-          w[a][3] = -0.0625;
+          c0[a] = m;
+          nt[a] = 2;
+          w[a][0] = 0.5;
+          w[a][1] = 0.5;
         }
       }
       double acc = 0.0;
@@ -288,7 +334,17 @@ int wb_amr3d_fill(double* buf, const void* blocks, int nblocks, const int32_t* l
   const Levels lv = *(const Levels*)levels;
   if (nblocks < 1 || !levels_ok(lv)) return WB_EINVAL;
   hipLaunchKernelGGL(k_fill3d, dim3(nblocks), dim3(256), 0, (hipStream_t)stream, buf,
-                     (const BlockDesc*)blocks, (const int*)lookup, lv);
+                     (const BlockDesc*)blocks, (const int*)lookup, lv, (const int*)nullptr);
+  return wb_last_error();
+}
+
+int wb_amr3d_prolong(double* buf, const void* blocks, const int32_t* ids, int nids,
+                     const int32_t* lookup, const void* levels, void* stream) {
+  const Levels lv = *(const Levels*)levels;
+  if (nids < 0 || !levels_ok(lv)) return WB_EINVAL;
+  if (nids == 0) return 0;
+  hipLaunchKernelGGL(k_fill3d, dim3(nids), dim3(256), 0, (hipStream_t)stream, buf,
+                     (const BlockDesc*)blocks, (const int*)lookup, lv, (const int*)ids);
   return wb_last_error();
 }
 

@@ -13,15 +13,19 @@ extern "C" {
 
 int wb_amr3d_stage_mixed(const double* in, double* out, const void* blocks, int nblocks,
                          double b0x, double b0y, double b0z, double gamma, int limiter,
-                         const double* dt, int heun, int32_t* bad, void* stream) {
-  if (nblocks < 1) return WB_EINVAL;
+                         const double* dt, int heun, int32_t* bad, int waves, void* stream) {
+  if (nblocks < 1 || (waves != 2 && waves != 3)) return WB_EINVAL;
   StageArgs a;
   wb_amr3d_base_args(&a, in, out, b0x, b0y, b0z, gamma, limiter, dt, heun, bad);
   hipStream_t s = (hipStream_t)stream;
   return with_limiter(limiter, [&](auto lim) {
     constexpr int L = decltype(lim)::value;
-    hipLaunchKernelGGL((k_stage3d_blocks<float, 3, L>), dim3(nblocks), dim3(Tile<B, B>::THREADS), 0,
-                       s, a, (const BlockDesc*)blocks);
+    if (waves == 3)
+      hipLaunchKernelGGL((k_stage3d_blocks<float, 3, L>), dim3(nblocks), dim3(Tile<B, B>::THREADS),
+                         0, s, a, (const BlockDesc*)blocks);
+    else
+      hipLaunchKernelGGL((k_stage3d_blocks<float, 2, L>), dim3(nblocks), dim3(Tile<B, B>::THREADS),
+                         0, s, a, (const BlockDesc*)blocks);
     return wb_last_error();
   });
 }
