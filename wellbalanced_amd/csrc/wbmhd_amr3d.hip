@@ -72,7 +72,8 @@ __device__ __forceinline__ void shell_at(int t, int e0, int e1, int e2, int p[3]
 // never writes are read, so one launch fills every level.
 //
 // With `ids` (prolongation of new blocks) the listed blocks take every cell and edge,
-// interior included, by interpolation from the level below.
+// interior included, by interpolation from the level below.  blockIdx.y: 0 the cells,
+// 1..3 the potential component blockIdx.y - 1.
 __global__ void __launch_bounds__(256) k_fill3d(double* __restrict__ buf,
                                                 const BlockDesc* __restrict__ blocks,
                                                 const int* __restrict__ lookup, Levels lv,
@@ -84,8 +85,9 @@ __global__ void __launch_bounds__(256) k_fill3d(double* __restrict__ buf,
   const int n = lv.nb[L] * B;
   const int bo[3] = {d.bx * B, d.by * B, d.bz * B};
   double* me = buf + d.off;
+  const int part = blockIdx.y;
   // cells
-  const int ncell = all ? NCELL : shell_size(X, X, X);
+  const int ncell = part != 0 ? 0 : (all ? NCELL : shell_size(X, X, X));
   for (int t = threadIdx.x; t < ncell; t += blockDim.x) {
     int p[3];
     if (all) {
@@ -130,7 +132,8 @@ __global__ void __launch_bounds__(256) k_fill3d(double* __restrict__ buf,
     }
   }
   // edges of component c: axis c is the cell axis, the others node axes
-  for (int c = 0; c < 3; ++c) {
+  if (part > 0) {
+    const int c = part - 1;
     const int e0 = c == 0 ? X : XN, e1 = c == 1 ? X : XN, e2 = c == 2 ? X : XN;
     const int nedge = all ? e0 * e1 * e2 : shell_size(e0, e1, e2);
     for (int t = threadIdx.x; t < nedge; t += blockDim.x) {
@@ -333,7 +336,7 @@ int wb_amr3d_fill(double* buf, const void* blocks, int nblocks, const int32_t* l
                   const void* levels, void* stream) {
   const Levels lv = *(const Levels*)levels;
   if (nblocks < 1 || !levels_ok(lv)) return WB_EINVAL;
-  hipLaunchKernelGGL(k_fill3d, dim3(nblocks), dim3(256), 0, (hipStream_t)stream, buf,
+  hipLaunchKernelGGL(k_fill3d, dim3(nblocks, 4), dim3(256), 0, (hipStream_t)stream, buf,
                      (const BlockDesc*)blocks, (const int*)lookup, lv, (const int*)nullptr);
   return wb_last_error();
 }
@@ -343,7 +346,7 @@ int wb_amr3d_prolong(double* buf, const void* blocks, const int32_t* ids, int ni
   const Levels lv = *(const Levels*)levels;
   if (nids < 0 || !levels_ok(lv)) return WB_EINVAL;
   if (nids == 0) return 0;
-  hipLaunchKernelGGL(k_fill3d, dim3(nids), dim3(256), 0, (hipStream_t)stream, buf,
+  hipLaunchKernelGGL(k_fill3d, dim3(nids, 4), dim3(256), 0, (hipStream_t)stream, buf,
                      (const BlockDesc*)blocks, (const int*)lookup, lv, (const int*)ids);
   return wb_last_error();
 }
