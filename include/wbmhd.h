@@ -75,21 +75,39 @@ int wb_mhd2d_update(const double* in, double* out, const double* fx, const doubl
                     const double* ez, int64_t nx, int64_t ny, const double* dt, double dx,
                     double dy, int heun, void* stream);
 
+/* Replaces kernels/mhd_resistive/mhd_resistive_pyccel.py:21-32 (mhd_resistive):
+ * adds eta J to the B_z fluxes and ez and eta J x B to the energy fluxes, in place.
+ * prim, bxf, byf as wb_primitive2d gives them; fx, fy as wb_sweep2d; ez as
+ * wb_ct_contact. */
+int wb_resistive2d(const double* prim, const double* bxf, const double* byf, int64_t X, int64_t Y,
+                   double dx, double dy, double eta, double* fx, double* fy, double* ez,
+                   void* stream);
+
+/* Replaces kernels/mhd_conductive/mhd_conductive_pyccel.py:41-52 (mhd_conductive):
+ * subtracts the heat flux along (parallel) and across (perpendicular) the field from
+ * the energy fluxes fx[4], fy[4], in place. */
+int wb_conductive2d(const double* prim, const double* bxf, const double* byf, int64_t X, int64_t Y,
+                    double dx, double dy, double parallel, double perpendicular, double* fx,
+                    double* fy, void* stream);
+
 /* One whole forward-Euler stage of MHD2D._advance (mhd.py:889-927: primitive,
- * both sweeps, CT-contact E_z, the wall faces, the update; with heun the average
- * of mhd.py:983-988) in a single launch from the haloed state `in` (halos filled)
- * into `out`.  FP64: the same bits as the split kernels above.  wall_mask bit 0:
- * x walls, bit 1: y walls.  nxs: cells along x per workgroup; tile: cells along
- * y per workgroup (62, 126 or 254). */
+ * both sweeps, CT-contact E_z, the heat flux and the resistive terms (eta,
+ * kpar / kperp, all >= 0; zeros give the ideal kernel), the wall faces, the update;
+ * with heun the average of mhd.py:983-988) in a single launch from the haloed state
+ * `in` (halos filled) into `out`.  FP64: the same bits as the split kernels above.
+ * wall_mask bit 0: x walls, bit 1: y walls.  nxs: cells along x per workgroup;
+ * tile: cells along y per workgroup (62, 126 or 254; 254 runs as 126 with a
+ * diffusivity). */
 int wb_mhd2d_stage(const double* in, double* out, int64_t nx, int64_t ny, double dx, double dy,
-                   double b0x, double b0y, double gamma, int limiter, const double* dt, int heun,
-                   int wall_mask, int32_t* bad, int nxs, int tile, void* stream);
+                   double b0x, double b0y, double gamma, double eta, double kpar, double kperp,
+                   int limiter, const double* dt, int heun, int wall_mask, int32_t* bad, int nxs,
+                   int tile, void* stream);
 
 /* The same 2D stage with the reconstruction and the HLLD flux in FP32. */
 int wb_mhd2d_stage_mixed(const double* in, double* out, int64_t nx, int64_t ny, double dx,
-                         double dy, double b0x, double b0y, double gamma, int limiter,
-                         const double* dt, int heun, int wall_mask, int32_t* bad, int nxs,
-                         int tile, void* stream);
+                         double dy, double b0x, double b0y, double gamma, double eta, double kpar,
+                         double kperp, int limiter, const double* dt, int heun, int wall_mask,
+                         int32_t* bad, int nxs, int tile, void* stream);
 
 /* The stable step on the device, mhd.py:954,963 (3D: mhd3d.py:722-724):
  * *dt = cfl * (1 / rate[0]) from the output of wb_rate2d/wb_rate3d, shortened to
@@ -97,6 +115,14 @@ int wb_mhd2d_stage_mixed(const double* in, double* out, int64_t nx, int64_t ny, 
  * *time += *dt.  No host synchronisation. */
 int wb_stable_dt(const double* rate, double cfl, double remaining, double* dt, double* time,
                  void* stream);
+
+/* The stable step with the explicit diffusion limit, mhd.py:931-963 (3D:
+ * mhd3d.py:717-721): *dt = cfl * min(1 / rate[0], 0.5 / (D s)) with D = max(eta,
+ * cond / rate[1]), cond = (gamma - 1) * max(kappa), s = dx^-2 + dy^-2 (+ dz^-2);
+ * s = 0 takes D s = rate[2] from wb_amr2d_rate (the largest over the patches).  The
+ * limit is left out where D s is 0.  remaining and time as wb_stable_dt. */
+int wb_stable_dt_diffusive(const double* rate, double cfl, double remaining, double eta,
+                           double cond, double s, double* dt, double* time, void* stream);
 
 /* ---- 3D stage pieces (MHD3D) ------------------------------------------- */
 
@@ -267,23 +293,27 @@ int wb_amr2d_average(double* buf, const long long* rows, int n, void* stream);
 /* Coarse A_z nodes := the fine node on them (rows of 2 int64), mhd.py:981,991. */
 int wb_amr2d_restrict(double* buf, const long long* rows, int n, void* stream);
 
-/* The CFL rate over every patch cell (mhd.py:930-963): out = device double[2].
- * chunks: int32 pairs (patch, first cell) cutting every patch's cells into runs of
- * 256 (built on the host with the other tables). */
+/* The CFL rate over every patch cell (mhd.py:930-963): out = device double[2], the
+ * largest rate and the smallest density; with spatch (per patch dx^-2 + dy^-2, or
+ * NULL) out[2] = the largest max(eta, cond / rho) * s of a cell (its patch's
+ * diffusion number, wb_stable_dt_diffusive with s = 0).  chunks: int32 pairs
+ * (patch, first cell) cutting every patch's cells into runs of 256 (built on the
+ * host with the other tables). */
 int wb_amr2d_rate(const double* buf, const void* patches, const int32_t* chunks, int nchunks,
-                  double b0x, double b0y, double gamma, double* out, void* stream);
+                  double b0x, double b0y, double gamma, double eta, double cond,
+                  const double* spatch, double* out, void* stream);
 
 /* One stage of every patch in ONE persistent launch: `workgroups` workgroups take
  * (patch, tile, segment) items (int32 triples) from *counter until `nitems` are done
  * (the counter is zeroed first).  FP64 parity path; _mixed: FP32 flux. */
 int wb_amr2d_stage(const double* in, double* out, const void* patches, const void* items,
-                   int nitems, int32_t* counter, double b0x, double b0y, double gamma, int limiter,
-                   const double* dt, int heun, int32_t* bad, int nxs, int tile, int workgroups,
-                   void* stream);
+                   int nitems, int32_t* counter, double b0x, double b0y, double gamma, double eta,
+                   double kpar, double kperp, int limiter, const double* dt, int heun,
+                   int32_t* bad, int nxs, int tile, int workgroups, void* stream);
 int wb_amr2d_stage_mixed(const double* in, double* out, const void* patches, const void* items,
                          int nitems, int32_t* counter, double b0x, double b0y, double gamma,
-                         int limiter, const double* dt, int heun, int32_t* bad, int nxs, int tile,
-                         int workgroups, void* stream);
+                         double eta, double kpar, double kperp, int limiter, const double* dt,
+                         int heun, int32_t* bad, int nxs, int tile, int workgroups, void* stream);
 
 /* Refluxing (FluxRegister.reflux, mhd.py:989-990) after a stage: for each group
  * (first row, count) of rows (13 int64: coarse patch, cell, axis, side, face, and
@@ -292,11 +322,12 @@ int wb_amr2d_stage_mixed(const double* in, double* out, const void* patches, con
  * correction, as the stage's average). */
 int wb_amr2d_reflux(const double* src, double* dst, const void* patches, const long long* rows,
                     const int32_t* groups, int ngroups, double b0x, double b0y, double gamma,
-                    int limiter, const double* dt, int heun, void* stream);
+                    double eta, double kpar, double kperp, int limiter, const double* dt, int heun,
+                    void* stream);
 int wb_amr2d_reflux_mixed(const double* src, double* dst, const void* patches,
                           const long long* rows, const int32_t* groups, int ngroups, double b0x,
-                          double b0y, double gamma, int limiter, const double* dt, int heun,
-                          void* stream);
+                          double b0y, double gamma, double eta, double kpar, double kperp,
+                          int limiter, const double* dt, int heun, void* stream);
 
 /* ---- 3D hierarchy of equal blocks (refined, periodic) ---------------------------- */
 /* Every level is a set of blocks of 16^3 cells with a halo of 2, each stored as the

@@ -43,7 +43,7 @@ class OracleAMR2D:
     """Heun steps of a periodic 2D hierarchy held in one flat buffer."""
 
     def __init__(self, maps, hierarchy, buf, *, gamma=5.0 / 3.0, mean_field=(0.0, 0.0),
-                 limiter="minmod", cfl=0.4):
+                 limiter="minmod", cfl=0.4, resistivity=0.0, conduction=(0.0, 0.0)):
         self.maps = maps
         self.h = hierarchy
         self.buf = np.array(buf, dtype=np.float64, copy=True)
@@ -51,6 +51,8 @@ class OracleAMR2D:
         self.b0 = tuple(float(b) for b in mean_field)
         self.code = kernels.LIMITER_CODES.index(limiter)
         self.cfl = float(cfl)
+        self.resistivity = float(resistivity)
+        self.conduction = (float(conduction[0]), float(conduction[1]))
         self.nlev = hierarchy.max_level + 1
         self.time = 0.0
 
@@ -72,6 +74,10 @@ class OracleAMR2D:
         fx = kernels.sweep(prim, bxf, 0, self.code, self.gamma)
         fy = kernels.sweep(prim, byf, 1, self.code, self.gamma)
         ez = kernels.ct_contact(fx[5], fy[5], fx[0], fy[0], ecc)
+        if max(self.conduction) > 0.0:  # mhd.py:868-871
+            kernels.conductive(prim, bxf, byf, dx, dy, *self.conduction, fx, fy)
+        if self.resistivity:
+            kernels.resistive(prim, bxf, byf, dx, dy, self.resistivity, fx, fy, ez)
         return fx, fy, ez
 
     def stage(self, src, dst, dt, heun):
@@ -162,7 +168,19 @@ class OracleAMR2D:
         return largest
 
     def stable_step(self):
-        return self.cfl * (1.0 / self.rate())
+        """``mhd.py:931-963`` over the patches: the fast-wave limit of each and, with a
+        diffusivity, its diffusion limit with the patch's smallest density."""
+        smallest = np.inf
+        for lay in self.maps.layouts:
+            cells, az = self.views(self.buf, lay)
+            dx, dy = self.h.spacing(lay.level)
+            r, rho_min = kernels.rate(*cells, az, HALO, dx, dy, *self.b0, self.gamma)
+            smallest = min(smallest, 1.0 / r)
+            diffusivity = max(self.resistivity,
+                              (self.gamma - 1.0) * max(self.conduction) / float(rho_min))
+            if diffusivity > 0.0:
+                smallest = min(smallest, 0.5 / (diffusivity * (dx**-2 + dy**-2)))
+        return self.cfl * float(smallest)
 
     def step(self, dt=None):
         if dt is None:

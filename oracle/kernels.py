@@ -299,3 +299,93 @@ def rate(rho, mx, my, mz, energy, bz, az, halo, dx, dy, b0x, b0y, gamma):
         largest = float(np.max(np.where(rates > 0.0, rates, 0.0)))
         smallest = float(np.nanmin(r))
     return largest, smallest
+
+
+def mc(a, b):
+    """Monotonized-central limiter of two differences (``mhd_conductive_pyccel.py:25-38``)."""
+    with np.errstate(all="ignore"):
+        central = 0.5 * (a + b)
+        limit = np.where(np.abs(a) < np.abs(b), 2.0 * a, 2.0 * b)
+        size = np.minimum(np.abs(central), np.abs(limit))
+        slope = np.where(central > 0.0, size, np.where(central < 0.0, -size, 0.0))
+        return np.where(a * b > 0.0, slope, 0.0)
+
+
+def resistive(prim, bxf, byf, dx, dy, eta, fx, fy, ez) -> None:
+    """Adds the resistive terms to ``fx``, ``fy`` and ``ez`` in place
+    (``kernels/mhd_resistive/mhd_resistive_pyccel.py:21-67``).
+
+    ``J_z`` at the haloed block's nodes from the face fields; on the x-faces the B_z
+    flux gains ``eta J_y`` and the energy flux ``eta (J_y B_z - J_z B_y)``, on the
+    y-faces ``-eta J_x`` and ``eta (J_z B_x - J_x B_z)`` (face means of the two cells
+    and of the face's two nodes); ``ez`` gains ``eta J_z``.
+    """
+    h = 2
+    X, Y = prim.shape[1:]
+    nx, ny = X - 2 * h, Y - 2 * h
+    jz = np.zeros((X, Y))
+    jz[1:, 1:] = (byf[1:X, 1:Y] - byf[0 : X - 1, 1:Y]) / dx - (bxf[1:X, 1:Y] - bxf[1:X, 0 : Y - 1]) / dy
+    i = h + np.arange(nx + 1)[:, None]
+    j = h + np.arange(ny)[None, :]
+    jy = -(prim[7, i, j] - prim[7, i - 1, j]) / dx
+    bz_x = 0.5 * (prim[7, i, j] + prim[7, i - 1, j])
+    by_x = 0.5 * (prim[6, i, j] + prim[6, i - 1, j])
+    jz_x = 0.5 * (jz[i, j] + jz[i, j + 1])
+    f = i - h
+    fx[6, f, j] = fx[6, f, j] + eta * jy
+    fx[4, f, j] = fx[4, f, j] + eta * (jy * bz_x - jz_x * by_x)
+    i = h + np.arange(nx)[:, None]
+    j = h + np.arange(ny + 1)[None, :]
+    jx = (prim[7, i, j] - prim[7, i, j - 1]) / dy
+    bz_y = 0.5 * (prim[7, i, j] + prim[7, i, j - 1])
+    bx_y = 0.5 * (prim[5, i, j] + prim[5, i, j - 1])
+    jz_y = 0.5 * (jz[i, j] + jz[i + 1, j])
+    f = j - h
+    fy[6, i, f] = fy[6, i, f] + -eta * jx
+    fy[4, i, f] = fy[4, i, f] + eta * (jz_y * bx_y - jx * bz_y)
+    ez[...] = ez + eta * jz[h : h + nx + 1, h : h + ny + 1]
+
+
+#: The smallest normal double (``mhd_conductive_pyccel.py:55``).
+TINY = 2.2250738585072014e-308
+
+
+def conductive(prim, bxf, byf, dx, dy, parallel, perpendicular, fx, fy) -> None:
+    """Subtracts the anisotropic heat flux from ``fx[4]`` and ``fy[4]`` in place
+    (``kernels/mhd_conductive/mhd_conductive_pyccel.py:41-100``).
+
+    ``q_n = parallel b_n (b_n dT/dn + b_t dT/dt) + perpendicular dT/dn`` with
+    ``T = p / rho``, ``b`` the unit field at the face (its length at least ``TINY``),
+    ``dT/dn`` the difference across the face and ``dT/dt`` the MC-limited mean of the
+    MC-limited gradients along the face of the two cells (zero in the block's first
+    and last line).
+    """
+    X, Y = prim.shape[1:]
+    with np.errstate(all="ignore"):
+        t = prim[4] / prim[0]
+        across = np.zeros((X, Y))
+        across[:, 1 : Y - 1] = mc((t[:, 2:] - t[:, 1:-1]) / dy, (t[:, 1:-1] - t[:, :-2]) / dy)
+        k = np.arange(X - 3)[:, None]
+        j = np.arange(Y)[None, :]
+        bn = bxf[k + 2, j]
+        bt = 0.5 * (prim[6, k + 1, j] + prim[6, k + 2, j])
+        bzf = 0.5 * (prim[7, k + 1, j] + prim[7, k + 2, j])
+        magnitude = np.maximum(np.sqrt(bn * bn + bt * bt + bzf * bzf), TINY)
+        b_n = bn / magnitude
+        b_t = bt / magnitude
+        normal = (t[k + 2, j] - t[k + 1, j]) / dx
+        transverse = mc(across[k + 1, j], across[k + 2, j])
+        fx[4] = fx[4] - (parallel * b_n * (b_n * normal + b_t * transverse) + perpendicular * normal)
+        across = np.zeros((X, Y))
+        across[1 : X - 1, :] = mc((t[2:, :] - t[1:-1, :]) / dx, (t[1:-1, :] - t[:-2, :]) / dx)
+        i = np.arange(X)[:, None]
+        k = np.arange(Y - 3)[None, :]
+        bn = byf[i, k + 2]
+        bt = 0.5 * (prim[5, i, k + 1] + prim[5, i, k + 2])
+        bzf = 0.5 * (prim[7, i, k + 1] + prim[7, i, k + 2])
+        magnitude = np.maximum(np.sqrt(bn * bn + bt * bt + bzf * bzf), TINY)
+        b_n = bn / magnitude
+        b_t = bt / magnitude
+        normal = (t[i, k + 2] - t[i, k + 1]) / dy
+        transverse = mc(across[i, k + 1], across[i, k + 2])
+        fy[4] = fy[4] - (parallel * b_n * (b_n * normal + b_t * transverse) + perpendicular * normal)

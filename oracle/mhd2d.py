@@ -24,7 +24,8 @@ class Oracle2D:
     """One patch covering a uniform 2D level."""
 
     def __init__(self, cons, az, spacing, *, gamma=5.0 / 3.0, mean_field=(0.0, 0.0),
-                 limiter="vanleer", cfl=0.4, periodic=(True, True), walls=(False, False)):
+                 limiter="vanleer", cfl=0.4, periodic=(True, True), walls=(False, False),
+                 resistivity=0.0, conduction=(0.0, 0.0)):
         self.cons = np.array(cons, dtype=np.float64, copy=True)
         self.az = np.array(az, dtype=np.float64, copy=True)
         self.nx = self.cons.shape[1] - 2 * HALO
@@ -37,6 +38,8 @@ class Oracle2D:
         self.cfl = float(cfl)
         self.periodic = tuple(bool(p) for p in periodic)
         self.walls = tuple(bool(w) for w in walls)
+        self.resistivity = float(resistivity)
+        self.conduction = (float(conduction[0]), float(conduction[1]))
         self.time = 0.0
         self.steps = 0
 
@@ -59,6 +62,11 @@ class Oracle2D:
         fx = kernels.sweep(prim, bxf, 0, self.code, self.gamma)
         fy = kernels.sweep(prim, byf, 1, self.code, self.gamma)
         ez = kernels.ct_contact(fx[5], fy[5], fx[0], fy[0], ecc)
+        # mhd.py:868-871: the heat flux, then the resistive terms, before the walls
+        if max(self.conduction) > 0.0:
+            kernels.conductive(prim, bxf, byf, self.dx, self.dy, *self.conduction, fx, fy)
+        if self.resistivity:
+            kernels.resistive(prim, bxf, byf, self.dx, self.dy, self.resistivity, fx, fy, ez)
         if self.walls[0]:
             ez[0, :] = 0.0
             ez[-1, :] = 0.0
@@ -86,11 +94,18 @@ class Oracle2D:
         a[...] = a - dt * ez
 
     def stable_step(self):
-        """``mhd.py:930-963`` with ``mhd_rate``."""
-        largest, _ = kernels.rate(
+        """``mhd.py:930-963`` with ``mhd_rate``: the fast-wave limit and, with a
+        diffusivity ``D = max(eta, (gamma - 1) kappa_max / rho_min)``, the explicit
+        diffusion limit ``0.5 / (D (dx^-2 + dy^-2))``."""
+        largest, rho_min = kernels.rate(
             *self.cons, self.az, HALO, self.dx, self.dy, *self.b0, self.gamma
         )
-        return self.cfl * (1.0 / largest)
+        smallest = 1.0 / largest
+        diffusivity = max(self.resistivity,
+                          (self.gamma - 1.0) * max(self.conduction) / float(rho_min))
+        if diffusivity > 0.0:
+            smallest = min(smallest, 0.5 / (diffusivity * (self.dx**-2 + self.dy**-2)))
+        return self.cfl * smallest
 
     def step(self, dt=None):
         """Heun (SSP-RK2) step (``mhd.py:965-994``); returns ``dt``."""

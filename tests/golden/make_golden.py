@@ -1,10 +1,13 @@
-"""Write the golden vectors of the two kernels that no closed form pins.
+"""Write the golden vectors of the kernels that no closed form pins.
 
 The HLLD sweep's star and double-star branches with the limiters and the
-first-order fallback, and the CT-contact upwind weights, are not fixed by any
-closed form or invariant that the reference's tests hold.  This script runs the
-reference's own scalar kernels -- ``kernels/mhd_sweep/mhd_sweep_pyccel.py`` and
-``kernels/ct_contact/ct_contact_pyccel.py``, which import only ``math`` and
+first-order fallback, the CT-contact upwind weights, the face averages of the
+resistive energy flux and the limited transverse gradient of the anisotropic heat
+flux are not fixed by any closed form or invariant that the reference's tests hold.
+This script runs the reference's own scalar kernels --
+``kernels/mhd_sweep/mhd_sweep_pyccel.py``, ``kernels/ct_contact/ct_contact_pyccel.py``,
+``kernels/mhd_resistive/mhd_resistive_pyccel.py`` and
+``kernels/mhd_conductive/mhd_conductive_pyccel.py``, which import only ``math`` and
 ``numpy`` -- as plain Python on the seeded blocks of the reference's kernel test
 (``tests/unit/test_mhd_kernels.py:17-33,54-76``) and stores inputs and outputs in
 ``golden.npz`` next to this file.  Run it only where the reference tree exists:
@@ -58,6 +61,29 @@ def ct_inputs(seed=2):
     return fx, fy, ecc
 
 
+#: Coefficients of the non-ideal vectors: dx, dy, eta, parallel, perpendicular.
+NONIDEAL = (0.1, 0.07, 0.37, 0.8, 0.05)
+
+
+def nonideal_inputs(seed=3):
+    """Face fluxes and node E_z to add the non-ideal terms to (any values do)."""
+    rng = np.random.default_rng(seed)
+    nx, ny = SHAPE
+    fx = rng.normal(size=(7, nx - 3, ny))
+    fy = rng.normal(size=(7, nx, ny - 3))
+    ez = rng.normal(size=(nx - 3, ny - 3))
+    return fx, fy, ez
+
+
+def field_free_face(prim):
+    """The block with no field at the x-face 4 of row 2 (bxf[4, 2] = 0 already) nor at
+    the y-face 4 of column 2 (byf[2, 4] = 0), where the unit field is 0 / tiny."""
+    prim = prim.copy()
+    prim[5:8, 3:5, 2] = 0.0
+    prim[5:8, 2, 3:5] = 0.0
+    return prim
+
+
 def main(reference: Path) -> None:
     kernels = reference / "src" / "carafe" / "kernels"
     sweep = _load(kernels / "mhd_sweep" / "mhd_sweep_pyccel.py", "ref_mhd_sweep")
@@ -79,6 +105,20 @@ def main(reference: Path) -> None:
     ez = np.full((nx - 3, ny - 3), np.nan)
     ct.ct_contact(fx[5], fy[5], fx[0], fy[0], ecc, ez)
     out.update(ct_fx=fx, ct_fy=fy, ct_ecc=ecc, ct_ez=ez)
+    res = _load(kernels / "mhd_resistive" / "mhd_resistive_pyccel.py", "ref_mhd_resistive")
+    con = _load(kernels / "mhd_conductive" / "mhd_conductive_pyccel.py", "ref_mhd_conductive")
+    dx, dy, eta, parallel, perpendicular = NONIDEAL
+    fx0, fy0, ez0 = nonideal_inputs()
+    out.update(ni_fx=fx0, ni_fy=fy0, ni_ez=ez0)
+    for seed in (1, 7):
+        prim, bxf, byf = block(seed)
+        fx, fy, ez = fx0.copy(), fy0.copy(), ez0.copy()
+        res.mhd_resistive(prim, bxf, byf, dx, dy, eta, fx, fy, ez)
+        out.update({f"res_fx_{seed}": fx, f"res_fy_{seed}": fy, f"res_ez_{seed}": ez})
+        cprim = field_free_face(prim)
+        fx, fy = fx0.copy(), fy0.copy()
+        con.mhd_conductive(cprim, bxf, byf, dx, dy, parallel, perpendicular, fx, fy)
+        out.update({f"con_prim_{seed}": cprim, f"con_fx_{seed}": fx, f"con_fy_{seed}": fy})
     target = Path(__file__).resolve().parent / "golden.npz"
     np.savez_compressed(target, **out)
     print(f"wrote {target} ({len(out)} arrays)")

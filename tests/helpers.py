@@ -20,6 +20,8 @@ def options2d(problem, **over):
         limiter=opts.get("limiter", "vanleer"),
         cfl=opts.get("cfl", 0.4),
         walls=tuple(walls),
+        resistivity=opts.get("resistivity", 0.0),
+        conduction=tuple(opts.get("conduction", (0.0, 0.0))),
     )
 
 

@@ -28,9 +28,12 @@ SIGNATURES: dict[str, tuple[type, list]] = {
     "wb_mhd2d_fill_halos": (I, [P, I64, I64, I, I, P]),
     "wb_mhd2d_walls": (I, [P, P, P, I64, I64, I, P]),
     "wb_mhd2d_update": (I, [P, P, P, P, P, I64, I64, P, D, D, I, P]),
+    "wb_resistive2d": (I, [P, P, P, I64, I64, D, D, D, P, P, P, P]),
+    "wb_conductive2d": (I, [P, P, P, I64, I64, D, D, D, D, P, P, P]),
     "wb_stable_dt": (I, [P, D, D, P, P, P]),
-    "wb_mhd2d_stage": (I, [P, P, I64, I64] + [D] * 5 + [I, P, I, I, P, I, I, P]),
-    "wb_mhd2d_stage_mixed": (I, [P, P, I64, I64] + [D] * 5 + [I, P, I, I, P, I, I, P]),
+    "wb_stable_dt_diffusive": (I, [P, D, D, D, D, D, P, P, P]),
+    "wb_mhd2d_stage": (I, [P, P, I64, I64] + [D] * 8 + [I, P, I, I, P, I, I, P]),
+    "wb_mhd2d_stage_mixed": (I, [P, P, I64, I64] + [D] * 8 + [I, P, I, I, P, I, I, P]),
     "wb_mhd3d_state_size": (I64, [I64, I64, I64]),
     "wb_mhd3d_fill_halos": (I, [P, I64, I64, I64, I, P]),
     "wb_primitive3d": (I, [P, I64, I64, I64] + [D] * 7 + [P] * 5 + [P]),
@@ -67,11 +70,11 @@ SIGNATURES: dict[str, tuple[type, list]] = {
     "wb_amr2d_interp": (I, [P] * 8 + [I] + [P] * 6 + [I, P]),
     "wb_amr2d_average": (I, [P, P, I, P]),
     "wb_amr2d_restrict": (I, [P, P, I, P]),
-    "wb_amr2d_rate": (I, [P, P, P, I, D, D, D, P, P]),
-    "wb_amr2d_stage": (I, [P, P, P, P, I, P, D, D, D, I, P, I, P, I, I, I, P]),
-    "wb_amr2d_stage_mixed": (I, [P, P, P, P, I, P, D, D, D, I, P, I, P, I, I, I, P]),
-    "wb_amr2d_reflux": (I, [P, P, P, P, P, I, D, D, D, I, P, I, P]),
-    "wb_amr2d_reflux_mixed": (I, [P, P, P, P, P, I, D, D, D, I, P, I, P]),
+    "wb_amr2d_rate": (I, [P, P, P, I, D, D, D, D, D, P, P, P]),
+    "wb_amr2d_stage": (I, [P, P, P, P, I, P] + [D] * 6 + [I, P, I, P, I, I, I, P]),
+    "wb_amr2d_stage_mixed": (I, [P, P, P, P, I, P] + [D] * 6 + [I, P, I, P, I, I, I, P]),
+    "wb_amr2d_reflux": (I, [P, P, P, P, P, I] + [D] * 6 + [I, P, I, P]),
+    "wb_amr2d_reflux_mixed": (I, [P, P, P, P, P, I] + [D] * 6 + [I, P, I, P]),
 }
 
 _lib = None

@@ -1,4 +1,4 @@
-"""``AMRMHD2D``: the 2D ideal-MHD step on a refined, periodic patch hierarchy.
+"""``AMRMHD2D``: the 2D MHD step on a refined, periodic patch hierarchy.
 
 The reference runs ``MHD2D`` on patchwork patches of several levels that all take
 the same time step (``mhd.py:40-44``): halos from the hierarchy, fluxes per patch,
@@ -15,7 +15,8 @@ ping-ponged as in `MHD2D`); a stage is
    A_z.
 
 dt comes from the largest fast-wave rate over every patch cell (the finest cells
-bind it).  ``precision="mixed"`` runs the FP32 reconstruction/HLLD stage.
+bind it) and, with resistivity or conduction, the explicit diffusion limit of each
+patch.  ``precision="mixed"`` runs the FP32 reconstruction/HLLD stage.
 """
 
 from __future__ import annotations
@@ -37,6 +38,7 @@ class AMRMHD2D:
 
     def __init__(self, hierarchy: AMRHierarchy, *, gamma: float = 5.0 / 3.0,
                  mean_field=(0.0, 0.0), limiter: str = "minmod", cfl: float = 0.4,
+                 resistivity: float = 0.0, conduction: tuple[float, float] = (0.0, 0.0),
                  precision: str = "fp64", tile: int = 62, nxs: int = 16, device=None) -> None:
         if hierarchy.ndim != 2:
             raise ValueError("AMRMHD2D needs a 2D hierarchy")
@@ -46,6 +48,10 @@ class AMRMHD2D:
             raise ValueError("cfl must be in (0, 1]")
         if gamma <= 1.0:
             raise ValueError("gamma must be larger than 1")
+        if not resistivity >= 0.0:
+            raise ValueError("resistivity must be non-negative")
+        if not min(conduction) >= 0.0:
+            raise ValueError("the conductivities must be non-negative")
         if precision not in PRECISIONS:
             raise ValueError(f"precision is one of {PRECISIONS}")
         if tile not in (62, 126):
@@ -55,6 +61,8 @@ class AMRMHD2D:
         self.mean_field = (float(mean_field[0]), float(mean_field[1]))
         self.limiter = limiter
         self.cfl = float(cfl)
+        self.resistivity = float(resistivity)
+        self.conduction = (float(conduction[0]), float(conduction[1]))
         self.precision = precision
         self.tile = tile
         self.nxs = nxs
@@ -66,7 +74,7 @@ class AMRMHD2D:
         self._buf = [torch.zeros(self.maps.total, **opts), torch.zeros(self.maps.total, **opts)]
         self._dt = torch.zeros(1, **opts)
         self._t = torch.zeros(1, **opts)
-        self._rate = torch.zeros(2, **opts)
+        self._rate = torch.zeros(3, **opts)
         self._bad = torch.zeros(2, dtype=torch.int32, device=self.device)
         self._counter = torch.zeros(1, dtype=torch.int32, device=self.device)
         self._upload_tables()
@@ -121,10 +129,12 @@ class AMRMHD2D:
             self._average[L] = (self._dev(m.average[L].reshape(-1)), len(m.average[L]))
             self._restrict[L] = (self._dev(m.restrict[L].reshape(-1)), len(m.restrict[L]))
         patches = np.zeros(len(self.layouts), _PATCH)
+        spatch = np.zeros(len(self.layouts))
         items = []
         for k, lay in enumerate(self.layouts):
             dx, dy = self.hierarchy.spacing(lay.level)
             patches[k] = (lay.offset, lay.nx, lay.ny, dx, dy)
+            spatch[k] = dx**-2 + dy**-2  # the diffusion limit's sum (mhd.py:961)
             for t in range((lay.ny + self.tile - 1) // self.tile):
                 for s in range((lay.nx + self.nxs - 1) // self.nxs):
                     items.append((k, t, s))
@@ -134,6 +144,7 @@ class AMRMHD2D:
         items = np.array([items[i] for i in order], _ITEM)
         self._patches = torch.from_numpy(patches.view(np.uint8).copy()).to(self.device)
         self._items = torch.from_numpy(items.view(np.uint8).copy()).to(self.device)
+        self._spatch = self._dev(spatch, torch.float64)
         self.nitems = len(items)
         # the CFL reduction walks every patch's cells in chunks of 256
         chunks = [(k, c) for k, lay in enumerate(self.layouts)
@@ -192,26 +203,45 @@ class AMRMHD2D:
         self._fill(src, stream)
         suffix = "" if self.precision == "fp64" else "_mixed"
         code = LIMITER_CODES.index(self.limiter)
+        diff = (self.resistivity, *self.conduction)
         _lib.call("wb_amr2d_stage" + suffix, _lib.ptr(src), _lib.ptr(dst), _lib.ptr(self._patches),
                   _lib.ptr(self._items), self.nitems, _lib.ptr(self._counter), *self.mean_field,
-                  self.gamma, code, _lib.ptr(self._dt), int(heun),
+                  self.gamma, *diff, code, _lib.ptr(self._dt), int(heun),
                   self._bad.data_ptr() + 4 * slot, int(self.nxs), int(self.tile),
                   int(self.workgroups), stream)
         for L in range(self.nlev - 2, -1, -1):
             rows, groups, n = self._reflux[L]
             _lib.call("wb_amr2d_reflux" + suffix, _lib.ptr(src), _lib.ptr(dst),
                       _lib.ptr(self._patches), _lib.ptr(rows), _lib.ptr(groups), n,
-                      *self.mean_field, self.gamma, code, _lib.ptr(self._dt), int(heun), stream)
+                      *self.mean_field, self.gamma, *diff, code, _lib.ptr(self._dt), int(heun),
+                      stream)
             self._coarsen(dst, L, stream)
 
+    @property
+    def _diffusive(self) -> bool:
+        return bool(self.resistivity) or max(self.conduction) > 0.0
+
     def _rate_kernel(self, stream: int) -> None:
+        spatch = _lib.ptr(self._spatch) if self._diffusive else None
         _lib.call("wb_amr2d_rate", _lib.ptr(self._buf[0]), _lib.ptr(self._patches),
                   _lib.ptr(self._chunks), self.nchunks, *self.mean_field, self.gamma,
+                  self.resistivity, (self.gamma - 1.0) * max(self.conduction), spatch,
                   _lib.ptr(self._rate), stream)
 
+    def _dt_kernel(self, time_ptr, stream: int) -> None:
+        if self._diffusive:  # the largest D s of the patches is rate[2]
+            _lib.call("wb_stable_dt_diffusive", _lib.ptr(self._rate), self.cfl, 0.0,
+                      self.resistivity, (self.gamma - 1.0) * max(self.conduction), 0.0,
+                      _lib.ptr(self._dt), time_ptr, stream)
+        else:
+            _lib.call("wb_stable_dt", _lib.ptr(self._rate), self.cfl, 0.0, _lib.ptr(self._dt),
+                      time_ptr, stream)
+
     def stable_step(self) -> float:
-        self._rate_kernel(_lib.stream_handle(self.device))
-        return self.cfl * (1.0 / float(self._rate[0].item()))
+        stream = _lib.stream_handle(self.device)
+        self._rate_kernel(stream)
+        self._dt_kernel(None, stream)
+        return float(self._dt[0].item())
 
     def _heun(self, stream: int, events=None) -> None:
         a, b = self._buf
@@ -248,8 +278,7 @@ class AMRMHD2D:
         marks = [] if events else None
         for _ in range(steps):
             self._rate_kernel(stream)
-            _lib.call("wb_stable_dt", _lib.ptr(self._rate), self.cfl, 0.0, _lib.ptr(self._dt),
-                      _lib.ptr(self._t), stream)
+            self._dt_kernel(_lib.ptr(self._t), stream)
             self._heun(stream, marks)
             self.steps += 1
         return marks

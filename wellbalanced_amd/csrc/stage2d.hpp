@@ -9,10 +9,13 @@
 // primitive row sits in LDS for the y-sweep, and the y-face fluxes and the cell
 // E_z of the last two rows sit in LDS rings for the CT-contact average.
 // P is the arithmetic of the reconstruction and the HLLD flux; the rest is FP64.
+// NI adds the resistive and heat-conduction face terms (FP64): T, B_x, B_y, B_z of
+// the last three rows and J_z at the nodes of the last two sit in LDS rings.
 #pragma once
 
 #include "mhd_device.hpp"
 #include "mhd_fast.hpp"
+#include "nonideal.hpp"
 #include "wbmhd_common.hpp"
 
 namespace wb {
@@ -24,13 +27,17 @@ struct Row {
   static constexpr int THREADS = ((FY + 63) / 64) * 64;
 };
 
-template <typename P, int TY>
+template <typename P, int TY, bool NI = false>
 struct Stage2Shared {
   using T = Row<TY>;
   P prim[2][8][T::PY];
   double ecc[3][T::FY];  // cell E_z of rows f-1, f, f+1
   P fx[2][T::FY];     // x-face f: rows 0 (mass) and 5 (B_y)
   P fy[2][7][T::FY];  // y-faces of rows f-1 and f
+  // NI: T = p / rho, B_x, B_y, B_z of rows (row % 3); J_z of node rows (row % 2) at
+  // node columns jy0 - 1 + c, c in [0, FY]
+  double cell[NI ? 3 : 1][4][NI ? T::PY : 1];
+  double jz[NI ? 2 : 1][NI ? T::FY + 1 : 1];
 };
 
 struct Stage2Args {
@@ -42,13 +49,14 @@ struct Stage2Args {
   double dx, dy, b0x, b0y, gamma;
   int limiter, heun, nxs;
   int walls;  // bit 0: x walls, bit 1: y walls
+  Diffusion diff;  // used by the NI kernels only
 };
 
 // LIM: the compile-time slope limiter of the branch-free FP32 flux (mixed path),
 // or -1 for the parity path's runtime limiter.
-template <typename P, int TY, int LIM>
+template <typename P, int TY, int LIM, bool NI = false>
 __device__ __forceinline__ void stage2d_body(const Stage2Args& a, int tile, int seg,
-                                             Stage2Shared<P, TY>& sh) {
+                                             Stage2Shared<P, TY, NI>& sh) {
   using T = Row<TY>;
   const int tid = threadIdx.x;
   const int X = a.X, Y = a.Y, YN = Y + 1;
@@ -102,6 +110,17 @@ __device__ __forceinline__ void stage2d_body(const Stage2Args& a, int tile, int 
       sh.prim[buf][6][idx] = P(by);
       sh.prim[buf][7][idx] = P(b);
       if (idx >= 1 && idx <= T::FY) sh.ecc[pl % 3][idx - 1] = bx * v - by * u;
+      if constexpr (NI) {
+        double* cr = &sh.cell[pl % 3][0][0];
+        cr[idx] = p / r;
+        cr[T::PY + idx] = bx;
+        cr[2 * T::PY + idx] = by;
+        cr[3 * T::PY + idx] = b;
+        if (idx <= T::FY) {  // J_z at node (pl, jy0 - 1 + idx), inside the haloed block
+          const int I = min(max(pl, 1), X - 1), J = min(max(jy0 - 1 + idx, 1), Y - 1);
+          sh.jz[pl & 1][idx] = node_current_az(az, YN, I, J, a.dx, a.dy, a.b0x, a.b0y);
+        }
+      }
       if (idx >= 2 && idx < 2 + tyv && pl >= ix0 && pl < ix0 + nxv) {
         const bool is_bad = !(r > 0.0 && p > 0.0) ||
                             !(finite(r) && finite(u) && finite(v) && finite(w) && finite(p) &&
@@ -142,8 +161,25 @@ __device__ __forceinline__ void stage2d_body(const Stage2Args& a, int tile, int 
       const int jj = min(max(gy, 0), Y - 1);
       if (f >= ix0) {  // x-face f of the column
         P fl[7];
-        const P bn = P((AZ(f, jj + 1) - AZ(f, jj)) / a.dy + a.b0x);
+        const double bnd = (AZ(f, jj + 1) - AZ(f, jj)) / a.dy + a.b0x;
+        const P bn = P(bnd);
         face_solve<P, LIM>(q, a.limiter, bn, P(a.gamma), P(1.0 / (a.gamma - 1.0)), fl);
+        if constexpr (NI) {  // mhd.py:868-871, cells L = f - 1 and R = f of column gy
+          const double(*cl)[T::PY] = sh.cell[(f + 2) % 3];
+          const double(*cr)[T::PY] = sh.cell[f % 3];
+          const int c = tid + 1;
+          double e4 = double(fl[4]), b6 = double(fl[6]);
+          if (a.diff.conduction())
+            conductive_face(bnd, cl[2][c], cr[2][c], cl[3][c], cr[3][c], cl[0][c], cr[0][c],
+                            limited_gradient(cl[0][c - 1], cl[0][c], cl[0][c + 1], a.dy),
+                            limited_gradient(cr[0][c - 1], cr[0][c], cr[0][c + 1], a.dy), a.dx,
+                            a.diff, e4);
+          if (a.diff.eta != 0.0)
+            resistive_xface(cl[3][c], cr[3][c], cl[2][c], cr[2][c], sh.jz[f & 1][tid],
+                            sh.jz[f & 1][tid + 1], a.dx, a.diff.eta, e4, b6);
+          fl[4] = P(e4);
+          fl[6] = P(b6);
+        }
         // walls: nothing but the pressure crosses (mhd.py:873-886), after the CT
         const bool wall = xwall && (f == 2 || f == a.nx + 2);
 #pragma unroll
@@ -164,8 +200,25 @@ __device__ __forceinline__ void stage2d_body(const Stage2Args& a, int tile, int 
           for (int s = 0; s < 4; ++s) qq[c][s] = sh.prim[cur][comp[c]][tid + 1 + s - 2];
         P fl[7];
         const int ii = min(max(f, 0), X - 1);
-        const P bn = P(-(AZ(ii + 1, jj) - AZ(ii, jj)) / a.dx + a.b0y);
+        const double bnd = -(AZ(ii + 1, jj) - AZ(ii, jj)) / a.dx + a.b0y;
+        const P bn = P(bnd);
         face_solve<P, LIM>(qq, a.limiter, bn, P(a.gamma), P(1.0 / (a.gamma - 1.0)), fl);
+        if constexpr (NI) {  // cells L = gy - 1 and R = gy of row f
+          const double(*cm)[T::PY] = sh.cell[(f + 2) % 3];
+          const double(*c0)[T::PY] = sh.cell[f % 3];
+          const double(*cp)[T::PY] = sh.cell[(f + 1) % 3];
+          const int c = tid + 1;
+          double e4 = double(fl[4]), b6 = double(fl[6]);
+          if (a.diff.conduction())
+            conductive_face(bnd, c0[1][c - 1], c0[1][c], c0[3][c - 1], c0[3][c], c0[0][c - 1],
+                            c0[0][c], limited_gradient(cm[0][c - 1], c0[0][c - 1], cp[0][c - 1], a.dx),
+                            limited_gradient(cm[0][c], c0[0][c], cp[0][c], a.dx), a.dy, a.diff, e4);
+          if (a.diff.eta != 0.0)
+            resistive_yface(c0[3][c - 1], c0[3][c], c0[1][c - 1], c0[1][c], sh.jz[f & 1][tid],
+                            sh.jz[(f + 1) & 1][tid], a.dy, a.diff.eta, e4, b6);
+          fl[4] = P(e4);
+          fl[6] = P(b6);
+        }
 #pragma unroll
         for (int r = 0; r < 7; ++r) sh.fy[cur][r][tid] = fl[r];
       }
@@ -184,6 +237,9 @@ __device__ __forceinline__ void stage2d_body(const Stage2Args& a, int tile, int 
                            double(sh.fy[prv][0][tid]), double(sh.fy[cur][0][tid]),
                            sh.ecc[(f + 2) % 3][tid - 1], sh.ecc[f % 3][tid - 1],
                            sh.ecc[(f + 2) % 3][tid], sh.ecc[f % 3][tid]);
+        if constexpr (NI) {
+          if (a.diff.eta != 0.0) e = e + a.diff.eta * sh.jz[f & 1][tid];
+        }
         if ((xwall && (f == 2 || f == a.nx + 2)) || (ywall && (gy == 2 || gy == a.ny + 2))) e = 0.0;
         const size_t o = 6 * PL + (size_t)f * YN + gy;
         double v = a.in[o] - dt * e;
@@ -217,22 +273,22 @@ __device__ __forceinline__ void stage2d_body(const Stage2Args& a, int tile, int 
   }
 }
 
-template <typename P, int TY, int LIM>
+template <typename P, int TY, int LIM, bool NI = false>
 __global__ void __launch_bounds__((Row<TY>::THREADS)) k_stage2d(Stage2Args a) {
-  __shared__ Stage2Shared<P, TY> sh;
-  stage2d_body<P, TY, LIM>(a, blockIdx.x, blockIdx.y, sh);
+  __shared__ Stage2Shared<P, TY, NI> sh;
+  stage2d_body<P, TY, LIM, NI>(a, blockIdx.x, blockIdx.y, sh);
 }
 
 
 // All tiles of all patches in one persistent launch: each workgroup takes the next
 // work item from a counter until the list is exhausted (the counter is zeroed by
 // k_zero_i32 before the launch).
-template <typename P, int TY, int LIM>
+template <typename P, int TY, int LIM, bool NI = false>
 __global__ void __launch_bounds__((Row<TY>::THREADS))
 k_stage2d_batched(Stage2Args base, const PatchDesc* __restrict__ patches,
                   const WorkItem* __restrict__ items, int nitems, int* __restrict__ counter,
                   double* __restrict__ out_base) {
-  __shared__ Stage2Shared<P, TY> sh;
+  __shared__ Stage2Shared<P, TY, NI> sh;
   __shared__ int next;
   for (;;) {
     if (threadIdx.x == 0) next = atomicAdd(counter, 1);
@@ -251,7 +307,7 @@ k_stage2d_batched(Stage2Args base, const PatchDesc* __restrict__ patches,
     a.Y = pd.ny + 4;
     a.dx = pd.dx;
     a.dy = pd.dy;
-    stage2d_body<P, TY, LIM>(a, w.tile, w.seg, sh);
+    stage2d_body<P, TY, LIM, NI>(a, w.tile, w.seg, sh);
     __syncthreads();
   }
 }
@@ -261,8 +317,8 @@ k_stage2d_batched(Stage2Args base, const PatchDesc* __restrict__ patches,
 // HLLD in P.  axis 0: x-face `fa` (low face of cell fa) at row fb; axis 1: y-face.
 template <typename P, int LIM>
 __device__ void face_flux2d(const double* in, int X, int Y, double dx, double dy, double b0x,
-                            double b0y, double gamma, int limiter, int axis, int fa, int fb,
-                            double* out7) {
+                            double b0y, double gamma, int limiter, const Diffusion& diff, int axis,
+                            int fa, int fb, double* out7) {
   const size_t PL = (size_t)X * Y;
   const double* az = in + 6 * PL;
   const int YN = Y + 1;
@@ -293,6 +349,52 @@ __device__ void face_flux2d(const double* in, int X, int Y, double dx, double dy
   P f[7];
   face_solve<P, LIM>(q, limiter, P(bn), P(gamma), P(1.0 / (gamma - 1.0)), f);
   for (int c = 0; c < 7; ++c) out7[c] = double(f[c]);
+  if (!diff.conduction() && diff.eta == 0.0) return;
+  // the non-ideal terms (mhd.py:868-871) from T, B of the cells around the face
+  auto cell = [&](int i, int j, double out4[4]) {
+    const size_t c = (size_t)i * Y + j;
+    const double bx0 = (AZ(i, j + 1) - AZ(i, j)) / dy + b0x;
+    const double bx1 = (AZ(i + 1, j + 1) - AZ(i + 1, j)) / dy + b0x;
+    const double by0 = -(AZ(i + 1, j) - AZ(i, j)) / dx + b0y;
+    const double by1 = -(AZ(i + 1, j + 1) - AZ(i, j + 1)) / dx + b0y;
+    const double r = in[c];
+    const double u = in[PL + c] / r, v = in[2 * PL + c] / r, w = in[3 * PL + c] / r;
+    const double bx = 0.5 * (bx0 + bx1);
+    const double by = 0.5 * (by0 + by1);
+    const double b = in[5 * PL + c];
+    const double p = (gamma - 1.0) * (in[4 * PL + c] - 0.5 * r * (u * u + v * v + w * w) -
+                                      0.5 * (bx * bx + by * by + b * b));
+    out4[0] = p / r;
+    out4[1] = bx;
+    out4[2] = by;
+    out4[3] = b;
+  };
+  double e4 = out7[4], b6 = out7[6];
+  // L and R beside the face, and the cells before (m) and after (p) each along it
+  const int di = axis == 0 ? 1 : 0, dj = 1 - di;
+  double L[4], R[4], Lm[4], Lp[4], Rm[4], Rp[4];
+  cell(fa - di, fb - dj, L);
+  cell(fa, fb, R);
+  if (diff.conduction()) {
+    cell(fa - di - dj, fb - dj - di, Lm);
+    cell(fa - di + dj, fb - dj + di, Lp);
+    cell(fa - dj, fb - di, Rm);
+    cell(fa + dj, fb + di, Rp);
+    const double ha = axis == 0 ? dy : dx;  // the spacing along the face
+    conductive_face(bn, axis == 0 ? L[2] : L[1], axis == 0 ? R[2] : R[1], L[3], R[3], L[0], R[0],
+                    limited_gradient(Lm[0], L[0], Lp[0], ha), limited_gradient(Rm[0], R[0], Rp[0], ha),
+                    axis == 0 ? dx : dy, diff, e4);
+  }
+  if (diff.eta != 0.0) {
+    const double j0 = node_current_az(az, YN, fa, fb, dx, dy, b0x, b0y);
+    const double j1 = node_current_az(az, YN, fa + dj, fb + di, dx, dy, b0x, b0y);
+    if (axis == 0)
+      resistive_xface(L[3], R[3], L[2], R[2], j0, j1, dx, diff.eta, e4, b6);
+    else
+      resistive_yface(L[3], R[3], L[1], R[1], j0, j1, dy, diff.eta, e4, b6);
+  }
+  out7[4] = double(P(e4));
+  out7[6] = double(P(b6));
 }
 
 // Refluxing (patchwork FluxRegister.reflux, mhd.py:989-990): per uncovered coarse
@@ -304,7 +406,8 @@ template <typename P, int LIM>
 __global__ void __launch_bounds__(64) k_reflux2d(const double* __restrict__ src, double* __restrict__ dst,
                            const PatchDesc* __restrict__ patches, const long long* __restrict__ rows,
                            const int* __restrict__ groups, int ngroups, double b0x, double b0y,
-                           double gamma, int limiter, const double* __restrict__ dtp, int heun) {
+                           double gamma, int limiter, Diffusion diff, const double* __restrict__ dtp,
+                           int heun) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ngroups) return;
   constexpr int XR[6] = {0, 1, 2, 3, 4, 6};
@@ -323,12 +426,12 @@ __global__ void __launch_bounds__(64) k_reflux2d(const double* __restrict__ src,
     const double dtd = axis == 0 ? dt / cq.dx : dt / cq.dy;
     double fc[7], f0[7], f1[7];
     face_flux2d<P, LIM>(src + cq.off, cq.nx + 4, cq.ny + 4, cq.dx, cq.dy, b0x, b0y, gamma, limiter,
-                   axis, (int)r[5], (int)r[6], fc);
+                        diff, axis, (int)r[5], (int)r[6], fc);
     const PatchDesc p0 = patches[r[7]], p1 = patches[r[10]];
     face_flux2d<P, LIM>(src + p0.off, p0.nx + 4, p0.ny + 4, p0.dx, p0.dy, b0x, b0y, gamma, limiter,
-                   axis, (int)r[8], (int)r[9], f0);
+                        diff, axis, (int)r[8], (int)r[9], f0);
     face_flux2d<P, LIM>(src + p1.off, p1.nx + 4, p1.ny + 4, p1.dx, p1.dy, b0x, b0y, gamma, limiter,
-                   axis, (int)r[11], (int)r[12], f1);
+                        diff, axis, (int)r[11], (int)r[12], f1);
     for (int f = 0; f < 6; ++f) {
       const int row = axis == 0 ? XR[f] : YR[f];
       const double mean = 0.5 * (f0[row] + f1[row]);
@@ -345,11 +448,12 @@ __global__ void __launch_bounds__(64) k_reflux2d(const double* __restrict__ src,
   }
 }
 
-template <typename P, int TY, int LIM = -1>
+template <typename P, int TY, int LIM = -1, bool NI = false>
 inline int launch_stage2d(const Stage2Args& a, hipStream_t s) {
   const int tiles = (a.ny + TY - 1) / TY;
   const int segs = (a.nx + a.nxs - 1) / a.nxs;
-  hipLaunchKernelGGL((k_stage2d<P, TY, LIM>), dim3(tiles, segs), dim3(Row<TY>::THREADS), 0, s, a);
+  hipLaunchKernelGGL((k_stage2d<P, TY, LIM, NI>), dim3(tiles, segs), dim3(Row<TY>::THREADS), 0, s,
+                     a);
   return wb_last_error();
 }
 

@@ -6,10 +6,10 @@ using namespace wb;
 
 extern "C" __attribute__((visibility("hidden"))) int wb_fill_stage_args2d(
     void* args, const double* in, double* out, int64_t nx, int64_t ny, double dx, double dy,
-    double b0x, double b0y, double gamma, int limiter, const double* dt, int heun, int walls,
-    int32_t* bad, int nxs) {
+    double b0x, double b0y, double gamma, double eta, double kpar, double kperp, int limiter,
+    const double* dt, int heun, int walls, int32_t* bad, int nxs) {
   if (nx < 2 || ny < 2 || nx > (1 << 20) || ny > (1 << 20) || limiter < 0 || limiter > 3 ||
-      nxs < 1)
+      nxs < 1 || !(eta >= 0.0) || !(kpar >= 0.0) || !(kperp >= 0.0))
     return WB_EINVAL;
   Stage2Args& a = *(Stage2Args*)args;
   a.in = in;
@@ -29,18 +29,23 @@ extern "C" __attribute__((visibility("hidden"))) int wb_fill_stage_args2d(
   a.heun = heun;
   a.nxs = nxs;
   a.walls = walls;
+  a.diff = Diffusion{eta, kpar, kperp};
   return 0;
 }
 
 extern "C" int wb_mhd2d_stage(const double* in, double* out, int64_t nx, int64_t ny, double dx,
-                              double dy, double b0x, double b0y, double gamma, int limiter,
-                              const double* dt, int heun, int wall_mask, int32_t* bad, int nxs,
-                              int tile, void* stream) {
+                              double dy, double b0x, double b0y, double gamma, double eta,
+                              double kpar, double kperp, int limiter, const double* dt, int heun,
+                              int wall_mask, int32_t* bad, int nxs, int tile, void* stream) {
   Stage2Args a;
-  const int rc = wb_fill_stage_args2d(&a, in, out, nx, ny, dx, dy, b0x, b0y, gamma, limiter, dt,
-                                      heun, wall_mask, bad, nxs);
+  const int rc = wb_fill_stage_args2d(&a, in, out, nx, ny, dx, dy, b0x, b0y, gamma, eta, kpar,
+                                      kperp, limiter, dt, heun, wall_mask, bad, nxs);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
+  if (a.diff.conduction() || eta != 0.0) {
+    if (tile == 62) return launch_stage2d<double, 62, -1, true>(a, s);
+    return launch_stage2d<double, 126, -1, true>(a, s);
+  }
   if (tile == 62) return launch_stage2d<double, 62>(a, s);
   if (tile == 254) return launch_stage2d<double, 254>(a, s);
   return launch_stage2d<double, 126>(a, s);
@@ -48,37 +53,46 @@ extern "C" int wb_mhd2d_stage(const double* in, double* out, int64_t nx, int64_t
 
 // All patches of a periodic 2D hierarchy in one persistent launch per stage, and the
 // refluxing of the coarse cells next to finer patches.
+template <int TY, bool NI>
+static int launch_batched(const Stage2Args& a, const void* patches, const void* items, int nitems,
+                          int32_t* counter, double* out, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL((k_stage2d_batched<double, TY, -1, NI>), dim3(grid), dim3(Row<TY>::THREADS),
+                     0, s, a, (const PatchDesc*)patches, (const WorkItem*)items, nitems,
+                     (int*)counter, out);
+  return wb_last_error();
+}
+
 extern "C" int wb_amr2d_stage(const double* in, double* out, const void* patches,
-                                    const void* items, int nitems, int32_t* counter, double b0x,
-                                    double b0y, double gamma, int limiter, const double* dt,
-                                    int heun, int32_t* bad, int nxs, int tile, int workgroups,
-                                    void* stream) {
+                              const void* items, int nitems, int32_t* counter, double b0x,
+                              double b0y, double gamma, double eta, double kpar, double kperp,
+                              int limiter, const double* dt, int heun, int32_t* bad, int nxs,
+                              int tile, int workgroups, void* stream) {
   Stage2Args a;
-  const int rc = wb_fill_stage_args2d(&a, in, out, 2, 2, 1.0, 1.0, b0x, b0y, gamma, limiter, dt,
-                                      heun, 0, bad, nxs);
+  const int rc = wb_fill_stage_args2d(&a, in, out, 2, 2, 1.0, 1.0, b0x, b0y, gamma, eta, kpar,
+                                      kperp, limiter, dt, heun, 0, bad, nxs);
   if (rc) return rc;
   if (nitems <= 0) return 0;
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, counter, 1);
-  const PatchDesc* pd = (const PatchDesc*)patches;
-  const WorkItem* wi = (const WorkItem*)items;
   const unsigned grid = (unsigned)(workgroups > 0 ? workgroups : 1);
+  const bool ni = a.diff.conduction() || eta != 0.0;
   if (tile == 62)
-    hipLaunchKernelGGL((k_stage2d_batched<double, 62, -1>), dim3(grid), dim3(Row<62>::THREADS), 0, s, a,
-                       pd, wi, nitems, (int*)counter, out);
-  else
-    hipLaunchKernelGGL((k_stage2d_batched<double, 126, -1>), dim3(grid), dim3(Row<126>::THREADS), 0, s,
-                       a, pd, wi, nitems, (int*)counter, out);
-  return wb_last_error();
+    return ni ? launch_batched<62, true>(a, patches, items, nitems, counter, out, grid, s)
+              : launch_batched<62, false>(a, patches, items, nitems, counter, out, grid, s);
+  return ni ? launch_batched<126, true>(a, patches, items, nitems, counter, out, grid, s)
+            : launch_batched<126, false>(a, patches, items, nitems, counter, out, grid, s);
 }
 
 extern "C" int wb_amr2d_reflux(const double* src, double* dst, const void* patches,
-                                     const long long* rows, const int32_t* groups, int ngroups,
-                                     double b0x, double b0y, double gamma, int limiter,
-                                     const double* dt, int heun, void* stream) {
+                               const long long* rows, const int32_t* groups, int ngroups,
+                               double b0x, double b0y, double gamma, double eta, double kpar,
+                               double kperp, int limiter, const double* dt, int heun,
+                               void* stream) {
+  if (!(eta >= 0.0) || !(kpar >= 0.0) || !(kperp >= 0.0)) return WB_EINVAL;
   if (ngroups <= 0) return 0;
   hipLaunchKernelGGL((k_reflux2d<double, -1>), dim3((unsigned)((ngroups + 63) / 64)), dim3(64), 0,
                      (hipStream_t)stream, src, dst, (const PatchDesc*)patches, rows,
-                     (const int*)groups, ngroups, b0x, b0y, gamma, limiter, dt, heun);
+                     (const int*)groups, ngroups, b0x, b0y, gamma, limiter,
+                     Diffusion{eta, kpar, kperp}, dt, heun);
   return wb_last_error();
 }

@@ -1,25 +1,32 @@
 // The fused 2D stage with the reconstruction and the HLLD flux in FP32; the
-// primitive conversion, the CT average, A_z and the update stay FP64.
+// primitive conversion, the non-ideal face terms, the CT average, A_z and the update
+// stay FP64.
 #include "stage2d.hpp"
 
 using namespace wb;
 
-extern "C" __attribute__((visibility("hidden"))) int wb_fill_stage_args2d(void* args, const double* in, double* out, int64_t nx,
-                                    int64_t ny, double dx, double dy, double b0x, double b0y,
-                                    double gamma, int limiter, const double* dt, int heun,
-                                    int walls, int32_t* bad, int nxs);
+extern "C" __attribute__((visibility("hidden"))) int wb_fill_stage_args2d(
+    void* args, const double* in, double* out, int64_t nx, int64_t ny, double dx, double dy,
+    double b0x, double b0y, double gamma, double eta, double kpar, double kperp, int limiter,
+    const double* dt, int heun, int walls, int32_t* bad, int nxs);
 
 extern "C" int wb_mhd2d_stage_mixed(const double* in, double* out, int64_t nx, int64_t ny,
                                     double dx, double dy, double b0x, double b0y, double gamma,
-                                    int limiter, const double* dt, int heun, int wall_mask,
-                                    int32_t* bad, int nxs, int tile, void* stream) {
+                                    double eta, double kpar, double kperp, int limiter,
+                                    const double* dt, int heun, int wall_mask, int32_t* bad,
+                                    int nxs, int tile, void* stream) {
   Stage2Args a;
-  const int rc = wb_fill_stage_args2d(&a, in, out, nx, ny, dx, dy, b0x, b0y, gamma, limiter, dt,
-                                      heun, wall_mask, bad, nxs);
+  const int rc = wb_fill_stage_args2d(&a, in, out, nx, ny, dx, dy, b0x, b0y, gamma, eta, kpar,
+                                      kperp, limiter, dt, heun, wall_mask, bad, nxs);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
+  const bool ni = a.diff.conduction() || eta != 0.0;
   return with_limiter(limiter, [&](auto lim) {
     constexpr int L = decltype(lim)::value;
+    if (ni) {
+      if (tile == 62) return launch_stage2d<float, 62, L, true>(a, s);
+      return launch_stage2d<float, 126, L, true>(a, s);
+    }
     if (tile == 62) return launch_stage2d<float, 62, L>(a, s);
     if (tile == 254) return launch_stage2d<float, 254, L>(a, s);
     return launch_stage2d<float, 126, L>(a, s);
@@ -28,43 +35,53 @@ extern "C" int wb_mhd2d_stage_mixed(const double* in, double* out, int64_t nx, i
 
 // All patches of a periodic 2D hierarchy in one persistent launch per stage, and the
 // refluxing of the coarse cells next to finer patches.
+template <int TY, int L, bool NI>
+static int launch_batched(const Stage2Args& a, const void* patches, const void* items, int nitems,
+                          int32_t* counter, double* out, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL((k_stage2d_batched<float, TY, L, NI>), dim3(grid), dim3(Row<TY>::THREADS), 0,
+                     s, a, (const PatchDesc*)patches, (const WorkItem*)items, nitems,
+                     (int*)counter, out);
+  return wb_last_error();
+}
+
 extern "C" int wb_amr2d_stage_mixed(const double* in, double* out, const void* patches,
                                     const void* items, int nitems, int32_t* counter, double b0x,
-                                    double b0y, double gamma, int limiter, const double* dt,
-                                    int heun, int32_t* bad, int nxs, int tile, int workgroups,
+                                    double b0y, double gamma, double eta, double kpar,
+                                    double kperp, int limiter, const double* dt, int heun,
+                                    int32_t* bad, int nxs, int tile, int workgroups,
                                     void* stream) {
   Stage2Args a;
-  const int rc = wb_fill_stage_args2d(&a, in, out, 2, 2, 1.0, 1.0, b0x, b0y, gamma, limiter, dt,
-                                      heun, 0, bad, nxs);
+  const int rc = wb_fill_stage_args2d(&a, in, out, 2, 2, 1.0, 1.0, b0x, b0y, gamma, eta, kpar,
+                                      kperp, limiter, dt, heun, 0, bad, nxs);
   if (rc) return rc;
   if (nitems <= 0) return 0;
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, counter, 1);
-  const PatchDesc* pd = (const PatchDesc*)patches;
-  const WorkItem* wi = (const WorkItem*)items;
   const unsigned grid = (unsigned)(workgroups > 0 ? workgroups : 1);
+  const bool ni = a.diff.conduction() || eta != 0.0;
   return with_limiter(limiter, [&](auto lim) {
     constexpr int L = decltype(lim)::value;
     if (tile == 62)
-      hipLaunchKernelGGL((k_stage2d_batched<float, 62, L>), dim3(grid), dim3(Row<62>::THREADS), 0,
-                         s, a, pd, wi, nitems, (int*)counter, out);
-    else
-      hipLaunchKernelGGL((k_stage2d_batched<float, 126, L>), dim3(grid), dim3(Row<126>::THREADS),
-                         0, s, a, pd, wi, nitems, (int*)counter, out);
-    return wb_last_error();
+      return ni ? launch_batched<62, L, true>(a, patches, items, nitems, counter, out, grid, s)
+                : launch_batched<62, L, false>(a, patches, items, nitems, counter, out, grid, s);
+    return ni ? launch_batched<126, L, true>(a, patches, items, nitems, counter, out, grid, s)
+              : launch_batched<126, L, false>(a, patches, items, nitems, counter, out, grid, s);
   });
 }
 
 extern "C" int wb_amr2d_reflux_mixed(const double* src, double* dst, const void* patches,
                                      const long long* rows, const int32_t* groups, int ngroups,
-                                     double b0x, double b0y, double gamma, int limiter,
-                                     const double* dt, int heun, void* stream) {
+                                     double b0x, double b0y, double gamma, double eta,
+                                     double kpar, double kperp, int limiter, const double* dt,
+                                     int heun, void* stream) {
+  if (!(eta >= 0.0) || !(kpar >= 0.0) || !(kperp >= 0.0)) return WB_EINVAL;
   if (ngroups <= 0) return 0;
   return with_limiter(limiter, [&](auto lim) {
     constexpr int L = decltype(lim)::value;
     hipLaunchKernelGGL((k_reflux2d<float, L>), dim3((unsigned)((ngroups + 63) / 64)), dim3(64), 0,
                        (hipStream_t)stream, src, dst, (const PatchDesc*)patches, rows,
-                       (const int*)groups, ngroups, b0x, b0y, gamma, limiter, dt, heun);
+                       (const int*)groups, ngroups, b0x, b0y, gamma, limiter,
+                       Diffusion{eta, kpar, kperp}, dt, heun);
     return wb_last_error();
   });
 }

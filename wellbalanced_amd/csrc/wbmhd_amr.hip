@@ -92,9 +92,10 @@ __global__ void k_restrict(double* __restrict__ buf, const long long* __restrict
 __global__ void __launch_bounds__(256) k_rate_amr(const double* __restrict__ buf,
                                                   const PatchDesc* __restrict__ patches,
                                                   const int* __restrict__ chunks, int nchunks,
-                                                  double b0x, double b0y, double gamma,
+                                                  double b0x, double b0y, double gamma, double eta,
+                                                  double cond, const double* __restrict__ spatch,
                                                   unsigned long long* keys) {
-  double rate = 0.0, r = __builtin_inf();
+  double rate = 0.0, r = __builtin_inf(), ds = 0.0;
   for (int ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
     const PatchDesc pd = patches[chunks[2 * ch]];
     const int t = chunks[2 * ch + 1] + (int)threadIdx.x;
@@ -119,8 +120,32 @@ __global__ void __launch_bounds__(256) k_rate_amr(const double* __restrict__ buf
     // NaN rates and densities drop out, as in the single-patch reduction
     rate = cr > rate ? cr : rate;
     r = rc < r ? rc : r;
+    if (spatch) {  // the diffusion number D s of the cell's patch (mhd.py:955-961)
+      const double c = cond / rc;
+      const double d = (c > eta ? c : eta) * spatch[chunks[2 * ch]];
+      ds = d > ds ? d : ds;
+    }
+  }
+  if (spatch) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_xor(ds, off, 64);
+      ds = o > ds ? o : ds;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(&keys[2], order_key(ds));
   }
   block_reduce_rate_rho_once(rate, r, keys);
+}
+
+// keys[2] (the largest D s) starts at key(0); finished into out[2].
+__global__ void k_rate3_init(unsigned long long* keys) {
+  if (threadIdx.x == 0) keys[2] = order_key(0.0);
+}
+__global__ void k_rate3_finish(double* out) {
+  if (threadIdx.x == 0) {
+    const unsigned long long k = ((const unsigned long long*)out)[2];
+    out[2] = key_to_double(k);
+  }
 }
 
 inline dim3 blocks(int n) { return dim3((unsigned)((n + 255) / 256)); }
@@ -160,15 +185,18 @@ int wb_amr2d_restrict(double* buf, const long long* rows, int n, void* stream) {
 }
 
 int wb_amr2d_rate(const double* buf, const void* patches, const int32_t* chunks, int nchunks,
-                  double b0x, double b0y, double gamma, double* out, void* stream) {
-  if (nchunks < 1) return WB_EINVAL;
+                  double b0x, double b0y, double gamma, double eta, double cond,
+                  const double* spatch, double* out, void* stream) {
+  if (nchunks < 1 || !(eta >= 0.0) || !(cond >= 0.0)) return WB_EINVAL;
   hipStream_t s = (hipStream_t)stream;
   unsigned long long* keys = (unsigned long long*)out;
   hipLaunchKernelGGL(k_rate_init, dim3(1), dim3(64), 0, s, keys);
+  if (spatch) hipLaunchKernelGGL(k_rate3_init, dim3(1), dim3(64), 0, s, keys);
   const unsigned grid = (unsigned)(nchunks < 1024 ? nchunks : 1024);
   hipLaunchKernelGGL(k_rate_amr, dim3(grid), dim3(256), 0, s, buf, (const PatchDesc*)patches,
-                     (const int*)chunks, nchunks, b0x, b0y, gamma, keys);
+                     (const int*)chunks, nchunks, b0x, b0y, gamma, eta, cond, spatch, keys);
   hipLaunchKernelGGL(k_rate_finish, dim3(1), dim3(64), 0, s, keys, out);
+  if (spatch) hipLaunchKernelGGL(k_rate3_finish, dim3(1), dim3(64), 0, s, out);
   return wb_last_error();
 }
 

@@ -91,19 +91,17 @@ class MHD2D:
             if kind == "wall" and mean_field[axis] != 0.0:
                 raise ValueError(f"a wall normal to {axis_name} needs a zero mean field along it")
             kinds[axis_name] = kind
-        if resistivity or max(conduction) > 0.0 or any(gravity):
-            raise NotImplementedError(
-                "the resistive, conductive and gravity terms are outside this ideal-MHD path"
-            )
+        if any(gravity):
+            raise NotImplementedError("the gravity source is outside this MHD path")
         if len(hierarchy.patches) != 1:
             raise NotImplementedError("refined patches run through wellbalanced_amd.amr.AMRMHD2D")
         self.hierarchy = hierarchy
         self.gamma = float(gamma)
         self.mean_field = (float(mean_field[0]), float(mean_field[1]))
-        self.resistivity = 0.0
+        self.resistivity = float(resistivity)
         self.limiter = limiter
         self.cfl = float(cfl)
-        self.conduction = (0.0, 0.0)
+        self.conduction = (float(conduction[0]), float(conduction[1]))
         self.gravity = (0.0, 0.0)
         self.walls = tuple(kinds[name] == "wall" for name in "xy")
         self.periodic = tuple(hierarchy.periodic)
@@ -249,6 +247,13 @@ class MHD2D:
         sx, sy = (X - 3) * Y * 8, X * (Y - 3) * 8
         call("wb_ct_contact", fxp + 5 * sx, fyp + 5 * sy, fxp, fyp, ptr(self._ecc), X, Y,
              ptr(self._ez), stream)
+        # mhd.py:868-871: the heat flux, then the resistive terms, before the walls
+        if max(self.conduction) > 0.0:
+            call("wb_conductive2d", ptr(self._prim), ptr(self._bxf), ptr(self._byf), X, Y,
+                 self.dx, self.dy, *self.conduction, fxp, fyp, stream)
+        if self.resistivity:
+            call("wb_resistive2d", ptr(self._prim), ptr(self._bxf), ptr(self._byf), X, Y,
+                 self.dx, self.dy, self.resistivity, fxp, fyp, ptr(self._ez), stream)
         if self._wall_mask():
             call("wb_mhd2d_walls", fxp, fyp, ptr(self._ez), nx, ny, self._wall_mask(), stream)
         call("wb_mhd2d_update", ptr(src), ptr(dst), fxp, fyp, ptr(self._ez), nx, ny,
@@ -261,12 +266,30 @@ class MHD2D:
                   self.X, self.Y, HALO, self.dx, self.dy, *self.mean_field, self.gamma,
                   _lib.ptr(self._rate), stream)
 
+    @property
+    def _diffusive(self) -> bool:
+        return bool(self.resistivity) or max(self.conduction) > 0.0
+
+    def _dt_kernel(self, remaining: float, time_ptr, stream: int) -> None:
+        """dt on the device from the rate: cfl / (largest fast-wave rate), and with a
+        diffusivity the explicit diffusion limit (mhd.py:931-963)."""
+        if self._diffusive:
+            _lib.call("wb_stable_dt_diffusive", _lib.ptr(self._rate), self.cfl, remaining,
+                      self.resistivity, (self.gamma - 1.0) * max(self.conduction),
+                      self.dx**-2 + self.dy**-2, _lib.ptr(self._dt), time_ptr, stream)
+        else:
+            _lib.call("wb_stable_dt", _lib.ptr(self._rate), self.cfl, remaining,
+                      _lib.ptr(self._dt), time_ptr, stream)
+
     def stable_step(self) -> float:
-        """cfl / (largest fast-wave rate) (mhd.py:930-963); synchronises."""
+        """cfl times the smallest of the fast-wave crossing time of a cell and, with
+        resistivity or conduction, the explicit diffusion limit (mhd.py:930-963);
+        synchronises."""
         self._check_ready()
-        self._rate_kernel(_lib.stream_handle(self.device))
-        largest = float(self._rate[0].item())
-        return self.cfl * (1.0 / largest)
+        stream = _lib.stream_handle(self.device)
+        self._rate_kernel(stream)
+        self._dt_kernel(0.0, None, stream)
+        return float(self._dt[0].item())
 
     def _stage_fused(self, src: torch.Tensor, dst: torch.Tensor, heun: bool, slot: int,
                      stream: int) -> None:
@@ -275,7 +298,8 @@ class MHD2D:
         _lib.call("wb_mhd2d_fill_halos", _lib.ptr(src), self.nx, self.ny, pmask, wmask, stream)
         name = "wb_mhd2d_stage" if self.precision == "fp64" else "wb_mhd2d_stage_mixed"
         _lib.call(name, _lib.ptr(src), _lib.ptr(dst), self.nx, self.ny, self.dx, self.dy,
-                  *self.mean_field, self.gamma, LIMITER_CODES.index(self.limiter),
+                  *self.mean_field, self.gamma, self.resistivity, *self.conduction,
+                  LIMITER_CODES.index(self.limiter),
                   _lib.ptr(self._dt), int(heun), wmask, self._bad.data_ptr() + 4 * slot,
                   int(self.nxs), int(self.tile), stream)
 
@@ -319,8 +343,7 @@ class MHD2D:
         marks = [] if events else None
         for _ in range(steps):
             self._rate_kernel(stream)
-            _lib.call("wb_stable_dt", _lib.ptr(self._rate), self.cfl, 0.0, _lib.ptr(self._dt),
-                      _lib.ptr(self._t), stream)
+            self._dt_kernel(0.0, _lib.ptr(self._t), stream)
             self._heun(stream, marks)
             self.steps += 1
         return marks
