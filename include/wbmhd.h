@@ -293,6 +293,44 @@ int wb_amr2d_average(double* buf, const long long* rows, int n, void* stream);
 /* Coarse A_z nodes := the fine node on them (rows of 2 int64), mhd.py:981,991. */
 int wb_amr2d_restrict(double* buf, const long long* rows, int n, void* stream);
 
+/* Regridding (MHD2D.regrid, mhd.py:1043-1056): copy into the new buffer `buf` the
+ * cells (6 fields; offsets and planes in the new and the old buffer) and A_z nodes
+ * that a patch of the same level held in the old buffer. */
+int wb_amr2d_regrid_copy(const double* old, double* buf, const long long* cdst,
+                         const long long* csrc, const long long* cpd, const long long* cps,
+                         int ncells, const long long* ndst, const long long* nsrc, int nnodes,
+                         void* stream);
+
+/* Regridding: the new cells by conservative linear prolongation from their coarse
+ * cell c (row length cy, plane pc; minmod slopes, sx/sy = -1 or +1 the child), and
+ * the new A_z nodes bilinear from the coarse nodes at nbase (row length nsx; kinds
+ * nkx/nky 1: the mean with the next node along that axis). */
+int wb_amr2d_prolong(double* buf, const long long* dst, const long long* pd, const long long* c,
+                     const long long* cy, const long long* pc, const long long* sx,
+                     const long long* sy, int ncells, const long long* ndst,
+                     const long long* nbase, const long long* nsx, const long long* nkx,
+                     const long long* nky, int nnodes, void* stream);
+
+/* Diagnostics of a hierarchy on the device (mhd.py:1059-1170).  leaf: int32 pairs
+ * (patch, haloed local cell a * (ny + 4) + b) of the leaf cells.
+ * wb_amr2d_leaf_values: out (9, nleaf) = density, velocity x/y/z, pressure, field
+ * x/y/z and the mean current J_z of the cell's four corners (halos filled first).
+ * wb_amr2d_totals: out[6] = the integrals of the 6 conserved fields over the leaf
+ * cells, by a fixed two-pass tree (the same bits on every run); workspace: device
+ * doubles, wb_amr2d_totals_workspace() of them.
+ * wb_amr2d_divergence: out[3] = the largest |div B| times min(dx, dy) over every
+ * patch cell, the largest |coarse node - fine node| over the restriction rows (pairs
+ * of int64 offsets), the largest A_z difference of neighbouring nodes of the patches
+ * in fine_chunks (composite divergence = out[1] / out[2]). */
+int wb_amr2d_leaf_values(const double* buf, const void* patches, const int32_t* leaf, int nleaf,
+                         double b0x, double b0y, double gamma, double* out, void* stream);
+int wb_amr2d_totals_workspace(void);
+int wb_amr2d_totals(const double* buf, const void* patches, const int32_t* leaf, int nleaf,
+                    double* workspace, double* out, void* stream);
+int wb_amr2d_divergence(const double* buf, const void* patches, const int32_t* chunks,
+                        int nchunks, double b0x, double b0y, const long long* rows, int nrows,
+                        const int32_t* fine_chunks, int nfine, double* out, void* stream);
+
 /* The CFL rate over every patch cell (mhd.py:930-963): out = device double[2], the
  * largest rate and the smallest density; with spatch (per patch dx^-2 + dy^-2, or
  * NULL) out[2] = the largest max(eta, cond / rho) * s of a cell (its patch's

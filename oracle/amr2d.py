@@ -233,3 +233,42 @@ def fill_ghosts_exact(buf: np.ndarray, maps, level: int) -> None:
     dst, base, sx, sy, kx, ky = maps.node_interp[level]
     for n in range(len(dst)):
         buf[dst[n]] = node_interp_value(buf, base[n], sx[n], sy[n], kx[n], ky[n])
+
+
+def _minmod(a, b):
+    return np.where(a * b > 0.0, np.where(np.abs(a) < np.abs(b), a, b), 0.0)
+
+
+def regrid_state(old_buf, new_maps, tables):
+    """The new hierarchy's buffer after a regrid, from the old buffer and the transfer
+    tables of ``wellbalanced_amd.regrid.transfer_tables``: per level, coarse to fine,
+    the same-level copies, the conservative minmod prolongation of the new cells, the
+    bilinear new nodes and the ghost fill; then average-down and restriction."""
+    buf = np.zeros(new_maps.total)
+    nlev = max(lay.level for lay in new_maps.layouts) + 1
+    for L in range(nlev):
+        t = tables[L]
+        cd, cs, cpd, cps = t["copy_cells"]
+        for f in range(6):
+            buf[cd + f * cpd] = old_buf[cs + f * cps]
+        nd, ns = t["copy_nodes"]
+        buf[nd] = old_buf[ns]
+        d, pd, c, cy, pc, sx, sy = t["cells"]
+        hx, hy = 0.25 * sx.astype(np.float64), 0.25 * sy.astype(np.float64)
+        for f in range(6):
+            q = c + f * pc
+            c0 = buf[q]
+            slx = _minmod(buf[q + cy] - c0, c0 - buf[q - cy])
+            sly = _minmod(buf[q + 1] - c0, c0 - buf[q - 1])
+            buf[d + f * pd] = (c0 + hx * slx) + hy * sly
+        qd, qb, qs, kx, ky = t["nodes"]
+        last = len(buf) - 1
+        a0 = buf[qb]
+        ax = buf[np.minimum(qb + qs, last)]
+        ay = buf[np.minimum(qb + 1, last)]
+        axy = buf[np.minimum(qb + qs + 1, last)]
+        both = 0.25 * ((a0 + ax) + (ay + axy))
+        buf[qd] = np.where((kx == 1) & (ky == 1), both,
+                           np.where(kx == 1, 0.5 * (a0 + ax), np.where(ky == 1, 0.5 * (a0 + ay), a0)))
+        fill_ghosts_exact(buf, new_maps, L)
+    return buf

@@ -1,0 +1,263 @@
+"""Regridding a periodic 2D hierarchy on tags (MHD2D.regrid, mhd.py:1043-1056): the
+clustering and nesting of the new patches, the move of the state (integrals kept,
+div B zero, linear data exact) on the CPU, and the device transfer against the
+oracle's restatement plus the reference's own regrid test (test_mhd_amr.py:69-90) on
+the GPU."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle.amr2d import OracleAMR2D, regrid_state
+from wellbalanced_amd import amr_maps, initial
+from wellbalanced_amd import regrid as rg
+from wellbalanced_amd.hierarchy import AMRHierarchy
+
+
+def _boxes_cover(boxes, flags):
+    cov = np.zeros(flags.shape, int)
+    for (i0, i1), (j0, j1) in boxes:
+        cov[i0:i1, j0:j1] += 1
+    return cov
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_clusters_cover_every_tag_once_within_the_size_limit(seed):
+    rng = np.random.default_rng(seed)
+    flags = np.zeros((48, 40), bool)
+    for _ in range(5):
+        i, j = rng.integers(0, 40, 2)
+        flags[i : i + rng.integers(1, 9), j : j + rng.integers(1, 12)] = True
+    flags |= rng.random(flags.shape) < 0.01
+    boxes = rg.cluster(flags, efficiency=0.7, max_size=8)
+    cov = _boxes_cover(boxes, flags)
+    assert np.all(cov[flags] == 1) and cov.max() == 1
+    assert all(i1 - i0 <= 8 and j1 - j0 <= 8 for (i0, i1), (j0, j1) in boxes)
+
+
+def test_a_full_block_is_one_box_and_two_blobs_are_two():
+    flags = np.zeros((32, 32), bool)
+    flags[4:10, 6:12] = True
+    assert rg.cluster(flags) == [((4, 10), (6, 12))]
+    flags[20:24, 20:30] = True
+    assert rg.cluster(flags) == [((4, 10), (6, 12)), ((20, 24), (20, 30))]
+
+
+def _nested(h, nest=rg.NEST):
+    """Every patch of level L + 1, its footprint widened by ``nest`` cells of level L,
+    lies on level L."""
+    for L in range(1, h.max_level + 1):
+        shape = h.level_shape(L - 1)
+        cover = np.zeros(shape, bool)
+        for p in h.patches_at_level(L - 1):
+            (i0, i1), (j0, j1) = p.box
+            cover[i0:i1, j0:j1] = True
+        for p in h.patches_at_level(L):
+            (i0, i1), (j0, j1) = p.box
+            foot = np.zeros(shape, bool)
+            foot[i0 // 2 : i1 // 2, j0 // 2 : j1 // 2] = True
+            if L - 1 > 0 and not cover[rg.dilate(foot, nest)].all():
+                return False
+    return True
+
+
+def _two_level_regrid():
+    h = AMRHierarchy((16, 16), periodic=(True, True))
+    tags = {0: np.zeros((16, 16), bool)}
+    tags[0][6:9, 5:12] = True
+    h1 = rg.regrid(h, tags, max_patch_size=16)
+    tags1 = {0: tags[0], 1: np.zeros((32, 32), bool)}
+    tags1[1][14:16, 12:18] = True
+    tags1[1][20, 3] = True  # outside level 1: the nesting grows level 1 around it
+    h2 = rg.regrid(h1, tags1, max_patch_size=16)
+    return h, h1, h2, tags1
+
+
+def test_regridded_levels_nest_and_build_tables():
+    _, h1, h2, tags1 = _two_level_regrid()
+    assert h1.max_level == 1 and h2.max_level == 2
+    assert _nested(h2)
+    amr_maps.build(h2)
+    cover = np.zeros(h2.level_shape(2), bool)
+    for p in h2.patches_at_level(2):
+        (i0, i1), (j0, j1) = p.box
+        cover[i0:i1, j0:j1] = True
+    fine_tags = np.repeat(np.repeat(tags1[1], 2, axis=0), 2, axis=1)
+    assert cover[fine_tags].all()
+    assert all(p.shape[0] <= 16 and p.shape[1] <= 16 for p in h2.patches if p.level > 0)
+
+
+def _state(kind):
+    if kind == "linear":  # reproduced exactly by the limited linear prolongation
+        return dict(density=lambda x, y: 2.0 + 0.25 * x - 0.5 * y, velocity=(0.0, 0.0, 0.0),
+                    pressure=1.0, vector_potential=lambda x, y: 0.3 * x - 0.2 * y + 0.1 * x * y,
+                    bz=lambda x, y: 0.1 * x)
+    return dict(density=lambda x, y: 1.0 + 0.3 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y),
+                velocity=lambda x, y: (np.sin(2 * np.pi * y), 0.5 * np.cos(2 * np.pi * x), 0.1),
+                pressure=lambda x, y: 1.0 + 0.2 * np.cos(2 * np.pi * (x + y)),
+                vector_potential=lambda x, y: 0.05 * np.cos(2 * np.pi * x) * np.sin(4 * np.pi * y),
+                bz=lambda x, y: 0.2 * np.sin(2 * np.pi * x))
+
+
+def _oracle_on(h, state):
+    maps = amr_maps.build(h)
+    buf = amr_maps.pack_state(h, maps.layouts, maps.total, 5.0 / 3.0, (0.0, 0.0), state)
+    o = OracleAMR2D(maps, h, buf)
+    for L in range(o.nlev - 2, -1, -1):
+        o.average(o.buf, L)
+        o.restrict(o.buf, L)
+    return maps, o
+
+
+def test_the_move_keeps_the_integrals_and_div_b():
+    _, h1, h2, _ = _two_level_regrid()
+    _, before = _oracle_on(h1, _state("smooth"))
+    before.step()
+    maps2 = amr_maps.build(h2)
+    tables = rg.transfer_tables(h1, h2, maps2)
+    after = OracleAMR2D(maps2, h2, regrid_state(before.buf, maps2, tables))
+    for L in range(after.nlev - 2, -1, -1):
+        after.average(after.buf, L)
+        after.restrict(after.buf, L)
+    np.testing.assert_allclose(after.leaf_totals(), before.leaf_totals(), rtol=0, atol=1e-15)
+    # div B on every patch and coarse nodes equal to the fine nodes on them
+    for lay in maps2.layouts:
+        az = after.views(after.buf, lay)[1][2:-2, 2:-2]
+        dx, dy = h2.spacing(lay.level)
+        bxf = (az[:, 1:] - az[:, :-1]) / dy
+        byf = -(az[1:, :] - az[:-1, :]) / dx
+        div = (bxf[1:] - bxf[:-1]) / dx + (byf[:, 1:] - byf[:, :-1]) / dy
+        assert np.abs(div).max() * dx < 1e-14
+    for L in range(after.nlev - 1):
+        rows = maps2.restrict[L]
+        assert np.array_equal(after.buf[rows[:, 0]], after.buf[rows[:, 1]])
+
+
+def test_linear_data_moves_exactly_onto_new_patches():
+    """A linear density and a bilinear potential are reproduced on the new level 2 to
+    round-off: the prolongation and the node interpolation are exact for them."""
+    _, h1, h2, _ = _two_level_regrid()
+    _, before = _oracle_on(h1, _state("linear"))
+    maps2 = amr_maps.build(h2)
+    moved = regrid_state(before.buf, maps2, rg.transfer_tables(h1, h2, maps2))
+    exact = amr_maps.pack_state(h2, maps2.layouts, maps2.total, 5.0 / 3.0, (0.0, 0.0),
+                                _state("linear"))
+    for lay in maps2.layouts:
+        if lay.level != 2:
+            continue
+        o = OracleAMR2D(maps2, h2, moved)
+        e = OracleAMR2D(maps2, h2, exact)
+        (c1, a1), (c2, a2) = o.views(moved, lay), e.views(exact, lay)
+        np.testing.assert_allclose(c1[0, 2:-2, 2:-2], c2[0, 2:-2, 2:-2], rtol=0, atol=1e-14)
+        np.testing.assert_allclose(a1[2:-2, 2:-2], a2[2:-2, 2:-2], rtol=0, atol=1e-14)
+
+
+# -- GPU ----------------------------------------------------------------------------------
+KW = 2.0 * np.pi
+
+
+def _orszag_tang_state():
+    """The Orszag-Tang state of test_mhd_amr.py:52-66."""
+    return dict(
+        density=25.0 / (36.0 * np.pi),
+        velocity=(lambda x, y: -np.sin(KW * y), lambda x, y: np.sin(KW * x), 0.0),
+        pressure=5.0 / (12.0 * np.pi),
+        vector_potential=lambda x, y: (np.cos(2 * KW * x) / (2 * KW) + np.cos(KW * y) / KW)
+        / np.sqrt(4 * np.pi),
+        bz=lambda x, y: 0.1 * np.cos(KW * x),
+    )
+
+
+def _amr_orszag_tang(n, **options):
+    from wellbalanced_amd.amr import AMRMHD2D
+
+    h = AMRHierarchy((n, n), ((0.0, 1.0), (0.0, 1.0)), periodic=(True, True))
+    model = AMRMHD2D(h, **{"limiter": "vanleer", **options})
+    model.set_state(**_orszag_tang_state())
+    return model
+
+
+@pytest.mark.gpu
+def test_gpu_move_is_bitwise_the_oracle(gpu):
+    """Two regrids of a stepped state (new level 1, then level 2 and a changed level 1):
+    the device copies, prolongations, ghost fills and coarsening give the oracle's bits."""
+    model = _amr_orszag_tang(16, limiter="minmod")
+    for tags in ({0: np.pad(np.ones((4, 7), bool), ((6, 6), (5, 4)))},
+                 {0: np.pad(np.ones((3, 6), bool), ((7, 6), (4, 6))),
+                  1: np.pad(np.ones((2, 6), bool), ((14, 16), (12, 14)))}):
+        model.step(model.stable_step())
+        old_h, old_buf = model.hierarchy, model.buffer()
+        tags = {L: t for L, t in tags.items() if L <= old_h.max_level}
+        model.regrid(tags, max_patch_size=16)
+        maps = amr_maps.build(model.hierarchy)
+        want = regrid_state(old_buf, maps, rg.transfer_tables(old_h, model.hierarchy, maps))
+        o = OracleAMR2D(maps, model.hierarchy, want)
+        for L in range(o.nlev - 2, -1, -1):
+            o.average(o.buf, L)
+            o.restrict(o.buf, L)
+        got = model.buffer()
+        for lay in maps.layouts:  # the patches' own cells and nodes
+            sl = slice(lay.offset, lay.offset + 6 * lay.plane)
+            np.testing.assert_array_equal(got[sl].reshape(6, lay.X, lay.Y)[:, 2:-2, 2:-2],
+                                          o.buf[sl].reshape(6, lay.X, lay.Y)[:, 2:-2, 2:-2])
+            sl = slice(lay.offset + 6 * lay.plane, lay.offset + lay.size)
+            np.testing.assert_array_equal(got[sl].reshape(lay.X + 1, lay.Y + 1)[2:-2, 2:-2],
+                                          o.buf[sl].reshape(lay.X + 1, lay.Y + 1)[2:-2, 2:-2])
+    assert model.hierarchy.max_level == 2
+
+
+@pytest.mark.gpu
+def test_gpu_regrids_conserve_and_keep_the_field_divergence_free(gpu):
+    """test_mhd_amr.py:69-90: three regrids on the current, four steps after each."""
+    model = _amr_orszag_tang(16, resistivity=1e-3)
+    start = model.totals()
+    for _ in range(3):
+        before = model.totals()
+        model.regrid(model.current_tags(0.1, levels=2), max_patch_size=16)
+        after = model.totals()
+        for name in before:
+            assert abs(after[name] - before[name]) < 1e-14
+        assert model.composite_divergence() < 1e-12
+        for _ in range(4):
+            model.step()
+    assert model.hierarchy.max_level == 2
+    end = model.totals()
+    for name in start:
+        assert abs(end[name] - start[name]) < 1e-13
+    assert model.divergence() < 1e-12
+    assert model.composite_divergence() < 1e-12
+    leaf = model.leaf_values()
+    assert np.all(leaf["pressure"] > 0) and np.all(leaf["density"] > 0)
+    assert np.unique(leaf["level"]).max() == 2
+
+
+@pytest.mark.gpu
+def test_gpu_leaf_values_of_one_level_are_the_uniform_model(gpu):
+    """test_mhd_amr.py:93-105 against the uniform model's dataset."""
+    from wellbalanced_amd import MHD2D
+
+    amr = _amr_orszag_tang(16)
+    uni = MHD2D(AMRHierarchy((16, 16), ((0.0, 1.0), (0.0, 1.0)), periodic=(True, True)))
+    uni.set_state(**_orszag_tang_state())
+    leaf = amr.leaf_values()
+    data = uni.dataset()
+    order = np.lexsort((leaf["y"], leaf["x"]))
+    np.testing.assert_allclose(leaf["density"][order], data["density"].ravel(), rtol=1e-14)
+    np.testing.assert_allclose(leaf["magnetic_field_y"][order], data["magnetic_field_y"].ravel(),
+                               rtol=1e-14, atol=1e-15)
+    assert np.all(leaf["level"] == 0)
+
+
+@pytest.mark.gpu
+def test_gpu_diagnostics_match_the_host_sums(gpu):
+    """The device totals are the leaf sums to round-off; the leaf cells tile the domain."""
+    model = _amr_orszag_tang(16, limiter="minmod")
+    model.regrid({0: np.pad(np.ones((4, 7), bool), ((6, 6), (5, 4)))}, max_patch_size=16)
+    model.step(model.stable_step())
+    leaf = model.leaf_values()
+    h = model.hierarchy
+    area = np.array([np.prod(h.spacing(L)) for L in leaf["level"]])
+    assert abs(area.sum() - 1.0) < 1e-14
+    totals = model.totals()
+    assert abs(totals["density"] - np.sum(leaf["density"] * area)) < 1e-14

@@ -68,6 +68,12 @@ SIGNATURES: dict[str, tuple[type, list]] = {
     "wb_cnn_wgrad": (I, [P, I, P, I64, I64, I64, P, I, P, P]),
     "wb_amr2d_copy": (I, [P] * 5 + [I] + [P, P, I, P]),
     "wb_amr2d_interp": (I, [P] * 8 + [I] + [P] * 6 + [I, P]),
+    "wb_amr2d_regrid_copy": (I, [P] * 6 + [I, P, P, I, P]),
+    "wb_amr2d_prolong": (I, [P] * 8 + [I] + [P] * 5 + [I, P]),
+    "wb_amr2d_leaf_values": (I, [P, P, P, I, D, D, D, P, P]),
+    "wb_amr2d_totals_workspace": (I, []),
+    "wb_amr2d_totals": (I, [P, P, P, I, P, P, P]),
+    "wb_amr2d_divergence": (I, [P, P, P, I, D, D, P, I, P, I, P, P]),
     "wb_amr2d_average": (I, [P, P, I, P]),
     "wb_amr2d_restrict": (I, [P, P, I, P]),
     "wb_amr2d_rate": (I, [P, P, P, I, D, D, D, D, D, P, P, P]),

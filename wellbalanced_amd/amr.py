@@ -26,7 +26,7 @@ import torch
 
 from . import _lib, amr_maps
 from .hierarchy import AMRHierarchy
-from .mhd import HALO, LIMITER_CODES, PRECISIONS, _device
+from .mhd import CONSERVED, HALO, LIMITER_CODES, PRECISIONS, _device
 
 _PATCH = np.dtype([("off", np.int64), ("nx", np.int32), ("ny", np.int32),
                    ("dx", np.float64), ("dy", np.float64)])
@@ -67,11 +67,8 @@ class AMRMHD2D:
         self.tile = tile
         self.nxs = nxs
         self.device = _device(device)
-        self.maps = amr_maps.build(hierarchy)
-        self.layouts = self.maps.layouts
-        self.nlev = hierarchy.max_level + 1
+        self._use(hierarchy, amr_maps.build(hierarchy), upload=False)
         opts = dict(dtype=torch.float64, device=self.device)
-        self._buf = [torch.zeros(self.maps.total, **opts), torch.zeros(self.maps.total, **opts)]
         self._dt = torch.zeros(1, **opts)
         self._t = torch.zeros(1, **opts)
         self._rate = torch.zeros(3, **opts)
@@ -82,6 +79,17 @@ class AMRMHD2D:
         self.check_interval = 1
         self.steps = 0
         self._ready = False
+
+    def _use(self, hierarchy: AMRHierarchy, maps, upload: bool = True) -> None:
+        """Take ``hierarchy`` (its tables ``maps``) with new, zeroed state buffers."""
+        self.hierarchy = hierarchy
+        self.maps = maps
+        self.layouts = maps.layouts
+        self.nlev = hierarchy.max_level + 1
+        opts = dict(dtype=torch.float64, device=self.device)
+        self._buf = [torch.zeros(maps.total, **opts), torch.zeros(maps.total, **opts)]
+        if upload:
+            self._upload_tables()
 
     def set_tiling(self, tile: int | None = None, nxs: int | None = None,
                    workgroups: int | None = None) -> None:
@@ -151,6 +159,37 @@ class AMRMHD2D:
                   for c in range(0, lay.nx * lay.ny, 256)]
         self._chunks = self._dev(np.array(chunks, np.int32).reshape(-1), torch.int32)
         self.nchunks = len(chunks)
+        fine = [c for c in chunks if self.layouts[c[0]].level > 0]
+        self._fine_chunks = self._dev(np.array(fine, np.int32).reshape(-1), torch.int32)
+        self.nfine = len(fine)
+        # the leaf cells (not covered by a finer patch): patch, haloed local cell
+        covered = np.zeros(self.maps.total, bool)
+        for L in range(self.nlev - 1):
+            rows = self.maps.average[L]
+            if len(rows):
+                covered[rows[:, 0]] = True
+        leaf, self._leaf_xy = [], []
+        for k, lay in enumerate(self.layouts):
+            a, b = np.meshgrid(np.arange(lay.nx) + HALO, np.arange(lay.ny) + HALO, indexing="ij")
+            keep = ~covered[lay.cell(a, b)]
+            a, b = a[keep], b[keep]
+            leaf.append(np.stack([np.full(a.size, k), a * lay.Y + b], axis=1))
+            dx, dy = self.hierarchy.spacing(lay.level)
+            lo = self.hierarchy.lower_bound
+            (i0, _), (j0, _) = lay.box
+            self._leaf_xy.append(np.stack([lo[0] + (i0 + a - HALO + 0.5) * dx,
+                                           lo[1] + (j0 + b - HALO + 0.5) * dy,
+                                           np.full(a.size, lay.level)], axis=1))
+        leaf = np.concatenate(leaf).astype(np.int32)
+        self._leaf_xy = np.concatenate(self._leaf_xy)
+        self._leaf = self._dev(leaf.reshape(-1), torch.int32)
+        self.nleaf = len(leaf)
+        restrict = [self.maps.restrict[L] for L in range(self.nlev - 1) if len(self.maps.restrict[L])]
+        restrict = np.concatenate(restrict) if restrict else np.zeros((0, 2), np.int64)
+        self._restrict_all = self._dev(restrict.reshape(-1))
+        self.nrestrict = len(restrict)
+        self._work = torch.zeros(_lib.call("wb_amr2d_totals_workspace"), dtype=torch.float64,
+                                 device=self.device)
 
     # -- state ---------------------------------------------------------------------------
     def set_state(self, density, velocity, pressure, *, vector_potential=0.0, bz=0.0) -> None:
@@ -183,15 +222,18 @@ class AMRMHD2D:
         return int(sum(lay.nx * lay.ny for lay in self.layouts))
 
     # -- the step ------------------------------------------------------------------------
-    def _fill(self, buf: torch.Tensor, stream: int) -> None:
+    def _fill_level(self, buf: torch.Tensor, L: int, stream: int) -> None:
         p = _lib.ptr(buf)
+        cd, cs, cpd, cps, nd, ns = self._copy[L]
+        _lib.call("wb_amr2d_copy", p, _lib.ptr(cd), _lib.ptr(cs), _lib.ptr(cpd), _lib.ptr(cps),
+                  len(cd), _lib.ptr(nd), _lib.ptr(ns), len(nd), stream)
+        t = self._interp[L]
+        _lib.call("wb_amr2d_interp", p, *[_lib.ptr(x) for x in t[:7]], len(t[0]),
+                  *[_lib.ptr(x) for x in t[7:]], len(t[7]), stream)
+
+    def _fill(self, buf: torch.Tensor, stream: int) -> None:
         for L in range(self.nlev):
-            cd, cs, cpd, cps, nd, ns = self._copy[L]
-            _lib.call("wb_amr2d_copy", p, _lib.ptr(cd), _lib.ptr(cs), _lib.ptr(cpd), _lib.ptr(cps),
-                      len(cd), _lib.ptr(nd), _lib.ptr(ns), len(nd), stream)
-            t = self._interp[L]
-            _lib.call("wb_amr2d_interp", p, *[_lib.ptr(x) for x in t[:7]], len(t[0]),
-                      *[_lib.ptr(x) for x in t[7:]], len(t[7]), stream)
+            self._fill_level(buf, L, stream)
 
     def _coarsen(self, buf: torch.Tensor, L: int, stream: int) -> None:
         rows, n = self._average[L]
@@ -315,59 +357,95 @@ class AMRMHD2D:
             tags[lay.level][i0:i1, j0:j1] |= cell
         return tags
 
+    def regrid(self, tags: dict, **options) -> None:
+        """Regrid on ``tags`` (level -> bool cells, as `current_tags` gives them) and move
+        the state onto the new patches (mhd.py:1043-1056): same-level cells and nodes
+        are copied, new cells take the conservative prolongation of their coarse cell
+        (integrals kept), new A_z nodes the bilinear interpolation (div B stays zero).
+        ``options``: ``max_patch_size``, ``buffer``, ``efficiency`` of
+        `wellbalanced_amd.regrid.regrid`."""
+        if not self._ready:
+            raise RuntimeError("set the state first (set_state)")
+        from . import regrid as rg
+
+        old_h, old = self.hierarchy, self._buf[0]
+        new_h = rg.regrid(old_h, tags, **options)
+        maps = amr_maps.build(new_h)
+        tables = rg.transfer_tables(old_h, new_h, maps)
+        self._use(new_h, maps)
+        new = self._buf[0]
+        stream = _lib.stream_handle(self.device)
+        ptr = _lib.ptr
+        for L in range(self.nlev):
+            t = tables[L]
+            cd, cs, cpd, cps = (self._dev(x) for x in t["copy_cells"])
+            nd, ns = (self._dev(x) for x in t["copy_nodes"])
+            _lib.call("wb_amr2d_regrid_copy", ptr(old), ptr(new), ptr(cd), ptr(cs), ptr(cpd),
+                      ptr(cps), len(cd), ptr(nd), ptr(ns), len(nd), stream)
+            cells = [self._dev(x) for x in t["cells"]]
+            nodes = [self._dev(x) for x in t["nodes"]]
+            _lib.call("wb_amr2d_prolong", ptr(new), *(ptr(x) for x in cells), len(cells[0]),
+                      *(ptr(x) for x in nodes), len(nodes[0]), stream)
+            self._fill_level(new, L, stream)
+        for L in range(self.nlev - 2, -1, -1):
+            self._coarsen(new, L, stream)
+        torch.cuda.synchronize(self.device)
+
     def check(self) -> None:
         if int(self._bad.cpu().sum()):
             raise FloatingPointError(f"non-positive density or pressure at t = {self.time}")
 
-    # -- diagnostics ---------------------------------------------------------------------
+    # -- diagnostics (on the device; only the results come back) ----------------------
     def leaf_totals(self) -> np.ndarray:
-        """Integrals of the 6 conserved fields over the leaf cells."""
-        buf = self.buffer()
-        covered = np.zeros(self.maps.total, bool)
-        for L in range(self.nlev - 1):
-            rows = self.maps.average[L]
-            if len(rows):
-                covered[rows[:, 0]] = True
-        totals = np.zeros(6)
-        for lay in self.layouts:
-            dx, dy = self.hierarchy.spacing(lay.level)
-            a, b = np.meshgrid(np.arange(lay.nx) + HALO, np.arange(lay.ny) + HALO, indexing="ij")
-            offs = lay.cell(a, b)
-            keep = ~covered[offs]
-            for f in range(6):
-                totals[f] += float(np.sum(buf[offs + f * lay.plane][keep])) * (dx * dy)
-        return totals
+        """Integrals of the 6 conserved fields over the leaf cells (a fixed-order FP64
+        tree on the device)."""
+        out = torch.zeros(6, dtype=torch.float64, device=self.device)
+        _lib.call("wb_amr2d_totals", _lib.ptr(self._buf[0]), _lib.ptr(self._patches),
+                  _lib.ptr(self._leaf), self.nleaf, _lib.ptr(self._work), _lib.ptr(out),
+                  _lib.stream_handle(self.device))
+        return out.cpu().numpy()
+
+    def totals(self) -> dict[str, float]:
+        """Integrals of the conserved fields over the domain, from the leaf cells
+        (mhd.py:1172-1175)."""
+        return {name: float(v) for name, v in zip(CONSERVED, self.leaf_totals())}
+
+    def _divergences(self) -> np.ndarray:
+        out = torch.zeros(3, dtype=torch.float64, device=self.device)
+        _lib.call("wb_amr2d_divergence", _lib.ptr(self._buf[0]), _lib.ptr(self._patches),
+                  _lib.ptr(self._chunks), self.nchunks, *self.mean_field,
+                  _lib.ptr(self._restrict_all), self.nrestrict, _lib.ptr(self._fine_chunks),
+                  self.nfine, _lib.ptr(out), _lib.stream_handle(self.device))
+        return out.cpu().numpy()
 
     def composite_divergence(self) -> float:
-        """Largest mismatch of a coarse node and the fine node on it, relative to the
-        largest fine A_z difference (mhd.py:1133-1170): zero after every stage."""
-        buf = self.buffer()
-        worst, scale = 0.0, 0.0
-        for L in range(self.nlev - 1):
-            rows = self.maps.restrict[L]
-            if len(rows):
-                worst = max(worst, float(np.abs(buf[rows[:, 0]] - buf[rows[:, 1]]).max()))
-        for lay in self.layouts:
-            if lay.level == 0:
-                continue
-            az = buf[lay.offset + 6 * lay.plane : lay.offset + lay.size].reshape(lay.X + 1, lay.Y + 1)
-            a = az[HALO:-HALO, HALO:-HALO]
-            scale = max(scale, float(np.abs(np.diff(a, axis=0)).max()), float(np.abs(np.diff(a, axis=1)).max()))
-        return worst / scale if scale else 0.0
+        """The largest mismatch of a coarse node and the fine node on it, relative to the
+        largest A_z difference of neighbouring fine nodes (mhd.py:1133-1170): the
+        coarse faces' flux equals the sum of the fine faces' over them."""
+        _, worst, scale = self._divergences()
+        return float(worst / scale) if scale else 0.0
 
     def divergence(self) -> float:
-        """Largest |div B| times the cell size over the cells of every patch."""
-        buf = self.buffer()
-        largest = 0.0
-        for lay in self.layouts:
-            dx, dy = self.hierarchy.spacing(lay.level)
-            az = buf[lay.offset + 6 * lay.plane : lay.offset + lay.size].reshape(lay.X + 1, lay.Y + 1)
-            a = az[HALO:-HALO, HALO:-HALO]
-            bxf = (a[:, 1:] - a[:, :-1]) / dy + self.mean_field[0]
-            byf = -(a[1:, :] - a[:-1, :]) / dx + self.mean_field[1]
-            div = (bxf[1:] - bxf[:-1]) / dx + (byf[:, 1:] - byf[:, :-1]) / dy
-            largest = max(largest, float(np.abs(div).max()) * min(dx, dy))
-        return largest
+        """Largest |div B| times the cell size over the cells of every patch
+        (mhd.py:1114-1131)."""
+        return float(self._divergences()[0])
+
+    def leaf_values(self) -> dict[str, np.ndarray]:
+        """The primitive variables on the leaf cells (mhd.py:1059-1112): ``x``, ``y``,
+        ``level``, the variables of `MHD2D.dataset` and ``current_z`` (the mean of the
+        cell's corners), patch by patch in row-major order."""
+        stream = _lib.stream_handle(self.device)
+        self._fill(self._buf[0], stream)
+        out = torch.empty(9 * max(self.nleaf, 1), dtype=torch.float64, device=self.device)
+        _lib.call("wb_amr2d_leaf_values", _lib.ptr(self._buf[0]), _lib.ptr(self._patches),
+                  _lib.ptr(self._leaf), self.nleaf, *self.mean_field, self.gamma, _lib.ptr(out),
+                  stream)
+        v = out[: 9 * self.nleaf].view(9, self.nleaf).cpu().numpy()
+        names = ("density", "velocity_x", "velocity_y", "velocity_z", "pressure",
+                 "magnetic_field_x", "magnetic_field_y", "magnetic_field_z", "current_z")
+        return {"x": self._leaf_xy[:, 0].copy(), "y": self._leaf_xy[:, 1].copy(),
+                "level": self._leaf_xy[:, 2].astype(np.int64),
+                **{name: v[k].copy() for k, name in enumerate(names)}}
 
 
 def blast_hierarchy(base: int = 512, levels: int = 3, sizes=(16, 128), seed: int = 0) -> AMRHierarchy:

@@ -66,6 +66,76 @@ __global__ void k_interp_nodes(double* __restrict__ buf, const long long* __rest
   buf[dst[i]] = total;
 }
 
+// Regridding: the cells (6 fields) and nodes a patch of the same level held in the old
+// hierarchy's buffer, copied into the new buffer.
+__global__ void k_regrid_copy_cells(const double* __restrict__ old, double* __restrict__ buf,
+                                    const long long* __restrict__ dst,
+                                    const long long* __restrict__ src,
+                                    const long long* __restrict__ pd,
+                                    const long long* __restrict__ ps, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long long d = dst[i], s = src[i], a = pd[i], b = ps[i];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) buf[d + f * a] = old[s + f * b];
+}
+
+__global__ void k_regrid_copy_nodes(const double* __restrict__ old, double* __restrict__ buf,
+                                    const long long* __restrict__ dst,
+                                    const long long* __restrict__ src, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) buf[dst[i]] = old[src[i]];
+}
+
+__device__ __forceinline__ double minmod2(double a, double b) {
+  if (!(a * b > 0.0)) return 0.0;
+  return __builtin_fabs(a) < __builtin_fabs(b) ? a : b;
+}
+
+// Conservative linear prolongation of a new fine cell from its coarse cell c (row
+// length cy, plane pc): c + 0.25 sx s_x + 0.25 sy s_y with minmod slopes, sx/sy = -1
+// for the lower child, +1 for the upper one; the four children sum to 4 c.
+__global__ void k_prolong_cells(double* __restrict__ buf, const long long* __restrict__ dst,
+                                const long long* __restrict__ pd, const long long* __restrict__ c,
+                                const long long* __restrict__ cy, const long long* __restrict__ pc,
+                                const long long* __restrict__ sx, const long long* __restrict__ sy,
+                                int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long long d = dst[i], a = pd[i], row = cy[i], p = pc[i];
+  const double hx = 0.25 * (double)sx[i], hy = 0.25 * (double)sy[i];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    const double* q = buf + c[i] + f * p;
+    const double c0 = q[0];
+    const double slx = minmod2(q[row] - c0, c0 - q[-row]);
+    const double sly = minmod2(q[1] - c0, c0 - q[-1]);
+    buf[d + f * a] = (c0 + hx * slx) + hy * sly;
+  }
+}
+
+// Bilinear prolongation of a new fine node from the coarse nodes base, base + sx
+// (kind x 1), base + 1 (kind y 1): the coarse node, the mean of two or of four.
+__global__ void k_prolong_nodes(double* __restrict__ buf, const long long* __restrict__ dst,
+                                const long long* __restrict__ base, const long long* __restrict__ sx,
+                                const long long* __restrict__ kx, const long long* __restrict__ ky,
+                                int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* q = buf + base[i];
+  const long long s = sx[i];
+  double v;
+  if (kx[i] && ky[i])
+    v = 0.25 * ((q[0] + q[s]) + (q[1] + q[s + 1]));
+  else if (kx[i])
+    v = 0.5 * (q[0] + q[s]);
+  else if (ky[i])
+    v = 0.5 * (q[0] + q[1]);
+  else
+    v = q[0];
+  buf[dst[i]] = v;
+}
+
 // Covered coarse cell = mean of its four fine children, ((f00 + f10) + (f01 + f11)) / 4.
 __global__ void k_average(double* __restrict__ buf, const long long* __restrict__ rows, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -171,6 +241,29 @@ int wb_amr2d_interp(double* buf, const long long* dst, const long long* pd, cons
   hipStream_t s = (hipStream_t)stream;
   if (ncells > 0) hipLaunchKernelGGL(k_interp_cells, blocks(ncells), dim3(256), 0, s, buf, dst, pd, c00, cy, pc, wx, wy, ncells);
   if (nnodes > 0) hipLaunchKernelGGL(k_interp_nodes, blocks(nnodes), dim3(256), 0, s, buf, ndst, nbase, nsx, nsy, nkx, nky, nnodes);
+  return wb_last_error();
+}
+
+int wb_amr2d_regrid_copy(const double* old, double* buf, const long long* cdst,
+                         const long long* csrc, const long long* cpd, const long long* cps,
+                         int ncells, const long long* ndst, const long long* nsrc, int nnodes,
+                         void* stream) {
+  if (ncells < 0 || nnodes < 0) return WB_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  if (ncells > 0) hipLaunchKernelGGL(k_regrid_copy_cells, blocks(ncells), dim3(256), 0, s, old, buf, cdst, csrc, cpd, cps, ncells);
+  if (nnodes > 0) hipLaunchKernelGGL(k_regrid_copy_nodes, blocks(nnodes), dim3(256), 0, s, old, buf, ndst, nsrc, nnodes);
+  return wb_last_error();
+}
+
+int wb_amr2d_prolong(double* buf, const long long* dst, const long long* pd, const long long* c,
+                     const long long* cy, const long long* pc, const long long* sx,
+                     const long long* sy, int ncells, const long long* ndst,
+                     const long long* nbase, const long long* nsx, const long long* nkx,
+                     const long long* nky, int nnodes, void* stream) {
+  if (ncells < 0 || nnodes < 0) return WB_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  if (ncells > 0) hipLaunchKernelGGL(k_prolong_cells, blocks(ncells), dim3(256), 0, s, buf, dst, pd, c, cy, pc, sx, sy, ncells);
+  if (nnodes > 0) hipLaunchKernelGGL(k_prolong_nodes, blocks(nnodes), dim3(256), 0, s, buf, ndst, nbase, nsx, nkx, nky, nnodes);
   return wb_last_error();
 }
 
