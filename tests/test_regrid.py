@@ -261,3 +261,96 @@ def test_gpu_diagnostics_match_the_host_sums(gpu):
     assert abs(area.sum() - 1.0) < 1e-14
     totals = model.totals()
     assert abs(totals["density"] - np.sum(leaf["density"] * area)) < 1e-14
+
+
+# -- reconnection diagnostics (mhd.py:1231-1329) --------------------------------------------
+def test_harris_sheets_balance_pressure_and_reverse_the_field():
+    h = AMRHierarchy((32, 16), bounds=((0, 4), (0, 2)), periodic=(True, True))
+    state = initial.harris_sheets(h, width=0.1, perturbation=0.0)
+    y = np.linspace(0.0, 2.0, 401)
+    eps = 1e-6
+    bx = (state["vector_potential"](0.3, y + eps) - state["vector_potential"](0.3, y - eps)) / (2 * eps)
+    want = np.tanh((y - 0.5) / 0.1) - np.tanh((y - 1.5) / 0.1) - 1.0
+    np.testing.assert_allclose(bx, want, atol=1e-7)
+    np.testing.assert_allclose(state["pressure"](0.3, y) + 0.5 * want**2, 0.6, atol=1e-12)
+    assert abs(bx[0] + 1.0) < 2e-4 and abs(bx[200] - 1.0) < 2e-4  # -1 outside, +1 between
+
+
+class _Leafy:
+    """A stand-in model: a hierarchy and the time (what the diagnostics read)."""
+
+    def __init__(self, h):
+        self.hierarchy = h
+        self.time = 0.0
+
+
+def test_sheet_diagnostics_count_the_turns_of_b_y_and_its_flux():
+    h = AMRHierarchy((64, 32), bounds=((0, 4), (0, 2)), periodic=(True, True))
+    x, y = h.cell_centers(h.patches[0])
+    x, y = x.ravel(), y.ravel()
+    from wellbalanced_amd.diagnostics import sheet_diagnostics
+
+    leaf = {"x": x, "y": y, "level": np.zeros(x.size, np.int64),
+            "magnetic_field_y": np.sin(2 * np.pi * x / 4.0), "current_z": np.cos(x),
+            "density": np.ones(x.size)}
+    d = sheet_diagnostics(_Leafy(h), leaf)
+    assert (d["o_points"], d["x_points"]) == (1, 1)
+    dx = 4.0 / 64
+    xs = (np.arange(64) + 0.5) * dx
+    assert d["psi"] == pytest.approx(0.5 * np.sum(np.abs(np.sin(2 * np.pi * xs / 4.0))) * dx, rel=1e-14)
+    assert d["leaf_cells"] == 64 * 32
+
+
+@pytest.mark.gpu
+def test_gpu_reconnection_on_patches_follows_the_uniform_mesh(gpu):
+    """test_mhd_amr.py:150-178: the double Harris sheet on 32 x 16 cells refined once
+    around the sheets (regridded every 5 steps) against the uniform 64 x 32 mesh."""
+    from wellbalanced_amd import MHD2D
+    from wellbalanced_amd.amr import AMRMHD2D
+    from wellbalanced_amd.diagnostics import sheet_diagnostics
+
+    def run(shape, levels):
+        mesh = AMRHierarchy(shape, bounds=((0, 4), (0, 2)), periodic=(True, True))
+        state = initial.harris_sheets(mesh, width=0.1, perturbation=0.05)
+        if levels:
+            model = AMRMHD2D(mesh, resistivity=4e-3, limiter="vanleer")
+        else:
+            model = MHD2D(mesh, resistivity=4e-3, kernels="fused")
+        model.set_state(**state)
+        for _ in range(levels):
+            model.regrid(model.current_tags(0.5, levels=levels), buffer=1)
+            model.set_state(**state)
+        first = sheet_diagnostics(model)
+        while model.time < 0.2:
+            if levels and model.steps % 5 == 0:
+                model.regrid(model.current_tags(0.5, levels=levels), buffer=1)
+            model.step(min(model.stable_step(), 0.2 - model.time))
+        return first, sheet_diagnostics(model), model
+
+    first, refined, model = run((32, 16), 1)
+    start, uniform, _ = run((64, 32), 0)
+    assert model.hierarchy.max_level == 1
+    assert refined["leaf_cells"] < 0.75 * uniform["leaf_cells"]
+    assert (first["o_points"], first["x_points"]) == (1, 1)
+    assert abs(first["psi"] - start["psi"]) < 1e-3 * start["psi"]
+    change = abs(uniform["psi"] - start["psi"])
+    assert change > 1e-2 * start["psi"]
+    assert abs(refined["psi"] - uniform["psi"]) < 0.05 * change
+
+
+@pytest.mark.gpu
+def test_gpu_uniform_diagnostics_are_the_dataset(gpu):
+    """MHD2D's device leaf values, totals and divergence against its host dataset."""
+    from wellbalanced_amd import MHD2D
+
+    model = MHD2D.from_problem(initial.orszag_tang(24), kernels="fused")
+    model.step(model.stable_step())
+    leaf, data = model.leaf_values(), model.dataset()
+    np.testing.assert_array_equal(leaf["density"], data["density"].ravel())
+    np.testing.assert_allclose(leaf["pressure"], data["pressure"].ravel(), rtol=1e-13)
+    cons, _ = model.arrays()
+    inner = cons[:, 2:-2, 2:-2]
+    area = model.dx * model.dy
+    for k, (name, v) in enumerate(model.totals().items()):
+        assert abs(v - inner[k].sum() * area) < 1e-14 * max(1.0, abs(v))
+    assert model.divergence() < 1e-14

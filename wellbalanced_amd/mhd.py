@@ -393,24 +393,67 @@ class MHD2D:
             taken += 1
         return taken
 
-    # -- diagnostics (mhd.py:1059-1226) -------------------------------------------------
+    # -- diagnostics (mhd.py:1059-1226), on the device -----------------------------------
+    def _diag_tables(self):
+        """The patch and leaf tables of the device diagnostics: the grid as one patch."""
+        if getattr(self, "_diag", None) is None:
+            from .amr import _PATCH
+
+            patch = np.zeros(1, _PATCH)
+            patch[0] = (0, self.nx, self.ny, self.dx, self.dy)
+            a, b = np.meshgrid(np.arange(self.nx) + HALO, np.arange(self.ny) + HALO, indexing="ij")
+            leaf = np.stack([np.zeros(a.size, np.int64), (a * self.Y + b).ravel()], axis=1)
+            chunks = [(0, c) for c in range(0, self.nx * self.ny, 256)]
+            dev = dict(device=self.device)
+            self._diag = (
+                torch.from_numpy(patch.view(np.uint8).copy()).to(self.device),
+                torch.as_tensor(leaf.astype(np.int32).reshape(-1), **dev),
+                torch.as_tensor(np.array(chunks, np.int32).reshape(-1), **dev),
+                len(chunks),
+                torch.zeros(_lib.call("wb_amr2d_totals_workspace"), dtype=torch.float64, **dev),
+            )
+        return self._diag
+
     def totals(self) -> dict[str, float]:
-        """Integrals of the conserved fields (FP64 sums on the device)."""
+        """Integrals of the conserved fields (a fixed-order FP64 tree on the device)."""
         self._check_ready()
-        cells, _ = self._views(self._buf[0])
-        inner = cells[:, HALO : HALO + self.nx, HALO : HALO + self.ny]
-        sums = inner.sum(dim=(1, 2)) * (self.dx * self.dy)
-        return {name: float(v) for name, v in zip(CONSERVED, sums.cpu().tolist())}
+        patch, leaf, _, _, work = self._diag_tables()
+        out = torch.zeros(6, dtype=torch.float64, device=self.device)
+        _lib.call("wb_amr2d_totals", _lib.ptr(self._buf[0]), _lib.ptr(patch), _lib.ptr(leaf),
+                  self.nx * self.ny, _lib.ptr(work), _lib.ptr(out), _lib.stream_handle(self.device))
+        return {name: float(v) for name, v in zip(CONSERVED, out.cpu().tolist())}
 
     def divergence(self) -> float:
-        """Largest |div B| times the cell size (mhd.py:1114-1131)."""
+        """Largest |div B| times the cell size (mhd.py:1114-1131), on the device."""
         self._check_ready()
-        _, az = self._views(self._buf[0])
-        az = az[HALO : HALO + self.nx + 1, HALO : HALO + self.ny + 1]
-        bxf = (az[:, 1:] - az[:, :-1]) / self.dy + self.mean_field[0]
-        byf = -(az[1:, :] - az[:-1, :]) / self.dx + self.mean_field[1]
-        div = (bxf[1:] - bxf[:-1]) / self.dx + (byf[:, 1:] - byf[:, :-1]) / self.dy
-        return float(div.abs().max()) * min(self.dx, self.dy)
+        patch, _, chunks, nchunks, _ = self._diag_tables()
+        out = torch.zeros(3, dtype=torch.float64, device=self.device)
+        _lib.call("wb_amr2d_divergence", _lib.ptr(self._buf[0]), _lib.ptr(patch), _lib.ptr(chunks),
+                  nchunks, *self.mean_field, None, 0, None, 0, _lib.ptr(out),
+                  _lib.stream_handle(self.device))
+        return float(out[0].item())
+
+    def leaf_values(self) -> dict[str, np.ndarray]:
+        """The primitive variables of every cell (one level: the leaf cells), with ``x``,
+        ``y``, ``level`` and ``current_z`` (the mean of the cell's corners), as
+        mhd.py:1059-1112 gives them; computed on the device."""
+        self._check_ready()
+        stream = _lib.stream_handle(self.device)
+        pmask = int(self.periodic[0]) | (int(self.periodic[1]) << 1)
+        wmask = int(self.walls[0]) | (int(self.walls[1]) << 1)
+        _lib.call("wb_mhd2d_fill_halos", _lib.ptr(self._buf[0]), self.nx, self.ny, pmask, wmask,
+                  stream)
+        patch, leaf, _, _, _ = self._diag_tables()
+        n = self.nx * self.ny
+        out = torch.empty(9 * n, dtype=torch.float64, device=self.device)
+        _lib.call("wb_amr2d_leaf_values", _lib.ptr(self._buf[0]), _lib.ptr(patch), _lib.ptr(leaf),
+                  n, *self.mean_field, self.gamma, _lib.ptr(out), stream)
+        v = out.view(9, n).cpu().numpy()
+        x, y = self.hierarchy.cell_centers(self.patch)
+        names = ("density", "velocity_x", "velocity_y", "velocity_z", "pressure",
+                 "magnetic_field_x", "magnetic_field_y", "magnetic_field_z", "current_z")
+        return {"x": x.ravel(), "y": y.ravel(), "level": np.zeros(n, np.int64),
+                **{name: v[k].copy() for k, name in enumerate(names)}}
 
     def dataset(self, level: int = 0) -> dict[str, np.ndarray]:
         """Primitive variables over the cell centres (mhd.py:1177-1226), as plain arrays."""
