@@ -64,7 +64,8 @@ def parse():
     p.add_argument("--calibrate", action="store_true",
                    help="also run one device copy of 1 GiB (the traffic counters' reference)")
     p.add_argument("--workload", default="turbulence3d",
-                   choices=("turbulence3d", "ot2d", "blast2d-amr", "harris2d", "blast3d-amr"))
+                   choices=("turbulence3d", "ot2d", "blast2d-amr", "harris2d", "blast3d-amr",
+                            "harris2d-resistive", "plasmoid2d-amr"))
     return p.parse_args()
 
 
@@ -113,7 +114,8 @@ def main():
                           "(the multi-GPU sharding is a design in DESIGN.md, not built)"}))
         sys.exit(2)
     runs = {"turbulence3d": run_turbulence3d, "ot2d": run_ot2d, "blast2d-amr": run_blast2d_amr,
-            "harris2d": run_harris2d, "blast3d-amr": run_blast3d_amr}
+            "harris2d": run_harris2d, "blast3d-amr": run_blast3d_amr,
+            "harris2d-resistive": run_harris2d_resistive, "plasmoid2d-amr": run_plasmoid2d_amr}
     print(json.dumps(runs[args.workload](args)), flush=True)
 
 
@@ -349,6 +351,90 @@ def run_harris2d(args):
                  stage_ms, STATE_BYTES_2D,
                  {"launch": {"fp64": res["fp64"][4], "mixed": launch}, "divergence": div,
                   "time": models["mixed"].time})
+
+
+def run_harris2d_resistive(args):
+    """Resistive double Harris sheets with conduction on 256 x 256 cells: the workload
+    of the reference's published step time (63 ms, CHANGELOG.md:32-33, CPU)."""
+    from wellbalanced_amd import initial
+    from wellbalanced_amd.hierarchy import AMRHierarchy
+
+    h = AMRHierarchy((256, 256), ((0.0, 2.0), (0.0, 2.0)), periodic=(True, True))
+    state = initial.harris_sheets(h, width=0.08, perturbation=0.05)
+    problem = initial.Problem(h, {"resistivity": 2e-3, "conduction": (0.01, 0.001),
+                                  "limiter": "vanleer"}, state)
+    res, l1, linf, div, models = _pair_2d(problem, args, graph_steps=10)
+    rate, sec, steps, stage_ms, launch = res["mixed"]
+    return _line(rate, res["fp64"][0], steps, args, sec,
+                 "2D double Harris sheets 256x256 periodic on [0,2]^2 (width 0.08, "
+                 "perturbation 0.05), eta 2e-3, conduction (0.01, 0.001), van Leer, CFL 0.4, "
+                 "SSP-RK2, steps replayed from HIP graphs of 10 steps", 256 * 256, "mixed",
+                 l1, linf, stage_ms, STATE_BYTES_2D,
+                 {"launch": {"fp64": res["fp64"][4], "mixed": launch}, "divergence": div,
+                  "time": models["mixed"].time,
+                  "reference_cpu_zone_updates_per_s": 256 * 256 / 0.063})
+
+
+def run_plasmoid2d_amr(args):
+    """The reference's plasmoid example at its production geometry (examples/
+    plasmoid_reconnection.py:39-48,75-91): 256 x 128 base on [0,4] x [0,2], 3 levels,
+    width 0.02, eta 1e-4, regrid on |J| dx > 0.08 every 10 steps (buffer 2, patches of
+    at most 32 cells). The timed steps include the regrids."""
+    import torch
+
+    from wellbalanced_amd import initial
+    from wellbalanced_amd.amr import AMRMHD2D
+    from wellbalanced_amd.diagnostics import sheet_diagnostics
+    from wellbalanced_amd.hierarchy import AMRHierarchy
+
+    levels, every = 3, 10
+    res, models = {}, {}
+    for precision in ("fp64", "mixed"):
+        mesh = AMRHierarchy((256, 128), ((0.0, 4.0), (0.0, 2.0)), periodic=(True, True))
+        state = initial.harris_sheets(mesh, width=0.02, perturbation=1e-3, noise=1e-4)
+        m = AMRMHD2D(mesh, resistivity=1e-4, limiter="vanleer", precision=precision)
+        m.set_state(**state)
+
+        def regrid():
+            m.regrid(m.current_tags(0.08, levels=levels), buffer=2, max_patch_size=32)
+
+        for _ in range(levels):
+            regrid()
+            m.set_state(**state)
+        m.check_interval = 0
+        for _ in range(args.warmup):
+            m.advance(1)
+        torch.cuda.synchronize()
+        zones, t_regrid = 0, 0.0
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            if k and k % every == 0:
+                r0 = time.perf_counter()
+                regrid()
+                t_regrid += time.perf_counter() - r0
+            m.advance(1)
+            zones += m.cells
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        m.check()
+        _, stage_ms = time_model(m, 2, 0)
+        res[precision] = (zones / sec, sec, stage_ms, t_regrid)
+        models[precision] = m
+    a, b = models["fp64"], models["mixed"]
+    same = [p.box for p in a.hierarchy.patches] == [p.box for p in b.hierarchy.patches]
+    l1, linf = _leaf_l1(a, b) if same else (None, None)
+    rate, sec, stage_ms, t_regrid = res["mixed"]
+    leaf = b.leaf_values()
+    return _line(rate, res["fp64"][0], args.steps, args, sec,
+                 "2D double Harris sheets, 256x128 base on [0,4]x[0,2], 3 levels regridded "
+                 "every 10 steps on |J| dx > 0.08, eta 1e-4, van Leer, CFL 0.4, SSP-RK2 "
+                 "(regrids inside the timed steps)", b.cells, "mixed", l1, linf, stage_ms,
+                 STATE_BYTES_2D,
+                 {"patches": len(b.layouts), "leaf_cells": int(leaf["density"].size),
+                  "levels": b.hierarchy.max_level + 1, "regrid_seconds": t_regrid,
+                  "fp64_regrid_seconds": res["fp64"][3], "same_hierarchy": same,
+                  "divergence": b.divergence(), "composite_divergence": b.composite_divergence(),
+                  "sheet": sheet_diagnostics(b, leaf)})
 
 
 def _leaf_l1(a_model, b_model):

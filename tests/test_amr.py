@@ -173,7 +173,7 @@ def test_gpu_blast_hierarchy_conserves_mixed(gpu):
         fp64.step(dt)
         mixed.step(dt)
     after = mixed.leaf_totals()
-    assert np.abs(after - before).max() < 1e-11 * np.abs(before).max()
+    assert np.abs(after - before).max() < 1e-13 * np.abs(before).max()
     assert mixed.composite_divergence() == 0.0
     assert mixed.divergence() < 1e-13
     a, b = fp64.buffer(), mixed.buffer()

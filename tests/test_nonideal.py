@@ -360,7 +360,8 @@ def test_gpu_the_step_keeps_within_the_diffusion_limit(gpu):
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_gpu_hierarchy_with_diffusion_matches_the_oracle_and_conserves(gpu, precision):
     """Three levels with resistivity and conduction: the refluxing carries the diffusive
-    fluxes, so the leaf totals are kept (FP64 to 1e-13, mixed to 1e-11 of the largest);
+    fluxes, so the leaf totals are kept (to 1e-13 of the largest, mixed as FP64: the
+    refluxing's recomputed FP32 face flux has the stage's bits);
     FP64 is bit for bit the oracle, including the step limited by each patch's diffusion
     number."""
     from oracle.amr2d import OracleAMR2D
@@ -386,10 +387,7 @@ def test_gpu_hierarchy_with_diffusion_matches_the_oracle_and_conserves(gpu, prec
         model.step(dt)
         oracle.step(dt)
     after = model.leaf_totals()
-    # mixed: the refluxing recomputes the FP32 face flux, which may round apart from the
-    # stage's (contraction is free in the mixed unit), as on the ideal hierarchy
-    tol = 1e-13 if precision == "fp64" else 1e-11
-    assert np.abs(after - before).max() < tol * np.abs(before).max()
+    assert np.abs(after - before).max() < 1e-13 * np.abs(before).max()
     assert model.composite_divergence() == 0.0
     if precision == "fp64":
         got = model.buffer()

@@ -118,121 +118,124 @@ def _owner_tables(layouts, level, shape):
     return cells, nodes
 
 
+def _patch_arrays(layouts) -> dict:
+    """Per-patch geometry as arrays (index = patch number)."""
+    return {
+        "off": np.array([lay.offset for lay in layouts], np.int64),
+        "i0": np.array([lay.box[0][0] for lay in layouts], np.int64),
+        "j0": np.array([lay.box[1][0] for lay in layouts], np.int64),
+        "nx": np.array([lay.nx for lay in layouts], np.int64),
+        "ny": np.array([lay.ny for lay in layouts], np.int64),
+        "level": np.array([lay.level for lay in layouts], np.int64),
+    }
+
+
+def _points(ids, ext_a, ext_b):
+    """Every local point (a, b) of the grids ext_a[k] x ext_b[k] of the patches ``ids``:
+    (patch, a, b) flattened, row-major per patch."""
+    sizes = ext_a * ext_b
+    k = np.repeat(ids, sizes)
+    t = np.arange(int(sizes.sum()), dtype=np.int64) - np.repeat(np.cumsum(sizes) - sizes, sizes)
+    eb = np.repeat(ext_b, sizes)
+    return k, t // eb, t % eb
+
+
 def build(hierarchy) -> Maps:
-    """All gather tables for a hierarchy whose axes are periodic."""
+    """All gather tables for a hierarchy whose axes are periodic (whole levels at once)."""
     if not all(hierarchy.periodic):
         raise NotImplementedError("refined 2D hierarchies run on periodic domains")
     layouts, total = layouts_of(hierarchy)
     maps = Maps(layouts=layouts, total=total)
     nlev = hierarchy.max_level + 1
     owners = [_owner_tables(layouts, L, hierarchy.level_shape(L)) for L in range(nlev)]
+    boxes = box_array(layouts)
+    g = _patch_arrays(layouts)
+    Y = g["ny"] + 2 * HALO
+    plane = (g["nx"] + 2 * HALO) * Y
+    YN = Y + 1
+    A0 = g["off"] + NFIELDS * plane  # first A_z node of each patch
+
+    def cell(k, a, b):
+        return g["off"][k] + a * Y[k] + b
+
+    def node(k, a, b):
+        return A0[k] + a * YN[k] + b
+
     for L in range(nlev):
         shape = hierarchy.level_shape(L)
         cell_own, node_own = owners[L]
-        cd, cs, cpd, cps = [], [], [], []
-        nd, ns = [], []
-        id_, ipd, ic00, icy, ipc, iwx, iwy = [], [], [], [], [], [], []
-        nid, nbase, nsx, nsy, nkx, nky = [], [], [], [], [], []
-        for k, lay in enumerate(layouts):
-            if lay.level != L:
-                continue
-            (i0, i1), (j0, j1) = lay.box
-            # ghost cells
-            a, b = np.meshgrid(np.arange(lay.X), np.arange(lay.Y), indexing="ij")
-            ghost = ~((a >= HALO) & (a < HALO + lay.nx) & (b >= HALO) & (b < HALO + lay.ny))
-            a, b = a[ghost], b[ghost]
-            I = (i0 + a - HALO) % shape[0]
-            J = (j0 + b - HALO) % shape[1]
-            src = cell_own[I, J]
-            have = src >= 0
-            for q in np.unique(src[have]):
-                sel = have & (src == q)
-                ql = layouts[q]
-                cd.append(lay.cell(a[sel], b[sel]))
-                cs.append(ql.cell(I[sel] - ql.box[0][0] + HALO, J[sel] - ql.box[1][0] + HALO))
-                cpd.append(np.full(sel.sum(), lay.plane, np.int64))
-                cps.append(np.full(sel.sum(), ql.plane, np.int64))
-            miss = ~have
-            if miss.any():
-                if L == 0:
-                    raise ValueError("level 0 must cover the domain")
-                ia, ib = a[miss], b[miss]
-                fi = i0 + ia - HALO  # unwrapped fine cell coordinates
-                fj = j0 + ib - HALO
-                ci0 = np.floor_divide(fi - 1, 2)  # lower coarse cell of the stencil
-                cj0 = np.floor_divide(fj - 1, 2)
-                wx1 = np.where(fi % 2 == 0, 0.75, 0.25)
-                wy1 = np.where(fj % 2 == 0, 0.75, 0.25)
-                q, qa, qb = _coarse_container(layouts, L - 1, ci0, cj0, 2, hierarchy, cells=True)
-                for qq in np.unique(q):
-                    sel = q == qq
-                    ql = layouts[qq]
-                    id_.append(lay.cell(ia[sel], ib[sel]))
-                    ipd.append(np.full(sel.sum(), lay.plane, np.int64))
-                    ic00.append(ql.cell(qa[sel], qb[sel]))
-                    icy.append(np.full(sel.sum(), ql.Y, np.int64))
-                    ipc.append(np.full(sel.sum(), ql.plane, np.int64))
-                    iwx.append(wx1[sel])
-                    iwy.append(wy1[sel])
-            # ghost nodes and shared boundary nodes
-            a, b = np.meshgrid(np.arange(lay.X + 1), np.arange(lay.Y + 1), indexing="ij")
-            inner = (a >= HALO) & (a <= HALO + lay.nx) & (b >= HALO) & (b <= HALO + lay.ny)
-            boundary = inner & ((a == HALO) | (a == HALO + lay.nx) | (b == HALO) | (b == HALO + lay.ny))
-            a_all, b_all = a[~inner | boundary], b[~inner | boundary]
-            isb = boundary[~inner | boundary]
-            I = (i0 + a_all - HALO) % shape[0]
-            J = (j0 + b_all - HALO) % shape[1]
-            src = node_own[I, J]
-            # a boundary node keeps its value unless an earlier patch owns it
-            take = (src >= 0) & ~(isb & (src >= k))
-            # the periodic duplicate node of a patch that spans the level is its own node 0
-            for q in np.unique(src[take]):
-                sel = take & (src == q)
-                ql = layouts[q]
-                qa = (I[sel] - ql.box[0][0]) % shape[0] + HALO
-                qb = (J[sel] - ql.box[1][0]) % shape[1] + HALO
-                nd.append(lay.node(a_all[sel], b_all[sel]))
-                ns.append(ql.node(qa, qb))
-            dup = isb & (src == k)
-            if dup.any():
-                # this patch owns the node; if it spans the level it may own it twice
-                # (node n and node 0 of a periodic axis): keep index HALO as the source
-                da, db = a_all[dup], b_all[dup]
-                sa = (I[dup] - i0) % shape[0] + HALO
-                sb = (J[dup] - j0) % shape[1] + HALO
-                moved = (sa != da) | (sb != db)
-                if moved.any():
-                    nd.append(lay.node(da[moved], db[moved]))
-                    ns.append(lay.node(sa[moved], sb[moved]))
-            miss = (src < 0) & ~isb
-            if miss.any():
-                if L == 0:
-                    raise ValueError("level 0 must cover the domain")
-                ia, ib = a_all[miss], b_all[miss]
-                fi = i0 + ia - HALO
-                fj = j0 + ib - HALO
-                # per axis: even -> coarse node fi/2 (stencil base fi/2, kind 0);
-                # odd -> 4-point rule on nodes m-1..m+2, m = (fi-1)/2 (base m-1, kind 1)
-                kx = (fi % 2 == 1).astype(np.int64)
-                ky = (fj % 2 == 1).astype(np.int64)
-                bx = np.where(kx == 1, (fi - 1) // 2 - 1, fi // 2)
-                by = np.where(ky == 1, (fj - 1) // 2 - 1, fj // 2)
-                q, qa, qb = _coarse_container(layouts, L - 1, bx, by, 4, hierarchy, cells=False)
-                for qq in np.unique(q):
-                    sel = q == qq
-                    ql = layouts[qq]
-                    nid.append(lay.node(ia[sel], ib[sel]))
-                    nbase.append(ql.node(qa[sel], qb[sel]))
-                    nsx.append(np.full(sel.sum(), ql.Y + 1, np.int64))
-                    nsy.append(np.full(sel.sum(), 1, np.int64))
-                    nkx.append(kx[sel])
-                    nky.append(ky[sel])
-        cat = _cat
-        maps.copy_cells[L] = (cat(cd), cat(cs), cat(cpd), cat(cps))
-        maps.copy_nodes[L] = (cat(nd), cat(ns))
-        maps.cell_interp[L] = (cat(id_), cat(ipd), cat(ic00), cat(icy), cat(ipc),
-                               cat(iwx, np.float64), cat(iwy, np.float64))
-        maps.node_interp[L] = (cat(nid), cat(nbase), cat(nsx), cat(nsy), cat(nkx), cat(nky))
+        ids = np.flatnonzero(g["level"] == L)
+        # ghost cells
+        k, a, b = _points(ids, g["nx"][ids] + 2 * HALO, g["ny"][ids] + 2 * HALO)
+        ghost = ~((a >= HALO) & (a < HALO + g["nx"][k]) & (b >= HALO) & (b < HALO + g["ny"][k]))
+        k, a, b = k[ghost], a[ghost], b[ghost]
+        fi = g["i0"][k] + a - HALO  # unwrapped level coordinates
+        fj = g["j0"][k] + b - HALO
+        I, J = fi % shape[0], fj % shape[1]
+        src = cell_own[I, J]
+        have = src >= 0
+        q = src[have]
+        maps.copy_cells[L] = (
+            cell(k[have], a[have], b[have]),
+            cell(q, I[have] - g["i0"][q] + HALO, J[have] - g["j0"][q] + HALO),
+            plane[k[have]], plane[q])
+        miss = ~have
+        if miss.any() and L == 0:
+            raise ValueError("level 0 must cover the domain")
+        if miss.any():
+            mk, ma, mb, mfi, mfj = k[miss], a[miss], b[miss], fi[miss], fj[miss]
+            ci0 = np.floor_divide(mfi - 1, 2)  # lower coarse cell of the stencil
+            cj0 = np.floor_divide(mfj - 1, 2)
+            q, qa, qb = _coarse_container(layouts, L - 1, ci0, cj0, 2, hierarchy, cells=True,
+                                          owner=owners[L - 1][0], boxes=boxes)
+            maps.cell_interp[L] = (
+                cell(mk, ma, mb), plane[mk], cell(q, qa, qb), Y[q], plane[q],
+                np.where(mfi % 2 == 0, 0.75, 0.25), np.where(mfj % 2 == 0, 0.75, 0.25))
+        else:
+            maps.cell_interp[L] = tuple([np.zeros(0, np.int64)] * 5 + [np.zeros(0)] * 2)
+        # ghost nodes and shared boundary nodes
+        k, a, b = _points(ids, g["nx"][ids] + 2 * HALO + 1, g["ny"][ids] + 2 * HALO + 1)
+        nxk, nyk = g["nx"][k], g["ny"][k]
+        inner = (a >= HALO) & (a <= HALO + nxk) & (b >= HALO) & (b <= HALO + nyk)
+        boundary = inner & ((a == HALO) | (a == HALO + nxk) | (b == HALO) | (b == HALO + nyk))
+        keep = ~inner | boundary
+        k, a, b, isb = k[keep], a[keep], b[keep], boundary[keep]
+        fi = g["i0"][k] + a - HALO
+        fj = g["j0"][k] + b - HALO
+        I, J = fi % shape[0], fj % shape[1]
+        src = node_own[I, J]
+        # a boundary node keeps its value unless an earlier patch owns it
+        take = (src >= 0) & ~(isb & (src >= k))
+        q = src[take]
+        nd = [node(k[take], a[take], b[take])]
+        ns = [node(q, (I[take] - g["i0"][q]) % shape[0] + HALO,
+                   (J[take] - g["j0"][q]) % shape[1] + HALO)]
+        # the periodic duplicate node of a patch that spans the level is its own node 0
+        dup = isb & (src == k)
+        sa = (I[dup] - g["i0"][k[dup]]) % shape[0] + HALO
+        sb = (J[dup] - g["j0"][k[dup]]) % shape[1] + HALO
+        moved = (sa != a[dup]) | (sb != b[dup])
+        kd = k[dup][moved]
+        nd.append(node(kd, a[dup][moved], b[dup][moved]))
+        ns.append(node(kd, sa[moved], sb[moved]))
+        maps.copy_nodes[L] = (np.concatenate(nd), np.concatenate(ns))
+        miss = (src < 0) & ~isb
+        if miss.any() and L == 0:
+            raise ValueError("level 0 must cover the domain")
+        if miss.any():
+            mk, ma, mb, mfi, mfj = k[miss], a[miss], b[miss], fi[miss], fj[miss]
+            # per axis: even -> coarse node fi/2 (stencil base fi/2, kind 0);
+            # odd -> 4-point rule on nodes m-1..m+2, m = (fi-1)/2 (base m-1, kind 1)
+            kx = (mfi % 2 == 1).astype(np.int64)
+            ky = (mfj % 2 == 1).astype(np.int64)
+            bx = np.where(kx == 1, (mfi - 1) // 2 - 1, mfi // 2)
+            by = np.where(ky == 1, (mfj - 1) // 2 - 1, mfj // 2)
+            q, qa, qb = _coarse_container(layouts, L - 1, bx, by, 4, hierarchy, cells=False,
+                                          owner=owners[L - 1][0], boxes=boxes)
+            maps.node_interp[L] = (node(mk, ma, mb), node(q, qa, qb), YN[q],
+                                   np.ones(q.size, np.int64), kx, ky)
+        else:
+            maps.node_interp[L] = tuple([np.zeros(0, np.int64)] * 6)
     for L in range(1, nlev):
         maps.reflux[L - 1] = _reflux_table(layouts, owners, L, hierarchy)
         maps.average[L - 1] = _average_table(layouts, owners, L, hierarchy)
@@ -246,27 +249,53 @@ def _cat(parts, dtype=np.int64):
     return np.concatenate(parts).astype(dtype)
 
 
-def _coarse_container(layouts, level, ci, cj, width, hierarchy, cells):
+def box_array(layouts) -> np.ndarray:
+    """(npatch, 4) int64: i0, i1, j0, j1 of every patch."""
+    return np.array([[*lay.box[0], *lay.box[1]] for lay in layouts], np.int64).reshape(-1, 4)
+
+
+def _coarse_container(layouts, level, ci, cj, width, hierarchy, cells, owner=None, boxes=None):
     """For stencils whose lower corner is (ci, cj) and extent ``width`` per axis, a
     patch of ``level`` whose haloed range holds the whole stencil, and the stencil's
-    local corner there.  Periodic images are tried."""
+    local corner there (the smallest non-negative periodic image).  The candidates are
+    the owners of the stencil's corners and centre (``owner``: the level's cell-owner
+    map, as `_owner_tables` gives it); stencils none of them holds try every patch."""
     shape = hierarchy.level_shape(level)
+    ci = np.asarray(ci, np.int64)
+    cj = np.asarray(cj, np.int64)
     n = len(ci)
     q = np.full(n, -1, np.int64)
     qa = np.zeros(n, np.int64)
     qb = np.zeros(n, np.int64)
-    for k, lay in enumerate(layouts):
-        if lay.level != level:
-            continue
-        (i0, i1), (j0, j1) = lay.box
-        hi_a = (i1 - i0) + 2 * HALO - (0 if cells else -1)  # haloed extent
-        hi_b = (j1 - j0) + 2 * HALO - (0 if cells else -1)
-        for sx in (-shape[0], 0, shape[0]):
-            for sy in (-shape[1], 0, shape[1]):
-                la = ci + sx - i0 + HALO
-                lb = cj + sy - j0 + HALO
-                ok = (q < 0) & (la >= 0) & (la + width <= hi_a) & (lb >= 0) & (lb + width <= hi_b)
-                q[ok], qa[ok], qb[ok] = k, la[ok], lb[ok]
+    ext = 0 if cells else 1
+    box = box_array(layouts) if boxes is None else boxes
+    if owner is None:
+        owner = _owner_tables(layouts, level, shape)[0]
+    half = width // 2
+    for di, dj in ((0, 0), (width - 1, width - 1), (0, width - 1), (width - 1, 0), (half, half)):
+        todo = np.flatnonzero(q < 0)
+        if todo.size == 0:
+            break
+        k = owner[(ci[todo] + di) % shape[0], (cj[todo] + dj) % shape[1]]
+        have = k >= 0
+        todo, k = todo[have], k[have]
+        i0, i1, j0, j1 = box[k, 0], box[k, 1], box[k, 2], box[k, 3]
+        la = (ci[todo] - i0 + HALO) % shape[0]
+        lb = (cj[todo] - j0 + HALO) % shape[1]
+        fit = (la + width <= i1 - i0 + 2 * HALO + ext) & (lb + width <= j1 - j0 + 2 * HALO + ext)
+        q[todo[fit]], qa[todo[fit]], qb[todo[fit]] = k[fit], la[fit], lb[fit]
+    rest = np.flatnonzero(q < 0)
+    if rest.size:
+        for k, lay in enumerate(layouts):
+            if lay.level != level:
+                continue
+            (i0, i1), (j0, j1) = lay.box
+            hi_a = (i1 - i0) + 2 * HALO + ext
+            hi_b = (j1 - j0) + 2 * HALO + ext
+            la = (ci[rest] - i0 + HALO) % shape[0]
+            lb = (cj[rest] - j0 + HALO) % shape[1]
+            ok = (q[rest] < 0) & (la + width <= hi_a) & (lb + width <= hi_b)
+            q[rest[ok]], qa[rest[ok]], qb[rest[ok]] = k, la[ok], lb[ok]
     if (q < 0).any():
         raise ValueError("a fine patch is not properly nested in the level below")
     return q, qa, qb
@@ -330,24 +359,32 @@ def _reflux_table(layouts, owners, L, hierarchy):
 
 def _average_table(layouts, owners, L, hierarchy):
     """Covered coarse cells of level L-1 with the offsets of their four fine children."""
-    fine_cells, _ = owners[L]
     coarse_cells, _ = owners[L - 1]
     rows = []
     for p, pl in enumerate(layouts):
         if pl.level != L:
             continue
         (i0, i1), (j0, j1) = pl.box
-        for ci in range(i0 // 2, (i1 + 1) // 2):
-            for cj in range(j0 // 2, (j1 + 1) // 2):
-                if not (i0 <= 2 * ci and 2 * ci + 1 < i1 and j0 <= 2 * cj and 2 * cj + 1 < j1):
-                    raise ValueError("fine patches are aligned to the coarse cells")
-                q = coarse_cells[ci, cj]
-                if q < 0:
-                    raise ValueError("a fine patch lies outside the level below")
-                ql = layouts[q]
-                rows.append((ql.cell(ci - ql.box[0][0] + HALO, cj - ql.box[1][0] + HALO), ql.plane,
-                             pl.cell(2 * ci - i0 + HALO, 2 * cj - j0 + HALO), pl.Y, pl.plane))
-    return np.array(rows, np.int64).reshape(-1, 5)
+        if i0 % 2 or i1 % 2 or j0 % 2 or j1 % 2:
+            raise ValueError("fine patches are aligned to the coarse cells")
+        ci, cj = np.meshgrid(np.arange(i0 // 2, i1 // 2), np.arange(j0 // 2, j1 // 2),
+                             indexing="ij")
+        ci, cj = ci.ravel(), cj.ravel()
+        q = coarse_cells[ci, cj]
+        if (q < 0).any():
+            raise ValueError("a fine patch lies outside the level below")
+        for qq in np.unique(q):
+            sel = q == qq
+            ql = layouts[qq]
+            m = int(sel.sum())
+            rows.append(np.stack([
+                ql.cell(ci[sel] - ql.box[0][0] + HALO, cj[sel] - ql.box[1][0] + HALO),
+                np.full(m, ql.plane, np.int64),
+                pl.cell(2 * ci[sel] - i0 + HALO, 2 * cj[sel] - j0 + HALO),
+                np.full(m, pl.Y, np.int64), np.full(m, pl.plane, np.int64)], axis=1))
+    if not rows:
+        return np.zeros((0, 5), np.int64)
+    return np.concatenate(rows).astype(np.int64)
 
 
 def _restrict_table(layouts, owners, L, hierarchy):

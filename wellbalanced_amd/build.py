@@ -3,7 +3,9 @@
 ``python -m wellbalanced_amd.build`` compiles every ``csrc/*.hip`` translation
 unit and links them into ``wellbalanced_amd/libwbmhd.so``.  The FP64 parity units
 are compiled with ``-ffp-contract=off`` so that no multiply-add is fused; the
-mixed-precision unit (``*_mixed.hip``) allows contraction.
+mixed-precision units (``*_mixed.hip``) fuse the multiply-adds within each source
+expression (``-ffp-contract=on``): the same expression gives the same bits wherever it
+is inlined, so a face flux recomputed by the refluxing equals the stage's.
 """
 
 from __future__ import annotations
@@ -40,7 +42,7 @@ def _flags(source: Path) -> list[str]:
         "-munsafe-fp-atomics",
     ]
     if source.stem.endswith("_mixed"):
-        flags.append("-ffp-contract=fast")
+        flags.append("-ffp-contract=on")
     else:
         flags.append("-ffp-contract=off")
     return flags

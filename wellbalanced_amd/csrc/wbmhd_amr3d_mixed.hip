@@ -1,5 +1,5 @@
 // The refined 3D hierarchy's stage and refluxing with the FP32 reconstruction and
-// HLLD (the mixed path, built with -ffp-contract=fast).
+// HLLD (the mixed path, built with -ffp-contract=on).
 #include "amr3d.hpp"
 
 using namespace wb;

@@ -163,6 +163,7 @@ def transfer_tables(old: AMRHierarchy, new: AMRHierarchy, new_maps) -> dict:
     the mean with the next coarse node) of the rest."""
     old_layouts, _ = amr_maps.layouts_of(old)
     new_layouts = new_maps.layouts
+    boxes = amr_maps.box_array(new_layouts)
     tables = {}
     for L in range(new.max_level + 1):
         shape = new.level_shape(L)
@@ -171,6 +172,8 @@ def transfer_tables(old: AMRHierarchy, new: AMRHierarchy, new_maps) -> dict:
         else:
             cell_own = np.full(shape, -1, np.int64)
             node_own = np.full(shape, -1, np.int64)
+        coarse_owner = (amr_maps._owner_tables(new_layouts, L - 1, new.level_shape(L - 1))[0]
+                        if L > 0 else None)
         cd, cs, cpd, cps, nd, ns = [], [], [], [], [], []
         pd, pp, pc0, pcy, pcp, psx, psy = [], [], [], [], [], [], []
         qd, qb, qs, qkx, qky = [], [], [], [], []
@@ -196,7 +199,8 @@ def transfer_tables(old: AMRHierarchy, new: AMRHierarchy, new_maps) -> dict:
                     raise ValueError("level 0 must cover the domain")
                 fi, fj = I[miss], J[miss]
                 q, qa, qbb = amr_maps._coarse_container(new_layouts, L - 1, fi // 2 - 1,
-                                                        fj // 2 - 1, 3, new, cells=True)
+                                                        fj // 2 - 1, 3, new, cells=True,
+                                                        owner=coarse_owner, boxes=boxes)
                 for qq in np.unique(q):
                     sel = q == qq
                     ql = new_layouts[qq]
@@ -226,7 +230,8 @@ def transfer_tables(old: AMRHierarchy, new: AMRHierarchy, new_maps) -> dict:
                 fi = i0 + a[miss] - HALO  # unwrapped: the container tries periodic images
                 fj = j0 + b[miss] - HALO
                 q, qa, qbb = amr_maps._coarse_container(new_layouts, L - 1, fi // 2, fj // 2, 2,
-                                                        new, cells=False)
+                                                        new, cells=False, owner=coarse_owner,
+                                                        boxes=boxes)
                 for qq in np.unique(q):
                     sel = q == qq
                     ql = new_layouts[qq]
