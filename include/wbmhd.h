@@ -367,6 +367,17 @@ int wb_amr2d_reflux_mixed(const double* src, double* dst, const void* patches,
                           double b0y, double gamma, double eta, double kpar, double kperp,
                           int limiter, const double* dt, int heun, void* stream);
 
+/* The resistive terms of MHD3D (mhd3d.py:561-602) after a stage: from the stage input
+ * `in` (halos filled), eta J on the block's edges (the potential of `out` loses
+ * scale * dt * eta * J) and eta J x B through its faces (the energy of `out` loses
+ * scale * the flux divergence), scale = 0.5 with heun (the stage output is the Heun
+ * average) else 1.  work: device doubles, wb_mhd3d_resistive_workspace() of them.
+ * The sum differs from the reference's single flux difference by round-off. */
+int64_t wb_mhd3d_resistive_workspace(int64_t nx, int64_t ny, int64_t nz);
+int wb_mhd3d_resistive(const double* in, double* out, int64_t nx, int64_t ny, int64_t nz,
+                       double dx, double dy, double dz, double b0x, double b0y, double b0z,
+                       double eta, const double* dt, int heun, double* work, void* stream);
+
 /* ---- 3D hierarchy of equal blocks (refined, periodic) ---------------------------- */
 /* Every level is a set of blocks of 16^3 cells with a halo of 2, each stored as the
  * uniform 3D state of a 16^3 grid, back to back in one buffer; level L+1 refines

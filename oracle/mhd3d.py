@@ -45,6 +45,57 @@ def cell_fields(faces):
     return out
 
 
+def _along(array, axis, start, stop):
+    index = [slice(None)] * array.ndim
+    index[axis] = slice(start, stop)
+    return array[tuple(index)]
+
+
+def _pair_mean(array, axis, start, count):
+    """Means of points k and k+1 along ``axis``, k = start .. start+count-1 (``mhd3d.py:97-102``)."""
+    return 0.5 * (_along(array, axis, start, start + count) + _along(array, axis, start + 1, start + 1 + count))
+
+
+def _field_at_face(face, normal, node, shape):
+    """A haloed face field at the block's faces normal to ``normal``: the mean of the four
+    faces around each, cells along ``normal`` first, then nodes along ``node``
+    (``mhd3d.py:105-117``)."""
+    h = HALO
+    face = _pair_mean(face, normal, h - 1, shape[normal] + 1)
+    face = _pair_mean(face, node, h, shape[node])
+    other = 3 - normal - node
+    return _along(face, other, h, h + shape[other])
+
+
+def resistive3d(faces, shape, spacing, eta, edges, energy):
+    """``eta J`` into the edge fields and ``eta J x B`` into the energy fluxes of the block
+    faces, in place (``mhd3d.py:561-602``): ``J_c = d_a B_b - d_b B_a`` on the block's
+    c-edges from the haloed face fields; on a face normal to s the energy flux gains
+    ``eta (J_b B_c - J_c B_b)``, J the mean of the face's two edges along each direction,
+    B the mean of the four faces around it."""
+    h = HALO
+    current = []
+    for c in range(3):
+        _, a, b = cyclic(c)
+        parts = []
+        for axis, field, sign in ((a, faces[b], 1.0), (b, faces[a], -1.0)):
+            derivative = np.diff(field, axis=axis) / spacing[axis]
+            index = [slice(h, h + shape[i]) for i in range(3)]
+            index[a] = slice(h, h + shape[a] + 1)
+            index[b] = slice(h, h + shape[b] + 1)
+            index[axis] = slice(h - 1, h + shape[axis])
+            parts.append(sign * derivative[tuple(index)])
+        current.append(parts[0] + parts[1])
+    for c in range(3):
+        edges[c] += eta * current[c]
+    for s in range(3):
+        _, b, c = cyclic(s)
+        jb = _pair_mean(current[b], c, 0, shape[c])
+        jc = _pair_mean(current[c], b, 0, shape[b])
+        energy[s] += eta * (jb * _field_at_face(faces[c], s, c, shape)
+                            - jc * _field_at_face(faces[b], s, b, shape))
+
+
 def fast_speed_np(rho, p, bn, bt1, bt2, gamma):
     """The NumPy fast speed of ``mhd3d.py:109-114`` (``bn**2`` grouped)."""
     a2 = gamma * p / rho
@@ -58,7 +109,7 @@ class Oracle3D:
 
     def __init__(self, cons, potential, spacing, *, gamma=5.0 / 3.0,
                  mean_field=(0.0, 0.0, 0.0), limiter="vanleer", cfl=0.3,
-                 periodic=(True, True, True)):
+                 periodic=(True, True, True), resistivity=0.0):
         self.cons = np.array(cons, dtype=np.float64, copy=True)
         self.pot = [np.array(p, dtype=np.float64, copy=True) for p in potential]
         self.n = tuple(s - 2 * HALO for s in self.cons.shape[1:])
@@ -72,6 +123,7 @@ class Oracle3D:
         self.code = kernels.LIMITER_CODES.index(limiter)
         self.cfl = float(cfl)
         self.periodic = tuple(bool(p) for p in periodic)
+        self.resistivity = float(resistivity)
         self.time = 0.0
         self.steps = 0
 
@@ -118,6 +170,7 @@ class Oracle3D:
             sl = slice(h, h + nc)
             e = kernels.ct_contact(fa[1, sl], fb[1, sl], fa[0, sl], fb[0, sl], cellp[sl])
             edges.append(np.transpose(e, np.argsort(order)))
+        self._faces = faces
         return sweeps, edges
 
     def advance(self, dt):
@@ -131,6 +184,12 @@ class Oracle3D:
         for s in range(3):
             idx = [slice(None)] + [slice(h, h + self.n[k]) if k != s else slice(None) for k in range(3)]
             block.append(sweeps[s][tuple(idx)])
+        if self.resistivity:  # mhd3d.py:631-632
+            energy = [np.array(block[s][4]) for s in range(3)]
+            resistive3d(self._faces, self.n, self.spacing, self.resistivity, edges, energy)
+            for s in range(3):
+                block[s] = block[s].copy()
+                block[s][4] = energy[s]
         # rows of each axis' flux for each conserved field (``mhd3d.py:620-631``)
         for field in range(5):
             div = None
@@ -175,7 +234,10 @@ class Oracle3D:
                 + fast_speed_np(rho, p, field[a], field[(a + 1) % 3], field[(a + 2) % 3], self.gamma)
             ) / self.spacing[a]
             rate = term if rate is None else rate + term
-        return self.cfl * (1.0 / float(np.max(rate)))
+        smallest = 1.0 / float(np.max(rate))
+        if self.resistivity:  # mhd3d.py:717-721
+            smallest = min(smallest, 0.5 / (self.resistivity * sum(d**-2 for d in self.spacing)))
+        return self.cfl * smallest
 
     def step(self, dt=None):
         """Heun (SSP-RK2) step (``mhd3d.py:726-754``)."""

@@ -39,6 +39,7 @@ def options3d(problem, **over):
         mean_field=opts.get("mean_field", (0.0, 0.0, 0.0)),
         limiter=opts.get("limiter", "vanleer"),
         cfl=opts.get("cfl", 0.3),
+        resistivity=opts.get("resistivity", 0.0),
     )
 
 

@@ -396,3 +396,124 @@ def test_gpu_hierarchy_with_diffusion_matches_the_oracle_and_conserves(gpu, prec
             a = got[cells].reshape(6, lay.X, lay.Y)[:, 2:-2, 2:-2]
             b = oracle.buf[cells].reshape(6, lay.X, lay.Y)[:, 2:-2, 2:-2]
             np.testing.assert_array_equal(a, b)
+
+
+# -- 3D resistivity (mhd3d.py:561-602, 674-724) ---------------------------------------------
+def _ot3d_in_plane(normal, cells=16, layers=2, **options):
+    """The Orszag-Tang vortex in the plane of the two axes after ``normal`` (cyclic
+    order: 2D x -> axis a, 2D y -> axis b), uniform along ``normal``."""
+    from wellbalanced_amd.hierarchy import AMRHierarchy as H
+
+    a, b = (normal + 1) % 3, (normal + 2) % 3
+    shape = [cells] * 3
+    shape[normal] = layers
+    bounds = [(0.0, 1.0)] * 3
+    bounds[normal] = (0.0, layers / cells)
+    h = H(tuple(shape), tuple(bounds), periodic=(True, True, True))
+    b0 = 1.0 / np.sqrt(4.0 * np.pi)
+
+    def velocity(*x):
+        v = [0.0 * x[0]] * 3
+        v[a] = -np.sin(KW * x[b])
+        v[b] = np.sin(KW * x[a])
+        return tuple(v)
+
+    pot = [0.0, 0.0, 0.0]
+    pot[normal] = lambda *x: b0 * (np.cos(2 * KW * x[a]) / (2 * KW) + np.cos(KW * x[b]) / KW)
+    state = dict(density=25.0 / (36.0 * np.pi), velocity=velocity, pressure=5.0 / (12.0 * np.pi),
+                 vector_potential=tuple(pot))
+    return initial.Problem(h, options, state)
+
+
+@pytest.mark.parametrize("normal", [0, 1, 2])
+def test_3d_resistive_run_uniform_along_an_axis_is_the_2d_run(normal):
+    """The resistive 3D oracle on a state uniform along ``normal`` equals the resistive
+    2D oracle (whose terms the reference kernel's golden vectors pin), for every
+    orientation: J_x, J_y and J_z and the face means of B all take part."""
+    from tests.helpers import oracle3d
+
+    eta = 5e-3
+    flat = oracle3d(_ot3d_in_plane(normal, limiter="minmod", resistivity=eta))
+    plane = oracle2d(initial.orszag_tang(16, limiter="minmod", resistivity=eta))
+    a, b = (normal + 1) % 3, (normal + 2) % 3
+    for _ in range(4):
+        dt = plane.stable_step()
+        plane.step(dt)
+        flat.step(dt)
+    c3 = np.moveaxis(flat.interior(), (1 + a, 1 + b, 1 + normal), (1, 2, 3))
+    c2 = plane.interior()
+    pairs = [(0, 0), (1 + a, 1), (1 + b, 2), (1 + normal, 3), (4, 4)]
+    for r3, r2 in pairs:
+        np.testing.assert_allclose(c3[r3], np.repeat(c2[r2][:, :, None], 2, axis=2), atol=1e-13)
+    assert flat.divergence() < 1e-14
+
+
+def test_3d_resistive_step_keeps_within_the_diffusion_limit():
+    """mhd3d.py:717-721: cfl * 0.5 / (eta * sum(d^-2)) when it binds."""
+    from tests.helpers import oracle3d
+
+    problem = _ot3d_in_plane(2, cells=8, layers=4, resistivity=1.0)
+    model = oracle3d(problem)
+    dx = 1.0 / 8
+    assert model.stable_step() == pytest.approx(model.cfl * 0.5 / (3.0 / dx**2))
+
+
+def test_3d_resistivity_conserves_energy_and_div_b():
+    from tests.helpers import oracle3d
+
+    model = oracle3d(initial.turbulence_3d(8, limiter="minmod", resistivity=0.01))
+    before = model.totals()
+    for _ in range(3):
+        model.step()
+    np.testing.assert_allclose(model.totals(), before, rtol=0, atol=1e-13)
+    assert model.divergence() < 1e-14
+
+
+def _compare3d(model, oracle, rtol):
+    cons = model.arrays()[0]
+    h = 2
+    got = cons[(slice(None),) + tuple(slice(h, h + n) for n in model.n)]
+    want = oracle.interior()
+    scale = np.abs(want).max(axis=(1, 2, 3), keepdims=True)
+    assert np.all(np.abs(got - want) <= rtol * scale)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kernels", ["split", "fused"])
+def test_gpu_3d_resistive_steps_match_the_oracle(gpu, kernels):
+    """MHD3D with resistivity against the oracle to 1e-12 of each field's largest value
+    (the resistive terms are added after the stage: round-off, not bit for bit), and the
+    diffusion-limited step exactly."""
+    from tests.helpers import oracle3d
+    from wellbalanced_amd import MHD3D
+
+    problem = initial.turbulence_3d(16, limiter="minmod", resistivity=0.02)
+    model = MHD3D.from_problem(problem, kernels=kernels)
+    oracle = oracle3d(problem)
+    for _ in range(3):
+        dt = oracle.stable_step()
+        assert model.stable_step() == dt
+        model.step(dt)
+        oracle.step(dt)
+    _compare3d(model, oracle, 1e-12)
+    assert model.divergence() < 1e-13
+
+
+@pytest.mark.gpu
+def test_gpu_3d_resistive_mixed_is_close_and_conserves(gpu):
+    from wellbalanced_amd import MHD3D
+
+    problem = initial.turbulence_3d(32, limiter="minmod", resistivity=0.01)
+    fp64 = MHD3D.from_problem(problem, kernels="fused")
+    mixed = MHD3D.from_problem(problem, kernels="fused", precision="mixed")
+    before = mixed.totals()
+    for _ in range(5):
+        dt = fp64.stable_step()
+        fp64.step(dt)
+        mixed.step(dt)
+    a, b = fp64.arrays()[0][0, 2:-2, 2:-2, 2:-2], mixed.arrays()[0][0, 2:-2, 2:-2, 2:-2]
+    assert 0.0 < float(np.mean(np.abs(a - b))) < 1e-5
+    after = mixed.totals()
+    for name in before:
+        assert abs(after[name] - before[name]) < 1e-12
+    assert mixed.divergence() < 1e-13

@@ -35,6 +35,8 @@ SIGNATURES: dict[str, tuple[type, list]] = {
     "wb_mhd2d_stage": (I, [P, P, I64, I64] + [D] * 8 + [I, P, I, I, P, I, I, P]),
     "wb_mhd2d_stage_mixed": (I, [P, P, I64, I64] + [D] * 8 + [I, P, I, I, P, I, I, P]),
     "wb_mhd3d_state_size": (I64, [I64, I64, I64]),
+    "wb_mhd3d_resistive_workspace": (I64, [I64, I64, I64]),
+    "wb_mhd3d_resistive": (I, [P, P, I64, I64, I64] + [D] * 7 + [P, I, P, P]),
     "wb_mhd3d_fill_halos": (I, [P, I64, I64, I64, I, P]),
     "wb_primitive3d": (I, [P, I64, I64, I64] + [D] * 7 + [P] * 5 + [P]),
     "wb_sweep3d": (I, [P, P, I64, I64, I64, I, I, D, P, P]),

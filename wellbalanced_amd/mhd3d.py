@@ -84,14 +84,15 @@ class MHD3D:
             raise ValueError(f"kernels is one of {KERNELS}")
         if kernels == "split" and precision != "fp64":
             raise ValueError("the split kernels are the FP64 parity path; mixed needs kernels='fused'")
-        if resistivity or any(gravity):
-            raise NotImplementedError("the resistive and gravity terms are outside this ideal-MHD path")
+        if any(gravity):
+            raise NotImplementedError("the gravity source is outside this MHD path")
         if len(hierarchy.patches) != 1:
             raise NotImplementedError("refined 3D patches are not supported by MHD3D")
         self.hierarchy = hierarchy
         self.gamma = float(gamma)
         self.mean_field = tuple(float(b) for b in mean_field)
-        self.resistivity = 0.0
+        self.resistivity = float(resistivity)
+        self._resistive_work = None
         self.limiter = limiter
         self.cfl = float(cfl)
         self.gravity = (0.0, 0.0, 0.0)
@@ -255,6 +256,21 @@ class MHD3D:
         call("wb_mhd3d_update", ptr(src), ptr(dst), ptr(w["s0"]), ptr(w["s1"]), ptr(w["s2"]),
              ptr(w["e0"]), ptr(w["e1"]), ptr(w["e2"]), nx, ny, nz, ptr(self._dt), *self.spacing,
              int(heun), stream)
+        self._resistive(src, dst, heun, stream)
+
+    def _resistive(self, src, dst, heun, stream) -> None:
+        """The resistive terms (mhd3d.py:561-602) on top of the stage just run."""
+        if not self.resistivity:
+            return
+        import torch
+
+        nx, ny, nz = self.n
+        if self._resistive_work is None:
+            size = _lib.call("wb_mhd3d_resistive_workspace", nx, ny, nz)
+            self._resistive_work = torch.empty(size, dtype=torch.float64, device=self.device)
+        _lib.call("wb_mhd3d_resistive", _lib.ptr(src), _lib.ptr(dst), nx, ny, nz, *self.spacing,
+                  *self.mean_field, self.resistivity, _lib.ptr(self._dt), int(heun),
+                  _lib.ptr(self._resistive_work), stream)
 
     def _stage_fused(self, src, dst, heun, slot, stream, marks=None):
         nx, ny, nz = self.n
@@ -269,6 +285,7 @@ class MHD3D:
             _lib.call("wb_mhd3d_stage", *args, stream)
         else:
             _lib.call("wb_mhd3d_stage_mixed", *args, int(self.waves), stream)
+        self._resistive(src, dst, heun, stream)
         if marks is not None:
             marks[1].record()
 
@@ -277,12 +294,23 @@ class MHD3D:
         _lib.call("wb_rate3d", _lib.ptr(self._buf[0]), nx, ny, nz, *self.spacing,
                   *self.mean_field, self.gamma, _lib.ptr(self._rate), stream)
 
+    def _dt_kernel(self, time_ptr, stream: int) -> None:
+        if self.resistivity:  # mhd3d.py:717-721
+            _lib.call("wb_stable_dt_diffusive", _lib.ptr(self._rate), self.cfl, 0.0,
+                      self.resistivity, 0.0, sum(d**-2 for d in self.spacing),
+                      _lib.ptr(self._dt), time_ptr, stream)
+        else:
+            _lib.call("wb_stable_dt", _lib.ptr(self._rate), self.cfl, 0.0, _lib.ptr(self._dt),
+                      time_ptr, stream)
+
     def stable_step(self) -> float:
-        """cfl / (largest summed fast-wave rate) (mhd3d.py:674-724); synchronises."""
+        """cfl times the smallest of 1 / (largest summed fast-wave rate) and, with a
+        resistivity, the diffusion limit (mhd3d.py:674-724); synchronises."""
         self._check_ready()
-        self._rate_kernel(_lib.stream_handle(self.device))
-        largest = float(self._rate[0].item())
-        return self.cfl * (1.0 / largest)
+        stream = _lib.stream_handle(self.device)
+        self._rate_kernel(stream)
+        self._dt_kernel(None, stream)
+        return float(self._dt[0].item())
 
     def _heun(self, stream: int, events=None) -> None:
         import torch
@@ -331,8 +359,7 @@ class MHD3D:
         marks = [] if events else None
         for _ in range(steps):
             self._rate_kernel(stream)
-            _lib.call("wb_stable_dt", _lib.ptr(self._rate), self.cfl, 0.0, _lib.ptr(self._dt),
-                      _lib.ptr(self._t), stream)
+            self._dt_kernel(_lib.ptr(self._t), stream)
             self._heun(stream, marks)
             self.steps += 1
         return marks
