@@ -84,6 +84,15 @@ class AMRHierarchy:
         self.patches.append(patch)
         return patch
 
+    def regrid(self, tags: dict, **options) -> "AMRHierarchy":
+        """A new hierarchy whose level L + 1 covers the tagged cells of level L
+        (``tags``: level -> bool array of the level's shape), as patchwork's regrid is
+        called by MHD2D.regrid (mhd.py:1052); ``options``: ``max_patch_size``,
+        ``buffer``, ``efficiency`` (`wellbalanced_amd.regrid.regrid`)."""
+        from .regrid import regrid
+
+        return regrid(self, tags, **options)
+
     # coordinates ----------------------------------------------------------
     def cell_centers(self, patch: Patch) -> list[np.ndarray]:
         """The cell-centre coordinates of a patch, one array per axis (``indexing='ij'``)."""
