@@ -47,7 +47,12 @@ __device__ __forceinline__ int edge_at(int c, int i, int j, int k) {
   return OFF_A + 2 * NEDGE + (i * XN + j) * X + k;
 }
 
+// g modulo n in [0, n); coordinates within one period of the range (the fills' and
+// the stencils' case) take no integer division.
 __device__ __forceinline__ int wrap(int g, int n) {
+  if (g >= 0 && g < n) return g;
+  if (g < 0 && g >= -n) return g + n;
+  if (g >= n && g < 2 * n) return g - n;
   g %= n;
   return g < 0 ? g + n : g;
 }

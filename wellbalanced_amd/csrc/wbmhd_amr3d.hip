@@ -8,29 +8,11 @@ using namespace wb::amr3;
 
 namespace {
 
-// The block of `level` whose copy of edge (component c) at level coordinates g is
-// the master: the block holding it in its own range [0, B) on every axis first, then
-// the blocks holding it on their upper boundary (local index B on a node axis where
-// g is a multiple of B), in a fixed order; -1 if the level has none.  *loc gets the
-// master's local (haloed) coordinates.
-__device__ int edge_master(const int* lookup, const Levels& lv, int level, int c, const int g[3],
-                           int loc[3]) {
-  const int nb = lv.nb[level];
-  for (int s = 0; s < 8; ++s) {
-    int blk[3];
-    bool ok = true;
-    for (int a = 0; a < 3; ++a) {
-      const int sa = (s >> (2 - a)) & 1;
-      if (sa && (a == c || g[a] % B != 0)) ok = false;
-      blk[a] = sa ? wrap(g[a] / B - 1, nb) : g[a] / B;
-      loc[a] = sa ? B + 2 : g[a] % B + 2;
-    }
-    if (!ok) continue;
-    const int b = lookup[lv.base[level] + (blk[0] * nb + blk[1]) * nb + blk[2]];
-    if (b >= 0) return b;
-  }
-  return -1;
-}
+// Workgroups per block that fill its cells (each a contiguous quarter of the shell);
+// three more fill the three potential components.
+constexpr int CELL_PARTS = 4;
+// Workgroups per refined block and potential component of the restriction.
+constexpr int RESTRICT_PARTS = 4;
 
 // The t-th position of the ghost shell of an array of extents e (the positions with a
 // coordinate outside the block's own range [2, 2 + B)): the slab of the low and high
@@ -62,33 +44,166 @@ __device__ __forceinline__ void shell_at(int t, int e0, int e1, int e2, int p[3]
   }
 }
 
-// Ghosts of every block (one workgroup per block).  A ghost cell takes the value of
-// the block of the same level holding it; an edge outside the block's own range
-// [2, 18) takes the value of its master copy on the same level (a block keeps the
-// edges it is the master of).  Without a block of the same level, cells are
-// interpolated from the level below trilinearly (weights 1/4, 3/4 per axis) and edges
-// per axis by the same weights along the cell axis and, across, the coarse node
-// (even fine nodes) or the mean of the two coarse nodes (odd).  Only values a fill
-// never writes are read, so one launch fills every level.
+// The 27 blocks around a block on its own level, or around its parent on the level
+// below (relative block offsets -1..1 per axis, x slowest): ids (-1: none) and first
+// doubles.  Every source of a block's ghost fill lies in one of the two, so a fill
+// workgroup loads them once into LDS and needs no level lookup per element.
+struct Hood {
+  long long off[27];
+  int id[27];
+};
+
+__device__ __forceinline__ int hood_index(int r0, int r1, int r2) {
+  return ((r0 + 1) * 3 + (r1 + 1)) * 3 + (r2 + 1);
+}
+
+// floor(u / B) for u in [-B, 2B).
+__device__ __forceinline__ int rel_of(int u) { return u < 0 ? -1 : (u >= B ? 1 : 0); }
+
+// The master copy of an edge (component c) at offsets u from the origin of the block at
+// the centre of h, in that level's cells: the block holding it in its own range [0, B)
+// first, then the blocks holding it on their upper boundary (local index B on a node
+// axis where the coordinate is a multiple of B), in a fixed order (bit s of the
+// candidate: x, y, z from the top); returns its index in h (-1: none) and its local
+// haloed coordinates.  The fills never write a master copy.
+__device__ __forceinline__ int hood_master(const Hood& h, int c, const int u[3], int loc[3]) {
+  int rel[3], um[3];
+  for (int a = 0; a < 3; ++a) {
+    rel[a] = rel_of(u[a]);
+    um[a] = u[a] - rel[a] * B;
+  }
+  for (int s = 0; s < 8; ++s) {
+    bool ok = true;
+    int r[3];
+    for (int a = 0; a < 3; ++a) {
+      const int sa = (s >> (2 - a)) & 1;
+      if (sa && (a == c || um[a] != 0)) ok = false;
+      r[a] = rel[a] - sa;
+      if (r[a] < -1) ok = false;
+      loc[a] = sa ? B + 2 : um[a] + 2;
+    }
+    if (!ok) continue;
+    const int k = hood_index(r[0], r[1], r[2]);
+    if (h.id[k] >= 0) return k;
+  }
+  return -1;
+}
+
+// The ghost edges of component C of one block (see k_fill3d): extents and shell
+// divisions are compile-time constants per component.
+template <int C>
+__device__ __forceinline__ void fill_edges(double* __restrict__ buf, const Hood& same,
+                                           const Hood& coarse, bool all, const int bo[3],
+                                           const int po[3], double* __restrict__ me) {
+  constexpr int c = C;
+  constexpr int e0 = c == 0 ? X : XN, e1 = c == 1 ? X : XN, e2 = c == 2 ? X : XN;
+  const int nedge = all ? e0 * e1 * e2 : shell_size(e0, e1, e2);
+  for (int t = threadIdx.x; t < nedge; t += blockDim.x) {
+    int p[3];
+    if (all) {
+      p[0] = t / (e1 * e2);
+      p[1] = (t / e2) % e1;
+      p[2] = t % e2;
+    } else {
+      shell_at(t, e0, e1, e2, p);
+    }
+    if (!all) {
+      const int u[3] = {p[0] - 2, p[1] - 2, p[2] - 2};
+      int loc[3];
+      const int k = hood_master(same, c, u, loc);
+      if (k >= 0) {
+        if (k == 13 && loc[0] == p[0] && loc[1] == p[1] && loc[2] == p[2])
+          continue;  // this block's copy is the master
+        me[edge_at(c, p[0], p[1], p[2])] = buf[same.off[k] + edge_at(c, loc[0], loc[1], loc[2])];
+        continue;
+      }
+    }
+    int c0[3], nt[3];
+    double w[3][2];
+    for (int a = 0; a < 3; ++a) {
+      const int g = bo[a] + p[a] - 2;  // level coordinate (unwrapped: the hood wraps)
+      const int m = g >> 1;
+      if (a == c) {  // cell axis
+        const double tt = (g & 1) ? 0.25 : 0.75;
+        c0[a] = (g & 1) ? m : m - 1;
+        nt[a] = 2;
+        w[a][0] = 1.0 - tt;
+        w[a][1] = tt;
+      } else if ((g & 1) == 0) {  // a coarse node
+        c0[a] = m;
+        nt[a] = 1;
+        w[a][0] = 1.0;
+        w[a][1] = 0.0;
+      } else {  // the midpoint of coarse nodes m and m + 1
+        c0[a] = m;
+        nt[a] = 2;
+        w[a][0] = 0.5;
+        w[a][1] = 0.5;
+      }
+    }
+    double acc = 0.0;
+    for (int x = 0; x < nt[0]; ++x)
+      for (int y = 0; y < nt[1]; ++y)
+        for (int z = 0; z < nt[2]; ++z) {
+          const int u[3] = {c0[0] + x - po[0], c0[1] + y - po[1], c0[2] + z - po[2]};
+          int cl[3];
+          const int k = hood_master(coarse, c, u, cl);
+          if (k < 0) continue;  // outside the level below: excluded by the nesting rule
+          const double v = buf[coarse.off[k] + edge_at(c, cl[0], cl[1], cl[2])];
+          acc = acc + ((w[0][x] * w[1][y]) * w[2][z]) * v;
+        }
+    me[edge_at(c, p[0], p[1], p[2])] = acc;
+  }
+}
+
+// Ghosts of every block (workgroups per block: CELL_PARTS for its cells, then one per
+// potential component).  A ghost cell takes the value of the block of the same level
+// holding it; an edge outside the block's own range [2, 18) takes the value of its
+// master copy on the same level (a block keeps the edges it is the master of).  Without
+// a block of the same level, cells are interpolated from the level below trilinearly
+// (weights 1/4, 3/4 per axis) and edges per axis by the same weights along the cell
+// axis and, across, the coarse node (even fine nodes) or the mean of the two coarse
+// nodes (odd).  Only values a fill never writes are read, so one launch fills every
+// level.
 //
 // With `ids` (prolongation of new blocks) the listed blocks take every cell and edge,
-// interior included, by interpolation from the level below.  blockIdx.y: 0 the cells,
-// 1..3 the potential component blockIdx.y - 1.
+// interior included, by interpolation from the level below.
 __global__ void __launch_bounds__(256) k_fill3d(double* __restrict__ buf,
                                                 const BlockDesc* __restrict__ blocks,
                                                 const int* __restrict__ lookup, Levels lv,
                                                 const int* __restrict__ ids) {
+  __shared__ Hood same, coarse;
   const int self = ids ? ids[blockIdx.x] : (int)blockIdx.x;
   const bool all = ids != nullptr;
   const BlockDesc d = blocks[self];
   const int L = d.level;
-  const int n = lv.nb[L] * B;
-  const int bo[3] = {d.bx * B, d.by * B, d.bz * B};
+  const int tid = threadIdx.x;
+  if (tid < 54) {  // the two neighbourhoods
+    const bool up = tid >= 27;
+    const int k = up ? tid - 27 : tid;
+    const int r[3] = {k / 9 - 1, (k / 3) % 3 - 1, k % 3 - 1};
+    int id = -1;
+    if (!up || L > 0) {
+      const int lvl = up ? L - 1 : L;
+      const int nb = lv.nb[lvl];
+      const int cb[3] = {up ? d.bx >> 1 : d.bx, up ? d.by >> 1 : d.by, up ? d.bz >> 1 : d.bz};
+      const int b0 = wrap(cb[0] + r[0], nb), b1 = wrap(cb[1] + r[1], nb),
+                b2 = wrap(cb[2] + r[2], nb);
+      id = lookup[lv.base[lvl] + (b0 * nb + b1) * nb + b2];
+    }
+    Hood& h = up ? coarse : same;
+    h.id[k] = id;
+    h.off[k] = id >= 0 ? blocks[id].off : -1;
+  }
+  __syncthreads();
+  const int bo[3] = {d.bx * B, d.by * B, d.bz * B};                    // origin, level L
+  const int po[3] = {(d.bx >> 1) * B, (d.by >> 1) * B, (d.bz >> 1) * B};  // parent, L - 1
   double* me = buf + d.off;
   const int part = blockIdx.y;
-  // cells
-  const int ncell = part != 0 ? 0 : (all ? NCELL : shell_size(X, X, X));
-  for (int t = threadIdx.x; t < ncell; t += blockDim.x) {
+  // cells: part `part` of them
+  const int ncell = part >= CELL_PARTS ? 0 : (all ? NCELL : shell_size(X, X, X));
+  const int t_end = (ncell * (part + 1)) / CELL_PARTS;
+  for (int t = (ncell * part) / CELL_PARTS + tid; t < t_end; t += blockDim.x) {
     int p[3];
     if (all) {
       p[0] = t / (X * X);
@@ -97,113 +212,79 @@ __global__ void __launch_bounds__(256) k_fill3d(double* __restrict__ buf,
     } else {
       shell_at(t, X, X, X, p);
     }
-    int g[3];
-    for (int a = 0; a < 3; ++a) g[a] = wrap(bo[a] + p[a] - 2, n);
-    const int owner = all ? -1 : block_of(lookup, lv, L, g[0], g[1], g[2]);
-    if (owner >= 0) {
-      const double* src = buf + blocks[owner].off;
-      const int o = cell_at(0, g[0] % B + 2, g[1] % B + 2, g[2] % B + 2);
-      const int m = cell_at(0, p[0], p[1], p[2]);
-      for (int f = 0; f < 5; ++f) me[m + f * NCELL] = src[o + f * NCELL];
-      continue;
+    if (!all) {
+      int r[3], lc[3];
+      for (int a = 0; a < 3; ++a) {
+        r[a] = rel_of(p[a] - 2);
+        lc[a] = p[a] - r[a] * B;
+      }
+      const int k = hood_index(r[0], r[1], r[2]);
+      if (same.id[k] >= 0) {
+        const double* src = buf + same.off[k];
+        const int o = cell_at(0, lc[0], lc[1], lc[2]);
+        const int m = cell_at(0, p[0], p[1], p[2]);
+        for (int f = 0; f < 5; ++f) me[m + f * NCELL] = src[o + f * NCELL];
+        continue;
+      }
     }
-    const int nc = lv.nb[L - 1] * B;
     int c0[3];
     double w[3][2];
     for (int a = 0; a < 3; ++a) {
-      const int m = g[a] >> 1;
-      const double tt = (g[a] & 1) ? 0.25 : 0.75;
-      c0[a] = (g[a] & 1) ? m : m - 1;
+      const int g = bo[a] + p[a] - 2;
+      const int m = g >> 1;
+      const double tt = (g & 1) ? 0.25 : 0.75;
+      c0[a] = (g & 1) ? m : m - 1;
       w[a][0] = 1.0 - tt;
       w[a][1] = tt;
     }
+    // the 8 coarse cells (field 0 offsets; -1: outside the level below, excluded by
+    // the nesting rule), found once for the 5 fields
+    long long src[8];
+    for (int x = 0; x < 2; ++x)
+      for (int y = 0; y < 2; ++y)
+        for (int z = 0; z < 2; ++z) {
+          const int u[3] = {c0[0] + x - po[0], c0[1] + y - po[1], c0[2] + z - po[2]};
+          const int r0 = rel_of(u[0]), r1 = rel_of(u[1]), r2 = rel_of(u[2]);
+          const int k = hood_index(r0, r1, r2);
+          src[(x * 2 + y) * 2 + z] =
+              coarse.id[k] < 0 ? -1
+                               : coarse.off[k] + cell_at(0, u[0] - r0 * B + 2, u[1] - r1 * B + 2,
+                                                         u[2] - r2 * B + 2);
+        }
     for (int f = 0; f < 5; ++f) {
       double acc = 0.0;
       for (int x = 0; x < 2; ++x)
         for (int y = 0; y < 2; ++y)
           for (int z = 0; z < 2; ++z) {
-            const int G0 = wrap(c0[0] + x, nc), G1 = wrap(c0[1] + y, nc), G2 = wrap(c0[2] + z, nc);
-            const int cb = block_of(lookup, lv, L - 1, G0, G1, G2);
-            if (cb < 0) continue;  // outside the level below: excluded by the nesting rule
-            const double v = buf[blocks[cb].off + cell_at(f, G0 % B + 2, G1 % B + 2, G2 % B + 2)];
+            const long long o = src[(x * 2 + y) * 2 + z];
+            if (o < 0) continue;
+            const double v = buf[o + (long long)f * NCELL];
             acc = acc + ((w[0][x] * w[1][y]) * w[2][z]) * v;
           }
       me[cell_at(f, p[0], p[1], p[2])] = acc;
     }
   }
-  // edges of component c: axis c is the cell axis, the others node axes
-  if (part > 0) {
-    const int c = part - 1;
-    const int e0 = c == 0 ? X : XN, e1 = c == 1 ? X : XN, e2 = c == 2 ? X : XN;
-    const int nedge = all ? e0 * e1 * e2 : shell_size(e0, e1, e2);
-    for (int t = threadIdx.x; t < nedge; t += blockDim.x) {
-      int p[3];
-      if (all) {
-        p[0] = t / (e1 * e2);
-        p[1] = (t / e2) % e1;
-        p[2] = t % e2;
-      } else {
-        shell_at(t, e0, e1, e2, p);
-      }
-      int g[3], loc[3];
-      for (int a = 0; a < 3; ++a) g[a] = wrap(bo[a] + p[a] - 2, n);
-      const int master = all ? -1 : edge_master(lookup, lv, L, c, g, loc);
-      if (master >= 0) {
-        if (master == self && loc[0] == p[0] && loc[1] == p[1] && loc[2] == p[2])
-          continue;  // this block's copy is the master
-        me[edge_at(c, p[0], p[1], p[2])] = buf[blocks[master].off + edge_at(c, loc[0], loc[1], loc[2])];
-        continue;
-      }
-      const int nc = lv.nb[L - 1] * B;
-      int c0[3], nt[3];
-      double w[3][2];
-      for (int a = 0; a < 3; ++a) {
-        const int m = g[a] >> 1;
-        if (a == c) {  // cell axis
-          const double tt = (g[a] & 1) ? 0.25 : 0.75;
-          c0[a] = (g[a] & 1) ? m : m - 1;
-          nt[a] = 2;
-          w[a][0] = 1.0 - tt;
-          w[a][1] = tt;
-        } else if ((g[a] & 1) == 0) {  // a coarse node
-          c0[a] = m;
-          nt[a] = 1;
-          w[a][0] = 1.0;
-        } else {  // the midpoint of coarse nodes m and m + 1
-          c0[a] = m;
-          nt[a] = 2;
-          w[a][0] = 0.5;
-          w[a][1] = 0.5;
-        }
-      }
-      double acc = 0.0;
-      for (int x = 0; x < nt[0]; ++x)
-        for (int y = 0; y < nt[1]; ++y)
-          for (int z = 0; z < nt[2]; ++z) {
-            const int G[3] = {wrap(c0[0] + x, nc), wrap(c0[1] + y, nc), wrap(c0[2] + z, nc)};
-            int cl[3];
-            const int cb = edge_master(lookup, lv, L - 1, c, G, cl);
-            if (cb < 0) continue;  // outside the level below: excluded by the nesting rule
-            const double v = buf[blocks[cb].off + edge_at(c, cl[0], cl[1], cl[2])];
-            acc = acc + ((w[0][x] * w[1][y]) * w[2][z]) * v;
-          }
-      me[edge_at(c, p[0], p[1], p[2])] = acc;
-    }
-  }
+  // edges of component c = part - CELL_PARTS (axis c is the cell axis, the others
+  // node axes)
+  if (part == CELL_PARTS) fill_edges<0>(buf, same, coarse, all, bo, po, me);
+  if (part == CELL_PARTS + 1) fill_edges<1>(buf, same, coarse, all, bo, po, me);
+  if (part == CELL_PARTS + 2) fill_edges<2>(buf, same, coarse, all, bo, po, me);
 }
 
-// Covered cells of a refined block := the mean of their 8 children (one workgroup
-// per refined block; children[9]: the block, then its children with x slowest).
+// Covered cells of a refined block := the mean of their 8 children (a workgroup per
+// refined block and child; children[9]: the block, then its children with x slowest).
 __global__ void __launch_bounds__(256) k_average3d(double* __restrict__ buf,
                                                    const BlockDesc* __restrict__ blocks,
                                                    const int* __restrict__ children) {
   const int* row = children + 9 * blockIdx.x;
   double* parent = buf + blocks[row[0]].off;
-  for (int t = threadIdx.x; t < B * B * B; t += blockDim.x) {
-    const int I = t / (B * B), J = (t / B) % B, K = t % B;
-    const int oct = ((I / 8) * 2 + J / 8) * 2 + K / 8;
-    const double* kid = buf + blocks[row[1 + oct]].off;
-    const int fi = 2 * (I % 8) + 2, fj = 2 * (J % 8) + 2, fk = 2 * (K % 8) + 2;
+  const int oct = blockIdx.y;  // the child (x slowest) whose 8^3 parent cells this group averages
+  const double* kid = buf + blocks[row[1 + oct]].off;
+  constexpr int H = B / 2;
+  for (int t = threadIdx.x; t < H * H * H; t += blockDim.x) {
+    const int li = t / (H * H), lj = (t / H) % H, lk = t % H;
+    const int I = (oct >> 2) * H + li, J = ((oct >> 1) & 1) * H + lj, K = (oct & 1) * H + lk;
+    const int fi = 2 * li + 2, fj = 2 * lj + 2, fk = 2 * lk + 2;
     for (int f = 0; f < 5; ++f) {
       const double s0 = (kid[cell_at(f, fi, fj, fk)] + kid[cell_at(f, fi, fj, fk + 1)]) +
                         (kid[cell_at(f, fi, fj + 1, fk)] + kid[cell_at(f, fi, fj + 1, fk + 1)]);
@@ -215,7 +296,9 @@ __global__ void __launch_bounds__(256) k_average3d(double* __restrict__ buf,
 }
 
 // Coarse edges of a refined block := the mean of the two fine edges along them
-// (mhd3d.py:31-36), written to every block of the coarse level holding the edge.
+// (mhd3d.py:31-36), written to every block of the coarse level holding the edge.  A
+// workgroup per refined block, component (blockIdx.y) and part of its edges
+// (blockIdx.z).
 __global__ void __launch_bounds__(256) k_restrict3d(double* __restrict__ buf,
                                                     const BlockDesc* __restrict__ blocks,
                                                     const int* __restrict__ children,
@@ -225,9 +308,11 @@ __global__ void __launch_bounds__(256) k_restrict3d(double* __restrict__ buf,
   const int L = pd.level;
   const int n = lv.nb[L] * B;
   const int bo[3] = {pd.bx * B, pd.by * B, pd.bz * B};
-  for (int c = 0; c < 3; ++c) {
+  {
+    const int c = blockIdx.y, part = blockIdx.z;
     const int e0 = c == 0 ? B : B + 1, e1 = c == 1 ? B : B + 1, e2 = c == 2 ? B : B + 1;
-    for (int t = threadIdx.x; t < e0 * e1 * e2; t += blockDim.x) {
+    const int ne = e0 * e1 * e2, t_end = (ne * (part + 1)) / RESTRICT_PARTS;
+    for (int t = (ne * part) / RESTRICT_PARTS + threadIdx.x; t < t_end; t += blockDim.x) {
       const int q[3] = {t / (e1 * e2), (t / e2) % e1, t % e2};  // 0..B (nodes) / B-1 (cells)
       // the two fine edges: fine coordinates within the block's footprint (0..2B)
       double v[2];
@@ -336,7 +421,7 @@ int wb_amr3d_fill(double* buf, const void* blocks, int nblocks, const int32_t* l
                   const void* levels, void* stream) {
   const Levels lv = *(const Levels*)levels;
   if (nblocks < 1 || !levels_ok(lv)) return WB_EINVAL;
-  hipLaunchKernelGGL(k_fill3d, dim3(nblocks, 4), dim3(256), 0, (hipStream_t)stream, buf,
+  hipLaunchKernelGGL(k_fill3d, dim3(nblocks, CELL_PARTS + 3), dim3(256), 0, (hipStream_t)stream, buf,
                      (const BlockDesc*)blocks, (const int*)lookup, lv, (const int*)nullptr);
   return wb_last_error();
 }
@@ -346,7 +431,7 @@ int wb_amr3d_prolong(double* buf, const void* blocks, const int32_t* ids, int ni
   const Levels lv = *(const Levels*)levels;
   if (nids < 0 || !levels_ok(lv)) return WB_EINVAL;
   if (nids == 0) return 0;
-  hipLaunchKernelGGL(k_fill3d, dim3(nids, 4), dim3(256), 0, (hipStream_t)stream, buf,
+  hipLaunchKernelGGL(k_fill3d, dim3(nids, CELL_PARTS + 3), dim3(256), 0, (hipStream_t)stream, buf,
                      (const BlockDesc*)blocks, (const int*)lookup, lv, (const int*)ids);
   return wb_last_error();
 }
@@ -357,10 +442,10 @@ int wb_amr3d_coarsen(double* buf, const void* blocks, const int32_t* children, i
   if (nrefined < 0 || !levels_ok(lv)) return WB_EINVAL;
   if (nrefined == 0) return 0;
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_average3d, dim3(nrefined), dim3(256), 0, s, buf, (const BlockDesc*)blocks,
-                     (const int*)children);
-  hipLaunchKernelGGL(k_restrict3d, dim3(nrefined), dim3(256), 0, s, buf, (const BlockDesc*)blocks,
-                     (const int*)children, (const int*)lookup, lv);
+  hipLaunchKernelGGL(k_average3d, dim3(nrefined, 8), dim3(256), 0, s, buf,
+                     (const BlockDesc*)blocks, (const int*)children);
+  hipLaunchKernelGGL(k_restrict3d, dim3(nrefined, 3, RESTRICT_PARTS), dim3(256), 0, s, buf,
+                     (const BlockDesc*)blocks, (const int*)children, (const int*)lookup, lv);
   return wb_last_error();
 }
 
