@@ -343,7 +343,8 @@ int wb_amr2d_rate(const double* buf, const void* patches, const int32_t* chunks,
 
 /* One stage of every patch in ONE persistent launch: `workgroups` workgroups take
  * (patch, tile, segment) items (int32 triples) from *counter until `nitems` are done
- * (the counter is zeroed first).  FP64 parity path; _mixed: FP32 flux. */
+ * (the counter is zeroed first); tile: 30, 62 or 126 cells along y.  FP64 parity path;
+ * _mixed: FP32 flux. */
 int wb_amr2d_stage(const double* in, double* out, const void* patches, const void* items,
                    int nitems, int32_t* counter, double b0x, double b0y, double gamma, double eta,
                    double kpar, double kperp, int limiter, const double* dt, int heun,

@@ -132,7 +132,7 @@ def test_blast_hierarchy_is_nested_and_uneven():
 
 # -- GPU ----------------------------------------------------------------------------------
 @pytest.mark.gpu
-@pytest.mark.parametrize("tile,nxs", [(62, 16), (126, 5)])
+@pytest.mark.parametrize("tile,nxs", [(30, 8), (62, 16), (126, 5)])
 def test_gpu_hierarchy_is_bitwise_the_oracle(gpu, tile, nxs):
     from wellbalanced_amd.amr import AMRMHD2D
 

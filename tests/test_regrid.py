@@ -354,3 +354,25 @@ def test_gpu_uniform_diagnostics_are_the_dataset(gpu):
     for k, (name, v) in enumerate(model.totals().items()):
         assert abs(v - inner[k].sum() * area) < 1e-14 * max(1.0, abs(v))
     assert model.divergence() < 1e-14
+
+
+@pytest.mark.gpu
+def test_gpu_mhd2d_on_refined_patches_is_the_hierarchy_model(gpu):
+    """MHD2D(hierarchy with patches) gives the hierarchy model, with the reference's
+    defaults (van Leer), run() to an end time and the per-level dataset."""
+    from wellbalanced_amd import MHD2D
+    from wellbalanced_amd.amr import AMRMHD2D
+
+    h = AMRHierarchy((16, 16), ((0.0, 1.0), (0.0, 1.0)), periodic=(True, True))
+    h.add_patch(1, ((8, 24), (8, 24)))
+    model = MHD2D(h, resistivity=1e-3)
+    assert isinstance(model, AMRMHD2D) and model.limiter == "vanleer"
+    model.set_state(**_orszag_tang_state())
+    before = model.totals()
+    assert model.run(0.02) > 0 and abs(model.time - 0.02) < 1e-15
+    after = model.totals()
+    for name in before:
+        assert abs(after[name] - before[name]) < 1e-13
+    fine = model.dataset(1)
+    assert np.isfinite(fine["density"][8:24, 8:24]).all() and np.isnan(fine["density"][0, 0])
+    assert np.isfinite(model.dataset(0)["pressure"]).all()

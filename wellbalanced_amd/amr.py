@@ -54,8 +54,8 @@ class AMRMHD2D:
             raise ValueError("the conductivities must be non-negative")
         if precision not in PRECISIONS:
             raise ValueError(f"precision is one of {PRECISIONS}")
-        if tile not in (62, 126):
-            raise ValueError("tile is 62 or 126 cells")
+        if tile not in (30, 62, 126):
+            raise ValueError("tile is 30, 62 or 126 cells")
         self.hierarchy = hierarchy
         self.gamma = float(gamma)
         self.mean_field = (float(mean_field[0]), float(mean_field[1]))
@@ -95,8 +95,8 @@ class AMRMHD2D:
                    workgroups: int | None = None) -> None:
         """Change the launch shape (no effect on the results) and rebuild the work list."""
         if tile is not None:
-            if tile not in (62, 126):
-                raise ValueError("tile is 62 or 126 cells")
+            if tile not in (30, 62, 126):
+                raise ValueError("tile is 30, 62 or 126 cells")
             self.tile = tile
         if nxs is not None:
             self.nxs = int(nxs)
@@ -325,6 +325,17 @@ class AMRMHD2D:
             self.steps += 1
         return marks
 
+    def run(self, duration: float, *, max_steps: int = 1_000_000) -> int:
+        """Advance by ``duration``, the last step shortened (mhd.py:996-1008)."""
+        if not np.isfinite(duration) or duration < 0.0:
+            raise ValueError("duration must be a finite, non-negative time")
+        end = self.time + duration
+        taken = 0
+        while self.time < end * (1.0 - 1e-14) and taken < max_steps:
+            self.step(min(self.stable_step(), end - self.time))
+            taken += 1
+        return taken
+
     def current_tags(self, threshold: float, levels: int) -> dict[int, np.ndarray]:
         """Tag the cells where |J_z| times the cell size exceeds ``threshold`` on levels
         ``0 .. levels - 1`` that exist, patch by patch (``mhd.py:1011-1041``)."""
@@ -429,6 +440,39 @@ class AMRMHD2D:
         """Largest |div B| times the cell size over the cells of every patch
         (mhd.py:1114-1131)."""
         return float(self._divergences()[0])
+
+    def dataset(self, level: int = 0) -> dict[str, np.ndarray]:
+        """The primitive variables of ``level`` over its cell centres, the level's patches
+        gathered into arrays of the level's shape with NaN where it has no patch
+        (mhd.py:1177-1226; plain arrays instead of xarray)."""
+        if not 0 <= level < self.nlev:
+            raise ValueError(f"the hierarchy has levels 0 .. {self.nlev - 1}")
+        buf = self.buffer()
+        nx, ny = self.hierarchy.level_shape(level)
+        dx, dy = self.hierarchy.spacing(level)
+        names = ("density", "velocity_x", "velocity_y", "velocity_z", "pressure",
+                 "magnetic_field_x", "magnetic_field_y", "magnetic_field_z", "magnetic_pressure")
+        out = {name: np.full((nx, ny), np.nan) for name in names}
+        for lay in self.layouts:
+            if lay.level != level:
+                continue
+            c = buf[lay.offset : lay.offset + 6 * lay.plane].reshape(6, lay.X, lay.Y)
+            c = c[:, HALO:-HALO, HALO:-HALO]
+            a = buf[lay.offset + 6 * lay.plane : lay.offset + lay.size].reshape(lay.X + 1, lay.Y + 1)
+            a = a[HALO:-HALO, HALO:-HALO]
+            bxf = (a[:, 1:] - a[:, :-1]) / dy + self.mean_field[0]
+            byf = -(a[1:, :] - a[:-1, :]) / dx + self.mean_field[1]
+            bx, by = 0.5 * (bxf[:-1] + bxf[1:]), 0.5 * (byf[:, :-1] + byf[:, 1:])
+            rho = c[0]
+            u, v, w = c[1] / rho, c[2] / rho, c[3] / rho
+            b2 = bx**2 + by**2 + c[5] ** 2
+            p = (self.gamma - 1.0) * (c[4] - 0.5 * rho * (u**2 + v**2 + w**2) - 0.5 * b2)
+            (i0, i1), (j0, j1) = lay.box
+            for name, value in zip(names, (rho, u, v, w, p, bx, by, c[5], 0.5 * b2)):
+                out[name][i0:i1, j0:j1] = value
+        lo = self.hierarchy.lower_bound
+        return {"x": lo[0] + (np.arange(nx) + 0.5) * dx, "y": lo[1] + (np.arange(ny) + 0.5) * dy,
+                **out, "time": self.time}
 
     def leaf_values(self) -> dict[str, np.ndarray]:
         """The primitive variables on the leaf cells (mhd.py:1059-1112): ``x``, ``y``,

@@ -76,6 +76,9 @@ extern "C" int wb_amr2d_stage(const double* in, double* out, const void* patches
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, counter, 1);
   const unsigned grid = (unsigned)(workgroups > 0 ? workgroups : 1);
   const bool ni = a.diff.conduction() || eta != 0.0;
+  if (tile == 30)
+    return ni ? launch_batched<30, true>(a, patches, items, nitems, counter, out, grid, s)
+              : launch_batched<30, false>(a, patches, items, nitems, counter, out, grid, s);
   if (tile == 62)
     return ni ? launch_batched<62, true>(a, patches, items, nitems, counter, out, grid, s)
               : launch_batched<62, false>(a, patches, items, nitems, counter, out, grid, s);

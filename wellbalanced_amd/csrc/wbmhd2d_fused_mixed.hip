@@ -61,6 +61,9 @@ extern "C" int wb_amr2d_stage_mixed(const double* in, double* out, const void* p
   const bool ni = a.diff.conduction() || eta != 0.0;
   return with_limiter(limiter, [&](auto lim) {
     constexpr int L = decltype(lim)::value;
+    if (tile == 30)
+      return ni ? launch_batched<30, L, true>(a, patches, items, nitems, counter, out, grid, s)
+                : launch_batched<30, L, false>(a, patches, items, nitems, counter, out, grid, s);
     if (tile == 62)
       return ni ? launch_batched<62, L, true>(a, patches, items, nitems, counter, out, grid, s)
                 : launch_batched<62, L, false>(a, patches, items, nitems, counter, out, grid, s);

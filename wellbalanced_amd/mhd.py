@@ -46,7 +46,21 @@ def _device(device) -> torch.device:
 
 
 class MHD2D:
-    """2.5D ideal MHD on one uniform patch of an `AMRHierarchy` (see the module)."""
+    """2.5D MHD on one uniform patch of an `AMRHierarchy` (see the module).  On a
+    hierarchy with refined patches the constructor returns a `wellbalanced_amd.amr.
+    AMRMHD2D` with the same API (periodic domains, no gravity)."""
+
+    def __new__(cls, hierarchy: AMRHierarchy, **options):
+        if cls is MHD2D and len(getattr(hierarchy, "patches", ())) > 1:
+            from .amr import AMRMHD2D
+
+            if options.get("boundaries") or any(options.get("gravity", (0.0, 0.0))):
+                raise NotImplementedError("refined hierarchies run periodic and without gravity")
+            keep = ("gamma", "mean_field", "limiter", "cfl", "resistivity", "conduction",
+                    "precision", "device")
+            return AMRMHD2D(hierarchy, **{"limiter": "vanleer",
+                                          **{k: v for k, v in options.items() if k in keep}})
+        return super().__new__(cls)
 
     def __init__(self, hierarchy: AMRHierarchy, *, gamma: float = 5.0 / 3.0,
                  mean_field: tuple[float, float] = (0.0, 0.0), resistivity: float = 0.0,
@@ -93,8 +107,6 @@ class MHD2D:
             kinds[axis_name] = kind
         if any(gravity):
             raise NotImplementedError("the gravity source is outside this MHD path")
-        if len(hierarchy.patches) != 1:
-            raise NotImplementedError("refined patches run through wellbalanced_amd.amr.AMRMHD2D")
         self.hierarchy = hierarchy
         self.gamma = float(gamma)
         self.mean_field = (float(mean_field[0]), float(mean_field[1]))
