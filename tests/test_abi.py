@@ -72,3 +72,46 @@ def test_validation_messages_follow_the_reference():
         MHD2D(AMRHierarchy((4, 4)), boundaries={"z": "wall"})
     with pytest.raises(ValueError, match="mean field"):
         MHD2D(AMRHierarchy((4, 4)), boundaries={"y": "wall"}, mean_field=(0.0, 1.0))
+
+
+def test_new_entry_points_reject_bad_arguments_before_any_launch():
+    """Every check happens on the host, before a kernel is enqueued (NULL pointers are
+    never touched), so these calls run without a GPU."""
+    if not _lib.LIBRARY.exists():
+        pytest.skip("libwbmhd.so is not built")
+    lib = _lib.load()
+    E = _lib.WB_EINVAL
+    assert lib.wb_resistive2d(None, None, None, 2, 9, 0.1, 0.1, 0.1, None, None, None, None) == E
+    assert lib.wb_resistive2d(None, None, None, 9, 9, 0.1, 0.1, -1.0, None, None, None, None) == E
+    assert lib.wb_conductive2d(None, None, None, 9, 9, 0.1, 0.1, -1.0, 0.0, None, None, None) == E
+    assert lib.wb_stable_dt_diffusive(None, 0.4, 0.0, -1.0, 0.0, 1.0, None, None, None) == E
+    assert lib.wb_mhd2d_stage(None, None, 8, 8, 0.1, 0.1, 0.0, 0.0, 5 / 3, -1.0, 0.0, 0.0, 0,
+                              None, 0, 0, None, 8, 126, None) == E
+    assert lib.wb_amr2d_reflux(None, None, None, None, None, 1, 0.0, 0.0, 5 / 3, 0.0, -1.0,
+                               0.0, 0, None, 0, None) == E
+    assert lib.wb_mhd3d_resistive_workspace(0, 4, 4) == E
+    assert lib.wb_mhd3d_resistive(None, None, 0, 4, 4, 0.1, 0.1, 0.1, 0.0, 0.0, 0.0, 0.1, None,
+                                  0, None, None) == E
+    assert lib.wb_amr2d_regrid_copy(None, None, None, None, None, None, -1, None, None, 0,
+                                    None) == E
+    assert lib.wb_amr2d_divergence(None, None, None, 0, 0.0, 0.0, None, 0, None, 0, None,
+                                   None) == E
+    assert lib.wb_amr2d_leaf_values(None, None, None, -1, 0.0, 0.0, 5 / 3, None, None) == E
+    assert lib.wb_amr2d_rate(None, None, None, 0, 0.0, 0.0, 5 / 3, 0.0, 0.0, None, None,
+                             None) == E
+
+
+def test_negative_diffusivities_are_refused():
+    """test_mhd_amr.py:144-147 (and the conductivities, mhd.py:437-440)."""
+    from wellbalanced_amd import MHD2D, MHD3D
+    from wellbalanced_amd.amr import AMRMHD2D
+    from wellbalanced_amd.hierarchy import AMRHierarchy
+
+    with pytest.raises(ValueError, match="resistivity"):
+        MHD2D(AMRHierarchy((8, 8), periodic=(True, True)), resistivity=-1.0)
+    with pytest.raises(ValueError, match="conductivities"):
+        MHD2D(AMRHierarchy((8, 8), periodic=(True, True)), conduction=(1.0, -1.0))
+    with pytest.raises(ValueError, match="resistivity"):
+        AMRMHD2D(AMRHierarchy((8, 8), periodic=(True, True)), resistivity=-1.0)
+    with pytest.raises(ValueError, match="resistivity"):
+        MHD3D(AMRHierarchy((4, 4, 4), periodic=(True, True, True)), resistivity=-1.0)

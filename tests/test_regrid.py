@@ -376,3 +376,28 @@ def test_gpu_mhd2d_on_refined_patches_is_the_hierarchy_model(gpu):
     fine = model.dataset(1)
     assert np.isfinite(fine["density"][8:24, 8:24]).all() and np.isnan(fine["density"][0, 0])
     assert np.isfinite(model.dataset(0)["pressure"]).all()
+
+
+@pytest.mark.gpu
+def test_gpu_hierarchy_and_3d_bad_cells_raise_the_reference_error(gpu):
+    """A non-positive pressure in a stage raises FloatingPointError (mhd.py:852-857) on
+    the refined 2D hierarchy and in 3D, as in 2D."""
+    from wellbalanced_amd import MHD3D
+    from wellbalanced_amd.amr import AMRMHD2D
+
+    h = AMRHierarchy((16, 16), ((0.0, 1.0), (0.0, 1.0)), periodic=(True, True))
+    h.add_patch(1, ((8, 24), (8, 24)))
+    model = AMRMHD2D(h)
+    model.set_state(**_orszag_tang_state())
+    buf = model.buffer()
+    lay = model.layouts[1]
+    buf[lay.offset + 4 * lay.plane + 5 * lay.Y + 5] = -1.0  # energy of one fine cell
+    model.load_buffer(buf)
+    with pytest.raises(FloatingPointError, match="pressure"):
+        model.step(1e-4)
+    m3 = MHD3D.from_problem(initial.turbulence_3d(8), kernels="fused")
+    cons, pots = m3.arrays()
+    cons[4, 4, 4, 4] = -1.0
+    m3.load_arrays(cons, pots)
+    with pytest.raises(FloatingPointError, match="pressure"):
+        m3.step(1e-4)

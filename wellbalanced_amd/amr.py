@@ -73,6 +73,8 @@ class AMRMHD2D:
         self._t = torch.zeros(1, **opts)
         self._rate = torch.zeros(3, **opts)
         self._bad = torch.zeros(2, dtype=torch.int32, device=self.device)
+        # the bad-cell count of kernels outside the step (tagging): never checked
+        self._scratch_bad = torch.zeros(1, dtype=torch.int32, device=self.device)
         self._counter = torch.zeros(1, dtype=torch.int32, device=self.device)
         self._upload_tables()
         self.workgroups = 2048
@@ -359,7 +361,7 @@ class AMRMHD2D:
             plane = 8 * lay.plane
             _lib.call("wb_primitive2d", *[base + k * plane for k in range(6)], base + 6 * plane,
                       lay.X, lay.Y, dx, dy, *self.mean_field, self.gamma, _lib.ptr(prim),
-                      _lib.ptr(bxf), _lib.ptr(byf), _lib.ptr(ecc), self._bad.data_ptr() + 4,
+                      _lib.ptr(bxf), _lib.ptr(byf), _lib.ptr(ecc), _lib.ptr(self._scratch_bad),
                       stream)
             _lib.call("wb_current_tags2d", _lib.ptr(bxf), _lib.ptr(byf), lay.nx, lay.ny, dx, dy,
                       float(threshold), _lib.ptr(out), stream)

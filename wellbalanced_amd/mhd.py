@@ -141,6 +141,8 @@ class MHD2D:
         self._fy = torch.empty(7 * X * (Y - 3), **opts)
         self._ez = torch.empty((self.nx + 1) * (self.ny + 1), **opts)
         self._bad = torch.zeros(2, dtype=torch.int32, device=self.device)
+        # the bad-cell count of kernels outside the step (tagging): never checked
+        self._scratch_bad = torch.zeros(1, dtype=torch.int32, device=self.device)
         self._rate = torch.zeros(2, **opts)
         self._dt = torch.zeros(1, **opts)
         self._t = torch.zeros(1, **opts)
@@ -377,7 +379,7 @@ class MHD2D:
         _lib.call("wb_primitive2d", *[base + k * plane for k in range(6)], base + self._cells * 8,
                   X, Y, self.dx, self.dy, *self.mean_field, self.gamma, _lib.ptr(self._prim),
                   _lib.ptr(self._bxf), _lib.ptr(self._byf), _lib.ptr(self._ecc),
-                  self._bad.data_ptr() + 4, stream)
+                  _lib.ptr(self._scratch_bad), stream)
         out = torch.empty(nx * ny, dtype=torch.uint8, device=self.device)
         _lib.call("wb_current_tags2d", _lib.ptr(self._bxf), _lib.ptr(self._byf), nx, ny, self.dx,
                   self.dy, float(threshold), _lib.ptr(out), stream)

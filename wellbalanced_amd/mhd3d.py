@@ -118,6 +118,8 @@ class MHD3D:
         self._buf = [torch.zeros(size, **opts), torch.zeros(size, **opts)]
         self._work: dict[str, torch.Tensor] = {}
         self._bad = torch.zeros(2, dtype=torch.int32, device=self.device)
+        # the bad-cell count of kernels outside the step (tagging): never checked
+        self._scratch_bad = torch.zeros(1, dtype=torch.int32, device=self.device)
         self._rate = torch.zeros(2, **opts)
         self._dt = torch.zeros(1, **opts)
         self._t = torch.zeros(1, **opts)
@@ -379,7 +381,7 @@ class MHD3D:
         _lib.call("wb_mhd3d_fill_halos", _lib.ptr(src), nx, ny, nz, pmask, stream)
         _lib.call("wb_primitive3d", _lib.ptr(src), nx, ny, nz, *self.spacing, *self.mean_field,
                   self.gamma, _lib.ptr(w["prim"]), _lib.ptr(w["f0"]), _lib.ptr(w["f1"]),
-                  _lib.ptr(w["f2"]), self._bad.data_ptr() + 4, stream)
+                  _lib.ptr(w["f2"]), _lib.ptr(self._scratch_bad), stream)
         out = torch.empty(nx * ny * nz, dtype=torch.uint8, device=self.device)
         _lib.call("wb_current_tags3d", _lib.ptr(w["prim"]), nx, ny, nz, *self.spacing,
                   float(threshold), _lib.ptr(out), stream)
