@@ -517,3 +517,58 @@ def test_gpu_3d_resistive_mixed_is_close_and_conserves(gpu):
     for name in before:
         assert abs(after[name] - before[name]) < 1e-12
     assert mixed.divergence() < 1e-13
+
+
+def _beltrami(eta=0.01):
+    """test_mhd3d.py:95-123: A = (cos kz, sin kz, 0) / k on 4 x 4 x 16 periodic cells gives
+    B = (cos kz, sin kz, 0), J = k B: no force, the field decays as exp(-eta k^2 t)."""
+    from wellbalanced_amd.hierarchy import AMRHierarchy as H
+
+    h = H((4, 4, 16), ((0.0, 1.0),) * 3, periodic=(True,) * 3)
+    k = KW
+    state = dict(density=1.0, velocity=(0.0, 0.0, 0.0), pressure=1.0,
+                 vector_potential=(lambda x, y, z: np.cos(k * z) / k,
+                                   lambda x, y, z: np.sin(k * z) / k, 0.0))
+    return initial.Problem(h, {"resistivity": eta}, state)
+
+
+def _beltrami_ratio(pb_before, pb_after, eta, duration):
+    h = 1.0 / 16
+    k_discrete = 2.0 * np.sin(KW * h / 2) / h
+    return pb_after / pb_before, np.exp(-2 * eta * k_discrete**2 * duration)
+
+
+def test_3d_resistivity_damps_a_force_free_field_at_the_diffusion_rate():
+    """The oracle on test_mhd3d.py:95-123 (ratio to 2e-3 of the discrete rate)."""
+    from oracle.mhd3d import cell_fields, face_fields
+    from tests.helpers import oracle3d
+
+    model = oracle3d(_beltrami())
+
+    def magnetic_pressure():
+        h = 2
+        local = [model.pot[c][tuple(slice(h, h + model.n[k] + (k != c)) for k in range(3))]
+                 for c in range(3)]
+        bx, by, bz = cell_fields(face_fields(local, model.spacing, model.b0))
+        return float(np.mean(0.5 * (bx * bx + by * by + bz * bz)))
+
+    before = magnetic_pressure()
+    while model.time < 0.5 * (1 - 1e-14):
+        model.step(min(model.stable_step(), 0.5 - model.time))
+    ratio, want = _beltrami_ratio(before, magnetic_pressure(), 0.01, model.time)
+    assert ratio == pytest.approx(want, rel=2e-3)
+    assert np.abs(model.interior()[1]).max() < 1e-12
+
+
+@pytest.mark.gpu
+def test_gpu_3d_resistivity_damps_a_force_free_field_at_the_diffusion_rate(gpu):
+    """test_mhd3d.py:95-123 on the device, through the fused stage."""
+    from wellbalanced_amd import MHD3D
+
+    model = MHD3D.from_problem(_beltrami(), kernels="fused")
+    before = float(model.dataset()["magnetic_pressure"].mean())
+    model.run(0.5)
+    after = model.dataset()
+    ratio, want = _beltrami_ratio(before, float(after["magnetic_pressure"].mean()), 0.01, 0.5)
+    assert ratio == pytest.approx(want, rel=2e-3)
+    assert np.abs(after["velocity_x"]).max() < 1e-12
