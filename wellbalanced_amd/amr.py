@@ -39,7 +39,8 @@ class AMRMHD2D:
     def __init__(self, hierarchy: AMRHierarchy, *, gamma: float = 5.0 / 3.0,
                  mean_field=(0.0, 0.0), limiter: str = "minmod", cfl: float = 0.4,
                  resistivity: float = 0.0, conduction: tuple[float, float] = (0.0, 0.0),
-                 precision: str = "fp64", tile: int = 62, nxs: int = 16, device=None) -> None:
+                 prefix: str = "mhd_", precision: str = "fp64", tile: int = 62, nxs: int = 16,
+                 device=None) -> None:
         if hierarchy.ndim != 2:
             raise ValueError("AMRMHD2D needs a 2D hierarchy")
         if limiter not in LIMITER_CODES:
@@ -66,6 +67,8 @@ class AMRMHD2D:
         self.precision = precision
         self.tile = tile
         self.nxs = nxs
+        self.names = {name: prefix + name for name in CONSERVED}
+        self.potential = prefix + "az"
         self.device = _device(device)
         self._use(hierarchy, amr_maps.build(hierarchy), upload=False)
         opts = dict(dtype=torch.float64, device=self.device)
@@ -85,6 +88,8 @@ class AMRMHD2D:
     def _use(self, hierarchy: AMRHierarchy, maps, upload: bool = True) -> None:
         """Take ``hierarchy`` (its tables ``maps``) with new, zeroed state buffers."""
         self.hierarchy = hierarchy
+        hierarchy._bind(self, {**{n: "cell" for n in self.names.values()},
+                               self.potential: "node"}, HALO)
         self.maps = maps
         self.layouts = maps.layouts
         self.nlev = hierarchy.max_level + 1
@@ -222,6 +227,30 @@ class AMRMHD2D:
     def cells(self) -> int:
         """Cells on every level (the zone-updates of one step)."""
         return int(sum(lay.nx * lay.ny for lay in self.layouts))
+
+    # -- the hierarchy's field interface (patch.field(name), gather, integral, ...) -------
+    def _field_view(self, index: int, name: str) -> torch.Tensor:
+        """The haloed device view of field ``name`` on patch ``index`` (the hierarchy's
+        order) in the current state."""
+        lay = self.layouts[index]
+        buf = self._buf[0]
+        if name == self.potential:
+            start = lay.offset + 6 * lay.plane
+            return buf[start : lay.offset + lay.size].view(lay.X + 1, lay.Y + 1)
+        start = lay.offset + list(self.names.values()).index(name) * lay.plane
+        return buf[start : start + lay.plane].view(lay.X, lay.Y)
+
+    def _exchange_halos(self) -> None:
+        self._fill(self._buf[0], _lib.stream_handle(self.device))
+
+    def _restrict_fields(self) -> None:
+        stream = _lib.stream_handle(self.device)
+        for L in range(self.nlev - 2, -1, -1):
+            self._coarsen(self._buf[0], L, stream)
+
+    def _integral(self, name: str) -> float:
+        key = next(k for k, v in self.names.items() if v == name)
+        return self.totals()[key]
 
     # -- the step ------------------------------------------------------------------------
     def _fill_level(self, buf: torch.Tensor, L: int, stream: int) -> None:

@@ -57,7 +57,7 @@ class MHD2D:
             if options.get("boundaries") or any(options.get("gravity", (0.0, 0.0))):
                 raise NotImplementedError("refined hierarchies run periodic and without gravity")
             keep = ("gamma", "mean_field", "limiter", "cfl", "resistivity", "conduction",
-                    "precision", "device")
+                    "prefix", "precision", "device")
             return AMRMHD2D(hierarchy, **{"limiter": "vanleer",
                                           **{k: v for k, v in options.items() if k in keep}})
         return super().__new__(cls)
@@ -119,6 +119,8 @@ class MHD2D:
         self.periodic = tuple(hierarchy.periodic)
         self.names = {name: prefix + name for name in CONSERVED}
         self.potential = prefix + "az"
+        hierarchy._bind(self, {**{n: "cell" for n in self.names.values()},
+                               self.potential: "node"}, HALO)
         self.precision = precision
         self.kernels = kernels
         self.nxs = 32
@@ -227,6 +229,27 @@ class MHD2D:
     def _check_ready(self) -> None:
         if not self._ready:
             raise RuntimeError("set the state first (set_state)")
+
+    # -- the hierarchy's field interface (patch.field(name), gather, integral, ...) ------
+    def _field_view(self, index: int, name: str) -> torch.Tensor:
+        """The haloed device view of field ``name`` in the current state."""
+        cells, az = self._views(self._buf[0])
+        if name == self.potential:
+            return az
+        return cells[list(self.names.values()).index(name)]
+
+    def _exchange_halos(self) -> None:
+        pmask = int(self.periodic[0]) | (int(self.periodic[1]) << 1)
+        wmask = int(self.walls[0]) | (int(self.walls[1]) << 1)
+        _lib.call("wb_mhd2d_fill_halos", _lib.ptr(self._buf[0]), self.nx, self.ny, pmask, wmask,
+                  _lib.stream_handle(self.device))
+
+    def _restrict_fields(self) -> None:
+        """One level: nothing to restrict."""
+
+    def _integral(self, name: str) -> float:
+        key = next(k for k, v in self.names.items() if v == name)
+        return self._totals()[key]
 
     # -- the step --------------------------------------------------------------------
     def _wall_mask(self) -> int:
@@ -431,6 +454,9 @@ class MHD2D:
     def totals(self) -> dict[str, float]:
         """Integrals of the conserved fields (a fixed-order FP64 tree on the device)."""
         self._check_ready()
+        return self._totals()
+
+    def _totals(self) -> dict[str, float]:
         patch, leaf, _, _, work = self._diag_tables()
         out = torch.zeros(6, dtype=torch.float64, device=self.device)
         _lib.call("wb_amr2d_totals", _lib.ptr(self._buf[0]), _lib.ptr(patch), _lib.ptr(leaf),

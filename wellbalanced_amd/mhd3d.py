@@ -99,6 +99,9 @@ class MHD3D:
         self.periodic = tuple(hierarchy.periodic)
         self.names = {name: prefix + name for name in CONSERVED_3D}
         self.potential = tuple(prefix + n for n in ("ax", "ay", "az"))
+        hierarchy._bind(self, {**{n: "cell" for n in self.names.values()},
+                               **dict(zip(self.potential, ("x-edge", "y-edge", "z-edge")))},
+                        HALO)
         self.precision = precision
         self.kernels = kernels
         self.device = _device(device)
@@ -417,8 +420,32 @@ class MHD3D:
             out.append(p[tuple(slice(HALO, HALO + self.n[k] + (k != c)) for k in range(3))])
         return out
 
+    # -- the hierarchy's field interface (patch.field(name), gather, integral, ...) -------
+    def _field_view(self, index: int, name: str) -> torch.Tensor:
+        """The haloed device view of field ``name`` in the current state."""
+        cells, pots = self._views(self._buf[0])
+        if name in self.potential:
+            return pots[self.potential.index(name)]
+        return cells[list(self.names.values()).index(name)]
+
+    def _exchange_halos(self) -> None:
+        nx, ny, nz = self.n
+        pmask = sum(int(p) << a for a, p in enumerate(self.periodic))
+        _lib.call("wb_mhd3d_fill_halos", _lib.ptr(self._buf[0]), nx, ny, nz, pmask,
+                  _lib.stream_handle(self.device))
+
+    def _restrict_fields(self) -> None:
+        """One level: nothing to restrict."""
+
+    def _integral(self, name: str) -> float:
+        key = next(k for k, v in self.names.items() if v == name)
+        return self._totals()[key]
+
     def totals(self) -> dict[str, float]:
         self._check_ready()
+        return self._totals()
+
+    def _totals(self) -> dict[str, float]:
         cells, _ = self._views(self._buf[0])
         nx, ny, nz = self.n
         inner = cells[:, HALO : HALO + nx, HALO : HALO + ny, HALO : HALO + nz]

@@ -146,7 +146,6 @@ def regrid(hierarchy: AMRHierarchy, tags: dict, *, max_patch_size: int | None = 
         found = cluster(f, efficiency=efficiency, max_size=fine_max)
         boxes[L + 1] = [((2 * i0, 2 * i1), (2 * j0, 2 * j1)) for (i0, i1), (j0, j1) in found]
     out = AMRHierarchy(hierarchy.base_shape, hierarchy.bounds, hierarchy.periodic)
-    out.patches = [p for p in hierarchy.patches if p.level == 0]
     for L in sorted(boxes):
         for box in boxes[L]:
             out.add_patch(L, box)
