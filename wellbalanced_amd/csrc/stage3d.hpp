@@ -88,41 +88,137 @@ struct EdgePot {
   }
 };
 
-// The primitive variables of one cell (mhd3d.py:475-509).  FAST (the mixed path)
-// multiplies by the reciprocal spacings and divides once for 1/rho; otherwise the
-// divisions of the reference are kept for bit-for-bit parity.
+// Byte addressing of the state in the stage kernel: every array is a base pointer that
+// is uniform over the workgroup (the array and the x-plane: scalar arithmetic) plus a
+// per-thread 32-bit byte offset of the (y, z) column, so an element's address costs no
+// vector multiply.  Planes are at most 4100 x 4101 doubles (the entry points' bound),
+// far below 4 GiB.
+struct Addr3 {
+  size_t cplane, xplane, yplane, zplane;  // bytes per x-plane: a cell field, A_x, A_y, A_z
+  size_t field;                           // bytes per cell field
+  size_t ox, oy, oz;                      // byte offsets of A_x, A_y, A_z in the state
+  uint32_t crow, arow;                    // bytes per y-row: cells and A_z; A_x and A_y
+};
+
+__device__ __forceinline__ Addr3 addr3(const StageArgs& a) {
+  const Grid3 g = a.g;
+  Addr3 d;
+  d.crow = (uint32_t)g.Z * 8u;
+  d.arow = (uint32_t)(g.Z + 1) * 8u;
+  d.cplane = (size_t)g.Y * g.Z * 8;
+  d.xplane = (size_t)(g.Y + 1) * (g.Z + 1) * 8;
+  d.yplane = (size_t)g.Y * (g.Z + 1) * 8;
+  d.zplane = (size_t)(g.Y + 1) * g.Z * 8;
+  d.field = (size_t)g.X * d.cplane;
+  d.ox = a.off_ax * 8;
+  d.oy = a.off_ay * 8;
+  d.oz = a.off_az * 8;
+  return d;
+}
+
+__device__ __forceinline__ double ldb(const char* base, uint32_t off) {
+  return *reinterpret_cast<const double*>(base + off);
+}
+__device__ __forceinline__ void stb(char* base, uint32_t off, double v) {
+  *reinterpret_cast<double*>(base + off) = v;
+}
+
+// The primitive variables of one cell (mhd3d.py:475-509) from its 5 conserved values
+// and the 12 potential values around it: ax[a][b] = A_x(i, j+a, k+b), ay[a][b] =
+// A_y(i+a, j, k+b), az[a][b] = A_z(i+a, j+b, k).  Also the face fields of the cell's
+// low x, y and z faces (fx, fy, fz, mean field included), which are the B_n of the
+// faces the stage solves there.  FAST (the mixed path) multiplies by the reciprocal
+// spacings and divides once for 1/rho; otherwise the divisions of the reference are
+// kept for bit-for-bit parity (mhd3d.py:361-378).
 template <bool FAST>
 struct CellPrim {
   double r, u, v, w, p, bx, by, bz;
-  __device__ __forceinline__ void compute(const StageArgs& a, const EdgePot& A, size_t PL, int i,
-                                          int j, int k, double rdx, double rdy, double rdz) {
-    const size_t c = cidx(a.g, i, j, k);
+  double fx, fy, fz;
+  __device__ __forceinline__ void finish(const StageArgs& a, const double (&ax)[2][2],
+                                         const double (&ay)[2][2], const double (&az)[2][2],
+                                         const double (&cv)[5], double rdx, double rdy,
+                                         double rdz) {
     if (FAST) {
-      const double bx0 = (A.AZ(i, j + 1, k) - A.AZ(i, j, k)) * rdy - (A.AY(i, j, k + 1) - A.AY(i, j, k)) * rdz;
-      const double bx1 = (A.AZ(i + 1, j + 1, k) - A.AZ(i + 1, j, k)) * rdy - (A.AY(i + 1, j, k + 1) - A.AY(i + 1, j, k)) * rdz;
-      const double by0 = (A.AX(i, j, k + 1) - A.AX(i, j, k)) * rdz - (A.AZ(i + 1, j, k) - A.AZ(i, j, k)) * rdx;
-      const double by1 = (A.AX(i, j + 1, k + 1) - A.AX(i, j + 1, k)) * rdz - (A.AZ(i + 1, j + 1, k) - A.AZ(i, j + 1, k)) * rdx;
-      const double bz0 = (A.AY(i + 1, j, k) - A.AY(i, j, k)) * rdx - (A.AX(i, j + 1, k) - A.AX(i, j, k)) * rdy;
-      const double bz1 = (A.AY(i + 1, j, k + 1) - A.AY(i, j, k + 1)) * rdx - (A.AX(i, j + 1, k + 1) - A.AX(i, j, k + 1)) * rdy;
+      const double bx0 = (az[0][1] - az[0][0]) * rdy - (ay[0][1] - ay[0][0]) * rdz;
+      const double bx1 = (az[1][1] - az[1][0]) * rdy - (ay[1][1] - ay[1][0]) * rdz;
+      const double by0 = (ax[0][1] - ax[0][0]) * rdz - (az[1][0] - az[0][0]) * rdx;
+      const double by1 = (ax[1][1] - ax[1][0]) * rdz - (az[1][1] - az[0][1]) * rdx;
+      const double bz0 = (ay[1][0] - ay[0][0]) * rdx - (ax[1][0] - ax[0][0]) * rdy;
+      const double bz1 = (ay[1][1] - ay[0][1]) * rdx - (ax[1][1] - ax[0][1]) * rdy;
       bx = 0.5 * (bx0 + bx1) + a.b0x;
       by = 0.5 * (by0 + by1) + a.b0y;
       bz = 0.5 * (bz0 + bz1) + a.b0z;
-      r = a.in[c];
+      fx = bx0 + a.b0x;
+      fy = by0 + a.b0y;
+      fz = bz0 + a.b0z;
+      r = cv[0];
       const double ri = 1.0 / r;
-      u = a.in[PL + c] * ri;
-      v = a.in[2 * PL + c] * ri;
-      w = a.in[3 * PL + c] * ri;
+      u = cv[1] * ri;
+      v = cv[2] * ri;
+      w = cv[3] * ri;
     } else {
-      bx = 0.5 * (A.bx(i, j, k, a.dy, a.dz, a.b0x) + A.bx(i + 1, j, k, a.dy, a.dz, a.b0x));
-      by = 0.5 * (A.by(i, j, k, a.dz, a.dx, a.b0y) + A.by(i, j + 1, k, a.dz, a.dx, a.b0y));
-      bz = 0.5 * (A.bz(i, j, k, a.dx, a.dy, a.b0z) + A.bz(i, j, k + 1, a.dx, a.dy, a.b0z));
-      r = a.in[c];
-      u = a.in[PL + c] / r;
-      v = a.in[2 * PL + c] / r;
-      w = a.in[3 * PL + c] / r;
+      const double bxa = (az[0][1] - az[0][0]) / a.dy - (ay[0][1] - ay[0][0]) / a.dz + a.b0x;
+      const double bxb = (az[1][1] - az[1][0]) / a.dy - (ay[1][1] - ay[1][0]) / a.dz + a.b0x;
+      const double bya = (ax[0][1] - ax[0][0]) / a.dz - (az[1][0] - az[0][0]) / a.dx + a.b0y;
+      const double byb = (ax[1][1] - ax[1][0]) / a.dz - (az[1][1] - az[0][1]) / a.dx + a.b0y;
+      const double bza = (ay[1][0] - ay[0][0]) / a.dx - (ax[1][0] - ax[0][0]) / a.dy + a.b0z;
+      const double bzb = (ay[1][1] - ay[0][1]) / a.dx - (ax[1][1] - ax[0][1]) / a.dy + a.b0z;
+      bx = 0.5 * (bxa + bxb);
+      by = 0.5 * (bya + byb);
+      bz = 0.5 * (bza + bzb);
+      fx = bxa;
+      fy = bya;
+      fz = bza;
+      r = cv[0];
+      u = cv[1] / r;
+      v = cv[2] / r;
+      w = cv[3] / r;
     }
-    p = (a.gamma - 1.0) * (a.in[4 * PL + c] - 0.5 * r * (u * u + v * v + w * w) -
+    p = (a.gamma - 1.0) * (cv[4] - 0.5 * r * (u * u + v * v + w * w) -
                            0.5 * (bx * bx + by * by + bz * bz));
+  }
+  // cell (i, j, k) of the haloed grid a.g, by element indices
+  __device__ __forceinline__ void compute(const StageArgs& a, const EdgePot& A, size_t PL, int i,
+                                          int j, int k, double rdx, double rdy, double rdz) {
+    const size_t c = cidx(a.g, i, j, k);
+    double ax[2][2], ay[2][2], az[2][2], cv[5];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        ax[s][t] = A.AX(i, j + s, k + t);
+        ay[s][t] = A.AY(i + s, j, k + t);
+        az[s][t] = A.AZ(i + s, j + t, k);
+      }
+#pragma unroll
+    for (int q = 0; q < 5; ++q) cv[q] = a.in[q * PL + c];
+    finish(a, ax, ay, az, cv, rdx, rdy, rdz);
+  }
+  // the cell of x-plane i whose column has the byte offsets c0 (cells, A_z) and c1
+  // (A_x, A_y): the uniform plane bases carry every other displacement
+  __device__ __forceinline__ void load(const StageArgs& a, const Addr3& d, const char* in, int i,
+                                       uint32_t c0, uint32_t c1, double rdx, double rdy,
+                                       double rdz) {
+    const char* px = in + d.ox + (size_t)i * d.xplane;
+    const char* py = in + d.oy + (size_t)i * d.yplane;
+    const char* pz = in + d.oz + (size_t)i * d.zplane;
+    const char* pc = in + (size_t)i * d.cplane;
+    double ax[2][2], ay[2][2], az[2][2], cv[5];
+    ax[0][0] = ldb(px, c1);
+    ax[0][1] = ldb(px + 8, c1);
+    ax[1][0] = ldb(px + d.arow, c1);
+    ax[1][1] = ldb(px + d.arow + 8, c1);
+    ay[0][0] = ldb(py, c1);
+    ay[0][1] = ldb(py + 8, c1);
+    ay[1][0] = ldb(py + d.yplane, c1);
+    ay[1][1] = ldb(py + d.yplane + 8, c1);
+    az[0][0] = ldb(pz, c0);
+    az[0][1] = ldb(pz + d.crow, c0);
+    az[1][0] = ldb(pz + d.zplane, c0);
+    az[1][1] = ldb(pz + d.zplane + d.crow, c0);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) cv[q] = ldb(pc + q * d.field, c0);
+    finish(a, ax, ay, az, cv, rdx, rdy, rdz);
   }
   __device__ __forceinline__ bool bad() const {
     return !(finite(r) && finite(u) && finite(v) && finite(w) && finite(p) && finite(bx) &&
@@ -153,8 +249,11 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
   const int ly = fa - 1, lz = fb - 1;
   const int gy = jy0 + ly, gz = kz0 + lz;
   const int cgy = min(max(gy, 0), g.Y - 1), cgz = min(max(gz, 0), g.Z - 1);
-  const EdgePot A{a.in + a.off_ax, a.in + a.off_ay, a.in + a.off_az, g.Y, g.Z};
-  const size_t PL = (size_t)g.X * g.Y * g.Z;
+  const Addr3 d = addr3(a);
+  const char* inb = reinterpret_cast<const char*>(a.in);
+  char* outb = reinterpret_cast<char*>(a.out);
+  const uint32_t c0 = (uint32_t)cgy * d.crow + (uint32_t)cgz * 8u;  // cells, A_z
+  const uint32_t c1 = (uint32_t)cgy * d.arow + (uint32_t)cgz * 8u;  // A_x, A_y
   const double dt = *a.dt;
   const double dtdx = dt / a.dx, dtdy = dt / a.dy, dtdz = dt / a.dz;
   const double rdx = 1.0 / a.dx, rdy = 1.0 / a.dy, rdz = 1.0 / a.dz;
@@ -163,6 +262,7 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
   // register state of the column
   P q[7][4];                       // x-frame primitives of planes f-2 .. f+1
   P own[8];                        // the column's primitives of plane f+1, until LDS is free
+  P bnp[3], bnn[3];                // face fields of the column's low x/y/z faces: planes f, f+1
   P fprev[5], fcur[5];             // x-face fluxes (rows 0-4) of faces f-1 and f
   double dyy[5], dzz[5];
 #pragma unroll
@@ -173,13 +273,17 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
   for (int r = 0; r < 5; ++r) { fprev[r] = fcur[r] = P(0); dyy[r] = dzz[r] = 0.0; }
 #pragma unroll
   for (int r = 0; r < 8; ++r) own[r] = P(0);
+#pragma unroll
+  for (int r = 0; r < 3; ++r) bnp[r] = bnn[r] = P(0);
 
   // the own column of plane `pl`: registers (ring + own), cell E, bad cells
   auto own_column = [&](int pl) {
     if (!fthread) return;
     const int pc = min(max(pl, 0), g.X - 1);
     CellPrim<FAST> c;
-    c.compute(a, A, PL, pc, cgy, cgz, rdx, rdy, rdz);
+    c.load(a, d, inb, pc, c0, c1, rdx, rdy, rdz);
+    bnp[0] = bnn[0]; bnp[1] = bnn[1]; bnp[2] = bnn[2];
+    bnn[0] = P(c.fx); bnn[1] = P(c.fy); bnn[2] = P(c.fz);
     own[0] = P(c.r); own[1] = P(c.u); own[2] = P(c.v); own[3] = P(c.w);
     own[4] = P(c.p); own[5] = P(c.bx); own[6] = P(c.by); own[7] = P(c.bz);
     constexpr int comp[7] = {0, 1, 2, 3, 4, 6, 7};
@@ -196,6 +300,17 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
     if (ly >= 0 && ly < tyv && lz >= 0 && lz < tzv && pl >= ix0 && pl < ix0 + nxv && c.bad())
       atomicAdd(a.bad, 1);
   };
+  // the outermost ring of primitive columns: one column per thread (tid < NH)
+  constexpr int NH = T::NP - T::NF;
+  static_assert(NH <= T::THREADS, "one ring column per thread");
+  const bool ring = tid < NH;
+  int rpy, rpz;
+  if (tid < T::PZ) { rpy = 0; rpz = tid; }
+  else if (tid < 2 * T::PZ) { rpy = T::PY - 1; rpz = tid - T::PZ; }
+  else { const int e = tid - 2 * T::PZ; rpy = 1 + e / 2; rpz = (e & 1) ? T::PZ - 1 : 0; }
+  const int rj = min(max(jy0 + rpy - 2, 0), g.Y - 1), rk = min(max(kz0 + rpz - 2, 0), g.Z - 1);
+  const uint32_t r0 = (uint32_t)rj * d.crow + (uint32_t)rk * 8u;
+  const uint32_t r1 = (uint32_t)rj * d.arow + (uint32_t)rk * 8u;
   // plane `pl` into LDS: the own column from registers, the outer ring computed here
   auto store_plane = [&](int pl) {
     if (fthread) {
@@ -203,33 +318,15 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
 #pragma unroll
       for (int r = 0; r < 8; ++r) sh.prim[r][idx] = own[r];
     }
-    constexpr int NH = T::NP - T::NF;  // the outermost ring of primitive columns
     const int pc = min(max(pl, 0), g.X - 1);
-    for (int hidx = tid; hidx < NH; hidx += T::THREADS) {
-      int py, pz;
-      if (hidx < T::PZ) { py = 0; pz = hidx; }
-      else if (hidx < 2 * T::PZ) { py = T::PY - 1; pz = hidx - T::PZ; }
-      else { const int e = hidx - 2 * T::PZ; py = 1 + e / 2; pz = (e & 1) ? T::PZ - 1 : 0; }
-      const int j = min(max(jy0 + py - 2, 0), g.Y - 1), k = min(max(kz0 + pz - 2, 0), g.Z - 1);
+    if (ring) {
       CellPrim<FAST> c;
-      c.compute(a, A, PL, pc, j, k, rdx, rdy, rdz);
-      const int idx = py * T::PZ + pz;
+      c.load(a, d, inb, pc, r0, r1, rdx, rdy, rdz);
+      const int idx = rpy * T::PZ + rpz;
       sh.prim[0][idx] = P(c.r); sh.prim[1][idx] = P(c.u); sh.prim[2][idx] = P(c.v);
       sh.prim[3][idx] = P(c.w); sh.prim[4][idx] = P(c.p); sh.prim[5][idx] = P(c.bx);
       sh.prim[6][idx] = P(c.by); sh.prim[7][idx] = P(c.bz);
     }
-  };
-  auto face_bx = [&](int i) -> double {
-    if (FAST) return (A.AZ(i, cgy + 1, cgz) - A.AZ(i, cgy, cgz)) * rdy - (A.AY(i, cgy, cgz + 1) - A.AY(i, cgy, cgz)) * rdz + a.b0x;
-    return A.bx(i, cgy, cgz, a.dy, a.dz, a.b0x);
-  };
-  auto face_by = [&](int i) -> double {
-    if (FAST) return (A.AX(i, cgy, cgz + 1) - A.AX(i, cgy, cgz)) * rdz - (A.AZ(i + 1, cgy, cgz) - A.AZ(i, cgy, cgz)) * rdx + a.b0y;
-    return A.by(i, cgy, cgz, a.dz, a.dx, a.b0y);
-  };
-  auto face_bz = [&](int i) -> double {
-    if (FAST) return (A.AY(i + 1, cgy, cgz) - A.AY(i, cgy, cgz)) * rdx - (A.AX(i, cgy + 1, cgz) - A.AX(i, cgy, cgz)) * rdy + a.b0z;
-    return A.bz(i, cgy, cgz, a.dx, a.dy, a.b0z);
   };
 
   // prologue: planes ix0-2 (ring) and ix0-1 (ring + LDS)
@@ -245,7 +342,7 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
     if (fthread) {
       if (f >= ix0) {
         P fl[7];
-        face_solve<P, LIM>(q, a.limiter, P(face_bx(f)), gam, inv_gm1, fl);
+        face_solve<P, LIM>(q, a.limiter, bnp[0], gam, inv_gm1, fl);
 #pragma unroll
         for (int r = 0; r < 5; ++r) {
           fprev[r] = fcur[r];
@@ -257,7 +354,6 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
       }
       // d. y- and z-faces of plane f below the column's cell, from the LDS plane
       const int pidx = (fa + 1) * T::PZ + (fb + 1);
-      const int fi = min(max(f, 0), g.X - 1);
       if (fa >= 1) {
         constexpr int comp[7] = {0, 2, 3, 1, 4, 7, 5};
         P qq[7][4];
@@ -266,7 +362,7 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
 #pragma unroll
           for (int s = 0; s < 4; ++s) qq[c][s] = sh.prim[comp[c]][pidx + (s - 2) * T::PZ];
         P fl[7];
-        face_solve<P, LIM>(qq, a.limiter, P(face_by(fi)), gam, inv_gm1, fl);
+        face_solve<P, LIM>(qq, a.limiter, bnp[1], gam, inv_gm1, fl);
 #pragma unroll
         for (int r = 0; r < 7; ++r) sh.fy[r][tid] = fl[r];
         sh.fyc[cur][0][tid] = fl[0];
@@ -280,7 +376,7 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
 #pragma unroll
           for (int s = 0; s < 4; ++s) qq[c][s] = sh.prim[comp[c]][pidx + (s - 2)];
         P fl[7];
-        face_solve<P, LIM>(qq, a.limiter, P(face_bz(fi)), gam, inv_gm1, fl);
+        face_solve<P, LIM>(qq, a.limiter, bnp[2], gam, inv_gm1, fl);
 #pragma unroll
         for (int r = 0; r < 7; ++r) sh.fz[r][tid] = fl[r];
         sh.fzc[cur][0][tid] = fl[0];
@@ -304,10 +400,10 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
             double(sh.fy[0][tzm]), double(sh.fy[0][tid]),
             double(sh.fz[0][tym]), double(sh.fz[0][tid]),
             sh.ex[f & 1][tymz], sh.ex[f & 1][tzm], sh.ex[f & 1][tym], sh.ex[f & 1][tid]);
-        const size_t o = a.off_ax + ((size_t)f * (g.Y + 1) + gy) * (g.Z + 1) + gz;
-        double v = a.in[o] - dt * e;
-        if (a.heun) v = (v + a.out[o]) * 0.5;
-        a.out[o] = v;
+        const size_t o = d.ox + (size_t)f * d.xplane;  // E_x's A_x at (f, gy, gz): column c1
+        double v = ldb(inb + o, c1) - dt * e;
+        if (a.heun) v = (v + ldb(outb + o, c1)) * 0.5;
+        stb(outb + o, c1, v);
       }
       const bool node_x = f >= ix0 && (f < ix0 + nxv || (f == ix0 + nxv && last_x));
       // f2. E_y at (x-node f, y-cell ly, z-node lz): plane (a=z, b=x)
@@ -318,10 +414,10 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
             double(sh.fzc[prv][0][tid]), double(sh.fzc[cur][0][tid]),
             double(sh.fx[0][tzm]), double(sh.fx[0][tid]),
             sh.ey[e0][tzm], sh.ey[e0][tid], sh.ey[e1][tzm], sh.ey[e1][tid]);
-        const size_t o = a.off_ay + ((size_t)f * g.Y + gy) * (g.Z + 1) + gz;
-        double v = a.in[o] - dt * e;
-        if (a.heun) v = (v + a.out[o]) * 0.5;
-        a.out[o] = v;
+        const size_t o = d.oy + (size_t)f * d.yplane;
+        double v = ldb(inb + o, c1) - dt * e;
+        if (a.heun) v = (v + ldb(outb + o, c1)) * 0.5;
+        stb(outb + o, c1, v);
       }
       // f3. E_z at (x-node f, y-node ly, z-cell lz): plane (a=x, b=y)
       if (node_x && in_y && cell_z) {
@@ -331,21 +427,22 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
             double(sh.fx[0][tym]), double(sh.fx[0][tid]),
             double(sh.fyc[prv][0][tid]), double(sh.fyc[cur][0][tid]),
             sh.ez[e0][tym], sh.ez[e1][tym], sh.ez[e0][tid], sh.ez[e1][tid]);
-        const size_t o = a.off_az + ((size_t)f * (g.Y + 1) + gy) * g.Z + gz;
-        double v = a.in[o] - dt * e;
-        if (a.heun) v = (v + a.out[o]) * 0.5;
-        a.out[o] = v;
+        const size_t o = d.oz + (size_t)f * d.zplane;
+        double v = ldb(inb + o, c0) - dt * e;
+        if (a.heun) v = (v + ldb(outb + o, c0)) * 0.5;
+        stb(outb + o, c0, v);
       }
       // g. update of cell (f-1, ly, lz): x-faces f-1 and f, y/z terms of plane f-1
       if (cell_y && cell_z) {
         if (f - 1 >= ix0 && f - 1 < ix0 + nxv) {
-          const size_t c = cidx(g, f - 1, gy, gz);
+          const size_t c = (size_t)(f - 1) * d.cplane;
 #pragma unroll
           for (int r = 0; r < 5; ++r) {
             const double dfx = double(fcur[r]) - double(fprev[r]);
-            double v = a.in[r * PL + c] - (dtdx * dfx + dyy[r] + dzz[r]);
-            if (a.heun) v = (v + a.out[r * PL + c]) * 0.5;
-            a.out[r * PL + c] = v;
+            const size_t o = c + r * d.field;
+            double v = ldb(inb + o, c0) - (dtdx * dfx + dyy[r] + dzz[r]);
+            if (a.heun) v = (v + ldb(outb + o, c0)) * 0.5;
+            stb(outb + o, c0, v);
           }
         }
         // y/z flux differences of cell (f, ly, lz) for the next iteration
