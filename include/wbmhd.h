@@ -134,6 +134,11 @@ int64_t wb_mhd3d_state_size(int64_t nx, int64_t ny, int64_t nz);
 int wb_mhd3d_fill_halos(double* state, int64_t nx, int64_t ny, int64_t nz, int periodic_mask,
                         void* stream);
 
+/* The outflow axes of MHD3D._fill_halos only (mhd3d.py:448-473): before the fused
+ * stages, which wrap their reads along the periodic axes (wrap = periodic_mask). */
+int wb_mhd3d_fill_outflow(double* state, int64_t nx, int64_t ny, int64_t nz,
+                          int periodic_mask, void* stream);
+
 /* Replaces MHD3D._primitive, mhd3d.py:475-509 (+ _face_fields/_cell_fields
  * :361-386): prim (8, X, Y, Z), face fields fbx (X+1,Y,Z), fby (X,Y+1,Z),
  * fbz (X,Y,Z+1); *bad counts the bad inner cells (the caller raises). */
@@ -161,12 +166,16 @@ int wb_mhd3d_update(const double* in, double* out, const double* fx, const doubl
 
 /* One whole forward-Euler stage of MHD3D._advance (mhd3d.py:635-671: primitive,
  * the three sweeps, the CT-contact edge field, the update; with heun the average
- * of mhd3d.py:742-747) in a single launch, from the haloed state `in` into `out`
- * (halos of `in` filled).  FP64: the same bits as the split kernels above.
- * nxs: cells along x per workgroup segment.  *bad (device) counts bad cells. */
+ * of mhd3d.py:742-747) in a single launch, from the haloed state `in` into `out`.
+ * FP64: the same bits as the split kernels above.  nxs: cells along x per
+ * workgroup segment.  wrap: bit a set for a periodic axis a along which the stage
+ * reads the periodic images of its halo cells itself (that halo is not read and
+ * need not be filled: wb_mhd3d_fill_outflow fills the others); the halos of the
+ * axes not in wrap must be filled.  *bad (device) counts bad cells. */
 int wb_mhd3d_stage(const double* in, double* out, int64_t nx, int64_t ny, int64_t nz, double dx,
                    double dy, double dz, double b0x, double b0y, double b0z, double gamma,
-                   int limiter, const double* dt, int heun, int32_t* bad, int nxs, void* stream);
+                   int limiter, const double* dt, int heun, int32_t* bad, int nxs, int wrap,
+                   void* stream);
 
 /* The same stage with the reconstruction and the HLLD flux in FP32; the primitive
  * conversion (E -> p), the cell electric field, the CT average, the potential
@@ -175,13 +184,20 @@ int wb_mhd3d_stage(const double* in, double* out, int64_t nx, int64_t ny, int64_
 int wb_mhd3d_stage_mixed(const double* in, double* out, int64_t nx, int64_t ny, int64_t nz,
                          double dx, double dy, double dz, double b0x, double b0y, double b0z,
                          double gamma, int limiter, const double* dt, int heun, int32_t* bad,
-                         int nxs, int waves, void* stream);
+                         int nxs, int waves, int wrap, void* stream);
 
 /* Replaces the rate of MHD3D.stable_step, mhd3d.py:674-724: out = device
  * double[2], largest rate and smallest density. */
 int wb_rate3d(const double* state, int64_t nx, int64_t ny, int64_t nz, double dx, double dy,
               double dz, double b0x, double b0y, double b0z, double gamma, double* out,
               void* stream);
+
+/* The same rate for the mixed path (MHD3D.stable_step, mhd3d.py:674-724): FP64 with
+ * one division per cell and the reciprocal spacings, so the rate can differ from
+ * wb_rate3d's in the last bits. */
+int wb_rate3d_mixed(const double* state, int64_t nx, int64_t ny, int64_t nz, double dx,
+                    double dy, double dz, double b0x, double b0y, double b0z, double gamma,
+                    double* out, void* stream);
 
 /* ---- HIP graphs of whole steps -------------------------------------------------- */
 /* The step of the reference (MHD2D.step mhd.py:965-994, MHD3D.step mhd3d.py:726-754)

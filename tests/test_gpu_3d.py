@@ -92,23 +92,42 @@ def test_turbulence_256_conserves_and_keeps_div_b_zero(gpu):
     assert model.divergence() < 1e-13
 
 
-@pytest.mark.parametrize("shape,nxs", [((12, 12, 12), 32), ((31, 17, 29), 4), ((16, 30, 15), 7)])
-@pytest.mark.parametrize("limiter", ["minmod", "vanleer", "mc", "none"])
-def test_fused_fp64_stage_is_bitwise_the_split_path(gpu, shape, nxs, limiter):
-    """The one-launch FP64 stage gives the same bits as the split kernels (and so as
-    the oracle), on uneven tiles and segments."""
+def _wavy_problem(shape, periodic, limiter="minmod"):
     from wellbalanced_amd.hierarchy import AMRHierarchy
 
-    h = AMRHierarchy(shape, ((0.0, 1.0), (0.0, 0.7), (0.0, 1.3)), periodic=(True, True, True))
-    base = initial.turbulence_3d(8, seed=2)
-    rng = np.random.default_rng(9)
-    problem = initial.Problem(h, {"mean_field": (0.6, -0.3, 0.2), "limiter": limiter}, dict(
+    h = AMRHierarchy(shape, ((0.0, 1.0), (0.0, 0.7), (0.0, 1.3)), periodic=periodic)
+    return initial.Problem(h, {"mean_field": (0.6, -0.3, 0.2), "limiter": limiter}, dict(
         density=lambda x, y, z: 1.0 + 0.3 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * z),
         velocity=lambda x, y, z: (0.2 * np.sin(2 * np.pi * y), 0.1 * np.cos(2 * np.pi * z + 1), 0.3 * np.sin(2 * np.pi * x)),
         pressure=lambda x, y, z: 0.8 + 0.1 * np.cos(2 * np.pi * y),
         vector_potential=(lambda x, y, z: 0.02 * np.sin(2 * np.pi * z), lambda x, y, z: 0.03 * np.cos(2 * np.pi * x),
                           lambda x, y, z: 0.01 * np.sin(2 * np.pi * (x + y)))))
-    del base, rng
+
+
+@pytest.mark.parametrize("shape", [(20, 20, 20), (31, 17, 29), (3, 70, 5), (40, 2, 2)])
+def test_stable_step_is_the_oracle(gpu, shape):
+    """The FP64 rate (column march) gives the oracle's dt bit for bit; the mixed path's
+    rate (reciprocals, one division per cell) agrees to 1e-14."""
+    problem = _wavy_problem(shape, (True, True, True))
+    oracle = oracle3d(problem)
+    assert MHD3D.from_problem(problem, kernels="fused").stable_step() == oracle.stable_step()
+    mixed = MHD3D.from_problem(problem, kernels="fused", precision="mixed").stable_step()
+    assert mixed == pytest.approx(oracle.stable_step(), rel=1e-14)
+
+
+@pytest.mark.parametrize("shape,nxs,periodic", [
+    ((12, 12, 12), 32, (True, True, True)), ((31, 17, 29), 4, (True, True, True)),
+    ((16, 30, 15), 7, (True, True, True)), ((12, 12, 12), 32, (False, True, False)),
+    ((31, 17, 29), 4, (True, False, True)), ((16, 30, 15), 7, (False, False, False)),
+    ((3, 2, 40), 5, (True, True, True))])
+@pytest.mark.parametrize("limiter", ["minmod", "vanleer", "mc", "none"])
+def test_fused_fp64_stage_is_bitwise_the_split_path(gpu, shape, nxs, periodic, limiter):
+    """The one-launch FP64 stage gives the same bits as the split kernels (and so as
+    the oracle), on uneven tiles and segments; along periodic axes it reads the
+    periodic images itself (no halo fill), along outflow axes the filled halo."""
+    from wellbalanced_amd.hierarchy import AMRHierarchy
+
+    problem = _wavy_problem(shape, periodic, limiter)
     split = MHD3D.from_problem(problem, kernels="split")
     fused = MHD3D.from_problem(problem, kernels="fused")
     fused.nxs = nxs

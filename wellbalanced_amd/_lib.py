@@ -38,13 +38,15 @@ SIGNATURES: dict[str, tuple[type, list]] = {
     "wb_mhd3d_resistive_workspace": (I64, [I64, I64, I64]),
     "wb_mhd3d_resistive": (I, [P, P, I64, I64, I64] + [D] * 7 + [P, I, P, P]),
     "wb_mhd3d_fill_halos": (I, [P, I64, I64, I64, I, P]),
+    "wb_mhd3d_fill_outflow": (I, [P, I64, I64, I64, I, P]),
     "wb_primitive3d": (I, [P, I64, I64, I64] + [D] * 7 + [P] * 5 + [P]),
     "wb_sweep3d": (I, [P, P, I64, I64, I64, I, I, D, P, P]),
     "wb_edge3d": (I, [P, P, P, I64, I64, I64, I, P, P]),
     "wb_mhd3d_update": (I, [P, P] + [P] * 6 + [I64] * 3 + [P] + [D] * 3 + [I, P]),
-    "wb_mhd3d_stage": (I, [P, P, I64, I64, I64] + [D] * 7 + [I, P, I, P, I, P]),
-    "wb_mhd3d_stage_mixed": (I, [P, P, I64, I64, I64] + [D] * 7 + [I, P, I, P, I, I, P]),
+    "wb_mhd3d_stage": (I, [P, P, I64, I64, I64] + [D] * 7 + [I, P, I, P, I, I, P]),
+    "wb_mhd3d_stage_mixed": (I, [P, P, I64, I64, I64] + [D] * 7 + [I, P, I, P, I, I, I, P]),
     "wb_rate3d": (I, [P, I64, I64, I64] + [D] * 7 + [P, P]),
+    "wb_rate3d_mixed": (I, [P, I64, I64, I64] + [D] * 7 + [P, P]),
     "wb_lohner3d": (I, [P, I64, I64, I64] + [D] * 8 + [P, P, P]),
     "wb_graph_capture_begin": (I, [P]),
     "wb_graph_capture_end": (I, [P, P]),

@@ -7,17 +7,21 @@
 // (y, z) plane and marches along x over a segment of NXS cells.  Its threads map
 // to the (TY+2) x (TZ+2) "face columns" (local y, z in [-1, T]).  Each thread keeps
 // the x-sweep's four planes of its column in registers (a ring shifted once per
-// plane), so the x-faces need no LDS; the current primitive plane (T+4)^2 is
+// plane), so the x-faces need no LDS; the current primitive plane (T+4)x(T+4) is
 // staged in LDS for the y- and z-sweeps; the y/z-face fluxes and the cell
-// electric field of the last two planes sit in LDS rings for the CT-contact
-// average; the x-face fluxes of the column stay in registers for the update.
-// Every face flux a tile needs is computed once per tile; faces on the tile
-// border are computed by both neighbouring tiles.
+// electric field of the last planes sit in LDS rings for the CT-contact average;
+// the x-face fluxes of the column stay in registers for the update.  Every face
+// flux a tile needs is computed once per tile; faces on the tile border are
+// computed by both neighbouring tiles.  Per plane: the y/z faces (from the LDS
+// plane), then the column's next plane and its x-face (registers), one barrier,
+// then the next LDS plane, the CT and the update.  The mixed path's LDS fits four
+// workgroups of 256 threads per CU.
 //
 // P is the arithmetic of the reconstruction and the HLLD flux (double: the
 // parity path; float: the mixed path).  The primitive conversion (E -> p), the
 // cell electric field, the CT average, the face field from the potential and
-// the update are FP64 in both.
+// the update are FP64 in both; the mixed path keeps the cell electric field for
+// the CT in FP32, as it keeps the face fluxes the CT averages.
 #pragma once
 
 #include "mhd_device.hpp"
@@ -48,14 +52,14 @@ template <typename P, int TY, int TZ>
 struct StageShared {
   using T = Tile<TY, TZ>;
   P prim[8][T::NP];      // the primitive plane f (the y/z sweeps read it)
-  double ex[2][T::NF];   // cell E_x of planes f, f+1
-  double ey[3][T::NF];   // cell E_y of planes f-1, f, f+1
-  double ez[3][T::NF];
+  P ex[2][T::NF];        // cell E_x of planes f, f+1
+  P ey[3][T::NF];        // cell E_y of planes f-1, f, f+1
+  P ez[3][T::NF];
   P fx[3][T::NF];        // x-face f: rows 0 (mass), 5 (B_y), 6 (B_z)
-  P fy[7][T::NF];        // y-faces of plane f (face below the column's cell)
-  P fz[7][T::NF];
-  P fyc[2][2][T::NF];    // rows 0 and 6 (B_x) of the y-faces of planes f-1, f
-  P fzc[2][2][T::NF];    // rows 0 and 5 (B_x) of the z-faces of planes f-1, f
+  P fy[5][T::NF];        // rows 1-5 of the y-faces of plane f (face below the column's cell)
+  P fz[5][T::NF];        // rows 1-4 and 6 of the z-faces of plane f
+  P fyc[2][2][T::NF];    // rows 0 (mass) and 6 (B_x) of the y-faces of planes f-1, f
+  P fzc[2][2][T::NF];    // rows 0 (mass) and 5 (B_x) of the z-faces of planes f-1, f
 };
 
 struct StageArgs {
@@ -66,6 +70,7 @@ struct StageArgs {
   Grid3 g;
   double dx, dy, dz, b0x, b0y, b0z, gamma;
   int limiter, heun, nxs;
+  int wrap;  // bit a: axis a is periodic and the stage wraps its reads (its halo is not read)
   size_t off_ax, off_ay, off_az;
 };
 
@@ -249,11 +254,31 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
   const int ly = fa - 1, lz = fb - 1;
   const int gy = jy0 + ly, gz = kz0 + lz;
   const int cgy = min(max(gy, 0), g.Y - 1), cgz = min(max(gz, 0), g.Z - 1);
+  // the cell a read of haloed index `h` along a y or z axis takes: its periodic image
+  // among the interior cells where the stage wraps that axis, else the clamped index
+  // (the halo, filled before the launch)
+  auto read_index = [&](int h, int n, int extent, int bit) {
+    if (a.wrap & bit) {
+      int t = (h - 2) % n;
+      return (t < 0 ? t + n : t) + 2;
+    }
+    return min(max(h, 0), extent - 1);
+  };
+  const int ry = read_index(gy, g.ny, g.Y, 2), rz = read_index(gz, g.nz, g.Z, 4);
   const Addr3 d = addr3(a);
   const char* inb = reinterpret_cast<const char*>(a.in);
   char* outb = reinterpret_cast<char*>(a.out);
-  const uint32_t c0 = (uint32_t)cgy * d.crow + (uint32_t)cgz * 8u;  // cells, A_z
-  const uint32_t c1 = (uint32_t)cgy * d.arow + (uint32_t)cgz * 8u;  // A_x, A_y
+  // byte offsets of the column: c0/c1 where the column writes (cells and A_z / A_x and
+  // A_y), w0/w1 where it reads its cells
+  const uint32_t c0 = (uint32_t)cgy * d.crow + (uint32_t)cgz * 8u;
+  const uint32_t c1 = (uint32_t)cgy * d.arow + (uint32_t)cgz * 8u;
+  const uint32_t w0 = (uint32_t)ry * d.crow + (uint32_t)rz * 8u;
+  const uint32_t w1 = (uint32_t)ry * d.arow + (uint32_t)rz * 8u;
+  // the x-plane a read of plane `pl` (in [ix0 - 2, ix0 + nxv + 1]) takes
+  auto read_plane = [&](int pl) {
+    if (a.wrap & 1) return pl < 2 ? pl + g.nx : (pl >= g.nx + 2 ? pl - g.nx : pl);
+    return min(max(pl, 0), g.X - 1);
+  };
   const double dt = *a.dt;
   const double dtdx = dt / a.dx, dtdy = dt / a.dy, dtdz = dt / a.dz;
   const double rdx = 1.0 / a.dx, rdy = 1.0 / a.dy, rdz = 1.0 / a.dz;
@@ -261,42 +286,42 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
 
   // register state of the column
   P q[7][4];                       // x-frame primitives of planes f-2 .. f+1
-  P own[8];                        // the column's primitives of plane f+1, until LDS is free
+  P own_bx = P(0);                 // cell B_x of plane f+1 (the ring holds the rest), for LDS
   P bnp[3], bnn[3];                // face fields of the column's low x/y/z faces: planes f, f+1
   P fprev[5], fcur[5];             // x-face fluxes (rows 0-4) of faces f-1 and f
-  double dyy[5], dzz[5];
+  // dt times the y/z flux differences of the cell of plane f-1, for its update; the
+  // mixed path keeps their sum (the parity path adds them in the reference's order)
+  double dyy[5], dzz[FAST ? 1 : 5];
 #pragma unroll
   for (int c = 0; c < 7; ++c)
 #pragma unroll
     for (int s = 0; s < 4; ++s) q[c][s] = P(0);
 #pragma unroll
-  for (int r = 0; r < 5; ++r) { fprev[r] = fcur[r] = P(0); dyy[r] = dzz[r] = 0.0; }
+  for (int r = 0; r < 5; ++r) { fprev[r] = fcur[r] = P(0); dyy[r] = 0.0; }
 #pragma unroll
-  for (int r = 0; r < 8; ++r) own[r] = P(0);
+  for (int r = 0; r < (FAST ? 1 : 5); ++r) dzz[r] = 0.0;
 #pragma unroll
   for (int r = 0; r < 3; ++r) bnp[r] = bnn[r] = P(0);
 
   // the own column of plane `pl`: registers (ring + own), cell E, bad cells
   auto own_column = [&](int pl) {
     if (!fthread) return;
-    const int pc = min(max(pl, 0), g.X - 1);
     CellPrim<FAST> c;
-    c.load(a, d, inb, pc, c0, c1, rdx, rdy, rdz);
+    c.load(a, d, inb, read_plane(pl), w0, w1, rdx, rdy, rdz);
     bnp[0] = bnn[0]; bnp[1] = bnn[1]; bnp[2] = bnn[2];
     bnn[0] = P(c.fx); bnn[1] = P(c.fy); bnn[2] = P(c.fz);
-    own[0] = P(c.r); own[1] = P(c.u); own[2] = P(c.v); own[3] = P(c.w);
-    own[4] = P(c.p); own[5] = P(c.bx); own[6] = P(c.by); own[7] = P(c.bz);
-    constexpr int comp[7] = {0, 1, 2, 3, 4, 6, 7};
+    const P x[7] = {P(c.r), P(c.u), P(c.v), P(c.w), P(c.p), P(c.by), P(c.bz)};
+    own_bx = P(c.bx);
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
       q[k][0] = q[k][1];
       q[k][1] = q[k][2];
       q[k][2] = q[k][3];
-      q[k][3] = own[comp[k]];
+      q[k][3] = x[k];
     }
-    sh.ex[pl & 1][tid] = c.by * c.w - c.bz * c.v;
-    sh.ey[pl % 3][tid] = c.bz * c.u - c.bx * c.w;
-    sh.ez[pl % 3][tid] = c.bx * c.v - c.by * c.u;
+    sh.ex[pl & 1][tid] = P(c.by * c.w - c.bz * c.v);
+    sh.ey[pl % 3][tid] = P(c.bz * c.u - c.bx * c.w);
+    sh.ez[pl % 3][tid] = P(c.bx * c.v - c.by * c.u);
     if (ly >= 0 && ly < tyv && lz >= 0 && lz < tzv && pl >= ix0 && pl < ix0 + nxv && c.bad())
       atomicAdd(a.bad, 1);
   };
@@ -308,7 +333,8 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
   if (tid < T::PZ) { rpy = 0; rpz = tid; }
   else if (tid < 2 * T::PZ) { rpy = T::PY - 1; rpz = tid - T::PZ; }
   else { const int e = tid - 2 * T::PZ; rpy = 1 + e / 2; rpz = (e & 1) ? T::PZ - 1 : 0; }
-  const int rj = min(max(jy0 + rpy - 2, 0), g.Y - 1), rk = min(max(kz0 + rpz - 2, 0), g.Z - 1);
+  const int rj = read_index(jy0 + rpy - 2, g.ny, g.Y, 2);
+  const int rk = read_index(kz0 + rpz - 2, g.nz, g.Z, 4);
   const uint32_t r0 = (uint32_t)rj * d.crow + (uint32_t)rk * 8u;
   const uint32_t r1 = (uint32_t)rj * d.arow + (uint32_t)rk * 8u;
   // plane `pl` into LDS: the own column from registers, the outer ring computed here
@@ -316,12 +342,14 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
     if (fthread) {
       const int idx = (fa + 1) * T::PZ + (fb + 1);
 #pragma unroll
-      for (int r = 0; r < 8; ++r) sh.prim[r][idx] = own[r];
+      for (int r = 0; r < 5; ++r) sh.prim[r][idx] = q[r][3];
+      sh.prim[5][idx] = own_bx;
+      sh.prim[6][idx] = q[5][3];
+      sh.prim[7][idx] = q[6][3];
     }
-    const int pc = min(max(pl, 0), g.X - 1);
     if (ring) {
       CellPrim<FAST> c;
-      c.load(a, d, inb, pc, r0, r1, rdx, rdy, rdz);
+      c.load(a, d, inb, read_plane(pl), r0, r1, rdx, rdy, rdz);
       const int idx = rpy * T::PZ + rpz;
       sh.prim[0][idx] = P(c.r); sh.prim[1][idx] = P(c.u); sh.prim[2][idx] = P(c.v);
       sh.prim[3][idx] = P(c.w); sh.prim[4][idx] = P(c.p); sh.prim[5][idx] = P(c.bx);
@@ -337,22 +365,9 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
 
   for (int f = ix0 - 1; f <= ix0 + nxv; ++f) {
     const int cur = f & 1, prv = cur ^ 1;
-    // a-b. the own column of plane f+1 into the ring; c. x-face f from the ring
-    own_column(f + 1);
     if (fthread) {
-      if (f >= ix0) {
-        P fl[7];
-        face_solve<P, LIM>(q, a.limiter, bnp[0], gam, inv_gm1, fl);
-#pragma unroll
-        for (int r = 0; r < 5; ++r) {
-          fprev[r] = fcur[r];
-          fcur[r] = fl[r];
-        }
-        sh.fx[0][tid] = fl[0];
-        sh.fx[1][tid] = fl[5];
-        sh.fx[2][tid] = fl[6];
-      }
-      // d. y- and z-faces of plane f below the column's cell, from the LDS plane
+      // a. y- and z-faces of plane f below the column's cell, from the LDS plane
+      //    (their B_n: the column's face fields of plane f)
       const int pidx = (fa + 1) * T::PZ + (fb + 1);
       if (fa >= 1) {
         constexpr int comp[7] = {0, 2, 3, 1, 4, 7, 5};
@@ -362,9 +377,9 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
 #pragma unroll
           for (int s = 0; s < 4; ++s) qq[c][s] = sh.prim[comp[c]][pidx + (s - 2) * T::PZ];
         P fl[7];
-        face_solve<P, LIM>(qq, a.limiter, bnp[1], gam, inv_gm1, fl);
+        face_solve<P, LIM>(qq, a.limiter, bnn[1], gam, inv_gm1, fl);
 #pragma unroll
-        for (int r = 0; r < 7; ++r) sh.fy[r][tid] = fl[r];
+        for (int r = 1; r < 6; ++r) sh.fy[r - 1][tid] = fl[r];
         sh.fyc[cur][0][tid] = fl[0];
         sh.fyc[cur][1][tid] = fl[6];
       }
@@ -376,12 +391,27 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
 #pragma unroll
           for (int s = 0; s < 4; ++s) qq[c][s] = sh.prim[comp[c]][pidx + (s - 2)];
         P fl[7];
-        face_solve<P, LIM>(qq, a.limiter, bnp[2], gam, inv_gm1, fl);
+        face_solve<P, LIM>(qq, a.limiter, bnn[2], gam, inv_gm1, fl);
 #pragma unroll
-        for (int r = 0; r < 7; ++r) sh.fz[r][tid] = fl[r];
+        for (int r = 1; r < 5; ++r) sh.fz[r - 1][tid] = fl[r];
+        sh.fz[4][tid] = fl[6];
         sh.fzc[cur][0][tid] = fl[0];
         sh.fzc[cur][1][tid] = fl[5];
       }
+    }
+    // b. the own column of plane f+1 into the ring; c. x-face f from the ring
+    own_column(f + 1);
+    if (fthread && f >= ix0) {
+      P fl[7];
+      face_solve<P, LIM>(q, a.limiter, bnp[0], gam, inv_gm1, fl);
+#pragma unroll
+      for (int r = 0; r < 5; ++r) {
+        fprev[r] = fcur[r];
+        fcur[r] = fl[r];
+      }
+      sh.fx[0][tid] = fl[0];
+      sh.fx[1][tid] = fl[5];
+      sh.fx[2][tid] = fl[6];
     }
     __syncthreads();
     // e. the LDS plane is free: plane f+1 goes in for the next iteration
@@ -395,11 +425,12 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
       // f1. E_x on the x-cell plane f at (y-node ly, z-node lz): plane (a=y, b=z)
       if (f >= ix0 && f < ix0 + nxv && in_y && in_z) {
         const double e = ct_node(
-            double(sh.fy[5][tzm]), double(sh.fy[5][tid]),
-            double(sh.fz[6][tym]), double(sh.fz[6][tid]),
-            double(sh.fy[0][tzm]), double(sh.fy[0][tid]),
-            double(sh.fz[0][tym]), double(sh.fz[0][tid]),
-            sh.ex[f & 1][tymz], sh.ex[f & 1][tzm], sh.ex[f & 1][tym], sh.ex[f & 1][tid]);
+            double(sh.fy[4][tzm]), double(sh.fy[4][tid]),
+            double(sh.fz[4][tym]), double(sh.fz[4][tid]),
+            double(sh.fyc[cur][0][tzm]), double(sh.fyc[cur][0][tid]),
+            double(sh.fzc[cur][0][tym]), double(sh.fzc[cur][0][tid]),
+            double(sh.ex[f & 1][tymz]), double(sh.ex[f & 1][tzm]), double(sh.ex[f & 1][tym]),
+            double(sh.ex[f & 1][tid]));
         const size_t o = d.ox + (size_t)f * d.xplane;  // E_x's A_x at (f, gy, gz): column c1
         double v = ldb(inb + o, c1) - dt * e;
         if (a.heun) v = (v + ldb(outb + o, c1)) * 0.5;
@@ -413,7 +444,8 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
             double(sh.fx[2][tzm]), double(sh.fx[2][tid]),
             double(sh.fzc[prv][0][tid]), double(sh.fzc[cur][0][tid]),
             double(sh.fx[0][tzm]), double(sh.fx[0][tid]),
-            sh.ey[e0][tzm], sh.ey[e0][tid], sh.ey[e1][tzm], sh.ey[e1][tid]);
+            double(sh.ey[e0][tzm]), double(sh.ey[e0][tid]), double(sh.ey[e1][tzm]),
+            double(sh.ey[e1][tid]));
         const size_t o = d.oy + (size_t)f * d.yplane;
         double v = ldb(inb + o, c1) - dt * e;
         if (a.heun) v = (v + ldb(outb + o, c1)) * 0.5;
@@ -426,7 +458,8 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
             double(sh.fyc[prv][1][tid]), double(sh.fyc[cur][1][tid]),
             double(sh.fx[0][tym]), double(sh.fx[0][tid]),
             double(sh.fyc[prv][0][tid]), double(sh.fyc[cur][0][tid]),
-            sh.ez[e0][tym], sh.ez[e1][tym], sh.ez[e0][tid], sh.ez[e1][tid]);
+            double(sh.ez[e0][tym]), double(sh.ez[e1][tym]), double(sh.ez[e0][tid]),
+            double(sh.ez[e1][tid]));
         const size_t o = d.oz + (size_t)f * d.zplane;
         double v = ldb(inb + o, c0) - dt * e;
         if (a.heun) v = (v + ldb(outb + o, c0)) * 0.5;
@@ -440,7 +473,9 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
           for (int r = 0; r < 5; ++r) {
             const double dfx = double(fcur[r]) - double(fprev[r]);
             const size_t o = c + r * d.field;
-            double v = ldb(inb + o, c0) - (dtdx * dfx + dyy[r] + dzz[r]);
+            double v;
+            if constexpr (FAST) v = ldb(inb + o, c0) - (dtdx * dfx + dyy[r]);
+            else v = ldb(inb + o, c0) - (dtdx * dfx + dyy[r] + dzz[r]);
             if (a.heun) v = (v + ldb(outb + o, c0)) * 0.5;
             stb(outb + o, c0, v);
           }
@@ -448,13 +483,22 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
         // y/z flux differences of cell (f, ly, lz) for the next iteration
         if (f >= ix0 && f < ix0 + nxv) {
           const int typ = tid + T::FZ, tzp = tid + 1;
-          // rows of the y-sweep (rho, m_y, m_z, m_x, E) and z-sweep (rho, m_z, m_x, m_y, E)
+          // rows of the y-sweep (rho, m_y, m_z, m_x, E) and z-sweep (rho, m_z, m_x, m_y, E);
+          // row 0 (mass) sits in fyc/fzc, rows r >= 1 at r - 1
           constexpr int yrow[5] = {0, 3, 1, 2, 4};
           constexpr int zrow[5] = {0, 2, 3, 1, 4};
 #pragma unroll
           for (int r = 0; r < 5; ++r) {
-            dyy[r] = dtdy * (double(sh.fy[yrow[r]][typ]) - double(sh.fy[yrow[r]][tid]));
-            dzz[r] = dtdz * (double(sh.fz[zrow[r]][tzp]) - double(sh.fz[zrow[r]][tid]));
+            const P* fyr = r == 0 ? sh.fyc[cur][0] : sh.fy[yrow[r] - 1];
+            const P* fzr = r == 0 ? sh.fzc[cur][0] : sh.fz[zrow[r] - 1];
+            const double ty = dtdy * (double(fyr[typ]) - double(fyr[tid]));
+            const double tz = dtdz * (double(fzr[tzp]) - double(fzr[tid]));
+            if constexpr (FAST) {
+              dyy[r] = ty + tz;
+            } else {
+              dyy[r] = ty;
+              dzz[r] = tz;
+            }
           }
         }
       }

@@ -7,6 +7,7 @@
 // cells.  The state is one buffer: 5 cell planes (rho, mx, my, mz, E), then the
 // edge potential A_x (X, Y+1, Z+1), A_y (X+1, Y, Z+1), A_z (X+1, Y+1, Z).
 #include "mhd_device.hpp"
+#include "rate3d.hpp"
 #include "wbmhd_common.hpp"
 
 using namespace wb;
@@ -247,35 +248,6 @@ __global__ void k_update3d(const double* __restrict__ in, double* __restrict__ o
 
 // mhd3d.py:674-724: the largest sum over axes of (|u_a| + c_f,a) / dx_a and the
 // smallest density of the block's own cells.
-__global__ void __launch_bounds__(256) k_rate3d(const double* __restrict__ state, Dims3 g,
-                                                double dx, double dy, double dz, double b0x,
-                                                double b0y, double b0z, double gamma,
-                                                unsigned long long* __restrict__ keys) {
-  const int nx = g.X - 4, ny = g.Y - 4, nz = g.Z - 4, h = HALO;
-  const unsigned total = (unsigned)nx * ny * nz;
-  const Pot A = pot_of(state, g);
-  const size_t P = g.cells();
-  double rate = 0.0, rmin = __builtin_inf();
-  for (unsigned id = blockIdx.x * blockDim.x + threadIdx.x; id < total; id += gridDim.x * blockDim.x) {
-    const int c = id % nz, t = id / nz, b = t % ny, a = t / ny;
-    const int i = a + h, j = b + h, k = c + h;
-    const size_t cell = ((size_t)i * g.Y + j) * g.Z + k;
-    const double bx = 0.5 * (A.bx(i, j, k, dy, dz, b0x) + A.bx(i + 1, j, k, dy, dz, b0x));
-    const double by = 0.5 * (A.by(i, j, k, dz, dx, b0y) + A.by(i, j + 1, k, dz, dx, b0y));
-    const double bz = 0.5 * (A.bz(i, j, k, dx, dy, b0z) + A.bz(i, j, k + 1, dx, dy, b0z));
-    const double r = state[cell];
-    const double u = state[P + cell] / r, v = state[2 * P + cell] / r, w = state[3 * P + cell] / r;
-    const double p = (gamma - 1.0) * (state[4 * P + cell] - 0.5 * r * (u * u + v * v + w * w) -
-                                      0.5 * (bx * bx + by * by + bz * bz));
-    const double rc = (__builtin_fabs(u) + fast_speed_rate(r, p, bx, by, bz, gamma)) / dx +
-                      (__builtin_fabs(v) + fast_speed_rate(r, p, by, bz, bx, gamma)) / dy +
-                      (__builtin_fabs(w) + fast_speed_rate(r, p, bz, bx, by, gamma)) / dz;
-    if (rc > rate) rate = rc;
-    if (r < rmin) rmin = r;
-  }
-  block_reduce_rate_rho_once(rate, rmin, keys);
-}
-
 // One halo pass along `axis` over the 8 arrays of the state.  kind 0 periodic,
 // 1 outflow (cells zero-gradient, potentials linearly extrapolated,
 // mhd3d.py:448-473 with mhd.py:298-325).
@@ -338,22 +310,40 @@ int64_t wb_mhd3d_state_size(int64_t nx, int64_t ny, int64_t nz) {
   return (int64_t)array_offset(g, 8);
 }
 
-int wb_mhd3d_fill_halos(double* state, int64_t nx, int64_t ny, int64_t nz, int periodic_mask,
-                        void* stream) {
+}  // extern "C"
+
+namespace {
+
+// The halo passes of the axes in `axes`: periodic axes first, then outflow axes.
+int fill_halos3d(double* state, int64_t nx, int64_t ny, int64_t nz, int periodic_mask, int axes,
+                 hipStream_t s) {
   if (!dims_ok(nx, ny, nz)) return WB_EINVAL;
-  hipStream_t s = (hipStream_t)stream;
   Dims3 g{(int)nx + 4, (int)ny + 4, (int)nz + 4};
   const int64_t n[3] = {g.X + 1, g.Y + 1, g.Z + 1};
   for (int pass = 0; pass < 2; ++pass)
     for (int axis = 0; axis < 3; ++axis) {
       const bool periodic = (periodic_mask >> axis) & 1;
-      if ((pass == 0) != periodic) continue;
+      if ((pass == 0) != periodic || !((axes >> axis) & 1)) continue;
       const int o1 = axis == 0 ? 1 : 0, o2 = axis == 2 ? 1 : 2;
       const int64_t other = n[o1] * n[o2];
       dim3 grid((unsigned)((other + 255) / 256), 5, 8);
       hipLaunchKernelGGL(k_halo3d, grid, dim3(256), 0, s, state, g, axis, periodic ? 0 : 1);
     }
   return wb_last_error();
+}
+
+}  // namespace
+
+extern "C" {
+
+int wb_mhd3d_fill_halos(double* state, int64_t nx, int64_t ny, int64_t nz, int periodic_mask,
+                        void* stream) {
+  return fill_halos3d(state, nx, ny, nz, periodic_mask, 7, (hipStream_t)stream);
+}
+
+int wb_mhd3d_fill_outflow(double* state, int64_t nx, int64_t ny, int64_t nz, int periodic_mask,
+                          void* stream) {
+  return fill_halos3d(state, nx, ny, nz, periodic_mask, ~periodic_mask & 7, (hipStream_t)stream);
 }
 
 int wb_primitive3d(const double* state, int64_t nx, int64_t ny, int64_t nz, double dx, double dy,
@@ -420,18 +410,8 @@ int wb_rate3d(const double* state, int64_t nx, int64_t ny, int64_t nz, double dx
               double dz, double b0x, double b0y, double b0z, double gamma, double* out,
               void* stream) {
   if (!dims_ok(nx, ny, nz)) return WB_EINVAL;
-  hipStream_t s = (hipStream_t)stream;
-  Dims3 g{(int)nx + 4, (int)ny + 4, (int)nz + 4};
-  unsigned long long* keys = (unsigned long long*)out;
-  hipLaunchKernelGGL(k_rate_init, dim3(1), dim3(64), 0, s, keys);
-  const int64_t total = nx * ny * nz;
-  if (total >= ((int64_t)1 << 31)) return WB_EINVAL;
-  const int64_t want = (total + 255) / 256;
-  dim3 grid((unsigned)(want < 2048 ? want : 2048));
-  hipLaunchKernelGGL(k_rate3d, grid, dim3(256), 0, s, state, g, dx, dy, dz, b0x, b0y, b0z, gamma,
-                     keys);
-  hipLaunchKernelGGL(k_rate_finish, dim3(1), dim3(64), 0, s, keys, out);
-  return wb_last_error();
+  return launch_rate3d<false>(state, nx, ny, nz, dx, dy, dz, b0x, b0y, b0z, gamma, out,
+                              (hipStream_t)stream);
 }
 
 }  // extern "C"

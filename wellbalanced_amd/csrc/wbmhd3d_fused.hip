@@ -20,14 +20,15 @@ size_t off3(int64_t nx, int64_t ny, int64_t nz, int which) {
 extern "C" __attribute__((visibility("hidden"))) int wb_fill_stage_args3d(void* args, const double* in, double* out, int64_t nx,
                                     int64_t ny, int64_t nz, double dx, double dy, double dz,
                                     double b0x, double b0y, double b0z, double gamma, int limiter,
-                                    const double* dt, int heun, int32_t* bad, int nxs);
+                                    const double* dt, int heun, int32_t* bad, int nxs,
+                                    int wrap);
 
 int wb_fill_stage_args3d(void* args, const double* in, double* out, int64_t nx, int64_t ny,
                          int64_t nz, double dx, double dy, double dz, double b0x, double b0y,
                          double b0z, double gamma, int limiter, const double* dt, int heun,
-                         int32_t* bad, int nxs) {
+                         int32_t* bad, int nxs, int wrap) {
   if (nx < 2 || ny < 2 || nz < 2 || nx > 4096 || ny > 4096 || nz > 4096 || limiter < 0 ||
-      limiter > 3 || nxs < 1)
+      limiter > 3 || nxs < 1 || wrap < 0 || wrap > 7)
     return WB_EINVAL;
   StageArgs& a = *(StageArgs*)args;
   a.in = in;
@@ -45,6 +46,7 @@ int wb_fill_stage_args3d(void* args, const double* in, double* out, int64_t nx, 
   a.limiter = limiter;
   a.heun = heun;
   a.nxs = nxs;
+  a.wrap = wrap;
   a.off_ax = off3(nx, ny, nz, 0);
   a.off_ay = off3(nx, ny, nz, 1);
   a.off_az = off3(nx, ny, nz, 2);
@@ -54,10 +56,10 @@ int wb_fill_stage_args3d(void* args, const double* in, double* out, int64_t nx, 
 extern "C" int wb_mhd3d_stage(const double* in, double* out, int64_t nx, int64_t ny, int64_t nz,
                               double dx, double dy, double dz, double b0x, double b0y, double b0z,
                               double gamma, int limiter, const double* dt, int heun, int32_t* bad,
-                              int nxs, void* stream) {
+                              int nxs, int wrap, void* stream) {
   StageArgs a;
   const int rc = wb_fill_stage_args3d(&a, in, out, nx, ny, nz, dx, dy, dz, b0x, b0y, b0z, gamma,
-                                      limiter, dt, heun, bad, nxs);
+                                      limiter, dt, heun, bad, nxs, wrap);
   if (rc) return rc;
-  return launch_stage3d<double, 14, 14, 1>(a, (hipStream_t)stream);
+  return launch_stage3d<double, 12, 16, 1>(a, (hipStream_t)stream);
 }

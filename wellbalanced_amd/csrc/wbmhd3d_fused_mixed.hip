@@ -1,6 +1,7 @@
 // The fused 3D stage with the reconstruction and the HLLD flux in FP32 and the
 // primitive conversion, the cell electric field, the CT average, the potential
 // and the update in FP64.
+#include "rate3d.hpp"
 #include "stage3d.hpp"
 
 using namespace wb;
@@ -8,21 +9,30 @@ using namespace wb;
 extern "C" __attribute__((visibility("hidden"))) int wb_fill_stage_args3d(void* args, const double* in, double* out, int64_t nx,
                                     int64_t ny, int64_t nz, double dx, double dy, double dz,
                                     double b0x, double b0y, double b0z, double gamma, int limiter,
-                                    const double* dt, int heun, int32_t* bad, int nxs);
+                                    const double* dt, int heun, int32_t* bad, int nxs,
+                                    int wrap);
 
 extern "C" int wb_mhd3d_stage_mixed(const double* in, double* out, int64_t nx, int64_t ny,
                                     int64_t nz, double dx, double dy, double dz, double b0x,
                                     double b0y, double b0z, double gamma, int limiter,
                                     const double* dt, int heun, int32_t* bad, int nxs,
-                                    int waves, void* stream) {
+                                    int waves, int wrap, void* stream) {
   StageArgs a;
   const int rc = wb_fill_stage_args3d(&a, in, out, nx, ny, nz, dx, dy, dz, b0x, b0y, b0z, gamma,
-                                      limiter, dt, heun, bad, nxs);
+                                      limiter, dt, heun, bad, nxs, wrap);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
   return with_limiter(limiter, [&](auto lim) {
     constexpr int L = decltype(lim)::value;
-    if (waves == 3) return launch_stage3d<float, 14, 14, 3, L>(a, s);
-    return launch_stage3d<float, 14, 14, 2, L>(a, s);
+    if (waves == 3) return launch_stage3d<float, 12, 16, 3, L>(a, s);
+    return launch_stage3d<float, 12, 16, 2, L>(a, s);
   });
+}
+
+// The CFL rate of the mixed path: FP64, one division per cell, reciprocal spacings.
+extern "C" int wb_rate3d_mixed(const double* state, int64_t nx, int64_t ny, int64_t nz, double dx,
+                               double dy, double dz, double b0x, double b0y, double b0z,
+                               double gamma, double* out, void* stream) {
+  return launch_rate3d<true>(state, nx, ny, nz, dx, dy, dz, b0x, b0y, b0z, gamma, out,
+                             (hipStream_t)stream);
 }

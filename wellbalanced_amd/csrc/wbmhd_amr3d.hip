@@ -412,6 +412,7 @@ __attribute__((visibility("hidden"))) void wb_amr3d_base_args(
   a.limiter = limiter;
   a.heun = heun;
   a.nxs = B;
+  a.wrap = 0;
   a.off_ax = OFF_A;
   a.off_ay = OFF_A + NEDGE;
   a.off_az = OFF_A + 2 * NEDGE;

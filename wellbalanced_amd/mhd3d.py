@@ -280,23 +280,29 @@ class MHD3D:
     def _stage_fused(self, src, dst, heun, slot, stream, marks=None):
         nx, ny, nz = self.n
         pmask = sum(int(p) << a for a, p in enumerate(self.periodic))
-        _lib.call("wb_mhd3d_fill_halos", _lib.ptr(src), nx, ny, nz, pmask, stream)
+        # the stage reads the periodic images itself: only the outflow axes are filled
+        # (all axes with resistivity, whose terms read the halo of the stage input)
+        if self.resistivity:
+            _lib.call("wb_mhd3d_fill_halos", _lib.ptr(src), nx, ny, nz, pmask, stream)
+        elif pmask != 7:
+            _lib.call("wb_mhd3d_fill_outflow", _lib.ptr(src), nx, ny, nz, pmask, stream)
         if marks is not None:
             marks[0].record()
         args = (_lib.ptr(src), _lib.ptr(dst), nx, ny, nz, *self.spacing, *self.mean_field,
                 self.gamma, LIMITER_CODES.index(self.limiter), _lib.ptr(self._dt), int(heun),
                 self._bad.data_ptr() + 4 * slot, int(self.nxs))
         if self.precision == "fp64":
-            _lib.call("wb_mhd3d_stage", *args, stream)
+            _lib.call("wb_mhd3d_stage", *args, pmask, stream)
         else:
-            _lib.call("wb_mhd3d_stage_mixed", *args, int(self.waves), stream)
+            _lib.call("wb_mhd3d_stage_mixed", *args, int(self.waves), pmask, stream)
         self._resistive(src, dst, heun, stream)
         if marks is not None:
             marks[1].record()
 
     def _rate_kernel(self, stream: int) -> None:
         nx, ny, nz = self.n
-        _lib.call("wb_rate3d", _lib.ptr(self._buf[0]), nx, ny, nz, *self.spacing,
+        name = "wb_rate3d" if self.precision == "fp64" else "wb_rate3d_mixed"
+        _lib.call(name, _lib.ptr(self._buf[0]), nx, ny, nz, *self.spacing,
                   *self.mean_field, self.gamma, _lib.ptr(self._rate), stream)
 
     def _dt_kernel(self, time_ptr, stream: int) -> None:
