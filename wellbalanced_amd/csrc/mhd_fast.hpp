@@ -102,35 +102,30 @@ __device__ __forceinline__ void side(const float* w, float bn, float gamma, floa
 // |den| < 1e-5 (p_T + B_n^2) as degenerate keeps the tangential state instead.
 constexpr float DEGENERATE32 = 1.0e-5f;
 
-// Star state beside the contact (mhd_sweep_pyccel.py:101-145): us, and
-// vt = (vt1, vt2, bt1, bt2).
-__device__ __forceinline__ void star(const float* w, const Side& sd, float bn, float s, float d,
-                                     float sm, float pts, float* us, float* vt) {
+// The star state beside the contact (mhd_sweep_pyccel.py:101-145) as both sides
+// need it for the double-star state: rho*, 1 / (S - S_M), and vt = (vt1, vt2, bt1,
+// bt2).  The momenta and energy of the star state are formed for the side of the
+// contact the face is on only (in `hlld`).
+struct Star {
+  float rho, inv, vt[4];
+};
+
+__device__ __forceinline__ Star star(const float* w, const Side& sd, float bn, float s, float d,
+                                     float sm) {
   const float vn = w[1];
-  const float inv = rcp(s - sm);
-  const float rho_s = d * inv;
+  Star t;
+  t.inv = rcp(s - sm);
+  t.rho = d * t.inv;
   const float den = d * (s - sm) - bn * bn;
   const bool regular = __builtin_fabsf(den) >= DEGENERATE32 * (sd.pt + bn * bn);
   const float iden = rcp(den);
   const float cv = bn * (sm - vn) * iden;
   const float cb = (d * (s - vn) - bn * bn) * iden;
-  const float vt1 = sel(regular, w[2] - w[5] * cv, w[2]);
-  const float vt2 = sel(regular, w[3] - w[6] * cv, w[3]);
-  const float bt1 = sel(regular, w[5] * cb, w[5]);
-  const float bt2 = sel(regular, w[6] * cb, w[6]);
-  const float vb = vn * bn + w[2] * w[5] + w[3] * w[6];
-  const float vb_s = sm * bn + vt1 * bt1 + vt2 * bt2;
-  us[0] = rho_s;
-  us[1] = rho_s * sm;
-  us[2] = rho_s * vt1;
-  us[3] = rho_s * vt2;
-  us[4] = ((s - vn) * sd.u[4] - sd.pt * vn + pts * sm + bn * (vb - vb_s)) * inv;
-  us[5] = bt1;
-  us[6] = bt2;
-  vt[0] = vt1;
-  vt[1] = vt2;
-  vt[2] = bt1;
-  vt[3] = bt2;
+  t.vt[0] = sel(regular, w[2] - w[5] * cv, w[2]);
+  t.vt[1] = sel(regular, w[3] - w[6] * cv, w[3]);
+  t.vt[2] = sel(regular, w[5] * cb, w[5]);
+  t.vt[3] = sel(regular, w[6] * cb, w[6]);
+  return t;
 }
 
 // The HLLD flux (Miyoshi and Kusano 2005) of normal-frame states L, R.
@@ -148,11 +143,11 @@ __device__ __forceinline__ void hlld(const float* L, const float* R, float bn, f
   const float idd = rcp(dr - dl);
   const float sm = (dr * vr - dl * vl - sr_.pt + sl_.pt) * idd;
   const float pts = (dr * sl_.pt - dl * sr_.pt + dl * dr * (vr - vl)) * idd;
-  float usl[7], usr[7], wl[4], wr[4];
-  star(L, sl_, bn, sl, dl, sm, pts, usl, wl);
-  star(R, sr_, bn, sr, dr, sm, pts, usr, wr);
-  const float isql = rsq(usl[0]), isqr = rsq(usr[0]);
-  const float sql = usl[0] * isql, sqr = usr[0] * isqr;
+  const Star tl = star(L, sl_, bn, sl, dl, sm), tr = star(R, sr_, bn, sr, dr, sm);
+  const float* wl = tl.vt;
+  const float* wr = tr.vt;
+  const float isql = rsq(tl.rho), isqr = rsq(tr.rho);
+  const float sql = tl.rho * isql, sqr = tr.rho * isqr;
   const float sign = sel(bn > 0.0f, 1.0f, sel(bn < 0.0f, -1.0f, 0.0f));
   const float isum = rcp(sql + sqr);
   const float vt2_1 = (sql * wl[0] + sqr * wr[0] + (wr[2] - wl[2]) * sign) * isum;
@@ -170,14 +165,21 @@ __device__ __forceinline__ void hlld(const float* L, const float* R, float bn, f
   const float w0 = sel(left, wl[0], wr[0]), w1 = sel(left, wl[1], wr[1]);
   const float w2 = sel(left, wl[2], wr[2]), w3 = sel(left, wl[3], wr[3]);
   const float vb_s = sm * bn + w0 * w2 + w1 * w3;
-  float us[7], uu[7], ff[7];
+  float uu[7], ff[7];
 #pragma unroll
   for (int c = 0; c < 7; ++c) {
-    us[c] = sel(left, usl[c], usr[c]);
     uu[c] = sel(left, sl_.u[c], sr_.u[c]);
     ff[c] = sel(left, sl_.f[c], sr_.f[c]);
   }
-  const float rho_s = us[0];
+  // the star state of that side (mhd_sweep_pyccel.py:131-145)
+  const float vn = sel(left, L[1], R[1]);
+  const float vb = vn * bn + sel(left, L[2], R[2]) * sel(left, L[5], R[5]) +
+                   sel(left, L[3], R[3]) * sel(left, L[6], R[6]);
+  const float rho_s = sel(left, tl.rho, tr.rho);
+  const float us[7] = {rho_s, rho_s * sm, rho_s * w0, rho_s * w1,
+                       ((s - vn) * uu[4] - sel(left, sl_.pt, sr_.pt) * vn + pts * sm +
+                        bn * (vb - vb_s)) * sel(left, tl.inv, tr.inv),
+                       w2, w3};
   const float udd[7] = {rho_s, rho_s * sm, rho_s * vt2_1, rho_s * vt2_2,
                         us[4] + sq * (vb_s - vb2) * sign, bt2_1, bt2_2};
   const bool supl = sl >= 0.0f, supr = sr < 0.0f;

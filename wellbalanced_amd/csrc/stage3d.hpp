@@ -284,8 +284,15 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
   const double rdx = 1.0 / a.dx, rdy = 1.0 / a.dy, rdz = 1.0 / a.dz;
   const P gam = P(a.gamma), inv_gm1 = P(1.0 / (a.gamma - 1.0));
 
+  // The mixed path's x-sweep computes each cell's slope once: the right state of face f
+  // and the left state of face f+1 (carried in lcar) come from the slope of cell f, so
+  // its ring needs planes f-1 .. f+1 only; the parity path keeps the reference's
+  // per-face reconstruction from planes f-2 .. f+1.
+  constexpr bool CARRY = sizeof(P) == 4 && LIM >= 0;
+  constexpr int NQ = CARRY ? 3 : 4;
   // register state of the column
-  P q[7][4];                       // x-frame primitives of planes f-2 .. f+1
+  P q[7][NQ];                      // x-frame primitives of the last NQ planes (to f+1)
+  P lcar[7];                       // CARRY: the left state of x-face f+1
   P own_bx = P(0);                 // cell B_x of plane f+1 (the ring holds the rest), for LDS
   P bnp[3], bnn[3];                // face fields of the column's low x/y/z faces: planes f, f+1
   P fprev[5], fcur[5];             // x-face fluxes (rows 0-4) of faces f-1 and f
@@ -293,9 +300,11 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
   // mixed path keeps their sum (the parity path adds them in the reference's order)
   double dyy[5], dzz[FAST ? 1 : 5];
 #pragma unroll
-  for (int c = 0; c < 7; ++c)
+  for (int c = 0; c < 7; ++c) {
+    lcar[c] = P(0);
 #pragma unroll
-    for (int s = 0; s < 4; ++s) q[c][s] = P(0);
+    for (int s = 0; s < NQ; ++s) q[c][s] = P(0);
+  }
 #pragma unroll
   for (int r = 0; r < 5; ++r) { fprev[r] = fcur[r] = P(0); dyy[r] = 0.0; }
 #pragma unroll
@@ -314,10 +323,9 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
     own_bx = P(c.bx);
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
-      q[k][0] = q[k][1];
-      q[k][1] = q[k][2];
-      q[k][2] = q[k][3];
-      q[k][3] = x[k];
+#pragma unroll
+      for (int s = 0; s + 1 < NQ; ++s) q[k][s] = q[k][s + 1];
+      q[k][NQ - 1] = x[k];
     }
     sh.ex[pl & 1][tid] = P(c.by * c.w - c.bz * c.v);
     sh.ey[pl % 3][tid] = P(c.bz * c.u - c.bx * c.w);
@@ -342,10 +350,10 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
     if (fthread) {
       const int idx = (fa + 1) * T::PZ + (fb + 1);
 #pragma unroll
-      for (int r = 0; r < 5; ++r) sh.prim[r][idx] = q[r][3];
+      for (int r = 0; r < 5; ++r) sh.prim[r][idx] = q[r][NQ - 1];
       sh.prim[5][idx] = own_bx;
-      sh.prim[6][idx] = q[5][3];
-      sh.prim[7][idx] = q[6][3];
+      sh.prim[6][idx] = q[5][NQ - 1];
+      sh.prim[7][idx] = q[6][NQ - 1];
     }
     if (ring) {
       CellPrim<FAST> c;
@@ -401,9 +409,7 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
     }
     // b. the own column of plane f+1 into the ring; c. x-face f from the ring
     own_column(f + 1);
-    if (fthread && f >= ix0) {
-      P fl[7];
-      face_solve<P, LIM>(q, a.limiter, bnp[0], gam, inv_gm1, fl);
+    auto keep_x_face = [&](const P* fl) {
 #pragma unroll
       for (int r = 0; r < 5; ++r) {
         fprev[r] = fcur[r];
@@ -412,6 +418,36 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
       sh.fx[0][tid] = fl[0];
       sh.fx[1][tid] = fl[5];
       sh.fx[2][tid] = fl[6];
+    };
+    if constexpr (CARRY) {
+      if (fthread) {
+        // the slope of cell f (ring: planes f-1, f, f+1; mhd_sweep_pyccel.py:290-307)
+        P rs[7], ln[7];
+#pragma unroll
+        for (int c = 0; c < 7; ++c) {
+          const P sl = fast::slope<(CARRY ? LIM : 0)>(q[c][1] - q[c][0], q[c][2] - q[c][1]);
+          rs[c] = q[c][1] - 0.5f * sl;
+          ln[c] = q[c][1] + 0.5f * sl;
+        }
+        if (f >= ix0) {
+          const bool first = lcar[0] <= 0.0f || lcar[4] <= 0.0f || rs[0] <= 0.0f || rs[4] <= 0.0f;
+          P ls[7];
+#pragma unroll
+          for (int c = 0; c < 7; ++c) {
+            ls[c] = fast::sel(first, q[c][0], lcar[c]);
+            rs[c] = fast::sel(first, q[c][1], rs[c]);
+          }
+          P fl[7];
+          fast::hlld(ls, rs, bnp[0], gam, inv_gm1, fl);
+          keep_x_face(fl);
+        }
+#pragma unroll
+        for (int c = 0; c < 7; ++c) lcar[c] = ln[c];
+      }
+    } else if (fthread && f >= ix0) {
+      P fl[7];
+      face_solve<P, LIM>(q, a.limiter, bnp[0], gam, inv_gm1, fl);
+      keep_x_face(fl);
     }
     __syncthreads();
     // e. the LDS plane is free: plane f+1 goes in for the next iteration
