@@ -426,8 +426,10 @@ int wb_amr3d_coarsen(double* buf, const void* blocks, const int32_t* children, i
 int wb_amr3d_rate(const double* buf, const void* blocks, int nblocks, double b0x, double b0y,
                   double b0z, double gamma, double* out, void* stream);
 
-/* One stage of every block (FP64 parity path / FP32 flux; waves: 2 or 3 per SIMD,
- * the register budget of the mixed kernel). */
+/* One stage of every block (FP64 parity path / FP32 flux).  waves, the mixed
+ * kernel's launch shape: 2, one 16x16 tile per block at 2 waves per SIMD; 3, half a
+ * block (8x16 tiles, two workgroups per block) at 3 waves per SIMD.  The results do
+ * not depend on it. */
 int wb_amr3d_stage(const double* in, double* out, const void* blocks, int nblocks, double b0x,
                    double b0y, double b0z, double gamma, int limiter, const double* dt, int heun,
                    int32_t* bad, void* stream);

@@ -52,7 +52,9 @@ class AMRMHD3D:
         self._bad = torch.zeros(2, dtype=torch.int32, device=self.device)
         self.steps = 0
         self.check_interval = 1
-        self.waves = 2  # registers of the mixed block stage (2 or 3 waves per SIMD)
+        # the mixed block stage's launch shape: 3 = half a block per workgroup at 3 waves
+        # per SIMD, 2 = a block per workgroup at 2 (the autotuner picks)
+        self.waves = 3
         self._upload_tables()
 
     # -- tables --------------------------------------------------------------------------

@@ -91,6 +91,28 @@ k_stage3d_blocks(StageArgs base, const BlockDesc* __restrict__ blocks) {
   stage3d_body<P, B, B, LIM>(a, 0, 0, 0, sh);
 }
 
+// Half a block per workgroup: an 8 x 16 tile of 192 threads; workgroups w and w + 8
+// are the two halves of one block and run on the same XCD.  Three waves per SIMD (the
+// register allocation spills a few values) run four such workgroups per CU, where one
+// 384-thread workgroup per block fits only six waves at its natural 187 registers:
+// 4.0 face solves per cell instead of 3.66, and the stage 30 % faster.
+template <typename P, int WAVES, int LIM>
+__global__ void __launch_bounds__((Tile<B / 2, B>::THREADS), WAVES)
+k_stage3d_halves(StageArgs base, const BlockDesc* __restrict__ blocks, int nblocks) {
+  __shared__ StageShared<P, B / 2, B> sh;
+  const int w = blockIdx.x;
+  const int b = (w >> 4) * 8 + (w & 7), half = (w >> 3) & 1;
+  if (b >= nblocks) return;
+  const BlockDesc d = blocks[b];
+  StageArgs a = base;
+  a.in = base.in + d.off;
+  a.out = base.out + d.off;
+  a.dx = d.dx;
+  a.dy = d.dy;
+  a.dz = d.dz;
+  stage3d_body<P, B / 2, B, LIM>(a, half, 0, 0, sh);
+}
+
 // The flux through the low `axis` face of haloed cell (i, j, k) of the block state
 // `a.in`, computed as the stage kernel computes it: the four cells along the normal
 // by CellPrim, the frame rows of the sweep, the face field of the potential, then

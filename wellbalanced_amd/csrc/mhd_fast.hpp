@@ -157,8 +157,12 @@ __device__ __forceinline__ void hlld(const float* L, const float* R, float bn, f
   const float vb2 = sm * bn + vt2_1 * bt2_1 + vt2_2 * bt2_2;
   const float abn = __builtin_fabsf(bn);
   const float ssl = sm - abn * isql, ssr = sm + abn * isqr;
-  // the side of the contact: the star and double-star states of that side only
-  const bool left = sm >= 0.0f;
+  // the side of the contact: the star and double-star states of that side only.  A
+  // supersonic face takes its upwind side's flux: that side is the one chosen here
+  // (S_L >= 0: left, S_R < 0: right), so the flux is ff there and no other side's flux
+  // stays live.
+  const bool supl = sl >= 0.0f, supr = sr < 0.0f;
+  const bool left = supl || (!supr && sm >= 0.0f);
   const float s = sel(left, sl, sr);
   const float ss = sel(left, __builtin_fminf(ssl, 0.0f), __builtin_fmaxf(ssr, 0.0f));
   const float sq = sel(left, -sql, sqr);
@@ -182,11 +186,10 @@ __device__ __forceinline__ void hlld(const float* L, const float* R, float bn, f
                        w2, w3};
   const float udd[7] = {rho_s, rho_s * sm, rho_s * vt2_1, rho_s * vt2_2,
                         us[4] + sq * (vb_s - vb2) * sign, bt2_1, bt2_2};
-  const bool supl = sl >= 0.0f, supr = sr < 0.0f;
 #pragma unroll
   for (int c = 0; c < 7; ++c) {
     const float mid = ff[c] + s * (us[c] - uu[c]) + ss * (udd[c] - us[c]);
-    flux[c] = sel(supl, sl_.f[c], sel(supr, sr_.f[c], mid));
+    flux[c] = sel(supl || supr, ff[c], mid);
   }
 }
 

@@ -21,8 +21,8 @@ int wb_amr3d_stage_mixed(const double* in, double* out, const void* blocks, int 
   return with_limiter(limiter, [&](auto lim) {
     constexpr int L = decltype(lim)::value;
     if (waves == 3)
-      hipLaunchKernelGGL((k_stage3d_blocks<float, 3, L>), dim3(nblocks), dim3(Tile<B, B>::THREADS),
-                         0, s, a, (const BlockDesc*)blocks);
+      hipLaunchKernelGGL((k_stage3d_halves<float, 3, L>), dim3(((nblocks + 7) / 8) * 16),
+                         dim3(Tile<B / 2, B>::THREADS), 0, s, a, (const BlockDesc*)blocks, nblocks);
     else
       hipLaunchKernelGGL((k_stage3d_blocks<float, 2, L>), dim3(nblocks), dim3(Tile<B, B>::THREADS),
                          0, s, a, (const BlockDesc*)blocks);
