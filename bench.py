@@ -283,13 +283,38 @@ def _line(metric_value, rate64, steps, args, sec, workload, cells, precision, l1
     return out
 
 
+def _timed_steps(m, args, graph_steps=0):
+    """Seconds and steps of the timed region: ``args.steps`` eager steps after
+    ``args.warmup``, or replays of a HIP graph of ``graph_steps`` steps (one replay
+    first, untimed).  Returns (seconds, steps)."""
+    import torch
+
+    from wellbalanced_amd.graph import StepGraph
+
+    if graph_steps:
+        graph = StepGraph(m, steps=graph_steps)
+        reps = max(1, args.steps // graph_steps)
+        graph.replay(1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        graph.replay(reps)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, reps * graph_steps
+    for _ in range(args.warmup):
+        m.advance(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    m.advance(args.steps)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, args.steps
+
+
 def _pair_2d(problem, args, graph_steps=0):
     """FP64 and mixed fused 2D models, tuned and timed; returns the line's pieces."""
     import numpy as np
     import torch
 
     from wellbalanced_amd import MHD2D, autotune
-    from wellbalanced_amd.graph import StepGraph
 
     res = {}
     models = {}
@@ -298,23 +323,7 @@ def _pair_2d(problem, args, graph_steps=0):
         launch = autotune.tune(m, steps=2) if not args.no_tune else {}
         m.check_interval = 0
         _, stage_ms = time_model(m, 3, 1)
-        if graph_steps:
-            graph = StepGraph(m, steps=graph_steps)
-            reps = max(1, args.steps // graph_steps)
-            graph.replay(1)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            graph.replay(reps)
-            torch.cuda.synchronize()
-            sec, steps = time.perf_counter() - t0, reps * graph_steps
-        else:
-            for _ in range(args.warmup):
-                m.advance(1)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            m.advance(args.steps)
-            torch.cuda.synchronize()
-            sec, steps = time.perf_counter() - t0, args.steps
+        sec, steps = _timed_steps(m, args, graph_steps)
         m.check()
         res[precision] = (m.nx * m.ny * steps / sec, sec, steps, stage_ms, launch)
         models[precision] = m
@@ -330,10 +339,11 @@ def run_ot2d(args):
 
     n = 256
     problem = initial.orszag_tang(n, limiter="minmod")
-    res, l1, linf, div, models = _pair_2d(problem, args)
+    res, l1, linf, div, models = _pair_2d(problem, args, graph_steps=10)
     rate, sec, steps, stage_ms, launch = res["mixed"]
     return _line(rate, res["fp64"][0], steps, args, sec,
-                 f"2D Orszag-Tang {n}^2 periodic, minmod, CFL 0.4, SSP-RK2", n * n, "mixed",
+                 f"2D Orszag-Tang {n}^2 periodic, minmod, CFL 0.4, SSP-RK2, steps replayed "
+                 "from HIP graphs of 10 steps", n * n, "mixed",
                  l1, linf, stage_ms, STATE_BYTES_2D,
                  {"launch": {"fp64": res["fp64"][4], "mixed": launch}, "divergence": div,
                   "time": models["mixed"].time})
@@ -475,24 +485,20 @@ def run_blast2d_amr(args):
         m = AMRMHD2D(h, limiter="minmod", mean_field=(2**-0.5, 2**-0.5), precision=precision)
         m.set_state(**blast_state())
         launch = autotune.tune(m, steps=2) if not args.no_tune else {}
+        m.check_interval = 0
         _, stage_ms = time_model(m, 3, 1)
-        for _ in range(args.warmup):
-            m.advance(1)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        m.advance(args.steps)
-        torch.cuda.synchronize()
-        sec = time.perf_counter() - t0
+        sec, steps = _timed_steps(m, args, graph_steps=10)
         m.check()
-        res[precision] = (m.cells * args.steps / sec, sec, stage_ms, launch)
+        res[precision] = (m.cells * steps / sec, sec, stage_ms, launch, steps)
         models[precision] = m
     l1, linf = _leaf_l1(models["fp64"], models["mixed"])
     m = models["mixed"]
     leaf = int(m.cells - sum(len(m.maps.average[L]) for L in range(m.nlev - 1)))
-    rate, sec, stage_ms, launch = res["mixed"]
-    return _line(rate, res["fp64"][0], args.steps, args, sec,
+    rate, sec, stage_ms, launch, steps = res["mixed"]
+    return _line(rate, res["fp64"][0], steps, args, sec,
                  "2D blast on a 3-level hierarchy (512^2 base, patches of 16..128 cells), "
-                 "minmod, CFL 0.4, SSP-RK2, one persistent launch per stage", m.cells, "mixed",
+                 "minmod, CFL 0.4, SSP-RK2, one persistent launch per stage, steps replayed "
+                 "from HIP graphs of 10 steps", m.cells, "mixed",
                  l1, linf, stage_ms, STATE_BYTES_2D,
                  {"launch": {"fp64": res["fp64"][3], "mixed": launch}, "patches": len(m.layouts),
                   "leaf_cells": leaf, "divergence": m.divergence(),
@@ -587,21 +593,18 @@ def run_blast3d_amr(args):
         m.check_interval = 0
         launch[precision] = autotune.tune(m, steps=2) if not args.no_tune else {}
         _, stage_ms = time_model(m, 2, 1)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        m.advance(args.steps)
-        torch.cuda.synchronize()
-        sec = time.perf_counter() - t0
+        sec, steps = _timed_steps(m, args, graph_steps=10)
         m.check()
-        res[precision] = (m.cells * args.steps / sec, sec, stage_ms)
+        res[precision] = (m.cells * steps / sec, sec, stage_ms, steps)
         models[precision] = m
     a, b = models["fp64"].level_state(0)[0][0], models["mixed"].level_state(0)[0][0]
     d = np.abs(a - b)[2:-2, 2:-2, 2:-2]
     m = models["mixed"]
-    rate, sec, stage_ms = res["mixed"]
-    return _line(rate, res["fp64"][0], args.steps, args, sec,
+    rate, sec, stage_ms, steps = res["mixed"]
+    return _line(rate, res["fp64"][0], steps, args, sec,
                  "3D blast in Alfvenic turbulence, 128^3 base, 3 levels of 16^3 blocks refined "
-                 "by the learned indicator, minmod, CFL 0.3, SSP-RK2", m.cells, "mixed",
+                 "by the learned indicator, minmod, CFL 0.3, SSP-RK2, steps replayed from HIP "
+                 "graphs of 10 steps", m.cells, "mixed",
                  float(d.mean()), float(d.max()), stage_ms, STATE_BYTES_3D,
                  {"blocks_per_level": [len(s) for s in h.levels], "launch": launch,
                   "leaf_cells": h.leaf_cells(), "divergence": m.divergence(),

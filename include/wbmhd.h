@@ -96,8 +96,8 @@ int wb_conductive2d(const double* prim, const double* bxf, const double* byf, in
  * with heun the average of mhd.py:983-988) in a single launch from the haloed state
  * `in` (halos filled) into `out`.  FP64: the same bits as the split kernels above.
  * wall_mask bit 0: x walls, bit 1: y walls.  nxs: cells along x per workgroup;
- * tile: cells along y per workgroup (62, 126 or 254; 254 runs as 126 with a
- * diffusivity). */
+ * tile: cells along y per workgroup (30, 62, 126 or 254; 254 runs as 126 with a
+ * diffusivity; other values run as 126). */
 int wb_mhd2d_stage(const double* in, double* out, int64_t nx, int64_t ny, double dx, double dy,
                    double b0x, double b0y, double gamma, double eta, double kpar, double kperp,
                    int limiter, const double* dt, int heun, int wall_mask, int32_t* bad, int nxs,

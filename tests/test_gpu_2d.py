@@ -229,7 +229,7 @@ def test_needs_a_state_first(gpu):
         model.set_state(1.0, (0.0, 0.0), 1.0)
 
 
-@pytest.mark.parametrize("tile,nxs", [(126, 32), (62, 5), (254, 64)])
+@pytest.mark.parametrize("tile,nxs", [(126, 32), (62, 5), (254, 64), (30, 4)])
 @pytest.mark.parametrize("limiter", ["minmod", "vanleer", "mc", "none"])
 def test_fused_fp64_stage_is_bitwise_the_split_path(gpu, tile, nxs, limiter):
     for problem in (initial.orszag_tang(40, limiter=limiter),

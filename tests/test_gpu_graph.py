@@ -42,3 +42,61 @@ def test_captured_3d_steps_equal_eager_steps(gpu):
         torch.cuda.synchronize()
         np.testing.assert_array_equal(graphed.arrays()[0], eager.arrays()[0])
         assert graphed.time == eager.time
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_captured_refined_2d_steps_equal_eager_steps(gpu, precision):
+    """A 3-level hierarchy of uneven patches: the captured steps (fills, the persistent
+    stage with its work counter cleared by a kernel, refluxing, coarsening, the CFL
+    reduction and the device dt) replay bit for bit as eager steps."""
+    import torch
+
+    from wellbalanced_amd import amr
+    from wellbalanced_amd.amr import AMRMHD2D
+
+    h = amr.blast_hierarchy(base=64, levels=3, sizes=(8, 24), seed=3)
+    eager = AMRMHD2D(h, mean_field=(0.5**0.5, 0.5**0.5), precision=precision)
+    graphed = AMRMHD2D(h, mean_field=(0.5**0.5, 0.5**0.5), precision=precision)
+    for m in (eager, graphed):
+        m.set_state(**amr.blast_state())
+        m.check_interval = 0
+    graph = StepGraph(graphed, steps=2)
+    for replay in range(3):
+        graph.replay()
+        eager.advance(2)
+        _ = eager.totals()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(graphed.buffer(), eager.buffer())
+        assert graphed.time == eager.time
+    graphed.check()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_captured_refined_3d_steps_equal_eager_steps(gpu, precision):
+    import torch
+
+    from wellbalanced_amd.amr3d import AMRMHD3D
+    from wellbalanced_amd.amr3d_maps import Hierarchy3D
+
+    h = Hierarchy3D(32)
+    h.refine(0, [(0, 0, 0)])
+    h.refine(1, [(1, 1, 1)])
+    state = dict(density=1.0, velocity=(0.1, -0.05, 0.02),
+                 pressure=lambda x, y, z: 1.0 + 4.0 * np.exp(
+                     -((x - 0.3) ** 2 + (y - 0.35) ** 2 + (z - 0.3) ** 2) / 0.01),
+                 vector_potential=(lambda x, y, z: 0.0 * x,
+                                   lambda x, y, z: 0.1 * np.sin(2 * np.pi * z),
+                                   lambda x, y, z: 0.1 * np.sin(2 * np.pi * x)))
+    eager = AMRMHD3D(h, mean_field=(0.3, 0.2, 0.1), limiter="minmod", precision=precision)
+    graphed = AMRMHD3D(h, mean_field=(0.3, 0.2, 0.1), limiter="minmod", precision=precision)
+    for m in (eager, graphed):
+        m.set_state(**state)
+        m.check_interval = 0
+    graph = StepGraph(graphed, steps=2)
+    for replay in range(3):
+        graph.replay()
+        eager.advance(2)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(graphed.buffer(), eager.buffer())
+        assert graphed.time == eager.time
+    graphed.check()

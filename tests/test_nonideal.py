@@ -287,7 +287,8 @@ def _bitwise(model, oracle):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("kernels,tile,nxs", [("split", 126, 32), ("fused", 126, 32),
-                                              ("fused", 62, 5), ("fused", 254, 64)])
+                                              ("fused", 62, 5), ("fused", 254, 64),
+                                              ("fused", 30, 4)])
 def test_nonideal_steps_are_bitwise_the_oracle(gpu, kernels, tile, nxs):
     from wellbalanced_amd import MHD2D
 

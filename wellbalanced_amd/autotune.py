@@ -2,10 +2,12 @@
 
 The fused stages have free parameters that do not change the results (the FP64
 path gives the same bits for any of them): the cells per workgroup along the
-contiguous axis (``tile``: 62, 126 or 254 in 2D -> 64, 128 or 256 threads; 30, 62 or
-126 on a hierarchy, whose small patches fill a 30-row tile better) and the cells each
-workgroup marches along x (``nxs``), and for the refined hierarchy the number of
-persistent workgroups.  `tune` times each candidate on the model's own
+contiguous axis (``tile``: 30, 62, 126 or 254 in 2D -> 32, 64, 128 or 256 threads, the
+small tiles and short segments giving a small grid enough workgroups to fill the chip;
+30, 62 or 126 on a hierarchy, whose small patches fill a 30-row tile better) and the
+cells each workgroup marches along x (``nxs``), for the refined hierarchy the number
+of persistent workgroups, and for the mixed 3D stages the waves per SIMD the register
+allocation targets (``waves``).  `tune` times each candidate on the model's own
 state for a few steps (state restored afterwards) and keeps the fastest; the choice
 is cached per (model kind, shape, precision) for the process.
 """
@@ -19,10 +21,10 @@ import torch
 
 _CACHE: dict[tuple, dict] = {}
 
-DEFAULTS_2D = {"tile": [62, 126, 254], "nxs": [8, 16, 32, 64]}
+DEFAULTS_2D = {"tile": [30, 62, 126, 254], "nxs": [4, 8, 16, 32, 64]}
 DEFAULTS_3D = {"nxs": [8, 16, 32, 64]}
 DEFAULTS_3D_MIXED = {"nxs": [8, 16, 32, 64], "waves": [2, 3]}
-DEFAULTS_AMR = {"tile": [30, 62, 126], "nxs": [8, 16, 32], "workgroups": [512, 1024, 2048]}
+DEFAULTS_AMR = {"tile": [30, 62, 126], "nxs": [4, 8, 16, 32], "workgroups": [512, 1024, 2048]}
 DEFAULTS_AMR3D_MIXED = {"waves": [2, 3]}
 
 

@@ -43,9 +43,11 @@ extern "C" int wb_mhd2d_stage(const double* in, double* out, int64_t nx, int64_t
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
   if (a.diff.conduction() || eta != 0.0) {
+    if (tile == 30) return launch_stage2d<double, 30, -1, true>(a, s);
     if (tile == 62) return launch_stage2d<double, 62, -1, true>(a, s);
     return launch_stage2d<double, 126, -1, true>(a, s);
   }
+  if (tile == 30) return launch_stage2d<double, 30>(a, s);
   if (tile == 62) return launch_stage2d<double, 62>(a, s);
   if (tile == 254) return launch_stage2d<double, 254>(a, s);
   return launch_stage2d<double, 126>(a, s);

@@ -24,9 +24,11 @@ extern "C" int wb_mhd2d_stage_mixed(const double* in, double* out, int64_t nx, i
   return with_limiter(limiter, [&](auto lim) {
     constexpr int L = decltype(lim)::value;
     if (ni) {
+      if (tile == 30) return launch_stage2d<float, 30, L, true>(a, s);
       if (tile == 62) return launch_stage2d<float, 62, L, true>(a, s);
       return launch_stage2d<float, 126, L, true>(a, s);
     }
+    if (tile == 30) return launch_stage2d<float, 30, L>(a, s);
     if (tile == 62) return launch_stage2d<float, 62, L>(a, s);
     if (tile == 254) return launch_stage2d<float, 254, L>(a, s);
     return launch_stage2d<float, 126, L>(a, s);
