@@ -410,6 +410,11 @@ int wb_mhd3d_resistive(const double* in, double* out, int64_t nx, int64_t ny, in
 int wb_amr3d_fill(double* buf, const void* blocks, int nblocks, const int32_t* lookup,
                   const void* levels, void* stream);
 
+/* The same for the listed blocks only (ids: nids int32 block ids): the blocks whose
+ * stage reads its ghosts (direct = 0 below) and the coarse blocks a reflux reads. */
+int wb_amr3d_fill_list(double* buf, const void* blocks, const int32_t* ids, int nids,
+                       const int32_t* lookup, const void* levels, void* stream);
+
 /* New blocks (ids, all of one level >= 1): every cell and edge, interior included, by
  * the fill's interpolation from the level below (regrid: mhd3d.py regrid's
  * interpolation of the new fine patches). */
@@ -426,14 +431,20 @@ int wb_amr3d_coarsen(double* buf, const void* blocks, const int32_t* children, i
 int wb_amr3d_rate(const double* buf, const void* blocks, int nblocks, double b0x, double b0y,
                   double b0z, double gamma, double* out, void* stream);
 
-/* One stage of every block (FP64 parity path / FP32 flux).  waves, the mixed
- * kernel's launch shape: 2, one 16x16 tile per block at 2 waves per SIMD; 3, half a
- * block (8x16 tiles, two workgroups per block) at 3 waves per SIMD.  The results do
- * not depend on it. */
-int wb_amr3d_stage(const double* in, double* out, const void* blocks, int nblocks, double b0x,
+/* One stage of the listed blocks (ids: nids int32 block ids; null: blocks 0 ..
+ * nids - 1), FP64 parity path / FP32 flux.  The first ndirect listed blocks have all
+ * 26 neighbours on their level, and the stage reads the cells around such a block in
+ * place from them (lookup and levels as for the fill), not from its ghosts, which need
+ * no fill; the other blocks read their (filled) ghosts.  The results are the same.
+ * waves, the mixed kernel's launch shape: 2, one 16x16 tile per block at 2 waves per
+ * SIMD; 3, half a block (8x16 tiles, two workgroups per block) at 3 waves per SIMD.
+ * The results do not depend on it. */
+int wb_amr3d_stage(const double* in, double* out, const void* blocks, const int32_t* ids,
+                   int nids, int ndirect, const int32_t* lookup, const void* levels, double b0x,
                    double b0y, double b0z, double gamma, int limiter, const double* dt, int heun,
                    int32_t* bad, void* stream);
-int wb_amr3d_stage_mixed(const double* in, double* out, const void* blocks, int nblocks,
+int wb_amr3d_stage_mixed(const double* in, double* out, const void* blocks, const int32_t* ids,
+                         int nids, int ndirect, const int32_t* lookup, const void* levels,
                          double b0x, double b0y, double b0z, double gamma, int limiter,
                          const double* dt, int heun, int32_t* bad, int waves, void* stream);
 

@@ -55,6 +55,9 @@ class AMRMHD3D:
         # the mixed block stage's launch shape: 3 = half a block per workgroup at 3 waves
         # per SIMD, 2 = a block per workgroup at 2 (the autotuner picks)
         self.waves = 3
+        # the stage reads the neighbours of blocks with all 26 in place, so that only the
+        # other blocks' ghosts are filled (a launch-shape choice; the autotuner picks)
+        self.direct = True
         self._upload_tables()
 
     # -- tables --------------------------------------------------------------------------
@@ -79,6 +82,14 @@ class AMRMHD3D:
                                                    device=self.device), len(rows)))
         self._reflux = [(self._bytes(items) if len(items) else self._blocks, len(items))
                         for items in t.reflux]
+        # the stage's blocks: first those reading their neighbours in place, then those
+        # reading their (filled) ghosts
+        ids = np.arange(self.nblocks, dtype=np.int32)
+        self._ndirect = int(t.direct.sum())
+        self._order = torch.as_tensor(np.concatenate([ids[t.direct], ids[~t.direct]]),
+                                      device=self.device)
+        self._filled = self._order[self._ndirect:]
+        self._all = torch.as_tensor(ids, device=self.device)
 
     # -- state ---------------------------------------------------------------------------
     def set_state(self, density, velocity, pressure, *, vector_potential=(0.0, 0.0, 0.0)) -> None:
@@ -184,15 +195,23 @@ class AMRMHD3D:
                       _lib.ptr(self._lookup), _lib.ptr(self._levels), stream)
 
     def _stage(self, src, dst, heun: bool, slot: int, stream: int, marks=None) -> None:
-        self._fill(src, stream)
+        # with direct reads, ghosts only where a stage or a reflux reads them
+        if not self.direct:
+            self._fill(src, stream)
+        elif len(self._filled):
+            _lib.call("wb_amr3d_fill_list", _lib.ptr(src), _lib.ptr(self._blocks),
+                      _lib.ptr(self._filled), len(self._filled), _lib.ptr(self._lookup),
+                      _lib.ptr(self._levels), stream)
         suffix = "" if self.precision == "fp64" else "_mixed"
         code = LIMITER_CODES.index(self.limiter)
         if marks is not None:
             marks[0].record()
         extra = () if self.precision == "fp64" else (int(self.waves),)
+        ids, ndirect = (self._order, self._ndirect) if self.direct else (self._all, 0)
         _lib.call("wb_amr3d_stage" + suffix, _lib.ptr(src), _lib.ptr(dst), _lib.ptr(self._blocks),
-                  self.nblocks, *self.mean_field, self.gamma, code, _lib.ptr(self._dt), int(heun),
-                  self._bad.data_ptr() + 4 * slot, *extra, stream)
+                  _lib.ptr(ids), self.nblocks, ndirect, _lib.ptr(self._lookup),
+                  _lib.ptr(self._levels), *self.mean_field, self.gamma, code, _lib.ptr(self._dt),
+                  int(heun), self._bad.data_ptr() + 4 * slot, *extra, stream)
         if marks is not None:
             marks[1].record()
         for d, (items, n) in enumerate(self._reflux):

@@ -6,8 +6,10 @@ contiguous axis (``tile``: 30, 62, 126 or 254 in 2D -> 32, 64, 128 or 256 thread
 small tiles and short segments giving a small grid enough workgroups to fill the chip;
 30, 62 or 126 on a hierarchy, whose small patches fill a 30-row tile better) and the
 cells each workgroup marches along x (``nxs``), for the refined hierarchy the number
-of persistent workgroups, and for the mixed 3D stages the waves per SIMD the register
-allocation targets (``waves``).  `tune` times each candidate on the model's own
+of persistent workgroups, for the mixed 3D stages the waves per SIMD the register
+allocation targets (``waves``), and for the 3D block hierarchy whether blocks with
+every neighbour read them in place instead of filled ghosts (``direct``).  `tune`
+times each candidate on the model's own
 state for a few steps (state restored afterwards) and keeps the fastest; the choice
 is cached per (model kind, shape, precision) for the process.
 """
@@ -25,7 +27,8 @@ DEFAULTS_2D = {"tile": [30, 62, 126, 254], "nxs": [4, 8, 16, 32, 64]}
 DEFAULTS_3D = {"nxs": [8, 16, 32, 64]}
 DEFAULTS_3D_MIXED = {"nxs": [8, 16, 32, 64], "waves": [2, 3]}
 DEFAULTS_AMR = {"tile": [30, 62, 126], "nxs": [4, 8, 16, 32], "workgroups": [512, 1024, 2048]}
-DEFAULTS_AMR3D_MIXED = {"waves": [2, 3]}
+DEFAULTS_AMR3D_MIXED = {"waves": [2, 3], "direct": [True, False]}
+DEFAULTS_AMR3D = {"direct": [True, False]}
 
 
 def _key(model) -> tuple:
@@ -54,7 +57,7 @@ def tune(model, space: dict | None = None, steps: int = 3) -> dict:
         return _CACHE[key]
     if space is None:
         if hasattr(model, "nblocks"):
-            space = DEFAULTS_AMR3D_MIXED if model.precision == "mixed" else {}
+            space = DEFAULTS_AMR3D_MIXED if model.precision == "mixed" else DEFAULTS_AMR3D
         elif hasattr(model, "layouts"):
             space = DEFAULTS_AMR
         elif hasattr(model, "n"):

@@ -32,6 +32,8 @@ struct Levels {
   int base[6];  // first entry of the level in the lookup table
 };
 
+inline bool levels_ok(const Levels& lv) { return lv.nlev >= 1 && lv.nlev <= 6; }
+
 struct RefluxItem {
   int fine, coarse, ci, cj, ck, pad;
 };
@@ -75,20 +77,39 @@ __device__ __forceinline__ StageArgs block_args(const StageArgs& base, const Blo
   return a;
 }
 
-// One stage of every block: one workgroup per block, a 16 x 16 tile marching over
-// the block's 16 planes.
+// The offsets (doubles) of the 27 blocks of block d's level around it, index
+// (x + 1) * 9 + (y + 1) * 3 + (z + 1), into `hood` (LDS) for a DIRECT stage: every one
+// of them exists.
+__device__ __forceinline__ void load_hood(long long* hood, const BlockDesc* __restrict__ blocks,
+                                          const int* __restrict__ lookup, const Levels& lv,
+                                          const BlockDesc& d) {
+  const int t = threadIdx.x;
+  if (t < 27) {
+    const int nb = lv.nb[d.level];
+    const int b0 = wrap(d.bx + t / 9 - 1, nb), b1 = wrap(d.by + (t / 3) % 3 - 1, nb),
+              b2 = wrap(d.bz + t % 3 - 1, nb);
+    hood[t] = blocks[lookup[lv.base[d.level] + (b0 * nb + b1) * nb + b2]].off;
+  }
+  __syncthreads();
+}
+
+// One stage of the listed blocks (ids; null: blocks 0, 1, ...): one workgroup per
+// block, a 16 x 16 tile marching over the block's 16 planes.  The first `ndirect`
+// listed blocks read their neighbours in place (load_hood), not their ghosts.
 template <typename P, int WAVES, int LIM>
 __global__ void __launch_bounds__((Tile<B, B>::THREADS), WAVES)
-k_stage3d_blocks(StageArgs base, const BlockDesc* __restrict__ blocks) {
+k_stage3d_blocks(StageArgs base, const BlockDesc* __restrict__ blocks, const int* __restrict__ ids,
+                 int ndirect, const int* __restrict__ lookup, Levels lv) {
   __shared__ StageShared<P, B, B> sh;
-  const BlockDesc d = blocks[blockIdx.x];
-  StageArgs a = base;
-  a.in = base.in + d.off;
-  a.out = base.out + d.off;
-  a.dx = d.dx;
-  a.dy = d.dy;
-  a.dz = d.dz;
-  stage3d_body<P, B, B, LIM>(a, 0, 0, 0, sh);
+  __shared__ long long hood[27];
+  const int k = blockIdx.x;
+  const BlockDesc d = blocks[ids ? ids[k] : k];
+  if (k < ndirect) {
+    load_hood(hood, blocks, lookup, lv, d);
+    stage3d_body<P, B, B, LIM, true>(block_args(base, d), 0, 0, 0, sh, hood, base.in);
+  } else {
+    stage3d_body<P, B, B, LIM, false>(block_args(base, d), 0, 0, 0, sh);
+  }
 }
 
 // Half a block per workgroup: an 8 x 16 tile of 192 threads; workgroups w and w + 8
@@ -98,19 +119,20 @@ k_stage3d_blocks(StageArgs base, const BlockDesc* __restrict__ blocks) {
 // 4.0 face solves per cell instead of 3.66, and the stage 30 % faster.
 template <typename P, int WAVES, int LIM>
 __global__ void __launch_bounds__((Tile<B / 2, B>::THREADS), WAVES)
-k_stage3d_halves(StageArgs base, const BlockDesc* __restrict__ blocks, int nblocks) {
+k_stage3d_halves(StageArgs base, const BlockDesc* __restrict__ blocks, const int* __restrict__ ids,
+                 int nblocks, int ndirect, const int* __restrict__ lookup, Levels lv) {
   __shared__ StageShared<P, B / 2, B> sh;
+  __shared__ long long hood[27];
   const int w = blockIdx.x;
   const int b = (w >> 4) * 8 + (w & 7), half = (w >> 3) & 1;
   if (b >= nblocks) return;
-  const BlockDesc d = blocks[b];
-  StageArgs a = base;
-  a.in = base.in + d.off;
-  a.out = base.out + d.off;
-  a.dx = d.dx;
-  a.dy = d.dy;
-  a.dz = d.dz;
-  stage3d_body<P, B / 2, B, LIM>(a, half, 0, 0, sh);
+  const BlockDesc d = blocks[ids ? ids[b] : b];
+  if (b < ndirect) {
+    load_hood(hood, blocks, lookup, lv, d);
+    stage3d_body<P, B / 2, B, LIM, true>(block_args(base, d), half, 0, 0, sh, hood, base.in);
+  } else {
+    stage3d_body<P, B / 2, B, LIM, false>(block_args(base, d), half, 0, 0, sh);
+  }
 }
 
 // The flux through the low `axis` face of haloed cell (i, j, k) of the block state

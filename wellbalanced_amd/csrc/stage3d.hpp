@@ -234,10 +234,16 @@ struct CellPrim {
 // One stage of the tile (ty_, tz_) and x segment seg_ of the grid a.g (the tiles
 // and segments of a launch together cover the grid).  LIM: the slope limiter fixed
 // at compile time (the mixed path's branch-free FP32 flux), or -1 for the parity
-// path's runtime limiter.
-template <typename P, int TY, int TZ, int LIM>
+// path's runtime limiter.  DIRECT (a block of a refined hierarchy whose 26 neighbours
+// on its level all exist): a cell outside the grid's interior is read in place from
+// the neighbouring block holding it, `hood` giving the offsets (doubles from `root`) of
+// the 27 blocks around this one (index (x + 1) * 9 + (y + 1) * 3 + (z + 1)); the halo
+// of a.in is not read.
+template <typename P, int TY, int TZ, int LIM, bool DIRECT = false>
 __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz_, int seg_,
-                                             StageShared<P, TY, TZ>& sh) {
+                                             StageShared<P, TY, TZ>& sh,
+                                             const long long* hood = nullptr,
+                                             const double* root = nullptr) {
   using T = Tile<TY, TZ>;
   constexpr bool FAST = sizeof(P) == 4;
   const Grid3 g = a.g;
@@ -264,7 +270,12 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
     }
     return min(max(h, 0), extent - 1);
   };
-  const int ry = read_index(gy, g.ny, g.Y, 2), rz = read_index(gz, g.nz, g.Z, 4);
+  // DIRECT: the block (-1, 0, +1) holding haloed index h along an axis of n interior
+  // cells, and h in that block's haloed indices
+  auto rel = [](int h, int n) { return h < 2 ? -1 : (h >= n + 2 ? 1 : 0); };
+  const int yr = DIRECT ? rel(gy, g.ny) : 0, zr = DIRECT ? rel(gz, g.nz) : 0;
+  const int ry = DIRECT ? gy - yr * g.ny : read_index(gy, g.ny, g.Y, 2);
+  const int rz = DIRECT ? gz - zr * g.nz : read_index(gz, g.nz, g.Z, 4);
   const Addr3 d = addr3(a);
   const char* inb = reinterpret_cast<const char*>(a.in);
   char* outb = reinterpret_cast<char*>(a.out);
@@ -278,6 +289,18 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
   auto read_plane = [&](int pl) {
     if (a.wrap & 1) return pl < 2 ? pl + g.nx : (pl >= g.nx + 2 ? pl - g.nx : pl);
     return min(max(pl, 0), g.X - 1);
+  };
+  // the state a read of plane `pl` of a column in block (., yrel, zrel) takes, and the
+  // plane there
+  auto source = [&](int pl, int yrel, int zrel, int& lp) -> const char* {
+    if constexpr (DIRECT) {
+      const int xr = rel(pl, g.nx);
+      lp = pl - xr * g.nx;
+      return reinterpret_cast<const char*>(root + hood[(xr + 1) * 9 + (yrel + 1) * 3 + zrel + 1]);
+    } else {
+      lp = read_plane(pl);
+      return inb;
+    }
   };
   const double dt = *a.dt;
   const double dtdx = dt / a.dx, dtdy = dt / a.dy, dtdz = dt / a.dz;
@@ -316,7 +339,9 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
   auto own_column = [&](int pl) {
     if (!fthread) return;
     CellPrim<FAST> c;
-    c.load(a, d, inb, read_plane(pl), w0, w1, rdx, rdy, rdz);
+    int lp;
+    const char* src = source(pl, yr, zr, lp);
+    c.load(a, d, src, lp, w0, w1, rdx, rdy, rdz);
     bnp[0] = bnn[0]; bnp[1] = bnn[1]; bnp[2] = bnn[2];
     bnn[0] = P(c.fx); bnn[1] = P(c.fy); bnn[2] = P(c.fz);
     const P x[7] = {P(c.r), P(c.u), P(c.v), P(c.w), P(c.p), P(c.by), P(c.bz)};
@@ -341,8 +366,10 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
   if (tid < T::PZ) { rpy = 0; rpz = tid; }
   else if (tid < 2 * T::PZ) { rpy = T::PY - 1; rpz = tid - T::PZ; }
   else { const int e = tid - 2 * T::PZ; rpy = 1 + e / 2; rpz = (e & 1) ? T::PZ - 1 : 0; }
-  const int rj = read_index(jy0 + rpy - 2, g.ny, g.Y, 2);
-  const int rk = read_index(kz0 + rpz - 2, g.nz, g.Z, 4);
+  const int hj = jy0 + rpy - 2, hk = kz0 + rpz - 2;  // the ring column's haloed indices
+  const int ryr = DIRECT ? rel(hj, g.ny) : 0, rzr = DIRECT ? rel(hk, g.nz) : 0;
+  const int rj = DIRECT ? hj - ryr * g.ny : read_index(hj, g.ny, g.Y, 2);
+  const int rk = DIRECT ? hk - rzr * g.nz : read_index(hk, g.nz, g.Z, 4);
   const uint32_t r0 = (uint32_t)rj * d.crow + (uint32_t)rk * 8u;
   const uint32_t r1 = (uint32_t)rj * d.arow + (uint32_t)rk * 8u;
   // plane `pl` into LDS: the own column from registers, the outer ring computed here
@@ -357,7 +384,9 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
     }
     if (ring) {
       CellPrim<FAST> c;
-      c.load(a, d, inb, read_plane(pl), r0, r1, rdx, rdy, rdz);
+      int lp;
+      const char* src = source(pl, ryr, rzr, lp);
+      c.load(a, d, src, lp, r0, r1, rdx, rdy, rdz);
       const int idx = rpy * T::PZ + rpz;
       sh.prim[0][idx] = P(c.r); sh.prim[1][idx] = P(c.u); sh.prim[2][idx] = P(c.v);
       sh.prim[3][idx] = P(c.w); sh.prim[4][idx] = P(c.p); sh.prim[5][idx] = P(c.bx);

@@ -166,15 +166,15 @@ __device__ __forceinline__ void fill_edges(double* __restrict__ buf, const Hood&
 // nodes (odd).  Only values a fill never writes are read, so one launch fills every
 // level.
 //
-// With `ids` (prolongation of new blocks) the listed blocks take every cell and edge,
-// interior included, by interpolation from the level below.
+// `ids`: the blocks to fill (null: every block).  With `all` (prolongation of new
+// blocks) the listed blocks take every cell and edge, interior included, by
+// interpolation from the level below.
 __global__ void __launch_bounds__(256) k_fill3d(double* __restrict__ buf,
                                                 const BlockDesc* __restrict__ blocks,
                                                 const int* __restrict__ lookup, Levels lv,
-                                                const int* __restrict__ ids) {
+                                                const int* __restrict__ ids, bool all) {
   __shared__ Hood same, coarse;
   const int self = ids ? ids[blockIdx.x] : (int)blockIdx.x;
-  const bool all = ids != nullptr;
   const BlockDesc d = blocks[self];
   const int L = d.level;
   const int tid = threadIdx.x;
@@ -388,7 +388,6 @@ __global__ void __launch_bounds__(256) k_rate3d_blocks(const double* __restrict_
   block_reduce_rate_rho_once(rate, rmin, keys);
 }
 
-bool levels_ok(const Levels& lv) { return lv.nlev >= 1 && lv.nlev <= 6; }
 
 }  // namespace
 
@@ -423,7 +422,17 @@ int wb_amr3d_fill(double* buf, const void* blocks, int nblocks, const int32_t* l
   const Levels lv = *(const Levels*)levels;
   if (nblocks < 1 || !levels_ok(lv)) return WB_EINVAL;
   hipLaunchKernelGGL(k_fill3d, dim3(nblocks, CELL_PARTS + 3), dim3(256), 0, (hipStream_t)stream, buf,
-                     (const BlockDesc*)blocks, (const int*)lookup, lv, (const int*)nullptr);
+                     (const BlockDesc*)blocks, (const int*)lookup, lv, (const int*)nullptr, false);
+  return wb_last_error();
+}
+
+int wb_amr3d_fill_list(double* buf, const void* blocks, const int32_t* ids, int nids,
+                       const int32_t* lookup, const void* levels, void* stream) {
+  const Levels lv = *(const Levels*)levels;
+  if (nids < 0 || !ids || !levels_ok(lv)) return WB_EINVAL;
+  if (nids == 0) return 0;
+  hipLaunchKernelGGL(k_fill3d, dim3(nids, CELL_PARTS + 3), dim3(256), 0, (hipStream_t)stream, buf,
+                     (const BlockDesc*)blocks, (const int*)lookup, lv, (const int*)ids, false);
   return wb_last_error();
 }
 
@@ -433,7 +442,7 @@ int wb_amr3d_prolong(double* buf, const void* blocks, const int32_t* ids, int ni
   if (nids < 0 || !levels_ok(lv)) return WB_EINVAL;
   if (nids == 0) return 0;
   hipLaunchKernelGGL(k_fill3d, dim3(nids, CELL_PARTS + 3), dim3(256), 0, (hipStream_t)stream, buf,
-                     (const BlockDesc*)blocks, (const int*)lookup, lv, (const int*)ids);
+                     (const BlockDesc*)blocks, (const int*)lookup, lv, (const int*)ids, true);
   return wb_last_error();
 }
 
@@ -463,14 +472,22 @@ int wb_amr3d_rate(const double* buf, const void* blocks, int nblocks, double b0x
   return wb_last_error();
 }
 
-int wb_amr3d_stage(const double* in, double* out, const void* blocks, int nblocks, double b0x,
+int wb_amr3d_stage(const double* in, double* out, const void* blocks, const int32_t* ids,
+                   int nids, int ndirect, const int32_t* lookup, const void* levels, double b0x,
                    double b0y, double b0z, double gamma, int limiter, const double* dt, int heun,
                    int32_t* bad, void* stream) {
-  if (nblocks < 1 || limiter < 0 || limiter > 3) return WB_EINVAL;
+  if (nids < 1 || limiter < 0 || limiter > 3 || ndirect < 0 || ndirect > nids) return WB_EINVAL;
+  Levels lv{};
+  if (ndirect) {
+    if (!levels || !lookup) return WB_EINVAL;
+    lv = *(const Levels*)levels;
+    if (!levels_ok(lv)) return WB_EINVAL;
+  }
   StageArgs a;
   wb_amr3d_base_args(&a, in, out, b0x, b0y, b0z, gamma, limiter, dt, heun, bad);
-  hipLaunchKernelGGL((k_stage3d_blocks<double, 1, -1>), dim3(nblocks), dim3(Tile<B, B>::THREADS),
-                     0, (hipStream_t)stream, a, (const BlockDesc*)blocks);
+  hipLaunchKernelGGL((k_stage3d_blocks<double, 1, -1>), dim3(nids), dim3(Tile<B, B>::THREADS), 0,
+                     (hipStream_t)stream, a, (const BlockDesc*)blocks, (const int*)ids, ndirect,
+                     (const int*)lookup, lv);
   return wb_last_error();
 }
 

@@ -11,21 +11,32 @@ extern "C" __attribute__((visibility("hidden"))) void wb_amr3d_base_args(
 
 extern "C" {
 
-int wb_amr3d_stage_mixed(const double* in, double* out, const void* blocks, int nblocks,
+int wb_amr3d_stage_mixed(const double* in, double* out, const void* blocks, const int32_t* ids,
+                         int nids, int ndirect, const int32_t* lookup, const void* levels,
                          double b0x, double b0y, double b0z, double gamma, int limiter,
                          const double* dt, int heun, int32_t* bad, int waves, void* stream) {
-  if (nblocks < 1 || (waves != 2 && waves != 3)) return WB_EINVAL;
+  if (nids < 1 || (waves != 2 && waves != 3) || ndirect < 0 || ndirect > nids) return WB_EINVAL;
+  Levels lv{};
+  if (ndirect) {
+    if (!levels || !lookup) return WB_EINVAL;
+    lv = *(const Levels*)levels;
+    if (!levels_ok(lv)) return WB_EINVAL;
+  }
   StageArgs a;
   wb_amr3d_base_args(&a, in, out, b0x, b0y, b0z, gamma, limiter, dt, heun, bad);
   hipStream_t s = (hipStream_t)stream;
+  const int* id = (const int*)ids;
+  const int* lk = (const int*)lookup;
   return with_limiter(limiter, [&](auto lim) {
     constexpr int L = decltype(lim)::value;
+    const dim3 halves(((nids + 7) / 8) * 16), tile_h(Tile<B / 2, B>::THREADS);
+    const dim3 tile_b(Tile<B, B>::THREADS);
     if (waves == 3)
-      hipLaunchKernelGGL((k_stage3d_halves<float, 3, L>), dim3(((nblocks + 7) / 8) * 16),
-                         dim3(Tile<B / 2, B>::THREADS), 0, s, a, (const BlockDesc*)blocks, nblocks);
+      hipLaunchKernelGGL((k_stage3d_halves<float, 3, L>), halves, tile_h, 0, s, a,
+                         (const BlockDesc*)blocks, id, nids, ndirect, lk, lv);
     else
-      hipLaunchKernelGGL((k_stage3d_blocks<float, 2, L>), dim3(nblocks), dim3(Tile<B, B>::THREADS),
-                         0, s, a, (const BlockDesc*)blocks);
+      hipLaunchKernelGGL((k_stage3d_blocks<float, 2, L>), dim3(nids), tile_b, 0, s, a,
+                         (const BlockDesc*)blocks, id, ndirect, lk, lv);
     return wb_last_error();
   });
 }
