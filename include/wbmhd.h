@@ -410,10 +410,13 @@ int wb_mhd3d_resistive(const double* in, double* out, int64_t nx, int64_t ny, in
 int wb_amr3d_fill(double* buf, const void* blocks, int nblocks, const int32_t* lookup,
                   const void* levels, void* stream);
 
-/* The same for the listed blocks only (ids: nids int32 block ids): the blocks whose
- * stage reads its ghosts (direct = 0 below) and the coarse blocks a reflux reads. */
+/* The same for the listed blocks only (ids: nids int32 block ids).  restricted = 1:
+ * only the ghost cells the level has no block for, and the ghost edges of those cells
+ * (what a stage or reflux reading its neighbours in place still reads from the ghosts;
+ * list the blocks with a missing neighbour). */
 int wb_amr3d_fill_list(double* buf, const void* blocks, const int32_t* ids, int nids,
-                       const int32_t* lookup, const void* levels, void* stream);
+                       int restricted, const int32_t* lookup, const void* levels,
+                       void* stream);
 
 /* New blocks (ids, all of one level >= 1): every cell and edge, interior included, by
  * the fill's interpolation from the level below (regrid: mhd3d.py regrid's
@@ -432,31 +435,36 @@ int wb_amr3d_rate(const double* buf, const void* blocks, int nblocks, double b0x
                   double b0z, double gamma, double* out, void* stream);
 
 /* One stage of the listed blocks (ids: nids int32 block ids; null: blocks 0 ..
- * nids - 1), FP64 parity path / FP32 flux.  The first ndirect listed blocks have all
- * 26 neighbours on their level, and the stage reads the cells around such a block in
- * place from them (lookup and levels as for the fill), not from its ghosts, which need
- * no fill; the other blocks read their (filled) ghosts.  The results are the same.
+ * nids - 1), FP64 parity path / FP32 flux.  direct = 1: the blocks read the cells
+ * around them in place from the blocks of their level holding them (lookup and levels
+ * as for the fill), and from their own ghosts only where the level has no such block
+ * (a restricted fill of the blocks with a missing neighbour suffices); direct = 0: from
+ * their (filled) ghosts.  The results are the same.
  * waves, the mixed kernel's launch shape: 2, one 16x16 tile per block at 2 waves per
  * SIMD; 3, half a block (8x16 tiles, two workgroups per block) at 3 waves per SIMD.
  * The results do not depend on it. */
 int wb_amr3d_stage(const double* in, double* out, const void* blocks, const int32_t* ids,
-                   int nids, int ndirect, const int32_t* lookup, const void* levels, double b0x,
+                   int nids, int direct, const int32_t* lookup, const void* levels, double b0x,
                    double b0y, double b0z, double gamma, int limiter, const double* dt, int heun,
                    int32_t* bad, void* stream);
 int wb_amr3d_stage_mixed(const double* in, double* out, const void* blocks, const int32_t* ids,
-                         int nids, int ndirect, const int32_t* lookup, const void* levels,
+                         int nids, int direct, const int32_t* lookup, const void* levels,
                          double b0x, double b0y, double b0z, double gamma, int limiter,
                          const double* dt, int heun, int32_t* bad, int waves, void* stream);
 
 /* Refluxing of one direction (axis, side 0: coarse cells at the fine blocks' low
  * faces, 1: at their high faces) after a stage; items: {int32 fine, coarse, ci, cj, ck,
- * pad} per coarse-fine block face.  Run the six directions in turn. */
+ * pad} per coarse-fine block face.  Run the six directions in turn.  lookup (with
+ * levels): the recomputed fluxes read the cells around a block in place as the stage
+ * does (null: from the blocks' filled ghosts). */
 int wb_amr3d_reflux(const double* src, double* dst, const void* blocks, const void* items,
-                    int nitems, int axis, int side, double b0x, double b0y, double b0z,
-                    double gamma, int limiter, const double* dt, int heun, void* stream);
+                    int nitems, int axis, int side, const int32_t* lookup, const void* levels,
+                    double b0x, double b0y, double b0z, double gamma, int limiter,
+                    const double* dt, int heun, void* stream);
 int wb_amr3d_reflux_mixed(const double* src, double* dst, const void* blocks, const void* items,
-                          int nitems, int axis, int side, double b0x, double b0y, double b0z,
-                          double gamma, int limiter, const double* dt, int heun, void* stream);
+                          int nitems, int axis, int side, const int32_t* lookup,
+                          const void* levels, double b0x, double b0y, double b0z, double gamma,
+                          int limiter, const double* dt, int heun, void* stream);
 
 #ifdef __cplusplus
 }

@@ -118,15 +118,15 @@ def test_refined_blocks_conserve_and_keep_div_b_zero(levels):
 @pytest.mark.parametrize("levels,direct", [(2, True), (2, False), (3, True)])
 def test_gpu_hierarchy_3d_is_bitwise_the_oracle(gpu, levels, direct):
     """Bit for bit the oracle, whose fill copies every ghost: with direct reads the
-    blocks with all 26 neighbours read them in place and keep stale ghosts."""
+    stage and the refluxing read the cells around a block in place from the blocks of
+    its level, and only the ghosts the level has no block for are filled."""
     from wellbalanced_amd.amr3d import AMRMHD3D
 
     h = _two_levels() if levels == 2 else _three_levels()
     t, amr = _oracle(h, _blast_state())
     model = AMRMHD3D(h, mean_field=B0, limiter="minmod")
     model.direct = direct
-    if direct:
-        assert 0 < model._ndirect < model.nblocks
+    assert 0 < len(model._missing) < model.nblocks
     model.set_state(**_blast_state())
     np.testing.assert_array_equal(model.buffer(), amr.buf)
     for _ in range(2):

@@ -55,8 +55,9 @@ class AMRMHD3D:
         # the mixed block stage's launch shape: 3 = half a block per workgroup at 3 waves
         # per SIMD, 2 = a block per workgroup at 2 (the autotuner picks)
         self.waves = 3
-        # the stage reads the neighbours of blocks with all 26 in place, so that only the
-        # other blocks' ghosts are filled (a launch-shape choice; the autotuner picks)
+        # the stage and the refluxing read the cells around a block in place from the
+        # blocks of its level, so that only the ghosts the level has no block for are
+        # filled (a launch-shape choice; the autotuner picks)
         self.direct = True
         self._upload_tables()
 
@@ -82,14 +83,10 @@ class AMRMHD3D:
                                                    device=self.device), len(rows)))
         self._reflux = [(self._bytes(items) if len(items) else self._blocks, len(items))
                         for items in t.reflux]
-        # the stage's blocks: first those reading their neighbours in place, then those
-        # reading their (filled) ghosts
+        # the blocks with a missing neighbour on their level: with direct reads, the only
+        # ones whose ghosts (those the level has no block for) are filled
         ids = np.arange(self.nblocks, dtype=np.int32)
-        self._ndirect = int(t.direct.sum())
-        self._order = torch.as_tensor(np.concatenate([ids[t.direct], ids[~t.direct]]),
-                                      device=self.device)
-        self._filled = self._order[self._ndirect:]
-        self._all = torch.as_tensor(ids, device=self.device)
+        self._missing = torch.as_tensor(ids[t.missing], device=self.device)
 
     # -- state ---------------------------------------------------------------------------
     def set_state(self, density, velocity, pressure, *, vector_potential=(0.0, 0.0, 0.0)) -> None:
@@ -195,29 +192,31 @@ class AMRMHD3D:
                       _lib.ptr(self._lookup), _lib.ptr(self._levels), stream)
 
     def _stage(self, src, dst, heun: bool, slot: int, stream: int, marks=None) -> None:
-        # with direct reads, ghosts only where a stage or a reflux reads them
+        # direct reads: the stage and the refluxing read the cells around a block in
+        # place from the blocks of its level, so only the ghosts the level has no block
+        # for are filled
         if not self.direct:
             self._fill(src, stream)
-        elif len(self._filled):
+        elif len(self._missing):
             _lib.call("wb_amr3d_fill_list", _lib.ptr(src), _lib.ptr(self._blocks),
-                      _lib.ptr(self._filled), len(self._filled), _lib.ptr(self._lookup),
+                      _lib.ptr(self._missing), len(self._missing), 1, _lib.ptr(self._lookup),
                       _lib.ptr(self._levels), stream)
         suffix = "" if self.precision == "fp64" else "_mixed"
         code = LIMITER_CODES.index(self.limiter)
         if marks is not None:
             marks[0].record()
         extra = () if self.precision == "fp64" else (int(self.waves),)
-        ids, ndirect = (self._order, self._ndirect) if self.direct else (self._all, 0)
         _lib.call("wb_amr3d_stage" + suffix, _lib.ptr(src), _lib.ptr(dst), _lib.ptr(self._blocks),
-                  _lib.ptr(ids), self.nblocks, ndirect, _lib.ptr(self._lookup),
+                  None, self.nblocks, int(self.direct), _lib.ptr(self._lookup),
                   _lib.ptr(self._levels), *self.mean_field, self.gamma, code, _lib.ptr(self._dt),
                   int(heun), self._bad.data_ptr() + 4 * slot, *extra, stream)
         if marks is not None:
             marks[1].record()
+        hood = (_lib.ptr(self._lookup), _lib.ptr(self._levels)) if self.direct else (None, None)
         for d, (items, n) in enumerate(self._reflux):
             if n:
                 _lib.call("wb_amr3d_reflux" + suffix, _lib.ptr(src), _lib.ptr(dst),
-                          _lib.ptr(self._blocks), _lib.ptr(items), n, d // 2, d % 2,
+                          _lib.ptr(self._blocks), _lib.ptr(items), n, d // 2, d % 2, *hood,
                           *self.mean_field, self.gamma, code, _lib.ptr(self._dt), int(heun), stream)
         for L in range(self.nlev - 2, -1, -1):
             self._coarsen(dst, L, stream)

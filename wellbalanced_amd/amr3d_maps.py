@@ -17,9 +17,9 @@ back to back in one buffer.  The tables:
 * ``children``: per refined block the 8 children (average-down and restriction);
 * ``reflux``: per direction, the coarse-fine block faces (the fine block, the coarse
   block across the face and the origin of the 8 x 8 coarse cells there);
-* ``direct``: per block, whether its stage reads the cells around it in place from its
-  26 neighbours on its level (all of them exist) instead of its ghosts, which then need
-  no fill (a coarse block a reflux reads keeps its filled ghosts).
+* ``missing``: per block, whether a block of its level around it is missing: only
+  such blocks need (restricted) ghost fills when the stage and the refluxing read the
+  cells around a block in place from the blocks of its level.
 
 Domains are periodic on every axis (the refined 3D workloads).
 """
@@ -152,7 +152,7 @@ class Tables3D:
     reflux: list              # per direction (axis, side): REFLUX_DTYPE rows
     total: int                # doubles of the buffer
     size: int                 # doubles per block
-    direct: np.ndarray = None  # bool per block: the stage reads its neighbours in place
+    missing: np.ndarray = None  # bool per block: a block of its level around it is missing
 
 
 def build(h: Hierarchy3D) -> Tables3D:
@@ -208,20 +208,16 @@ def build(h: Hierarchy3D) -> Tables3D:
                     coarse = index[(L - 1, *cb)]
                     items.append((index[(L, *bc)], coarse, *(lo[a] % h.block for a in range(3)), 0))
             reflux.append(np.array(items, REFLUX_DTYPE))
-    # blocks with all 26 neighbours on their level read them in place, except the coarse
-    # blocks a reflux reads (its recomputed coarse fluxes read their ghosts)
-    targets = {int(c) for items in reflux for c in items["coarse"]}
-    direct = np.zeros(len(rows), bool)
+    # the blocks with a block of their level missing around them
+    missing = np.zeros(len(rows), bool)
     around = list(itertools.product((-1, 0, 1), repeat=3))
     for L in range(len(h.levels)):
         nb = h.nb(L)
         for bc in h.levels[L]:
-            k = index[(L, *bc)]
-            if k not in targets and all(
-                    tuple((bc[a] + d[a]) % nb for a in range(3)) in h.levels[L] for d in around):
-                direct[k] = True
+            missing[index[(L, *bc)]] = not all(
+                tuple((bc[a] + d[a]) % nb for a in range(3)) in h.levels[L] for d in around)
     return Tables3D(blocks, index, lookup, level_nb, level_base, children, reflux,
-                    len(rows) * size, size, direct)
+                    len(rows) * size, size, missing)
 
 
 # -- host views of a block ----------------------------------------------------------------

@@ -78,38 +78,37 @@ __device__ __forceinline__ StageArgs block_args(const StageArgs& base, const Blo
 }
 
 // The offsets (doubles) of the 27 blocks of block d's level around it, index
-// (x + 1) * 9 + (y + 1) * 3 + (z + 1), into `hood` (LDS) for a DIRECT stage: every one
-// of them exists.
+// (x + 1) * 9 + (y + 1) * 3 + (z + 1), -1 where the level has no such block, into
+// `hood` (LDS) for DIRECT reads; threads `t0` .. `t0` + 26 load it.
 __device__ __forceinline__ void load_hood(long long* hood, const BlockDesc* __restrict__ blocks,
                                           const int* __restrict__ lookup, const Levels& lv,
-                                          const BlockDesc& d) {
-  const int t = threadIdx.x;
-  if (t < 27) {
+                                          const BlockDesc& d, int t0 = 0) {
+  const int t = threadIdx.x - t0;
+  if (t >= 0 && t < 27) {
     const int nb = lv.nb[d.level];
     const int b0 = wrap(d.bx + t / 9 - 1, nb), b1 = wrap(d.by + (t / 3) % 3 - 1, nb),
               b2 = wrap(d.bz + t % 3 - 1, nb);
-    hood[t] = blocks[lookup[lv.base[d.level] + (b0 * nb + b1) * nb + b2]].off;
+    const int id = lookup[lv.base[d.level] + (b0 * nb + b1) * nb + b2];
+    hood[t] = id >= 0 ? blocks[id].off : -1;
   }
-  __syncthreads();
 }
 
 // One stage of the listed blocks (ids; null: blocks 0, 1, ...): one workgroup per
-// block, a 16 x 16 tile marching over the block's 16 planes.  The first `ndirect`
-// listed blocks read their neighbours in place (load_hood), not their ghosts.
-template <typename P, int WAVES, int LIM>
+// block, a 16 x 16 tile marching over the block's 16 planes.  DIRECT: the blocks read
+// the cells around them in place through their neighbourhoods (load_hood).
+template <typename P, int WAVES, int LIM, bool DIRECT>
 __global__ void __launch_bounds__((Tile<B, B>::THREADS), WAVES)
 k_stage3d_blocks(StageArgs base, const BlockDesc* __restrict__ blocks, const int* __restrict__ ids,
-                 int ndirect, const int* __restrict__ lookup, Levels lv) {
+                 const int* __restrict__ lookup, Levels lv) {
   __shared__ StageShared<P, B, B> sh;
   __shared__ long long hood[27];
   const int k = blockIdx.x;
   const BlockDesc d = blocks[ids ? ids[k] : k];
-  if (k < ndirect) {
+  if (DIRECT) {
     load_hood(hood, blocks, lookup, lv, d);
-    stage3d_body<P, B, B, LIM, true>(block_args(base, d), 0, 0, 0, sh, hood, base.in);
-  } else {
-    stage3d_body<P, B, B, LIM, false>(block_args(base, d), 0, 0, 0, sh);
+    __syncthreads();
   }
+  stage3d_body<P, B, B, LIM, DIRECT>(block_args(base, d), 0, 0, 0, sh, hood, base.in);
 }
 
 // Half a block per workgroup: an 8 x 16 tile of 192 threads; workgroups w and w + 8
@@ -117,34 +116,38 @@ k_stage3d_blocks(StageArgs base, const BlockDesc* __restrict__ blocks, const int
 // register allocation spills a few values) run four such workgroups per CU, where one
 // 384-thread workgroup per block fits only six waves at its natural 187 registers:
 // 4.0 face solves per cell instead of 3.66, and the stage 30 % faster.
-template <typename P, int WAVES, int LIM>
+template <typename P, int WAVES, int LIM, bool DIRECT>
 __global__ void __launch_bounds__((Tile<B / 2, B>::THREADS), WAVES)
 k_stage3d_halves(StageArgs base, const BlockDesc* __restrict__ blocks, const int* __restrict__ ids,
-                 int nblocks, int ndirect, const int* __restrict__ lookup, Levels lv) {
+                 int nblocks, const int* __restrict__ lookup, Levels lv) {
   __shared__ StageShared<P, B / 2, B> sh;
   __shared__ long long hood[27];
   const int w = blockIdx.x;
   const int b = (w >> 4) * 8 + (w & 7), half = (w >> 3) & 1;
   if (b >= nblocks) return;
   const BlockDesc d = blocks[ids ? ids[b] : b];
-  if (b < ndirect) {
+  if (DIRECT) {
     load_hood(hood, blocks, lookup, lv, d);
-    stage3d_body<P, B / 2, B, LIM, true>(block_args(base, d), half, 0, 0, sh, hood, base.in);
-  } else {
-    stage3d_body<P, B / 2, B, LIM, false>(block_args(base, d), half, 0, 0, sh);
+    __syncthreads();
   }
+  stage3d_body<P, B / 2, B, LIM, DIRECT>(block_args(base, d), half, 0, 0, sh, hood, base.in);
 }
 
 // The flux through the low `axis` face of haloed cell (i, j, k) of the block state
 // `a.in`, computed as the stage kernel computes it: the four cells along the normal
 // by CellPrim, the frame rows of the sweep, the face field of the potential, then
 // the face solve of the path (rows of the normal frame, as double).
+// `hood` (with `root`): the block's 27-block neighbourhood (load_hood), through which a
+// cell outside the block's interior is read in place from the block of its level
+// holding it, or, where the level has none, from the block's own (filled) ghosts; null:
+// every cell from the block's ghosts.
 template <typename P, int LIM>
-__device__ void face_flux3d(const StageArgs& a, int axis, int i, int j, int k, double out[7]) {
+__device__ void face_flux3d(const StageArgs& a, int axis, int i, int j, int k, double out[7],
+                            const long long* hood = nullptr, const double* root = nullptr) {
   constexpr bool FAST = sizeof(P) == 4;
   const Grid3 g = a.g;
   const EdgePot A{a.in + a.off_ax, a.in + a.off_ay, a.in + a.off_az, g.Y, g.Z};
-  const size_t PL = (size_t)g.X * g.Y * g.Z;
+  const Addr3 d = addr3(a);
   const double rdx = 1.0 / a.dx, rdy = 1.0 / a.dy, rdz = 1.0 / a.dz;
   constexpr int cx[7] = {0, 1, 2, 3, 4, 6, 7};
   constexpr int cy[7] = {0, 2, 3, 1, 4, 7, 5};
@@ -153,8 +156,17 @@ __device__ void face_flux3d(const StageArgs& a, int axis, int i, int j, int k, d
   for (int s = 0; s < 4; ++s) {
     const int ii = i + (axis == 0 ? s - 2 : 0), jj = j + (axis == 1 ? s - 2 : 0),
               kk = k + (axis == 2 ? s - 2 : 0);
+    const int ri = ii < 2 ? -1 : (ii >= B + 2 ? 1 : 0), rj = jj < 2 ? -1 : (jj >= B + 2 ? 1 : 0),
+              rk = kk < 2 ? -1 : (kk >= B + 2 ? 1 : 0);
+    const long long off = hood ? hood[(ri + 1) * 9 + (rj + 1) * 3 + rk + 1] : -1;
+    const bool there = off >= 0 && (ri | rj | rk) != 0;
+    const int pi = there ? ii - ri * B : ii, pj = there ? jj - rj * B : jj,
+              pk = there ? kk - rk * B : kk;
+    const char* src = there ? reinterpret_cast<const char*>(root + off)
+                            : reinterpret_cast<const char*>(a.in);
     CellPrim<FAST> c;
-    c.compute(a, A, PL, ii, jj, kk, rdx, rdy, rdz);
+    c.load(a, d, src, pi, (uint32_t)pj * d.crow + (uint32_t)pk * 8u,
+           (uint32_t)pj * d.arow + (uint32_t)pk * 8u, rdx, rdy, rdz);
     const P pr[8] = {P(c.r), P(c.u), P(c.v), P(c.w), P(c.p), P(c.bx), P(c.by), P(c.bz)};
     for (int r = 0; r < 7; ++r) q[r][s] = pr[axis == 0 ? cx[r] : (axis == 1 ? cy[r] : cz[r])];
   }
@@ -184,12 +196,22 @@ __device__ void face_flux3d(const StageArgs& a, int axis, int i, int j, int k, d
 template <typename P, int LIM>
 __global__ void __launch_bounds__(64) k_reflux3d(StageArgs base, const BlockDesc* __restrict__ blocks,
                                                  const RefluxItem* __restrict__ items, int nitems,
-                                                 int axis, int side, double* __restrict__ dst) {
+                                                 int axis, int side, double* __restrict__ dst,
+                                                 const int* __restrict__ lookup, Levels lv) {
+  __shared__ long long hoods[2][27];  // the fine and the coarse block's neighbourhoods
   const int it = blockIdx.x;
   if (it >= nitems) return;
   const RefluxItem item = items[it];
   const BlockDesc fd = blocks[item.fine], cd = blocks[item.coarse];
   const StageArgs fa = block_args(base, fd), ca = block_args(base, cd);
+  const bool direct = lookup != nullptr;
+  if (direct) {
+    load_hood(hoods[0], blocks, lookup, lv, fd);
+    load_hood(hoods[1], blocks, lookup, lv, cd, 32);
+  }
+  __syncthreads();
+  const long long* fh = direct ? hoods[0] : nullptr;
+  const long long* ch = direct ? hoods[1] : nullptr;
   const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
   const int u = threadIdx.x / 8, v = threadIdx.x % 8;
   // the coarse cell (haloed local) and the face of it on the fine side
@@ -199,7 +221,7 @@ __global__ void __launch_bounds__(64) k_reflux3d(StageArgs base, const BlockDesc
   int cf[3] = {c[0], c[1], c[2]};
   if (side == 0) cf[axis] += 1;  // its +axis face is the low face of the next cell
   double fc[7];
-  face_flux3d<P, LIM>(ca, axis, cf[0], cf[1], cf[2], fc);
+  face_flux3d<P, LIM>(ca, axis, cf[0], cf[1], cf[2], fc, ch, base.in);
   // the four fine faces: the fine block's low face (side 0) or high face (side 1)
   double fm[7] = {0, 0, 0, 0, 0, 0, 0};
   double ff[2][2][7];
@@ -209,7 +231,7 @@ __global__ void __launch_bounds__(64) k_reflux3d(StageArgs base, const BlockDesc
       f[axis] = side == 0 ? 2 : 2 + B;
       f[a1] = 2 + 2 * u + p;
       f[a2] = 2 + 2 * v + q;
-      face_flux3d<P, LIM>(fa, axis, f[0], f[1], f[2], ff[p][q]);
+      face_flux3d<P, LIM>(fa, axis, f[0], f[1], f[2], ff[p][q], fh, base.in);
     }
   for (int r = 0; r < 7; ++r) fm[r] = ((ff[0][0][r] + ff[0][1][r]) + (ff[1][0][r] + ff[1][1][r])) * 0.25;
   const double dt = *base.dt;

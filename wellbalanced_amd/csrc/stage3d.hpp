@@ -234,11 +234,11 @@ struct CellPrim {
 // One stage of the tile (ty_, tz_) and x segment seg_ of the grid a.g (the tiles
 // and segments of a launch together cover the grid).  LIM: the slope limiter fixed
 // at compile time (the mixed path's branch-free FP32 flux), or -1 for the parity
-// path's runtime limiter.  DIRECT (a block of a refined hierarchy whose 26 neighbours
-// on its level all exist): a cell outside the grid's interior is read in place from
-// the neighbouring block holding it, `hood` giving the offsets (doubles from `root`) of
-// the 27 blocks around this one (index (x + 1) * 9 + (y + 1) * 3 + (z + 1)); the halo
-// of a.in is not read.
+// path's runtime limiter.  DIRECT (a block of a refined hierarchy): a cell outside the
+// grid's interior is read in place from the neighbouring block of the same level
+// holding it, `hood` giving the offsets (doubles from `root`) of the 27 blocks around
+// this one (index (x + 1) * 9 + (y + 1) * 3 + (z + 1); -1: no such block, and the
+// cell is read from this block's own ghosts, which must then be filled).
 template <typename P, int TY, int TZ, int LIM, bool DIRECT = false>
 __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz_, int seg_,
                                              StageShared<P, TY, TZ>& sh,
@@ -274,8 +274,7 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
   // cells, and h in that block's haloed indices
   auto rel = [](int h, int n) { return h < 2 ? -1 : (h >= n + 2 ? 1 : 0); };
   const int yr = DIRECT ? rel(gy, g.ny) : 0, zr = DIRECT ? rel(gz, g.nz) : 0;
-  const int ry = DIRECT ? gy - yr * g.ny : read_index(gy, g.ny, g.Y, 2);
-  const int rz = DIRECT ? gz - zr * g.nz : read_index(gz, g.nz, g.Z, 4);
+  const int ry = read_index(gy, g.ny, g.Y, 2), rz = read_index(gz, g.nz, g.Z, 4);
   const Addr3 d = addr3(a);
   const char* inb = reinterpret_cast<const char*>(a.in);
   char* outb = reinterpret_cast<char*>(a.out);
@@ -285,22 +284,33 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
   const uint32_t c1 = (uint32_t)cgy * d.arow + (uint32_t)cgz * 8u;
   const uint32_t w0 = (uint32_t)ry * d.crow + (uint32_t)rz * 8u;
   const uint32_t w1 = (uint32_t)ry * d.arow + (uint32_t)rz * 8u;
+  // DIRECT: the column's offsets in the neighbouring block holding it
+  const int ly_ = gy - yr * g.ny, lz_ = gz - zr * g.nz;
+  const uint32_t n0 = (uint32_t)ly_ * d.crow + (uint32_t)lz_ * 8u;
+  const uint32_t n1 = (uint32_t)ly_ * d.arow + (uint32_t)lz_ * 8u;
   // the x-plane a read of plane `pl` (in [ix0 - 2, ix0 + nxv + 1]) takes
   auto read_plane = [&](int pl) {
     if (a.wrap & 1) return pl < 2 ? pl + g.nx : (pl >= g.nx + 2 ? pl - g.nx : pl);
     return min(max(pl, 0), g.X - 1);
   };
-  // the state a read of plane `pl` of a column in block (., yrel, zrel) takes, and the
-  // plane there
-  auto source = [&](int pl, int yrel, int zrel, int& lp) -> const char* {
+  // the state a read of plane `pl` of a column in block (., yrel, zrel) takes, the
+  // plane there and the column's offsets there (own: h0/h1, in the neighbour: l0/l1)
+  auto source = [&](int pl, int yrel, int zrel, uint32_t h0, uint32_t h1, uint32_t l0,
+                    uint32_t l1, int& lp, uint32_t& o0, uint32_t& o1) -> const char* {
     if constexpr (DIRECT) {
       const int xr = rel(pl, g.nx);
-      lp = pl - xr * g.nx;
-      return reinterpret_cast<const char*>(root + hood[(xr + 1) * 9 + (yrel + 1) * 3 + zrel + 1]);
-    } else {
-      lp = read_plane(pl);
-      return inb;
+      const long long off = hood[(xr + 1) * 9 + (yrel + 1) * 3 + zrel + 1];
+      if (off >= 0) {
+        lp = pl - xr * g.nx;
+        o0 = l0;
+        o1 = l1;
+        return reinterpret_cast<const char*>(root + off);
+      }
     }
+    lp = read_plane(pl);
+    o0 = h0;
+    o1 = h1;
+    return inb;
   };
   const double dt = *a.dt;
   const double dtdx = dt / a.dx, dtdy = dt / a.dy, dtdz = dt / a.dz;
@@ -340,8 +350,9 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
     if (!fthread) return;
     CellPrim<FAST> c;
     int lp;
-    const char* src = source(pl, yr, zr, lp);
-    c.load(a, d, src, lp, w0, w1, rdx, rdy, rdz);
+    uint32_t o0, o1;
+    const char* src = source(pl, yr, zr, w0, w1, n0, n1, lp, o0, o1);
+    c.load(a, d, src, lp, o0, o1, rdx, rdy, rdz);
     bnp[0] = bnn[0]; bnp[1] = bnn[1]; bnp[2] = bnn[2];
     bnn[0] = P(c.fx); bnn[1] = P(c.fy); bnn[2] = P(c.fz);
     const P x[7] = {P(c.r), P(c.u), P(c.v), P(c.w), P(c.p), P(c.by), P(c.bz)};
@@ -368,10 +379,12 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
   else { const int e = tid - 2 * T::PZ; rpy = 1 + e / 2; rpz = (e & 1) ? T::PZ - 1 : 0; }
   const int hj = jy0 + rpy - 2, hk = kz0 + rpz - 2;  // the ring column's haloed indices
   const int ryr = DIRECT ? rel(hj, g.ny) : 0, rzr = DIRECT ? rel(hk, g.nz) : 0;
-  const int rj = DIRECT ? hj - ryr * g.ny : read_index(hj, g.ny, g.Y, 2);
-  const int rk = DIRECT ? hk - rzr * g.nz : read_index(hk, g.nz, g.Z, 4);
+  const int rj = read_index(hj, g.ny, g.Y, 2), rk = read_index(hk, g.nz, g.Z, 4);
   const uint32_t r0 = (uint32_t)rj * d.crow + (uint32_t)rk * 8u;
   const uint32_t r1 = (uint32_t)rj * d.arow + (uint32_t)rk * 8u;
+  const int lj_ = hj - ryr * g.ny, lk_ = hk - rzr * g.nz;
+  const uint32_t m0 = (uint32_t)lj_ * d.crow + (uint32_t)lk_ * 8u;
+  const uint32_t m1 = (uint32_t)lj_ * d.arow + (uint32_t)lk_ * 8u;
   // plane `pl` into LDS: the own column from registers, the outer ring computed here
   auto store_plane = [&](int pl) {
     if (fthread) {
@@ -385,8 +398,9 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
     if (ring) {
       CellPrim<FAST> c;
       int lp;
-      const char* src = source(pl, ryr, rzr, lp);
-      c.load(a, d, src, lp, r0, r1, rdx, rdy, rdz);
+      uint32_t o0, o1;
+      const char* src = source(pl, ryr, rzr, r0, r1, m0, m1, lp, o0, o1);
+      c.load(a, d, src, lp, o0, o1, rdx, rdy, rdz);
       const int idx = rpy * T::PZ + rpz;
       sh.prim[0][idx] = P(c.r); sh.prim[1][idx] = P(c.u); sh.prim[2][idx] = P(c.v);
       sh.prim[3][idx] = P(c.w); sh.prim[4][idx] = P(c.p); sh.prim[5][idx] = P(c.bx);

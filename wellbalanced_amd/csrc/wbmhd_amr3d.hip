@@ -93,8 +93,9 @@ __device__ __forceinline__ int hood_master(const Hood& h, int c, const int u[3],
 // divisions are compile-time constants per component.
 template <int C>
 __device__ __forceinline__ void fill_edges(double* __restrict__ buf, const Hood& same,
-                                           const Hood& coarse, bool all, const int bo[3],
-                                           const int po[3], double* __restrict__ me) {
+                                           const Hood& coarse, bool all, bool restricted,
+                                           const int bo[3], const int po[3],
+                                           double* __restrict__ me) {
   constexpr int c = C;
   constexpr int e0 = c == 0 ? X : XN, e1 = c == 1 ? X : XN, e2 = c == 2 ? X : XN;
   const int nedge = all ? e0 * e1 * e2 : shell_size(e0, e1, e2);
@@ -106,6 +107,26 @@ __device__ __forceinline__ void fill_edges(double* __restrict__ buf, const Hood&
       p[2] = t % e2;
     } else {
       shell_at(t, e0, e1, e2, p);
+    }
+    if (restricted) {
+      // only an edge of a ghost cell whose level has no block holding it (the cells
+      // with one are read in place by the stage): c is the edge's cell axis, the
+      // others node axes shared by the cells at p - 1 and p
+      bool needed = false;
+      for (int s = 0; s < 4 && !needed; ++s) {
+        int q[3];
+        bool ok = true;
+        for (int a = 0, bit = 0; a < 3; ++a) {
+          q[a] = p[a];
+          if (a != c) q[a] -= (s >> bit++) & 1;
+          ok = ok && q[a] >= 0 && q[a] < X;
+        }
+        if (!ok || (q[0] >= 2 && q[0] < B + 2 && q[1] >= 2 && q[1] < B + 2 && q[2] >= 2 &&
+                    q[2] < B + 2))
+          continue;
+        needed = same.id[hood_index(rel_of(q[0] - 2), rel_of(q[1] - 2), rel_of(q[2] - 2))] < 0;
+      }
+      if (!needed) continue;
     }
     if (!all) {
       const int u[3] = {p[0] - 2, p[1] - 2, p[2] - 2};
@@ -172,7 +193,8 @@ __device__ __forceinline__ void fill_edges(double* __restrict__ buf, const Hood&
 __global__ void __launch_bounds__(256) k_fill3d(double* __restrict__ buf,
                                                 const BlockDesc* __restrict__ blocks,
                                                 const int* __restrict__ lookup, Levels lv,
-                                                const int* __restrict__ ids, bool all) {
+                                                const int* __restrict__ ids, bool all,
+                                                bool restricted) {
   __shared__ Hood same, coarse;
   const int self = ids ? ids[blockIdx.x] : (int)blockIdx.x;
   const BlockDesc d = blocks[self];
@@ -220,6 +242,7 @@ __global__ void __launch_bounds__(256) k_fill3d(double* __restrict__ buf,
       }
       const int k = hood_index(r[0], r[1], r[2]);
       if (same.id[k] >= 0) {
+        if (restricted) continue;  // the stage reads it in place
         const double* src = buf + same.off[k];
         const int o = cell_at(0, lc[0], lc[1], lc[2]);
         const int m = cell_at(0, p[0], p[1], p[2]);
@@ -266,9 +289,9 @@ __global__ void __launch_bounds__(256) k_fill3d(double* __restrict__ buf,
   }
   // edges of component c = part - CELL_PARTS (axis c is the cell axis, the others
   // node axes)
-  if (part == CELL_PARTS) fill_edges<0>(buf, same, coarse, all, bo, po, me);
-  if (part == CELL_PARTS + 1) fill_edges<1>(buf, same, coarse, all, bo, po, me);
-  if (part == CELL_PARTS + 2) fill_edges<2>(buf, same, coarse, all, bo, po, me);
+  if (part == CELL_PARTS) fill_edges<0>(buf, same, coarse, all, restricted, bo, po, me);
+  if (part == CELL_PARTS + 1) fill_edges<1>(buf, same, coarse, all, restricted, bo, po, me);
+  if (part == CELL_PARTS + 2) fill_edges<2>(buf, same, coarse, all, restricted, bo, po, me);
 }
 
 // Covered cells of a refined block := the mean of their 8 children (a workgroup per
@@ -422,17 +445,19 @@ int wb_amr3d_fill(double* buf, const void* blocks, int nblocks, const int32_t* l
   const Levels lv = *(const Levels*)levels;
   if (nblocks < 1 || !levels_ok(lv)) return WB_EINVAL;
   hipLaunchKernelGGL(k_fill3d, dim3(nblocks, CELL_PARTS + 3), dim3(256), 0, (hipStream_t)stream, buf,
-                     (const BlockDesc*)blocks, (const int*)lookup, lv, (const int*)nullptr, false);
+                     (const BlockDesc*)blocks, (const int*)lookup, lv, (const int*)nullptr, false,
+                     false);
   return wb_last_error();
 }
 
 int wb_amr3d_fill_list(double* buf, const void* blocks, const int32_t* ids, int nids,
-                       const int32_t* lookup, const void* levels, void* stream) {
+                       int restricted, const int32_t* lookup, const void* levels, void* stream) {
   const Levels lv = *(const Levels*)levels;
-  if (nids < 0 || !ids || !levels_ok(lv)) return WB_EINVAL;
+  if (nids < 0 || !ids || !levels_ok(lv) || restricted < 0 || restricted > 1) return WB_EINVAL;
   if (nids == 0) return 0;
   hipLaunchKernelGGL(k_fill3d, dim3(nids, CELL_PARTS + 3), dim3(256), 0, (hipStream_t)stream, buf,
-                     (const BlockDesc*)blocks, (const int*)lookup, lv, (const int*)ids, false);
+                     (const BlockDesc*)blocks, (const int*)lookup, lv, (const int*)ids, false,
+                     restricted != 0);
   return wb_last_error();
 }
 
@@ -442,7 +467,8 @@ int wb_amr3d_prolong(double* buf, const void* blocks, const int32_t* ids, int ni
   if (nids < 0 || !levels_ok(lv)) return WB_EINVAL;
   if (nids == 0) return 0;
   hipLaunchKernelGGL(k_fill3d, dim3(nids, CELL_PARTS + 3), dim3(256), 0, (hipStream_t)stream, buf,
-                     (const BlockDesc*)blocks, (const int*)lookup, lv, (const int*)ids, true);
+                     (const BlockDesc*)blocks, (const int*)lookup, lv, (const int*)ids, true,
+                     false);
   return wb_last_error();
 }
 
@@ -473,33 +499,46 @@ int wb_amr3d_rate(const double* buf, const void* blocks, int nblocks, double b0x
 }
 
 int wb_amr3d_stage(const double* in, double* out, const void* blocks, const int32_t* ids,
-                   int nids, int ndirect, const int32_t* lookup, const void* levels, double b0x,
+                   int nids, int direct, const int32_t* lookup, const void* levels, double b0x,
                    double b0y, double b0z, double gamma, int limiter, const double* dt, int heun,
                    int32_t* bad, void* stream) {
-  if (nids < 1 || limiter < 0 || limiter > 3 || ndirect < 0 || ndirect > nids) return WB_EINVAL;
+  if (nids < 1 || limiter < 0 || limiter > 3 || direct < 0 || direct > 1) return WB_EINVAL;
   Levels lv{};
-  if (ndirect) {
+  if (direct) {
     if (!levels || !lookup) return WB_EINVAL;
     lv = *(const Levels*)levels;
     if (!levels_ok(lv)) return WB_EINVAL;
   }
   StageArgs a;
   wb_amr3d_base_args(&a, in, out, b0x, b0y, b0z, gamma, limiter, dt, heun, bad);
-  hipLaunchKernelGGL((k_stage3d_blocks<double, 1, -1>), dim3(nids), dim3(Tile<B, B>::THREADS), 0,
-                     (hipStream_t)stream, a, (const BlockDesc*)blocks, (const int*)ids, ndirect,
-                     (const int*)lookup, lv);
+  const dim3 grid(nids), tile(Tile<B, B>::THREADS);
+  hipStream_t s = (hipStream_t)stream;
+  if (direct)
+    hipLaunchKernelGGL((k_stage3d_blocks<double, 1, -1, true>), grid, tile, 0, s, a,
+                       (const BlockDesc*)blocks, (const int*)ids, (const int*)lookup, lv);
+  else
+    hipLaunchKernelGGL((k_stage3d_blocks<double, 1, -1, false>), grid, tile, 0, s, a,
+                       (const BlockDesc*)blocks, (const int*)ids, (const int*)lookup, lv);
   return wb_last_error();
 }
 
 int wb_amr3d_reflux(const double* src, double* dst, const void* blocks, const void* items,
-                    int nitems, int axis, int side, double b0x, double b0y, double b0z,
-                    double gamma, int limiter, const double* dt, int heun, void* stream) {
+                    int nitems, int axis, int side, const int32_t* lookup, const void* levels,
+                    double b0x, double b0y, double b0z, double gamma, int limiter,
+                    const double* dt, int heun, void* stream) {
   if (nitems < 0 || axis < 0 || axis > 2 || side < 0 || side > 1) return WB_EINVAL;
+  Levels lv{};
+  if (lookup) {
+    if (!levels) return WB_EINVAL;
+    lv = *(const Levels*)levels;
+    if (!levels_ok(lv)) return WB_EINVAL;
+  }
   if (nitems == 0) return 0;
   StageArgs a;
   wb_amr3d_base_args(&a, src, dst, b0x, b0y, b0z, gamma, limiter, dt, heun, nullptr);
   hipLaunchKernelGGL((k_reflux3d<double, -1>), dim3(nitems), dim3(64), 0, (hipStream_t)stream, a,
-                     (const BlockDesc*)blocks, (const RefluxItem*)items, nitems, axis, side, dst);
+                     (const BlockDesc*)blocks, (const RefluxItem*)items, nitems, axis, side, dst,
+                     (const int*)lookup, lv);
   return wb_last_error();
 }
 

@@ -12,12 +12,12 @@ extern "C" __attribute__((visibility("hidden"))) void wb_amr3d_base_args(
 extern "C" {
 
 int wb_amr3d_stage_mixed(const double* in, double* out, const void* blocks, const int32_t* ids,
-                         int nids, int ndirect, const int32_t* lookup, const void* levels,
+                         int nids, int direct, const int32_t* lookup, const void* levels,
                          double b0x, double b0y, double b0z, double gamma, int limiter,
                          const double* dt, int heun, int32_t* bad, int waves, void* stream) {
-  if (nids < 1 || (waves != 2 && waves != 3) || ndirect < 0 || ndirect > nids) return WB_EINVAL;
+  if (nids < 1 || (waves != 2 && waves != 3) || direct < 0 || direct > 1) return WB_EINVAL;
   Levels lv{};
-  if (ndirect) {
+  if (direct) {
     if (!levels || !lookup) return WB_EINVAL;
     lv = *(const Levels*)levels;
     if (!levels_ok(lv)) return WB_EINVAL;
@@ -31,20 +31,33 @@ int wb_amr3d_stage_mixed(const double* in, double* out, const void* blocks, cons
     constexpr int L = decltype(lim)::value;
     const dim3 halves(((nids + 7) / 8) * 16), tile_h(Tile<B / 2, B>::THREADS);
     const dim3 tile_b(Tile<B, B>::THREADS);
-    if (waves == 3)
-      hipLaunchKernelGGL((k_stage3d_halves<float, 3, L>), halves, tile_h, 0, s, a,
-                         (const BlockDesc*)blocks, id, nids, ndirect, lk, lv);
+    if (waves == 3 && direct)
+      hipLaunchKernelGGL((k_stage3d_halves<float, 3, L, true>), halves, tile_h, 0, s, a,
+                         (const BlockDesc*)blocks, id, nids, lk, lv);
+    else if (waves == 3)
+      hipLaunchKernelGGL((k_stage3d_halves<float, 3, L, false>), halves, tile_h, 0, s, a,
+                         (const BlockDesc*)blocks, id, nids, lk, lv);
+    else if (direct)
+      hipLaunchKernelGGL((k_stage3d_blocks<float, 2, L, true>), dim3(nids), tile_b, 0, s, a,
+                         (const BlockDesc*)blocks, id, lk, lv);
     else
-      hipLaunchKernelGGL((k_stage3d_blocks<float, 2, L>), dim3(nids), tile_b, 0, s, a,
-                         (const BlockDesc*)blocks, id, ndirect, lk, lv);
+      hipLaunchKernelGGL((k_stage3d_blocks<float, 2, L, false>), dim3(nids), tile_b, 0, s, a,
+                         (const BlockDesc*)blocks, id, lk, lv);
     return wb_last_error();
   });
 }
 
 int wb_amr3d_reflux_mixed(const double* src, double* dst, const void* blocks, const void* items,
-                          int nitems, int axis, int side, double b0x, double b0y, double b0z,
-                          double gamma, int limiter, const double* dt, int heun, void* stream) {
+                          int nitems, int axis, int side, const int32_t* lookup,
+                          const void* levels, double b0x, double b0y, double b0z, double gamma,
+                          int limiter, const double* dt, int heun, void* stream) {
   if (nitems < 0 || axis < 0 || axis > 2 || side < 0 || side > 1) return WB_EINVAL;
+  Levels lv{};
+  if (lookup) {
+    if (!levels) return WB_EINVAL;
+    lv = *(const Levels*)levels;
+    if (!levels_ok(lv)) return WB_EINVAL;
+  }
   if (nitems == 0) return 0;
   StageArgs a;
   wb_amr3d_base_args(&a, src, dst, b0x, b0y, b0z, gamma, limiter, dt, heun, nullptr);
@@ -52,7 +65,8 @@ int wb_amr3d_reflux_mixed(const double* src, double* dst, const void* blocks, co
   return with_limiter(limiter, [&](auto lim) {
     constexpr int L = decltype(lim)::value;
     hipLaunchKernelGGL((k_reflux3d<float, L>), dim3(nitems), dim3(64), 0, s, a,
-                       (const BlockDesc*)blocks, (const RefluxItem*)items, nitems, axis, side, dst);
+                       (const BlockDesc*)blocks, (const RefluxItem*)items, nitems, axis, side, dst,
+                       (const int*)lookup, lv);
     return wb_last_error();
   });
 }
