@@ -289,25 +289,25 @@ int wb_cnn_wgrad(const uint16_t* x, int cin, const uint16_t* g16, int64_t nx, in
  * geometry (wellbalanced_amd/amr_maps.py).  Replaces patchwork's exchange_halos,
  * FluxRegister.reflux and restrict as MHD2D calls them (mhd.py:744-777,889-994). */
 
-/* Same-level halo copies: ncells ghost cells (6 fields, plane sizes per entry) and
- * nnodes A_z nodes (ghosts and shared boundary nodes from their canonical owner). */
-int wb_amr2d_copy(double* buf, const long long* cdst, const long long* csrc, const long long* cpd,
-                  const long long* cps, int ncells, const long long* ndst, const long long* nsrc,
-                  int nnodes, void* stream);
+/* One level's halo fill in one launch: ncells ghost cells (6 fields, plane sizes per
+ * entry) and nnodes A_z nodes (ghosts and shared boundary nodes) copied from their
+ * canonical same-level owner; the nic other ghost cells bilinear from the level below
+ * (upper-cell weights iwx, iwy) and the nin other ghost nodes by the coarse node or the
+ * 4-point midpoint rule per axis (kind jkx, jky).  Levels in increasing order. */
+int wb_amr2d_fill_level(double* buf, const long long* cdst, const long long* csrc,
+                        const long long* cpd, const long long* cps, int ncells,
+                        const long long* ndst, const long long* nsrc, int nnodes,
+                        const long long* idst, const long long* ipd, const long long* ic00,
+                        const long long* icy, const long long* ipc, const double* iwx,
+                        const double* iwy, int nic, const long long* jdst,
+                        const long long* jbase, const long long* jsx, const long long* jsy,
+                        const long long* jkx, const long long* jky, int nin, void* stream);
 
-/* Halos from the level below: bilinear cells (upper-cell weights wx, wy) and A_z
- * nodes by the coarse node or the 4-point midpoint rule per axis (kind nkx, nky). */
-int wb_amr2d_interp(double* buf, const long long* dst, const long long* pd, const long long* c00,
-                    const long long* cy, const long long* pc, const double* wx, const double* wy,
-                    int ncells, const long long* ndst, const long long* nbase,
-                    const long long* nsx, const long long* nsy, const long long* nkx,
-                    const long long* nky, int nnodes, void* stream);
-
-/* Covered coarse cells := mean of their four fine children (rows of 5 int64). */
-int wb_amr2d_average(double* buf, const long long* rows, int n, void* stream);
-
-/* Coarse A_z nodes := the fine node on them (rows of 2 int64), mhd.py:981,991. */
-int wb_amr2d_restrict(double* buf, const long long* rows, int n, void* stream);
+/* One level pair's coarsening in one launch: the na covered coarse cells := mean of their
+ * four fine children (arows, 5 int64 each) and the nr coarse A_z nodes := the fine node
+ * on them (rrows, 2 int64 each), mhd.py:981,991. */
+int wb_amr2d_coarsen(double* buf, const long long* arows, int na, const long long* rrows, int nr,
+                     void* stream);
 
 /* Regridding (MHD2D.regrid, mhd.py:1043-1056): copy into the new buffer `buf` the
  * cells (6 fields; offsets and planes in the new and the old buffer) and A_z nodes

@@ -71,16 +71,14 @@ SIGNATURES: dict[str, tuple[type, list]] = {
     "wb_cnn_adam": (I, [P, P, P, P, I, F, F, F, F, I, P]),
     "wb_cnn_conv": (I, [P, I, P, P, I64, I64, I64, I, P, P, P, I, P]),
     "wb_cnn_wgrad": (I, [P, I, P, I64, I64, I64, P, I, P, P]),
-    "wb_amr2d_copy": (I, [P] * 5 + [I] + [P, P, I, P]),
-    "wb_amr2d_interp": (I, [P] * 8 + [I] + [P] * 6 + [I, P]),
+    "wb_amr2d_fill_level": (I, [P] * 5 + [I, P, P, I] + [P] * 7 + [I] + [P] * 6 + [I, P]),
+    "wb_amr2d_coarsen": (I, [P, P, I, P, I, P]),
     "wb_amr2d_regrid_copy": (I, [P] * 6 + [I, P, P, I, P]),
     "wb_amr2d_prolong": (I, [P] * 8 + [I] + [P] * 5 + [I, P]),
     "wb_amr2d_leaf_values": (I, [P, P, P, I, D, D, D, P, P]),
     "wb_amr2d_totals_workspace": (I, []),
     "wb_amr2d_totals": (I, [P, P, P, I, P, P, P]),
     "wb_amr2d_divergence": (I, [P, P, P, I, D, D, P, I, P, I, P, P]),
-    "wb_amr2d_average": (I, [P, P, I, P]),
-    "wb_amr2d_restrict": (I, [P, P, I, P]),
     "wb_amr2d_rate": (I, [P, P, P, I, D, D, D, D, D, P, P, P]),
     "wb_amr2d_stage": (I, [P, P, P, P, I, P] + [D] * 6 + [I, P, I, P, I, I, I, P]),
     "wb_amr2d_stage_mixed": (I, [P, P, P, P, I, P] + [D] * 6 + [I, P, I, P, I, I, I, P]),

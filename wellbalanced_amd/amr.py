@@ -254,23 +254,22 @@ class AMRMHD2D:
 
     # -- the step ------------------------------------------------------------------------
     def _fill_level(self, buf: torch.Tensor, L: int, stream: int) -> None:
-        p = _lib.ptr(buf)
         cd, cs, cpd, cps, nd, ns = self._copy[L]
-        _lib.call("wb_amr2d_copy", p, _lib.ptr(cd), _lib.ptr(cs), _lib.ptr(cpd), _lib.ptr(cps),
-                  len(cd), _lib.ptr(nd), _lib.ptr(ns), len(nd), stream)
         t = self._interp[L]
-        _lib.call("wb_amr2d_interp", p, *[_lib.ptr(x) for x in t[:7]], len(t[0]),
-                  *[_lib.ptr(x) for x in t[7:]], len(t[7]), stream)
+        _lib.call("wb_amr2d_fill_level", _lib.ptr(buf), _lib.ptr(cd), _lib.ptr(cs),
+                  _lib.ptr(cpd), _lib.ptr(cps), len(cd), _lib.ptr(nd), _lib.ptr(ns), len(nd),
+                  *[_lib.ptr(x) for x in t[:7]], len(t[0]), *[_lib.ptr(x) for x in t[7:]],
+                  len(t[7]), stream)
 
     def _fill(self, buf: torch.Tensor, stream: int) -> None:
         for L in range(self.nlev):
             self._fill_level(buf, L, stream)
 
     def _coarsen(self, buf: torch.Tensor, L: int, stream: int) -> None:
-        rows, n = self._average[L]
-        _lib.call("wb_amr2d_average", _lib.ptr(buf), _lib.ptr(rows), n, stream)
-        rows, n = self._restrict[L]
-        _lib.call("wb_amr2d_restrict", _lib.ptr(buf), _lib.ptr(rows), n, stream)
+        arows, na = self._average[L]
+        rrows, nr = self._restrict[L]
+        _lib.call("wb_amr2d_coarsen", _lib.ptr(buf), _lib.ptr(arows), na, _lib.ptr(rrows), nr,
+                  stream)
 
     def _stage(self, src, dst, heun: bool, slot: int, stream: int) -> None:
         self._fill(src, stream)

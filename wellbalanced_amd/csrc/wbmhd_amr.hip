@@ -10,62 +10,6 @@ using namespace wb;
 
 namespace {
 
-__global__ void k_copy_cells(double* __restrict__ buf, const long long* __restrict__ dst,
-                             const long long* __restrict__ src, const long long* __restrict__ pd,
-                             const long long* __restrict__ ps, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const long long d = dst[i], s = src[i], a = pd[i], b = ps[i];
-#pragma unroll
-  for (int f = 0; f < 6; ++f) buf[d + f * a] = buf[s + f * b];
-}
-
-__global__ void k_copy_nodes(double* __restrict__ buf, const long long* __restrict__ dst,
-                             const long long* __restrict__ src, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) buf[dst[i]] = buf[src[i]];
-}
-
-// Bilinear from the 2x2 nearest coarse cells; wx1, wy1: weights of the upper cells.
-__global__ void k_interp_cells(double* __restrict__ buf, const long long* __restrict__ dst,
-                               const long long* __restrict__ pd, const long long* __restrict__ c00,
-                               const long long* __restrict__ cy, const long long* __restrict__ pc,
-                               const double* __restrict__ wx, const double* __restrict__ wy, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double wx1 = wx[i], wy1 = wy[i];
-  const double wx0 = 1.0 - wx1, wy0 = 1.0 - wy1;
-  const long long d = dst[i], a = pd[i], sy = cy[i], p = pc[i];
-#pragma unroll
-  for (int f = 0; f < 6; ++f) {
-    const long long b = c00[i] + f * p;
-    buf[d + f * a] = wy0 * (wx0 * buf[b] + wx1 * buf[b + sy]) + wy1 * (wx0 * buf[b + 1] + wx1 * buf[b + sy + 1]);
-  }
-}
-
-// Per axis: the coarse node itself (kind 0) or the 4-point midpoint rule
-// (-1, 9, 9, -1) / 16 on the nodes base .. base+3 (kind 1); a tensor product.
-__global__ void k_interp_nodes(double* __restrict__ buf, const long long* __restrict__ dst,
-                               const long long* __restrict__ base, const long long* __restrict__ sx,
-                               const long long* __restrict__ sy, const long long* __restrict__ kx,
-                               const long long* __restrict__ ky, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double w4[4] = {-0.0625, 0.5625, 0.5625, -0.0625};
-  const int nxw = kx[i] ? 4 : 1, nyw = ky[i] ? 4 : 1;
-  double total = 0.0;
-  for (int j = 0; j < nyw; ++j) {
-    double row = 0.0;
-    for (int k = 0; k < nxw; ++k) {
-      const double wk = kx[i] ? w4[k] : 1.0;
-      row = row + wk * buf[base[i] + k * sx[i] + j * sy[i]];
-    }
-    const double wj = ky[i] ? w4[j] : 1.0;
-    total = total + wj * row;
-  }
-  buf[dst[i]] = total;
-}
-
 // Regridding: the cells (6 fields) and nodes a patch of the same level held in the old
 // hierarchy's buffer, copied into the new buffer.
 __global__ void k_regrid_copy_cells(const double* __restrict__ old, double* __restrict__ buf,
@@ -136,25 +80,6 @@ __global__ void k_prolong_nodes(double* __restrict__ buf, const long long* __res
   buf[dst[i]] = v;
 }
 
-// Covered coarse cell = mean of its four fine children, ((f00 + f10) + (f01 + f11)) / 4.
-__global__ void k_average(double* __restrict__ buf, const long long* __restrict__ rows, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const long long* r = rows + 5 * (size_t)i;
-  const long long coarse = r[0], pc = r[1], fine = r[2], fy = r[3], pf = r[4];
-#pragma unroll
-  for (int f = 0; f < 6; ++f) {
-    const long long b = fine + f * pf;
-    buf[coarse + f * pc] = ((buf[b] + buf[b + fy]) + (buf[b + 1] + buf[b + fy + 1])) * 0.25;
-  }
-}
-
-// A coarse node that coincides with a fine node takes the fine value (mhd.py:981,991).
-__global__ void k_restrict(double* __restrict__ buf, const long long* __restrict__ rows, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) buf[rows[2 * (size_t)i]] = buf[rows[2 * (size_t)i + 1]];
-}
-
 // The CFL rate over the own cells of every patch (mhd_rate_pyccel.py:30-83 per patch,
 // the largest over all).  The cells are cut into chunks of 256 of one patch (chunks:
 // int32 pairs patch, first cell, built on the host); a fixed grid walks the chunks
@@ -219,30 +144,105 @@ __global__ void k_rate3_finish(double* out) {
 }
 
 inline dim3 blocks(int n) { return dim3((unsigned)((n + 255) / 256)); }
+__host__ __device__ inline int nblocks(int n) { return n > 0 ? (n + 255) / 256 : 0; }
+
+// The tables of one level's halo fill: same-level copies of cells and nodes, then the
+// interpolation of cells and nodes from the level below.
+struct FillLevel {
+  const long long *cdst, *csrc, *cpd, *cps;
+  const long long *ndst, *nsrc;
+  const long long *idst, *ipd, *ic00, *icy, *ipc;
+  const double *iwx, *iwy;
+  const long long *jdst, *jbase, *jsx, *jsy, *jkx, *jky;
+  int ncells, nnodes, nic, nin;
+};
+
+// One level's halo fill in one launch, the workgroups cut into four parts of 256
+// threads each: ghost cells (6 fields) and ghost or shared boundary A_z nodes copied
+// from a same-level owner; the other ghost cells bilinear from the 2x2 nearest coarse
+// cells (wx1, wy1: weights of the upper cells); the other ghost nodes, per axis the
+// coarse node itself (kind 0) or the 4-point midpoint rule (-1, 9, 9, -1) / 16 on the
+// nodes base .. base+3 (kind 1), a tensor product.  The parts write disjoint ghosts and
+// read only interior values of the level or values of the level below, so they need no
+// order among themselves.
+__global__ void __launch_bounds__(256) k_fill_level(double* __restrict__ buf, FillLevel t) {
+  int b = blockIdx.x;
+  const int b0 = nblocks(t.ncells), b1 = nblocks(t.nnodes), b2 = nblocks(t.nic);
+  const int tid = threadIdx.x;
+  if (b < b0) {
+    const int i = b * 256 + tid;
+    if (i >= t.ncells) return;
+    const long long d = t.cdst[i], sr = t.csrc[i], a = t.cpd[i], q = t.cps[i];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) buf[d + f * a] = buf[sr + f * q];
+    return;
+  }
+  b -= b0;
+  if (b < b1) {
+    const int i = b * 256 + tid;
+    if (i < t.nnodes) buf[t.ndst[i]] = buf[t.nsrc[i]];
+    return;
+  }
+  b -= b1;
+  if (b < b2) {
+    const int i = b * 256 + tid;
+    if (i >= t.nic) return;
+    const double wx1 = t.iwx[i], wy1 = t.iwy[i];
+    const double wx0 = 1.0 - wx1, wy0 = 1.0 - wy1;
+    const long long d = t.idst[i], a = t.ipd[i], sy = t.icy[i], pc = t.ipc[i];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+      const long long q = t.ic00[i] + f * pc;
+      buf[d + f * a] = wy0 * (wx0 * buf[q] + wx1 * buf[q + sy]) +
+                       wy1 * (wx0 * buf[q + 1] + wx1 * buf[q + sy + 1]);
+    }
+    return;
+  }
+  b -= b2;
+  const int i = b * 256 + tid;
+  if (i >= t.nin) return;
+  const double w4[4] = {-0.0625, 0.5625, 0.5625, -0.0625};
+  const int nxw = t.jkx[i] ? 4 : 1, nyw = t.jky[i] ? 4 : 1;
+  double total = 0.0;
+  for (int j = 0; j < nyw; ++j) {
+    double row = 0.0;
+    for (int k = 0; k < nxw; ++k) {
+      const double wk = t.jkx[i] ? w4[k] : 1.0;
+      row = row + wk * buf[t.jbase[i] + k * t.jsx[i] + j * t.jsy[i]];
+    }
+    const double wj = t.jky[i] ? w4[j] : 1.0;
+    total = total + wj * row;
+  }
+  buf[t.jdst[i]] = total;
+}
+
+// A level pair's coarsening in one launch: a covered coarse cell becomes the mean of its
+// four fine children, ((f00 + f10) + (f01 + f11)) / 4 (rows of 5 int64), and (the
+// remaining workgroups) a coarse A_z node that coincides with a fine node takes the
+// fine value (rows of 2 int64, mhd.py:981,991); cells and nodes are disjoint.
+__global__ void __launch_bounds__(256) k_coarsen(double* __restrict__ buf,
+                                                 const long long* __restrict__ arows, int na,
+                                                 const long long* __restrict__ rrows, int nr) {
+  const int b0 = nblocks(na);
+  if ((int)blockIdx.x < b0) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= na) return;
+    const long long* r = arows + 5 * (size_t)i;
+    const long long coarse = r[0], pc = r[1], fine = r[2], fy = r[3], pf = r[4];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+      const long long q = fine + f * pf;
+      buf[coarse + f * pc] = ((buf[q] + buf[q + fy]) + (buf[q + 1] + buf[q + fy + 1])) * 0.25;
+    }
+    return;
+  }
+  const int i = (blockIdx.x - b0) * 256 + threadIdx.x;
+  if (i < nr) buf[rrows[2 * (size_t)i]] = buf[rrows[2 * (size_t)i + 1]];
+}
 
 }  // namespace
 
 extern "C" {
-
-int wb_amr2d_copy(double* buf, const long long* cdst, const long long* csrc, const long long* cpd,
-                  const long long* cps, int ncells, const long long* ndst, const long long* nsrc,
-                  int nnodes, void* stream) {
-  hipStream_t s = (hipStream_t)stream;
-  if (ncells > 0) hipLaunchKernelGGL(k_copy_cells, blocks(ncells), dim3(256), 0, s, buf, cdst, csrc, cpd, cps, ncells);
-  if (nnodes > 0) hipLaunchKernelGGL(k_copy_nodes, blocks(nnodes), dim3(256), 0, s, buf, ndst, nsrc, nnodes);
-  return wb_last_error();
-}
-
-int wb_amr2d_interp(double* buf, const long long* dst, const long long* pd, const long long* c00,
-                    const long long* cy, const long long* pc, const double* wx, const double* wy,
-                    int ncells, const long long* ndst, const long long* nbase,
-                    const long long* nsx, const long long* nsy, const long long* nkx,
-                    const long long* nky, int nnodes, void* stream) {
-  hipStream_t s = (hipStream_t)stream;
-  if (ncells > 0) hipLaunchKernelGGL(k_interp_cells, blocks(ncells), dim3(256), 0, s, buf, dst, pd, c00, cy, pc, wx, wy, ncells);
-  if (nnodes > 0) hipLaunchKernelGGL(k_interp_nodes, blocks(nnodes), dim3(256), 0, s, buf, ndst, nbase, nsx, nsy, nkx, nky, nnodes);
-  return wb_last_error();
-}
 
 int wb_amr2d_regrid_copy(const double* old, double* buf, const long long* cdst,
                          const long long* csrc, const long long* cpd, const long long* cps,
@@ -267,13 +267,30 @@ int wb_amr2d_prolong(double* buf, const long long* dst, const long long* pd, con
   return wb_last_error();
 }
 
-int wb_amr2d_average(double* buf, const long long* rows, int n, void* stream) {
-  if (n > 0) hipLaunchKernelGGL(k_average, blocks(n), dim3(256), 0, (hipStream_t)stream, buf, rows, n);
+int wb_amr2d_fill_level(double* buf, const long long* cdst, const long long* csrc,
+                        const long long* cpd, const long long* cps, int ncells,
+                        const long long* ndst, const long long* nsrc, int nnodes,
+                        const long long* idst, const long long* ipd, const long long* ic00,
+                        const long long* icy, const long long* ipc, const double* iwx,
+                        const double* iwy, int nic, const long long* jdst,
+                        const long long* jbase, const long long* jsx, const long long* jsy,
+                        const long long* jkx, const long long* jky, int nin, void* stream) {
+  if (ncells < 0 || nnodes < 0 || nic < 0 || nin < 0) return WB_EINVAL;
+  const int n = nblocks(ncells) + nblocks(nnodes) + nblocks(nic) + nblocks(nin);
+  if (n == 0) return 0;
+  const FillLevel t{cdst, csrc, cpd, cps, ndst, nsrc, idst, ipd, ic00, icy, ipc, iwx, iwy,
+                    jdst, jbase, jsx, jsy, jkx, jky, ncells, nnodes, nic, nin};
+  hipLaunchKernelGGL(k_fill_level, dim3((unsigned)n), dim3(256), 0, (hipStream_t)stream, buf, t);
   return wb_last_error();
 }
 
-int wb_amr2d_restrict(double* buf, const long long* rows, int n, void* stream) {
-  if (n > 0) hipLaunchKernelGGL(k_restrict, blocks(n), dim3(256), 0, (hipStream_t)stream, buf, rows, n);
+int wb_amr2d_coarsen(double* buf, const long long* arows, int na, const long long* rrows, int nr,
+                     void* stream) {
+  if (na < 0 || nr < 0) return WB_EINVAL;
+  const int n = nblocks(na) + nblocks(nr);
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_coarsen, dim3((unsigned)n), dim3(256), 0, (hipStream_t)stream, buf, arows,
+                     na, rrows, nr);
   return wb_last_error();
 }
 
