@@ -452,19 +452,23 @@ int wb_amr3d_stage_mixed(const double* in, double* out, const void* blocks, cons
                          double b0x, double b0y, double b0z, double gamma, int limiter,
                          const double* dt, int heun, int32_t* bad, int waves, void* stream);
 
-/* Refluxing of one direction (axis, side 0: coarse cells at the fine blocks' low
- * faces, 1: at their high faces) after a stage; items: {int32 fine, coarse, ci, cj, ck,
- * pad} per coarse-fine block face.  Run the six directions in turn.  lookup (with
- * levels): the recomputed fluxes read the cells around a block in place as the stage
- * does (null: from the blocks' filled ghosts). */
+/* Refluxing after a stage, every coarse-fine block face of every level in one launch
+ * (FluxRegister.reflux, mhd3d.py:745-750).  items: {int32 fine, coarse, ci, cj, ck, dir}
+ * per coarse-fine block face (dir = 2 axis + side; side 0: the coarse cells at the fine
+ * block's low face, 1: at its high face); cells: {int32 n, item[3], lane[3], pad} per
+ * coarse cell with such faces, its n faces as (item, lane = u * 8 + v inside the item's
+ * 8 x 8 face) in increasing dir.  hoods: per block the 27 offsets (int64 doubles, -1:
+ * none) of the blocks of its level around it, index (x + 1) * 9 + (y + 1) * 3 + z + 1;
+ * the recomputed fluxes read the cells around a block in place as the stage does (null:
+ * from the blocks' filled ghosts). */
 int wb_amr3d_reflux(const double* src, double* dst, const void* blocks, const void* items,
-                    int nitems, int axis, int side, const int32_t* lookup, const void* levels,
-                    double b0x, double b0y, double b0z, double gamma, int limiter,
-                    const double* dt, int heun, void* stream);
+                    const void* cells, int ncells, const long long* hoods, double b0x,
+                    double b0y, double b0z, double gamma, int limiter, const double* dt, int heun,
+                    void* stream);
 int wb_amr3d_reflux_mixed(const double* src, double* dst, const void* blocks, const void* items,
-                          int nitems, int axis, int side, const int32_t* lookup,
-                          const void* levels, double b0x, double b0y, double b0z, double gamma,
-                          int limiter, const double* dt, int heun, void* stream);
+                          const void* cells, int ncells, const long long* hoods, double b0x,
+                          double b0y, double b0z, double gamma, int limiter, const double* dt,
+                          int heun, void* stream);
 
 #ifdef __cplusplus
 }

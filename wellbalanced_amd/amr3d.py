@@ -81,8 +81,10 @@ class AMRMHD3D:
             rows = t.children[level_of[t.children[:, 0]] == L] if len(t.children) else t.children
             self._children.append((torch.as_tensor(rows.reshape(-1), dtype=torch.int32,
                                                    device=self.device), len(rows)))
-        self._reflux = [(self._bytes(items) if len(items) else self._blocks, len(items))
-                        for items in t.reflux]
+        self._reflux_items = self._bytes(t.reflux_items) if len(t.reflux_items) else self._blocks
+        self._reflux_cells = self._bytes(t.reflux_cells) if len(t.reflux_cells) else self._blocks
+        self.nreflux = len(t.reflux_cells)
+        self._hoods = torch.as_tensor(t.hoods, device=self.device)
         # the blocks with a missing neighbour on their level: with direct reads, the only
         # ones whose ghosts (those the level has no block for) are filled
         ids = np.arange(self.nblocks, dtype=np.int32)
@@ -212,12 +214,12 @@ class AMRMHD3D:
                   int(heun), self._bad.data_ptr() + 4 * slot, *extra, stream)
         if marks is not None:
             marks[1].record()
-        hood = (_lib.ptr(self._lookup), _lib.ptr(self._levels)) if self.direct else (None, None)
-        for d, (items, n) in enumerate(self._reflux):
-            if n:
-                _lib.call("wb_amr3d_reflux" + suffix, _lib.ptr(src), _lib.ptr(dst),
-                          _lib.ptr(self._blocks), _lib.ptr(items), n, d // 2, d % 2, *hood,
-                          *self.mean_field, self.gamma, code, _lib.ptr(self._dt), int(heun), stream)
+        if self.nreflux:
+            _lib.call("wb_amr3d_reflux" + suffix, _lib.ptr(src), _lib.ptr(dst),
+                      _lib.ptr(self._blocks), _lib.ptr(self._reflux_items),
+                      _lib.ptr(self._reflux_cells), self.nreflux,
+                      _lib.ptr(self._hoods) if self.direct else None, *self.mean_field,
+                      self.gamma, code, _lib.ptr(self._dt), int(heun), stream)
         for L in range(self.nlev - 2, -1, -1):
             self._coarsen(dst, L, stream)
 

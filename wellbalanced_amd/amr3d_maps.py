@@ -48,6 +48,9 @@ BLOCK_DTYPE = np.dtype([("off", np.int64), ("level", np.int32), ("bx", np.int32)
                         ("dy", np.float64), ("dz", np.float64)])
 REFLUX_DTYPE = np.dtype([("fine", np.int32), ("coarse", np.int32), ("ci", np.int32),
                          ("cj", np.int32), ("ck", np.int32), ("pad", np.int32)])
+# a coarse cell with coarse-fine faces: its n faces as (item, lane) in direction order
+REFLUX_CELL_DTYPE = np.dtype([("n", np.int32), ("item", np.int32, 3), ("lane", np.int32, 3),
+                              ("pad", np.int32)])
 
 
 @dataclass
@@ -153,6 +156,9 @@ class Tables3D:
     total: int                # doubles of the buffer
     size: int                 # doubles per block
     missing: np.ndarray = None  # bool per block: a block of its level around it is missing
+    hoods: np.ndarray = None    # int64 (nblocks, 27): offsets of the blocks of its level around
+    reflux_items: np.ndarray = None  # REFLUX_DTYPE, every direction, pad = 2 axis + side
+    reflux_cells: np.ndarray = None  # REFLUX_CELL_DTYPE per refluxed coarse cell
 
 
 def build(h: Hierarchy3D) -> Tables3D:
@@ -216,8 +222,56 @@ def build(h: Hierarchy3D) -> Tables3D:
         for bc in h.levels[L]:
             missing[index[(L, *bc)]] = not all(
                 tuple((bc[a] + d[a]) % nb for a in range(3)) in h.levels[L] for d in around)
+    # the 27 blocks of its level around each block (x slowest), -1 where there is none
+    hoods = np.full((len(rows), 27), -1, np.int64)
+    for (L, *bc), k in index.items():
+        nb = h.nb(L)
+        for r, d in enumerate(around):
+            j = index.get((L, *((bc[a] + d[a]) % nb for a in range(3))))
+            if j is not None:
+                hoods[k, r] = blocks["off"][j]
+    items, cells = reflux_cells(reflux, h.block)
     return Tables3D(blocks, index, lookup, level_nb, level_base, children, reflux,
-                    len(rows) * size, size, missing)
+                    len(rows) * size, size, missing, hoods, items, cells)
+
+
+def reflux_cells(reflux: list, b: int = B) -> tuple[np.ndarray, np.ndarray]:
+    """Every direction's refluxing items in one table (pad = 2 axis + side) and, per
+    coarse cell with coarse-fine faces, its faces as (item, lane) in direction order;
+    cells with more faces first (a wave then runs one face count)."""
+    parts = []
+    for d, rows in enumerate(reflux):
+        r = rows.copy()
+        r["pad"] = d
+        parts.append(r)
+    items = np.concatenate(parts) if parts else np.zeros(0, REFLUX_DTYPE)
+    if len(items) == 0:
+        return items, np.zeros(0, REFLUX_CELL_DTYPE)
+    lane = np.tile(np.arange(64), len(items))
+    it = np.repeat(np.arange(len(items)), 64)
+    axis = items["pad"][it] // 2
+    c = np.stack([items["ci"][it], items["cj"][it], items["ck"][it]], axis=1)
+    a1 = np.where(axis == 0, 1, 0)
+    a2 = np.where(axis == 2, 1, 2)
+    rows_ = np.arange(len(it))
+    c[rows_, a1] += lane // 8
+    c[rows_, a2] += lane % 8
+    key = (items["coarse"][it].astype(np.int64) * b + c[:, 0]) * b * b + c[:, 1] * b + c[:, 2]
+    order = np.argsort(key, kind="stable")  # items are in direction order
+    key, it, lane = key[order], it[order], lane[order]
+    first = np.flatnonzero(np.r_[True, key[1:] != key[:-1]])
+    count = np.diff(np.r_[first, len(key)])
+    if count.max() > 3:
+        raise ValueError("a coarse cell with more than three coarse-fine faces")
+    cells = np.zeros(len(first), REFLUX_CELL_DTYPE)
+    cells["n"] = count
+    cells["item"] = -1
+    cells["lane"] = -1
+    for f in range(3):
+        has = count > f
+        cells["item"][has, f] = it[first[has] + f]
+        cells["lane"][has, f] = lane[first[has] + f]
+    return items, cells[np.argsort(-count, kind="stable")]
 
 
 # -- host views of a block ----------------------------------------------------------------

@@ -186,65 +186,92 @@ __device__ void face_flux3d(const StageArgs& a, int axis, int i, int j, int k, d
   for (int r = 0; r < 7; ++r) out[r] = double(fl[r]);
 }
 
-// Refluxing of one direction (axis, side) after a stage: per coarse-fine block face,
-// the 8 x 8 coarse cells across it replace the flux of their face on the fine side
-// by the mean of the four fine fluxes, both recomputed from the stage input `src`
-// (heun: half of the correction, as the stage's average).  side 0: the coarse cells
-// at the fine block's low face (their +axis face); side 1: at its high face.  One
-// coarse cell is refluxed at most once per direction, so the six directions run in
-// turn without atomics (deterministic).
+// Refluxing after a stage (FluxRegister.reflux as MHD3D.step calls it,
+// mhd3d.py:745-750): per coarse-fine block face, the 8 x 8 coarse cells across it
+// replace the flux of their face on the fine side by the mean of the four fine fluxes,
+// both recomputed from the stage input `src` (heun: half of the correction, as the
+// stage's average).  An item is one such face (dir = 2 axis + side; side 0: the coarse
+// cells at the fine block's low face, their +axis face; side 1: at its high face).
+//
+// A workgroup per 64 coarse cells that have such faces, five waves: wave 0 computes the
+// coarse flux of a cell's face, waves 1-4 the four fine fluxes (p, q) = ((w - 1) / 2,
+// (w - 1) % 2), and wave 0 combines them through LDS.  `cells` lists, per cell, its n
+// (1 to 3) faces as (item, lane) pairs in increasing direction order, the lane u * 8 + v
+// naming the cell inside the item's 8 x 8 face, and the cells with more faces first (the
+// first cell of a workgroup has its largest n).  Wave 0 applies the corrections of a
+// cell's faces in that order, which is the order of one pass per direction, without
+// atomics (deterministic).  `hoods`: per block the 27 offsets of the blocks of its level
+// around it (-1: none) for reads in place, or null to read the filled ghosts.
+struct RefluxCell {
+  int n, item[3], lane[3], pad;
+};
+
 template <typename P, int LIM>
-__global__ void __launch_bounds__(64) k_reflux3d(StageArgs base, const BlockDesc* __restrict__ blocks,
-                                                 const RefluxItem* __restrict__ items, int nitems,
-                                                 int axis, int side, double* __restrict__ dst,
-                                                 const int* __restrict__ lookup, Levels lv) {
-  __shared__ long long hoods[2][27];  // the fine and the coarse block's neighbourhoods
-  const int it = blockIdx.x;
-  if (it >= nitems) return;
-  const RefluxItem item = items[it];
-  const BlockDesc fd = blocks[item.fine], cd = blocks[item.coarse];
-  const StageArgs fa = block_args(base, fd), ca = block_args(base, cd);
-  const bool direct = lookup != nullptr;
-  if (direct) {
-    load_hood(hoods[0], blocks, lookup, lv, fd);
-    load_hood(hoods[1], blocks, lookup, lv, cd, 32);
-  }
-  __syncthreads();
-  const long long* fh = direct ? hoods[0] : nullptr;
-  const long long* ch = direct ? hoods[1] : nullptr;
-  const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
-  const int u = threadIdx.x / 8, v = threadIdx.x % 8;
-  // the coarse cell (haloed local) and the face of it on the fine side
-  int c[3] = {item.ci + 2, item.cj + 2, item.ck + 2};
-  c[a1] += u;
-  c[a2] += v;
-  int cf[3] = {c[0], c[1], c[2]};
-  if (side == 0) cf[axis] += 1;  // its +axis face is the low face of the next cell
-  double fc[7];
-  face_flux3d<P, LIM>(ca, axis, cf[0], cf[1], cf[2], fc, ch, base.in);
-  // the four fine faces: the fine block's low face (side 0) or high face (side 1)
-  double fm[7] = {0, 0, 0, 0, 0, 0, 0};
-  double ff[2][2][7];
-  for (int p = 0; p < 2; ++p)
-    for (int q = 0; q < 2; ++q) {
-      int f[3];
-      f[axis] = side == 0 ? 2 : 2 + B;
-      f[a1] = 2 + 2 * u + p;
-      f[a2] = 2 + 2 * v + q;
-      face_flux3d<P, LIM>(fa, axis, f[0], f[1], f[2], ff[p][q], fh, base.in);
-    }
-  for (int r = 0; r < 7; ++r) fm[r] = ((ff[0][0][r] + ff[0][1][r]) + (ff[1][0][r] + ff[1][1][r])) * 0.25;
+__global__ void __launch_bounds__(320) k_reflux3d(StageArgs base, const BlockDesc* __restrict__ blocks,
+                                                  const RefluxItem* __restrict__ items,
+                                                  const RefluxCell* __restrict__ cells, int ncells,
+                                                  const long long* __restrict__ hoods,
+                                                  double* __restrict__ dst) {
+  __shared__ double fine[4][7][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int t = blockIdx.x * 64 + lane;
+  const int nmax = cells[blockIdx.x * 64].n;
+  const int n = t < ncells ? cells[t].n : 0;
   const double dt = *base.dt;
-  const double dtd = axis == 0 ? dt / cd.dx : (axis == 1 ? dt / cd.dy : dt / cd.dz);
-  constexpr int xrow[5] = {0, 1, 2, 3, 4};
-  constexpr int yrow[5] = {0, 3, 1, 2, 4};
-  constexpr int zrow[5] = {0, 2, 3, 1, 4};
-  double* out = dst + cd.off;
-  for (int fld = 0; fld < 5; ++fld) {
-    const int row = axis == 0 ? xrow[fld] : (axis == 1 ? yrow[fld] : zrow[fld]);
-    const double d = side == 0 ? dtd * (fc[row] - fm[row]) : dtd * (fm[row] - fc[row]);
-    const int o = cell_at(fld, c[0], c[1], c[2]);
-    out[o] = out[o] + (base.heun ? 0.5 * d : d);
+  for (int f = 0; f < nmax; ++f) {
+    const bool has = f < n;
+    int axis = 0, side = 0, c[3] = {0, 0, 0};
+    double fl[7];
+    BlockDesc cd{};
+    if (has) {
+      const RefluxItem item = items[cells[t].item[f]];
+      const int lc = cells[t].lane[f];
+      axis = item.pad >> 1;
+      side = item.pad & 1;
+      const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+      const int u = lc >> 3, v = lc & 7;
+      // the coarse cell (haloed local) and the face of it on the fine side
+      c[0] = item.ci + 2;
+      c[1] = item.cj + 2;
+      c[2] = item.ck + 2;
+      c[a1] += u;
+      c[a2] += v;
+      int g[3];
+      if (w == 0) {
+        g[0] = c[0];
+        g[1] = c[1];
+        g[2] = c[2];
+        if (side == 0) g[axis] += 1;  // its +axis face is the low face of the next cell
+      } else {  // fine face (p, q): the fine block's low face (side 0) or high face
+        g[axis] = side == 0 ? 2 : 2 + B;
+        g[a1] = 2 + 2 * u + ((w - 1) >> 1);
+        g[a2] = 2 + 2 * v + ((w - 1) & 1);
+      }
+      const int blk = w == 0 ? item.coarse : item.fine;
+      const BlockDesc bd = blocks[blk];
+      cd = bd;
+      face_flux3d<P, LIM>(block_args(base, bd), axis, g[0], g[1], g[2], fl,
+                          hoods ? hoods + 27 * (size_t)blk : nullptr, base.in);
+      if (w > 0)
+        for (int r = 0; r < 7; ++r) fine[w - 1][r][lane] = fl[r];
+    }
+    __syncthreads();
+    if (has && w == 0) {
+      const double dtd = axis == 0 ? dt / cd.dx : (axis == 1 ? dt / cd.dy : dt / cd.dz);
+      constexpr int xrow[5] = {0, 1, 2, 3, 4};
+      constexpr int yrow[5] = {0, 3, 1, 2, 4};
+      constexpr int zrow[5] = {0, 2, 3, 1, 4};
+      double* out = dst + cd.off;
+      for (int fld = 0; fld < 5; ++fld) {
+        const int row = axis == 0 ? xrow[fld] : (axis == 1 ? yrow[fld] : zrow[fld]);
+        const double fm =
+            ((fine[0][row][lane] + fine[1][row][lane]) + (fine[2][row][lane] + fine[3][row][lane])) * 0.25;
+        const double d = side == 0 ? dtd * (fl[row] - fm) : dtd * (fm - fl[row]);
+        const int o = cell_at(fld, c[0], c[1], c[2]);
+        out[o] = out[o] + (base.heun ? 0.5 * d : d);
+      }
+    }
+    __syncthreads();
   }
 }
 

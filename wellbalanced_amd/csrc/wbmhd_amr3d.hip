@@ -523,22 +523,16 @@ int wb_amr3d_stage(const double* in, double* out, const void* blocks, const int3
 }
 
 int wb_amr3d_reflux(const double* src, double* dst, const void* blocks, const void* items,
-                    int nitems, int axis, int side, const int32_t* lookup, const void* levels,
-                    double b0x, double b0y, double b0z, double gamma, int limiter,
-                    const double* dt, int heun, void* stream) {
-  if (nitems < 0 || axis < 0 || axis > 2 || side < 0 || side > 1) return WB_EINVAL;
-  Levels lv{};
-  if (lookup) {
-    if (!levels) return WB_EINVAL;
-    lv = *(const Levels*)levels;
-    if (!levels_ok(lv)) return WB_EINVAL;
-  }
-  if (nitems == 0) return 0;
+                    const void* cells, int ncells, const long long* hoods, double b0x,
+                    double b0y, double b0z, double gamma, int limiter, const double* dt, int heun,
+                    void* stream) {
+  if (ncells < 0) return WB_EINVAL;
+  if (ncells == 0) return 0;
   StageArgs a;
   wb_amr3d_base_args(&a, src, dst, b0x, b0y, b0z, gamma, limiter, dt, heun, nullptr);
-  hipLaunchKernelGGL((k_reflux3d<double, -1>), dim3(nitems), dim3(64), 0, (hipStream_t)stream, a,
-                     (const BlockDesc*)blocks, (const RefluxItem*)items, nitems, axis, side, dst,
-                     (const int*)lookup, lv);
+  hipLaunchKernelGGL((k_reflux3d<double, -1>), dim3((unsigned)((ncells + 63) / 64)), dim3(320), 0,
+                     (hipStream_t)stream, a, (const BlockDesc*)blocks, (const RefluxItem*)items,
+                     (const RefluxCell*)cells, ncells, hoods, dst);
   return wb_last_error();
 }
 
