@@ -424,6 +424,15 @@ int wb_amr3d_fill_list(double* buf, const void* blocks, const int32_t* ids, int 
 int wb_amr3d_prolong(double* buf, const void* blocks, const int32_t* ids, int nids,
                      const int32_t* lookup, const void* levels, void* stream);
 
+/* The ghosts of listed regions a block's level has no block for (the only ghosts the
+ * stage and the refluxing read when they read the cells around a block in place):
+ * tasks = ntasks pairs {int32 block, region}, region (x + 1) * 9 + (y + 1) * 3 + z + 1
+ * of the 26 around the block.  Its cells interpolated from the level below, the edges
+ * of those cells outside the block's own range from their master copy on the level or
+ * interpolated, as wb_amr3d_fill. */
+int wb_amr3d_fill_regions(double* buf, const void* blocks, const int32_t* tasks, int ntasks,
+                          const int32_t* lookup, const void* levels, void* stream);
+
 /* One level's coarsening: the covered cells of each refined block := the mean of its
  * 8 children, and its edges := the mean of the two fine edges along them (every coarse
  * copy); children: rows of 9 int32 (the block, then its 8 children, x slowest). */
@@ -433,6 +442,9 @@ int wb_amr3d_coarsen(double* buf, const void* blocks, const int32_t* children, i
 /* The CFL rate over every block cell (mhd3d.py:674-724): out = device double[2]. */
 int wb_amr3d_rate(const double* buf, const void* blocks, int nblocks, double b0x, double b0y,
                   double b0z, double gamma, double* out, void* stream);
+/* The same for the mixed path: one division per cell and reciprocals, as wb_rate3d_mixed. */
+int wb_amr3d_rate_mixed(const double* buf, const void* blocks, int nblocks, double b0x,
+                        double b0y, double b0z, double gamma, double* out, void* stream);
 
 /* One stage of the listed blocks (ids: nids int32 block ids; null: blocks 0 ..
  * nids - 1), FP64 parity path / FP32 flux.  direct = 1: the blocks read the cells

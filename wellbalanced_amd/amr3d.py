@@ -89,6 +89,8 @@ class AMRMHD3D:
         # ones whose ghosts (those the level has no block for) are filled
         ids = np.arange(self.nblocks, dtype=np.int32)
         self._missing = torch.as_tensor(ids[t.missing], device=self.device)
+        self._regions = torch.as_tensor(t.fill_regions.reshape(-1), device=self.device)
+        self.nregions = len(t.fill_regions)
 
     # -- state ---------------------------------------------------------------------------
     def set_state(self, density, velocity, pressure, *, vector_potential=(0.0, 0.0, 0.0)) -> None:
@@ -199,9 +201,9 @@ class AMRMHD3D:
         # for are filled
         if not self.direct:
             self._fill(src, stream)
-        elif len(self._missing):
-            _lib.call("wb_amr3d_fill_list", _lib.ptr(src), _lib.ptr(self._blocks),
-                      _lib.ptr(self._missing), len(self._missing), 1, _lib.ptr(self._lookup),
+        elif self.nregions:
+            _lib.call("wb_amr3d_fill_regions", _lib.ptr(src), _lib.ptr(self._blocks),
+                      _lib.ptr(self._regions), self.nregions, _lib.ptr(self._lookup),
                       _lib.ptr(self._levels), stream)
         suffix = "" if self.precision == "fp64" else "_mixed"
         code = LIMITER_CODES.index(self.limiter)
@@ -234,8 +236,9 @@ class AMRMHD3D:
             events.append((marks[2], marks[3]))
 
     def _rate_kernel(self, stream: int) -> None:
-        _lib.call("wb_amr3d_rate", _lib.ptr(self._buf[0]), _lib.ptr(self._blocks), self.nblocks,
-                  *self.mean_field, self.gamma, _lib.ptr(self._rate), stream)
+        suffix = "" if self.precision == "fp64" else "_mixed"
+        _lib.call("wb_amr3d_rate" + suffix, _lib.ptr(self._buf[0]), _lib.ptr(self._blocks),
+                  self.nblocks, *self.mean_field, self.gamma, _lib.ptr(self._rate), stream)
 
     def stable_step(self) -> float:
         """``cfl`` over the largest fast-wave rate of any block cell (mhd3d.py:674-724)."""

@@ -159,6 +159,7 @@ class Tables3D:
     hoods: np.ndarray = None    # int64 (nblocks, 27): offsets of the blocks of its level around
     reflux_items: np.ndarray = None  # REFLUX_DTYPE, every direction, pad = 2 axis + side
     reflux_cells: np.ndarray = None  # REFLUX_CELL_DTYPE per refluxed coarse cell
+    fill_regions: np.ndarray = None  # int32 (n, 2): block, ghost region its level has no block for
 
 
 def build(h: Hierarchy3D) -> Tables3D:
@@ -232,7 +233,19 @@ def build(h: Hierarchy3D) -> Tables3D:
                 hoods[k, r] = blocks["off"][j]
     items, cells = reflux_cells(reflux, h.block)
     return Tables3D(blocks, index, lookup, level_nb, level_base, children, reflux,
-                    len(rows) * size, size, missing, hoods, items, cells)
+                    len(rows) * size, size, missing, hoods, items, cells,
+                    fill_regions(hoods))
+
+
+def fill_regions(hoods: np.ndarray) -> np.ndarray:
+    """The (block, region) pairs of the 26 ghost regions around each block (region
+    (x + 1) * 9 + (y + 1) * 3 + z + 1) that its level has no block for, the largest
+    regions (faces, then edges, then corners) first."""
+    b, r = np.nonzero(hoods == -1)
+    rel = np.stack([r // 9 - 1, (r // 3) % 3 - 1, r % 3 - 1], axis=1)
+    zeros = (rel == 0).sum(axis=1)  # 2: a face, 1: an edge, 0: a corner
+    order = np.lexsort((r, b, -zeros))
+    return np.stack([b[order], r[order]], axis=1).astype(np.int32)
 
 
 def reflux_cells(reflux: list, b: int = B) -> tuple[np.ndarray, np.ndarray]:

@@ -8,6 +8,7 @@
 // every block coordinate of a level.  The domain is periodic.
 #pragma once
 
+#include "rate3d.hpp"
 #include "stage3d.hpp"
 
 namespace wb {
@@ -273,6 +274,20 @@ __global__ void __launch_bounds__(320) k_reflux3d(StageArgs base, const BlockDes
     }
     __syncthreads();
   }
+}
+
+// The CFL rate (mhd3d.py:674-724) over the cells of every block: a workgroup per
+// block, each thread one (y, z) column marched along x as on the uniform grid (FAST:
+// the mixed path's reciprocals), one atomic pair per workgroup.
+template <bool FAST>
+__global__ void __launch_bounds__(256) k_rate3d_blocks(StageArgs base,
+                                                       const BlockDesc* __restrict__ blocks,
+                                                       unsigned long long* __restrict__ keys) {
+  const BlockDesc d = blocks[blockIdx.x];
+  double rate = 0.0, rmin = __builtin_inf();
+  rate_column<FAST>(block_args(base, d), 2 + (threadIdx.x >> 4), 2 + (threadIdx.x & 15), 2,
+                    2 + B, rate, rmin);
+  block_reduce_rate_rho_once(rate, rmin, keys);
 }
 
 }  // namespace amr3

@@ -17,16 +17,12 @@ namespace wb {
 
 constexpr int RATE_SEG = 16;
 
+// Column (j, k) of the state `a` over planes i0 .. i1 - 1 (haloed indices): the largest
+// rate into `rate`, the smallest density into `rmin`.
 template <bool FAST>
-__global__ void __launch_bounds__(256) k_rate3d_columns(StageArgs a,
-                                                       unsigned long long* __restrict__ keys) {
-  const Grid3 g = a.g;
-  const int k = 2 + blockIdx.x * 64 + (threadIdx.x & 63);
-  const int j = 2 + blockIdx.y * 4 + (threadIdx.x >> 6);
-  const int i0 = 2 + blockIdx.z * RATE_SEG;
-  const int i1 = min(i0 + RATE_SEG, g.nx + 2);
-  double rate = 0.0, rmin = __builtin_inf();
-  if (j < g.ny + 2 && k < g.nz + 2) {
+__device__ __forceinline__ void rate_column(const StageArgs& a, int j, int k, int i0, int i1,
+                                            double& rate, double& rmin) {
+  {
     const Addr3 d = addr3(a);
     const char* in = reinterpret_cast<const char*>(a.in);
     const uint32_t c0 = (uint32_t)j * d.crow + (uint32_t)k * 8u;  // cells, A_z
@@ -76,6 +72,18 @@ __global__ void __launch_bounds__(256) k_rate3d_columns(StageArgs a,
       az1 = az[1][1];
     }
   }
+}
+
+template <bool FAST>
+__global__ void __launch_bounds__(256) k_rate3d_columns(StageArgs a,
+                                                       unsigned long long* __restrict__ keys) {
+  const Grid3 g = a.g;
+  const int k = 2 + blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = 2 + blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int i0 = 2 + blockIdx.z * RATE_SEG;
+  const int i1 = min(i0 + RATE_SEG, g.nx + 2);
+  double rate = 0.0, rmin = __builtin_inf();
+  if (j < g.ny + 2 && k < g.nz + 2) rate_column<FAST>(a, j, k, i0, i1, rate, rmin);
   block_reduce_rate_rho_once(rate, rmin, keys);
 }
 

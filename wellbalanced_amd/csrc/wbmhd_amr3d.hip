@@ -89,6 +89,65 @@ __device__ __forceinline__ int hood_master(const Hood& h, int c, const int u[3],
   return -1;
 }
 
+// Edge p (haloed, component C) of the block at `me` outside its own range: unless
+// `all`, the value of its master copy on the same level (none written if the block
+// holds the master itself); without one (or with `all`), interpolated from the level
+// below: along the cell axis by the weights 1/4, 3/4, across it the coarse node (even
+// fine nodes) or the mean of the two coarse nodes (odd).
+template <int C>
+__device__ __forceinline__ void fill_edge_at(double* __restrict__ buf, const Hood& same,
+                                             const Hood& coarse, bool all, const int bo[3],
+                                             const int po[3], double* __restrict__ me,
+                                             const int p[3]) {
+  constexpr int c = C;
+  if (!all) {
+    const int u[3] = {p[0] - 2, p[1] - 2, p[2] - 2};
+    int loc[3];
+    const int k = hood_master(same, c, u, loc);
+    if (k >= 0) {
+      if (k == 13 && loc[0] == p[0] && loc[1] == p[1] && loc[2] == p[2])
+        return;  // this block's copy is the master
+      me[edge_at(c, p[0], p[1], p[2])] = buf[same.off[k] + edge_at(c, loc[0], loc[1], loc[2])];
+      return;
+    }
+  }
+  int c0[3], nt[3];
+  double w[3][2];
+  for (int a = 0; a < 3; ++a) {
+    const int g = bo[a] + p[a] - 2;  // level coordinate (unwrapped: the hood wraps)
+    const int m = g >> 1;
+    if (a == c) {  // cell axis
+      const double tt = (g & 1) ? 0.25 : 0.75;
+      c0[a] = (g & 1) ? m : m - 1;
+      nt[a] = 2;
+      w[a][0] = 1.0 - tt;
+      w[a][1] = tt;
+    } else if ((g & 1) == 0) {  // a coarse node
+      c0[a] = m;
+      nt[a] = 1;
+      w[a][0] = 1.0;
+      w[a][1] = 0.0;
+    } else {  // the midpoint of coarse nodes m and m + 1
+      c0[a] = m;
+      nt[a] = 2;
+      w[a][0] = 0.5;
+      w[a][1] = 0.5;
+    }
+  }
+  double acc = 0.0;
+  for (int x = 0; x < nt[0]; ++x)
+    for (int y = 0; y < nt[1]; ++y)
+      for (int z = 0; z < nt[2]; ++z) {
+        const int u[3] = {c0[0] + x - po[0], c0[1] + y - po[1], c0[2] + z - po[2]};
+        int cl[3];
+        const int k = hood_master(coarse, c, u, cl);
+        if (k < 0) continue;  // outside the level below: excluded by the nesting rule
+        const double v = buf[coarse.off[k] + edge_at(c, cl[0], cl[1], cl[2])];
+        acc = acc + ((w[0][x] * w[1][y]) * w[2][z]) * v;
+      }
+  me[edge_at(c, p[0], p[1], p[2])] = acc;
+}
+
 // The ghost edges of component C of one block (see k_fill3d): extents and shell
 // divisions are compile-time constants per component.
 template <int C>
@@ -128,52 +187,77 @@ __device__ __forceinline__ void fill_edges(double* __restrict__ buf, const Hood&
       }
       if (!needed) continue;
     }
-    if (!all) {
-      const int u[3] = {p[0] - 2, p[1] - 2, p[2] - 2};
-      int loc[3];
-      const int k = hood_master(same, c, u, loc);
-      if (k >= 0) {
-        if (k == 13 && loc[0] == p[0] && loc[1] == p[1] && loc[2] == p[2])
-          continue;  // this block's copy is the master
-        me[edge_at(c, p[0], p[1], p[2])] = buf[same.off[k] + edge_at(c, loc[0], loc[1], loc[2])];
-        continue;
+    fill_edge_at<C>(buf, same, coarse, all, bo, po, me, p);
+  }
+}
+
+// Ghost cell p of the block at `me`: trilinear from the 8 nearest cells of the level
+// below (weights 1/4, 3/4 per axis), the 5 fields.
+__device__ __forceinline__ void interp_cell(const double* __restrict__ buf, const Hood& coarse,
+                                            const int bo[3], const int po[3],
+                                            double* __restrict__ me, const int p[3]) {
+  int c0[3];
+  double w[3][2];
+  for (int a = 0; a < 3; ++a) {
+    const int g = bo[a] + p[a] - 2;
+    const int m = g >> 1;
+    const double tt = (g & 1) ? 0.25 : 0.75;
+    c0[a] = (g & 1) ? m : m - 1;
+    w[a][0] = 1.0 - tt;
+    w[a][1] = tt;
+  }
+  // the 8 coarse cells (field 0 offsets; -1: outside the level below, excluded by
+  // the nesting rule), found once for the 5 fields
+  long long src[8];
+  for (int x = 0; x < 2; ++x)
+    for (int y = 0; y < 2; ++y)
+      for (int z = 0; z < 2; ++z) {
+        const int u[3] = {c0[0] + x - po[0], c0[1] + y - po[1], c0[2] + z - po[2]};
+        const int r0 = rel_of(u[0]), r1 = rel_of(u[1]), r2 = rel_of(u[2]);
+        const int k = hood_index(r0, r1, r2);
+        src[(x * 2 + y) * 2 + z] =
+            coarse.id[k] < 0 ? -1
+                             : coarse.off[k] + cell_at(0, u[0] - r0 * B + 2, u[1] - r1 * B + 2,
+                                                       u[2] - r2 * B + 2);
       }
-    }
-    int c0[3], nt[3];
-    double w[3][2];
-    for (int a = 0; a < 3; ++a) {
-      const int g = bo[a] + p[a] - 2;  // level coordinate (unwrapped: the hood wraps)
-      const int m = g >> 1;
-      if (a == c) {  // cell axis
-        const double tt = (g & 1) ? 0.25 : 0.75;
-        c0[a] = (g & 1) ? m : m - 1;
-        nt[a] = 2;
-        w[a][0] = 1.0 - tt;
-        w[a][1] = tt;
-      } else if ((g & 1) == 0) {  // a coarse node
-        c0[a] = m;
-        nt[a] = 1;
-        w[a][0] = 1.0;
-        w[a][1] = 0.0;
-      } else {  // the midpoint of coarse nodes m and m + 1
-        c0[a] = m;
-        nt[a] = 2;
-        w[a][0] = 0.5;
-        w[a][1] = 0.5;
-      }
-    }
+  for (int f = 0; f < 5; ++f) {
     double acc = 0.0;
-    for (int x = 0; x < nt[0]; ++x)
-      for (int y = 0; y < nt[1]; ++y)
-        for (int z = 0; z < nt[2]; ++z) {
-          const int u[3] = {c0[0] + x - po[0], c0[1] + y - po[1], c0[2] + z - po[2]};
-          int cl[3];
-          const int k = hood_master(coarse, c, u, cl);
-          if (k < 0) continue;  // outside the level below: excluded by the nesting rule
-          const double v = buf[coarse.off[k] + edge_at(c, cl[0], cl[1], cl[2])];
+    for (int x = 0; x < 2; ++x)
+      for (int y = 0; y < 2; ++y)
+        for (int z = 0; z < 2; ++z) {
+          const long long o = src[(x * 2 + y) * 2 + z];
+          if (o < 0) continue;
+          const double v = buf[o + (long long)f * NCELL];
           acc = acc + ((w[0][x] * w[1][y]) * w[2][z]) * v;
         }
-    me[edge_at(c, p[0], p[1], p[2])] = acc;
+    me[cell_at(f, p[0], p[1], p[2])] = acc;
+  }
+}
+
+// The same-level and parent-level neighbourhoods of block d (threads 0..53 load them;
+// the caller synchronises).
+__device__ __forceinline__ void load_fill_hoods(Hood& same, Hood& coarse,
+                                                const BlockDesc* __restrict__ blocks,
+                                                const int* __restrict__ lookup, const Levels& lv,
+                                                const BlockDesc& d) {
+  const int L = d.level;
+  const int tid = threadIdx.x;
+  if (tid < 54) {
+    const bool up = tid >= 27;
+    const int k = up ? tid - 27 : tid;
+    const int r[3] = {k / 9 - 1, (k / 3) % 3 - 1, k % 3 - 1};
+    int id = -1;
+    if (!up || L > 0) {
+      const int lvl = up ? L - 1 : L;
+      const int nb = lv.nb[lvl];
+      const int cb[3] = {up ? d.bx >> 1 : d.bx, up ? d.by >> 1 : d.by, up ? d.bz >> 1 : d.bz};
+      const int b0 = wrap(cb[0] + r[0], nb), b1 = wrap(cb[1] + r[1], nb),
+                b2 = wrap(cb[2] + r[2], nb);
+      id = lookup[lv.base[lvl] + (b0 * nb + b1) * nb + b2];
+    }
+    Hood& h = up ? coarse : same;
+    h.id[k] = id;
+    h.off[k] = id >= 0 ? blocks[id].off : -1;
   }
 }
 
@@ -198,25 +282,8 @@ __global__ void __launch_bounds__(256) k_fill3d(double* __restrict__ buf,
   __shared__ Hood same, coarse;
   const int self = ids ? ids[blockIdx.x] : (int)blockIdx.x;
   const BlockDesc d = blocks[self];
-  const int L = d.level;
   const int tid = threadIdx.x;
-  if (tid < 54) {  // the two neighbourhoods
-    const bool up = tid >= 27;
-    const int k = up ? tid - 27 : tid;
-    const int r[3] = {k / 9 - 1, (k / 3) % 3 - 1, k % 3 - 1};
-    int id = -1;
-    if (!up || L > 0) {
-      const int lvl = up ? L - 1 : L;
-      const int nb = lv.nb[lvl];
-      const int cb[3] = {up ? d.bx >> 1 : d.bx, up ? d.by >> 1 : d.by, up ? d.bz >> 1 : d.bz};
-      const int b0 = wrap(cb[0] + r[0], nb), b1 = wrap(cb[1] + r[1], nb),
-                b2 = wrap(cb[2] + r[2], nb);
-      id = lookup[lv.base[lvl] + (b0 * nb + b1) * nb + b2];
-    }
-    Hood& h = up ? coarse : same;
-    h.id[k] = id;
-    h.off[k] = id >= 0 ? blocks[id].off : -1;
-  }
+  load_fill_hoods(same, coarse, blocks, lookup, lv, d);
   __syncthreads();
   const int bo[3] = {d.bx * B, d.by * B, d.bz * B};                    // origin, level L
   const int po[3] = {(d.bx >> 1) * B, (d.by >> 1) * B, (d.bz >> 1) * B};  // parent, L - 1
@@ -250,42 +317,7 @@ __global__ void __launch_bounds__(256) k_fill3d(double* __restrict__ buf,
         continue;
       }
     }
-    int c0[3];
-    double w[3][2];
-    for (int a = 0; a < 3; ++a) {
-      const int g = bo[a] + p[a] - 2;
-      const int m = g >> 1;
-      const double tt = (g & 1) ? 0.25 : 0.75;
-      c0[a] = (g & 1) ? m : m - 1;
-      w[a][0] = 1.0 - tt;
-      w[a][1] = tt;
-    }
-    // the 8 coarse cells (field 0 offsets; -1: outside the level below, excluded by
-    // the nesting rule), found once for the 5 fields
-    long long src[8];
-    for (int x = 0; x < 2; ++x)
-      for (int y = 0; y < 2; ++y)
-        for (int z = 0; z < 2; ++z) {
-          const int u[3] = {c0[0] + x - po[0], c0[1] + y - po[1], c0[2] + z - po[2]};
-          const int r0 = rel_of(u[0]), r1 = rel_of(u[1]), r2 = rel_of(u[2]);
-          const int k = hood_index(r0, r1, r2);
-          src[(x * 2 + y) * 2 + z] =
-              coarse.id[k] < 0 ? -1
-                               : coarse.off[k] + cell_at(0, u[0] - r0 * B + 2, u[1] - r1 * B + 2,
-                                                         u[2] - r2 * B + 2);
-        }
-    for (int f = 0; f < 5; ++f) {
-      double acc = 0.0;
-      for (int x = 0; x < 2; ++x)
-        for (int y = 0; y < 2; ++y)
-          for (int z = 0; z < 2; ++z) {
-            const long long o = src[(x * 2 + y) * 2 + z];
-            if (o < 0) continue;
-            const double v = buf[o + (long long)f * NCELL];
-            acc = acc + ((w[0][x] * w[1][y]) * w[2][z]) * v;
-          }
-      me[cell_at(f, p[0], p[1], p[2])] = acc;
-    }
+    interp_cell(buf, coarse, bo, po, me, p);
   }
   // edges of component c = part - CELL_PARTS (axis c is the cell axis, the others
   // node axes)
@@ -294,14 +326,68 @@ __global__ void __launch_bounds__(256) k_fill3d(double* __restrict__ buf,
   if (part == CELL_PARTS + 2) fill_edges<2>(buf, same, coarse, all, restricted, bo, po, me);
 }
 
+// The ghosts of listed regions of blocks whose level has no block there: a workgroup per
+// task {block, region}, region (x + 1) * 9 + (y + 1) * 3 + z + 1 of the 26 around the
+// block (per axis -1: haloed cells 0..1, 0: the block's own 2..B+1, 1: B+2..B+3).  It
+// interpolates the region's cells from the level below and sets every edge of those
+// cells outside the block's own range as k_fill3d does (the master copy on the level,
+// else interpolated), so that it leaves what k_fill3d leaves on the cells and edges the
+// stage reads from the block's ghosts.  An edge shared by two regions is written by
+// both tasks with the same value (its sources are never written by a fill).
+__global__ void __launch_bounds__(256) k_fill3d_regions(double* __restrict__ buf,
+                                                        const BlockDesc* __restrict__ blocks,
+                                                        const int* __restrict__ lookup, Levels lv,
+                                                        const int2* __restrict__ tasks) {
+  __shared__ Hood same, coarse;
+  const int2 task = tasks[blockIdx.x];
+  const BlockDesc d = blocks[task.x];
+  load_fill_hoods(same, coarse, blocks, lookup, lv, d);
+  __syncthreads();
+  const int bo[3] = {d.bx * B, d.by * B, d.bz * B};
+  const int po[3] = {(d.bx >> 1) * B, (d.by >> 1) * B, (d.bz >> 1) * B};
+  double* me = buf + d.off;
+  const int rel[3] = {task.y / 9 - 1, (task.y / 3) % 3 - 1, task.y % 3 - 1};
+  int lo[3], n[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = rel[a] < 0 ? 0 : (rel[a] == 0 ? 2 : B + 2);
+    n[a] = rel[a] == 0 ? B : 2;
+  }
+  const int ncell = n[0] * n[1] * n[2];
+  for (int t = threadIdx.x; t < ncell; t += blockDim.x) {
+    const int p[3] = {lo[0] + t / (n[1] * n[2]), lo[1] + (t / n[2]) % n[1], lo[2] + t % n[2]};
+    interp_cell(buf, coarse, bo, po, me, p);
+  }
+  // edges of component c: the cell range on axis c, the node range lo .. lo + n on the
+  // others (the nodes of the region's cells)
+  int m[3][3], ne[3];
+  for (int c = 0; c < 3; ++c) {
+    for (int a = 0; a < 3; ++a) m[c][a] = n[a] + (a == c ? 0 : 1);
+    ne[c] = m[c][0] * m[c][1] * m[c][2];
+  }
+  const int nall = ne[0] + ne[1] + ne[2];
+  for (int t = threadIdx.x; t < nall; t += blockDim.x) {
+    const int c = t < ne[0] ? 0 : (t < ne[0] + ne[1] ? 1 : 2);
+    const int r = t - (c == 0 ? 0 : (c == 1 ? ne[0] : ne[0] + ne[1]));
+    const int p[3] = {lo[0] + r / (m[c][1] * m[c][2]), lo[1] + (r / m[c][2]) % m[c][1],
+                      lo[2] + r % m[c][2]};
+    if (p[0] >= 2 && p[0] < B + 2 && p[1] >= 2 && p[1] < B + 2 && p[2] >= 2 && p[2] < B + 2)
+      continue;  // the block's own edge
+    if (c == 0)
+      fill_edge_at<0>(buf, same, coarse, false, bo, po, me, p);
+    else if (c == 1)
+      fill_edge_at<1>(buf, same, coarse, false, bo, po, me, p);
+    else
+      fill_edge_at<2>(buf, same, coarse, false, bo, po, me, p);
+  }
+}
+
 // Covered cells of a refined block := the mean of their 8 children (a workgroup per
 // refined block and child; children[9]: the block, then its children with x slowest).
-__global__ void __launch_bounds__(256) k_average3d(double* __restrict__ buf,
-                                                   const BlockDesc* __restrict__ blocks,
-                                                   const int* __restrict__ children) {
-  const int* row = children + 9 * blockIdx.x;
+__device__ __forceinline__ void average3d(double* __restrict__ buf,
+                                          const BlockDesc* __restrict__ blocks,
+                                          const int* __restrict__ row, int oct) {
   double* parent = buf + blocks[row[0]].off;
-  const int oct = blockIdx.y;  // the child (x slowest) whose 8^3 parent cells this group averages
+  // oct: the child (x slowest) whose 8^3 parent cells this group averages
   const double* kid = buf + blocks[row[1 + oct]].off;
   constexpr int H = B / 2;
   for (int t = threadIdx.x; t < H * H * H; t += blockDim.x) {
@@ -320,96 +406,73 @@ __global__ void __launch_bounds__(256) k_average3d(double* __restrict__ buf,
 
 // Coarse edges of a refined block := the mean of the two fine edges along them
 // (mhd3d.py:31-36), written to every block of the coarse level holding the edge.  A
-// workgroup per refined block, component (blockIdx.y) and part of its edges
-// (blockIdx.z).
-__global__ void __launch_bounds__(256) k_restrict3d(double* __restrict__ buf,
-                                                    const BlockDesc* __restrict__ blocks,
-                                                    const int* __restrict__ children,
-                                                    const int* __restrict__ lookup, Levels lv) {
-  const int* row = children + 9 * blockIdx.x;
-  const BlockDesc pd = blocks[row[0]];
-  const int L = pd.level;
-  const int n = lv.nb[L] * B;
-  const int bo[3] = {pd.bx * B, pd.by * B, pd.bz * B};
-  {
-    const int c = blockIdx.y, part = blockIdx.z;
-    const int e0 = c == 0 ? B : B + 1, e1 = c == 1 ? B : B + 1, e2 = c == 2 ? B : B + 1;
-    const int ne = e0 * e1 * e2, t_end = (ne * (part + 1)) / RESTRICT_PARTS;
-    for (int t = (ne * part) / RESTRICT_PARTS + threadIdx.x; t < t_end; t += blockDim.x) {
-      const int q[3] = {t / (e1 * e2), (t / e2) % e1, t % e2};  // 0..B (nodes) / B-1 (cells)
-      // the two fine edges: fine coordinates within the block's footprint (0..2B)
-      double v[2];
-      for (int s = 0; s < 2; ++s) {
-        int oct[3], loc[3];
-        for (int a = 0; a < 3; ++a) {
-          const int fcoord = 2 * q[a] + (a == c ? s : 0);
-          oct[a] = fcoord >= 2 * B ? 1 : fcoord / B;
-          loc[a] = fcoord - B * oct[a] + 2;
-        }
-        const double* kid = buf + blocks[row[1 + (oct[0] * 2 + oct[1]) * 2 + oct[2]]].off;
-        v[s] = kid[edge_at(c, loc[0], loc[1], loc[2])];
+// workgroup per refined block, component c and part of its edges; `hood`: the offsets
+// of the 27 blocks of the coarse level around the block (-1: none), `kids`: those of its
+// 8 children (x slowest), both in LDS.
+__device__ __forceinline__ void restrict3d(double* __restrict__ buf, const long long* hood,
+                                           const long long* kids, int c, int part) {
+  const int e0 = c == 0 ? B : B + 1, e1 = c == 1 ? B : B + 1, e2 = c == 2 ? B : B + 1;
+  const int ne = e0 * e1 * e2, t_end = (ne * (part + 1)) / RESTRICT_PARTS;
+  for (int t = (ne * part) / RESTRICT_PARTS + threadIdx.x; t < t_end; t += blockDim.x) {
+    const int q[3] = {t / (e1 * e2), (t / e2) % e1, t % e2};  // 0..B (nodes) / B-1 (cells)
+    // the two fine edges: fine coordinates within the block's footprint (0..2B)
+    double v[2];
+    for (int s = 0; s < 2; ++s) {
+      int oct[3], loc[3];
+      for (int a = 0; a < 3; ++a) {
+        const int fcoord = 2 * q[a] + (a == c ? s : 0);
+        oct[a] = fcoord >= 2 * B ? 1 : fcoord / B;
+        loc[a] = fcoord - B * oct[a] + 2;
       }
-      const double value = 0.5 * (v[0] + v[1]);
-      // every coarse block holding edge (bo + q): each node coordinate on a block
-      // boundary is held with local index 2 (the block above) and 2 + B (below)
-      int g[3];
-      for (int a = 0; a < 3; ++a) g[a] = wrap(bo[a] + q[a], n);
-      for (int s0 = 0; s0 < 2; ++s0)
-        for (int s1 = 0; s1 < 2; ++s1)
-          for (int s2 = 0; s2 < 2; ++s2) {
-            const int sh[3] = {s0, s1, s2};
-            int blk[3], loc[3];
-            bool ok = true;
-            for (int a = 0; a < 3; ++a) {
-              if (sh[a] == 0) {
-                blk[a] = g[a] / B;
-                loc[a] = g[a] % B + 2;
-              } else {  // the copy on the upper boundary of the block below
-                ok = ok && a != c && g[a] % B == 0;
-                blk[a] = wrap(g[a] / B - 1, lv.nb[L]);
-                loc[a] = B + 2;
-              }
-            }
-            if (!ok) continue;
-            const int b = lookup[lv.base[L] + (blk[0] * lv.nb[L] + blk[1]) * lv.nb[L] + blk[2]];
-            if (b >= 0) buf[blocks[b].off + edge_at(c, loc[0], loc[1], loc[2])] = value;
-          }
+      v[s] = buf[kids[(oct[0] * 2 + oct[1]) * 2 + oct[2]] + edge_at(c, loc[0], loc[1], loc[2])];
     }
+    const double value = 0.5 * (v[0] + v[1]);
+    // every coarse block holding edge q of the block: the block containing it (q = B
+    // on a node axis: the next block, local index 2) and, for a node coordinate on a
+    // block boundary, the block below it (local index 2 + B)
+    for (int s0 = 0; s0 < 2; ++s0)
+      for (int s1 = 0; s1 < 2; ++s1)
+        for (int s2 = 0; s2 < 2; ++s2) {
+          const int sh[3] = {s0, s1, s2};
+          int rel[3], loc[3];
+          bool ok = true;
+          for (int a = 0; a < 3; ++a) {
+            if (sh[a] == 0) {
+              rel[a] = q[a] / B;
+              loc[a] = q[a] % B + 2;
+            } else {  // the copy on the upper boundary of the block below
+              ok = ok && a != c && q[a] % B == 0;
+              rel[a] = q[a] / B - 1;
+              loc[a] = B + 2;
+            }
+          }
+          if (!ok) continue;
+          const long long off = hood[(rel[0] + 1) * 9 + (rel[1] + 1) * 3 + rel[2] + 1];
+          if (off >= 0) buf[off + edge_at(c, loc[0], loc[1], loc[2])] = value;
+        }
   }
 }
 
-// The CFL rate (mhd3d.py:674-724) over the cells of every block: a fixed grid walks
-// chunks of 256 cells (16 per block), one atomic pair per workgroup.
-__global__ void __launch_bounds__(256) k_rate3d_blocks(const double* __restrict__ buf,
-                                                       const BlockDesc* __restrict__ blocks,
-                                                       int nblocks, double b0x, double b0y,
-                                                       double b0z, double gamma,
-                                                       unsigned long long* keys) {
-  double rate = 0.0, rmin = __builtin_inf();
-  const int nchunks = nblocks * (B * B * B / 256);
-  for (int ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
-    const BlockDesc d = blocks[ch / (B * B * B / 256)];
-    const int t = (ch % (B * B * B / 256)) * 256 + (int)threadIdx.x;
-    const int i = t / (B * B) + 2, j = (t / B) % B + 2, k = t % B + 2;
-    const double* in = buf + d.off;
-    const EdgePot A{in + OFF_A, in + OFF_A + NEDGE, in + OFF_A + 2 * NEDGE, X, X};
-    const double dx = d.dx, dy = d.dy, dz = d.dz;
-    const double bx = 0.5 * (A.bx(i, j, k, dy, dz, b0x) + A.bx(i + 1, j, k, dy, dz, b0x));
-    const double by = 0.5 * (A.by(i, j, k, dz, dx, b0y) + A.by(i, j + 1, k, dz, dx, b0y));
-    const double bz = 0.5 * (A.bz(i, j, k, dx, dy, b0z) + A.bz(i, j, k + 1, dx, dy, b0z));
-    const int c = cell_at(0, i, j, k);
-    const double r = in[c];
-    const double u = in[c + NCELL] / r, v = in[c + 2 * NCELL] / r, w = in[c + 3 * NCELL] / r;
-    const double p = (gamma - 1.0) * (in[c + 4 * NCELL] - 0.5 * r * (u * u + v * v + w * w) -
-                                      0.5 * (bx * bx + by * by + bz * bz));
-    const double rc = (__builtin_fabs(u) + fast_speed_rate(r, p, bx, by, bz, gamma)) / dx +
-                      (__builtin_fabs(v) + fast_speed_rate(r, p, by, bz, bx, gamma)) / dy +
-                      (__builtin_fabs(w) + fast_speed_rate(r, p, bz, bx, by, gamma)) / dz;
-    if (rc > rate) rate = rc;
-    if (r < rmin) rmin = r;
+// One level pair's coarsening in one launch: per refined block (blockIdx.x), 8
+// workgroups average its cells (one per child), then 3 x RESTRICT_PARTS restrict its
+// edges; the two write disjoint data (cells, edges) from disjoint data.
+__global__ void __launch_bounds__(256) k_coarsen3d(double* __restrict__ buf,
+                                                   const BlockDesc* __restrict__ blocks,
+                                                   const int* __restrict__ children,
+                                                   const int* __restrict__ lookup, Levels lv) {
+  __shared__ long long hood[27], kids[8];
+  const int* row = children + 9 * blockIdx.x;
+  const int y = blockIdx.y;
+  if (y < 8) {
+    average3d(buf, blocks, row, y);
+    return;
   }
-  block_reduce_rate_rho_once(rate, rmin, keys);
+  load_hood(hood, blocks, lookup, lv, blocks[row[0]]);
+  if (threadIdx.x >= 32 && threadIdx.x < 40) kids[threadIdx.x - 32] = blocks[row[threadIdx.x - 31]].off;
+  __syncthreads();
+  restrict3d(buf, hood, kids, (y - 8) / RESTRICT_PARTS, (y - 8) % RESTRICT_PARTS);
 }
+
 
 
 }  // namespace
@@ -472,15 +535,23 @@ int wb_amr3d_prolong(double* buf, const void* blocks, const int32_t* ids, int ni
   return wb_last_error();
 }
 
+int wb_amr3d_fill_regions(double* buf, const void* blocks, const int32_t* tasks, int ntasks,
+                          const int32_t* lookup, const void* levels, void* stream) {
+  const Levels lv = *(const Levels*)levels;
+  if (ntasks < 0 || !levels_ok(lv)) return WB_EINVAL;
+  if (ntasks == 0) return 0;
+  hipLaunchKernelGGL(k_fill3d_regions, dim3((unsigned)ntasks), dim3(256), 0, (hipStream_t)stream,
+                     buf, (const BlockDesc*)blocks, (const int*)lookup, lv, (const int2*)tasks);
+  return wb_last_error();
+}
+
 int wb_amr3d_coarsen(double* buf, const void* blocks, const int32_t* children, int nrefined,
                      const int32_t* lookup, const void* levels, void* stream) {
   const Levels lv = *(const Levels*)levels;
   if (nrefined < 0 || !levels_ok(lv)) return WB_EINVAL;
   if (nrefined == 0) return 0;
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_average3d, dim3(nrefined, 8), dim3(256), 0, s, buf,
-                     (const BlockDesc*)blocks, (const int*)children);
-  hipLaunchKernelGGL(k_restrict3d, dim3(nrefined, 3, RESTRICT_PARTS), dim3(256), 0, s, buf,
+  hipLaunchKernelGGL(k_coarsen3d, dim3(nrefined, 8 + 3 * RESTRICT_PARTS), dim3(256), 0, s, buf,
                      (const BlockDesc*)blocks, (const int*)children, (const int*)lookup, lv);
   return wb_last_error();
 }
@@ -489,11 +560,13 @@ int wb_amr3d_rate(const double* buf, const void* blocks, int nblocks, double b0x
                   double b0z, double gamma, double* out, void* stream) {
   if (nblocks < 1) return WB_EINVAL;
   hipStream_t s = (hipStream_t)stream;
+  StageArgs a;
+  wb_amr3d_base_args(&a, buf, const_cast<double*>(buf), b0x, b0y, b0z, gamma, 0, nullptr, 0,
+                     nullptr);
   unsigned long long* keys = (unsigned long long*)out;
   hipLaunchKernelGGL(k_rate_init, dim3(1), dim3(64), 0, s, keys);
-  const int nchunks = nblocks * (B * B * B / 256);
-  hipLaunchKernelGGL(k_rate3d_blocks, dim3((unsigned)(nchunks < 1024 ? nchunks : 1024)), dim3(256),
-                     0, s, buf, (const BlockDesc*)blocks, nblocks, b0x, b0y, b0z, gamma, keys);
+  hipLaunchKernelGGL(k_rate3d_blocks<false>, dim3((unsigned)nblocks), dim3(256), 0, s, a,
+                     (const BlockDesc*)blocks, keys);
   hipLaunchKernelGGL(k_rate_finish, dim3(1), dim3(64), 0, s, keys, out);
   return wb_last_error();
 }

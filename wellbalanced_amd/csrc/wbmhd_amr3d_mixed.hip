@@ -65,4 +65,21 @@ int wb_amr3d_reflux_mixed(const double* src, double* dst, const void* blocks, co
   });
 }
 
+// The mixed path's CFL rate over every block: the column march of the uniform grid's
+// mixed rate (one division per cell, reciprocals), in this FP32-contracted unit.
+int wb_amr3d_rate_mixed(const double* buf, const void* blocks, int nblocks, double b0x,
+                        double b0y, double b0z, double gamma, double* out, void* stream) {
+  if (nblocks < 1) return WB_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  StageArgs a;
+  wb_amr3d_base_args(&a, buf, const_cast<double*>(buf), b0x, b0y, b0z, gamma, 0, nullptr, 0,
+                     nullptr);
+  unsigned long long* keys = (unsigned long long*)out;
+  hipLaunchKernelGGL(k_rate_init, dim3(1), dim3(64), 0, s, keys);
+  hipLaunchKernelGGL(k_rate3d_blocks<true>, dim3((unsigned)nblocks), dim3(256), 0, s, a,
+                     (const BlockDesc*)blocks, keys);
+  hipLaunchKernelGGL(k_rate_finish, dim3(1), dim3(64), 0, s, keys, out);
+  return wb_last_error();
+}
+
 }  // extern "C"
