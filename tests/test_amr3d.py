@@ -1,6 +1,8 @@
 """The refined 3D hierarchy of equal blocks: the oracle's invariants (CPU) and the
 GPU path against the oracle (FP64: bit for bit)."""
 
+import itertools
+
 import numpy as np
 import pytest
 
@@ -58,6 +60,109 @@ def _three_levels():
     return h
 
 
+def _l_shape():
+    """Three of the eight base blocks refined: coarse cells in the concave corner have
+    coarse-fine faces in two directions."""
+    h = Hierarchy3D(32)
+    h.refine(0, [(0, 0, 0), (1, 0, 0), (0, 1, 0)])
+    return h
+
+
+def _hierarchy(kind):
+    return {2: _two_levels, 3: _three_levels, "L": _l_shape}[kind]()
+
+
+# -- host tables -----------------------------------------------------------------------------
+@pytest.mark.parametrize("kind", [2, 3, "L"])
+def test_reflux_cells_list_every_face_once_in_direction_order(kind):
+    t = amr3d_maps.build(_hierarchy(kind))
+    items, cells = t.reflux_items, t.reflux_cells
+    assert [int((items["pad"] == d).sum()) for d in range(6)] == [len(r) for r in t.reflux]
+    seen = set()
+    for c in cells:
+        n = int(c["n"])
+        assert 1 <= n <= 3 and (c["item"][n:] == -1).all()
+        dirs = [int(items["pad"][c["item"][f]]) for f in range(n)]
+        assert dirs == sorted(set(dirs))
+        # the faces of one cell name the same coarse cell
+        where = set()
+        for f in range(n):
+            it = items[c["item"][f]]
+            axis, lane = int(it["pad"]) // 2, int(c["lane"][f])
+            q = [int(it["ci"]), int(it["cj"]), int(it["ck"])]
+            q[1 if axis == 0 else 0] += lane // 8
+            q[1 if axis == 2 else 2] += lane % 8
+            where.add((int(it["coarse"]), *q))
+            seen.add((int(c["item"][f]), lane))
+        assert len(where) == 1
+    assert len(seen) == 64 * len(items) == int(cells["n"].sum())
+    assert (np.diff(cells["n"]) <= 0).all()
+    if kind == "L":
+        assert (cells["n"] == 2).any()
+
+
+def _shell_fill_sets(ids, b=16):
+    """The ghost cells and (component, edge) positions the restricted shell walk of the
+    fill sets for a block whose 27 same-level neighbours are `ids` (-1: none)."""
+    X, XN = b + 4, b + 5
+    rel = lambda u: -1 if u < 0 else (1 if u >= b else 0)
+    region = lambda q: ((rel(q[0] - 2) + 1) * 3 + rel(q[1] - 2) + 1) * 3 + rel(q[2] - 2) + 1
+    own = lambda q, lim: all(2 <= q[a] < lim for a in range(3))
+    cells = {p for p in itertools.product(range(X), repeat=3)
+             if not own(p, b + 2) and ids[region(p)] < 0}
+    edges = set()
+    for c in range(3):
+        ext = [X if a == c else XN for a in range(3)]
+        for p in itertools.product(*(range(e) for e in ext)):
+            if own(p, b + 2):
+                continue
+            for s in range(4):
+                q, bit = list(p), 0
+                for a in range(3):
+                    if a != c:
+                        q[a] -= (s >> bit) & 1
+                        bit += 1
+                if min(q) < 0 or max(q) >= X or own(q, b + 2):
+                    continue
+                if ids[region(q)] < 0:
+                    edges.add((c, p))
+                    break
+    return cells, edges
+
+
+def _region_fill_sets(ids, b=16):
+    """The same for the (block, region) tasks: each missing region's cells and the
+    edges of those cells outside the block's own range."""
+    cells, edges = set(), set()
+    for r in range(27):
+        if r == 13 or ids[r] >= 0:
+            continue
+        rel = (r // 9 - 1, (r // 3) % 3 - 1, r % 3 - 1)
+        lo = [0 if x < 0 else (2 if x == 0 else b + 2) for x in rel]
+        n = [b if x == 0 else 2 for x in rel]
+        cells |= set(itertools.product(*(range(lo[a], lo[a] + n[a]) for a in range(3))))
+        for c in range(3):
+            box = (range(lo[a], lo[a] + n[a] + (a != c)) for a in range(3))
+            edges |= {(c, p) for p in itertools.product(*box)
+                      if not all(2 <= p[a] < b + 2 for a in range(3))}
+    return cells, edges
+
+
+def test_fill_regions_set_what_the_restricted_shell_fill_sets():
+    """The per-region fill tasks cover exactly the ghost cells and edges the shell walk
+    of the restricted fill sets (same per-element code, so the same values)."""
+    t = amr3d_maps.build(_l_shape())
+    missing = np.flatnonzero(t.missing)
+    assert len(missing) and set(t.fill_regions[:, 0]) == set(missing)
+    rng = np.random.default_rng(3)
+    patterns = [t.hoods[k] for k in missing[:: max(1, len(missing) // 3)]]
+    patterns += [np.where(rng.random(27) < 0.4, -1, 1) for _ in range(2)]
+    for ids in patterns:
+        ids = np.array(ids)
+        ids[13] = 1
+        assert _shell_fill_sets(ids) == _region_fill_sets(ids)
+
+
 def test_nesting_adds_the_parent_neighbourhood():
     h = _three_levels()
     assert len(h.levels) == 3 and len(h.levels[2]) == 8
@@ -96,7 +201,7 @@ def test_a_refined_level_covering_the_domain_is_the_uniform_run():
 
 @pytest.mark.parametrize("levels", [2, 3])
 def test_refined_blocks_conserve_and_keep_div_b_zero(levels):
-    h = _two_levels() if levels == 2 else _three_levels()
+    h = _hierarchy(levels)
     t, amr = _oracle(h, _blast_state())
     before = amr.leaf_totals()
     for _ in range(2):
@@ -115,14 +220,14 @@ def test_refined_blocks_conserve_and_keep_div_b_zero(levels):
 
 # -- GPU ----------------------------------------------------------------------------------
 @pytest.mark.gpu
-@pytest.mark.parametrize("levels,direct", [(2, True), (2, False), (3, True)])
+@pytest.mark.parametrize("levels,direct", [(2, True), (2, False), (3, True), ("L", True)])
 def test_gpu_hierarchy_3d_is_bitwise_the_oracle(gpu, levels, direct):
     """Bit for bit the oracle, whose fill copies every ghost: with direct reads the
     stage and the refluxing read the cells around a block in place from the blocks of
     its level, and only the ghosts the level has no block for are filled."""
     from wellbalanced_amd.amr3d import AMRMHD3D
 
-    h = _two_levels() if levels == 2 else _three_levels()
+    h = _hierarchy(levels)
     t, amr = _oracle(h, _blast_state())
     model = AMRMHD3D(h, mean_field=B0, limiter="minmod")
     model.direct = direct
@@ -142,6 +247,33 @@ def test_gpu_hierarchy_3d_is_bitwise_the_oracle(gpu, levels, direct):
         for c in range(3):
             sl = tuple(slice(2, 18 + (a != c)) for a in range(3))
             np.testing.assert_array_equal(gp[c][sl], wp[c][sl])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", [3, "L"])
+def test_gpu_region_fill_is_the_restricted_shell_fill(gpu, kind):
+    """wb_amr3d_fill_regions and the restricted shell walk of wb_amr3d_fill_list leave
+    the same buffer, bit for bit."""
+    import torch
+
+    from wellbalanced_amd import _lib
+    from wellbalanced_amd.amr3d import AMRMHD3D
+
+    model = AMRMHD3D(_hierarchy(kind), mean_field=B0, limiter="minmod")
+    model.set_state(**_blast_state())
+    model.advance(1)
+    stream = _lib.stream_handle(model.device)
+    a = model._buf[0].clone()
+    b = model._buf[0].clone()
+    _lib.call("wb_amr3d_fill_regions", _lib.ptr(a), _lib.ptr(model._blocks),
+              _lib.ptr(model._regions), model.nregions, _lib.ptr(model._lookup),
+              _lib.ptr(model._levels), stream)
+    _lib.call("wb_amr3d_fill_list", _lib.ptr(b), _lib.ptr(model._blocks),
+              _lib.ptr(model._missing), len(model._missing), 1, _lib.ptr(model._lookup),
+              _lib.ptr(model._levels), stream)
+    torch.cuda.synchronize()
+    assert model.nregions > 0
+    assert torch.equal(a, b)
 
 
 @pytest.mark.gpu
