@@ -584,14 +584,14 @@ int wb_amr3d_stage(const double* in, double* out, const void* blocks, const int3
   }
   StageArgs a;
   wb_amr3d_base_args(&a, in, out, b0x, b0y, b0z, gamma, limiter, dt, heun, bad);
-  const dim3 grid(nids), tile(Tile<B, B>::THREADS);
+  const dim3 grid(((nids + 7) / 8) * 8), tile(Tile<B, B>::THREADS);
   hipStream_t s = (hipStream_t)stream;
   if (direct)
     hipLaunchKernelGGL((k_stage3d_blocks<double, 1, -1, true>), grid, tile, 0, s, a,
-                       (const BlockDesc*)blocks, (const int*)ids, (const int*)lookup, lv);
+                       (const BlockDesc*)blocks, (const int*)ids, nids, (const int*)lookup, lv);
   else
     hipLaunchKernelGGL((k_stage3d_blocks<double, 1, -1, false>), grid, tile, 0, s, a,
-                       (const BlockDesc*)blocks, (const int*)ids, (const int*)lookup, lv);
+                       (const BlockDesc*)blocks, (const int*)ids, nids, (const int*)lookup, lv);
   return wb_last_error();
 }
 

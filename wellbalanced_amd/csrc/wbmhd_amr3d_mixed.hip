@@ -30,7 +30,7 @@ int wb_amr3d_stage_mixed(const double* in, double* out, const void* blocks, cons
   return with_limiter(limiter, [&](auto lim) {
     constexpr int L = decltype(lim)::value;
     const dim3 halves(((nids + 7) / 8) * 16), tile_h(Tile<B / 2, B>::THREADS);
-    const dim3 tile_b(Tile<B, B>::THREADS);
+    const dim3 tile_b(Tile<B, B>::THREADS), blocks8(((nids + 7) / 8) * 8);
     if (waves == 3 && direct)
       hipLaunchKernelGGL((k_stage3d_halves<float, 3, L, true>), halves, tile_h, 0, s, a,
                          (const BlockDesc*)blocks, id, nids, lk, lv);
@@ -38,11 +38,11 @@ int wb_amr3d_stage_mixed(const double* in, double* out, const void* blocks, cons
       hipLaunchKernelGGL((k_stage3d_halves<float, 3, L, false>), halves, tile_h, 0, s, a,
                          (const BlockDesc*)blocks, id, nids, lk, lv);
     else if (direct)
-      hipLaunchKernelGGL((k_stage3d_blocks<float, 2, L, true>), dim3(nids), tile_b, 0, s, a,
-                         (const BlockDesc*)blocks, id, lk, lv);
+      hipLaunchKernelGGL((k_stage3d_blocks<float, 2, L, true>), blocks8, tile_b, 0, s, a,
+                         (const BlockDesc*)blocks, id, nids, lk, lv);
     else
-      hipLaunchKernelGGL((k_stage3d_blocks<float, 2, L, false>), dim3(nids), tile_b, 0, s, a,
-                         (const BlockDesc*)blocks, id, lk, lv);
+      hipLaunchKernelGGL((k_stage3d_blocks<float, 2, L, false>), blocks8, tile_b, 0, s, a,
+                         (const BlockDesc*)blocks, id, nids, lk, lv);
     return wb_last_error();
   });
 }
