@@ -208,11 +208,12 @@ __device__ void face_flux3d(const StageArgs& a, int axis, int i, int j, int k, d
 // coarse flux of a cell's face, waves 1-4 the four fine fluxes (p, q) = ((w - 1) / 2,
 // (w - 1) % 2), and wave 0 combines them through LDS.  `cells` lists, per cell, its n
 // (1 to 3) faces as (item, lane) pairs in increasing direction order, the lane u * 8 + v
-// naming the cell inside the item's 8 x 8 face, and the cells with more faces first (the
-// first cell of a workgroup has its largest n).  Wave 0 applies the corrections of a
-// cell's faces in that order, which is the order of one pass per direction, without
-// atomics (deterministic).  `hoods`: per block the 27 offsets of the blocks of its level
-// around it (-1: none) for reads in place, or null to read the filled ghosts.
+// naming the cell inside the item's 8 x 8 face (the host puts the cells with more faces
+// first, so that a workgroup's cells share one face count).  Wave 0 applies the
+// corrections of a cell's faces in that order, which is the order of one pass per
+// direction, with no atomics on the state (deterministic).  `hoods`: per block the 27
+// offsets of the blocks of its level around it (-1: none) for reads in place, or null to
+// read the filled ghosts.
 struct RefluxCell {
   int n, item[3], lane[3], pad;
 };
@@ -224,10 +225,15 @@ __global__ void __launch_bounds__(320) k_reflux3d(StageArgs base, const BlockDes
                                                   const long long* __restrict__ hoods,
                                                   double* __restrict__ dst) {
   __shared__ double fine[4][7][64];
+  __shared__ int most;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int t = blockIdx.x * 64 + lane;
-  const int nmax = cells[blockIdx.x * 64].n;
   const int n = t < ncells ? cells[t].n : 0;
+  if (threadIdx.x == 0) most = 0;
+  __syncthreads();
+  if (w == 0 && n > 0) atomicMax(&most, n);
+  __syncthreads();
+  const int nmax = most;
   const double dt = *base.dt;
   for (int f = 0; f < nmax; ++f) {
     const bool has = f < n;
