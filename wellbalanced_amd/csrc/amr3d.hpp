@@ -204,34 +204,38 @@ __device__ void face_flux3d(const StageArgs& a, int axis, int i, int j, int k, d
 // stage's average).  An item is one such face (dir = 2 axis + side; side 0: the coarse
 // cells at the fine block's low face, their +axis face; side 1: at its high face).
 //
-// A workgroup per 64 coarse cells that have such faces, five waves: wave 0 computes the
-// coarse flux of a cell's face, waves 1-4 the four fine fluxes (p, q) = ((w - 1) / 2,
-// (w - 1) % 2), and wave 0 combines them through LDS.  `cells` lists, per cell, its n
+// A workgroup of 256 threads per REFLUX_CELLS = 51 coarse cells that have such faces:
+// thread 5 m + k takes flux k of cell m, k = 0 the coarse flux of the cell's face and
+// k = 1..4 the four fine fluxes (p, q) = ((k - 1) / 2, (k - 1) % 2); the fine ones meet
+// the coarse one's thread through LDS (four waves, so that two workgroups fill the 8
+// waves a CU holds at this kernel's register count).  `cells` lists, per cell, its n
 // (1 to 3) faces as (item, lane) pairs in increasing direction order, the lane u * 8 + v
 // naming the cell inside the item's 8 x 8 face (the host puts the cells with more faces
-// first, so that a workgroup's cells share one face count).  Wave 0 applies the
-// corrections of a cell's faces in that order, which is the order of one pass per
-// direction, with no atomics on the state (deterministic).  `hoods`: per block the 27
-// offsets of the blocks of its level around it (-1: none) for reads in place, or null to
-// read the filled ghosts.
+// first, so that a workgroup's cells share one face count).  The coarse flux's thread
+// applies the corrections of a cell's faces in that order, which is the order of one
+// pass per direction, with no atomics on the state (deterministic).  `hoods`: per block
+// the 27 offsets of the blocks of its level around it (-1: none) for reads in place, or
+// null to read the filled ghosts.
+constexpr int REFLUX_CELLS = 51;
+
 struct RefluxCell {
   int n, item[3], lane[3], pad;
 };
 
 template <typename P, int LIM>
-__global__ void __launch_bounds__(320) k_reflux3d(StageArgs base, const BlockDesc* __restrict__ blocks,
+__global__ void __launch_bounds__(256) k_reflux3d(StageArgs base, const BlockDesc* __restrict__ blocks,
                                                   const RefluxItem* __restrict__ items,
                                                   const RefluxCell* __restrict__ cells, int ncells,
                                                   const long long* __restrict__ hoods,
                                                   double* __restrict__ dst) {
-  __shared__ double fine[4][7][64];
+  __shared__ double fine[4][7][REFLUX_CELLS];
   __shared__ int most;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int t = blockIdx.x * 64 + lane;
-  const int n = t < ncells ? cells[t].n : 0;
+  const int m = threadIdx.x / 5, kf = threadIdx.x % 5;
+  const int t = blockIdx.x * REFLUX_CELLS + m;
+  const int n = m < REFLUX_CELLS && t < ncells ? cells[t].n : 0;
   if (threadIdx.x == 0) most = 0;
   __syncthreads();
-  if (w == 0 && n > 0) atomicMax(&most, n);
+  if (kf == 0 && n > 0) atomicMax(&most, n);
   __syncthreads();
   const int nmax = most;
   const double dt = *base.dt;
@@ -254,26 +258,26 @@ __global__ void __launch_bounds__(320) k_reflux3d(StageArgs base, const BlockDes
       c[a1] += u;
       c[a2] += v;
       int g[3];
-      if (w == 0) {
+      if (kf == 0) {
         g[0] = c[0];
         g[1] = c[1];
         g[2] = c[2];
         if (side == 0) g[axis] += 1;  // its +axis face is the low face of the next cell
       } else {  // fine face (p, q): the fine block's low face (side 0) or high face
         g[axis] = side == 0 ? 2 : 2 + B;
-        g[a1] = 2 + 2 * u + ((w - 1) >> 1);
-        g[a2] = 2 + 2 * v + ((w - 1) & 1);
+        g[a1] = 2 + 2 * u + ((kf - 1) >> 1);
+        g[a2] = 2 + 2 * v + ((kf - 1) & 1);
       }
-      const int blk = w == 0 ? item.coarse : item.fine;
+      const int blk = kf == 0 ? item.coarse : item.fine;
       const BlockDesc bd = blocks[blk];
       cd = bd;
       face_flux3d<P, LIM>(block_args(base, bd), axis, g[0], g[1], g[2], fl,
                           hoods ? hoods + 27 * (size_t)blk : nullptr, base.in);
-      if (w > 0)
-        for (int r = 0; r < 7; ++r) fine[w - 1][r][lane] = fl[r];
+      if (kf > 0)
+        for (int r = 0; r < 7; ++r) fine[kf - 1][r][m] = fl[r];
     }
     __syncthreads();
-    if (has && w == 0) {
+    if (has && kf == 0) {
       const double dtd = axis == 0 ? dt / cd.dx : (axis == 1 ? dt / cd.dy : dt / cd.dz);
       constexpr int xrow[5] = {0, 1, 2, 3, 4};
       constexpr int yrow[5] = {0, 3, 1, 2, 4};
@@ -282,7 +286,7 @@ __global__ void __launch_bounds__(320) k_reflux3d(StageArgs base, const BlockDes
       for (int fld = 0; fld < 5; ++fld) {
         const int row = axis == 0 ? xrow[fld] : (axis == 1 ? yrow[fld] : zrow[fld]);
         const double fm =
-            ((fine[0][row][lane] + fine[1][row][lane]) + (fine[2][row][lane] + fine[3][row][lane])) * 0.25;
+            ((fine[0][row][m] + fine[1][row][m]) + (fine[2][row][m] + fine[3][row][m])) * 0.25;
         const double d = side == 0 ? dtd * (fl[row] - fm) : dtd * (fm - fl[row]);
         const int o = cell_at(fld, c[0], c[1], c[2]);
         out[o] = out[o] + (base.heun ? 0.5 * d : d);

@@ -603,7 +603,7 @@ int wb_amr3d_reflux(const double* src, double* dst, const void* blocks, const vo
   if (ncells == 0) return 0;
   StageArgs a;
   wb_amr3d_base_args(&a, src, dst, b0x, b0y, b0z, gamma, limiter, dt, heun, nullptr);
-  hipLaunchKernelGGL((k_reflux3d<double, -1>), dim3((unsigned)((ncells + 63) / 64)), dim3(320), 0,
+  hipLaunchKernelGGL((k_reflux3d<double, -1>), dim3((unsigned)((ncells + REFLUX_CELLS - 1) / REFLUX_CELLS)), dim3(256), 0,
                      (hipStream_t)stream, a, (const BlockDesc*)blocks, (const RefluxItem*)items,
                      (const RefluxCell*)cells, ncells, hoods, dst);
   return wb_last_error();

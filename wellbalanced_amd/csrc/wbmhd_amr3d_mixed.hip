@@ -58,7 +58,7 @@ int wb_amr3d_reflux_mixed(const double* src, double* dst, const void* blocks, co
   hipStream_t s = (hipStream_t)stream;
   return with_limiter(limiter, [&](auto lim) {
     constexpr int L = decltype(lim)::value;
-    hipLaunchKernelGGL((k_reflux3d<float, L>), dim3((unsigned)((ncells + 63) / 64)), dim3(320), 0, s,
+    hipLaunchKernelGGL((k_reflux3d<float, L>), dim3((unsigned)((ncells + REFLUX_CELLS - 1) / REFLUX_CELLS)), dim3(256), 0, s,
                        a, (const BlockDesc*)blocks, (const RefluxItem*)items,
                        (const RefluxCell*)cells, ncells, hoods, dst);
     return wb_last_error();
