@@ -426,12 +426,15 @@ int wb_amr3d_prolong(double* buf, const void* blocks, const int32_t* ids, int ni
 
 /* The ghosts of listed regions a block's level has no block for (the only ghosts the
  * stage and the refluxing read when they read the cells around a block in place):
- * tasks = ntasks pairs {int32 block, region}, region (x + 1) * 9 + (y + 1) * 3 + z + 1
- * of the 26 around the block.  Its cells interpolated from the level below, the edges
- * of those cells outside the block's own range from their master copy on the level or
- * interpolated, as wb_amr3d_fill. */
-int wb_amr3d_fill_regions(double* buf, const void* blocks, const int32_t* tasks, int ntasks,
-                          const int32_t* lookup, const void* levels, void* stream);
+ * tasks = ntasks rows {int32 block, region + 32 part, parent block, 0}, region (x + 1) *
+ * 9 + (y + 1) * 3 + z + 1 of the 26 around the block.  Its cells interpolated from the
+ * level below, the edges of those cells outside the block's own range from their master
+ * copy on the level or interpolated, as wb_amr3d_fill: part 0 all of it, part 1 the
+ * cells, part 2 + c the edges of component c.  hoods: per block the 27 offsets (int64
+ * doubles, -1: none) of the blocks of its level around it, index (x + 1) * 9 + (y + 1)
+ * * 3 + z + 1. */
+int wb_amr3d_fill_regions(double* buf, const void* blocks, const long long* hoods,
+                          const int32_t* tasks, int ntasks, void* stream);
 
 /* One level's coarsening: the covered cells of each refined block := the mean of its
  * 8 children, and its edges := the mean of the two fine edges along them (every coarse

@@ -154,6 +154,15 @@ def test_fill_regions_set_what_the_restricted_shell_fill_sets():
     t = amr3d_maps.build(_l_shape())
     missing = np.flatnonzero(t.missing)
     assert len(missing) and set(t.fill_regions[:, 0]) == set(missing)
+    # every missing region once: whole (part 0) or as its four parts; the parent named
+    assert (t.fill_regions[:, 2] >= 0).all()
+    for k in missing:
+        got = t.fill_regions[t.fill_regions[:, 0] == k, 1]
+        regions = sorted(set(int(v) & 31 for v in got))
+        assert regions == sorted(int(r) for r in np.flatnonzero(t.hoods[k] == -1))
+        for r in regions:
+            parts = sorted(int(v) >> 5 for v in got if int(v) & 31 == r)
+            assert parts in ([0], [1, 2, 3, 4])
     rng = np.random.default_rng(3)
     patterns = [t.hoods[k] for k in missing[:: max(1, len(missing) // 3)]]
     patterns += [np.where(rng.random(27) < 0.4, -1, 1) for _ in range(2)]
@@ -266,8 +275,7 @@ def test_gpu_region_fill_is_the_restricted_shell_fill(gpu, kind):
     a = model._buf[0].clone()
     b = model._buf[0].clone()
     _lib.call("wb_amr3d_fill_regions", _lib.ptr(a), _lib.ptr(model._blocks),
-              _lib.ptr(model._regions), model.nregions, _lib.ptr(model._lookup),
-              _lib.ptr(model._levels), stream)
+              _lib.ptr(model._hoods), _lib.ptr(model._regions), model.nregions, stream)
     _lib.call("wb_amr3d_fill_list", _lib.ptr(b), _lib.ptr(model._blocks),
               _lib.ptr(model._missing), len(model._missing), 1, _lib.ptr(model._lookup),
               _lib.ptr(model._levels), stream)

@@ -56,7 +56,7 @@ SIGNATURES: dict[str, tuple[type, list]] = {
     "wb_current_tags3d": (I, [P, I64, I64, I64, D, D, D, D, P, P]),
     "wb_amr3d_fill": (I, [P, P, I, P, P, P]),
     "wb_amr3d_fill_list": (I, [P, P, P, I, I, P, P, P]),
-    "wb_amr3d_fill_regions": (I, [P, P, P, I, P, P, P]),
+    "wb_amr3d_fill_regions": (I, [P, P, P, P, I, P]),
     "wb_amr3d_coarsen": (I, [P, P, P, I, P, P, P]),
     "wb_amr3d_prolong": (I, [P, P, P, I, P, P, P]),
     "wb_amr3d_rate": (I, [P, P, I, D, D, D, D, P, P]),

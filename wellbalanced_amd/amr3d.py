@@ -203,8 +203,7 @@ class AMRMHD3D:
             self._fill(src, stream)
         elif self.nregions:
             _lib.call("wb_amr3d_fill_regions", _lib.ptr(src), _lib.ptr(self._blocks),
-                      _lib.ptr(self._regions), self.nregions, _lib.ptr(self._lookup),
-                      _lib.ptr(self._levels), stream)
+                      _lib.ptr(self._hoods), _lib.ptr(self._regions), self.nregions, stream)
         suffix = "" if self.precision == "fp64" else "_mixed"
         code = LIMITER_CODES.index(self.limiter)
         if marks is not None:

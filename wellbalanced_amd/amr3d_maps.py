@@ -159,7 +159,7 @@ class Tables3D:
     hoods: np.ndarray = None    # int64 (nblocks, 27): offsets of the blocks of its level around
     reflux_items: np.ndarray = None  # REFLUX_DTYPE, every direction, pad = 2 axis + side
     reflux_cells: np.ndarray = None  # REFLUX_CELL_DTYPE per refluxed coarse cell
-    fill_regions: np.ndarray = None  # int32 (n, 2): block, ghost region its level has no block for
+    fill_regions: np.ndarray = None  # int32 (n, 4): block, region + 32 part, parent, 0
 
 
 def build(h: Hierarchy3D) -> Tables3D:
@@ -232,20 +232,32 @@ def build(h: Hierarchy3D) -> Tables3D:
             if j is not None:
                 hoods[k, r] = blocks["off"][j]
     items, cells = reflux_cells(reflux, h.block)
+    parent = np.full(len(rows), -1, np.int32)
+    for (L, *bc), k in index.items():
+        if L > 0:
+            parent[k] = index[(L - 1, *(v // 2 for v in bc))]
     return Tables3D(blocks, index, lookup, level_nb, level_base, children, reflux,
                     len(rows) * size, size, missing, hoods, items, cells,
-                    fill_regions(hoods))
+                    fill_regions(hoods, parent))
 
 
-def fill_regions(hoods: np.ndarray) -> np.ndarray:
-    """The (block, region) pairs of the 26 ghost regions around each block (region
-    (x + 1) * 9 + (y + 1) * 3 + z + 1) that its level has no block for, the largest
-    regions (faces, then edges, then corners) first."""
+def fill_regions(hoods: np.ndarray, parent: np.ndarray, split: bool = False) -> np.ndarray:
+    """The fill tasks of the 26 ghost regions around each block (region (x + 1) * 9 +
+    (y + 1) * 3 + z + 1) that its level has no block for: (block, region + 32 part,
+    parent block, 0), part 0 the whole region, or (`split`) for a face region four tasks,
+    its cells (part 1) and the edges of each component (parts 2-4); the largest tasks
+    first."""
     b, r = np.nonzero(hoods == -1)
     rel = np.stack([r // 9 - 1, (r // 3) % 3 - 1, r % 3 - 1], axis=1)
+    face = ((rel == 0).sum(axis=1) == 2) & split
     zeros = (rel == 0).sum(axis=1)  # 2: a face, 1: an edge, 0: a corner
-    order = np.lexsort((r, b, -zeros))
-    return np.stack([b[order], r[order]], axis=1).astype(np.int32)
+    tb = np.concatenate([np.repeat(b[face], 4), b[~face]])
+    tr = np.concatenate([np.repeat(r[face], 4) + 32 * np.tile(np.arange(1, 5), face.sum()),
+                         r[~face]])
+    tz = np.concatenate([np.full(4 * face.sum(), 2), zeros[~face]])
+    order = np.lexsort((tr, tb, -tz))
+    tb, tr = tb[order], tr[order]
+    return np.stack([tb, tr, parent[tb], np.zeros_like(tb)], axis=1).astype(np.int32)
 
 
 def reflux_cells(reflux: list, b: int = B) -> tuple[np.ndarray, np.ndarray]:

@@ -327,32 +327,45 @@ __global__ void __launch_bounds__(256) k_fill3d(double* __restrict__ buf,
 }
 
 // The ghosts of listed regions of blocks whose level has no block there: a workgroup per
-// task {block, region}, region (x + 1) * 9 + (y + 1) * 3 + z + 1 of the 26 around the
-// block (per axis -1: haloed cells 0..1, 0: the block's own 2..B+1, 1: B+2..B+3).  It
-// interpolates the region's cells from the level below and sets every edge of those
-// cells outside the block's own range as k_fill3d does (the master copy on the level,
-// else interpolated), so that it leaves what k_fill3d leaves on the cells and edges the
-// stage reads from the block's ghosts.  An edge shared by two regions is written by
-// both tasks with the same value (its sources are never written by a fill).
+// task {block, region + 32 part, parent}, region (x + 1) * 9 + (y + 1) * 3 + z + 1 of the 26
+// around the block (per axis -1: haloed cells 0..1, 0: the block's own 2..B+1, 1:
+// B+2..B+3).  It interpolates the region's cells from the level below and sets every
+// edge of those cells outside the block's own range as k_fill3d does (the master copy
+// on the level, else interpolated), so that it leaves what k_fill3d leaves on the cells
+// and edges the stage reads from the block's ghosts: part 0 all of it, part 1 the cells,
+// part 2 + c the edges of component c (a face region's work in four workgroups).  An
+// edge shared by two regions is written by both tasks with the same value (its sources
+// are never written by a fill).  The two neighbourhoods come from `hoods` (per block the
+// 27 offsets of the blocks of its level around it, -1: none): the block's and its
+// parent's, so a workgroup starts after two dependent loads.
 __global__ void __launch_bounds__(256) k_fill3d_regions(double* __restrict__ buf,
                                                         const BlockDesc* __restrict__ blocks,
-                                                        const int* __restrict__ lookup, Levels lv,
-                                                        const int2* __restrict__ tasks) {
+                                                        const long long* __restrict__ hoods,
+                                                        const int4* __restrict__ tasks) {
   __shared__ Hood same, coarse;
-  const int2 task = tasks[blockIdx.x];
+  const int4 task = tasks[blockIdx.x];
   const BlockDesc d = blocks[task.x];
-  load_fill_hoods(same, coarse, blocks, lookup, lv, d);
+  if (threadIdx.x < 54) {
+    const bool up = threadIdx.x >= 27;
+    const int k = up ? threadIdx.x - 27 : threadIdx.x;
+    const int owner = up ? task.z : task.x;
+    const long long off = owner >= 0 ? hoods[27 * (size_t)owner + k] : -1;
+    Hood& h = up ? coarse : same;
+    h.off[k] = off;
+    h.id[k] = off >= 0 ? 0 : -1;
+  }
   __syncthreads();
   const int bo[3] = {d.bx * B, d.by * B, d.bz * B};
   const int po[3] = {(d.bx >> 1) * B, (d.by >> 1) * B, (d.bz >> 1) * B};
   double* me = buf + d.off;
-  const int rel[3] = {task.y / 9 - 1, (task.y / 3) % 3 - 1, task.y % 3 - 1};
+  const int region = task.y & 31, part = task.y >> 5;  // task.z: the parent block
+  const int rel[3] = {region / 9 - 1, (region / 3) % 3 - 1, region % 3 - 1};
   int lo[3], n[3];
   for (int a = 0; a < 3; ++a) {
     lo[a] = rel[a] < 0 ? 0 : (rel[a] == 0 ? 2 : B + 2);
     n[a] = rel[a] == 0 ? B : 2;
   }
-  const int ncell = n[0] * n[1] * n[2];
+  const int ncell = part <= 1 ? n[0] * n[1] * n[2] : 0;
   for (int t = threadIdx.x; t < ncell; t += blockDim.x) {
     const int p[3] = {lo[0] + t / (n[1] * n[2]), lo[1] + (t / n[2]) % n[1], lo[2] + t % n[2]};
     interp_cell(buf, coarse, bo, po, me, p);
@@ -362,7 +375,7 @@ __global__ void __launch_bounds__(256) k_fill3d_regions(double* __restrict__ buf
   int m[3][3], ne[3];
   for (int c = 0; c < 3; ++c) {
     for (int a = 0; a < 3; ++a) m[c][a] = n[a] + (a == c ? 0 : 1);
-    ne[c] = m[c][0] * m[c][1] * m[c][2];
+    ne[c] = part == 0 || part == 2 + c ? m[c][0] * m[c][1] * m[c][2] : 0;
   }
   const int nall = ne[0] + ne[1] + ne[2];
   for (int t = threadIdx.x; t < nall; t += blockDim.x) {
@@ -535,13 +548,12 @@ int wb_amr3d_prolong(double* buf, const void* blocks, const int32_t* ids, int ni
   return wb_last_error();
 }
 
-int wb_amr3d_fill_regions(double* buf, const void* blocks, const int32_t* tasks, int ntasks,
-                          const int32_t* lookup, const void* levels, void* stream) {
-  const Levels lv = *(const Levels*)levels;
-  if (ntasks < 0 || !levels_ok(lv)) return WB_EINVAL;
+int wb_amr3d_fill_regions(double* buf, const void* blocks, const long long* hoods,
+                          const int32_t* tasks, int ntasks, void* stream) {
+  if (ntasks < 0) return WB_EINVAL;
   if (ntasks == 0) return 0;
   hipLaunchKernelGGL(k_fill3d_regions, dim3((unsigned)ntasks), dim3(256), 0, (hipStream_t)stream,
-                     buf, (const BlockDesc*)blocks, (const int*)lookup, lv, (const int2*)tasks);
+                     buf, (const BlockDesc*)blocks, hoods, (const int4*)tasks);
   return wb_last_error();
 }
 
