@@ -279,9 +279,22 @@ def test_gpu_region_fill_is_the_restricted_shell_fill(gpu, kind):
     _lib.call("wb_amr3d_fill_list", _lib.ptr(b), _lib.ptr(model._blocks),
               _lib.ptr(model._missing), len(model._missing), 1, _lib.ptr(model._lookup),
               _lib.ptr(model._levels), stream)
+    # the same with each face region's cells and edge components as separate tasks
+    t = model.tables
+    parent = np.full(len(t.blocks), -1, np.int32)
+    for (L, *bc), k in t.index.items():
+        if L > 0:
+            parent[k] = t.index[(L - 1, *(v // 2 for v in bc))]
+    split = amr3d_maps.fill_regions(t.hoods, parent, split=True)
+    assert len(split) > model.nregions
+    tasks = torch.as_tensor(split.reshape(-1), device=model.device)
+    c = model._buf[0].clone()
+    _lib.call("wb_amr3d_fill_regions", _lib.ptr(c), _lib.ptr(model._blocks),
+              _lib.ptr(model._hoods), _lib.ptr(tasks), len(split), stream)
     torch.cuda.synchronize()
     assert model.nregions > 0
     assert torch.equal(a, b)
+    assert torch.equal(c, b)
 
 
 @pytest.mark.gpu
