@@ -5,9 +5,11 @@ of a `wellbalanced_amd.amr3d_maps.Hierarchy3D`: one time step for all levels, th
 stage of every block in one launch, refluxing at coarse-fine faces, the covered
 cells averaged down and the potential restricted (the mean of the fine edges along
 a coarse edge, ``mhd3d.py:31-36``) so that div B is zero on the composite mesh.
-The ghost cells of a block come from the blocks of its level, or by interpolation
-from the level below (``wb_amr3d_fill``).  ``precision="mixed"`` runs the FP32
-reconstruction/HLLD stage.
+The stage and the refluxing read the cells around a block in place from the blocks
+of its level (``direct``); the ghosts its level has no block for are interpolated from
+the level below (``wb_amr3d_fill_regions``; without ``direct``, ``wb_amr3d_fill``
+fills every ghost).  ``precision="mixed"`` runs the FP32 reconstruction/HLLD stage and
+the mixed CFL rate.
 """
 
 from __future__ import annotations

@@ -17,9 +17,15 @@ back to back in one buffer.  The tables:
 * ``children``: per refined block the 8 children (average-down and restriction);
 * ``reflux``: per direction, the coarse-fine block faces (the fine block, the coarse
   block across the face and the origin of the 8 x 8 coarse cells there);
-* ``missing``: per block, whether a block of its level around it is missing: only
-  such blocks need (restricted) ghost fills when the stage and the refluxing read the
-  cells around a block in place from the blocks of its level.
+  ``reflux_items`` / ``reflux_cells``: all of them in one table and, per coarse cell
+  with such faces, its faces in direction order (the one-launch refluxing);
+* ``hoods``: per block, the offsets of the 27 blocks of its level around it (-1:
+  none), through which the stage and the refluxing read the cells around a block in
+  place;
+* ``missing``: per block, whether a block of its level around it is missing, and
+  ``fill_regions``: the ghost regions of such blocks that their level has no block for,
+  the only ghosts filled (by interpolation) when the cells around a block are read in
+  place.
 
 Domains are periodic on every axis (the refined 3D workloads).
 """

@@ -1,6 +1,7 @@
-// The refined 3D hierarchy: ghost fill, average-down, restriction of the potential,
-// the CFL rate over every block, and the FP64 stage and refluxing (the parity path,
-// built with -ffp-contract=off).
+// The refined 3D hierarchy: ghost fill (whole shells, or the regions a block's level
+// has no block for), average-down and restriction of the potential, the FP64 CFL rate
+// over every block, and the FP64 stage and refluxing (the parity path, built with
+// -ffp-contract=off).
 #include "amr3d.hpp"
 
 using namespace wb;
