@@ -127,20 +127,27 @@ class AMRMHD2D:
             self._interp[L] = ([self._dev(x) for x in (dst, pd, c00, cy, pc)]
                                + [self._dev(wx, torch.float64), self._dev(wy, torch.float64)]
                                + [self._dev(x) for x in (nid, nbase, nsx, nsy, nkx, nky)])
-        self._reflux, self._average, self._restrict = {}, {}, {}
+        self._average, self._restrict = {}, {}
+        # the refluxing of every level pair in one table: each level pair writes only its
+        # coarse level's cells and reads only the stage input, so one launch does them all
+        # before the coarsening; a group is one coarse cell's rows
+        rows_all, groups = [], []
+        base = 0
         for L in range(self.nlev - 1):
             rows = m.reflux[L]
-            groups = []
             i = 0
             while i < len(rows):
                 j = i
                 while j < len(rows) and tuple(rows[j, :3]) == tuple(rows[i, :3]):
                     j += 1
-                groups.append((i, j - i))
+                groups.append((base + i, j - i))
                 i = j
-            self._reflux[L] = (self._dev(rows.reshape(-1)),
-                               self._dev(np.array(groups, np.int32).reshape(-1), torch.int32),
-                               len(groups))
+            rows_all.append(rows.reshape(-1, 13))
+            base += len(rows)
+        rows_all = np.concatenate(rows_all) if rows_all else np.zeros((0, 13), np.int64)
+        self._reflux = (self._dev(rows_all.reshape(-1)),
+                        self._dev(np.array(groups, np.int32).reshape(-1), torch.int32), len(groups))
+        for L in range(self.nlev - 1):
             self._average[L] = (self._dev(m.average[L].reshape(-1)), len(m.average[L]))
             self._restrict[L] = (self._dev(m.restrict[L].reshape(-1)), len(m.restrict[L]))
         patches = np.zeros(len(self.layouts), _PATCH)
@@ -281,12 +288,13 @@ class AMRMHD2D:
                   self.gamma, *diff, code, _lib.ptr(self._dt), int(heun),
                   self._bad.data_ptr() + 4 * slot, int(self.nxs), int(self.tile),
                   int(self.workgroups), stream)
-        for L in range(self.nlev - 2, -1, -1):
-            rows, groups, n = self._reflux[L]
+        rows, groups, n = self._reflux
+        if n:
             _lib.call("wb_amr2d_reflux" + suffix, _lib.ptr(src), _lib.ptr(dst),
                       _lib.ptr(self._patches), _lib.ptr(rows), _lib.ptr(groups), n,
                       *self.mean_field, self.gamma, *diff, code, _lib.ptr(self._dt), int(heun),
                       stream)
+        for L in range(self.nlev - 2, -1, -1):
             self._coarsen(dst, L, stream)
 
     @property
