@@ -99,6 +99,23 @@ def test_new_entry_points_reject_bad_arguments_before_any_launch():
     assert lib.wb_amr2d_leaf_values(None, None, None, -1, 0.0, 0.0, 5 / 3, None, None) == E
     assert lib.wb_amr2d_rate(None, None, None, 0, 0.0, 0.0, 5 / 3, 0.0, 0.0, None, None,
                              None) == E
+    # the hierarchies' fused glue: negative counts are refused, empty work enqueues nothing
+    fill = lambda nc, nn, ni, nj: lib.wb_amr2d_fill_level(
+        None, None, None, None, None, nc, None, None, nn, None, None, None, None, None, None,
+        None, ni, None, None, None, None, None, None, nj, None)
+    assert fill(-1, 0, 0, 0) == E and fill(0, 0, 0, -1) == E and fill(0, 0, 0, 0) == 0
+    assert lib.wb_amr2d_coarsen(None, None, -1, None, 0, None) == E
+    assert lib.wb_amr2d_coarsen(None, None, 0, None, 0, None) == 0
+    assert lib.wb_amr3d_fill_regions(None, None, None, None, -1, None) == E
+    assert lib.wb_amr3d_fill_regions(None, None, None, None, 0, None) == 0
+    for name in ("wb_amr3d_reflux", "wb_amr3d_reflux_mixed"):
+        reflux = getattr(lib, name)
+        assert reflux(None, None, None, None, None, -1, None, 0.0, 0.0, 0.0, 5 / 3, 0, None, 0,
+                      None) == E
+        assert reflux(None, None, None, None, None, 0, None, 0.0, 0.0, 0.0, 5 / 3, 0, None, 0,
+                      None) == 0
+    for name in ("wb_amr3d_rate", "wb_amr3d_rate_mixed"):
+        assert getattr(lib, name)(None, None, 0, 0.0, 0.0, 0.0, 5 / 3, None, None) == E
 
 
 def test_negative_diffusivities_are_refused():
