@@ -58,12 +58,15 @@ __device__ __forceinline__ void reconstruct(const float (*q)[4], float* L, float
   }
 }
 
-// One side of the face: conserved state u, flux f, total pressure and fast speed.
+// One side of the face: total pressure and fast speed (both sides enter the wave speeds
+// and the star states); `side_uf` forms the conserved state and flux, needed for the
+// side of the contact the face is on only.
 struct Side {
-  float u[7], f[7], pt, cf;
+  float pt, cf;
 };
 
-__device__ __forceinline__ void side(const float* w, float bn, float gamma, float inv_gm1, Side& s) {
+__device__ __forceinline__ void side_uf(const float* w, float bn, float inv_gm1, float* u,
+                                        float* f) {
   const float rho = w[0], vn = w[1], vt1 = w[2], vt2 = w[3], p = w[4], bt1 = w[5], bt2 = w[6];
   const float bb = bn * bn + bt1 * bt1 + bt2 * bt2;
   const float magnetic = 0.5f * bb;
@@ -71,21 +74,26 @@ __device__ __forceinline__ void side(const float* w, float bn, float gamma, floa
   const float pt = p + magnetic;
   const float vb = vn * bn + vt1 * bt1 + vt2 * bt2;
   const float mn = rho * vn;
-  s.u[0] = rho;
-  s.u[1] = mn;
-  s.u[2] = rho * vt1;
-  s.u[3] = rho * vt2;
-  s.u[4] = energy;
-  s.u[5] = bt1;
-  s.u[6] = bt2;
-  s.f[0] = mn;
-  s.f[1] = mn * vn + pt - bn * bn;
-  s.f[2] = mn * vt1 - bn * bt1;
-  s.f[3] = mn * vt2 - bn * bt2;
-  s.f[4] = (energy + pt) * vn - bn * vb;
-  s.f[5] = bt1 * vn - bn * vt1;
-  s.f[6] = bt2 * vn - bn * vt2;
-  s.pt = pt;
+  u[0] = rho;
+  u[1] = mn;
+  u[2] = rho * vt1;
+  u[3] = rho * vt2;
+  u[4] = energy;
+  u[5] = bt1;
+  u[6] = bt2;
+  f[0] = mn;
+  f[1] = mn * vn + pt - bn * bn;
+  f[2] = mn * vt1 - bn * bt1;
+  f[3] = mn * vt2 - bn * bt2;
+  f[4] = (energy + pt) * vn - bn * vb;
+  f[5] = bt1 * vn - bn * vt1;
+  f[6] = bt2 * vn - bn * vt2;
+}
+
+__device__ __forceinline__ void side(const float* w, float bn, float gamma, Side& s) {
+  const float rho = w[0], p = w[4], bt1 = w[5], bt2 = w[6];
+  const float bb = bn * bn + bt1 * bt1 + bt2 * bt2;
+  s.pt = p + 0.5f * bb;
   // fast magnetosonic speed (mhd_sweep_pyccel.py:51-61)
   const float irho = rcp(rho);
   const float a2 = gamma * p * irho, b2 = bb * irho;
@@ -132,8 +140,8 @@ __device__ __forceinline__ Star star(const float* w, const Side& sd, float bn, f
 __device__ __forceinline__ void hlld(const float* L, const float* R, float bn, float gamma,
                                      float inv_gm1, float* flux) {
   Side sl_, sr_;
-  side(L, bn, gamma, inv_gm1, sl_);
-  side(R, bn, gamma, inv_gm1, sr_);
+  side(L, bn, gamma, sl_);
+  side(R, bn, gamma, sr_);
   const float vl = L[1], vr = R[1];
   const float cf = __builtin_fmaxf(sl_.cf, sr_.cf);
   const float sl = __builtin_fminf(vl, vr) - cf;
@@ -169,16 +177,13 @@ __device__ __forceinline__ void hlld(const float* L, const float* R, float bn, f
   const float w0 = sel(left, wl[0], wr[0]), w1 = sel(left, wl[1], wr[1]);
   const float w2 = sel(left, wl[2], wr[2]), w3 = sel(left, wl[3], wr[3]);
   const float vb_s = sm * bn + w0 * w2 + w1 * w3;
-  float uu[7], ff[7];
+  float wc[7], uu[7], ff[7];
 #pragma unroll
-  for (int c = 0; c < 7; ++c) {
-    uu[c] = sel(left, sl_.u[c], sr_.u[c]);
-    ff[c] = sel(left, sl_.f[c], sr_.f[c]);
-  }
+  for (int c = 0; c < 7; ++c) wc[c] = sel(left, L[c], R[c]);
+  side_uf(wc, bn, inv_gm1, uu, ff);
   // the star state of that side (mhd_sweep_pyccel.py:131-145)
-  const float vn = sel(left, L[1], R[1]);
-  const float vb = vn * bn + sel(left, L[2], R[2]) * sel(left, L[5], R[5]) +
-                   sel(left, L[3], R[3]) * sel(left, L[6], R[6]);
+  const float vn = wc[1];
+  const float vb = vn * bn + wc[2] * wc[5] + wc[3] * wc[6];
   const float rho_s = sel(left, tl.rho, tr.rho);
   const float us[7] = {rho_s, rho_s * sm, rho_s * w0, rho_s * w1,
                        ((s - vn) * uu[4] - sel(left, sl_.pt, sr_.pt) * vn + pts * sm +

@@ -274,7 +274,8 @@ __device__ __forceinline__ void stage2d_body(const Stage2Args& a, int tile, int 
 }
 
 template <typename P, int TY, int LIM, bool NI = false>
-__global__ void __launch_bounds__((Row<TY>::THREADS)) k_stage2d(Stage2Args a) {
+__global__ void __launch_bounds__((Row<TY>::THREADS)) __attribute__((amdgpu_waves_per_eu(1, 3)))
+k_stage2d(Stage2Args a) {
   __shared__ Stage2Shared<P, TY, NI> sh;
   stage2d_body<P, TY, LIM, NI>(a, blockIdx.x, blockIdx.y, sh);
 }
@@ -284,7 +285,7 @@ __global__ void __launch_bounds__((Row<TY>::THREADS)) k_stage2d(Stage2Args a) {
 // work item from a counter until the list is exhausted (the counter is zeroed by
 // k_zero_i32 before the launch).
 template <typename P, int TY, int LIM, bool NI = false>
-__global__ void __launch_bounds__((Row<TY>::THREADS))
+__global__ void __launch_bounds__((Row<TY>::THREADS)) __attribute__((amdgpu_waves_per_eu(1, 3)))
 k_stage2d_batched(Stage2Args base, const PatchDesc* __restrict__ patches,
                   const WorkItem* __restrict__ items, int nitems, int* __restrict__ counter,
                   double* __restrict__ out_base) {
