@@ -10,8 +10,10 @@ of persistent workgroups, for the mixed 3D stages the waves per SIMD the registe
 allocation targets (``waves``), and for the 3D block hierarchy whether blocks with
 every neighbour read them in place instead of filled ghosts (``direct``).  `tune`
 times each candidate on the model's own
-state for a few steps (state restored afterwards) and keeps the fastest; the choice
-is cached per (model kind, shape, precision) for the process.
+state (state restored afterwards) for `steps` steps, or for more steps until the run
+lasts MIN_SECONDS (on small grids a step is ~0.1 ms, below the host clock's noise),
+and keeps the fastest per step; the choice is cached per (model kind, shape,
+precision) for the process.
 """
 
 from __future__ import annotations
@@ -22,6 +24,7 @@ import time
 import torch
 
 _CACHE: dict[tuple, dict] = {}
+MIN_SECONDS = 0.005
 
 DEFAULTS_2D = {"tile": [30, 62, 126, 254], "nxs": [4, 8, 16, 32, 64]}
 DEFAULTS_3D = {"nxs": [8, 16, 32, 64]}
@@ -75,10 +78,16 @@ def tune(model, space: dict | None = None, steps: int = 3) -> dict:
         _apply(model, choice)
         model.advance(1)
         torch.cuda.synchronize(model.device)
-        t0 = time.perf_counter()
-        model.advance(steps)
-        torch.cuda.synchronize(model.device)
-        elapsed = time.perf_counter() - t0
+        reps = steps
+        while True:
+            t0 = time.perf_counter()
+            model.advance(reps)
+            torch.cuda.synchronize(model.device)
+            elapsed = time.perf_counter() - t0
+            if elapsed >= MIN_SECONDS or reps >= 64 * steps:
+                break
+            reps *= 4
+        elapsed /= reps
         if elapsed < best_t:
             best, best_t = choice, elapsed
         for b, s in zip(model._buf, saved[0]):
