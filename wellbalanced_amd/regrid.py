@@ -159,10 +159,19 @@ def transfer_tables(old: AMRHierarchy, new: AMRHierarchy, new_maps) -> dict:
     ``cells`` (new offset, new plane, coarse cell offset, coarse row length, coarse
     plane, side x, side y; side -1 / +1: the lower / upper child) and ``nodes`` (new
     offset, coarse base node offset, coarse node row length, kind x, kind y; kind 1:
-    the mean with the next coarse node) of the rest."""
+    the mean with the next coarse node) of the rest.  Whole levels at once."""
     old_layouts, _ = amr_maps.layouts_of(old)
     new_layouts = new_maps.layouts
     boxes = amr_maps.box_array(new_layouts)
+    g = amr_maps._patch_arrays(new_layouts)
+    Y = g["ny"] + 2 * HALO
+    plane = (g["nx"] + 2 * HALO) * Y
+    A0 = g["off"] + amr_maps.NFIELDS * plane
+    if old_layouts:
+        go = amr_maps._patch_arrays(old_layouts)
+        Yo = go["ny"] + 2 * HALO
+        planeo = (go["nx"] + 2 * HALO) * Yo
+        A0o = go["off"] + amr_maps.NFIELDS * planeo
     tables = {}
     for L in range(new.max_level + 1):
         shape = new.level_shape(L)
@@ -173,77 +182,57 @@ def transfer_tables(old: AMRHierarchy, new: AMRHierarchy, new_maps) -> dict:
             node_own = np.full(shape, -1, np.int64)
         coarse_owner = (amr_maps._owner_tables(new_layouts, L - 1, new.level_shape(L - 1))[0]
                         if L > 0 else None)
-        cd, cs, cpd, cps, nd, ns = [], [], [], [], [], []
-        pd, pp, pc0, pcy, pcp, psx, psy = [], [], [], [], [], [], []
-        qd, qb, qs, qkx, qky = [], [], [], [], []
-        for lay in new_layouts:
-            if lay.level != L:
-                continue
-            (i0, i1), (j0, j1) = lay.box
-            a, b = np.meshgrid(np.arange(HALO, HALO + lay.nx), np.arange(HALO, HALO + lay.ny),
-                               indexing="ij")
-            a, b = a.ravel(), b.ravel()
-            I, J = i0 + a - HALO, j0 + b - HALO
-            src = cell_own[I, J]
-            for q in np.unique(src[src >= 0]):
-                sel = src == q
-                ql = old_layouts[q]
-                cd.append(lay.cell(a[sel], b[sel]))
-                cs.append(ql.cell(I[sel] - ql.box[0][0] + HALO, J[sel] - ql.box[1][0] + HALO))
-                cpd.append(np.full(sel.sum(), lay.plane, np.int64))
-                cps.append(np.full(sel.sum(), ql.plane, np.int64))
-            miss = src < 0
-            if miss.any():
-                if L == 0:
-                    raise ValueError("level 0 must cover the domain")
-                fi, fj = I[miss], J[miss]
-                q, qa, qbb = amr_maps._coarse_container(new_layouts, L - 1, fi // 2 - 1,
-                                                        fj // 2 - 1, 3, new, cells=True,
-                                                        owner=coarse_owner, boxes=boxes)
-                for qq in np.unique(q):
-                    sel = q == qq
-                    ql = new_layouts[qq]
-                    pd.append(lay.cell(a[miss][sel], b[miss][sel]))
-                    pp.append(np.full(sel.sum(), lay.plane, np.int64))
-                    pc0.append(ql.cell(qa[sel] + 1, qbb[sel] + 1))
-                    pcy.append(np.full(sel.sum(), ql.Y, np.int64))
-                    pcp.append(np.full(sel.sum(), ql.plane, np.int64))
-                    psx.append(np.where(fi[sel] % 2 == 1, 1, -1).astype(np.int64))
-                    psy.append(np.where(fj[sel] % 2 == 1, 1, -1).astype(np.int64))
-            # nodes, the patch's boundary nodes included (periodic: node n is node 0)
-            a, b = np.meshgrid(np.arange(HALO, HALO + lay.nx + 1),
-                               np.arange(HALO, HALO + lay.ny + 1), indexing="ij")
-            a, b = a.ravel(), b.ravel()
-            I, J = (i0 + a - HALO) % shape[0], (j0 + b - HALO) % shape[1]
-            src = node_own[I, J]
-            for q in np.unique(src[src >= 0]):
-                sel = src == q
-                ql = old_layouts[q]
-                nd.append(lay.node(a[sel], b[sel]))
-                ns.append(ql.node((I[sel] - ql.box[0][0]) % shape[0] + HALO,
-                                  (J[sel] - ql.box[1][0]) % shape[1] + HALO))
-            miss = src < 0
-            if miss.any():
-                if L == 0:
-                    raise ValueError("level 0 must cover the domain")
-                fi = i0 + a[miss] - HALO  # unwrapped: the container tries periodic images
-                fj = j0 + b[miss] - HALO
-                q, qa, qbb = amr_maps._coarse_container(new_layouts, L - 1, fi // 2, fj // 2, 2,
-                                                        new, cells=False, owner=coarse_owner,
-                                                        boxes=boxes)
-                for qq in np.unique(q):
-                    sel = q == qq
-                    ql = new_layouts[qq]
-                    qd.append(lay.node(a[miss][sel], b[miss][sel]))
-                    qb.append(ql.node(qa[sel], qbb[sel]))
-                    qs.append(np.full(sel.sum(), ql.Y + 1, np.int64))
-                    qkx.append((fi[sel] % 2).astype(np.int64))
-                    qky.append((fj[sel] % 2).astype(np.int64))
-        cat = amr_maps._cat
-        tables[L] = {
-            "copy_cells": (cat(cd), cat(cs), cat(cpd), cat(cps)),
-            "copy_nodes": (cat(nd), cat(ns)),
-            "cells": (cat(pd), cat(pp), cat(pc0), cat(pcy), cat(pcp), cat(psx), cat(psy)),
-            "nodes": (cat(qd), cat(qb), cat(qs), cat(qkx), cat(qky)),
-        }
+        ids = np.flatnonzero(g["level"] == L)
+        # cells: every interior cell of the level's new patches
+        k, a, b = amr_maps._points(ids, g["nx"][ids], g["ny"][ids])
+        a, b = a + HALO, b + HALO
+        I, J = g["i0"][k] + a - HALO, g["j0"][k] + b - HALO
+        src = cell_own[I, J]
+        have = src >= 0
+        q = src[have]
+        copy_cells = (g["off"][k[have]] + a[have] * Y[k[have]] + b[have],
+                      go["off"][q] + (I[have] - go["i0"][q] + HALO) * Yo[q]
+                      + (J[have] - go["j0"][q] + HALO) if len(q) else np.zeros(0, np.int64),
+                      plane[k[have]], planeo[q] if len(q) else np.zeros(0, np.int64))
+        miss = ~have
+        if miss.any():
+            if L == 0:
+                raise ValueError("level 0 must cover the domain")
+            km, fi, fj = k[miss], I[miss], J[miss]
+            q, qa, qbb = amr_maps._coarse_container(new_layouts, L - 1, fi // 2 - 1, fj // 2 - 1,
+                                                    3, new, cells=True, owner=coarse_owner,
+                                                    boxes=boxes)
+            cells = (g["off"][km] + a[miss] * Y[km] + b[miss], plane[km],
+                     g["off"][q] + (qa + 1) * Y[q] + (qbb + 1), Y[q], plane[q],
+                     np.where(fi % 2 == 1, 1, -1).astype(np.int64),
+                     np.where(fj % 2 == 1, 1, -1).astype(np.int64))
+        else:
+            cells = tuple(np.zeros(0, np.int64) for _ in range(7))
+        # nodes, the patch's boundary nodes included (periodic: node n is node 0)
+        k, a, b = amr_maps._points(ids, g["nx"][ids] + 1, g["ny"][ids] + 1)
+        a, b = a + HALO, b + HALO
+        I = (g["i0"][k] + a - HALO) % shape[0]
+        J = (g["j0"][k] + b - HALO) % shape[1]
+        src = node_own[I, J]
+        have = src >= 0
+        q = src[have]
+        copy_nodes = (A0[k[have]] + a[have] * (Y[k[have]] + 1) + b[have],
+                      A0o[q] + ((I[have] - go["i0"][q]) % shape[0] + HALO) * (Yo[q] + 1)
+                      + (J[have] - go["j0"][q]) % shape[1] + HALO if len(q)
+                      else np.zeros(0, np.int64))
+        miss = ~have
+        if miss.any():
+            if L == 0:
+                raise ValueError("level 0 must cover the domain")
+            km = k[miss]
+            fi = g["i0"][km] + a[miss] - HALO  # unwrapped: the container tries periodic images
+            fj = g["j0"][km] + b[miss] - HALO
+            q, qa, qbb = amr_maps._coarse_container(new_layouts, L - 1, fi // 2, fj // 2, 2, new,
+                                                    cells=False, owner=coarse_owner, boxes=boxes)
+            nodes = (A0[km] + a[miss] * (Y[km] + 1) + b[miss], A0[q] + qa * (Y[q] + 1) + qbb,
+                     Y[q] + 1, (fi % 2).astype(np.int64), (fj % 2).astype(np.int64))
+        else:
+            nodes = tuple(np.zeros(0, np.int64) for _ in range(5))
+        tables[L] = {"copy_cells": copy_cells, "copy_nodes": copy_nodes, "cells": cells,
+                     "nodes": nodes}
     return tables
