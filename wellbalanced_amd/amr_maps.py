@@ -358,33 +358,27 @@ def _reflux_table(layouts, owners, L, hierarchy):
 
 
 def _average_table(layouts, owners, L, hierarchy):
-    """Covered coarse cells of level L-1 with the offsets of their four fine children."""
+    """Covered coarse cells of level L-1 with the offsets of their four fine children
+    (the level's patches at once)."""
     coarse_cells, _ = owners[L - 1]
-    rows = []
-    for p, pl in enumerate(layouts):
-        if pl.level != L:
-            continue
-        (i0, i1), (j0, j1) = pl.box
-        if i0 % 2 or i1 % 2 or j0 % 2 or j1 % 2:
-            raise ValueError("fine patches are aligned to the coarse cells")
-        ci, cj = np.meshgrid(np.arange(i0 // 2, i1 // 2), np.arange(j0 // 2, j1 // 2),
-                             indexing="ij")
-        ci, cj = ci.ravel(), cj.ravel()
-        q = coarse_cells[ci, cj]
-        if (q < 0).any():
-            raise ValueError("a fine patch lies outside the level below")
-        for qq in np.unique(q):
-            sel = q == qq
-            ql = layouts[qq]
-            m = int(sel.sum())
-            rows.append(np.stack([
-                ql.cell(ci[sel] - ql.box[0][0] + HALO, cj[sel] - ql.box[1][0] + HALO),
-                np.full(m, ql.plane, np.int64),
-                pl.cell(2 * ci[sel] - i0 + HALO, 2 * cj[sel] - j0 + HALO),
-                np.full(m, pl.Y, np.int64), np.full(m, pl.plane, np.int64)], axis=1))
-    if not rows:
+    g = _patch_arrays(layouts)
+    ids = np.flatnonzero(g["level"] == L)
+    if len(ids) == 0:
         return np.zeros((0, 5), np.int64)
-    return np.concatenate(rows).astype(np.int64)
+    if ((g["i0"][ids] | g["nx"][ids] | g["j0"][ids] | g["ny"][ids]) % 2).any():
+        raise ValueError("fine patches are aligned to the coarse cells")
+    k, a, b = _points(ids, g["nx"][ids] // 2, g["ny"][ids] // 2)
+    ci, cj = g["i0"][k] // 2 + a, g["j0"][k] // 2 + b
+    q = coarse_cells[ci, cj]
+    if (q < 0).any():
+        raise ValueError("a fine patch lies outside the level below")
+    Y = g["ny"] + 2 * HALO
+    plane = (g["nx"] + 2 * HALO) * Y
+    return np.stack([
+        g["off"][q] + (ci - g["i0"][q] + HALO) * Y[q] + (cj - g["j0"][q] + HALO),
+        plane[q],
+        g["off"][k] + (2 * ci - g["i0"][k] + HALO) * Y[k] + (2 * cj - g["j0"][k] + HALO),
+        Y[k], plane[k]], axis=1).astype(np.int64)
 
 
 def _restrict_table(layouts, owners, L, hierarchy):
