@@ -293,7 +293,8 @@ int wb_cnn_wgrad(const uint16_t* x, int cin, const uint16_t* g16, int64_t nx, in
  * entry) and nnodes A_z nodes (ghosts and shared boundary nodes) copied from their
  * canonical same-level owner; the nic other ghost cells bilinear from the level below
  * (upper-cell weights iwx, iwy) and the nin other ghost nodes by the coarse node or the
- * 4-point midpoint rule per axis (kind jkx, jky).  Levels in increasing order. */
+ * quadratic midpoint rule (-1/8, 3/4, 3/8) per axis (kind jkx, jky; CHANGELOG.md:10-13).
+ * Levels in increasing order. */
 int wb_amr2d_fill_level(double* buf, const long long* cdst, const long long* csrc,
                         const long long* cpd, const long long* cps, int ncells,
                         const long long* ndst, const long long* nsrc, int nnodes,
@@ -370,19 +371,21 @@ int wb_amr2d_stage_mixed(const double* in, double* out, const void* patches, con
                          double eta, double kpar, double kperp, int limiter, const double* dt,
                          int heun, int32_t* bad, int nxs, int tile, int workgroups, void* stream);
 
-/* Refluxing (FluxRegister.reflux, mhd.py:989-990) after a stage: for each group
- * (first row, count) of rows (13 int64: coarse patch, cell, axis, side, face, and
- * the two fine faces), the coarse flux the cell was updated with is replaced by the
- * mean of the fine fluxes, recomputed from the stage input `src` (heun: half of the
- * correction, as the stage's average). */
+/* The flux register and refluxing (FluxRegister.update with scale 0.5 dt and
+ * FluxRegister.reflux, mhd.py:911-919,989-990).  For each group (first row, count) of
+ * rows (13 int64: coarse patch, cell, axis, side, face, and the two fine faces) the
+ * correction d = sum of dt/h (coarse flux - mean of the fine fluxes), signed by the
+ * side, is recomputed from the stage input `src`.  record != 0 (after stage 1):
+ * reg[6 g + f] = 0.5 d (dst untouched); record == 0 (after stage 2 and its Heun
+ * average): dst cell += reg + 0.5 d.  reg: 6 doubles per group, caller-owned. */
 int wb_amr2d_reflux(const double* src, double* dst, const void* patches, const long long* rows,
                     const int32_t* groups, int ngroups, double b0x, double b0y, double gamma,
-                    double eta, double kpar, double kperp, int limiter, const double* dt, int heun,
-                    void* stream);
+                    double eta, double kpar, double kperp, int limiter, const double* dt,
+                    int record, double* reg, void* stream);
 int wb_amr2d_reflux_mixed(const double* src, double* dst, const void* patches,
                           const long long* rows, const int32_t* groups, int ngroups, double b0x,
                           double b0y, double gamma, double eta, double kpar, double kperp,
-                          int limiter, const double* dt, int heun, void* stream);
+                          int limiter, const double* dt, int record, double* reg, void* stream);
 
 /* The resistive terms of MHD3D (mhd3d.py:561-602) after a stage: from the stage input
  * `in` (halos filled), eta J on the block's edges (the potential of `out` loses

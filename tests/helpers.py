@@ -84,3 +84,35 @@ def state_of(fields):
     rho, u, v, w, p, bx, by, bz = fields
     energy = p / (g - 1.0) + 0.5 * rho * (u * u + v * v + w * w) + 0.5 * (bx * bx + by * by + bz * bz)
     return rho, rho * u, rho * v, rho * w, energy
+
+
+def amr_oracle(h, state, *, gamma=5.0 / 3.0, mean_field=(0.0, 0.0), **options):
+    """The refined 2D oracle of hierarchy ``h`` from the host state the GPU gets, the
+    covered cells averaged and A_z restricted as ``set_state`` does (mhd.py:683-737)."""
+    from oracle.amr2d import OracleAMR2D, pack
+
+    patches = [(p.level, p.box) for p in h.patches]
+    per = [initial.host_state_2d(h, p, gamma, mean_field, **state) for p in h.patches]
+    o = OracleAMR2D(patches, h.base_shape, h.bounds, pack(per, patches), periodic=h.periodic,
+                    gamma=gamma, mean_field=mean_field, **options)
+    o.average_down(o.state)
+    o.restrict(o.state)
+    return o
+
+
+def composite_mismatch(o) -> float:
+    """The largest difference of a coarse A_z node and the fine node on it (every
+    coarse patch's own nodes), over the oracle's patches."""
+    worst = 0.0
+    geo = o.geo
+    for q, (L, ((i0, i1), (j0, j1))) in enumerate(o.patches):
+        if L + 1 >= o.nlev:
+            continue
+        for i in range(i0, i1 + 1):
+            for j in range(j0, j1 + 1):
+                src = geo.node_of(L + 1, 2 * i, 2 * j)
+                if src is None:
+                    continue
+                p, fa, fb = src
+                worst = max(worst, abs(o.state[q][1][i - i0 + 2, j - j0 + 2] - o.state[p][1][fa, fb]))
+    return worst

@@ -88,7 +88,12 @@ def test_new_entry_points_reject_bad_arguments_before_any_launch():
     assert lib.wb_mhd2d_stage(None, None, 8, 8, 0.1, 0.1, 0.0, 0.0, 5 / 3, -1.0, 0.0, 0.0, 0,
                               None, 0, 0, None, 8, 126, None) == E
     assert lib.wb_amr2d_reflux(None, None, None, None, None, 1, 0.0, 0.0, 5 / 3, 0.0, -1.0,
-                               0.0, 0, None, 0, None) == E
+                               0.0, 0, None, 0, None, None) == E
+    # a register is needed when there is a coarse cell to reflux
+    assert lib.wb_amr2d_reflux(None, None, None, None, None, 1, 0.0, 0.0, 5 / 3, 0.0, 0.0,
+                               0.0, 0, None, 1, None, None) == E
+    assert lib.wb_amr2d_reflux_mixed(None, None, None, None, None, 1, 0.0, 0.0, 5 / 3, 0.0,
+                                     0.0, 0.0, 0, None, 0, None, None) == E
     assert lib.wb_mhd3d_resistive_workspace(0, 4, 4) == E
     assert lib.wb_mhd3d_resistive(None, None, 0, 4, 4, 0.1, 0.1, 0.1, 0.0, 0.0, 0.0, 0.1, None,
                                   0, None, None) == E

@@ -4,8 +4,7 @@ the GPU path against the oracle (FP64: bit for bit)."""
 import numpy as np
 import pytest
 
-from oracle.amr2d import OracleAMR2D
-from tests.helpers import oracle2d
+from tests.helpers import amr_oracle, composite_mismatch, oracle2d
 from wellbalanced_amd import amr_maps, initial
 from wellbalanced_amd.hierarchy import AMRHierarchy
 
@@ -32,13 +31,7 @@ def _refined(levels=2):
 
 
 def _oracle(h, state, limiter="minmod"):
-    maps = amr_maps.build(h)
-    buf = amr_maps.pack_state(h, maps.layouts, maps.total, 5.0 / 3.0, (0.0, 0.0), state)
-    amr = OracleAMR2D(maps, h, buf, limiter=limiter)
-    for L in range(amr.nlev - 2, -1, -1):
-        amr.average(amr.buf, L)
-        amr.restrict(amr.buf, L)
-    return maps, amr
+    return amr_oracle(h, state, limiter=limiter)
 
 
 def test_a_refined_level_covering_the_domain_is_the_uniform_run():
@@ -48,15 +41,13 @@ def test_a_refined_level_covering_the_domain_is_the_uniform_run():
     for box in [((0, 16), (0, 16)), ((16, 32), (0, 16)), ((0, 16), (16, 32)), ((16, 32), (16, 32))]:
         h.add_patch(1, box)
     problem = initial.orszag_tang(32, limiter="minmod")
-    maps, amr = _oracle(h, problem.state)
+    amr = _oracle(h, problem.state)
     uniform = oracle2d(problem)
     for _ in range(3):
         dt = uniform.stable_step()
         uniform.step(dt)
         amr.step(dt)
-    for lay in maps.layouts[1:]:
-        cells, az = amr.views(amr.buf, lay)
-        (i0, i1), (j0, j1) = lay.box
+    for (L, ((i0, i1), (j0, j1))), (cells, az) in list(zip(amr.patches, amr.state))[1:]:
         np.testing.assert_array_equal(cells[:, 2:-2, 2:-2], uniform.interior()[:, i0:i1, j0:j1])
         np.testing.assert_array_equal(az[2:-2, 2:-2], uniform.nodes()[i0 : i1 + 1, j0 : j1 + 1])
 
@@ -66,20 +57,13 @@ def test_refined_patches_conserve_and_keep_div_b_zero(levels):
     """test_mhd.py:179-191 and test_mhd_amr.py:69-89: the leaf totals stay to 1e-13
     and every coarse node on a finer patch equals the fine node (composite div B 0)."""
     h = _refined(levels)
-    maps, amr = _oracle(h, _smooth_state())
+    amr = _oracle(h, _smooth_state())
     before = amr.leaf_totals()
     for _ in range(6):
         amr.step()
     np.testing.assert_allclose(amr.leaf_totals(), before, rtol=0, atol=1e-13)
-    for L, rows in maps.restrict.items():
-        if len(rows):
-            assert np.array_equal(amr.buf[rows[:, 0]], amr.buf[rows[:, 1]])
-    for lay in maps.layouts:
-        cells, az = amr.views(amr.buf, lay)
-        dx, dy = h.spacing(lay.level)
-        a = az[2:-2, 2:-2]
-        div = np.diff((a[:, 1:] - a[:, :-1]) / dy, axis=0) / dx + np.diff(-(a[1:, :] - a[:-1, :]) / dx, axis=1) / dy
-        assert np.abs(div).max() * min(dx, dy) < 1e-14
+    assert composite_mismatch(amr) == 0.0
+    assert amr.divergence() < 1e-14
 
 
 def test_refined_patch_follows_the_uniform_solution():
@@ -89,7 +73,7 @@ def test_refined_patch_follows_the_uniform_solution():
     solution stays within 0.05 of the uniform coarse run, and under it within 0.05 of
     the uniform fine run averaged to the coarse cells."""
     h = _refined(2)
-    maps, amr = _oracle(h, _smooth_state())
+    amr = _oracle(h, _smooth_state())
     runs = []
     for n in (32, 64):
         hu = AMRHierarchy((n, n // 2), ((-1.0, 1.0), (-0.5, 0.5)), periodic=(True, True))
@@ -97,9 +81,7 @@ def test_refined_patch_follows_the_uniform_solution():
     for model in (amr, *runs):
         while model.time < 0.1 - 1e-12:
             model.step(min(model.stable_step(), 0.1 - model.time))
-    coarse = maps.layouts[0]
-    cells, _ = amr.views(amr.buf, coarse)
-    rho = cells[0, 2:-2, 2:-2]
+    rho = amr.state[0][0][0, 2:-2, 2:-2]
     f = runs[1].interior()[0]
     fine_avg = 0.25 * ((f[0::2, 0::2] + f[1::2, 0::2]) + (f[0::2, 1::2] + f[1::2, 1::2]))
     covered = np.zeros(rho.shape, bool)
@@ -110,11 +92,10 @@ def test_refined_patch_follows_the_uniform_solution():
 
 def test_uniform_state_stays_uniform_across_levels():
     h = _refined(3)
-    maps, amr = _oracle(h, dict(density=1.3, velocity=(0.2, -0.1, 0.05), pressure=0.7, bz=0.4))
+    amr = _oracle(h, dict(density=1.3, velocity=(0.2, -0.1, 0.05), pressure=0.7, bz=0.4))
     for _ in range(3):
         amr.step()
-    for lay in maps.layouts:
-        cells, _ = amr.views(amr.buf, lay)
+    for cells, _ in amr.state:
         np.testing.assert_allclose(cells[0, 2:-2, 2:-2], 1.3, atol=1e-13)
         np.testing.assert_allclose(cells[5, 2:-2, 2:-2], 0.4, atol=1e-13)
 
@@ -130,6 +111,15 @@ def test_blast_hierarchy_is_nested_and_uneven():
     assert maps.total > 0 and len(maps.reflux[0]) > 0 and len(maps.reflux[1]) > 0
 
 
+def _assert_interiors_equal(buf, oracle):
+    """Every patch's own cells and nodes of a device buffer equal the oracle's."""
+    from oracle.amr2d import unpack
+
+    for (cells, az), (oc, oa) in zip(unpack(buf, oracle.patches), oracle.state):
+        np.testing.assert_array_equal(cells[:, 2:-2, 2:-2], oc[:, 2:-2, 2:-2])
+        np.testing.assert_array_equal(az[2:-2, 2:-2], oa[2:-2, 2:-2])
+
+
 # -- GPU ----------------------------------------------------------------------------------
 @pytest.mark.gpu
 @pytest.mark.parametrize("tile,nxs", [(30, 8), (62, 16), (126, 5)])
@@ -137,7 +127,7 @@ def test_gpu_hierarchy_is_bitwise_the_oracle(gpu, tile, nxs):
     from wellbalanced_amd.amr import AMRMHD2D
 
     h = _refined(3)
-    maps, amr = _oracle(h, _smooth_state())
+    amr = _oracle(h, _smooth_state())
     model = AMRMHD2D(h, limiter="minmod", tile=tile, nxs=nxs)
     model.set_state(**_smooth_state())
     np.testing.assert_array_equal(model.buffer(), amr.buf)
@@ -146,15 +136,7 @@ def test_gpu_hierarchy_is_bitwise_the_oracle(gpu, tile, nxs):
         assert model.stable_step() == dt
         model.step(dt)
         amr.step(dt)
-    got = model.buffer()
-    for lay in maps.layouts:
-        for f in range(6):
-            a = got[lay.offset + f * lay.plane : lay.offset + (f + 1) * lay.plane].reshape(lay.X, lay.Y)
-            b = amr.buf[lay.offset + f * lay.plane : lay.offset + (f + 1) * lay.plane].reshape(lay.X, lay.Y)
-            np.testing.assert_array_equal(a[2:-2, 2:-2], b[2:-2, 2:-2])
-        za = got[lay.offset + 6 * lay.plane : lay.offset + lay.size].reshape(lay.X + 1, lay.Y + 1)
-        zb = amr.buf[lay.offset + 6 * lay.plane : lay.offset + lay.size].reshape(lay.X + 1, lay.Y + 1)
-        np.testing.assert_array_equal(za[2:-2, 2:-2], zb[2:-2, 2:-2])
+    _assert_interiors_equal(model.buffer(), amr)
 
 
 @pytest.mark.gpu

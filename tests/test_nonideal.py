@@ -365,19 +365,13 @@ def test_gpu_hierarchy_with_diffusion_matches_the_oracle_and_conserves(gpu, prec
     refluxing's recomputed FP32 face flux has the stage's bits);
     FP64 is bit for bit the oracle, including the step limited by each patch's diffusion
     number."""
-    from oracle.amr2d import OracleAMR2D
-    from tests.test_amr import _refined, _smooth_state
-    from wellbalanced_amd import amr_maps
+    from tests.helpers import amr_oracle
+    from tests.test_amr import _assert_interiors_equal, _refined, _smooth_state
     from wellbalanced_amd.amr import AMRMHD2D
 
     h = _refined(3)
     diff = dict(resistivity=2e-3, conduction=(0.03, 0.003))
-    maps = amr_maps.build(h)
-    buf = amr_maps.pack_state(h, maps.layouts, maps.total, 5.0 / 3.0, (0.0, 0.0), _smooth_state())
-    oracle = OracleAMR2D(maps, h, buf, limiter="minmod", **diff)
-    for L in range(oracle.nlev - 2, -1, -1):
-        oracle.average(oracle.buf, L)
-        oracle.restrict(oracle.buf, L)
+    oracle = amr_oracle(h, _smooth_state(), limiter="minmod", **diff)
     model = AMRMHD2D(h, limiter="minmod", precision=precision, **diff)
     model.set_state(**_smooth_state())
     before = model.leaf_totals()
@@ -391,12 +385,7 @@ def test_gpu_hierarchy_with_diffusion_matches_the_oracle_and_conserves(gpu, prec
     assert np.abs(after - before).max() < 1e-13 * np.abs(before).max()
     assert model.composite_divergence() == 0.0
     if precision == "fp64":
-        got = model.buffer()
-        for lay in maps.layouts:
-            cells = slice(lay.offset, lay.offset + 6 * lay.plane)
-            a = got[cells].reshape(6, lay.X, lay.Y)[:, 2:-2, 2:-2]
-            b = oracle.buf[cells].reshape(6, lay.X, lay.Y)[:, 2:-2, 2:-2]
-            np.testing.assert_array_equal(a, b)
+        _assert_interiors_equal(model.buffer(), oracle)
 
 
 # -- 3D resistivity (mhd3d.py:561-602, 674-724) ---------------------------------------------

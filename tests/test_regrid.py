@@ -9,7 +9,8 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from oracle.amr2d import OracleAMR2D, regrid_state
+from oracle.amr2d import OracleAMR2D, regrid_state, unpack
+from tests.helpers import amr_oracle, composite_mismatch
 from wellbalanced_amd import amr_maps, initial
 from wellbalanced_amd import regrid as rg
 from wellbalanced_amd.hierarchy import AMRHierarchy
@@ -101,56 +102,53 @@ def _state(kind):
 
 
 def _oracle_on(h, state):
-    maps = amr_maps.build(h)
-    buf = amr_maps.pack_state(h, maps.layouts, maps.total, 5.0 / 3.0, (0.0, 0.0), state)
-    o = OracleAMR2D(maps, h, buf)
-    for L in range(o.nlev - 2, -1, -1):
-        o.average(o.buf, L)
-        o.restrict(o.buf, L)
-    return maps, o
+    return amr_oracle(h, state)
+
+
+def _patches(h):
+    return [(p.level, p.box) for p in h.patches]
 
 
 def test_the_move_keeps_the_integrals_and_div_b():
     _, h1, h2, _ = _two_level_regrid()
-    _, before = _oracle_on(h1, _state("smooth"))
+    before = _oracle_on(h1, _state("smooth"))
     before.step()
-    maps2 = amr_maps.build(h2)
-    tables = rg.transfer_tables(h1, h2, maps2)
-    after = OracleAMR2D(maps2, h2, regrid_state(before.buf, maps2, tables))
-    for L in range(after.nlev - 2, -1, -1):
-        after.average(after.buf, L)
-        after.restrict(after.buf, L)
+    after = regrid_state(before, _patches(h2))
     np.testing.assert_allclose(after.leaf_totals(), before.leaf_totals(), rtol=0, atol=1e-15)
     # div B on every patch and coarse nodes equal to the fine nodes on them
-    for lay in maps2.layouts:
-        az = after.views(after.buf, lay)[1][2:-2, 2:-2]
-        dx, dy = h2.spacing(lay.level)
-        bxf = (az[:, 1:] - az[:, :-1]) / dy
-        byf = -(az[1:, :] - az[:-1, :]) / dx
-        div = (bxf[1:] - bxf[:-1]) / dx + (byf[:, 1:] - byf[:, :-1]) / dy
-        assert np.abs(div).max() * dx < 1e-14
-    for L in range(after.nlev - 1):
-        rows = maps2.restrict[L]
-        assert np.array_equal(after.buf[rows[:, 0]], after.buf[rows[:, 1]])
+    assert after.divergence() < 1e-14
+    assert composite_mismatch(after) == 0.0
 
 
 def test_linear_data_moves_exactly_onto_new_patches():
     """A linear density and a bilinear potential are reproduced on the new level 2 to
     round-off: the prolongation and the node interpolation are exact for them."""
     _, h1, h2, _ = _two_level_regrid()
-    _, before = _oracle_on(h1, _state("linear"))
-    maps2 = amr_maps.build(h2)
-    moved = regrid_state(before.buf, maps2, rg.transfer_tables(h1, h2, maps2))
-    exact = amr_maps.pack_state(h2, maps2.layouts, maps2.total, 5.0 / 3.0, (0.0, 0.0),
-                                _state("linear"))
-    for lay in maps2.layouts:
-        if lay.level != 2:
+    before = _oracle_on(h1, _state("linear"))
+    moved = regrid_state(before, _patches(h2))
+    exact = _oracle_on(h2, _state("linear"))
+    for (L, _), (c1, a1), (c2, a2) in zip(moved.patches, moved.state, exact.state):
+        if L != 2:
             continue
-        o = OracleAMR2D(maps2, h2, moved)
-        e = OracleAMR2D(maps2, h2, exact)
-        (c1, a1), (c2, a2) = o.views(moved, lay), e.views(exact, lay)
         np.testing.assert_allclose(c1[0, 2:-2, 2:-2], c2[0, 2:-2, 2:-2], rtol=0, atol=1e-14)
         np.testing.assert_allclose(a1[2:-2, 2:-2], a2[2:-2, 2:-2], rtol=0, atol=1e-14)
+
+
+def test_a_planted_prolongation_slope_error_breaks_the_linear_move(monkeypatch):
+    """The linear-data check pins the prolongation: a child on the wrong side of its
+    coarse cell (the slope's sign flipped) is caught."""
+    import oracle.amr2d as OA
+
+    _, h1, h2, _ = _two_level_regrid()
+    before = _oracle_on(h1, _state("linear"))
+    monkeypatch.setattr(OA, "_minmod", lambda a, b: -np.where(a * b > 0.0, np.where(
+        np.abs(a) < np.abs(b), a, b), 0.0))
+    moved = regrid_state(before, _patches(h2))
+    exact = _oracle_on(h2, _state("linear"))
+    worst = max(np.abs(c1[0, 2:-2, 2:-2] - c2[0, 2:-2, 2:-2]).max()
+                for (L, _), (c1, _a1), (c2, _a2) in zip(moved.patches, moved.state, exact.state)
+                if L == 2)
+    assert worst > 1e-3
 
 
 # -- GPU ----------------------------------------------------------------------------------
@@ -188,22 +186,15 @@ def test_gpu_move_is_bitwise_the_oracle(gpu):
                   1: np.pad(np.ones((2, 6), bool), ((14, 16), (12, 14)))}):
         model.step(model.stable_step())
         old_h, old_buf = model.hierarchy, model.buffer()
+        old_oracle = OracleAMR2D(_patches(old_h), old_h.base_shape, old_h.bounds, old_buf,
+                                 limiter="minmod")
         tags = {L: t for L, t in tags.items() if L <= old_h.max_level}
         model.regrid(tags, max_patch_size=16)
-        maps = amr_maps.build(model.hierarchy)
-        want = regrid_state(old_buf, maps, rg.transfer_tables(old_h, model.hierarchy, maps))
-        o = OracleAMR2D(maps, model.hierarchy, want)
-        for L in range(o.nlev - 2, -1, -1):
-            o.average(o.buf, L)
-            o.restrict(o.buf, L)
-        got = model.buffer()
-        for lay in maps.layouts:  # the patches' own cells and nodes
-            sl = slice(lay.offset, lay.offset + 6 * lay.plane)
-            np.testing.assert_array_equal(got[sl].reshape(6, lay.X, lay.Y)[:, 2:-2, 2:-2],
-                                          o.buf[sl].reshape(6, lay.X, lay.Y)[:, 2:-2, 2:-2])
-            sl = slice(lay.offset + 6 * lay.plane, lay.offset + lay.size)
-            np.testing.assert_array_equal(got[sl].reshape(lay.X + 1, lay.Y + 1)[2:-2, 2:-2],
-                                          o.buf[sl].reshape(lay.X + 1, lay.Y + 1)[2:-2, 2:-2])
+        want = regrid_state(old_oracle, _patches(model.hierarchy))
+        got = unpack(model.buffer(), want.patches)
+        for (cells, az), (oc, oa) in zip(got, want.state):  # the patches' own cells and nodes
+            np.testing.assert_array_equal(cells[:, 2:-2, 2:-2], oc[:, 2:-2, 2:-2])
+            np.testing.assert_array_equal(az[2:-2, 2:-2], oa[2:-2, 2:-2])
     assert model.hierarchy.max_level == 2
 
 

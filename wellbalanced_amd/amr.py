@@ -1,18 +1,23 @@
 """``AMRMHD2D``: the 2D MHD step on a refined, periodic patch hierarchy.
 
 The reference runs ``MHD2D`` on patchwork patches of several levels that all take
-the same time step (``mhd.py:40-44``): halos from the hierarchy, fluxes per patch,
-the register update, A_z restriction after each stage, refluxing at the end
+the same time step (``mhd.py:40-44``): per stage the halos from the hierarchy, the
+fluxes of every patch and the register update with scale ``0.5 dt``; A_z restricted
+between the stages; after the Heun average one reflux and the restriction
 (``mhd.py:889-994``).  Here every patch's state sits in one FP64 buffer in HBM (two,
 ping-ponged as in `MHD2D`); a stage is
 
 1. the halo fill, level by level: same-level copies, then bilinear cells and
-   4-point nodes from the level below (gather tables of `amr_maps`);
+   quadratic A_z nodes from the level below (gather tables of `amr_maps`);
 2. ONE persistent launch over every tile of every patch (`k_stage2d_batched`: the
-   fused 2D stage, workgroups pulling (patch, tile, segment) items from a counter);
-3. per coarse/fine level pair, finest first: refluxing of the uncovered coarse cells
-   next to finer patches, the average of covered coarse cells, the restriction of
-   A_z.
+   fused 2D stage, workgroups pulling (patch, tile, segment) items from a counter),
+   which updates every patch cell, covered coarse cells included;
+3. the flux register of the coarse cells next to finer patches (one launch for every
+   level pair): stage 1 records half its correction, stage 2 (whose output is the Heun
+   average) adds the recorded half and its own;
+4. after stage 1 the restriction of A_z (coarse nodes on fine nodes take the fine
+   value); after stage 2 the average of covered coarse cells over their children and
+   the restriction, finest level pair first.
 
 dt comes from the largest fast-wave rate over every patch cell (the finest cells
 bind it) and, with resistivity or conduction, the explicit diffusion limit of each
@@ -147,6 +152,8 @@ class AMRMHD2D:
         rows_all = np.concatenate(rows_all) if rows_all else np.zeros((0, 13), np.int64)
         self._reflux = (self._dev(rows_all.reshape(-1)),
                         self._dev(np.array(groups, np.int32).reshape(-1), torch.int32), len(groups))
+        # the flux register: half the first stage's correction of each refluxed cell
+        self._reg = torch.zeros(6 * max(len(groups), 1), dtype=torch.float64, device=self.device)
         for L in range(self.nlev - 1):
             self._average[L] = (self._dev(m.average[L].reshape(-1)), len(m.average[L]))
             self._restrict[L] = (self._dev(m.restrict[L].reshape(-1)), len(m.restrict[L]))
@@ -278,6 +285,10 @@ class AMRMHD2D:
         _lib.call("wb_amr2d_coarsen", _lib.ptr(buf), _lib.ptr(arows), na, _lib.ptr(rrows), nr,
                   stream)
 
+    def _restrict_nodes(self, buf: torch.Tensor, L: int, stream: int) -> None:
+        rrows, nr = self._restrict[L]
+        _lib.call("wb_amr2d_coarsen", _lib.ptr(buf), None, 0, _lib.ptr(rrows), nr, stream)
+
     def _stage(self, src, dst, heun: bool, slot: int, stream: int) -> None:
         self._fill(src, stream)
         suffix = "" if self.precision == "fp64" else "_mixed"
@@ -289,13 +300,16 @@ class AMRMHD2D:
                   self._bad.data_ptr() + 4 * slot, int(self.nxs), int(self.tile),
                   int(self.workgroups), stream)
         rows, groups, n = self._reflux
-        if n:
+        if n:  # stage 1 records into the register, stage 2 refluxes (mhd.py:911-919,989)
             _lib.call("wb_amr2d_reflux" + suffix, _lib.ptr(src), _lib.ptr(dst),
                       _lib.ptr(self._patches), _lib.ptr(rows), _lib.ptr(groups), n,
-                      *self.mean_field, self.gamma, *diff, code, _lib.ptr(self._dt), int(heun),
-                      stream)
+                      *self.mean_field, self.gamma, *diff, code, _lib.ptr(self._dt),
+                      int(not heun), _lib.ptr(self._reg), stream)
         for L in range(self.nlev - 2, -1, -1):
-            self._coarsen(dst, L, stream)
+            if heun:  # the synchronisation after the Heun average (mhd.py:989-991)
+                self._coarsen(dst, L, stream)
+            else:  # A_z between the stages (mhd.py:981)
+                self._restrict_nodes(dst, L, stream)
 
     @property
     def _diffusive(self) -> bool:

@@ -15,9 +15,11 @@ NumPy and consumed by gather kernels:
 * ``cell_interp``: the remaining ghost cells, bilinear from the 2x2 nearest coarse
   cells (weights 1/4, 3/4 per axis) for each of the 6 fields;
 * ``node_interp``: the remaining ghost nodes, per axis the coarse node itself
-  (even fine index) or the 4-point midpoint rule (-1, 9, 9, -1)/16 (odd index);
+  (even fine index) or, midway between coarse nodes m and m + 1 (odd index), the
+  quadratic through nodes m - 1, m, m + 1 (weights -1/8, 3/4, 3/8);
 * ``reflux``: per uncovered coarse cell next to a finer patch, up to four faces
-  whose coarse flux is replaced by the mean of the two fine fluxes;
+  whose coarse flux (of both Heun stages) is traded for the mean of the two fine
+  fluxes once per step;
 * ``average``: covered coarse cells, the mean of their four fine children;
 * ``restrict``: coarse nodes that coincide with fine nodes take the fine value.
 
@@ -225,12 +227,12 @@ def build(hierarchy) -> Maps:
         if miss.any():
             mk, ma, mb, mfi, mfj = k[miss], a[miss], b[miss], fi[miss], fj[miss]
             # per axis: even -> coarse node fi/2 (stencil base fi/2, kind 0);
-            # odd -> 4-point rule on nodes m-1..m+2, m = (fi-1)/2 (base m-1, kind 1)
+            # odd -> quadratic on nodes m-1..m+1, m = (fi-1)/2 (base m-1, kind 1)
             kx = (mfi % 2 == 1).astype(np.int64)
             ky = (mfj % 2 == 1).astype(np.int64)
             bx = np.where(kx == 1, (mfi - 1) // 2 - 1, mfi // 2)
             by = np.where(ky == 1, (mfj - 1) // 2 - 1, mfj // 2)
-            q, qa, qb = _coarse_container(layouts, L - 1, bx, by, 4, hierarchy, cells=False,
+            q, qa, qb = _coarse_container(layouts, L - 1, bx, by, 3, hierarchy, cells=False,
                                           owner=owners[L - 1][0], boxes=boxes)
             maps.node_interp[L] = (node(mk, ma, mb), node(q, qa, qb), YN[q],
                                    np.ones(q.size, np.int64), kx, ky)

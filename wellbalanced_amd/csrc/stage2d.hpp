@@ -398,17 +398,22 @@ __device__ void face_flux2d(const double* in, int X, int Y, double dx, double dy
   out7[6] = double(P(b6));
 }
 
-// Refluxing (patchwork FluxRegister.reflux, mhd.py:989-990): per uncovered coarse
-// cell next to finer patches, the coarse face fluxes it was updated with are
-// replaced by the mean of the two fine face fluxes.  rows (13 int64 each):
-// q, ca, cb, axis, side, face_a, face_b, p0, fa0, fb0, p1, fa1, fb1; groups[g] =
-// (first row, row count) of coarse cell g.  Deterministic: one thread per cell.
+// The flux register and refluxing (patchwork FluxRegister.update/reflux as MHD2D.step
+// calls them, mhd.py:911-919,989-990): per uncovered coarse cell next to finer patches,
+// d = sum over its coarse-fine faces (x-high, x-low, y-high, y-low) of dt/h (F_c - mean)
+// (high face) or dt/h (mean - F_c) (low face), mean = 0.5 (F_f0 + F_f1), the fluxes
+// recomputed from the stage input.  record (stage 1): reg = 0.5 d; otherwise (stage 2,
+// after the Heun average): cell += reg + 0.5 d, the coarse fluxes of both stages traded
+// for the fine ones at the 0.5 dt weight each stage recorded them with.  rows (13 int64
+// each): q, ca, cb, axis, side, face_a, face_b, p0, fa0, fb0, p1, fa1, fb1; groups[g] =
+// (first row, row count) of coarse cell g; reg holds 6 doubles per group.
+// Deterministic: one thread per cell.
 template <typename P, int LIM>
 __global__ void __launch_bounds__(64) k_reflux2d(const double* __restrict__ src, double* __restrict__ dst,
                            const PatchDesc* __restrict__ patches, const long long* __restrict__ rows,
                            const int* __restrict__ groups, int ngroups, double b0x, double b0y,
                            double gamma, int limiter, Diffusion diff, const double* __restrict__ dtp,
-                           int heun) {
+                           int record, double* __restrict__ reg) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ngroups) return;
   constexpr int XR[6] = {0, 1, 2, 3, 4, 6};
@@ -440,13 +445,15 @@ __global__ void __launch_bounds__(64) k_reflux2d(const double* __restrict__ src,
       delta[f] = delta[f] + d;
     }
   }
+  double* kept = reg + 6 * (size_t)g;
+  if (record) {
+    for (int f = 0; f < 6; ++f) kept[f] = 0.5 * delta[f];
+    return;
+  }
   const PatchDesc cq = patches[q];
   const size_t PL = (size_t)(cq.nx + 4) * (cq.ny + 4);
   const size_t c = cq.off + (size_t)ca * (cq.ny + 4) + cb;
-  for (int f = 0; f < 6; ++f) {
-    const double d = heun ? 0.5 * delta[f] : delta[f];
-    dst[c + f * PL] = dst[c + f * PL] + d;
-  }
+  for (int f = 0; f < 6; ++f) dst[c + f * PL] = dst[c + f * PL] + (kept[f] + 0.5 * delta[f]);
 }
 
 template <typename P, int TY, int LIM = -1, bool NI = false>

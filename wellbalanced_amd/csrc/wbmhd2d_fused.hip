@@ -91,13 +91,14 @@ extern "C" int wb_amr2d_stage(const double* in, double* out, const void* patches
 extern "C" int wb_amr2d_reflux(const double* src, double* dst, const void* patches,
                                const long long* rows, const int32_t* groups, int ngroups,
                                double b0x, double b0y, double gamma, double eta, double kpar,
-                               double kperp, int limiter, const double* dt, int heun,
-                               void* stream) {
+                               double kperp, int limiter, const double* dt, int record,
+                               double* reg, void* stream) {
   if (!(eta >= 0.0) || !(kpar >= 0.0) || !(kperp >= 0.0)) return WB_EINVAL;
+  if (ngroups > 0 && reg == nullptr) return WB_EINVAL;
   if (ngroups <= 0) return 0;
   hipLaunchKernelGGL((k_reflux2d<double, -1>), dim3((unsigned)((ngroups + 63) / 64)), dim3(64), 0,
                      (hipStream_t)stream, src, dst, (const PatchDesc*)patches, rows,
                      (const int*)groups, ngroups, b0x, b0y, gamma, limiter,
-                     Diffusion{eta, kpar, kperp}, dt, heun);
+                     Diffusion{eta, kpar, kperp}, dt, record, reg);
   return wb_last_error();
 }

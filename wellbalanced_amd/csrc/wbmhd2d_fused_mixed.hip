@@ -78,15 +78,16 @@ extern "C" int wb_amr2d_reflux_mixed(const double* src, double* dst, const void*
                                      const long long* rows, const int32_t* groups, int ngroups,
                                      double b0x, double b0y, double gamma, double eta,
                                      double kpar, double kperp, int limiter, const double* dt,
-                                     int heun, void* stream) {
+                                     int record, double* reg, void* stream) {
   if (!(eta >= 0.0) || !(kpar >= 0.0) || !(kperp >= 0.0)) return WB_EINVAL;
+  if (ngroups > 0 && reg == nullptr) return WB_EINVAL;
   if (ngroups <= 0) return 0;
   return with_limiter(limiter, [&](auto lim) {
     constexpr int L = decltype(lim)::value;
     hipLaunchKernelGGL((k_reflux2d<float, L>), dim3((unsigned)((ngroups + 63) / 64)), dim3(64), 0,
                        (hipStream_t)stream, src, dst, (const PatchDesc*)patches, rows,
                        (const int*)groups, ngroups, b0x, b0y, gamma, limiter,
-                       Diffusion{eta, kpar, kperp}, dt, heun);
+                       Diffusion{eta, kpar, kperp}, dt, record, reg);
     return wb_last_error();
   });
 }

@@ -161,10 +161,11 @@ struct FillLevel {
 // threads each: ghost cells (6 fields) and ghost or shared boundary A_z nodes copied
 // from a same-level owner; the other ghost cells bilinear from the 2x2 nearest coarse
 // cells (wx1, wy1: weights of the upper cells); the other ghost nodes, per axis the
-// coarse node itself (kind 0) or the 4-point midpoint rule (-1, 9, 9, -1) / 16 on the
-// nodes base .. base+3 (kind 1), a tensor product.  The parts write disjoint ghosts and
-// read only interior values of the level or values of the level below, so they need no
-// order among themselves.
+// coarse node itself (kind 0) or, midway between coarse nodes m and m+1, the quadratic
+// through nodes m-1, m, m+1, weights (-1/8, 3/4, 3/8) on base .. base+2 (kind 1), a
+// tensor product (the reference's quadratic potential halos, CHANGELOG.md:10-13).  The
+// parts write disjoint ghosts and read only interior values of the level or values of
+// the level below, so they need no order among themselves.
 __global__ void __launch_bounds__(256) k_fill_level(double* __restrict__ buf, FillLevel t) {
   int b = blockIdx.x;
   const int b0 = nblocks(t.ncells), b1 = nblocks(t.nnodes), b2 = nblocks(t.nic);
@@ -201,16 +202,16 @@ __global__ void __launch_bounds__(256) k_fill_level(double* __restrict__ buf, Fi
   b -= b2;
   const int i = b * 256 + tid;
   if (i >= t.nin) return;
-  const double w4[4] = {-0.0625, 0.5625, 0.5625, -0.0625};
-  const int nxw = t.jkx[i] ? 4 : 1, nyw = t.jky[i] ? 4 : 1;
+  const double w3[3] = {-0.125, 0.75, 0.375};
+  const int nxw = t.jkx[i] ? 3 : 1, nyw = t.jky[i] ? 3 : 1;
   double total = 0.0;
   for (int j = 0; j < nyw; ++j) {
     double row = 0.0;
     for (int k = 0; k < nxw; ++k) {
-      const double wk = t.jkx[i] ? w4[k] : 1.0;
+      const double wk = t.jkx[i] ? w3[k] : 1.0;
       row = row + wk * buf[t.jbase[i] + k * t.jsx[i] + j * t.jsy[i]];
     }
-    const double wj = t.jky[i] ? w4[j] : 1.0;
+    const double wj = t.jky[i] ? w3[j] : 1.0;
     total = total + wj * row;
   }
   buf[t.jdst[i]] = total;

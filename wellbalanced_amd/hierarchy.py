@@ -256,7 +256,7 @@ class AMRHierarchy:
         """Fill the ghost layers of ``name`` on every patch.  A model's fields are
         filled together by the model's halo kernels (periodic copies, coarse-fine
         interpolation, outflow and wall rows); ``interpolation`` is then fixed by the
-        model (bilinear cells, 4-point nodes)."""
+        model (bilinear cells, quadratic nodes)."""
         model = self._spec(name).model()
         if model is None:
             raise NotImplementedError(
