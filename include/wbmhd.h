@@ -440,10 +440,11 @@ int wb_amr3d_fill_regions(double* buf, const void* blocks, const long long* hood
                           const int32_t* tasks, int ntasks, void* stream);
 
 /* One level's coarsening: the covered cells of each refined block := the mean of its
- * 8 children, and its edges := the mean of the two fine edges along them (every coarse
- * copy); children: rows of 9 int32 (the block, then its 8 children, x slowest). */
+ * 8 children (cells = 1 only), and its edges := the mean of the two fine edges along
+ * them (every coarse copy; the restriction of the potential, mhd3d.py:740-751);
+ * children: rows of 9 int32 (the block, then its 8 children, x slowest). */
 int wb_amr3d_coarsen(double* buf, const void* blocks, const int32_t* children, int nrefined,
-                     const int32_t* lookup, const void* levels, void* stream);
+                     int cells, const int32_t* lookup, const void* levels, void* stream);
 
 /* The CFL rate over every block cell (mhd3d.py:674-724): out = device double[2]. */
 int wb_amr3d_rate(const double* buf, const void* blocks, int nblocks, double b0x, double b0y,
@@ -470,8 +471,13 @@ int wb_amr3d_stage_mixed(const double* in, double* out, const void* blocks, cons
                          double b0x, double b0y, double b0z, double gamma, int limiter,
                          const double* dt, int heun, int32_t* bad, int waves, void* stream);
 
-/* Refluxing after a stage, every coarse-fine block face of every level in one launch
- * (FluxRegister.reflux, mhd3d.py:745-750).  items: {int32 fine, coarse, ci, cj, ck, dir}
+/* The flux register and refluxing, every coarse-fine block face of every level in one
+ * launch (FluxRegister.update with scale 0.5 dt and FluxRegister.reflux,
+ * mhd3d.py:650-655,745-750): per listed coarse cell t, d = the sum over its faces of
+ * dt/h (coarse flux - mean of the four fine fluxes), signed by the side, recomputed
+ * from the stage input `src`; record != 0 (after stage 1): reg[5 t + f] = 0.5 d;
+ * record == 0 (after stage 2 and its Heun average): the cell of `dst` gains
+ * reg + 0.5 d.  reg: 5 doubles per listed cell, caller-owned.  items: {int32 fine, coarse, ci, cj, ck, dir}
  * per coarse-fine block face (dir = 2 axis + side; side 0: the coarse cells at the fine
  * block's low face, 1: at its high face); cells: {int32 n, item[3], lane[3], pad} per
  * coarse cell with such faces, its n faces as (item, lane = u * 8 + v inside the item's
@@ -481,12 +487,12 @@ int wb_amr3d_stage_mixed(const double* in, double* out, const void* blocks, cons
  * from the blocks' filled ghosts). */
 int wb_amr3d_reflux(const double* src, double* dst, const void* blocks, const void* items,
                     const void* cells, int ncells, const long long* hoods, double b0x,
-                    double b0y, double b0z, double gamma, int limiter, const double* dt, int heun,
-                    void* stream);
+                    double b0y, double b0z, double gamma, int limiter, const double* dt,
+                    int record, double* reg, void* stream);
 int wb_amr3d_reflux_mixed(const double* src, double* dst, const void* blocks, const void* items,
                           const void* cells, int ncells, const long long* hoods, double b0x,
                           double b0y, double b0z, double gamma, int limiter, const double* dt,
-                          int heun, void* stream);
+                          int record, double* reg, void* stream);
 
 #ifdef __cplusplus
 }

@@ -4,9 +4,11 @@ Restates this package's device algorithm for ``MHD3D.step`` on a patchwork
 hierarchy (``mhd3d.py:635-671,726-754``) on the geometry and tables of
 ``wellbalanced_amd.amr3d_maps``: ghost fill (copies from the block of the same level
 holding a cell or edge, else interpolation from the level below), the stage of every
-block (``Oracle3D.fluxes`` on the haloed block and this package's update form),
-refluxing of the six directions in turn, average-down of covered cells and the
-restriction of the potential.  patchwork's own operation order is not available,
+block (``Oracle3D.fluxes`` on the haloed block and this package's update form), the
+flux register of both stages (the corrections of the six directions summed per coarse
+cell; stage 1 keeps half), the potential restricted between the stages, and after the
+Heun average one reflux, the average-down of covered cells and the restriction of the
+potential (``mhd3d.py:739-751``).  patchwork's own operation order is not available,
 so these rules are parity unpinned; the invariants in the tests pin them.
 """
 
@@ -245,12 +247,23 @@ class OracleAMR3D:
                 if heun:
                     v = (v + opots[c][idx]) * 0.5
                 opots[c][idx] = v
+        acc = {}
         for d in range(6):
-            self.reflux(sweeps, dst, dt, heun, d // 2, d % 2)
+            self.reflux(sweeps, acc, dt, d // 2, d % 2)
+        if not heun:  # the register keeps half the first stage's correction
+            self.kept = {key: 0.5 * v for key, v in acc.items()}
+        else:  # the reflux after the Heun average (mhd3d.py:745-750)
+            for (coarse, c), v in acc.items():
+                cells, _ = self.views(dst, coarse)
+                r = self.kept[coarse, c]
+                for fld in range(5):
+                    cells[fld][c] = cells[fld][c] + (r[fld] + 0.5 * v[fld])
         for L in range(self.nlev - 2, -1, -1):
-            self.coarsen(dst, L)
+            self.coarsen(dst, L, cells=heun)
 
-    def reflux(self, sweeps, dst, dt, heun, axis, side):
+    def reflux(self, sweeps, acc, dt, axis, side):
+        """Add the corrections of the coarse-fine faces of one direction to ``acc``
+        (per coarse block and haloed cell: 5 values, summed in direction order)."""
         B = self.B
         a1, a2 = [a for a in range(3) if a != axis]
         xrow, yrow, zrow = (0, 1, 2, 3, 4), (0, 3, 1, 2, 4), (0, 2, 3, 1, 4)
@@ -258,7 +271,6 @@ class OracleAMR3D:
         for item in self.t.reflux[2 * axis + side]:
             fine, coarse = int(item["fine"]), int(item["coarse"])
             origin = np.array([item["ci"], item["cj"], item["ck"]]) + 2
-            cells, _ = self.views(dst, coarse)
             dtd = dt / self.spacing(coarse)[axis]
             for u in range(8):
                 for v in range(8):
@@ -280,12 +292,15 @@ class OracleAMR3D:
                             fi[a2] = 2 + 2 * v + q
                             ff[p, q] = sweeps[fine][axis][(slice(None), *fi)]
                     fm = ((ff[0, 0] + ff[0, 1]) + (ff[1, 0] + ff[1, 1])) * 0.25
+                    total = acc.setdefault((coarse, tuple(c)), np.zeros(5))
                     for fld in range(5):
                         r = rows[fld]
                         d = dtd * (fc[r] - fm[r]) if side == 0 else dtd * (fm[r] - fc[r])
-                        cells[fld][tuple(c)] = cells[fld][tuple(c)] + (0.5 * d if heun else d)
+                        total[fld] = total[fld] + d
 
-    def coarsen(self, buf, L):
+    def coarsen(self, buf, L, cells=True):
+        """The covered cells := the mean of their children (``cells``) and the coarse
+        edges := the mean of the two fine edges along them."""
         B = self.B
         level = self.t.blocks["level"]
         for row in self.t.children:
@@ -294,7 +309,7 @@ class OracleAMR3D:
             pc, pp = self.views(buf, row[0])
             kids = [self.views(buf, k) for k in row[1:]]
             # cells: the mean of 8 children
-            for oct_ in range(8):
+            for oct_ in range(8 if cells else 0):
                 ox, oy, oz = (oct_ >> 2) & 1, (oct_ >> 1) & 1, oct_ & 1
                 kc = kids[oct_][0]
                 f = lambda a, b, c: kc[:, 2 + a : 2 + B : 2, 2 + b : 2 + B : 2, 2 + c : 2 + B : 2]  # noqa: E731
@@ -341,6 +356,7 @@ class OracleAMR3D:
     def prepare(self):
         """As the model does after loading a state: coarsen, finest level first, then
         fill every ghost."""
+        self.kept = {}
         for L in range(self.nlev - 2, -1, -1):
             self.coarsen(self.buf, L)
         self.fill(self.buf)

@@ -116,9 +116,12 @@ def test_new_entry_points_reject_bad_arguments_before_any_launch():
     for name in ("wb_amr3d_reflux", "wb_amr3d_reflux_mixed"):
         reflux = getattr(lib, name)
         assert reflux(None, None, None, None, None, -1, None, 0.0, 0.0, 0.0, 5 / 3, 0, None, 0,
-                      None) == E
+                      None, None) == E
         assert reflux(None, None, None, None, None, 0, None, 0.0, 0.0, 0.0, 5 / 3, 0, None, 0,
-                      None) == 0
+                      None, None) == 0
+        # a register is needed when there is a cell to reflux
+        assert reflux(None, None, None, None, None, 1, None, 0.0, 0.0, 0.0, 5 / 3, 0, None, 1,
+                      None, None) == E
     for name in ("wb_amr3d_rate", "wb_amr3d_rate_mixed"):
         assert getattr(lib, name)(None, None, 0, 0.0, 0.0, 0.0, 5 / 3, None, None) == E
 

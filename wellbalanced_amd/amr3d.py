@@ -2,9 +2,11 @@
 
 `AMRMHD3D` runs ``MHD3D.step`` (``mhd3d.py:726-754``) on every block of every level
 of a `wellbalanced_amd.amr3d_maps.Hierarchy3D`: one time step for all levels, the
-stage of every block in one launch, refluxing at coarse-fine faces, the covered
-cells averaged down and the potential restricted (the mean of the fine edges along
-a coarse edge, ``mhd3d.py:31-36``) so that div B is zero on the composite mesh.
+stage of every block in one launch, the flux register of the coarse-fine faces over
+both stages, the potential restricted between the stages (the mean of the fine edges
+along a coarse edge, ``mhd3d.py:31-36``), and after the Heun average one reflux, the
+covered cells averaged down and the potential restricted, so that div B is zero on
+the composite mesh.
 The stage and the refluxing read the cells around a block in place from the blocks
 of its level (``direct``); the ghosts its level has no block for are interpolated from
 the level below (``wb_amr3d_fill_regions``; without ``direct``, ``wb_amr3d_fill``
@@ -86,6 +88,8 @@ class AMRMHD3D:
         self._reflux_items = self._bytes(t.reflux_items) if len(t.reflux_items) else self._blocks
         self._reflux_cells = self._bytes(t.reflux_cells) if len(t.reflux_cells) else self._blocks
         self.nreflux = len(t.reflux_cells)
+        # the flux register: half the first stage's correction of each refluxed cell
+        self._reg = torch.zeros(5 * max(self.nreflux, 1), dtype=torch.float64, device=self.device)
         self._hoods = torch.as_tensor(t.hoods, device=self.device)
         # the blocks with a missing neighbour on their level: with direct reads, the only
         # ones whose ghosts (those the level has no block for) are filled
@@ -191,11 +195,12 @@ class AMRMHD3D:
         _lib.call("wb_amr3d_fill", _lib.ptr(buf), _lib.ptr(self._blocks), self.nblocks,
                   _lib.ptr(self._lookup), _lib.ptr(self._levels), stream)
 
-    def _coarsen(self, buf: torch.Tensor, L: int, stream: int) -> None:
+    def _coarsen(self, buf: torch.Tensor, L: int, stream: int, cells: bool = True) -> None:
+        """The covered cells averaged (``cells``) and the edges restricted."""
         rows, n = self._children[L]
         if n:
             _lib.call("wb_amr3d_coarsen", _lib.ptr(buf), _lib.ptr(self._blocks), _lib.ptr(rows), n,
-                      _lib.ptr(self._lookup), _lib.ptr(self._levels), stream)
+                      int(cells), _lib.ptr(self._lookup), _lib.ptr(self._levels), stream)
 
     def _stage(self, src, dst, heun: bool, slot: int, stream: int, marks=None) -> None:
         # direct reads: the stage and the refluxing read the cells around a block in
@@ -217,14 +222,17 @@ class AMRMHD3D:
                   int(heun), self._bad.data_ptr() + 4 * slot, *extra, stream)
         if marks is not None:
             marks[1].record()
-        if self.nreflux:
+        if self.nreflux:  # stage 1 records into the register, stage 2 refluxes
             _lib.call("wb_amr3d_reflux" + suffix, _lib.ptr(src), _lib.ptr(dst),
                       _lib.ptr(self._blocks), _lib.ptr(self._reflux_items),
                       _lib.ptr(self._reflux_cells), self.nreflux,
                       _lib.ptr(self._hoods) if self.direct else None, *self.mean_field,
-                      self.gamma, code, _lib.ptr(self._dt), int(heun), stream)
+                      self.gamma, code, _lib.ptr(self._dt), int(not heun), _lib.ptr(self._reg),
+                      stream)
+        # after stage 1 the potential is restricted (mhd3d.py:740-741), after the Heun
+        # average the covered cells are averaged down and the potential restricted
         for L in range(self.nlev - 2, -1, -1):
-            self._coarsen(dst, L, stream)
+            self._coarsen(dst, L, stream, cells=heun)
 
     def _heun(self, stream: int, events=None):
         marks = None

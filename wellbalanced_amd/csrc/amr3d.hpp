@@ -197,12 +197,15 @@ __device__ void face_flux3d(const StageArgs& a, int axis, int i, int j, int k, d
   for (int r = 0; r < 7; ++r) out[r] = double(fl[r]);
 }
 
-// Refluxing after a stage (FluxRegister.reflux as MHD3D.step calls it,
-// mhd3d.py:745-750): per coarse-fine block face, the 8 x 8 coarse cells across it
-// replace the flux of their face on the fine side by the mean of the four fine fluxes,
-// both recomputed from the stage input `src` (heun: half of the correction, as the
-// stage's average).  An item is one such face (dir = 2 axis + side; side 0: the coarse
-// cells at the fine block's low face, their +axis face; side 1: at its high face).
+// The flux register and refluxing (FluxRegister.update with scale 0.5 dt and
+// FluxRegister.reflux as MHD3D.step calls them, mhd3d.py:650-655,745-750): per
+// coarse-fine block face, the 8 x 8 coarse cells across it trade the flux of their face
+// on the fine side for the mean of the four fine fluxes, both recomputed from the stage
+// input `src`: d = sum over the cell's faces of dt/h (F_c - mean) or dt/h (mean - F_c).
+// record (after stage 1): reg[5 t + f] = 0.5 d; otherwise (after stage 2 and its Heun
+// average) the cell gains reg + 0.5 d.  An item is one such face (dir = 2 axis + side;
+// side 0: the coarse cells at the fine block's low face, their +axis face; side 1: at
+// its high face).
 //
 // A workgroup of 256 threads per REFLUX_CELLS = 51 coarse cells that have such faces:
 // thread 5 m + k takes flux k of cell m, k = 0 the coarse flux of the cell's face and
@@ -212,8 +215,8 @@ __device__ void face_flux3d(const StageArgs& a, int axis, int i, int j, int k, d
 // (1 to 3) faces as (item, lane) pairs in increasing direction order, the lane u * 8 + v
 // naming the cell inside the item's 8 x 8 face (the host puts the cells with more faces
 // first, so that a workgroup's cells share one face count).  The coarse flux's thread
-// applies the corrections of a cell's faces in that order, which is the order of one
-// pass per direction, with no atomics on the state (deterministic).  `hoods`: per block
+// sums the corrections of a cell's faces in that order, with no atomics on the state
+// (deterministic).  `hoods`: per block
 // the 27 offsets of the blocks of its level around it (-1: none) for reads in place, or
 // null to read the filled ghosts.
 constexpr int REFLUX_CELLS = 51;
@@ -227,7 +230,8 @@ __global__ void __launch_bounds__(256) k_reflux3d(StageArgs base, const BlockDes
                                                   const RefluxItem* __restrict__ items,
                                                   const RefluxCell* __restrict__ cells, int ncells,
                                                   const long long* __restrict__ hoods,
-                                                  double* __restrict__ dst) {
+                                                  double* __restrict__ dst, int record,
+                                                  double* __restrict__ reg) {
   __shared__ double fine[4][7][REFLUX_CELLS];
   __shared__ int most;
   const int m = threadIdx.x / 5, kf = threadIdx.x % 5;
@@ -239,11 +243,13 @@ __global__ void __launch_bounds__(256) k_reflux3d(StageArgs base, const BlockDes
   __syncthreads();
   const int nmax = most;
   const double dt = *base.dt;
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  int c[3] = {0, 0, 0};
+  BlockDesc cd{};
   for (int f = 0; f < nmax; ++f) {
     const bool has = f < n;
-    int axis = 0, side = 0, c[3] = {0, 0, 0};
+    int axis = 0, side = 0;
     double fl[7];
-    BlockDesc cd{};
     if (has) {
       const RefluxItem item = items[cells[t].item[f]];
       const int lc = cells[t].lane[f];
@@ -282,17 +288,26 @@ __global__ void __launch_bounds__(256) k_reflux3d(StageArgs base, const BlockDes
       constexpr int xrow[5] = {0, 1, 2, 3, 4};
       constexpr int yrow[5] = {0, 3, 1, 2, 4};
       constexpr int zrow[5] = {0, 2, 3, 1, 4};
-      double* out = dst + cd.off;
       for (int fld = 0; fld < 5; ++fld) {
         const int row = axis == 0 ? xrow[fld] : (axis == 1 ? yrow[fld] : zrow[fld]);
         const double fm =
             ((fine[0][row][m] + fine[1][row][m]) + (fine[2][row][m] + fine[3][row][m])) * 0.25;
         const double d = side == 0 ? dtd * (fl[row] - fm) : dtd * (fm - fl[row]);
-        const int o = cell_at(fld, c[0], c[1], c[2]);
-        out[o] = out[o] + (base.heun ? 0.5 * d : d);
+        acc[fld] = acc[fld] + d;
       }
     }
     __syncthreads();
+  }
+  if (n == 0 || kf != 0) return;
+  double* kept = reg + 5 * (size_t)t;
+  if (record) {
+    for (int fld = 0; fld < 5; ++fld) kept[fld] = 0.5 * acc[fld];
+    return;
+  }
+  double* out = dst + cd.off;
+  for (int fld = 0; fld < 5; ++fld) {
+    const int o = cell_at(fld, c[0], c[1], c[2]);
+    out[o] = out[o] + (kept[fld] + 0.5 * acc[fld]);
   }
 }
 

@@ -469,14 +469,16 @@ __device__ __forceinline__ void restrict3d(double* __restrict__ buf, const long 
 
 // One level pair's coarsening in one launch: per refined block (blockIdx.x), 8
 // workgroups average its cells (one per child), then 3 x RESTRICT_PARTS restrict its
-// edges; the two write disjoint data (cells, edges) from disjoint data.
+// edges; the two write disjoint data (cells, edges) from disjoint data.  first = 8
+// (a grid of 3 x RESTRICT_PARTS) restricts the edges only.
 __global__ void __launch_bounds__(256) k_coarsen3d(double* __restrict__ buf,
                                                    const BlockDesc* __restrict__ blocks,
                                                    const int* __restrict__ children,
-                                                   const int* __restrict__ lookup, Levels lv) {
+                                                   const int* __restrict__ lookup, Levels lv,
+                                                   int first) {
   __shared__ long long hood[27], kids[8];
   const int* row = children + 9 * blockIdx.x;
-  const int y = blockIdx.y;
+  const int y = blockIdx.y + first;
   if (y < 8) {
     average3d(buf, blocks, row, y);
     return;
@@ -559,13 +561,15 @@ int wb_amr3d_fill_regions(double* buf, const void* blocks, const long long* hood
 }
 
 int wb_amr3d_coarsen(double* buf, const void* blocks, const int32_t* children, int nrefined,
-                     const int32_t* lookup, const void* levels, void* stream) {
+                     int cells, const int32_t* lookup, const void* levels, void* stream) {
   const Levels lv = *(const Levels*)levels;
-  if (nrefined < 0 || !levels_ok(lv)) return WB_EINVAL;
+  if (nrefined < 0 || !levels_ok(lv) || cells < 0 || cells > 1) return WB_EINVAL;
   if (nrefined == 0) return 0;
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_coarsen3d, dim3(nrefined, 8 + 3 * RESTRICT_PARTS), dim3(256), 0, s, buf,
-                     (const BlockDesc*)blocks, (const int*)children, (const int*)lookup, lv);
+  const int first = cells ? 0 : 8;
+  hipLaunchKernelGGL(k_coarsen3d, dim3(nrefined, 8 + 3 * RESTRICT_PARTS - first), dim3(256), 0, s,
+                     buf, (const BlockDesc*)blocks, (const int*)children, (const int*)lookup, lv,
+                     first);
   return wb_last_error();
 }
 
@@ -610,15 +614,15 @@ int wb_amr3d_stage(const double* in, double* out, const void* blocks, const int3
 
 int wb_amr3d_reflux(const double* src, double* dst, const void* blocks, const void* items,
                     const void* cells, int ncells, const long long* hoods, double b0x,
-                    double b0y, double b0z, double gamma, int limiter, const double* dt, int heun,
-                    void* stream) {
-  if (ncells < 0) return WB_EINVAL;
+                    double b0y, double b0z, double gamma, int limiter, const double* dt,
+                    int record, double* reg, void* stream) {
+  if (ncells < 0 || (ncells > 0 && reg == nullptr)) return WB_EINVAL;
   if (ncells == 0) return 0;
   StageArgs a;
-  wb_amr3d_base_args(&a, src, dst, b0x, b0y, b0z, gamma, limiter, dt, heun, nullptr);
+  wb_amr3d_base_args(&a, src, dst, b0x, b0y, b0z, gamma, limiter, dt, 0, nullptr);
   hipLaunchKernelGGL((k_reflux3d<double, -1>), dim3((unsigned)((ncells + REFLUX_CELLS - 1) / REFLUX_CELLS)), dim3(256), 0,
                      (hipStream_t)stream, a, (const BlockDesc*)blocks, (const RefluxItem*)items,
-                     (const RefluxCell*)cells, ncells, hoods, dst);
+                     (const RefluxCell*)cells, ncells, hoods, dst, record, reg);
   return wb_last_error();
 }
 

@@ -50,17 +50,17 @@ int wb_amr3d_stage_mixed(const double* in, double* out, const void* blocks, cons
 int wb_amr3d_reflux_mixed(const double* src, double* dst, const void* blocks, const void* items,
                           const void* cells, int ncells, const long long* hoods, double b0x,
                           double b0y, double b0z, double gamma, int limiter, const double* dt,
-                          int heun, void* stream) {
-  if (ncells < 0) return WB_EINVAL;
+                          int record, double* reg, void* stream) {
+  if (ncells < 0 || (ncells > 0 && reg == nullptr)) return WB_EINVAL;
   if (ncells == 0) return 0;
   StageArgs a;
-  wb_amr3d_base_args(&a, src, dst, b0x, b0y, b0z, gamma, limiter, dt, heun, nullptr);
+  wb_amr3d_base_args(&a, src, dst, b0x, b0y, b0z, gamma, limiter, dt, 0, nullptr);
   hipStream_t s = (hipStream_t)stream;
   return with_limiter(limiter, [&](auto lim) {
     constexpr int L = decltype(lim)::value;
     hipLaunchKernelGGL((k_reflux3d<float, L>), dim3((unsigned)((ncells + REFLUX_CELLS - 1) / REFLUX_CELLS)), dim3(256), 0, s,
                        a, (const BlockDesc*)blocks, (const RefluxItem*)items,
-                       (const RefluxCell*)cells, ncells, hoods, dst);
+                       (const RefluxCell*)cells, ncells, hoods, dst, record, reg);
     return wb_last_error();
   });
 }
