@@ -22,6 +22,8 @@ extern "C" {
 #endif
 
 #define WB_EINVAL (-100000)
+/* The input needs more scratch than an allocation-free entry point's static buffer. */
+#define WB_ESCRATCH (-100001)
 
 /* ---- 2D per-kernel twins: same arguments as the reference's kernels ------ */
 
@@ -397,6 +399,21 @@ int64_t wb_mhd3d_resistive_workspace(int64_t nx, int64_t ny, int64_t nz);
 int wb_mhd3d_resistive(const double* in, double* out, int64_t nx, int64_t ny, int64_t nz,
                        double dx, double dy, double dz, double b0x, double b0y, double b0z,
                        double eta, const double* dt, int heun, double* work, void* stream);
+
+/* No counterpart in the reference: wb_mhd3d_resistive for callers that cannot
+ * allocate (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  The same kernels and results, with the workspace in a static device
+ * array of wb_mhd3d_resistive_noalloc_limit() doubles (16 MiB per GPU, held until the
+ * process ends; grids up to 64^3); a grid whose workspace is larger returns
+ * WB_ESCRATCH.  The array's address on each device is looked up once, on the first
+ * call with that device current (the stream must belong to the current device).  No
+ * hipMalloc or hipFree.  Every call on one GPU shares the array: calls that overlap on
+ * one GPU (concurrent streams) race.  Nothing in it needs to start at zero. */
+int64_t wb_mhd3d_resistive_noalloc_limit(void);
+int wb_mhd3d_resistive_noalloc(const double* in, double* out, int64_t nx, int64_t ny,
+                               int64_t nz, double dx, double dy, double dz, double b0x,
+                               double b0y, double b0z, double eta, const double* dt, int heun,
+                               void* stream);
 
 /* ---- 3D hierarchy of equal blocks (refined, periodic) ---------------------------- */
 /* Every level is a set of blocks of 16^3 cells with a halo of 2, each stored as the

@@ -12,6 +12,7 @@ from pathlib import Path
 
 LIBRARY = Path(__file__).resolve().parent / "libwbmhd.so"
 WB_EINVAL = -100000
+WB_ESCRATCH = -100001
 
 P = c_void_p
 I64 = c_int64
@@ -37,6 +38,8 @@ SIGNATURES: dict[str, tuple[type, list]] = {
     "wb_mhd3d_state_size": (I64, [I64, I64, I64]),
     "wb_mhd3d_resistive_workspace": (I64, [I64, I64, I64]),
     "wb_mhd3d_resistive": (I, [P, P, I64, I64, I64] + [D] * 7 + [P, I, P, P]),
+    "wb_mhd3d_resistive_noalloc_limit": (I64, []),
+    "wb_mhd3d_resistive_noalloc": (I, [P, P, I64, I64, I64] + [D] * 7 + [P, I, P]),
     "wb_mhd3d_fill_halos": (I, [P, I64, I64, I64, I, P]),
     "wb_mhd3d_fill_outflow": (I, [P, I64, I64, I64, I, P]),
     "wb_primitive3d": (I, [P, I64, I64, I64] + [D] * 7 + [P] * 5 + [P]),
@@ -114,7 +117,9 @@ def call(name: str, *args) -> int:
     """Call entry point ``name`` and raise ``RuntimeError`` on a negative return."""
     rc = getattr(load(), name)(*args)
     if rc < 0 and SIGNATURES[name][0] is I:
-        what = "invalid arguments" if rc == WB_EINVAL else f"HIP error {-rc}"
+        what = ("invalid arguments" if rc == WB_EINVAL else
+                "the input needs more than the static scratch buffer" if rc == WB_ESCRATCH
+                else f"HIP error {-rc}")
         raise RuntimeError(f"{name} failed: {what}")
     return rc
 

@@ -12,6 +12,8 @@
 // scale is 1, or 0.5 for the Heun stage (the stage output is already the average).
 // The ideal and resistive parts are summed in a different order than the reference's
 // single flux difference: the results agree to round-off, not bit for bit.
+#include <mutex>
+
 #include "wbmhd_common.hpp"
 
 namespace {
@@ -161,23 +163,17 @@ __global__ void k_energy3d(double* __restrict__ out, Grid g, double dx, double d
 
 inline unsigned grid_of(long long n) { return (unsigned)((n + 255) / 256); }
 
-}  // namespace
 
-extern "C" {
-
-int64_t wb_mhd3d_resistive_workspace(int64_t nx, int64_t ny, int64_t nz) {
-  if (nx < 1 || ny < 1 || nz < 1 || nx > 4096 || ny > 4096 || nz > 4096) return WB_EINVAL;
-  return (int64_t)workspace_size(make_grid(nx, ny, nz));
+bool extents_ok(int64_t nx, int64_t ny, int64_t nz) {
+  return nx >= 1 && ny >= 1 && nz >= 1 && nx <= 4096 && ny <= 4096 && nz <= 4096;
 }
 
-int wb_mhd3d_resistive(const double* in, double* out, int64_t nx, int64_t ny, int64_t nz,
-                       double dx, double dy, double dz, double b0x, double b0y, double b0z,
-                       double eta, const double* dt, int heun, double* work, void* stream) {
-  if (nx < 1 || ny < 1 || nz < 1 || nx > 4096 || ny > 4096 || nz > 4096 || !(eta >= 0.0))
-    return WB_EINVAL;
-  if (eta == 0.0) return 0;
+// The three passes on the workspace `work`; every workspace value they read is written
+// by an earlier pass of the same call, so the workspace needs no clearing.
+int launch_resistive(const double* in, double* out, int64_t nx, int64_t ny, int64_t nz,
+                     double dx, double dy, double dz, double b0x, double b0y, double b0z,
+                     double eta, const double* dt, int heun, double* work, hipStream_t s) {
   const Grid g = make_grid(nx, ny, nz);
-  hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(k_faces3d, dim3(grid_of((long long)(g.X + 1) * (g.Y + 1) * (g.Z + 1))),
                      dim3(256), 0, s, in, g, dx, dy, dz, b0x, b0y, b0z, work);
   hipLaunchKernelGGL(k_current3d, dim3(grid_of((long long)(nx + 1) * (ny + 1) * (nz + 1))),
@@ -185,6 +181,67 @@ int wb_mhd3d_resistive(const double* in, double* out, int64_t nx, int64_t ny, in
   hipLaunchKernelGGL(k_energy3d, dim3(grid_of(nx * ny * nz)), dim3(256), 0, s, out, g, dx, dy, dz,
                      eta, dt, heun, (const double*)work);
   return wb_last_error();
+}
+
+// The workspace of the allocation-free entry point: one array per GPU, held until the
+// process ends, for grids whose workspace fits (up to 64^3 cells).  Host code never
+// passes `scratch` itself (its host-side name is no device address): each device's
+// address comes from hipGetSymbolAddress, taken once with that device current.
+constexpr size_t SCRATCH_DOUBLES = size_t(1) << 21;  // 16 MiB
+__device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_lock;
+double* scratch_address[MAX_DEVICES] = {};
+
+int scratch_of_current_device(double** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return -(int)e;
+  if (dev < 0 || dev >= MAX_DEVICES) return WB_EINVAL;
+  std::lock_guard<std::mutex> hold(scratch_lock);
+  if (scratch_address[dev] == nullptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return -(int)e;
+    scratch_address[dev] = (double*)p;
+  }
+  *out = scratch_address[dev];
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t wb_mhd3d_resistive_workspace(int64_t nx, int64_t ny, int64_t nz) {
+  if (!extents_ok(nx, ny, nz)) return WB_EINVAL;
+  return (int64_t)workspace_size(make_grid(nx, ny, nz));
+}
+
+int wb_mhd3d_resistive(const double* in, double* out, int64_t nx, int64_t ny, int64_t nz,
+                       double dx, double dy, double dz, double b0x, double b0y, double b0z,
+                       double eta, const double* dt, int heun, double* work, void* stream) {
+  if (!extents_ok(nx, ny, nz) || !(eta >= 0.0)) return WB_EINVAL;
+  if (eta == 0.0) return 0;
+  return launch_resistive(in, out, nx, ny, nz, dx, dy, dz, b0x, b0y, b0z, eta, dt, heun, work,
+                          (hipStream_t)stream);
+}
+
+int64_t wb_mhd3d_resistive_noalloc_limit(void) { return (int64_t)SCRATCH_DOUBLES; }
+
+int wb_mhd3d_resistive_noalloc(const double* in, double* out, int64_t nx, int64_t ny,
+                               int64_t nz, double dx, double dy, double dz, double b0x,
+                               double b0y, double b0z, double eta, const double* dt, int heun,
+                               void* stream) {
+  if (!extents_ok(nx, ny, nz) || !(eta >= 0.0)) return WB_EINVAL;
+  if (workspace_size(make_grid(nx, ny, nz)) > SCRATCH_DOUBLES) return WB_ESCRATCH;
+  if (eta == 0.0) return 0;
+  double* work = nullptr;
+  const int rc = scratch_of_current_device(&work);
+  if (rc) return rc;
+  return launch_resistive(in, out, nx, ny, nz, dx, dy, dz, b0x, b0y, b0z, eta, dt, heun, work,
+                          (hipStream_t)stream);
 }
 
 }  // extern "C"
