@@ -140,3 +140,21 @@ def test_negative_diffusivities_are_refused():
         AMRMHD2D(AMRHierarchy((8, 8), periodic=(True, True)), resistivity=-1.0)
     with pytest.raises(ValueError, match="resistivity"):
         MHD3D(AMRHierarchy((4, 4, 4), periodic=(True, True, True)), resistivity=-1.0)
+
+
+def test_a_step_graph_checks_the_model_generation_before_launching():
+    """Host logic of StepGraph.replay (no GPU): a model whose buffers or tables were
+    replaced since the capture is refused before the library is called."""
+    from types import SimpleNamespace
+
+    from wellbalanced_amd import autotune
+    from wellbalanced_amd.graph import StepGraph
+
+    model = SimpleNamespace(generation=3, device=None, steps=0)
+    graph = object.__new__(StepGraph)
+    graph.model, graph.steps, graph._graph, graph._generation = model, 1, None, 2
+    with pytest.raises(RuntimeError, match="capture a new StepGraph"):
+        graph.replay()
+    # the autotuner's choice bumps the generation of a model without set_tiling
+    autotune._apply(model, {"nxs": 8})
+    assert model.generation == 4 and model.nxs == 8

@@ -100,3 +100,26 @@ def test_captured_refined_3d_steps_equal_eager_steps(gpu, precision):
         np.testing.assert_array_equal(graphed.buffer(), eager.buffer())
         assert graphed.time == eager.time
     graphed.check()
+
+
+def test_a_graph_refuses_to_replay_after_a_regrid_or_a_new_tiling(gpu):
+    """The graph holds the addresses of the buffers and tables at capture: after a
+    regrid (new buffers and tables) or a new tiling (new work list) it raises before
+    any launch, and the model's state is untouched."""
+    from tests.test_regrid import _amr_orszag_tang
+
+    model = _amr_orszag_tang(16, limiter="minmod")
+    tags = {0: np.pad(np.ones((4, 7), bool), ((6, 6), (5, 4)))}
+    model.regrid(tags, max_patch_size=16)
+    graph = StepGraph(model, steps=1)
+    graph.replay()
+    model.set_tiling(tile=30)
+    before = model.buffer()
+    with pytest.raises(RuntimeError, match="capture a new StepGraph"):
+        graph.replay()
+    np.testing.assert_array_equal(model.buffer(), before)
+    graph = StepGraph(model, steps=1)
+    graph.replay()
+    model.regrid(tags, max_patch_size=8)
+    with pytest.raises(RuntimeError, match="capture a new StepGraph"):
+        graph.replay()

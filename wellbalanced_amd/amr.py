@@ -121,6 +121,9 @@ class AMRMHD2D:
         return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=self.device)
 
     def _upload_tables(self) -> None:
+        # new device tables (and, after _use, new state buffers): a captured graph holds
+        # the old addresses (wellbalanced_amd.graph)
+        self.generation = getattr(self, "generation", 0) + 1
         m = self.maps
         self._copy, self._interp = {}, {}
         for L in range(self.nlev):

@@ -70,6 +70,8 @@ class AMRMHD3D:
         return torch.from_numpy(np.ascontiguousarray(a).view(np.uint8).copy()).to(self.device)
 
     def _upload_tables(self) -> None:
+        # new device tables: a captured graph holds the old addresses
+        self.generation = getattr(self, "generation", 0) + 1
         t = self.tables
         self._blocks = self._bytes(t.blocks)
         self._lookup = torch.as_tensor(t.lookup, dtype=torch.int32, device=self.device)

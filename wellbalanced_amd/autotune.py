@@ -50,6 +50,8 @@ def _apply(model, choice: dict) -> None:
     else:
         for k, v in choice.items():
             setattr(model, k, v)
+    # a captured graph holds the old launch shape (wellbalanced_amd.graph)
+    model.generation = getattr(model, "generation", 0) + 1
 
 
 def tune(model, space: dict | None = None, steps: int = 3) -> dict:

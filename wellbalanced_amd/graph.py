@@ -5,6 +5,10 @@ enqueues only this library's kernels, whose arguments are fixed device pointers 
 dt and the time live on the device and counters are cleared by kernels, not
 memsets -- so it is captured once into a HIP graph on a side stream and replayed:
 one graph launch per k steps instead of ~10 kernel launches per step.
+
+The graph holds the device addresses of the model's buffers and tables at capture.
+A model bumps its ``generation`` whenever it replaces them (a regrid, a new tiling,
+the autotuner's choice); `StepGraph.replay` then refuses to launch.
 """
 
 from __future__ import annotations
@@ -46,9 +50,14 @@ class StepGraph:
         self._graph = handle.value
         model.steps -= self.steps  # capture does not run the steps
         self._side = side
+        self._generation = getattr(model, "generation", 0)
 
     def replay(self, times: int = 1) -> None:
-        """Launch the captured steps ``times`` times on the current stream."""
+        """Launch the captured steps ``times`` times on the current stream; raises,
+        before any launch, if the model replaced its buffers or tables since."""
+        if getattr(self.model, "generation", 0) != self._generation:
+            raise RuntimeError("the model's buffers or tables changed since the capture "
+                               "(regrid, tiling or autotuning): capture a new StepGraph")
         stream = _lib.stream_handle(self.model.device)
         _lib.call("wb_graph_launch", self._graph, int(times), stream)
         self.model.steps += self.steps * int(times)
