@@ -92,6 +92,12 @@ def _star(w, bn, s, d, pt, sm, pts, energy0):
     return us, (vt1, vt2, bt1, bt2)
 
 
+#: The sign of the rotational-discontinuity term in the double-star energy of the left
+#: and right states (``mhd_sweep_pyccel.py:233-238``); a module constant so that a test
+#: can plant a sign error there.
+DOUBLE_STAR_SENSE = (-1.0, 1.0)
+
+
 def hlld_flux(left, right, bn, gamma) -> np.ndarray:
     """HLLD flux of the faces between ``left`` and ``right`` (``mhd_sweep_pyccel.py:148-251``).
 
@@ -142,8 +148,8 @@ def hlld_flux(left, right, bn, gamma) -> np.ndarray:
                 bt2_2,
             ]
 
-        uddl = double_star(usl, vl1, vl2, bl1, bl2, sql, -1.0)
-        uddr = double_star(usr, vr1, vr2, br1, br2, sqr, 1.0)
+        uddl = double_star(usl, vl1, vl2, bl1, bl2, sql, DOUBLE_STAR_SENSE[0])
+        uddr = double_star(usr, vr1, vr2, br1, br2, sqr, DOUBLE_STAR_SENSE[1])
         out = []
         for c in range(7):
             star_l = fl[c] + sl * (usl[c] - ul[c])

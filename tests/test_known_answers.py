@@ -1,0 +1,179 @@
+"""The reference's whole-run checks on the uniform grid, on the oracle (CPU) and on the
+GPU path (whose FP64 kernels must also equal the oracle bit for bit):
+
+* ``tests/unit/test_mhd.py:78-86``: the oblique Alfvén wave converges at second order
+  (log2 > 1.85) with div B < 1e-14;
+* ``tests/unit/test_mhd.py:379-387``: the rotor keeps its point symmetry to 1e-10,
+  positive pressure and div B < 1e-14;
+* ``tests/unit/test_mhd.py:390-405``: the blast wave conserves to 1e-12, keeps its point
+  symmetry to 1e-10 and runs further along the field than across it;
+* ``tests/unit/test_mhd3d.py:13-21``: the 3D Alfvén wave converges at second order
+  (log2 > 1.8) with div B < 1e-14.
+"""
+
+import numpy as np
+import pytest
+
+from oracle.mhd3d import cell_fields, face_fields
+from tests.helpers import oracle2d, oracle3d
+from wellbalanced_amd import initial
+
+
+def _cell_fields_2d(o):
+    a = o.nodes()
+    bxf = (a[:, 1:] - a[:, :-1]) / o.dy + o.b0[0]
+    byf = -(a[1:, :] - a[:-1, :]) / o.dx + o.b0[1]
+    return 0.5 * (bxf[:-1] + bxf[1:]), 0.5 * (byf[:, :-1] + byf[:, 1:])
+
+
+def _pressure_2d(o):
+    c = o.interior()
+    bx, by = _cell_fields_2d(o)
+    rho = c[0]
+    u, v, w = c[1] / rho, c[2] / rho, c[3] / rho
+    return (o.gamma - 1.0) * (c[4] - 0.5 * rho * (u**2 + v**2 + w**2)
+                              - 0.5 * (bx**2 + by**2 + c[5] ** 2))
+
+
+def _cell_by_3d(o):
+    h = 2
+    local = [o.pot[c][tuple(slice(h, h + o.n[k] + (k != c)) for k in range(3))] for c in range(3)]
+    return cell_fields(face_fields(local, o.spacing, o.b0))[1]
+
+
+def _run3d(o, duration):
+    end = o.time + duration
+    while o.time < end * (1.0 - 1e-14):
+        o.step(min(o.stable_step(), end - o.time))
+
+
+# -- the oracle ----------------------------------------------------------------------------
+def test_alfven_wave_converges_at_second_order():
+    errors = []
+    for cells in (8, 16):
+        problem = initial.alfven_wave(cells, limiter="none")
+        o = oracle2d(problem)
+        exact = _cell_fields_2d(o)[1].copy()
+        o.run(problem.period)
+        errors.append(np.mean(np.abs(_cell_fields_2d(o)[1] - exact)))
+        assert o.divergence() < 1e-14
+    assert np.log2(errors[0] / errors[1]) > 1.85
+
+
+def test_rotor_keeps_its_point_symmetry_and_div_b_zero():
+    o = oracle2d(initial.rotor(32))
+    o.run(0.05)
+    density = o.interior()[0]
+    np.testing.assert_allclose(density, density[::-1, ::-1], rtol=1e-10)
+    assert _pressure_2d(o).min() > 0.0
+    assert o.divergence() < 1e-14
+
+
+def _blast_checks(totals_before, totals_after, pressure, x, y, divergence):
+    assert np.abs(totals_after - totals_before).max() < 1e-12
+    np.testing.assert_allclose(pressure, pressure[::-1, ::-1], rtol=1e-10)
+    hot = pressure > 1.0
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    assert np.abs(X + Y)[hot].max() > 1.5 * np.abs(X - Y)[hot].max()
+    assert divergence < 1e-14
+
+
+def test_blast_wave_conserves_and_runs_along_the_field():
+    problem = initial.blast_wave(32)
+    o = oracle2d(problem)
+    before = o.totals()
+    o.run(0.1)
+    x, y = problem.hierarchy.cell_centers(problem.hierarchy.patches[0])
+    _blast_checks(before, o.totals(), _pressure_2d(o), x[:, 0], y[0, :], o.divergence())
+
+
+def test_3d_alfven_wave_converges_at_second_order():
+    errors = []
+    for cells in (8, 16):
+        problem = initial.alfven_wave_3d(cells, limiter="none")
+        o = oracle3d(problem)
+        exact = _cell_by_3d(o).copy()
+        _run3d(o, problem.period)
+        errors.append(np.mean(np.abs(_cell_by_3d(o) - exact)))
+        assert o.divergence() < 1e-14
+    assert np.log2(errors[0] / errors[1]) > 1.8
+
+
+# -- the GPU path --------------------------------------------------------------------------
+def _same_2d(model, o):
+    cons, az = model.arrays()
+    np.testing.assert_array_equal(cons[:, 2:-2, 2:-2], o.interior())
+    np.testing.assert_array_equal(az[2:-2, 2:-2], o.nodes())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kernels,precision", [("split", "fp64"), ("fused", "fp64"),
+                                               ("fused", "mixed")])
+def test_gpu_alfven_wave_converges_at_second_order(gpu, kernels, precision):
+    from wellbalanced_amd import MHD2D
+
+    errors = []
+    for cells in (8, 16):
+        model, exact = MHD2D.alfven_wave(cells, limiter="none", kernels=kernels,
+                                         precision=precision)
+        model.run(model.period)
+        errors.append(np.mean(np.abs(model.dataset()["magnetic_field_y"]
+                                     - exact["magnetic_field_y"])))
+        assert model.divergence() < 1e-14
+        if precision == "fp64" and cells == 8:
+            problem = initial.alfven_wave(cells, limiter="none")
+            o = oracle2d(problem)
+            o.run(problem.period)
+            assert o.time == model.time
+            _same_2d(model, o)
+    assert np.log2(errors[0] / errors[1]) > 1.85
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kernels,precision", [("split", "fp64"), ("fused", "fp64"),
+                                               ("fused", "mixed")])
+def test_gpu_rotor_keeps_its_point_symmetry_and_div_b_zero(gpu, kernels, precision):
+    from wellbalanced_amd import MHD2D
+
+    model = MHD2D.rotor(32, kernels=kernels, precision=precision)
+    model.run(0.05)
+    data = model.dataset()
+    density = data["density"]
+    np.testing.assert_allclose(density, density[::-1, ::-1], rtol=1e-10)
+    assert data["pressure"].min() > 0.0
+    assert model.divergence() < 1e-14
+    if precision == "fp64":
+        o = oracle2d(initial.rotor(32))
+        o.run(0.05)
+        _same_2d(model, o)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kernels,precision", [("split", "fp64"), ("fused", "fp64"),
+                                               ("fused", "mixed")])
+def test_gpu_blast_wave_conserves_and_runs_along_the_field(gpu, kernels, precision):
+    from wellbalanced_amd import MHD2D
+
+    model = MHD2D.blast_wave(32, kernels=kernels, precision=precision)
+    before = np.array(list(model.totals().values()))
+    model.run(0.1)
+    data = model.dataset()
+    _blast_checks(before, np.array(list(model.totals().values())), data["pressure"], data["x"],
+                  data["y"], model.divergence())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kernels,precision", [("split", "fp64"), ("fused", "fp64"),
+                                               ("fused", "mixed")])
+def test_gpu_3d_alfven_wave_converges_at_second_order(gpu, kernels, precision):
+    from wellbalanced_amd import MHD3D
+
+    errors = []
+    for cells in (8, 16):
+        model, exact = MHD3D.alfven_wave(cells, limiter="none", kernels=kernels,
+                                         precision=precision)
+        model.run(model.period)
+        errors.append(np.mean(np.abs(model.dataset()["magnetic_field_y"]
+                                     - exact["magnetic_field_y"])))
+        assert model.divergence() < 1e-14
+    assert np.log2(errors[0] / errors[1]) > 1.8

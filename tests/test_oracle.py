@@ -57,14 +57,11 @@ def test_golden_inputs_are_the_reference_test_blocks():
 
 
 def test_a_planted_sign_error_in_the_double_star_state_fails_the_golden(monkeypatch):
-    """The golden vectors catch a plausible mistake in the HLLD double-star energy."""
+    """The golden vectors catch a sign error in the HLLD double-star energy (the
+    rotational-discontinuity term of the left and right states swapped in sign)."""
     g = np.load(GOLDEN)
-    original = K.hlld_flux
-
-    def wrong(left, right, bn, gamma):
-        return original(left, right, -np.asarray(bn) * 0 + np.asarray(bn) * (1 - 1e-9), gamma)
-
-    monkeypatch.setattr(K, "hlld_flux", wrong)
+    assert np.array_equal(K.sweep(g["prim_1"], g["bxf_1"], 0, 0, GAMMA), g["flux_1_0_0"])
+    monkeypatch.setattr(K, "DOUBLE_STAR_SENSE", (1.0, -1.0))
     got = K.sweep(g["prim_1"], g["bxf_1"], 0, 0, GAMMA)
     assert not np.array_equal(got, g["flux_1_0_0"])
 
@@ -301,13 +298,41 @@ def test_turbulence_fft_evaluation_is_exact():
     np.testing.assert_allclose(az, direct, atol=1e-12)
 
 
-def test_oracle_step_is_heun():
-    """One step equals (U0 + Euler(Euler(U0))) / 2 built from the oracle's stages."""
-    model = oracle2d(initial.orszag_tang(16))
-    twin = Oracle2D(model.cons, model.az, (model.dx, model.dy), limiter=["minmod", "vanleer", "mc", "none"][model.code])
-    dt = model.stable_step()
-    model.step(dt)
-    start = twin.interior().copy()
-    twin.advance(dt)
-    twin.advance(dt)
-    np.testing.assert_array_equal(model.interior(), (twin.interior() + start) * 0.5)
+def test_oracle_step_is_second_order_in_time():
+    """On a fixed grid the smooth Alfvén wave run with dt and dt/2 differs from a run
+    with dt/16 by a ratio of about 4: the step is second order in time (Heun), where
+    forward Euler or a wrong stage average would give about 2."""
+    def run(dt, steps):
+        o = oracle2d(initial.alfven_wave(8, limiter="none"))
+        for _ in range(steps):
+            o.step(dt)
+        return o.interior().copy()
+
+    base = 0.01
+    ref = run(base / 16, 160)
+    errors = [np.abs(run(base / m, 10 * m) - ref).max() for m in (1, 2)]
+    assert 3.5 < errors[0] / errors[1] < 4.5, errors
+
+
+def test_a_first_order_stage_combination_fails_the_time_order(monkeypatch):
+    """The time-order check catches a wrong combination of the stages (the second
+    stage's result kept instead of the Heun average)."""
+    from oracle import mhd2d
+
+    def euler_euler(self, dt=None):
+        self.advance(dt)
+        self.time += dt
+        self.steps += 1
+        return dt
+
+    monkeypatch.setattr(mhd2d.Oracle2D, "step", euler_euler)
+
+    def run(dt, steps):
+        o = oracle2d(initial.alfven_wave(8, limiter="none"))
+        for _ in range(steps):
+            o.step(dt)
+        return o.interior().copy()
+
+    ref = run(0.01 / 16, 160)
+    errors = [np.abs(run(0.01 / m, 10 * m) - ref).max() for m in (1, 2)]
+    assert errors[0] / errors[1] < 3.0, errors
