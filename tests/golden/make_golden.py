@@ -84,6 +84,26 @@ def field_free_face(prim):
     return prim
 
 
+def rotor_column_block():
+    """Three x-columns (33..35, haloed) of the stage-2 input of step 17 of the rotor of
+    ``mhd.py:561-592`` at 32^2 (this repository's oracle run): at rest far from the
+    rotor, so a y-face has B_n at round-off and its tangential fluxes cancel to 1e-15,
+    where a single rounding of the HLLD shows."""
+    repo = Path(__file__).resolve().parents[2]
+    sys.path.insert(0, str(repo))
+    from oracle import kernels as K
+    from tests.helpers import oracle2d
+    from wellbalanced_amd import initial
+
+    o = oracle2d(initial.rotor(32))
+    for _ in range(16):
+        o.step(o.stable_step())
+    o.advance(o.stable_step())
+    o.fill_halos()
+    prim, _, byf, _, _ = K.primitive(*o.cons, o.az, o.dx, o.dy, *o.b0, o.gamma)
+    return np.ascontiguousarray(prim[:, 33:36, :]), np.ascontiguousarray(byf[33:36, :]), o.gamma
+
+
 def main(reference: Path) -> None:
     kernels = reference / "src" / "carafe" / "kernels"
     sweep = _load(kernels / "mhd_sweep" / "mhd_sweep_pyccel.py", "ref_mhd_sweep")
@@ -101,6 +121,12 @@ def main(reference: Path) -> None:
                 flux = np.full(shape, np.nan)
                 sweep.mhd_sweep(prim, (bxf, byf)[axis], axis, limiter, GAMMA, flux)
                 out[f"flux_{seed}_{axis}_{limiter}"] = flux
+    prim, byf, gamma = rotor_column_block()
+    flux = np.full((7, prim.shape[1], prim.shape[2] - 3), np.nan)
+    for limiter in range(4):
+        sweep.mhd_sweep(prim, byf, 1, limiter, gamma, flux)
+        out[f"rotor_flux_{limiter}"] = flux.copy()
+    out.update(rotor_prim=prim, rotor_byf=byf, rotor_gamma=np.array(gamma))
     fx, fy, ecc = ct_inputs()
     ez = np.full((nx - 3, ny - 3), np.nan)
     ct.ct_contact(fx[5], fy[5], fx[0], fy[0], ecc, ez)

@@ -101,9 +101,18 @@ def test_3d_alfven_wave_converges_at_second_order():
 
 # -- the GPU path --------------------------------------------------------------------------
 def _same_2d(model, o):
+    """The FP64 GPU state against the oracle: every conserved variable and A_z within
+    1e-12 of its largest magnitude (the stated parity bar).  In practice bit for bit, but
+    gfx950's FP64 division is not correctly rounded at exact near-ties (the rotor meets
+    one: 1.3999999999999992 / 0.9999999999999994 gives 1.4, IEEE 1.4000000000000001), so
+    a value that cancels to round-off can differ in its low digits."""
     cons, az = model.arrays()
-    np.testing.assert_array_equal(cons[:, 2:-2, 2:-2], o.interior())
-    np.testing.assert_array_equal(az[2:-2, 2:-2], o.nodes())
+    for got, want in [*zip(cons[:, 2:-2, 2:-2], o.interior()), (az[2:-2, 2:-2], o.nodes())]:
+        scale = np.abs(want).max()
+        if scale == 0.0:
+            np.testing.assert_array_equal(got, want)
+        else:
+            assert np.abs(got - want).max() <= 1e-12 * scale
 
 
 @pytest.mark.gpu
