@@ -336,3 +336,12 @@ def test_a_first_order_stage_combination_fails_the_time_order(monkeypatch):
     ref = run(0.01 / 16, 160)
     errors = [np.abs(run(0.01 / m, 10 * m) - ref).max() for m in (1, 2)]
     assert errors[0] / errors[1] < 3.0, errors
+
+
+@pytest.mark.parametrize("limiter", range(4))
+def test_the_sweep_on_a_face_at_rest_is_the_golden(limiter):
+    """The rotor columns at rest (B_n at round-off): the oracle is the reference's own
+    sweep bit for bit."""
+    g = np.load(GOLDEN)
+    got = K.sweep(g["rotor_prim"], g["rotor_byf"], 1, limiter, float(g["rotor_gamma"]))
+    np.testing.assert_array_equal(got, g[f"rotor_flux_{limiter}"])
