@@ -511,6 +511,43 @@ int wb_amr3d_reflux_mixed(const double* src, double* dst, const void* blocks, co
                           double b0y, double b0z, double gamma, int limiter, const double* dt,
                           int record, double* reg, void* stream);
 
+/* ---- 3D hierarchies of patches of any size (MHD3D on patchwork patches) -------- */
+/* Each patch holds a 3D patch state as above, patch after patch in one buffer; the
+ * tables are built on the host from the boxes (wellbalanced_amd/patches3d_maps.py).
+ * Replaces patchwork's exchange_halos, restrict and the regrid's move as MHD3D.step
+ * and MHD3D.regrid call them (mhd3d.py:448-473,726-754,803-821). */
+
+/* dst_buf[dst[i]] = w[s] src_buf[src[s]] + w[s+1] src_buf[src[s+1]] + ... over
+ * s = start[i] .. start[i] + count[i] - 1, in that order (ghost fill of a level,
+ * average-down and restriction of a level pair, the copies and interpolations of a
+ * regrid).  No destination may be a source of the same call. */
+int wb_patches3d_gather(double* dst_buf, const double* src_buf, const long long* dst,
+                        const long long* start, const int32_t* count, const long long* src,
+                        const double* w, int64_t n, void* stream);
+
+/* The new cells of a regrid: rows of 9 int64 {dst, dst plane, coarse cell, coarse x
+ * and y strides, coarse plane, sx, sy, sz}: c0 + s/4 minmod slopes per axis, 5 fields
+ * (the children of a coarse cell keep its integral). */
+int wb_patches3d_prolong(double* buf, const long long* rows, int64_t n, void* stream);
+
+/* The flux register and refluxing (FluxRegister.update with scale 0.5 dt and
+ * FluxRegister.reflux, mhd3d.py:650-655,745-750) from the stage's sweep arrays: per
+ * refluxed cell {offset, plane, first face, face count}; per face 12 int64 {axis, side,
+ * coarse flux offset and row stride, four fine ones} and the coarse spacing.  record:
+ * reg[5 g + f] = half the correction; else the cell of `out` gains reg plus half the
+ * correction of this stage. */
+int wb_patches3d_reflux(double* out, const double* sweeps, const long long* cells,
+                        const long long* faces, const double* spacing, int64_t ncells,
+                        const double* dt, int record, double* reg, void* stream);
+
+/* Replaces MHD3D._resistive (mhd3d.py:561-602) inside the split stage: from the haloed
+ * face fields f0 (X+1,Y,Z), f1, f2 of wb_primitive3d, eta J is added to the edge fields
+ * e0..e2 of wb_edge3d and eta (J x B)_s to the energy row of the sweeps s0..s2 of
+ * wb_sweep3d on the block faces, before wb_mhd3d_update. */
+int wb_resistive3d(const double* f0, const double* f1, const double* f2, int64_t nx, int64_t ny,
+                   int64_t nz, double dx, double dy, double dz, double eta, double* s0,
+                   double* s1, double* s2, double* e0, double* e1, double* e2, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -306,7 +306,7 @@ def test_mixed_harris_wings_stay_close_to_fp64(gpu):
 
 @pytest.mark.parametrize("limiter", range(4))
 def test_sweep_twin_on_a_face_at_rest_is_the_golden_to_round_off(gpu, limiter):
-    """The reference's own sweep on three columns of the rotor at rest (B_n at round-off,
+    """The reference's own sweep on four columns of the rotor at rest (B_n at round-off,
     tangential fluxes cancelling to 1e-15): the GPU's fluxes equal the golden vector to
     1e-15 of the largest flux.  Not always bit for bit: gfx950's FP64 division misrounds
     the exact near-tie 1.3999999999999992 / 0.9999999999999994 (1.4, IEEE

@@ -61,6 +61,10 @@ SIGNATURES: dict[str, tuple[type, list]] = {
     "wb_amr3d_fill_list": (I, [P, P, P, I, I, P, P, P]),
     "wb_amr3d_fill_regions": (I, [P, P, P, P, I, P]),
     "wb_amr3d_coarsen": (I, [P, P, P, I, I, P, P, P]),
+    "wb_patches3d_gather": (I, [P, P, P, P, P, P, P, I64, P]),
+    "wb_patches3d_prolong": (I, [P, P, I64, P]),
+    "wb_patches3d_reflux": (I, [P, P, P, P, P, I64, P, I, P, P]),
+    "wb_resistive3d": (I, [P, P, P, I64, I64, I64, D, D, D, D, P, P, P, P, P, P, P]),
     "wb_amr3d_prolong": (I, [P, P, P, I, P, P, P]),
     "wb_amr3d_rate": (I, [P, P, I, D, D, D, D, P, P]),
     "wb_amr3d_rate_mixed": (I, [P, P, I, D, D, D, D, P, P]),

@@ -47,12 +47,17 @@ def dilate(flags: np.ndarray, width: int) -> np.ndarray:
     return out
 
 
+def _signature(sub: np.ndarray, axis: int) -> np.ndarray:
+    """The tag count of every slab of ``sub`` normal to ``axis``."""
+    return sub.sum(axis=tuple(a for a in range(sub.ndim) if a != axis))
+
+
 def _cut(sub: np.ndarray, min_size: int):
     """(axis, position) to split a tag block at, or None (see the module)."""
     sides = sub.shape
-    order = sorted(range(2), key=lambda a: -sides[a])  # the longest side first
+    order = sorted(range(sub.ndim), key=lambda a: -sides[a])  # the longest side first
     for axis in order:  # holes
-        sig = sub.sum(axis=1 - axis)
+        sig = _signature(sub, axis)
         n = sides[axis]
         holes = [k for k in range(min_size, n - min_size + 1) if k < n and sig[k] == 0]
         if holes:
@@ -60,7 +65,7 @@ def _cut(sub: np.ndarray, min_size: int):
             return axis, k
     best = None
     for axis in order:  # inflections of the signature
-        sig = sub.sum(axis=1 - axis).astype(np.int64)
+        sig = _signature(sub, axis).astype(np.int64)
         n = sides[axis]
         if n < 4:
             continue
@@ -82,45 +87,48 @@ def _cut(sub: np.ndarray, min_size: int):
 
 def cluster(flags: np.ndarray, *, efficiency: float = 0.7, max_size: int | None = None,
             min_size: int = 1) -> list[tuple]:
-    """Boxes ``((i0, i1), (j0, j1))`` covering every flagged cell (see the module)."""
+    """Boxes ``((i0, i1), (j0, j1)[, (k0, k1)])`` covering every flagged cell (see the
+    module); 2D or 3D."""
     flags = np.asarray(flags, bool)
     boxes = []
-    todo = [((0, flags.shape[0]), (0, flags.shape[1]))]
+    todo = [tuple((0, n) for n in flags.shape)]
     while todo:
-        (i0, i1), (j0, j1) = todo.pop()
-        sub = flags[i0:i1, j0:j1]
+        box = todo.pop()
+        sub = flags[tuple(slice(lo, hi) for lo, hi in box)]
         if not sub.any():
             continue
-        rows = np.flatnonzero(sub.any(axis=1))
-        cols = np.flatnonzero(sub.any(axis=0))
-        i0, i1 = i0 + int(rows[0]), i0 + int(rows[-1]) + 1
-        j0, j1 = j0 + int(cols[0]), j0 + int(cols[-1]) + 1
-        sub = flags[i0:i1, j0:j1]
-        big = max_size is not None and (i1 - i0 > max_size or j1 - j0 > max_size)
+        shrunk = []
+        for a, (lo, _) in enumerate(box):
+            hit = np.flatnonzero(_signature(sub, a))
+            shrunk.append((lo + int(hit[0]), lo + int(hit[-1]) + 1))
+        box = tuple(shrunk)
+        sub = flags[tuple(slice(lo, hi) for lo, hi in box)]
+        big = max_size is not None and any(hi - lo > max_size for lo, hi in box)
         if sub.mean() >= efficiency and not big:
-            boxes.append(((i0, i1), (j0, j1)))
+            boxes.append(box)
             continue
         cut = _cut(sub, min_size)
         if cut is None:
             if not big:
-                boxes.append(((i0, i1), (j0, j1)))
+                boxes.append(box)
                 continue
             axis = int(np.argmax(sub.shape))  # too long for a patch: halve it
             cut = (axis, sub.shape[axis] // 2)
         axis, k = cut
-        if axis == 0:
-            todo += [((i0, i0 + k), (j0, j1)), ((i0 + k, i1), (j0, j1))]
-        else:
-            todo += [((i0, i1), (j0, j0 + k)), ((i0, i1), (j0 + k, j1))]
+        lo, hi = box[axis]
+        first = tuple((lo, lo + k) if a == axis else b for a, b in enumerate(box))
+        second = tuple((lo + k, hi) if a == axis else b for a, b in enumerate(box))
+        todo += [first, second]
     return sorted(boxes)
 
 
 def regrid(hierarchy: AMRHierarchy, tags: dict, *, max_patch_size: int | None = None,
            buffer: int = 1, efficiency: float = 0.7, nest: int = NEST) -> AMRHierarchy:
     """A new hierarchy over the same domain whose level ``L + 1`` covers the tagged
-    cells of level ``L`` (``tags``: level -> bool array of the level's shape)."""
-    if hierarchy.ndim != 2 or not all(hierarchy.periodic):
-        raise NotImplementedError("regridding runs on periodic 2D hierarchies")
+    cells of level ``L`` (``tags``: level -> bool array of the level's shape); 2D or 3D,
+    periodic."""
+    if not all(hierarchy.periodic):
+        raise NotImplementedError("regridding runs on periodic hierarchies")
     flags = {}
     for L, t in tags.items():
         t = np.asarray(t, bool)
@@ -132,19 +140,21 @@ def regrid(hierarchy: AMRHierarchy, tags: dict, *, max_patch_size: int | None = 
     top = max(flags) + 1 if flags else 0
     boxes: dict[int, list] = {}
     fine_max = None if max_patch_size is None else max(1, int(max_patch_size) // 2)
+    ndim = hierarchy.ndim
     for L in range(top - 1, -1, -1):
         shape = hierarchy.level_shape(L)
         f = flags.get(L, np.zeros(shape, bool)).copy()
         if L + 2 in boxes:  # the new level L + 2 nests in the new level L + 1
             fine = np.zeros(hierarchy.level_shape(L + 1), bool)
-            for (i0, i1), (j0, j1) in boxes[L + 2]:
-                fine[i0 // 2 : i1 // 2, j0 // 2 : j1 // 2] = True
+            for box in boxes[L + 2]:
+                fine[tuple(slice(lo // 2, hi // 2) for lo, hi in box)] = True
             fine = dilate(fine, nest)
-            f |= fine.reshape(shape[0], 2, shape[1], 2).any(axis=(1, 3))
+            split = [v for n in shape for v in (n, 2)]
+            f |= fine.reshape(split).any(axis=tuple(range(1, 2 * ndim, 2)))
         if not f.any():
             continue
         found = cluster(f, efficiency=efficiency, max_size=fine_max)
-        boxes[L + 1] = [((2 * i0, 2 * i1), (2 * j0, 2 * j1)) for (i0, i1), (j0, j1) in found]
+        boxes[L + 1] = [tuple((2 * lo, 2 * hi) for lo, hi in box) for box in found]
     out = AMRHierarchy(hierarchy.base_shape, hierarchy.bounds, hierarchy.periodic)
     for L in sorted(boxes):
         for box in boxes[L]:
