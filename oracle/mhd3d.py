@@ -213,6 +213,11 @@ class Oracle3D:
 
     def stable_step(self):
         """``mhd3d.py:674-724``."""
+        return self.cfl * self.smallest_time()
+
+    def smallest_time(self):
+        """The smallest of 1 / (largest summed fast-wave rate) and, with a resistivity, the
+        diffusion limit: the stable step before the factor ``cfl``."""
         h = HALO
         nx, ny, nz = self.n
         local = []
@@ -237,7 +242,7 @@ class Oracle3D:
         smallest = 1.0 / float(np.max(rate))
         if self.resistivity:  # mhd3d.py:717-721
             smallest = min(smallest, 0.5 / (self.resistivity * sum(d**-2 for d in self.spacing)))
-        return self.cfl * smallest
+        return smallest
 
     def step(self, dt=None):
         """Heun (SSP-RK2) step (``mhd3d.py:726-754``)."""

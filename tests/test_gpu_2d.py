@@ -316,6 +316,7 @@ def test_sweep_twin_on_a_face_at_rest_is_the_golden_to_round_off(gpu, limiter):
     prim, byf, gamma = g["rotor_prim"], g["rotor_byf"], float(g["rotor_gamma"])
     X, Y = prim.shape[1:]
     out = _empty((7, X, Y - 3))
-    _call("wb_sweep2d", _p(_dev(prim)), _p(_dev(byf)), X, Y, 1, limiter, gamma, _p(out))
+    dp, db = _dev(prim), _dev(byf)  # held: their memory must outlive the call
+    _call("wb_sweep2d", _p(dp), _p(db), X, Y, 1, limiter, gamma, _p(out))
     want = g[f"rotor_flux_{limiter}"]
     assert np.abs(out.cpu().numpy() - want).max() <= 1e-15 * np.abs(want).max()

@@ -58,7 +58,22 @@ def _diff(x, axis):
 
 
 class MHD3D:
-    """3D ideal MHD on one uniform patch of an `AMRHierarchy` (see the module)."""
+    """3D ideal MHD on one uniform patch of an `AMRHierarchy` (see the module).  On a
+    hierarchy with refined patches the constructor returns a `wellbalanced_amd.
+    patches3d.RefinedMHD3D` with the same API (periodic domains, FP64)."""
+
+    def __new__(cls, hierarchy: AMRHierarchy, **options):
+        if cls is MHD3D and len(getattr(hierarchy, "patches", ())) > 1:
+            from .patches3d import RefinedMHD3D
+
+            if any(options.get("gravity", (0.0, 0.0, 0.0))):
+                raise NotImplementedError("the gravity source is outside this MHD path")
+            if options.get("precision", "fp64") != "fp64":
+                raise NotImplementedError("refined 3D patches run the FP64 split kernels; the "
+                                          "block hierarchy (wellbalanced_amd.amr3d) runs mixed")
+            keep = ("gamma", "mean_field", "resistivity", "limiter", "cfl", "prefix", "device")
+            return RefinedMHD3D(hierarchy, **{k: v for k, v in options.items() if k in keep})
+        return super().__new__(cls)
 
     def __init__(self, hierarchy: AMRHierarchy, *, gamma: float = 5.0 / 3.0,
                  mean_field=(0.0, 0.0, 0.0), resistivity: float = 0.0, limiter: str = "vanleer",
@@ -86,8 +101,6 @@ class MHD3D:
             raise ValueError("the split kernels are the FP64 parity path; mixed needs kernels='fused'")
         if any(gravity):
             raise NotImplementedError("the gravity source is outside this MHD path")
-        if len(hierarchy.patches) != 1:
-            raise NotImplementedError("refined 3D patches are not supported by MHD3D")
         self.hierarchy = hierarchy
         self.gamma = float(gamma)
         self.mean_field = tuple(float(b) for b in mean_field)

@@ -431,3 +431,111 @@ def pack_state(hierarchy, layouts, total, gamma, mean_field, state) -> np.ndarra
             b = lay.base(name)
             buf[b : b + pot.size] = pot.ravel()
     return buf
+
+
+def _regrid_rule(node, f):
+    """Per axis (coarse indices, weights) of a new edge of a regrid: along the edge the
+    coarse cell holding it, across it the coarse node (even index) or the mean of the
+    two around it (odd)."""
+    f = np.asarray(f, np.int64)
+    if not node:
+        return [(np.floor_divide(f, 2), np.ones(f.shape))]
+    odd = f % 2 == 1
+    m = np.floor_divide(f, 2)
+    return [(m, np.where(odd, 0.5, 1.0)), (m + 1, np.where(odd, 0.5, 0.0))]
+
+
+def transfer_tables(old_hierarchy, old_maps, new_hierarchy, new_maps) -> dict:
+    """The move of a regrid (``MHD3D.regrid``, mhd3d.py:803-821), per new level: the
+    cells and edges the old hierarchy held on the same level (a gather from the old
+    buffer), the conservative minmod prolongation of the new cells from the new level
+    below (rows of `wb_patches3d_prolong`), and the new edges interpolated from the new
+    level below (along the edge the coarse edge holding it, across it linearly), a
+    gather in the new buffer."""
+    old_layouts, new_layouts = old_maps.layouts, new_maps.layouts
+    nlev = new_hierarchy.max_level + 1
+    out = {}
+    for L in range(nlev):
+        shape = new_hierarchy.level_shape(L)
+        old_own = {name: (_owners(old_layouts, L, shape, name) if L <= old_hierarchy.max_level
+                          else np.full(shape, -1, np.int64)) for name in FIELDS}
+        coarse_own = ({name: _owners(new_layouts, L - 1, new_hierarchy.level_shape(L - 1), name)
+                       for name in FIELDS} if L > 0 else None)
+        copies, prolong, interp = [], [], []
+        for lay in new_layouts:
+            if lay.level != L:
+                continue
+            lo = np.array([b for b, _ in lay.box])[:, None]
+            plane = _cell_plane(lay)
+            for name in FIELDS:
+                nodes = CENTRING[name]
+                ext = [lay.n[a] + int(nodes[a]) for a in range(3)]
+                P = np.indices(ext).reshape(3, -1).astype(np.int64) + HALO
+                fine = P - HALO + lo
+                I = np.stack([fine[a] % shape[a] for a in range(3)])
+                own = old_own[name][I[0], I[1], I[2]]
+                dst = lay.index(name, *P)
+                have = own >= 0
+                src = np.empty(P.shape[1], np.int64)
+                src_plane = np.empty(P.shape[1], np.int64)
+                for q in np.unique(own[have]):
+                    sel = own == q
+                    src[sel] = old_layouts[q].index(name, *_local(old_layouts[q], shape, I[:, sel]))
+                    src_plane[sel] = _cell_plane(old_layouts[q])
+                comps = NCELL if name == "cells" else 1
+                for comp in range(comps):
+                    copies.append((dst[have] + comp * plane, [(src[have] + comp * src_plane[have], 1.0)]))
+                miss = ~have
+                if not miss.any():
+                    continue
+                if L == 0:
+                    raise ValueError("level 0 must cover the domain")
+                shape_c = new_hierarchy.level_shape(L - 1)
+                if name == "cells":
+                    C = np.stack([np.floor_divide(fine[a][miss], 2) % shape_c[a] for a in range(3)])
+                    q = coarse_own["cells"][C[0], C[1], C[2]]
+                    if (q < 0).any():
+                        raise ValueError("a new patch is not properly nested in the level below")
+                    rows = np.zeros((int(miss.sum()), 9), np.int64)
+                    rows[:, 0] = dst[miss]
+                    rows[:, 1] = plane
+                    for p in np.unique(q):
+                        sel = q == p
+                        cl = new_layouts[p]
+                        _, Yc, Zc = cl.dims
+                        rows[sel, 2] = cl.index("cells", *_local(cl, shape_c, C[:, sel]))
+                        rows[sel, 3] = Yc * Zc
+                        rows[sel, 4] = Zc
+                        rows[sel, 5] = _cell_plane(cl)
+                    for a in range(3):
+                        rows[:, 6 + a] = np.where(fine[a][miss] % 2 == 1, 1, -1)
+                    prolong.append(rows)
+                    continue
+                F = fine[:, miss]
+                rules = [_regrid_rule(nodes[a], F[a]) for a in range(3)]
+                pattern = sum((np.stack([r[1] != 0.0 for r in rule]).sum(0)) * 10**a
+                              for a, rule in enumerate(rules))
+                for pat in np.unique(pattern):
+                    sel = pattern == pat
+                    terms = []
+                    for tx in rules[0]:
+                        for ty in rules[1]:
+                            for tz in rules[2]:
+                                w = ((tx[1] * ty[1]) * tz[1])[sel]
+                                if np.all(w == 0.0):
+                                    continue
+                                Ic = np.stack([tx[0][sel] % shape_c[0], ty[0][sel] % shape_c[1],
+                                               tz[0][sel] % shape_c[2]])
+                                qq = coarse_own[name][Ic[0], Ic[1], Ic[2]]
+                                if (qq < 0).any():
+                                    raise ValueError("a new patch is not properly nested")
+                                off = np.empty(len(qq), np.int64)
+                                for p in np.unique(qq):
+                                    s2 = qq == p
+                                    off[s2] = new_layouts[p].index(name, *_local(new_layouts[p], shape_c, Ic[:, s2]))
+                                terms.append((off, w))
+                    interp.append((dst[miss][sel], terms))
+        out[L] = {"copies": Stencils.build(copies),
+                  "prolong": np.concatenate(prolong) if prolong else np.zeros((0, 9), np.int64),
+                  "interp": Stencils.build(interp)}
+    return out
