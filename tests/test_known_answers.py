@@ -69,9 +69,9 @@ def test_rotor_keeps_its_point_symmetry_and_div_b_zero():
     assert o.divergence() < 1e-14
 
 
-def _blast_checks(totals_before, totals_after, pressure, x, y, divergence):
+def _blast_checks(totals_before, totals_after, pressure, x, y, divergence, rtol=1e-10):
     assert np.abs(totals_after - totals_before).max() < 1e-12
-    np.testing.assert_allclose(pressure, pressure[::-1, ::-1], rtol=1e-10)
+    np.testing.assert_allclose(pressure, pressure[::-1, ::-1], rtol=rtol)
     hot = pressure > 1.0
     X, Y = np.meshgrid(x, y, indexing="ij")
     assert np.abs(X + Y)[hot].max() > 1.5 * np.abs(X - Y)[hot].max()
@@ -139,22 +139,42 @@ def test_gpu_alfven_wave_converges_at_second_order(gpu, kernels, precision):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("kernels,precision", [("split", "fp64"), ("fused", "fp64"),
-                                               ("fused", "mixed")])
-def test_gpu_rotor_keeps_its_point_symmetry_and_div_b_zero(gpu, kernels, precision):
+@pytest.mark.parametrize("kernels", ["split", "fused"])
+def test_gpu_rotor_keeps_its_point_symmetry_and_div_b_zero(gpu, kernels):
     from wellbalanced_amd import MHD2D
 
-    model = MHD2D.rotor(32, kernels=kernels, precision=precision)
+    model = MHD2D.rotor(32, kernels=kernels)
     model.run(0.05)
     data = model.dataset()
     density = data["density"]
     np.testing.assert_allclose(density, density[::-1, ::-1], rtol=1e-10)
     assert data["pressure"].min() > 0.0
     assert model.divergence() < 1e-14
-    if precision == "fp64":
-        o = oracle2d(initial.rotor(32))
-        o.run(0.05)
-        _same_2d(model, o)
+    o = oracle2d(initial.rotor(32))
+    o.run(0.05)
+    _same_2d(model, o)
+
+
+@pytest.mark.gpu
+def test_gpu_mixed_rotor_stays_near_fp64_and_conserves(gpu):
+    """The mixed path on the rotor, advanced at the FP64 path's step: FP32 rounding in
+    the fluxes is not mirror-symmetric, and the limiters' switches at the rotor's sharp
+    edge amplify it, so the point symmetry holds to about 6e-3 (not 1e-10) and the
+    density differs from FP64 by L1 1.0e-4, L-inf 3.5e-3 at t = 0.05; div B stays zero
+    and the pressure positive."""
+    from wellbalanced_amd import MHD2D
+
+    fp64 = MHD2D.rotor(32, kernels="fused")
+    mixed = MHD2D.rotor(32, kernels="fused", precision="mixed")
+    while fp64.time < 0.05 * (1.0 - 1e-14):
+        dt = min(fp64.stable_step(), 0.05 - fp64.time)
+        fp64.step(dt)
+        mixed.step(dt)
+    a, b = fp64.dataset()["density"], mixed.dataset()["density"]
+    assert np.mean(np.abs(a - b)) < 2e-4
+    assert np.abs(a - b).max() < 1e-2
+    assert mixed.divergence() < 1e-14
+    assert mixed.dataset()["pressure"].min() > 0.0
 
 
 @pytest.mark.gpu
@@ -167,8 +187,10 @@ def test_gpu_blast_wave_conserves_and_runs_along_the_field(gpu, kernels, precisi
     before = np.array(list(model.totals().values()))
     model.run(0.1)
     data = model.dataset()
+    # FP32 rounding in the mixed fluxes is not mirror-symmetric: the point symmetry holds
+    # to its level (9.5e-8 of the largest pressure measured), FP64 to 1e-10
     _blast_checks(before, np.array(list(model.totals().values())), data["pressure"], data["x"],
-                  data["y"], model.divergence())
+                  data["y"], model.divergence(), rtol=1e-6 if precision == "mixed" else 1e-10)
 
 
 @pytest.mark.gpu

@@ -271,13 +271,17 @@ class MHD3D:
             a, b = (c + 1) % 3, (c + 2) % 3
             call("wb_edge3d", ptr(w["prim"]), ptr(w[f"s{a}"]), ptr(w[f"s{b}"]), nx, ny, nz, c,
                  ptr(w[f"e{c}"]), stream)
+        if self.resistivity:  # into the fluxes and edges before the update (mhd3d.py:631-632)
+            call("wb_resistive3d", ptr(w["f0"]), ptr(w["f1"]), ptr(w["f2"]), nx, ny, nz,
+                 *self.spacing, self.resistivity, ptr(w["s0"]), ptr(w["s1"]), ptr(w["s2"]),
+                 ptr(w["e0"]), ptr(w["e1"]), ptr(w["e2"]), stream)
         call("wb_mhd3d_update", ptr(src), ptr(dst), ptr(w["s0"]), ptr(w["s1"]), ptr(w["s2"]),
              ptr(w["e0"]), ptr(w["e1"]), ptr(w["e2"]), nx, ny, nz, ptr(self._dt), *self.spacing,
              int(heun), stream)
-        self._resistive(src, dst, heun, stream)
 
     def _resistive(self, src, dst, heun, stream) -> None:
-        """The resistive terms (mhd3d.py:561-602) on top of the stage just run."""
+        """The resistive terms (mhd3d.py:561-602) on top of the fused stage just run (the
+        split stage adds them into its fluxes and edges before the update)."""
         if not self.resistivity:
             return
         import torch
