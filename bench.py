@@ -63,6 +63,11 @@ def parse():
     p.add_argument("--waves", type=int, default=None, help="waves/SIMD of the mixed stage")
     p.add_argument("--calibrate", action="store_true",
                    help="also run one device copy of 1 GiB (the traffic counters' reference)")
+    p.add_argument("--accuracy-steps", type=int, default=200,
+                   help="steps of the same-dt mixed-vs-FP64 error run (the config's 200)")
+    p.add_argument("--with-c5", dest="with_c5", action="store_true", default=True,
+                   help="add the refined learned-indicator step's summary (default)")
+    p.add_argument("--no-c5", dest="with_c5", action="store_false")
     p.add_argument("--workload", default="turbulence3d",
                    choices=("turbulence3d", "ot2d", "blast2d-amr", "harris2d", "blast3d-amr",
                             "harris2d-resistive", "plasmoid2d-amr"))
@@ -89,7 +94,7 @@ def time_model(model, steps, warmup):
     return seconds, sum(stage_ms) / max(1, len(stage_ms))
 
 
-def cpu_baseline(cells_sample: int, budget: float):
+def cpu_baseline(cells_sample: int, budget: float, max_steps: int = 100):
     """The NumPy oracle step (one core) on a turbulence box of ``cells_sample``^3."""
     from tests.helpers import oracle3d
     from wellbalanced_amd import initial
@@ -102,7 +107,7 @@ def cpu_baseline(cells_sample: int, budget: float):
         oracle.step()
         steps += 1
         elapsed = time.perf_counter() - t0
-        if elapsed >= budget or steps >= 100:
+        if elapsed >= budget or steps >= max_steps:
             break
     return cells_sample**3 * steps / elapsed, steps, elapsed
 
@@ -172,18 +177,15 @@ def run_turbulence3d(args):
     rate64 = cells * args.steps / sec64
     result_model, sec, stage_ms, precision = fp64, sec64, stage64, "fp64"
     l1 = linf = 0.0
+    accuracy = {}
     if mixed is not None:
         mixed.check_interval = 0
         secm, stagem = time_model(mixed, args.steps, args.warmup)
         result_model, sec, stage_ms, precision = mixed, secm, stagem, "mixed"
         fp64.check()
         mixed.check()
-        a = fp64.dataset()["density"]
-        b = mixed.dataset()["density"]
-        import numpy as np
-
-        l1 = float(np.mean(np.abs(a - b)))
-        linf = float(np.max(np.abs(a - b)))
+        accuracy = same_dt_error_3d(problem, args.accuracy_steps, tuned)
+        l1, linf = accuracy["l1"], accuracy["linf"]
     else:
         fp64.check()
     value = cells * args.steps / sec
@@ -219,6 +221,7 @@ def run_turbulence3d(args):
         "speedup_vs_fp64": value / rate64,
         "l1_density_vs_fp64": l1,
         "linf_density_vs_fp64": linf,
+        "error_run": accuracy.get("note"),
         "hbm_fraction_algorithmic": value * 5 * STATE_BYTES_3D / (HBM_PEAK_GBS * 1e9),
         "fp64_stage_ms": stage64,
         "roofline": {
@@ -237,15 +240,59 @@ def run_turbulence3d(args):
     if not args.no_cpu:
         sample = 48
         cpu_rate, steps, elapsed = cpu_baseline(sample, args.cpu_seconds)
+        big_rate, big_steps, big_elapsed = cpu_baseline(96, 0.0, max_steps=2)
         out["cpu_baseline"] = {
             "value": cpu_rate,
             "unit": "zone-updates/s",
             "cores": 1,
             "kind": "port",
             "sample": f"NumPy FP64 oracle step, turbulence {sample}^3, {steps} steps in {elapsed:.1f} s",
+            "per_cell_check": {"cells": 96**3, "steps": big_steps, "seconds": big_elapsed,
+                               "zone_updates_per_s": big_rate},
             "host": host_info(),
         }
+    if args.with_c5:
+        out["c5"] = c5_summary(args)
     return out
+
+
+def same_dt_error_3d(problem, steps, tuned):
+    """L1 / L-inf of the mixed density against FP64 after ``steps`` steps of both at the
+    FP64 path's stable step (so that no time offset enters the error)."""
+    import numpy as np
+
+    from wellbalanced_amd import MHD3D
+
+    pair = {}
+    for precision in ("fp64", "mixed"):
+        m = MHD3D.from_problem(problem, kernels="fused", precision=precision)
+        for key, value in tuned.get(precision, {}).items():
+            setattr(m, key, value)
+        m.check_interval = 0
+        pair[precision] = m
+    for _ in range(steps):
+        dt = pair["fp64"].stable_step()
+        pair["fp64"].step(dt)
+        pair["mixed"].step(dt)
+    pair["fp64"].check()
+    pair["mixed"].check()
+    a = pair["fp64"].dataset()["density"]
+    b = pair["mixed"].dataset()["density"]
+    return {"l1": float(np.mean(np.abs(a - b))), "linf": float(np.max(np.abs(a - b))),
+            "note": f"mixed vs FP64 density after {steps} steps of both at the FP64 stable step "
+                    f"(t = {pair['fp64'].time:.6g})"}
+
+
+def c5_summary(args):
+    """The refined 3D step of the learned-indicator workload (see run_blast3d_amr), in
+    short: its mixed and FP64 zone-updates/s, their ratio, the L1 error and the block
+    stage's fraction of the HBM roofline."""
+    line = run_blast3d_amr(args, judge=False)
+    return {k: line[k] for k in ("value", "fp64_zone_updates_per_s", "speedup_vs_fp64",
+                                 "l1_density_vs_fp64", "linf_density_vs_fp64", "ms_per_step",
+                                 "hbm_fraction_algorithmic")} | {
+        "stage_frac": line["roofline"]["frac"], "stage_ms": line["roofline"]["stage_ms"],
+        "workload": line["config"]["workload"], "cells": line["config"]["cells"]}
 
 
 STATE_BYTES_2D = 56  # 6 FP64 cell fields + 1 FP64 node (A_z) per cell
@@ -519,51 +566,134 @@ def _footprint(h, level: int = 1):
     return mask
 
 
-def _hierarchy_from_flags(n0, flags1, flags2):
-    """Level 1 over the base blocks holding a flagged cell, level 2 over the level-1
-    blocks holding a strongly flagged base cell (nesting added by the hierarchy)."""
+def _restrict_arrays(cons, pots):
+    """Haloed uniform arrays (``cons (5, n+4, n+4, n+4)``, edge potentials) restricted to
+    half the resolution: each coarse cell the mean of its 8 children, each coarse edge
+    the mean of the two fine edges along it (halos zero)."""
     import numpy as np
 
-    from wellbalanced_amd.amr3d_maps import Hierarchy3D
+    n = cons.shape[1] - 4
+    m = n // 2
+    inner = cons[:, 2:-2, 2:-2, 2:-2].reshape(5, m, 2, m, 2, m, 2)
+    out = np.zeros((5, m + 4, m + 4, m + 4))
+    out[:, 2:-2, 2:-2, 2:-2] = inner.mean(axis=(2, 4, 6))
+    coarse = []
+    for c, p in enumerate(pots):
+        ext = [n + (a != c) for a in range(3)]
+        e = p[tuple(slice(2, 2 + ext[a]) for a in range(3))]
+        e = e[tuple(slice(None) if a == c else slice(None, None, 2) for a in range(3))]
+        lo = e[tuple(slice(0, None, 2) if a == c else slice(None) for a in range(3))]
+        hi = e[tuple(slice(1, None, 2) if a == c else slice(None) for a in range(3))]
+        q = np.zeros(tuple(m + 4 + (a != c) for a in range(3)))
+        q[tuple(slice(2, 2 + m + (a != c)) for a in range(3))] = 0.5 * lo + 0.5 * hi
+        coarse.append(q)
+    return out, coarse
 
-    h = Hierarchy3D(n0)
-    h.refine_flagged(0, flags1)
-    if len(h.levels) > 1 and flags2.any():
-        cells = np.argwhere(flags2)
-        parents = {tuple(int(v) for v in c // 8) for c in cells}
-        parents = [p for p in parents if p in h.levels[1]]
-        if parents:
-            h.refine(1, parents)
-    return h
+
+def _hierarchy_buffer(model, levels):
+    """A host buffer of an `AMRMHD3D` whose every block copies its own level of the
+    haloed uniform arrays ``levels[L] = (cons, pots)``."""
+    import numpy as np
+
+    from wellbalanced_amd.amr3d_maps import block_views
+
+    buf = np.zeros(model.tables.total)
+    b = model.hierarchy.block
+    for row in model.tables.blocks:
+        cons, pots = levels[int(row["level"])]
+        cells, views = block_views(buf, int(row["off"]), b)
+        o = [int(row[a]) * b for a in ("bx", "by", "bz")]
+        cells[:, 2:-2, 2:-2, 2:-2] = cons[:, 2 + o[0] : 2 + o[0] + b, 2 + o[1] : 2 + o[1] + b,
+                                          2 + o[2] : 2 + o[2] + b]
+        for c in range(3):
+            sl = tuple(slice(2, 2 + b + (a != c)) for a in range(3))
+            src = tuple(slice(2 + o[a], 2 + o[a] + b + (a != c)) for a in range(3))
+            views[c][sl] = pots[c][src]
+    return buf
 
 
-def run_blast3d_amr(args):
-    """The learned-indicator refined step: a blast in Alfvenic turbulence on a 128^3
-    base with 16^3 blocks and 3 levels, refined by the CNN indicator (trained on the
-    GPU on the Loehner flags 8 steps ahead) and, for comparison, by the plain Loehner
-    flags; the hierarchies start from the uniform run at step 12."""
+def judge_errors(problem, n0, hierarchies, end_time, factor: int = 4):
+    """The plain estimator's question turned on the solution.  A uniform FP64 run
+    ``factor`` times finer than the base (the finest level of the hierarchies) is the
+    reference; every other run starts from its initial state restricted to its own
+    levels (cell means, edge means), so that a refined block starts from the
+    reference's data, and runs in FP64 to ``end_time``: each hierarchy, the unrefined
+    base grid and a uniform grid of twice the base.  The level-0 density (covered
+    cells averaged down) against the reference's, averaged onto the base cells: L1
+    over all base cells and over those where the reference departs from 1 by more
+    than 0.05 (the blast and its shell)."""
     import numpy as np
     import torch
 
-    from wellbalanced_amd import MHD3D, cnn, indicator, initial
+    from wellbalanced_amd import MHD3D, initial
     from wellbalanced_amd.amr3d import AMRMHD3D
 
-    n0, every, lead, start_step = 128, 4, 8, 12
+    ref = MHD3D.from_problem(initial.blast_turbulence_3d(factor * n0, seed=0), kernels="fused",
+                             precision="fp64")
+    levels = {int(np.log2(factor)): ref.arrays()}
+    for L in range(int(np.log2(factor)) - 1, -1, -1):
+        levels[L] = _restrict_arrays(*levels[L + 1])
+    ref.check_interval = 0
+    ref_steps = ref.run(end_time)
+    r = ref.dataset()["density"].reshape(n0, factor, n0, factor, n0, factor).mean(axis=(1, 3, 5))
+    del ref
+    torch.cuda.empty_cache()
+    blast = np.abs(r - 1.0) > 0.05
+    out = {"reference": f"uniform {factor * n0}^3 FP64 run ({ref_steps} steps), averaged onto "
+                        f"the {n0}^3 base; every run starts from its initial state restricted",
+           "end_time": end_time, "blast_cells": int(blast.sum())}
+
+    def errors(rho):
+        d = np.abs(rho - r)
+        return {"l1": float(d.mean()), "l1_blast": float(d[blast].mean())}
+
+    opts = dict(gamma=problem.options.get("gamma", 5.0 / 3.0),
+                mean_field=problem.options["mean_field"],
+                limiter=problem.options.get("limiter", "minmod"), precision="fp64")
+    for name, h in hierarchies.items():
+        m = AMRMHD3D(h, **opts)
+        m.load_buffer(_hierarchy_buffer(m, levels))
+        m.check_interval = 0
+        steps = 0
+        while m.time < end_time * (1.0 - 1e-14):
+            m.step(min(m.stable_step(), end_time - m.time))
+            steps += 1
+        m.check()
+        out[name] = {**errors(m.level_state(0)[0][0][2:-2, 2:-2, 2:-2]), "steps": steps,
+                     "leaf_cells": h.leaf_cells(), "blocks": [len(s) for s in h.levels]}
+        del m
+        torch.cuda.empty_cache()
+    for name, L in (("unrefined", 0), ("uniform_2x", 1)):
+        u = MHD3D.from_problem(initial.blast_turbulence_3d(n0 * 2**L, seed=0), kernels="fused",
+                               precision="fp64")
+        u.load_arrays(*levels[L])
+        u.check_interval = 0
+        steps = u.run(end_time)
+        u.check()
+        rho = u.dataset()["density"]
+        if L:
+            rho = rho.reshape(n0, 2, n0, 2, n0, 2).mean(axis=(1, 3, 5))
+        out[name] = {**errors(rho), "steps": steps, "cells": (n0 * 2**L) ** 3}
+        del u
+    return out
+
+
+def run_blast3d_amr(args, judge: bool = True):
+    """The learned-indicator refined step: a blast in Alfvenic turbulence on a 128^3
+    base with 16^3 blocks and 3 levels, refined by the CNN indicator (trained on the
+    GPU on the Loehner flags 8 steps ahead) and, for comparison, by the plain Loehner
+    flags; the hierarchies start from the uniform run at step 12.  The mixed-vs-FP64
+    error is taken after 20 steps of both at the FP64 stable step; with ``judge`` the
+    solution error of both hierarchies is judged against a finer uniform run."""
+    import numpy as np
+    import torch
+
+    from wellbalanced_amd import MHD3D, autotune, indicator, initial
+    from wellbalanced_amd.amr3d import AMRMHD3D
+
+    n0, lead, start_step = 128, 8, 12
     problem = initial.blast_turbulence_3d(n0, seed=0)
-    # train the indicator on a uniform run: features at steps 4..40, flags 8 steps later
-    uni = MHD3D.from_problem(problem, kernels="fused", precision="mixed")
-    uni.check_interval = 0
-    net = cnn.Indicator(uni.n, lr=3e-3, seed=0)
-    uni.advance(every)
-    net.fit_normalisation(uni)
-    samples = indicator.lookahead_samples(uni, net, snapshots=10, every=every, lead=lead)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    losses = net.fit([(x, (later > 0.8).float().reshape(-1).contiguous())
-                      for _, x, _, later in samples], 150)
-    torch.cuda.synchronize()
-    train_s = time.perf_counter() - t0
-    del uni, samples
+    net, losses, train_s = indicator.train_lookahead(problem, lead=lead)
     # the start state and the flags each indicator raises there; the truth 8 steps on
     start = MHD3D.from_problem(problem, kernels="fused", precision="fp64")
     start.check_interval = 0
@@ -574,19 +704,18 @@ def run_blast3d_amr(args):
     later.check_interval = 0
     later.advance(start_step + lead)
     truth = (indicator.lohner(later) > indicator.CUTOFF).cpu().numpy()
+    end_time = later.time
     del later
     est_np, logit_np = est.cpu().numpy(), logit.cpu().numpy()
     hierarchies = {
-        "learned": _hierarchy_from_flags(n0, logit_np > 0.0, logit_np > 3.0),
-        "plain": _hierarchy_from_flags(n0, est_np > indicator.CUTOFF, est_np > 0.95),
+        "learned": indicator.block_hierarchy(n0, logit_np > 0.0, logit_np > 3.0),
+        "plain": indicator.block_hierarchy(n0, est_np > indicator.CUTOFF, est_np > 0.95),
     }
     coverage = {k: [float(_footprint(h, L)[truth].mean()) for L in (1, 2)]
                 for k, h in hierarchies.items()}
     # the refined step on the learned hierarchy, mixed and FP64
     h = hierarchies["learned"]
     res, models = {}, {}
-    from wellbalanced_amd import autotune
-
     launch = {}
     for precision in ("fp64", "mixed"):
         m = AMRMHD3D.from_uniform(start, h, precision=precision)
@@ -597,24 +726,36 @@ def run_blast3d_amr(args):
         m.check()
         res[precision] = (m.cells * steps / sec, sec, stage_ms, steps)
         models[precision] = m
-    a, b = models["fp64"].level_state(0)[0][0], models["mixed"].level_state(0)[0][0]
+    # the error of the mixed step: both from the start state at the FP64 stable step
+    pair = {p: AMRMHD3D.from_uniform(start, h, precision=p) for p in ("fp64", "mixed")}
+    for _ in range(20):
+        dt = pair["fp64"].stable_step()
+        for m in pair.values():
+            m.step(dt)
+    a, b = pair["fp64"].level_state(0)[0][0], pair["mixed"].level_state(0)[0][0]
     d = np.abs(a - b)[2:-2, 2:-2, 2:-2]
+    del pair
     m = models["mixed"]
     rate, sec, stage_ms, steps = res["mixed"]
+    extra = {"blocks_per_level": [len(s) for s in h.levels], "launch": launch,
+             "leaf_cells": h.leaf_cells(), "divergence": m.divergence(),
+             "indicator": {"train_seconds": train_s, "loss_first": losses[0],
+                           "loss_last": losses[-1],
+                           "flags_8_steps_on_covered_by_levels_1_2": coverage,
+                           "flagged_now": {"learned": int((logit_np > 0).sum()),
+                                           "plain": int((est_np > indicator.CUTOFF).sum())},
+                           "blocks_plain": [len(s) for s in hierarchies["plain"].levels]},
+             "l1_note": "level-0 (averaged-down) density, mixed vs FP64 after 20 steps of "
+                        "both at the FP64 stable step"}
+    del models
+    torch.cuda.empty_cache()
+    if judge:
+        extra["judge"] = judge_errors(problem, n0, hierarchies, end_time)
     return _line(rate, res["fp64"][0], steps, args, sec,
                  "3D blast in Alfvenic turbulence, 128^3 base, 3 levels of 16^3 blocks refined "
                  "by the learned indicator, minmod, CFL 0.3, SSP-RK2, steps replayed from HIP "
                  "graphs of 10 steps", m.cells, "mixed",
-                 float(d.mean()), float(d.max()), stage_ms, STATE_BYTES_3D,
-                 {"blocks_per_level": [len(s) for s in h.levels], "launch": launch,
-                  "leaf_cells": h.leaf_cells(), "divergence": m.divergence(),
-                  "indicator": {"train_seconds": train_s, "loss_first": losses[0],
-                                "loss_last": losses[-1],
-                                "flags_8_steps_on_covered_by_levels_1_2": coverage,
-                                "flagged_now": {"learned": int((logit_np > 0).sum()),
-                                                "plain": int((est_np > indicator.CUTOFF).sum())},
-                                "blocks_plain": [len(s) for s in hierarchies["plain"].levels]},
-                  "l1_note": "level-0 (averaged-down) density, mixed vs FP64"})
+                 float(d.mean()), float(d.max()), stage_ms, STATE_BYTES_3D, extra)
 
 
 if __name__ == "__main__":

@@ -75,3 +75,48 @@ def recall_at_count(score: torch.Tensor, target: torch.Tensor, count: int) -> fl
     """The fraction of ``target`` cells among the ``count`` highest-scoring cells."""
     idx = torch.topk(score.reshape(-1), count).indices
     return float(target.reshape(-1)[idx].float().sum() / target.float().sum())
+
+
+def train_lookahead(problem, *, snapshots: int = 10, every: int = 4, lead: int = 8,
+                    iterations: int = 150, lr: float = 3e-3, seed: int = 0, cutoff: float = CUTOFF):
+    """The learned indicator trained on the GPU on a uniform mixed run of ``problem``:
+    the features of ``snapshots`` states, one every ``every`` steps, against the Löhner
+    flags ``lead`` steps later.  Returns (network, losses, training seconds)."""
+    import time
+
+    from . import cnn
+    from .mhd3d import MHD3D
+
+    uni = MHD3D.from_problem(problem, kernels="fused", precision="mixed")
+    uni.check_interval = 0
+    net = cnn.Indicator(uni.n, lr=lr, seed=seed)
+    uni.advance(every)
+    net.fit_normalisation(uni)
+    samples = lookahead_samples(uni, net, snapshots=snapshots, every=every, lead=lead,
+                                cutoff=cutoff)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = net.fit([(x, (later > cutoff).float().reshape(-1).contiguous())
+                      for _, x, _, later in samples], iterations)
+    torch.cuda.synchronize()
+    return net, losses, time.perf_counter() - t0
+
+
+def block_hierarchy(n0: int, flags1, flags2, block: int = 16):
+    """A 3-level hierarchy of ``block``^3 blocks over an ``n0``^3 base: level 1 over the
+    base blocks holding a cell of ``flags1``, level 2 over the level-1 blocks holding a
+    base cell of ``flags2`` (each parent's 8 children), the nesting added."""
+    import numpy as np
+
+    from .amr3d_maps import Hierarchy3D
+
+    h = Hierarchy3D(n0, block=block)
+    h.refine_flagged(0, np.asarray(flags1, bool))
+    flags2 = np.asarray(flags2, bool)
+    if len(h.levels) > 1 and flags2.any():
+        half = block // 2  # a level-1 block covers half a base block per axis
+        parents = {tuple(int(v) for v in c // half) for c in np.argwhere(flags2)}
+        parents = [p for p in parents if p in h.levels[1]]
+        if parents:
+            h.refine(1, parents)
+    return h
