@@ -123,3 +123,39 @@ def test_a_graph_refuses_to_replay_after_a_regrid_or_a_new_tiling(gpu):
     model.regrid(tags, max_patch_size=8)
     with pytest.raises(RuntimeError, match="capture a new StepGraph"):
         graph.replay()
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_lockstep_graph_equals_eager_steps_at_the_first_models_dt(gpu, dims):
+    """FP64 and mixed models captured in lockstep: each replay equals eager steps in
+    which the mixed model takes the FP64 model's stable step, and the follower's time
+    is the leader's."""
+    import torch
+
+    from wellbalanced_amd.graph import LockstepGraph
+
+    if dims == 2:
+        problem = initial.harris_sheet(96, 48, length=3.2, noise=1e-2, seed=2)
+        make = lambda p: MHD2D.from_problem(problem, kernels="fused", precision=p)
+    else:
+        problem = initial.turbulence_3d(24, seed=7)
+        make = lambda p: MHD3D.from_problem(problem, kernels="fused", precision=p)
+    eager = [make("fp64"), make("mixed")]
+    graphed = [make("fp64"), make("mixed")]
+    graph = LockstepGraph(graphed, steps=2)
+    for replay in range(3):
+        graph.replay()
+        for _ in range(2):
+            dt = eager[0].stable_step()
+            for m in eager:
+                m.step(dt)
+        _ = eager[1].totals()
+        torch.cuda.synchronize()
+        for g, e in zip(graphed, eager):
+            np.testing.assert_array_equal(g.arrays()[0], e.arrays()[0])
+            for a, b in zip(g.arrays()[1], e.arrays()[1]):
+                np.testing.assert_array_equal(a, b)
+        assert graphed[1].time == graphed[0].time == eager[0].time
+    assert [m.steps for m in graphed] == [6, 6]
+    # the follower keeps its own dt: an eager step of it after the replays is its own
+    assert graphed[1].stable_step() == eager[1].stable_step()
