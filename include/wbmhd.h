@@ -286,8 +286,9 @@ int wb_cnn_wgrad(const uint16_t* x, int cin, const uint16_t* g16, int64_t nx, in
 
 /* ---- 2D patch hierarchy (refined, periodic) ------------------------------------ */
 /* A hierarchy lives in one FP64 buffer, patch after patch (each a 2D patch state as
- * above).  `patches` is an array of {int64 offset; int32 nx, ny; double dx, dy} per
- * patch; the int64 tables are offsets into the buffer built on the host from the
+ * above).  `patches` is an array of {int64 offset; int32 nx, ny; double dx, dy; int32
+ * walls, pad} per patch (walls: the patch's faces on domain walls, 1 x-low, 2 x-high,
+ * 4 y-low, 8 y-high: the stage zeroes their fluxes but the pressure's and E_z there); the int64 tables are offsets into the buffer built on the host from the
  * geometry (wellbalanced_amd/amr_maps.py).  Replaces patchwork's exchange_halos,
  * FluxRegister.reflux and restrict as MHD2D calls them (mhd.py:744-777,889-994). */
 
@@ -305,6 +306,14 @@ int wb_amr2d_fill_level(double* buf, const long long* cdst, const long long* csr
                         const double* iwy, int nic, const long long* jdst,
                         const long long* jbase, const long long* jsx, const long long* jsy,
                         const long long* jkx, const long long* jky, int nin, void* stream);
+
+/* The outflow or wall ghosts of the patches of a level that lie on a non-periodic
+ * domain face, after the level's fill (MHD2D._fill_halos, mhd.py:748-777, with the
+ * rules of mhd.py:298-353): rows of 3 int32 {patch, faces (1 x-low, 2 x-high, 4 y-low,
+ * 8 y-high), walls (bit 0 x walls, bit 1 y walls)}, one per such patch; axis x first,
+ * then y, each over the whole haloed extent of the other axis. */
+int wb_amr2d_domain_faces(double* buf, const void* patches, const int32_t* rows, int nrows,
+                          void* stream);
 
 /* One level pair's coarsening in one launch: the na covered coarse cells := mean of their
  * four fine children (arows, 5 int64 each) and the nr coarse A_z nodes := the fine node

@@ -448,7 +448,8 @@ class OracleAMR2D:
             f = fx if axis == 0 else fy
             lower, upper = box[axis][0] == 0, box[axis][1] == shape[axis]
             n = box[axis][1] - box[axis][0]
-            for on, face, node in ((lower, HALO - 1, 0), (upper, HALO - 1 + n, -1)):
+            # the sweep's face 0 is the patch's low boundary face, face n its high one
+            for on, face, node in ((lower, 0, 0), (upper, n, -1)):
                 if not on:
                     continue
                 if axis == 0:

@@ -82,6 +82,7 @@ SIGNATURES: dict[str, tuple[type, list]] = {
     "wb_cnn_wgrad": (I, [P, I, P, I64, I64, I64, P, I, P, P]),
     "wb_amr2d_fill_level": (I, [P] * 5 + [I, P, P, I] + [P] * 7 + [I] + [P] * 6 + [I, P]),
     "wb_amr2d_coarsen": (I, [P, P, I, P, I, P]),
+    "wb_amr2d_domain_faces": (I, [P, P, P, I, P]),
     "wb_amr2d_regrid_copy": (I, [P] * 6 + [I, P, P, I, P]),
     "wb_amr2d_prolong": (I, [P] * 8 + [I] + [P] * 5 + [I, P]),
     "wb_amr2d_leaf_values": (I, [P, P, P, I, D, D, D, P, P]),

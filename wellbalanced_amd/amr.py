@@ -1,4 +1,4 @@
-"""``AMRMHD2D``: the 2D MHD step on a refined, periodic patch hierarchy.
+"""``AMRMHD2D``: the 2D MHD step on a refined patch hierarchy.
 
 The reference runs ``MHD2D`` on patchwork patches of several levels that all take
 the same time step (``mhd.py:40-44``): per stage the halos from the hierarchy, the
@@ -8,7 +8,9 @@ between the stages; after the Heun average one reflux and the restriction
 ping-ponged as in `MHD2D`); a stage is
 
 1. the halo fill, level by level: same-level copies, then bilinear cells and
-   quadratic A_z nodes from the level below (gather tables of `amr_maps`);
+   quadratic A_z nodes from the level below (gather tables of `amr_maps`), then on a
+   non-periodic axis the outflow or wall ghosts of the level's patches on the domain
+   faces (``mhd.py:748-777``), which the next level's interpolation may read;
 2. ONE persistent launch over every tile of every patch (`k_stage2d_batched`: the
    fused 2D stage, workgroups pulling (patch, tile, segment) items from a counter),
    which updates every patch cell, covered coarse cells included;
@@ -31,21 +33,24 @@ import torch
 
 from . import _lib, amr_maps
 from .hierarchy import AMRHierarchy
-from .mhd import CONSERVED, HALO, LIMITER_CODES, PRECISIONS, _device
+from .mhd import BOUNDARIES, CONSERVED, HALO, LIMITER_CODES, PRECISIONS, _device
 
 _PATCH = np.dtype([("off", np.int64), ("nx", np.int32), ("ny", np.int32),
-                   ("dx", np.float64), ("dy", np.float64)])
+                   ("dx", np.float64), ("dy", np.float64), ("walls", np.int32),
+                   ("pad", np.int32)])
 _ITEM = np.dtype([("patch", np.int32), ("tile", np.int32), ("seg", np.int32)])
 
 
 class AMRMHD2D:
-    """Ideal MHD on every patch of a periodic 2D `AMRHierarchy` (see the module)."""
+    """MHD on every patch of a 2D `AMRHierarchy` (see the module).  ``boundaries`` maps
+    a non-periodic axis ('x' or 'y') to 'outflow' (the default) or 'wall'
+    (``mhd.py:410-455``)."""
 
     def __init__(self, hierarchy: AMRHierarchy, *, gamma: float = 5.0 / 3.0,
                  mean_field=(0.0, 0.0), limiter: str = "minmod", cfl: float = 0.4,
                  resistivity: float = 0.0, conduction: tuple[float, float] = (0.0, 0.0),
-                 prefix: str = "mhd_", precision: str = "fp64", tile: int = 62, nxs: int = 16,
-                 device=None) -> None:
+                 boundaries: dict[str, str] | None = None, prefix: str = "mhd_",
+                 precision: str = "fp64", tile: int = 62, nxs: int = 16, device=None) -> None:
         if hierarchy.ndim != 2:
             raise ValueError("AMRMHD2D needs a 2D hierarchy")
         if limiter not in LIMITER_CODES:
@@ -62,6 +67,22 @@ class AMRMHD2D:
             raise ValueError(f"precision is one of {PRECISIONS}")
         if tile not in (30, 62, 126):
             raise ValueError("tile is 30, 62 or 126 cells")
+        kinds = {"x": "outflow", "y": "outflow"}
+        for axis_name, kind in (boundaries or {}).items():
+            if axis_name not in kinds or kind not in BOUNDARIES:
+                raise ValueError(
+                    f"boundaries maps 'x' or 'y' to one of {BOUNDARIES}, "
+                    f"got {axis_name!r}: {kind!r}"
+                )
+            axis = "xy".index(axis_name)
+            if hierarchy.periodic[axis]:
+                raise ValueError(f"the hierarchy is periodic along {axis_name}")
+            if kind == "wall" and mean_field[axis] != 0.0:
+                raise ValueError(f"a wall normal to {axis_name} needs a zero mean field along it")
+            kinds[axis_name] = kind
+        self.walls = tuple(kinds[name] == "wall" and not hierarchy.periodic[a]
+                           for a, name in enumerate("xy"))
+        self.periodic = tuple(bool(p) for p in hierarchy.periodic)
         self.hierarchy = hierarchy
         self.gamma = float(gamma)
         self.mean_field = (float(mean_field[0]), float(mean_field[1]))
@@ -163,9 +184,22 @@ class AMRMHD2D:
         patches = np.zeros(len(self.layouts), _PATCH)
         spatch = np.zeros(len(self.layouts))
         items = []
+        # the patches on a non-periodic domain face, per level: rows {patch, faces, walls}
+        wall_bits = int(self.walls[0]) | (int(self.walls[1]) << 1)
+        faces = np.zeros(len(self.layouts), np.int64)
+        self._faces = {}
+        for L in range(self.nlev):
+            rows = m.domain_faces.get(L, np.zeros((0, 2), np.int64))
+            faces[rows[:, 0]] = rows[:, 1]
+            table = np.zeros((len(rows), 3), np.int32)
+            table[:, :2] = rows
+            table[:, 2] = wall_bits
+            self._faces[L] = (self._dev(table.reshape(-1), torch.int32), len(rows))
+        # a face is a wall face if its axis has walls: x faces bits 0-1, y faces bits 2-3
+        wall_faces = faces & ((3 if self.walls[0] else 0) | (12 if self.walls[1] else 0))
         for k, lay in enumerate(self.layouts):
             dx, dy = self.hierarchy.spacing(lay.level)
-            patches[k] = (lay.offset, lay.nx, lay.ny, dx, dy)
+            patches[k] = (lay.offset, lay.nx, lay.ny, dx, dy, int(wall_faces[k]), 0)
             spatch[k] = dx**-2 + dy**-2  # the diffusion limit's sum (mhd.py:961)
             for t in range((lay.ny + self.tile - 1) // self.tile):
                 for s in range((lay.nx + self.nxs - 1) // self.nxs):
@@ -270,7 +304,7 @@ class AMRMHD2D:
         return self.totals()[key]
 
     # -- the step ------------------------------------------------------------------------
-    def _fill_level(self, buf: torch.Tensor, L: int, stream: int) -> None:
+    def _fill_gathers(self, buf: torch.Tensor, L: int, stream: int) -> None:
         cd, cs, cpd, cps, nd, ns = self._copy[L]
         t = self._interp[L]
         _lib.call("wb_amr2d_fill_level", _lib.ptr(buf), _lib.ptr(cd), _lib.ptr(cs),
@@ -281,6 +315,13 @@ class AMRMHD2D:
     def _fill(self, buf: torch.Tensor, stream: int) -> None:
         for L in range(self.nlev):
             self._fill_level(buf, L, stream)
+
+    def _fill_level(self, buf: torch.Tensor, L: int, stream: int) -> None:
+        self._fill_gathers(buf, L, stream)
+        rows, n = self._faces[L]
+        if n:  # the domain faces of the level's patches (mhd.py:748-777)
+            _lib.call("wb_amr2d_domain_faces", _lib.ptr(buf), _lib.ptr(self._patches),
+                      _lib.ptr(rows), n, stream)
 
     def _coarsen(self, buf: torch.Tensor, L: int, stream: int) -> None:
         arows, na = self._average[L]
@@ -432,6 +473,8 @@ class AMRMHD2D:
         `wellbalanced_amd.regrid.regrid`."""
         if not self._ready:
             raise RuntimeError("set the state first (set_state)")
+        if not all(self.periodic):
+            raise NotImplementedError("regridding runs on periodic hierarchies")
         from . import regrid as rg
 
         old_h, old = self.hierarchy, self._buf[0]

@@ -23,7 +23,10 @@ NumPy and consumed by gather kernels:
 * ``average``: covered coarse cells, the mean of their four fine children;
 * ``restrict``: coarse nodes that coincide with fine nodes take the fine value.
 
-Domains are periodic on both axes (the refined set-ups this path runs).
+On a non-periodic axis the ghosts outside the domain are left to the outflow or
+wall rules (`wb_amr2d_domain_faces`, after each level's fill), interpolation stencils
+that reach outside the domain read them from a coarse patch's halo, and the nodes of
+a level run 0 .. n along that axis (on a periodic axis node n is node 0).
 """
 
 from __future__ import annotations
@@ -90,6 +93,7 @@ class Maps:
     reflux: dict = field(default_factory=dict)          # level -> arrays (coarse level)
     average: dict = field(default_factory=dict)         # level -> (coarse, plane_c, fine, fine_Y, plane_f)
     restrict: dict = field(default_factory=dict)        # level -> (coarse, fine)
+    domain_faces: dict = field(default_factory=dict)    # level -> rows (patch, faces)
 
 
 def layouts_of(hierarchy) -> tuple[list[PatchLayout], int]:
@@ -101,23 +105,31 @@ def layouts_of(hierarchy) -> tuple[list[PatchLayout], int]:
     return out, offset
 
 
-def _owner_tables(layouts, level, shape):
-    """Per level: the index of the first patch owning each cell (-1 if none) and node."""
+def _owner_tables(layouts, level, shape, periodic=(True, True)):
+    """Per level: the index of the first patch owning each cell (-1 if none) and node
+    (nodes 0 .. n - 1 along a periodic axis, where node n is node 0; 0 .. n along a
+    non-periodic one)."""
     nx, ny = shape
     cells = np.full((nx, ny), -1, np.int64)
-    nodes = np.full((nx, ny), -1, np.int64)  # periodic: node n is node 0
+    nodes = np.full((nx + (not periodic[0]), ny + (not periodic[1])), -1, np.int64)
     for k, lay in enumerate(layouts):
         if lay.level != level:
             continue
         (i0, i1), (j0, j1) = lay.box
         view = cells[i0:i1, j0:j1]
         view[view < 0] = k
-        ii = np.arange(i0, i1 + 1) % nx
-        jj = np.arange(j0, j1 + 1) % ny
+        ii = np.arange(i0, i1 + 1) % nx if periodic[0] else np.arange(i0, i1 + 1)
+        jj = np.arange(j0, j1 + 1) % ny if periodic[1] else np.arange(j0, j1 + 1)
         sub = nodes[np.ix_(ii, jj)]
         sub[sub < 0] = k
         nodes[np.ix_(ii, jj)] = sub
     return cells, nodes
+
+
+def _wrap(v, n, periodic):
+    """Level coordinates ``v`` on an axis of ``n`` cells: the periodic image in 0 .. n - 1,
+    or unchanged on a non-periodic axis."""
+    return v % n if periodic else v
 
 
 def _patch_arrays(layouts) -> dict:
@@ -143,13 +155,12 @@ def _points(ids, ext_a, ext_b):
 
 
 def build(hierarchy) -> Maps:
-    """All gather tables for a hierarchy whose axes are periodic (whole levels at once)."""
-    if not all(hierarchy.periodic):
-        raise NotImplementedError("refined 2D hierarchies run on periodic domains")
+    """All gather tables for a hierarchy (whole levels at once)."""
+    per = tuple(bool(p) for p in hierarchy.periodic)
     layouts, total = layouts_of(hierarchy)
     maps = Maps(layouts=layouts, total=total)
     nlev = hierarchy.max_level + 1
-    owners = [_owner_tables(layouts, L, hierarchy.level_shape(L)) for L in range(nlev)]
+    owners = [_owner_tables(layouts, L, hierarchy.level_shape(L), per) for L in range(nlev)]
     boxes = box_array(layouts)
     g = _patch_arrays(layouts)
     Y = g["ny"] + 2 * HALO
@@ -173,7 +184,11 @@ def build(hierarchy) -> Maps:
         k, a, b = k[ghost], a[ghost], b[ghost]
         fi = g["i0"][k] + a - HALO  # unwrapped level coordinates
         fj = g["j0"][k] + b - HALO
-        I, J = fi % shape[0], fj % shape[1]
+        # ghosts outside the domain on a non-periodic axis: the boundary rules fill them
+        inside = ((per[0] | ((fi >= 0) & (fi < shape[0])))
+                  & (per[1] | ((fj >= 0) & (fj < shape[1]))))
+        k, a, b, fi, fj = k[inside], a[inside], b[inside], fi[inside], fj[inside]
+        I, J = _wrap(fi, shape[0], per[0]), _wrap(fj, shape[1], per[1])
         src = cell_own[I, J]
         have = src >= 0
         q = src[have]
@@ -189,7 +204,7 @@ def build(hierarchy) -> Maps:
             ci0 = np.floor_divide(mfi - 1, 2)  # lower coarse cell of the stencil
             cj0 = np.floor_divide(mfj - 1, 2)
             q, qa, qb = _coarse_container(layouts, L - 1, ci0, cj0, 2, hierarchy, cells=True,
-                                          owner=owners[L - 1][0], boxes=boxes)
+                                          owner=owners[L - 1][0], boxes=boxes, periodic=per)
             maps.cell_interp[L] = (
                 cell(mk, ma, mb), plane[mk], cell(q, qa, qb), Y[q], plane[q],
                 np.where(mfi % 2 == 0, 0.75, 0.25), np.where(mfj % 2 == 0, 0.75, 0.25))
@@ -204,18 +219,21 @@ def build(hierarchy) -> Maps:
         k, a, b, isb = k[keep], a[keep], b[keep], boundary[keep]
         fi = g["i0"][k] + a - HALO
         fj = g["j0"][k] + b - HALO
-        I, J = fi % shape[0], fj % shape[1]
+        inside = ((per[0] | ((fi >= 0) & (fi <= shape[0])))
+                  & (per[1] | ((fj >= 0) & (fj <= shape[1]))))
+        k, a, b, isb, fi, fj = (x[inside] for x in (k, a, b, isb, fi, fj))
+        I, J = _wrap(fi, shape[0], per[0]), _wrap(fj, shape[1], per[1])
         src = node_own[I, J]
         # a boundary node keeps its value unless an earlier patch owns it
         take = (src >= 0) & ~(isb & (src >= k))
         q = src[take]
         nd = [node(k[take], a[take], b[take])]
-        ns = [node(q, (I[take] - g["i0"][q]) % shape[0] + HALO,
-                   (J[take] - g["j0"][q]) % shape[1] + HALO)]
+        ns = [node(q, _wrap(I[take] - g["i0"][q], shape[0], per[0]) + HALO,
+                   _wrap(J[take] - g["j0"][q], shape[1], per[1]) + HALO)]
         # the periodic duplicate node of a patch that spans the level is its own node 0
         dup = isb & (src == k)
-        sa = (I[dup] - g["i0"][k[dup]]) % shape[0] + HALO
-        sb = (J[dup] - g["j0"][k[dup]]) % shape[1] + HALO
+        sa = _wrap(I[dup] - g["i0"][k[dup]], shape[0], per[0]) + HALO
+        sb = _wrap(J[dup] - g["j0"][k[dup]], shape[1], per[1]) + HALO
         moved = (sa != a[dup]) | (sb != b[dup])
         kd = k[dup][moved]
         nd.append(node(kd, a[dup][moved], b[dup][moved]))
@@ -233,7 +251,7 @@ def build(hierarchy) -> Maps:
             bx = np.where(kx == 1, (mfi - 1) // 2 - 1, mfi // 2)
             by = np.where(ky == 1, (mfj - 1) // 2 - 1, mfj // 2)
             q, qa, qb = _coarse_container(layouts, L - 1, bx, by, 3, hierarchy, cells=False,
-                                          owner=owners[L - 1][0], boxes=boxes)
+                                          owner=owners[L - 1][0], boxes=boxes, periodic=per)
             maps.node_interp[L] = (node(mk, ma, mb), node(q, qa, qb), YN[q],
                                    np.ones(q.size, np.int64), kx, ky)
         else:
@@ -242,7 +260,32 @@ def build(hierarchy) -> Maps:
         maps.reflux[L - 1] = _reflux_table(layouts, owners, L, hierarchy)
         maps.average[L - 1] = _average_table(layouts, owners, L, hierarchy)
         maps.restrict[L - 1] = _restrict_table(layouts, owners, L, hierarchy)
+    maps.domain_faces = _domain_face_rows(layouts, hierarchy)
     return maps
+
+
+def _domain_face_rows(layouts, hierarchy) -> dict:
+    """Per level: the patches on a non-periodic domain face, as rows {patch, faces
+    (1 x-low, 2 x-high, 4 y-low, 8 y-high)} (`wb_amr2d_domain_faces` adds the walls)."""
+    out = {}
+    for L in range(hierarchy.max_level + 1):
+        shape = hierarchy.level_shape(L)
+        rows = []
+        for k, lay in enumerate(layouts):
+            if lay.level != L:
+                continue
+            faces = 0
+            for axis in range(2):
+                if hierarchy.periodic[axis]:
+                    continue
+                if lay.box[axis][0] == 0:
+                    faces |= 1 << (2 * axis)
+                if lay.box[axis][1] == shape[axis]:
+                    faces |= 2 << (2 * axis)
+            if faces:
+                rows.append((k, faces))
+        out[L] = np.array(rows, np.int64).reshape(-1, 2)
+    return out
 
 
 def _cat(parts, dtype=np.int64):
@@ -256,13 +299,22 @@ def box_array(layouts) -> np.ndarray:
     return np.array([[*lay.box[0], *lay.box[1]] for lay in layouts], np.int64).reshape(-1, 4)
 
 
-def _coarse_container(layouts, level, ci, cj, width, hierarchy, cells, owner=None, boxes=None):
+def _coarse_container(layouts, level, ci, cj, width, hierarchy, cells, owner=None, boxes=None,
+                      periodic=(True, True)):
     """For stencils whose lower corner is (ci, cj) and extent ``width`` per axis, a
     patch of ``level`` whose haloed range holds the whole stencil, and the stencil's
-    local corner there (the smallest non-negative periodic image).  The candidates are
+    local corner there (the smallest non-negative periodic image; on a non-periodic
+    axis the stencil may reach into the halo outside the domain).  The candidates are
     the owners of the stencil's corners and centre (``owner``: the level's cell-owner
     map, as `_owner_tables` gives it); stencils none of them holds try every patch."""
     shape = hierarchy.level_shape(level)
+    per = tuple(bool(p) for p in periodic)
+
+    def local(v, lo, n, p):
+        return (v - lo + HALO) % n if p else v - lo + HALO
+
+    def probe(v, n, p):  # a cell of the level to ask for its owner
+        return v % n if p else np.clip(v, 0, n - 1)
     ci = np.asarray(ci, np.int64)
     cj = np.asarray(cj, np.int64)
     n = len(ci)
@@ -278,13 +330,14 @@ def _coarse_container(layouts, level, ci, cj, width, hierarchy, cells, owner=Non
         todo = np.flatnonzero(q < 0)
         if todo.size == 0:
             break
-        k = owner[(ci[todo] + di) % shape[0], (cj[todo] + dj) % shape[1]]
+        k = owner[probe(ci[todo] + di, shape[0], per[0]), probe(cj[todo] + dj, shape[1], per[1])]
         have = k >= 0
         todo, k = todo[have], k[have]
         i0, i1, j0, j1 = box[k, 0], box[k, 1], box[k, 2], box[k, 3]
-        la = (ci[todo] - i0 + HALO) % shape[0]
-        lb = (cj[todo] - j0 + HALO) % shape[1]
-        fit = (la + width <= i1 - i0 + 2 * HALO + ext) & (lb + width <= j1 - j0 + 2 * HALO + ext)
+        la = local(ci[todo], i0, shape[0], per[0])
+        lb = local(cj[todo], j0, shape[1], per[1])
+        fit = ((la >= 0) & (lb >= 0) & (la + width <= i1 - i0 + 2 * HALO + ext)
+               & (lb + width <= j1 - j0 + 2 * HALO + ext))
         q[todo[fit]], qa[todo[fit]], qb[todo[fit]] = k[fit], la[fit], lb[fit]
     rest = np.flatnonzero(q < 0)
     if rest.size:
@@ -294,9 +347,9 @@ def _coarse_container(layouts, level, ci, cj, width, hierarchy, cells, owner=Non
             (i0, i1), (j0, j1) = lay.box
             hi_a = (i1 - i0) + 2 * HALO + ext
             hi_b = (j1 - j0) + 2 * HALO + ext
-            la = (ci[rest] - i0 + HALO) % shape[0]
-            lb = (cj[rest] - j0 + HALO) % shape[1]
-            ok = (q[rest] < 0) & (la + width <= hi_a) & (lb + width <= hi_b)
+            la = local(ci[rest], i0, shape[0], per[0])
+            lb = local(cj[rest], j0, shape[1], per[1])
+            ok = (q[rest] < 0) & (la >= 0) & (lb >= 0) & (la + width <= hi_a) & (lb + width <= hi_b)
             q[rest[ok]], qa[rest[ok]], qb[rest[ok]] = k, la[ok], lb[ok]
     if (q < 0).any():
         raise ValueError("a fine patch is not properly nested in the level below")
@@ -318,6 +371,10 @@ def _reflux_table(layouts, owners, L, hierarchy):
         for side in (0, 1):  # side 0: the fine patch is on the high side of the coarse cell
             # coarse cells whose neighbour across the face along `axis` is covered
             nb = np.roll(covered, -1 if side == 0 else 1, axis=axis)
+            if not hierarchy.periodic[axis]:  # no neighbour across the domain face
+                edge = [slice(None), slice(None)]
+                edge[axis] = -1 if side == 0 else 0
+                nb[tuple(edge)] = False
             cells = np.argwhere(~covered & nb)
             for ci, cj in cells:
                 key = (ci, cj)
@@ -397,15 +454,16 @@ def _restrict_table(layouts, owners, L, hierarchy):
         ci = np.arange(i0, i1 + 1)
         cj = np.arange(j0, j1 + 1)
         CI, CJ = np.meshgrid(ci, cj, indexing="ij")
-        FI = (2 * CI) % shape_f[0]
-        FJ = (2 * CJ) % shape_f[1]
+        per = hierarchy.periodic
+        FI = _wrap(2 * CI, shape_f[0], per[0])
+        FJ = _wrap(2 * CJ, shape_f[1], per[1])
         own = fine_nodes[FI, FJ]
         sel = own >= 0
         for p in np.unique(own[sel]):
             s = sel & (own == p)
             pl = layouts[p]
-            fa = (FI[s] - pl.box[0][0]) % shape_f[0] + HALO
-            fb = (FJ[s] - pl.box[1][0]) % shape_f[1] + HALO
+            fa = _wrap(FI[s] - pl.box[0][0], shape_f[0], per[0]) + HALO
+            fb = _wrap(FJ[s] - pl.box[1][0], shape_f[1], per[1]) + HALO
             ca = CI[s] - i0 + HALO
             cb = CJ[s] - j0 + HALO
             rows.append(np.stack([ql.node(ca, cb), pl.node(fa, fb)], axis=1))

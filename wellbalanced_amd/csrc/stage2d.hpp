@@ -48,7 +48,7 @@ struct Stage2Args {
   int nx, ny, X, Y;
   double dx, dy, b0x, b0y, gamma;
   int limiter, heun, nxs;
-  int walls;  // bit 0: x walls, bit 1: y walls
+  int walls;  // faces that are walls: 1 x-low, 2 x-high, 4 y-low, 8 y-high
   Diffusion diff;  // used by the NI kernels only
 };
 
@@ -71,7 +71,7 @@ __device__ __forceinline__ void stage2d_body(const Stage2Args& a, int tile, int 
   const double* az = a.in + 6 * PL;
   const double dt = *a.dt;
   const double dtdx = dt / a.dx, dtdy = dt / a.dy;
-  const bool xwall = a.walls & 1, ywall = a.walls & 2;
+  const bool xlo = a.walls & 1, xhi = a.walls & 2, ylo = a.walls & 4, yhi = a.walls & 8;
 
   P q[7][4];
   P fprev[6], fcur[6];
@@ -181,7 +181,7 @@ __device__ __forceinline__ void stage2d_body(const Stage2Args& a, int tile, int 
           fl[6] = P(b6);
         }
         // walls: nothing but the pressure crosses (mhd.py:873-886), after the CT
-        const bool wall = xwall && (f == 2 || f == a.nx + 2);
+        const bool wall = (xlo && f == 2) || (xhi && f == a.nx + 2);
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
           fprev[r] = fcur[r];
@@ -240,7 +240,9 @@ __device__ __forceinline__ void stage2d_body(const Stage2Args& a, int tile, int 
         if constexpr (NI) {
           if (a.diff.eta != 0.0) e = e + a.diff.eta * sh.jz[f & 1][tid];
         }
-        if ((xwall && (f == 2 || f == a.nx + 2)) || (ywall && (gy == 2 || gy == a.ny + 2))) e = 0.0;
+        if ((xlo && f == 2) || (xhi && f == a.nx + 2) || (ylo && gy == 2) ||
+            (yhi && gy == a.ny + 2))
+          e = 0.0;
         const size_t o = 6 * PL + (size_t)f * YN + gy;
         double v = a.in[o] - dt * e;
         if (a.heun) v = (v + a.out[o]) * 0.5;
@@ -258,7 +260,7 @@ __device__ __forceinline__ void stage2d_body(const Stage2Args& a, int tile, int 
           }
         }
         if (f >= ix0 && f < ix0 + nxv) {
-          const bool wlo = ywall && gy == 2, whi = ywall && gy + 1 == a.ny + 2;
+          const bool wlo = ylo && gy == 2, whi = yhi && gy + 1 == a.ny + 2;
 #pragma unroll
           for (int r = 0; r < 6; ++r) {
             const int row = YR[r];
@@ -308,6 +310,7 @@ k_stage2d_batched(Stage2Args base, const PatchDesc* __restrict__ patches,
     a.Y = pd.ny + 4;
     a.dx = pd.dx;
     a.dy = pd.dy;
+    a.walls = pd.walls;
     stage2d_body<P, TY, LIM, NI>(a, w.tile, w.seg, sh);
     __syncthreads();
   }

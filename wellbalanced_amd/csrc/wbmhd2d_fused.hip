@@ -28,7 +28,8 @@ extern "C" __attribute__((visibility("hidden"))) int wb_fill_stage_args2d(
   a.limiter = limiter;
   a.heun = heun;
   a.nxs = nxs;
-  a.walls = walls;
+  // wall_mask (bit 0: x walls, bit 1: y walls) as the faces of the one patch
+  a.walls = ((walls & 1) ? 3 : 0) | ((walls & 2) ? 12 : 0);
   a.diff = Diffusion{eta, kpar, kperp};
   return 0;
 }

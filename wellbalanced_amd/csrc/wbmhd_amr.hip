@@ -241,9 +241,88 @@ __global__ void __launch_bounds__(256) k_coarsen(double* __restrict__ buf,
   if (i < nr) buf[rrows[2 * (size_t)i]] = buf[rrows[2 * (size_t)i + 1]];
 }
 
+// The outflow and wall ghosts of the patches on a non-periodic domain face
+// (mhd.py:298-353,748-777), one workgroup per listed patch: rows {patch, faces,
+// walls} with faces 1 x-low, 2 x-high, 4 y-low, 8 y-high and walls bit 0 (x walls),
+// bit 1 (y walls).  Axis 0 first over the full haloed extent in y, then axis 1 over
+// the full extent in x (so corner ghosts take the second rule on the first's
+// values), each thread one line of a field: a cell field's ghosts copy the first
+// interior cell (outflow) or mirror the interior with the sign -1 on the normal
+// momentum (wall); A_z's ghost nodes extrapolate the edge node linearly (outflow) or
+// are odd about the edge node (wall).
+__global__ void k_domain_faces2d(double* __restrict__ buf, const PatchDesc* __restrict__ patches,
+                                 const int* __restrict__ rows) {
+  const int* r = rows + 3 * blockIdx.x;
+  const PatchDesc pd = patches[r[0]];
+  const int faces = r[1], walls = r[2];
+  const int X = pd.nx + 4, Y = pd.ny + 4;
+  const size_t PL = (size_t)X * Y;
+  double* base = buf + pd.off;
+  constexpr int h = HALO;
+  for (int axis = 0; axis < 2; ++axis) {
+    const bool lower = faces & (axis == 0 ? 1 : 4), upper = faces & (axis == 0 ? 2 : 8);
+    if (lower || upper) {
+      const bool wall = walls & (1 << axis);
+      const int lc = axis == 0 ? X : Y, oc = axis == 0 ? Y : X;  // cell lengths
+      const int nlines = 6 * oc + (oc + 1);                        // 6 fields, then A_z
+      for (int t = threadIdx.x; t < nlines; t += blockDim.x) {
+        const bool node = t >= 6 * oc;
+        const int f = node ? 6 : t / oc, o = node ? t - 6 * oc : t % oc;
+        const int len = node ? lc + 1 : lc;
+        const size_t stride_other = axis == 0 ? 1 : (size_t)(node ? Y + 1 : Y);
+        const size_t sa = axis == 0 ? (size_t)(node ? Y + 1 : Y) : 1;
+        double* line = base + (size_t)f * PL + (size_t)o * stride_other;
+        auto at = [&](int i) -> double& { return line[(size_t)i * sa]; };
+        if (!node) {
+          const double sign = (wall && f == 1 + axis) ? -1.0 : 1.0;
+          if (wall) {
+            if (lower)
+              for (int k = 0; k < h; ++k) at(h - 1 - k) = sign * at(h + k);
+            if (upper)
+              for (int k = 0; k < h; ++k) at(len - h + k) = sign * at(len - h - 1 - k);
+          } else {
+            if (lower)
+              for (int k = 0; k < h; ++k) at(k) = at(h);
+            if (upper)
+              for (int k = 0; k < h; ++k) at(len - h + k) = at(len - h - 1);
+          }
+        } else if (wall) {
+          if (lower) {
+            const double w = at(h);
+            for (int k = 1; k <= h; ++k) at(h - k) = 2.0 * w - at(h + k);
+          }
+          if (upper) {
+            const double w = at(len - h - 1);
+            for (int k = 1; k <= h; ++k) at(len - h - 1 + k) = 2.0 * w - at(len - h - 1 - k);
+          }
+        } else {
+          if (lower) {
+            const double edge = at(h), inner = at(h + 1);
+            for (int k = 1; k <= h; ++k) at(h - k) = edge + (double)k * (edge - inner);
+          }
+          if (upper) {
+            const double edge = at(len - h - 1), inner = at(len - h - 2);
+            for (int k = 1; k <= h; ++k) at(len - h - 1 + k) = edge + (double)k * (edge - inner);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
 extern "C" {
+
+int wb_amr2d_domain_faces(double* buf, const void* patches, const int32_t* rows, int nrows,
+                          void* stream) {
+  if (nrows < 0 || (nrows > 0 && (!buf || !patches || !rows))) return WB_EINVAL;
+  if (nrows == 0) return 0;
+  hipLaunchKernelGGL(k_domain_faces2d, dim3((unsigned)nrows), dim3(256), 0, (hipStream_t)stream,
+                     buf, (const PatchDesc*)patches, (const int*)rows);
+  return wb_last_error();
+}
 
 int wb_amr2d_regrid_copy(const double* old, double* buf, const long long* cdst,
                          const long long* csrc, const long long* cpd, const long long* cps,

@@ -19,6 +19,8 @@ struct PatchDesc {
   long long off;  // first double of the patch's state
   int nx, ny;
   double dx, dy;
+  int walls;  // the patch's faces on domain walls: 1 x-low, 2 x-high, 4 y-low, 8 y-high
+  int pad;
 };
 struct WorkItem {
   int patch, tile, seg;

@@ -48,16 +48,16 @@ def _device(device) -> torch.device:
 class MHD2D:
     """2.5D MHD on one uniform patch of an `AMRHierarchy` (see the module).  On a
     hierarchy with refined patches the constructor returns a `wellbalanced_amd.amr.
-    AMRMHD2D` with the same API (periodic domains, no gravity)."""
+    AMRMHD2D` with the same API (no gravity)."""
 
     def __new__(cls, hierarchy: AMRHierarchy, **options):
         if cls is MHD2D and len(getattr(hierarchy, "patches", ())) > 1:
             from .amr import AMRMHD2D
 
-            if options.get("boundaries") or any(options.get("gravity", (0.0, 0.0))):
-                raise NotImplementedError("refined hierarchies run periodic and without gravity")
+            if any(options.get("gravity", (0.0, 0.0))):
+                raise NotImplementedError("the gravity source is outside this MHD path")
             keep = ("gamma", "mean_field", "limiter", "cfl", "resistivity", "conduction",
-                    "prefix", "precision", "device")
+                    "boundaries", "prefix", "precision", "device")
             return AMRMHD2D(hierarchy, **{"limiter": "vanleer",
                                           **{k: v for k, v in options.items() if k in keep}})
         return super().__new__(cls)
@@ -437,7 +437,7 @@ class MHD2D:
             from .amr import _PATCH
 
             patch = np.zeros(1, _PATCH)
-            patch[0] = (0, self.nx, self.ny, self.dx, self.dy)
+            patch[0] = (0, self.nx, self.ny, self.dx, self.dy, 0, 0)
             a, b = np.meshgrid(np.arange(self.nx) + HALO, np.arange(self.ny) + HALO, indexing="ij")
             leaf = np.stack([np.zeros(a.size, np.int64), (a * self.Y + b).ravel()], axis=1)
             chunks = [(0, c) for c in range(0, self.nx * self.ny, 256)]
