@@ -257,14 +257,16 @@ int wb_cnn_input(const double* state, int64_t nx, int64_t ny, int64_t nz, double
                  double dz, double b0x, double b0y, double b0z, double gamma, const float* norm,
                  uint16_t* out, void* stream);
 
-/* Forward: a1, a2 (n x 16 bf16) and the logits (n floats). */
+/* Forward: a1, a2 (n x 16 bf16) and the logits (n floats).  v_mfma_f32_16x16x32_bf16 on
+ * LDS-staged input tiles; the 1-channel logit layer puts 16 voxels along y in N with
+ * banded weights instead of padding its output channel. */
 int wb_cnn_forward(const uint16_t* x, const float* params, const uint16_t* kw, int64_t nx,
                    int64_t ny, int64_t nz, uint16_t* a1, uint16_t* a2, float* logit,
                    int workgroups, void* stream);
 
 /* Backward of the mean BCE-with-logits against `label` (n floats, 0/1): the
  * gradient of every parameter into grad (params' layout) and the summed loss into
- * *loss; g3, g2, g1: n x 16 bf16 scratch; partial: workgroups x 6928 floats. */
+ * *loss; g3: n bf16, g2, g1: n x 16 bf16 scratch; partial: workgroups x 6928 floats. */
 int wb_cnn_backward(const uint16_t* x, const uint16_t* a1, const uint16_t* a2, const float* logit,
                     const float* label, const uint16_t* kw, int64_t nx, int64_t ny, int64_t nz,
                     uint16_t* g3, uint16_t* g2, uint16_t* g1, float* partial, int workgroups,
@@ -275,9 +277,10 @@ int wb_cnn_adam(float* p, const float* g, float* m, float* v, int n, float lr, f
                 float eps, int step, void* stream);
 
 /* Single layers, for tests: conv (mode 0: bf16 relu(acc + bias) into out16; 1:
- * acc + bias of channel 0 into logit; 2: bf16 acc * (mask > 0), the backward-data
- * pass with kernel weights K'[t][ci][co] = W[26-t][ci][co]) and the weight
- * gradient of one layer (27 x 16 x 16 + 16 floats into out). */
+ * acc + bias of output channel 0 into logit, cin 16; 2: bf16 acc * (mask > 0), the
+ * backward-data pass with kernel weights K'[t][ci][co] = W[26-t][ci][co]; `workgroups`
+ * is not used by conv) and the weight gradient of one 16-output layer (27 x 16 x 16 +
+ * 16 floats into out, `workgroups` partials). */
 int wb_cnn_conv(const uint16_t* in, int cin, const uint16_t* wk, const float* bias, int64_t nx,
                 int64_t ny, int64_t nz, int mode, const uint16_t* mask, uint16_t* out16,
                 float* logit, int workgroups, void* stream);

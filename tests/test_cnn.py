@@ -141,6 +141,59 @@ def test_gpu_layer_matches_oracle(gpu, cin):
 
 
 @pytest.mark.gpu
+def test_gpu_logit_layer_matches_oracle(gpu):
+    """The 1-channel layer (mode 1: 16 voxels along y in N, banded weights) on a grid
+    smaller than a tile in x and ragged in y: FP32 sums of the same BF16 products."""
+    import torch
+    from wellbalanced_amd import _lib
+
+    shape = (3, 21, 32)
+    rng = np.random.default_rng(5)
+    x = C.bf16(rng.standard_normal(shape + (16,)).astype(np.float32))
+    w = C.bf16(0.2 * rng.standard_normal((27, 16, 1)).astype(np.float32))
+    b = np.array([0.3] + [0.0] * 15, np.float32)
+    ref = C.conv(x, w, b[:1])[..., 0]
+    kw = np.zeros((27, 16, 16), np.float32)
+    kw[:, 0, :] = w[:, :, 0]  # the forward layout K[t][co][ci], row co = 0
+    kw = torch.from_numpy(_bf16_bits(kw.ravel())).to(gpu)
+    xin = torch.from_numpy(_bf16_bits(x.ravel())).to(gpu)
+    logit = torch.empty(int(np.prod(shape)), dtype=torch.float32, device=gpu)
+    bias = torch.from_numpy(b).to(gpu)
+    _lib.call("wb_cnn_conv", _lib.ptr(xin), 16, _lib.ptr(kw), _lib.ptr(bias), *shape, 1, None,
+              None, _lib.ptr(logit), 1, _lib.stream_handle(gpu))
+    got = logit.cpu().numpy().reshape(shape)
+    np.testing.assert_allclose(got, ref, rtol=1e-5, atol=1e-5 * np.abs(ref).max())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cin", [8, 16])
+def test_gpu_weight_gradient_matches_oracle(gpu, cin):
+    """One layer's weight and bias gradients (every tap, a persistent grid of 5
+    workgroups over a grid ragged in x and y)."""
+    import torch
+    from wellbalanced_amd import _lib
+
+    shape = (6, 11, 32)
+    rng = np.random.default_rng(20 + cin)
+    x = C.bf16(rng.standard_normal(shape + (cin,)).astype(np.float32))
+    g = C.bf16(rng.standard_normal(shape + (16,)).astype(np.float32))
+    ref_w = C.weight_grad(x, g)
+    ref_b = g.reshape(-1, 16).sum(axis=0)
+    xin = torch.from_numpy(_bf16_bits(x.ravel())).to(gpu)
+    gin = torch.from_numpy(_bf16_bits(g.ravel())).to(gpu)
+    wg = 5
+    partial = torch.empty(wg * (27 * 256 + 16), dtype=torch.float32, device=gpu)
+    out = torch.empty(27 * 256 + 16, dtype=torch.float32, device=gpu)
+    _lib.call("wb_cnn_wgrad", _lib.ptr(xin), cin, _lib.ptr(gin), *shape, _lib.ptr(partial), wg,
+              _lib.ptr(out), _lib.stream_handle(gpu))
+    o = out.cpu().numpy()
+    got_w = o[: 27 * 256].reshape(27, 16, 16)[:, :cin, :]
+    np.testing.assert_allclose(got_w, ref_w, rtol=1e-4, atol=1e-4 * np.abs(ref_w).max())
+    np.testing.assert_allclose(o[27 * 256 :], ref_b, rtol=1e-4, atol=1e-4 * np.abs(ref_b).max())
+    assert not o[: 27 * 256].reshape(27, 16, 16)[:, cin:, :].any()
+
+
+@pytest.mark.gpu
 def test_gpu_network_matches_oracle(gpu):
     """Forward, loss and every gradient of the whole network against the oracle.
     Tolerances: logits 2e-3 absolute + 1e-2 relative, loss 1e-4 relative, each

@@ -63,7 +63,7 @@ def unpack(flat: np.ndarray) -> list[np.ndarray]:
 class Indicator:
     """The network and its training state on one device, for grids of ``shape``.
 
-    ``shape[2]`` must be a multiple of 16 (a wavefront takes 16 cells along z).
+    ``shape[2]`` must be a multiple of 16 (an MFMA tile takes 16 cells along z).
     """
 
     def __init__(self, shape, *, device="cuda", seed: int = 0, lr: float = 2e-3,
@@ -87,7 +87,7 @@ class Indicator:
         self.a1 = torch.empty(self.n * 16, **i16)
         self.a2 = torch.empty(self.n * 16, **i16)
         self.logit = torch.empty(self.n, **f32)
-        self.g3 = torch.empty(self.n * 16, **i16)
+        self.g3 = torch.empty(self.n, **i16)  # the logits' gradient: one channel
         self.g2 = torch.empty(self.n * 16, **i16)
         self.g1 = torch.empty(self.n * 16, **i16)
         self.partial = torch.empty(workgroups * LAYER, **f32)
