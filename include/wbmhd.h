@@ -490,38 +490,37 @@ int wb_amr3d_rate_mixed(const double* buf, const void* blocks, int nblocks, doub
  * their (filled) ghosts.  The results are the same.
  * waves, the mixed kernel's launch shape: 2, one 16x16 tile per block at 2 waves per
  * SIMD; 3, half a block (8x16 tiles, two workgroups per block) at 3 waves per SIMD.
- * The results do not depend on it. */
+ * The results do not depend on it.
+ * rec_slots, rec (both or neither): per block 6 int32 slots (face 2 axis + side, side
+ * 0 the low face; -1 none) and the record they index: the stage writes the fluxes it
+ * solves on those faces there, 256 face cells (u, v in the two other axes, u slower) x
+ * the 5 conserved fields (double) per slot, for wb_amr3d_reflux. */
 int wb_amr3d_stage(const double* in, double* out, const void* blocks, const int32_t* ids,
                    int nids, int direct, const int32_t* lookup, const void* levels, double b0x,
                    double b0y, double b0z, double gamma, int limiter, const double* dt, int heun,
-                   int32_t* bad, void* stream);
+                   int32_t* bad, const int32_t* rec_slots, double* rec, void* stream);
 int wb_amr3d_stage_mixed(const double* in, double* out, const void* blocks, const int32_t* ids,
                          int nids, int direct, const int32_t* lookup, const void* levels,
                          double b0x, double b0y, double b0z, double gamma, int limiter,
-                         const double* dt, int heun, int32_t* bad, int waves, void* stream);
+                         const double* dt, int heun, int32_t* bad, int waves,
+                         const int32_t* rec_slots, double* rec, void* stream);
 
 /* The flux register and refluxing, every coarse-fine block face of every level in one
  * launch (FluxRegister.update with scale 0.5 dt and FluxRegister.reflux,
  * mhd3d.py:650-655,745-750): per listed coarse cell t, d = the sum over its faces of
- * dt/h (coarse flux - mean of the four fine fluxes), signed by the side, recomputed
- * from the stage input `src`; record != 0 (after stage 1): reg[5 t + f] = 0.5 d;
- * record == 0 (after stage 2 and its Heun average): the cell of `dst` gains
- * reg + 0.5 d.  reg: 5 doubles per listed cell, caller-owned.  items: {int32 fine, coarse, ci, cj, ck, dir}
- * per coarse-fine block face (dir = 2 axis + side; side 0: the coarse cells at the fine
- * block's low face, 1: at its high face); cells: {int32 n, item[3], lane[3], pad} per
- * coarse cell with such faces, its n faces as (item, lane = u * 8 + v inside the item's
- * 8 x 8 face) in increasing dir.  hoods: per block the 27 offsets (int64 doubles, -1:
- * none) of the blocks of its level around it, index (x + 1) * 9 + (y + 1) * 3 + z + 1;
- * the recomputed fluxes read the cells around a block in place as the stage does (null:
- * from the blocks' filled ghosts). */
-int wb_amr3d_reflux(const double* src, double* dst, const void* blocks, const void* items,
-                    const void* cells, int ncells, const long long* hoods, double b0x,
-                    double b0y, double b0z, double gamma, int limiter, const double* dt,
+ * dt/h (coarse flux - mean of the four fine fluxes), signed by the side, from the
+ * fluxes the last stage recorded in `rec` (slots as for wb_amr3d_stage: the coarse
+ * block's face toward the fine block and the fine block's face; both precisions);
+ * record != 0 (after stage 1): reg[5 t + f] = 0.5 d; record == 0 (after stage 2 and its
+ * Heun average): the cell of `dst` gains reg + 0.5 d.  reg: 5 doubles per listed cell,
+ * caller-owned.  items: {int32 fine, coarse, ci, cj, ck, dir} per coarse-fine block
+ * face (dir = 2 axis + side; side 0: the coarse cells at the fine block's low face, 1:
+ * at its high face); cells: {int32 n, item[3], lane[3], pad} per coarse cell with such
+ * faces, its n faces as (item, lane = u * 8 + v inside the item's 8 x 8 face) in
+ * increasing dir. */
+int wb_amr3d_reflux(double* dst, const void* blocks, const void* items, const void* cells,
+                    int ncells, const int32_t* rec_slots, const double* rec, const double* dt,
                     int record, double* reg, void* stream);
-int wb_amr3d_reflux_mixed(const double* src, double* dst, const void* blocks, const void* items,
-                          const void* cells, int ncells, const long long* hoods, double b0x,
-                          double b0y, double b0z, double gamma, int limiter, const double* dt,
-                          int record, double* reg, void* stream);
 
 /* ---- 3D hierarchies of patches of any size (MHD3D on patchwork patches) -------- */
 /* Each patch holds a 3D patch state as above, patch after patch in one buffer; the

@@ -113,15 +113,15 @@ def test_new_entry_points_reject_bad_arguments_before_any_launch():
     assert lib.wb_amr2d_coarsen(None, None, 0, None, 0, None) == 0
     assert lib.wb_amr3d_fill_regions(None, None, None, None, -1, None) == E
     assert lib.wb_amr3d_fill_regions(None, None, None, None, 0, None) == 0
-    for name in ("wb_amr3d_reflux", "wb_amr3d_reflux_mixed"):
-        reflux = getattr(lib, name)
-        assert reflux(None, None, None, None, None, -1, None, 0.0, 0.0, 0.0, 5 / 3, 0, None, 0,
-                      None, None) == E
-        assert reflux(None, None, None, None, None, 0, None, 0.0, 0.0, 0.0, 5 / 3, 0, None, 0,
-                      None, None) == 0
-        # a register is needed when there is a cell to reflux
-        assert reflux(None, None, None, None, None, 1, None, 0.0, 0.0, 0.0, 5 / 3, 0, None, 1,
-                      None, None) == E
+    reflux = lib.wb_amr3d_reflux
+    assert reflux(None, None, None, None, -1, None, None, None, 0, None, None) == E
+    assert reflux(None, None, None, None, 0, None, None, None, 0, None, None) == 0
+    # a register and the face record are needed when there is a cell to reflux
+    assert reflux(None, None, None, None, 1, None, None, None, 1, None, None) == E
+    # the stage takes the face record's slots and the record together or neither
+    rec = ctypes.c_double(0.0)
+    assert lib.wb_amr3d_stage(None, None, None, None, 1, 0, None, None, 0.0, 0.0, 0.0, 5 / 3,
+                              0, None, 0, None, None, ctypes.byref(rec), None) == E
     for name in ("wb_amr3d_rate", "wb_amr3d_rate_mixed"):
         assert getattr(lib, name)(None, None, 0, 0.0, 0.0, 0.0, 5 / 3, None, None) == E
 

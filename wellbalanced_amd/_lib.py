@@ -68,10 +68,9 @@ SIGNATURES: dict[str, tuple[type, list]] = {
     "wb_amr3d_prolong": (I, [P, P, P, I, P, P, P]),
     "wb_amr3d_rate": (I, [P, P, I, D, D, D, D, P, P]),
     "wb_amr3d_rate_mixed": (I, [P, P, I, D, D, D, D, P, P]),
-    "wb_amr3d_stage": (I, [P, P, P, P, I, I, P, P, D, D, D, D, I, P, I, P, P]),
-    "wb_amr3d_stage_mixed": (I, [P, P, P, P, I, I, P, P, D, D, D, D, I, P, I, P, I, P]),
-    "wb_amr3d_reflux": (I, [P, P, P, P, P, I, P, D, D, D, D, I, P, I, P, P]),
-    "wb_amr3d_reflux_mixed": (I, [P, P, P, P, P, I, P, D, D, D, D, I, P, I, P, P]),
+    "wb_amr3d_stage": (I, [P, P, P, P, I, I, P, P, D, D, D, D, I, P, I, P, P, P, P]),
+    "wb_amr3d_stage_mixed": (I, [P, P, P, P, I, I, P, P, D, D, D, D, I, P, I, P, I, P, P, P]),
+    "wb_amr3d_reflux": (I, [P, P, P, P, I, P, P, P, I, P, P]),
     "wb_cnn_prep": (I, [P, P, P]),
     "wb_cnn_moments": (I, [P, I64, I64, I64] + [D] * 7 + [P, I, P, P]),
     "wb_cnn_input": (I, [P, I64, I64, I64] + [D] * 7 + [P, P, P]),

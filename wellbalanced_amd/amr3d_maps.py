@@ -305,6 +305,22 @@ def reflux_cells(reflux: list, b: int = B) -> tuple[np.ndarray, np.ndarray]:
     return items, cells[np.argsort(-count, kind="stable")]
 
 
+def face_slots(nblocks: int, items: np.ndarray):
+    """The block faces whose fluxes the stage records for the refluxing: per block 6
+    slots (face 2 axis + side, side 0 the low face; -1: not recorded) and their count.
+    An item (a fine block's face over coarse cells, pad = 2 axis + side) needs the fine
+    block's face 2 axis + side and the coarse block's face 2 axis + (1 - side)."""
+    slots = np.full((nblocks, 6), -1, np.int32)
+    n = 0
+    for it in items:
+        axis, side = int(it["pad"]) >> 1, int(it["pad"]) & 1
+        for blk, face in ((int(it["fine"]), 2 * axis + side), (int(it["coarse"]), 2 * axis + 1 - side)):
+            if slots[blk, face] < 0:
+                slots[blk, face] = n
+                n += 1
+    return slots, n
+
+
 # -- host views of a block ----------------------------------------------------------------
 def block_views(buf: np.ndarray, off: int, b: int = B):
     """(cells (5, X, X, X), [A_x, A_y, A_z]) views of the block at ``off``."""

@@ -72,6 +72,11 @@ struct StageArgs {
   int limiter, heun, nxs;
   int wrap;  // bit a: axis a is periodic and the stage wraps its reads (its halo is not read)
   size_t off_ax, off_ay, off_az;
+  // the face-flux record of a refined hierarchy's blocks (null: none): per block 6 slot
+  // indices (face 2 axis + side, -1: not recorded), and per slot the 5 conserved fluxes
+  // of each of the face's cells, face cell (u, v) in the two other axes in order
+  const int* rec_slots = nullptr;
+  double* rec = nullptr;
 };
 
 // Face-normal fields from the edge potential (mhd3d.py:361-378), the same
@@ -239,11 +244,15 @@ struct CellPrim {
 // holding it, `hood` giving the offsets (doubles from `root`) of the 27 blocks around
 // this one (index (x + 1) * 9 + (y + 1) * 3 + (z + 1); -1: no such block, and the
 // cell is read from this block's own ghosts, which must then be filled).
+// `rslots` (a block of a refined hierarchy, or null): the block's 6 face slots in
+// a.rec; the fluxes the stage solves on those faces of the block are written there as
+// double, in the order of the cell fields (the refluxing reads them).
 template <typename P, int TY, int TZ, int LIM, bool DIRECT = false>
 __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz_, int seg_,
                                              StageShared<P, TY, TZ>& sh,
                                              const long long* hood = nullptr,
-                                             const double* root = nullptr) {
+                                             const double* root = nullptr,
+                                             const int* rslots = nullptr) {
   using T = Tile<TY, TZ>;
   constexpr bool FAST = sizeof(P) == 4;
   const Grid3 g = a.g;
@@ -316,6 +325,19 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
   const double dtdx = dt / a.dx, dtdy = dt / a.dy, dtdz = dt / a.dz;
   const double rdx = 1.0 / a.dx, rdy = 1.0 / a.dy, rdz = 1.0 / a.dz;
   const P gam = P(a.gamma), inv_gm1 = P(1.0 / (a.gamma - 1.0));
+  // the block's 6 record slots (-1: none; uniform over the workgroup)
+  int slot[6] = {-1, -1, -1, -1, -1, -1};
+  if (rslots) {
+#pragma unroll
+    for (int s6 = 0; s6 < 6; ++s6) slot[s6] = rslots[s6];
+  }
+  // slot `sl`, face cell `cell` (square blocks: g.ny * g.nz cells per face): the 5 fluxes
+  // of the cell fields, which sit in rows row[0..4] of the face's frame
+  auto record = [&](int sl, int cell, const P* fl, const int (&row)[5]) {
+    double* o = a.rec + ((size_t)sl * (g.ny * g.nz) + cell) * 5;
+#pragma unroll
+    for (int r = 0; r < 5; ++r) o[r] = double(fl[row[r]]);
+  };
 
   // The mixed path's x-sweep computes each cell's slope once: the right state of face f
   // and the left state of face f+1 (carried in lcar) come from the slope of cell f, so
@@ -433,6 +455,11 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
         for (int r = 1; r < 6; ++r) sh.fy[r - 1][tid] = fl[r];
         sh.fyc[cur][0][tid] = fl[0];
         sh.fyc[cur][1][tid] = fl[6];
+        if (rslots && f >= ix0 && f < ix0 + nxv && lz >= 0 && lz < tzv) {
+          const int sl = gy == 2 ? slot[2] : (gy == g.ny + 2 ? slot[3] : -1);
+          constexpr int yrow[5] = {0, 3, 1, 2, 4};  // y-frame rows of rho, m_x, m_y, m_z, E
+          if (sl >= 0) record(sl, (f - 2) * g.nz + (gz - 2), fl, yrow);
+        }
       }
       if (fb >= 1) {
         constexpr int comp[7] = {0, 3, 1, 2, 4, 5, 6};
@@ -448,6 +475,11 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
         sh.fz[4][tid] = fl[6];
         sh.fzc[cur][0][tid] = fl[0];
         sh.fzc[cur][1][tid] = fl[5];
+        if (rslots && f >= ix0 && f < ix0 + nxv && ly >= 0 && ly < tyv) {
+          const int sl = gz == 2 ? slot[4] : (gz == g.nz + 2 ? slot[5] : -1);
+          constexpr int zrow[5] = {0, 2, 3, 1, 4};  // z-frame rows of rho, m_x, m_y, m_z, E
+          if (sl >= 0) record(sl, (f - 2) * g.ny + (gy - 2), fl, zrow);
+        }
       }
     }
     // b. the own column of plane f+1 into the ring; c. x-face f from the ring
@@ -461,6 +493,11 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
       sh.fx[0][tid] = fl[0];
       sh.fx[1][tid] = fl[5];
       sh.fx[2][tid] = fl[6];
+      const int sxf = f == 2 ? slot[0] : (f == g.nx + 2 ? slot[1] : -1);
+      if (sxf >= 0 && ly >= 0 && ly < tyv && lz >= 0 && lz < tzv) {
+        constexpr int xrow[5] = {0, 1, 2, 3, 4};
+        record(sxf, (gy - 2) * g.nz + (gz - 2), fl, xrow);
+      }
     };
     if constexpr (CARRY) {
       if (fthread) {

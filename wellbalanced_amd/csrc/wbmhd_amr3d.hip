@@ -493,6 +493,79 @@ __global__ void __launch_bounds__(256) k_coarsen3d(double* __restrict__ buf,
 
 }  // namespace
 
+namespace {
+
+// The flux register and refluxing (FluxRegister.update with scale 0.5 dt and
+// FluxRegister.reflux as MHD3D.step calls them, mhd3d.py:650-655,745-750): per
+// coarse-fine block face, the 8 x 8 coarse cells across it trade the flux of their face
+// on the fine side for the mean of the four fine fluxes.  Both are the fluxes the stage
+// solved on those block faces (recorded by the stage, `rec` with the blocks' 6 face
+// slots `slots`): d = sum over the cell's faces of dt/h (F_c - mean) or dt/h (mean - F_c).
+// record (after stage 1): reg[5 t + f] = 0.5 d; otherwise (after stage 2 and its Heun
+// average) the cell gains reg + 0.5 d.  An item is one such face (pad = 2 axis + side;
+// side 0: the coarse cells at the fine block's low face, their +axis face, the coarse
+// block's high face; side 1: at the fine block's high face, the coarse block's low face).
+// `cells` lists, per coarse cell, its n (1 to 3) faces as (item, lane) pairs in
+// increasing direction order, the lane u * 8 + v naming the cell inside the item's 8 x 8
+// face; a thread takes one cell and sums its faces' corrections in that order, with no
+// atomics on the state (deterministic).
+__global__ void __launch_bounds__(256) k_reflux3d(const BlockDesc* __restrict__ blocks,
+                                                  const RefluxItem* __restrict__ items,
+                                                  const RefluxCell* __restrict__ cells, int ncells,
+                                                  const int* __restrict__ slots,
+                                                  const double* __restrict__ rec,
+                                                  const double* __restrict__ dtp,
+                                                  double* __restrict__ dst, int record,
+                                                  double* __restrict__ reg) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncells) return;
+  const RefluxCell cl = cells[t];
+  const double dt = *dtp;
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  int c[3] = {0, 0, 0};
+  BlockDesc cd{};
+  for (int f = 0; f < cl.n; ++f) {
+    const RefluxItem item = items[cl.item[f]];
+    const int lc = cl.lane[f];
+    const int axis = item.pad >> 1, side = item.pad & 1;
+    const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+    const int u = lc >> 3, v = lc & 7;
+    // the coarse cell (haloed local) and its face on the fine side
+    c[0] = item.ci + 2;
+    c[1] = item.cj + 2;
+    c[2] = item.ck + 2;
+    c[a1] += u;
+    c[a2] += v;
+    cd = blocks[item.coarse];
+    const int cs = slots[6 * item.coarse + 2 * axis + (1 - side)];
+    const int fs = slots[6 * item.fine + 2 * axis + side];
+    const double* fc = rec + ((size_t)cs * (B * B) + (c[a1] - 2) * B + (c[a2] - 2)) * 5;
+    const double* ff[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      ff[k] = rec + ((size_t)fs * (B * B) + (2 * u + (k >> 1)) * B + 2 * v + (k & 1)) * 5;
+    const double dtd = axis == 0 ? dt / cd.dx : (axis == 1 ? dt / cd.dy : dt / cd.dz);
+#pragma unroll
+    for (int fld = 0; fld < 5; ++fld) {
+      const double fm = ((ff[0][fld] + ff[1][fld]) + (ff[2][fld] + ff[3][fld])) * 0.25;
+      const double d = side == 0 ? dtd * (fc[fld] - fm) : dtd * (fm - fc[fld]);
+      acc[fld] = acc[fld] + d;
+    }
+  }
+  double* kept = reg + 5 * (size_t)t;
+  if (record) {
+    for (int fld = 0; fld < 5; ++fld) kept[fld] = 0.5 * acc[fld];
+    return;
+  }
+  double* out = dst + cd.off;
+  for (int fld = 0; fld < 5; ++fld) {
+    const int o = cell_at(fld, c[0], c[1], c[2]);
+    out[o] = out[o] + (kept[fld] + 0.5 * acc[fld]);
+  }
+}
+
+}  // namespace
+
 extern "C" {
 
 // Hidden: the stage arguments of a block (shared with the mixed unit).
@@ -582,7 +655,7 @@ int wb_amr3d_rate(const double* buf, const void* blocks, int nblocks, double b0x
                      nullptr);
   unsigned long long* keys = (unsigned long long*)out;
   hipLaunchKernelGGL(k_rate_init, dim3(1), dim3(64), 0, s, keys);
-  hipLaunchKernelGGL(k_rate3d_blocks<false>, dim3((unsigned)nblocks), dim3(256), 0, s, a,
+  hipLaunchKernelGGL(k_rate3d_blocks<false>, dim3((unsigned)nblocks), dim3(256 * RATE_HALVES), 0, s, a,
                      (const BlockDesc*)blocks, keys);
   hipLaunchKernelGGL(k_rate_finish, dim3(1), dim3(64), 0, s, keys, out);
   return wb_last_error();
@@ -591,8 +664,9 @@ int wb_amr3d_rate(const double* buf, const void* blocks, int nblocks, double b0x
 int wb_amr3d_stage(const double* in, double* out, const void* blocks, const int32_t* ids,
                    int nids, int direct, const int32_t* lookup, const void* levels, double b0x,
                    double b0y, double b0z, double gamma, int limiter, const double* dt, int heun,
-                   int32_t* bad, void* stream) {
+                   int32_t* bad, const int32_t* rec_slots, double* rec, void* stream) {
   if (nids < 1 || limiter < 0 || limiter > 3 || direct < 0 || direct > 1) return WB_EINVAL;
+  if ((rec_slots == nullptr) != (rec == nullptr)) return WB_EINVAL;
   Levels lv{};
   if (direct) {
     if (!levels || !lookup) return WB_EINVAL;
@@ -601,6 +675,8 @@ int wb_amr3d_stage(const double* in, double* out, const void* blocks, const int3
   }
   StageArgs a;
   wb_amr3d_base_args(&a, in, out, b0x, b0y, b0z, gamma, limiter, dt, heun, bad);
+  a.rec_slots = (const int*)rec_slots;
+  a.rec = rec;
   const dim3 grid(((nids + 7) / 8) * 8), tile(Tile<B, B>::THREADS);
   hipStream_t s = (hipStream_t)stream;
   if (direct)
@@ -612,17 +688,16 @@ int wb_amr3d_stage(const double* in, double* out, const void* blocks, const int3
   return wb_last_error();
 }
 
-int wb_amr3d_reflux(const double* src, double* dst, const void* blocks, const void* items,
-                    const void* cells, int ncells, const long long* hoods, double b0x,
-                    double b0y, double b0z, double gamma, int limiter, const double* dt,
+int wb_amr3d_reflux(double* dst, const void* blocks, const void* items, const void* cells,
+                    int ncells, const int32_t* rec_slots, const double* rec, const double* dt,
                     int record, double* reg, void* stream) {
-  if (ncells < 0 || (ncells > 0 && reg == nullptr)) return WB_EINVAL;
+  if (ncells < 0 || (ncells > 0 && (reg == nullptr || rec_slots == nullptr || rec == nullptr)))
+    return WB_EINVAL;
   if (ncells == 0) return 0;
-  StageArgs a;
-  wb_amr3d_base_args(&a, src, dst, b0x, b0y, b0z, gamma, limiter, dt, 0, nullptr);
-  hipLaunchKernelGGL((k_reflux3d<double, -1>), dim3((unsigned)((ncells + REFLUX_CELLS - 1) / REFLUX_CELLS)), dim3(256), 0,
-                     (hipStream_t)stream, a, (const BlockDesc*)blocks, (const RefluxItem*)items,
-                     (const RefluxCell*)cells, ncells, hoods, dst, record, reg);
+  hipLaunchKernelGGL(k_reflux3d, dim3((unsigned)((ncells + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, (const BlockDesc*)blocks, (const RefluxItem*)items,
+                     (const RefluxCell*)cells, ncells, (const int*)rec_slots, rec, dt, dst,
+                     record, reg);
   return wb_last_error();
 }
 

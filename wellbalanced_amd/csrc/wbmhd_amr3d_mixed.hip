@@ -1,5 +1,6 @@
-// The refined 3D hierarchy's stage and refluxing with the FP32 reconstruction and
-// HLLD (the mixed path, built with -ffp-contract=on).
+// The refined 3D hierarchy's stage with the FP32 reconstruction and HLLD (the mixed
+// path, built with -ffp-contract=on); its refluxing reads the fluxes the stage recorded
+// (wb_amr3d_reflux, shared with the FP64 path).
 #include "amr3d.hpp"
 
 using namespace wb;
@@ -14,8 +15,10 @@ extern "C" {
 int wb_amr3d_stage_mixed(const double* in, double* out, const void* blocks, const int32_t* ids,
                          int nids, int direct, const int32_t* lookup, const void* levels,
                          double b0x, double b0y, double b0z, double gamma, int limiter,
-                         const double* dt, int heun, int32_t* bad, int waves, void* stream) {
+                         const double* dt, int heun, int32_t* bad, int waves,
+                         const int32_t* rec_slots, double* rec, void* stream) {
   if (nids < 1 || (waves != 2 && waves != 3) || direct < 0 || direct > 1) return WB_EINVAL;
+  if ((rec_slots == nullptr) != (rec == nullptr)) return WB_EINVAL;
   Levels lv{};
   if (direct) {
     if (!levels || !lookup) return WB_EINVAL;
@@ -24,6 +27,8 @@ int wb_amr3d_stage_mixed(const double* in, double* out, const void* blocks, cons
   }
   StageArgs a;
   wb_amr3d_base_args(&a, in, out, b0x, b0y, b0z, gamma, limiter, dt, heun, bad);
+  a.rec_slots = (const int*)rec_slots;
+  a.rec = rec;
   hipStream_t s = (hipStream_t)stream;
   const int* id = (const int*)ids;
   const int* lk = (const int*)lookup;
@@ -47,24 +52,6 @@ int wb_amr3d_stage_mixed(const double* in, double* out, const void* blocks, cons
   });
 }
 
-int wb_amr3d_reflux_mixed(const double* src, double* dst, const void* blocks, const void* items,
-                          const void* cells, int ncells, const long long* hoods, double b0x,
-                          double b0y, double b0z, double gamma, int limiter, const double* dt,
-                          int record, double* reg, void* stream) {
-  if (ncells < 0 || (ncells > 0 && reg == nullptr)) return WB_EINVAL;
-  if (ncells == 0) return 0;
-  StageArgs a;
-  wb_amr3d_base_args(&a, src, dst, b0x, b0y, b0z, gamma, limiter, dt, 0, nullptr);
-  hipStream_t s = (hipStream_t)stream;
-  return with_limiter(limiter, [&](auto lim) {
-    constexpr int L = decltype(lim)::value;
-    hipLaunchKernelGGL((k_reflux3d<float, L>), dim3((unsigned)((ncells + REFLUX_CELLS - 1) / REFLUX_CELLS)), dim3(256), 0, s,
-                       a, (const BlockDesc*)blocks, (const RefluxItem*)items,
-                       (const RefluxCell*)cells, ncells, hoods, dst, record, reg);
-    return wb_last_error();
-  });
-}
-
 // The mixed path's CFL rate over every block: the column march of the uniform grid's
 // mixed rate (one division per cell, reciprocals), in this FP32-contracted unit.
 int wb_amr3d_rate_mixed(const double* buf, const void* blocks, int nblocks, double b0x,
@@ -76,7 +63,7 @@ int wb_amr3d_rate_mixed(const double* buf, const void* blocks, int nblocks, doub
                      nullptr);
   unsigned long long* keys = (unsigned long long*)out;
   hipLaunchKernelGGL(k_rate_init, dim3(1), dim3(64), 0, s, keys);
-  hipLaunchKernelGGL(k_rate3d_blocks<true>, dim3((unsigned)nblocks), dim3(256), 0, s, a,
+  hipLaunchKernelGGL(k_rate3d_blocks<true>, dim3((unsigned)nblocks), dim3(256 * RATE_HALVES), 0, s, a,
                      (const BlockDesc*)blocks, keys);
   hipLaunchKernelGGL(k_rate_finish, dim3(1), dim3(64), 0, s, keys, out);
   return wb_last_error();
