@@ -134,7 +134,7 @@ k_stage3d_blocks(StageArgs base, const BlockDesc* __restrict__ blocks, const int
 // register allocation spills a few values) run four such workgroups per CU, where one
 // 384-thread workgroup per block fits only six waves at its natural 187 registers:
 // 4.0 face solves per cell instead of 3.66, and the stage 30 % faster.
-template <typename P, int WAVES, int LIM, bool DIRECT>
+template <typename P, int WAVES, int LIM, bool DIRECT, int EARLY = 0>
 __global__ void __launch_bounds__((Tile<B / 2, B>::THREADS), WAVES)
 k_stage3d_halves(StageArgs base, const BlockDesc* __restrict__ blocks, const int* __restrict__ ids,
                  int nblocks, const int* __restrict__ lookup, Levels lv) {
@@ -149,7 +149,7 @@ k_stage3d_halves(StageArgs base, const BlockDesc* __restrict__ blocks, const int
     load_hood(hood, blocks, lookup, lv, d);
     __syncthreads();
   }
-  stage3d_body<P, B / 2, B, LIM, DIRECT>(block_args(base, d), half, 0, 0, sh, hood, base.in,
+  stage3d_body<P, B / 2, B, LIM, DIRECT, EARLY>(block_args(base, d), half, 0, 0, sh, hood, base.in,
                                          base.rec_slots ? base.rec_slots + 6 * id : nullptr);
 }
 

@@ -53,8 +53,11 @@ struct Stage2Args {
 };
 
 // LIM: the compile-time slope limiter of the branch-free FP32 flux (mixed path),
-// or -1 for the parity path's runtime limiter.
-template <typename P, int TY, int LIM, bool NI = false>
+// or -1 for the parity path's runtime limiter.  EARLY: which stored values the CT and
+// the update start from are loaded before the row's second barrier rather than where
+// they are used (bit 0: those of `in`, bit 1: those of `out` for the Heun average), so
+// that their latency overlaps the barrier, at the price of the registers holding them.
+template <typename P, int TY, int LIM, bool NI = false, int EARLY = 3>
 __device__ __forceinline__ void stage2d_body(const Stage2Args& a, int tile, int seg,
                                              Stage2Shared<P, TY, NI>& sh) {
   using T = Row<TY>;
@@ -223,14 +226,37 @@ __device__ __forceinline__ void stage2d_body(const Stage2Args& a, int tile, int 
         for (int r = 0; r < 7; ++r) sh.fy[cur][r][tid] = fl[r];
       }
     }
+    // the stored values of A_z at node (f, gy) and of the cell (f - 1, gy) that the CT
+    // and the update start from, and their values in `out` for the Heun average
+    const bool in_y = ly >= 0 && (ly < tyv || (ly == tyv && last_y));
+    const bool cell_y = ly >= 0 && ly < tyv;
+    const bool node_x = f >= ix0 && (f < ix0 + nxv || (f == ix0 + nxv && last_x));
+    const bool do_node = fthread && node_x && in_y;
+    const bool do_cell = fthread && cell_y && f - 1 >= ix0 && f - 1 < ix0 + nxv;
+    const size_t onode = 6 * PL + (size_t)f * YN + gy;
+    const size_t ocell = (size_t)(f - 1) * Y + gy;
+    double vin[7], vout[7];  // A_z, then the 6 cell fields
+#pragma unroll
+    for (int r = 0; r < 7; ++r) vin[r] = vout[r] = 0.0;
+    if constexpr (EARLY & 1) {
+      if (do_node) vin[0] = a.in[onode];
+      if (do_cell) {
+#pragma unroll
+        for (int r = 0; r < 6; ++r) vin[1 + r] = a.in[r * PL + ocell];
+      }
+    }
+    if ((EARLY & 2) && a.heun) {
+      if (do_node) vout[0] = a.out[onode];
+      if (do_cell) {
+#pragma unroll
+        for (int r = 0; r < 6; ++r) vout[1 + r] = a.out[r * PL + ocell];
+      }
+    }
     __syncthreads();
     if (fthread) {
       const int prv = cur ^ 1;
-      const bool in_y = ly >= 0 && (ly < tyv || (ly == tyv && last_y));
-      const bool cell_y = ly >= 0 && ly < tyv;
       // E_z at (x-node f, y-node ly) (ct_contact_pyccel.py:35-61), walls zero it
-      const bool node_x = f >= ix0 && (f < ix0 + nxv || (f == ix0 + nxv && last_x));
-      if (node_x && in_y) {
+      if (do_node) {
         double e = ct_node(double(sh.fx[1][tid - 1]), double(sh.fx[1][tid]),
                            double(sh.fy[prv][5][tid]), double(sh.fy[cur][5][tid]),
                            double(sh.fx[0][tid - 1]), double(sh.fx[0][tid]),
@@ -243,20 +269,19 @@ __device__ __forceinline__ void stage2d_body(const Stage2Args& a, int tile, int 
         if ((xlo && f == 2) || (xhi && f == a.nx + 2) || (ylo && gy == 2) ||
             (yhi && gy == a.ny + 2))
           e = 0.0;
-        const size_t o = 6 * PL + (size_t)f * YN + gy;
-        double v = a.in[o] - dt * e;
-        if (a.heun) v = (v + a.out[o]) * 0.5;
-        a.out[o] = v;
+        double v = ((EARLY & 1) ? vin[0] : a.in[onode]) - dt * e;
+        if (a.heun) v = (v + ((EARLY & 2) ? vout[0] : a.out[onode])) * 0.5;
+        a.out[onode] = v;
       }
       if (cell_y) {
-        if (f - 1 >= ix0 && f - 1 < ix0 + nxv) {
-          const size_t c = (size_t)(f - 1) * Y + gy;
+        if (do_cell) {
 #pragma unroll
           for (int r = 0; r < 6; ++r) {
             const double dfx = double(fcur[r]) - double(fprev[r]);
-            double v = a.in[r * PL + c] - (dtdx * dfx + dyy[r]);
-            if (a.heun) v = (v + a.out[r * PL + c]) * 0.5;
-            a.out[r * PL + c] = v;
+            const size_t o = r * PL + ocell;
+            double v = ((EARLY & 1) ? vin[1 + r] : a.in[o]) - (dtdx * dfx + dyy[r]);
+            if (a.heun) v = (v + ((EARLY & 2) ? vout[1 + r] : a.out[o])) * 0.5;
+            a.out[o] = v;
           }
         }
         if (f >= ix0 && f < ix0 + nxv) {

@@ -247,7 +247,11 @@ struct CellPrim {
 // `rslots` (a block of a refined hierarchy, or null): the block's 6 face slots in
 // a.rec; the fluxes the stage solves on those faces of the block are written there as
 // double, in the order of the cell fields (the refluxing reads them).
-template <typename P, int TY, int TZ, int LIM, bool DIRECT = false>
+// EARLY: which stored values the CT and the update start from are loaded before the
+// plane's barrier rather than where they are used (bit 0: those of `in`, bit 1: those
+// of `out` for the Heun average); early loads overlap their latency with the barrier
+// and the ring plane's loads, at the price of the registers that hold them.
+template <typename P, int TY, int TZ, int LIM, bool DIRECT = false, int EARLY = 3>
 __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz_, int seg_,
                                              StageShared<P, TY, TZ>& sh,
                                              const long long* hood = nullptr,
@@ -529,17 +533,49 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
       face_solve<P, LIM>(q, a.limiter, bnp[0], gam, inv_gm1, fl);
       keep_x_face(fl);
     }
+    // d. the stored values the CT and the update below start from (the potentials of
+    //    plane f's edges, the cells of plane f-1; with the Heun average also their
+    //    values in `out`), loaded before the barrier so that their latency overlaps it
+    //    and the loads of the ring plane
+    const bool in_y = ly >= 0 && (ly < tyv || (ly == tyv && last_y));
+    const bool in_z = lz >= 0 && (lz < tzv || (lz == tzv && last_z));
+    const bool cell_y = ly >= 0 && ly < tyv, cell_z = lz >= 0 && lz < tzv;
+    const bool node_x = f >= ix0 && (f < ix0 + nxv || (f == ix0 + nxv && last_x));
+    const bool do_ex = fthread && f >= ix0 && f < ix0 + nxv && in_y && in_z;
+    const bool do_ey = fthread && node_x && cell_y && in_z;
+    const bool do_ez = fthread && node_x && in_y && cell_z;
+    const bool do_cell = fthread && cell_y && cell_z && f - 1 >= ix0 && f - 1 < ix0 + nxv;
+    const size_t oex = d.ox + (size_t)f * d.xplane, oey = d.oy + (size_t)f * d.yplane;
+    const size_t oez = d.oz + (size_t)f * d.zplane, ocell = (size_t)(f - 1) * d.cplane;
+    double vin[8], vout[8];  // A_x, A_y, A_z at the edges, then the 5 cell fields
+#pragma unroll
+    for (int r = 0; r < 8; ++r) vin[r] = vout[r] = 0.0;
+    if constexpr (EARLY & 1) {
+      if (do_ex) vin[0] = ldb(inb + oex, c1);
+      if (do_ey) vin[1] = ldb(inb + oey, c1);
+      if (do_ez) vin[2] = ldb(inb + oez, c0);
+      if (do_cell) {
+#pragma unroll
+        for (int r = 0; r < 5; ++r) vin[3 + r] = ldb(inb + ocell + r * d.field, c0);
+      }
+    }
+    if ((EARLY & 2) && a.heun) {
+      if (do_ex) vout[0] = ldb(outb + oex, c1);
+      if (do_ey) vout[1] = ldb(outb + oey, c1);
+      if (do_ez) vout[2] = ldb(outb + oez, c0);
+      if (do_cell) {
+#pragma unroll
+        for (int r = 0; r < 5; ++r) vout[3 + r] = ldb(outb + ocell + r * d.field, c0);
+      }
+    }
     __syncthreads();
     // e. the LDS plane is free: plane f+1 goes in for the next iteration
     store_plane(f + 1);
     if (fthread) {
       const int e0 = (f + 2) % 3, e1 = f % 3;  // planes f-1 and f of the E rings
-      const bool in_y = ly >= 0 && (ly < tyv || (ly == tyv && last_y));
-      const bool in_z = lz >= 0 && (lz < tzv || (lz == tzv && last_z));
-      const bool cell_y = ly >= 0 && ly < tyv, cell_z = lz >= 0 && lz < tzv;
       const int tym = tid - T::FZ, tzm = tid - 1, tymz = tid - T::FZ - 1;
       // f1. E_x on the x-cell plane f at (y-node ly, z-node lz): plane (a=y, b=z)
-      if (f >= ix0 && f < ix0 + nxv && in_y && in_z) {
+      if (do_ex) {
         const double e = ct_node(
             double(sh.fy[4][tzm]), double(sh.fy[4][tid]),
             double(sh.fz[4][tym]), double(sh.fz[4][tid]),
@@ -547,14 +583,13 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
             double(sh.fzc[cur][0][tym]), double(sh.fzc[cur][0][tid]),
             double(sh.ex[f & 1][tymz]), double(sh.ex[f & 1][tzm]), double(sh.ex[f & 1][tym]),
             double(sh.ex[f & 1][tid]));
-        const size_t o = d.ox + (size_t)f * d.xplane;  // E_x's A_x at (f, gy, gz): column c1
-        double v = ldb(inb + o, c1) - dt * e;
-        if (a.heun) v = (v + ldb(outb + o, c1)) * 0.5;
-        stb(outb + o, c1, v);
+        // E_x's A_x at (f, gy, gz): column c1
+        double v = ((EARLY & 1) ? vin[0] : ldb(inb + oex, c1)) - dt * e;
+        if (a.heun) v = (v + ((EARLY & 2) ? vout[0] : ldb(outb + oex, c1))) * 0.5;
+        stb(outb + oex, c1, v);
       }
-      const bool node_x = f >= ix0 && (f < ix0 + nxv || (f == ix0 + nxv && last_x));
       // f2. E_y at (x-node f, y-cell ly, z-node lz): plane (a=z, b=x)
-      if (node_x && cell_y && in_z) {
+      if (do_ey) {
         const double e = ct_node(
             double(sh.fzc[prv][1][tid]), double(sh.fzc[cur][1][tid]),
             double(sh.fx[2][tzm]), double(sh.fx[2][tid]),
@@ -562,13 +597,12 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
             double(sh.fx[0][tzm]), double(sh.fx[0][tid]),
             double(sh.ey[e0][tzm]), double(sh.ey[e0][tid]), double(sh.ey[e1][tzm]),
             double(sh.ey[e1][tid]));
-        const size_t o = d.oy + (size_t)f * d.yplane;
-        double v = ldb(inb + o, c1) - dt * e;
-        if (a.heun) v = (v + ldb(outb + o, c1)) * 0.5;
-        stb(outb + o, c1, v);
+        double v = ((EARLY & 1) ? vin[1] : ldb(inb + oey, c1)) - dt * e;
+        if (a.heun) v = (v + ((EARLY & 2) ? vout[1] : ldb(outb + oey, c1))) * 0.5;
+        stb(outb + oey, c1, v);
       }
       // f3. E_z at (x-node f, y-node ly, z-cell lz): plane (a=x, b=y)
-      if (node_x && in_y && cell_z) {
+      if (do_ez) {
         const double e = ct_node(
             double(sh.fx[1][tym]), double(sh.fx[1][tid]),
             double(sh.fyc[prv][1][tid]), double(sh.fyc[cur][1][tid]),
@@ -576,23 +610,22 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
             double(sh.fyc[prv][0][tid]), double(sh.fyc[cur][0][tid]),
             double(sh.ez[e0][tym]), double(sh.ez[e1][tym]), double(sh.ez[e0][tid]),
             double(sh.ez[e1][tid]));
-        const size_t o = d.oz + (size_t)f * d.zplane;
-        double v = ldb(inb + o, c0) - dt * e;
-        if (a.heun) v = (v + ldb(outb + o, c0)) * 0.5;
-        stb(outb + o, c0, v);
+        double v = ((EARLY & 1) ? vin[2] : ldb(inb + oez, c0)) - dt * e;
+        if (a.heun) v = (v + ((EARLY & 2) ? vout[2] : ldb(outb + oez, c0))) * 0.5;
+        stb(outb + oez, c0, v);
       }
       // g. update of cell (f-1, ly, lz): x-faces f-1 and f, y/z terms of plane f-1
       if (cell_y && cell_z) {
-        if (f - 1 >= ix0 && f - 1 < ix0 + nxv) {
-          const size_t c = (size_t)(f - 1) * d.cplane;
+        if (do_cell) {
 #pragma unroll
           for (int r = 0; r < 5; ++r) {
             const double dfx = double(fcur[r]) - double(fprev[r]);
-            const size_t o = c + r * d.field;
+            const size_t o = ocell + r * d.field;
+            const double u0 = (EARLY & 1) ? vin[3 + r] : ldb(inb + o, c0);
             double v;
-            if constexpr (FAST) v = ldb(inb + o, c0) - (dtdx * dfx + dyy[r]);
-            else v = ldb(inb + o, c0) - (dtdx * dfx + dyy[r] + dzz[r]);
-            if (a.heun) v = (v + ldb(outb + o, c0)) * 0.5;
+            if constexpr (FAST) v = u0 - (dtdx * dfx + dyy[r]);
+            else v = u0 - (dtdx * dfx + dyy[r] + dzz[r]);
+            if (a.heun) v = (v + ((EARLY & 2) ? vout[3 + r] : ldb(outb + o, c0))) * 0.5;
             stb(outb + o, c0, v);
           }
         }
