@@ -133,13 +133,17 @@ k_stage3d_blocks(StageArgs base, const BlockDesc* __restrict__ blocks, const int
 // an XCD are the two halves of its m-th block.  Three waves per SIMD (the
 // register allocation spills a few values) run four such workgroups per CU, where one
 // 384-thread workgroup per block fits only six waves at its natural 187 registers:
-// 4.0 face solves per cell instead of 3.66, and the stage 30 % faster.
+// 4.0 face solves per cell instead of 3.66, and the stage 30 % faster.  EARLY as in
+// stage3d_body: the registers leave room for the early loads of the Heun average's
+// edge values only, so the mixed path takes its cell values through LDS
+// (LDS_OUT_CELLS | EARLY_OUT_EDGES; 7.7 KB more LDS, four workgroups still fit a CU).
 template <typename P, int WAVES, int LIM, bool DIRECT, int EARLY = 0>
 __global__ void __launch_bounds__((Tile<B / 2, B>::THREADS), WAVES)
 k_stage3d_halves(StageArgs base, const BlockDesc* __restrict__ blocks, const int* __restrict__ ids,
                  int nblocks, const int* __restrict__ lookup, Levels lv) {
   __shared__ StageShared<P, B / 2, B> sh;
   __shared__ long long hood[27];
+  __shared__ uint32_t dma[(EARLY & LDS_OUT_CELLS) ? 10 * Tile<B / 2, B>::THREADS : 1];
   const int slot = blockIdx.x >> 3, half = slot & 1;
   const int b = xcd_block(slot >> 1, blockIdx.x & 7, nblocks);
   if (b >= nblocks || (slot >> 1) >= (nblocks + 7) / 8) return;
@@ -150,7 +154,8 @@ k_stage3d_halves(StageArgs base, const BlockDesc* __restrict__ blocks, const int
     __syncthreads();
   }
   stage3d_body<P, B / 2, B, LIM, DIRECT, EARLY>(block_args(base, d), half, 0, 0, sh, hood, base.in,
-                                         base.rec_slots ? base.rec_slots + 6 * id : nullptr);
+                                                base.rec_slots ? base.rec_slots + 6 * id : nullptr,
+                                                dma);
 }
 
 // The CFL rate (mhd3d.py:674-724) over the cells of every block: a workgroup per

@@ -247,16 +247,23 @@ struct CellPrim {
 // `rslots` (a block of a refined hierarchy, or null): the block's 6 face slots in
 // a.rec; the fluxes the stage solves on those faces of the block are written there as
 // double, in the order of the cell fields (the refluxing reads them).
-// EARLY: which stored values the CT and the update start from are loaded before the
-// plane's barrier rather than where they are used (bit 0: those of `in`, bit 1: those
-// of `out` for the Heun average); early loads overlap their latency with the barrier
-// and the ring plane's loads, at the price of the registers that hold them.
-template <typename P, int TY, int TZ, int LIM, bool DIRECT = false, int EARLY = 3>
+// EARLY (the EARLY_* bits): which stored values the CT and the update start from are
+// loaded before the plane's barrier rather than where they are used -- the cells and
+// the edges of `in`, and of `out` for the Heun average; early loads overlap their
+// latency with the barrier and the ring plane's loads, at the price of the registers
+// that hold them.  LDS_OUT_CELLS: the 5 cell values of `out` go to LDS instead
+// (`dma`: 10 x THREADS words, the low and high halves of each), by loads that write
+// LDS directly and hold no register.
+constexpr int EARLY_IN_CELLS = 1, EARLY_OUT_CELLS = 2, LDS_OUT_CELLS = 4, EARLY_IN_EDGES = 8,
+              EARLY_OUT_EDGES = 16;
+constexpr int EARLY_ALL = EARLY_IN_CELLS | EARLY_OUT_CELLS | EARLY_IN_EDGES | EARLY_OUT_EDGES;
+template <typename P, int TY, int TZ, int LIM, bool DIRECT = false, int EARLY = EARLY_ALL>
 __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz_, int seg_,
                                              StageShared<P, TY, TZ>& sh,
                                              const long long* hood = nullptr,
                                              const double* root = nullptr,
-                                             const int* rslots = nullptr) {
+                                             const int* rslots = nullptr,
+                                             uint32_t* dma = nullptr) {
   using T = Tile<TY, TZ>;
   constexpr bool FAST = sizeof(P) == 4;
   const Grid3 g = a.g;
@@ -550,27 +557,62 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
     double vin[8], vout[8];  // A_x, A_y, A_z at the edges, then the 5 cell fields
 #pragma unroll
     for (int r = 0; r < 8; ++r) vin[r] = vout[r] = 0.0;
-    if constexpr (EARLY & 1) {
+    constexpr bool in_e = EARLY & EARLY_IN_EDGES, out_e = EARLY & EARLY_OUT_EDGES;
+    constexpr bool in_c = EARLY & EARLY_IN_CELLS;
+    constexpr bool out_c = EARLY & (EARLY_OUT_CELLS | LDS_OUT_CELLS);
+    if constexpr (in_e) {
       if (do_ex) vin[0] = ldb(inb + oex, c1);
       if (do_ey) vin[1] = ldb(inb + oey, c1);
       if (do_ez) vin[2] = ldb(inb + oez, c0);
+    }
+    if constexpr (in_c) {
       if (do_cell) {
 #pragma unroll
         for (int r = 0; r < 5; ++r) vin[3 + r] = ldb(inb + ocell + r * d.field, c0);
       }
     }
-    if ((EARLY & 2) && a.heun) {
+    if (out_e && a.heun) {
       if (do_ex) vout[0] = ldb(outb + oex, c1);
       if (do_ey) vout[1] = ldb(outb + oey, c1);
       if (do_ez) vout[2] = ldb(outb + oez, c0);
+    }
+    if ((EARLY & EARLY_OUT_CELLS) && a.heun) {
       if (do_cell) {
 #pragma unroll
         for (int r = 0; r < 5; ++r) vout[3 + r] = ldb(outb + ocell + r * d.field, c0);
       }
     }
+    if constexpr ((EARLY & LDS_OUT_CELLS) != 0) {
+      // every face column's cell of plane f - 1 (in the block: plane >= ix0 - 2 >= 0,
+      // clamped column), whether or not it is updated; lane l of wave w writes word
+      // w * 64 + l of each row
+      if (a.heun && fthread) {
+        const char* src = outb + ocell + c0;
+        uint32_t* row = dma + (tid & ~63);
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+#pragma unroll
+          for (int h = 0; h < 2; ++h)
+            __builtin_amdgcn_global_load_lds(
+                (const __attribute__((address_space(1))) void*)(src + r * d.field + 4 * h),
+                (__attribute__((address_space(3))) void*)(row + (2 * r + h) * T::THREADS), 4, 0,
+                0);
+      }
+    }
     __syncthreads();
     // e. the LDS plane is free: plane f+1 goes in for the next iteration
     store_plane(f + 1);
+    if constexpr ((EARLY & LDS_OUT_CELLS) != 0) {
+      // the ring plane's loads were issued after these and have been waited for, so
+      // the wait for every vector memory operation here costs nothing more
+      if (a.heun && do_cell) {
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+          vout[3 + r] = __hiloint2double((int)dma[(2 * r + 1) * T::THREADS + tid],
+                                         (int)dma[2 * r * T::THREADS + tid]);
+      }
+    }
     if (fthread) {
       const int e0 = (f + 2) % 3, e1 = f % 3;  // planes f-1 and f of the E rings
       const int tym = tid - T::FZ, tzm = tid - 1, tymz = tid - T::FZ - 1;
@@ -584,8 +626,8 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
             double(sh.ex[f & 1][tymz]), double(sh.ex[f & 1][tzm]), double(sh.ex[f & 1][tym]),
             double(sh.ex[f & 1][tid]));
         // E_x's A_x at (f, gy, gz): column c1
-        double v = ((EARLY & 1) ? vin[0] : ldb(inb + oex, c1)) - dt * e;
-        if (a.heun) v = (v + ((EARLY & 2) ? vout[0] : ldb(outb + oex, c1))) * 0.5;
+        double v = (in_e ? vin[0] : ldb(inb + oex, c1)) - dt * e;
+        if (a.heun) v = (v + (out_e ? vout[0] : ldb(outb + oex, c1))) * 0.5;
         stb(outb + oex, c1, v);
       }
       // f2. E_y at (x-node f, y-cell ly, z-node lz): plane (a=z, b=x)
@@ -597,8 +639,8 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
             double(sh.fx[0][tzm]), double(sh.fx[0][tid]),
             double(sh.ey[e0][tzm]), double(sh.ey[e0][tid]), double(sh.ey[e1][tzm]),
             double(sh.ey[e1][tid]));
-        double v = ((EARLY & 1) ? vin[1] : ldb(inb + oey, c1)) - dt * e;
-        if (a.heun) v = (v + ((EARLY & 2) ? vout[1] : ldb(outb + oey, c1))) * 0.5;
+        double v = (in_e ? vin[1] : ldb(inb + oey, c1)) - dt * e;
+        if (a.heun) v = (v + (out_e ? vout[1] : ldb(outb + oey, c1))) * 0.5;
         stb(outb + oey, c1, v);
       }
       // f3. E_z at (x-node f, y-node ly, z-cell lz): plane (a=x, b=y)
@@ -610,8 +652,8 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
             double(sh.fyc[prv][0][tid]), double(sh.fyc[cur][0][tid]),
             double(sh.ez[e0][tym]), double(sh.ez[e1][tym]), double(sh.ez[e0][tid]),
             double(sh.ez[e1][tid]));
-        double v = ((EARLY & 1) ? vin[2] : ldb(inb + oez, c0)) - dt * e;
-        if (a.heun) v = (v + ((EARLY & 2) ? vout[2] : ldb(outb + oez, c0))) * 0.5;
+        double v = (in_e ? vin[2] : ldb(inb + oez, c0)) - dt * e;
+        if (a.heun) v = (v + (out_e ? vout[2] : ldb(outb + oez, c0))) * 0.5;
         stb(outb + oez, c0, v);
       }
       // g. update of cell (f-1, ly, lz): x-faces f-1 and f, y/z terms of plane f-1
@@ -621,11 +663,11 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
           for (int r = 0; r < 5; ++r) {
             const double dfx = double(fcur[r]) - double(fprev[r]);
             const size_t o = ocell + r * d.field;
-            const double u0 = (EARLY & 1) ? vin[3 + r] : ldb(inb + o, c0);
+            const double u0 = in_c ? vin[3 + r] : ldb(inb + o, c0);
             double v;
             if constexpr (FAST) v = u0 - (dtdx * dfx + dyy[r]);
             else v = u0 - (dtdx * dfx + dyy[r] + dzz[r]);
-            if (a.heun) v = (v + ((EARLY & 2) ? vout[3 + r] : ldb(outb + o, c0))) * 0.5;
+            if (a.heun) v = (v + (out_c ? vout[3 + r] : ldb(outb + o, c0))) * 0.5;
             stb(outb + o, c0, v);
           }
         }
@@ -661,10 +703,11 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
 // The workgroups are renumbered so that XCD x runs a contiguous run of tiles (z
 // fastest, then y, then the x segment): tiles running together on one XCD are
 // neighbours, and the halo lines they share are fetched into that L2 once.
-template <typename P, int TY, int TZ, int WAVES, int LIM>
+template <typename P, int TY, int TZ, int WAVES, int LIM, int EARLY>
 __global__ void __launch_bounds__((Tile<TY, TZ>::THREADS), WAVES)
 k_stage3d(StageArgs a) {
   __shared__ StageShared<P, TY, TZ> sh;
+  __shared__ uint32_t dma[(EARLY & LDS_OUT_CELLS) ? 10 * Tile<TY, TZ>::THREADS : 1];
   const Grid3 g = a.g;
   const int tiles_z = (g.nz + TZ - 1) / TZ, tiles_y = (g.ny + TY - 1) / TY;
   const int n = gridDim.x, xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
@@ -672,18 +715,20 @@ k_stage3d(StageArgs a) {
   const int logical = xcd * per + min(xcd, rem) + k;
   const int tz = logical % tiles_z;
   const int rest = logical / tiles_z;
-  stage3d_body<P, TY, TZ, LIM>(a, rest % tiles_y, tz, rest / tiles_y, sh);
+  stage3d_body<P, TY, TZ, LIM, false, EARLY>(a, rest % tiles_y, tz, rest / tiles_y, sh, nullptr,
+                                            nullptr, nullptr, dma);
 }
 
 
 // WAVES: the waves per SIMD the register allocation must allow (2 or 3 for the
 // mixed path: 3 trades a few spills for occupancy; the autotuner picks).
-template <typename P, int TY, int TZ, int WAVES, int LIM = -1>
+template <typename P, int TY, int TZ, int WAVES, int LIM = -1, int EARLY = EARLY_ALL>
 inline int launch_stage3d(const StageArgs& a, hipStream_t s) {
   using T = Tile<TY, TZ>;
   const int tiles = ((a.g.ny + TY - 1) / TY) * ((a.g.nz + TZ - 1) / TZ);
   const int segs = (a.g.nx + a.nxs - 1) / a.nxs;
-  hipLaunchKernelGGL((k_stage3d<P, TY, TZ, WAVES, LIM>), dim3(tiles * segs), dim3(T::THREADS), 0, s, a);
+  hipLaunchKernelGGL((k_stage3d<P, TY, TZ, WAVES, LIM, EARLY>), dim3(tiles * segs), dim3(T::THREADS),
+                     0, s, a);
   return wb_last_error();
 }
 

@@ -135,16 +135,38 @@ __device__ __forceinline__ void fill_edge_at(double* __restrict__ buf, const Hoo
       w[a][1] = 0.5;
     }
   }
-  double acc = 0.0;
-  for (int x = 0; x < nt[0]; ++x)
-    for (int y = 0; y < nt[1]; ++y)
-      for (int z = 0; z < nt[2]; ++z) {
+  // the (up to) 8 sources first, then their loads together, then the sum in source
+  // order over the ones that exist (a source outside the level below is excluded by
+  // the nesting rule; its load reads a valid dummy address and is not summed)
+  long long src[8];
+  bool ok[8];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int z = 0; z < 2; ++z) {
+        const int i = (x * 2 + y) * 2 + z;
         const int u[3] = {c0[0] + x - po[0], c0[1] + y - po[1], c0[2] + z - po[2]};
         int cl[3];
-        const int k = hood_master(coarse, c, u, cl);
-        if (k < 0) continue;  // outside the level below: excluded by the nesting rule
-        const double v = buf[coarse.off[k] + edge_at(c, cl[0], cl[1], cl[2])];
-        acc = acc + ((w[0][x] * w[1][y]) * w[2][z]) * v;
+        const bool used = x < nt[0] && y < nt[1] && z < nt[2];
+        const int k = used ? hood_master(coarse, c, u, cl) : -1;
+        ok[i] = k >= 0;
+        src[i] = ok[i] ? coarse.off[k] + edge_at(c, cl[0], cl[1], cl[2]) : 0;
+      }
+  double v[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) v[i] = buf[src[i]];
+  double acc = 0.0;
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int z = 0; z < 2; ++z) {
+        const int i = (x * 2 + y) * 2 + z;
+        const double t = acc + ((w[0][x] * w[1][y]) * w[2][z]) * v[i];
+        acc = ok[i] ? t : acc;
       }
   me[edge_at(c, p[0], p[1], p[2])] = acc;
 }
@@ -210,26 +232,40 @@ __device__ __forceinline__ void interp_cell(const double* __restrict__ buf, cons
   // the 8 coarse cells (field 0 offsets; -1: outside the level below, excluded by
   // the nesting rule), found once for the 5 fields
   long long src[8];
+  bool ok[8];
+#pragma unroll
   for (int x = 0; x < 2; ++x)
+#pragma unroll
     for (int y = 0; y < 2; ++y)
+#pragma unroll
       for (int z = 0; z < 2; ++z) {
+        const int i = (x * 2 + y) * 2 + z;
         const int u[3] = {c0[0] + x - po[0], c0[1] + y - po[1], c0[2] + z - po[2]};
         const int r0 = rel_of(u[0]), r1 = rel_of(u[1]), r2 = rel_of(u[2]);
         const int k = hood_index(r0, r1, r2);
-        src[(x * 2 + y) * 2 + z] =
-            coarse.id[k] < 0 ? -1
-                             : coarse.off[k] + cell_at(0, u[0] - r0 * B + 2, u[1] - r1 * B + 2,
-                                                       u[2] - r2 * B + 2);
+        ok[i] = coarse.id[k] >= 0;
+        // a missing source reads a valid dummy address and is not summed
+        src[i] = ok[i] ? coarse.off[k] + cell_at(0, u[0] - r0 * B + 2, u[1] - r1 * B + 2,
+                                                 u[2] - r2 * B + 2)
+                       : 0;
       }
+  // every load of the 5 fields is issued before the sums, in source order over the
+  // sources that exist
+#pragma unroll
   for (int f = 0; f < 5; ++f) {
+    double v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = buf[src[i] + (long long)f * NCELL];
     double acc = 0.0;
+#pragma unroll
     for (int x = 0; x < 2; ++x)
+#pragma unroll
       for (int y = 0; y < 2; ++y)
+#pragma unroll
         for (int z = 0; z < 2; ++z) {
-          const long long o = src[(x * 2 + y) * 2 + z];
-          if (o < 0) continue;
-          const double v = buf[o + (long long)f * NCELL];
-          acc = acc + ((w[0][x] * w[1][y]) * w[2][z]) * v;
+          const int i = (x * 2 + y) * 2 + z;
+          const double t = acc + ((w[0][x] * w[1][y]) * w[2][z]) * v[i];
+          acc = ok[i] ? t : acc;
         }
     me[cell_at(f, p[0], p[1], p[2])] = acc;
   }

@@ -37,7 +37,8 @@ int wb_amr3d_stage_mixed(const double* in, double* out, const void* blocks, cons
     const dim3 halves(((nids + 7) / 8) * 16), tile_h(Tile<B / 2, B>::THREADS);
     const dim3 tile_b(Tile<B, B>::THREADS), blocks8(((nids + 7) / 8) * 8);
     if (waves == 3 && direct)
-      hipLaunchKernelGGL((k_stage3d_halves<float, 3, L, true>), halves, tile_h, 0, s, a,
+      hipLaunchKernelGGL((k_stage3d_halves<float, 3, L, true, LDS_OUT_CELLS | EARLY_OUT_EDGES>),
+                         halves, tile_h, 0, s, a,
                          (const BlockDesc*)blocks, id, nids, lk, lv);
     else if (waves == 3)
       hipLaunchKernelGGL((k_stage3d_halves<float, 3, L, false>), halves, tile_h, 0, s, a,
