@@ -385,6 +385,20 @@ int wb_amr2d_stage_mixed(const double* in, double* out, const void* patches, con
                          double eta, double kpar, double kperp, int limiter, const double* dt,
                          int heun, int32_t* bad, int nxs, int tile, int workgroups, void* stream);
 
+/* The same stage with a workgroup per 16 x 16 tile of a patch and no march: items are
+ * int32 triples (patch, tile along y, tile along x), each tile inside its patch
+ * (16 tile < ny, 16 x-tile < nx), one launch of nitems workgroups.  Ideal MHD only (the
+ * non-ideal terms stay with wb_amr2d_stage); the same bits as wb_amr2d_stage (the
+ * same expressions on the same values).  Replaces, like wb_amr2d_stage, MHD2D._advance
+ * on every patch of a level (mhd.py:889-927). */
+int wb_amr2d_stage_tiles(const double* in, double* out, const void* patches, const void* items,
+                         int nitems, double b0x, double b0y, double gamma, int limiter,
+                         const double* dt, int heun, int32_t* bad, void* stream);
+int wb_amr2d_stage_tiles_mixed(const double* in, double* out, const void* patches,
+                               const void* items, int nitems, double b0x, double b0y,
+                               double gamma, int limiter, const double* dt, int heun,
+                               int32_t* bad, void* stream);
+
 /* The flux register and refluxing (FluxRegister.update with scale 0.5 dt and
  * FluxRegister.reflux, mhd.py:911-919,989-990).  For each group (first row, count) of
  * rows (13 int64: coarse patch, cell, axis, side, face, and the two fine faces) the

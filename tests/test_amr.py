@@ -122,13 +122,14 @@ def _assert_interiors_equal(buf, oracle):
 
 # -- GPU ----------------------------------------------------------------------------------
 @pytest.mark.gpu
-@pytest.mark.parametrize("tile,nxs", [(30, 8), (62, 16), (126, 5)])
-def test_gpu_hierarchy_is_bitwise_the_oracle(gpu, tile, nxs):
+@pytest.mark.parametrize("kernel,tile,nxs", [("march", 30, 8), ("march", 62, 16),
+                                             ("march", 126, 5), ("tiles", 62, 16)])
+def test_gpu_hierarchy_is_bitwise_the_oracle(gpu, kernel, tile, nxs):
     from wellbalanced_amd.amr import AMRMHD2D
 
     h = _refined(3)
     amr = _oracle(h, _smooth_state())
-    model = AMRMHD2D(h, limiter="minmod", tile=tile, nxs=nxs)
+    model = AMRMHD2D(h, limiter="minmod", tile=tile, nxs=nxs, kernel=kernel)
     model.set_state(**_smooth_state())
     np.testing.assert_array_equal(model.buffer(), amr.buf)
     for _ in range(3):
@@ -171,14 +172,38 @@ def test_gpu_autotuned_launch_shape_keeps_the_bits(gpu):
     from wellbalanced_amd.amr import AMRMHD2D
 
     h = _refined(3)
-    a = AMRMHD2D(h, limiter="minmod")
+    a = AMRMHD2D(h, limiter="minmod", kernel="march")
     b = AMRMHD2D(h, limiter="minmod")
     a.set_state(**_smooth_state())
     b.set_state(**_smooth_state())
     choice = autotune.tune(b, steps=1)
-    assert set(choice) == {"tile", "nxs", "workgroups"}
+    assert choice["kernel"] in ("tiles", "march")
+    if choice["kernel"] == "march":
+        assert set(choice) == {"kernel", "tile", "nxs", "workgroups"}
     for _ in range(2):
         dt = a.stable_step()
         a.step(dt)
         b.step(dt)
     np.testing.assert_array_equal(a.buffer(), b.buffer())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("limiter", ["minmod", "vanleer", "mc", "none"])
+def test_gpu_tiled_mixed_stage_is_bitwise_the_marching_one(gpu, limiter):
+    """The tiled stage evaluates every flux, E_z and update with the marching stage's
+    expressions on the same values: the mixed paths agree bit for bit too, on patches
+    of sizes that are not multiples of the 16-cell tile."""
+    from wellbalanced_amd.amr import AMRMHD2D, blast_hierarchy, blast_state
+
+    h = blast_hierarchy(64, 3, sizes=(10, 40), seed=3)
+    assert any(side % 16 for p in h.patches_at_level(2) for side in p.shape)
+    b0 = (1.0 / np.sqrt(2.0), 1.0 / np.sqrt(2.0))
+    models = [AMRMHD2D(h, mean_field=b0, limiter=limiter, precision="mixed", kernel=k)
+              for k in ("tiles", "march")]
+    for m in models:
+        m.set_state(**blast_state())
+    for _ in range(4):
+        dt = models[1].stable_step()
+        for m in models:
+            m.step(dt)
+    np.testing.assert_array_equal(models[0].buffer(), models[1].buffer())

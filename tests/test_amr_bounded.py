@@ -185,14 +185,15 @@ def test_the_fill_tables_fill_the_ghosts_as_the_oracle(periodic_x):
 # -- GPU ----------------------------------------------------------------------------------
 @pytest.mark.gpu
 @pytest.mark.parametrize("periodic_x", [False, True])
-@pytest.mark.parametrize("tile,nxs", [(30, 8), (62, 16)])
-def test_gpu_bounded_hierarchy_is_bitwise_the_oracle(gpu, periodic_x, tile, nxs):
+@pytest.mark.parametrize("kernel,tile,nxs", [("march", 30, 8), ("march", 62, 16),
+                                             ("tiles", 62, 16)])
+def test_gpu_bounded_hierarchy_is_bitwise_the_oracle(gpu, periodic_x, kernel, tile, nxs):
     from wellbalanced_amd.amr import AMRMHD2D
 
     h = _hierarchy(periodic_x)
     amr = _oracle(h, periodic_x)
     model = AMRMHD2D(h, limiter="minmod", mean_field=B0, boundaries=_boundaries(periodic_x),
-                     tile=tile, nxs=nxs)
+                     tile=tile, nxs=nxs, kernel=kernel)
     model.set_state(**_state())
     np.testing.assert_array_equal(model.buffer(), amr.buf)
     for _ in range(4):

@@ -11,9 +11,11 @@ ping-ponged as in `MHD2D`); a stage is
    quadratic A_z nodes from the level below (gather tables of `amr_maps`), then on a
    non-periodic axis the outflow or wall ghosts of the level's patches on the domain
    faces (``mhd.py:748-777``), which the next level's interpolation may read;
-2. ONE persistent launch over every tile of every patch (`k_stage2d_batched`: the
-   fused 2D stage, workgroups pulling (patch, tile, segment) items from a counter),
-   which updates every patch cell, covered coarse cells included;
+2. ONE launch over every tile of every patch, which updates every patch cell, covered
+   coarse cells included: a workgroup per 16 x 16 tile (`k_stage2d_tiles`, ideal MHD,
+   the default) or the persistent marching stage (`k_stage2d_batched`: workgroups
+   pulling (patch, tile, segment) items from a counter; with diffusion, or as
+   ``kernel="march"``);
 3. the flux register of the coarse cells next to finer patches (one launch for every
    level pair): stage 1 records half its correction, stage 2 (whose output is the Heun
    average) adds the recorded half and its own;
@@ -39,6 +41,8 @@ _PATCH = np.dtype([("off", np.int64), ("nx", np.int32), ("ny", np.int32),
                    ("dx", np.float64), ("dy", np.float64), ("walls", np.int32),
                    ("pad", np.int32)])
 _ITEM = np.dtype([("patch", np.int32), ("tile", np.int32), ("seg", np.int32)])
+KERNELS = ("tiles", "march")
+TILE = 16  # cells per side of the tiled stage's tiles
 
 
 class AMRMHD2D:
@@ -50,7 +54,8 @@ class AMRMHD2D:
                  mean_field=(0.0, 0.0), limiter: str = "minmod", cfl: float = 0.4,
                  resistivity: float = 0.0, conduction: tuple[float, float] = (0.0, 0.0),
                  boundaries: dict[str, str] | None = None, prefix: str = "mhd_",
-                 precision: str = "fp64", tile: int = 62, nxs: int = 16, device=None) -> None:
+                 precision: str = "fp64", tile: int = 62, nxs: int = 16, kernel: str = "tiles",
+                 device=None) -> None:
         if hierarchy.ndim != 2:
             raise ValueError("AMRMHD2D needs a 2D hierarchy")
         if limiter not in LIMITER_CODES:
@@ -67,6 +72,8 @@ class AMRMHD2D:
             raise ValueError(f"precision is one of {PRECISIONS}")
         if tile not in (30, 62, 126):
             raise ValueError("tile is 30, 62 or 126 cells")
+        if kernel not in KERNELS:
+            raise ValueError(f"kernel is one of {KERNELS}")
         kinds = {"x": "outflow", "y": "outflow"}
         for axis_name, kind in (boundaries or {}).items():
             if axis_name not in kinds or kind not in BOUNDARIES:
@@ -93,6 +100,7 @@ class AMRMHD2D:
         self.precision = precision
         self.tile = tile
         self.nxs = nxs
+        self.kernel = kernel
         self.names = {name: prefix + name for name in CONSERVED}
         self.potential = prefix + "az"
         self.device = _device(device)
@@ -125,8 +133,15 @@ class AMRMHD2D:
             self._upload_tables()
 
     def set_tiling(self, tile: int | None = None, nxs: int | None = None,
-                   workgroups: int | None = None) -> None:
-        """Change the launch shape (no effect on the results) and rebuild the work list."""
+                   workgroups: int | None = None, kernel: str | None = None) -> None:
+        """Change the launch shape (no effect on the results) and rebuild the work list:
+        ``kernel`` "tiles" (a workgroup per 16 x 16 tile of a patch; ideal MHD, a model
+        with diffusion runs the marching stage whatever this says) or "march" (the
+        persistent marching stage with ``tile``, ``nxs`` and ``workgroups``)."""
+        if kernel is not None:
+            if kernel not in KERNELS:
+                raise ValueError(f"kernel is one of {KERNELS}")
+            self.kernel = kernel
         if tile is not None:
             if tile not in (30, 62, 126):
                 raise ValueError("tile is 30, 62 or 126 cells")
@@ -208,6 +223,12 @@ class AMRMHD2D:
         sizes = np.array([self.layouts[i[0]].nx * self.layouts[i[0]].ny for i in items])
         order = np.argsort(-sizes, kind="stable")
         items = np.array([items[i] for i in order], _ITEM)
+        # the tiled stage's items: (patch, tile along y, tile along x), a patch's tiles
+        # together and y fastest, so neighbouring tiles run on one XCD
+        tiles = [(k, ty, tx) for k, lay in enumerate(self.layouts)
+                 for tx in range(-(-lay.nx // TILE)) for ty in range(-(-lay.ny // TILE))]
+        self._tiles = torch.from_numpy(np.array(tiles, _ITEM).view(np.uint8).copy()).to(self.device)
+        self.ntiles = len(tiles)
         self._patches = torch.from_numpy(patches.view(np.uint8).copy()).to(self.device)
         self._items = torch.from_numpy(items.view(np.uint8).copy()).to(self.device)
         self._spatch = self._dev(spatch, torch.float64)
@@ -338,11 +359,17 @@ class AMRMHD2D:
         suffix = "" if self.precision == "fp64" else "_mixed"
         code = LIMITER_CODES.index(self.limiter)
         diff = (self.resistivity, *self.conduction)
-        _lib.call("wb_amr2d_stage" + suffix, _lib.ptr(src), _lib.ptr(dst), _lib.ptr(self._patches),
-                  _lib.ptr(self._items), self.nitems, _lib.ptr(self._counter), *self.mean_field,
-                  self.gamma, *diff, code, _lib.ptr(self._dt), int(heun),
-                  self._bad.data_ptr() + 4 * slot, int(self.nxs), int(self.tile),
-                  int(self.workgroups), stream)
+        if self.kernel == "tiles" and not self._diffusive:
+            _lib.call("wb_amr2d_stage_tiles" + suffix, _lib.ptr(src), _lib.ptr(dst),
+                      _lib.ptr(self._patches), _lib.ptr(self._tiles), self.ntiles,
+                      *self.mean_field, self.gamma, code, _lib.ptr(self._dt), int(heun),
+                      self._bad.data_ptr() + 4 * slot, stream)
+        else:
+            _lib.call("wb_amr2d_stage" + suffix, _lib.ptr(src), _lib.ptr(dst),
+                      _lib.ptr(self._patches), _lib.ptr(self._items), self.nitems,
+                      _lib.ptr(self._counter), *self.mean_field, self.gamma, *diff, code,
+                      _lib.ptr(self._dt), int(heun), self._bad.data_ptr() + 4 * slot,
+                      int(self.nxs), int(self.tile), int(self.workgroups), stream)
         rows, groups, n = self._reflux
         if n:  # stage 1 records into the register, stage 2 refluxes (mhd.py:911-919,989)
             _lib.call("wb_amr2d_reflux" + suffix, _lib.ptr(src), _lib.ptr(dst),

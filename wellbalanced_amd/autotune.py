@@ -2,11 +2,12 @@
 
 The fused stages have free parameters that do not change the results (the FP64
 path gives the same bits for any of them): the cells per workgroup along the
-contiguous axis (``tile``: 30, 62, 126 or 254 in 2D -> 32, 64, 128 or 256 threads, the
+contiguous axis (``tile``: 30, 62, 126 or 254 in 2D -> 64, 64, 128 or 256 threads, the
 small tiles and short segments giving a small grid enough workgroups to fill the chip;
 30, 62 or 126 on a hierarchy, whose small patches fill a 30-row tile better) and the
-cells each workgroup marches along x (``nxs``), for the refined hierarchy the number
-of persistent workgroups, for the mixed 3D stages the waves per SIMD the register
+cells each workgroup marches along x (``nxs``), for the refined 2D hierarchy the stage
+kernel (``kernel``: a workgroup per 16 x 16 tile, or the persistent marching stage and
+its number of workgroups), for the mixed 3D stages the waves per SIMD the register
 allocation targets (``waves``), and for the 3D block hierarchy whether blocks with
 every neighbour read them in place instead of filled ghosts (``direct``).  `tune`
 times each candidate on the model's own
@@ -30,6 +31,14 @@ DEFAULTS_2D = {"tile": [30, 62, 126, 254], "nxs": [4, 8, 16, 32, 64]}
 DEFAULTS_3D = {"nxs": [8, 16, 32, 64]}
 DEFAULTS_3D_MIXED = {"nxs": [8, 16, 32, 64], "waves": [2, 3]}
 DEFAULTS_AMR = {"tile": [30, 62, 126], "nxs": [4, 8, 16, 32], "workgroups": [512, 1024, 2048]}
+
+
+def _amr2d_candidates(model) -> list[dict]:
+    """The tiled stage (ideal MHD only), then every marching launch shape."""
+    names = list(DEFAULTS_AMR)
+    march = [dict(zip(names, v), kernel="march")
+             for v in itertools.product(*(DEFAULTS_AMR[n] for n in names))]
+    return ([] if model._diffusive else [{"kernel": "tiles"}]) + march
 DEFAULTS_AMR3D_MIXED = {"waves": [2, 3], "direct": [True, False]}
 DEFAULTS_AMR3D = {"direct": [True, False]}
 
@@ -54,8 +63,9 @@ def _apply(model, choice: dict) -> None:
     model.generation = getattr(model, "generation", 0) + 1
 
 
-def tune(model, space: dict | None = None, steps: int = 3) -> dict:
-    """Pick the fastest launch shape for ``model``; returns and applies it."""
+def tune(model, space: dict | list | None = None, steps: int = 3) -> dict:
+    """Pick the fastest launch shape for ``model``; returns and applies it.  ``space``:
+    a dict of parameter values (every combination is tried) or a list of choices."""
     key = _key(model)
     if key in _CACHE:
         _apply(model, _CACHE[key])
@@ -64,7 +74,7 @@ def tune(model, space: dict | None = None, steps: int = 3) -> dict:
         if hasattr(model, "nblocks"):
             space = DEFAULTS_AMR3D_MIXED if model.precision == "mixed" else DEFAULTS_AMR3D
         elif hasattr(model, "layouts"):
-            space = DEFAULTS_AMR
+            space = _amr2d_candidates(model)
         elif hasattr(model, "n"):
             space = DEFAULTS_3D if model.precision == "fp64" else DEFAULTS_3D_MIXED
         else:
@@ -74,9 +84,10 @@ def tune(model, space: dict | None = None, steps: int = 3) -> dict:
         return {}
     saved = [b.clone() for b in model._buf], model._t.clone(), model.steps
     best, best_t = None, float("inf")
-    names = list(space)
-    for values in itertools.product(*(space[n] for n in names)):
-        choice = dict(zip(names, values))
+    if isinstance(space, dict):  # the product of every parameter's values
+        names = list(space)
+        space = [dict(zip(names, v)) for v in itertools.product(*(space[n] for n in names))]
+    for choice in space:
         _apply(model, choice)
         model.advance(1)
         torch.cuda.synchronize(model.device)

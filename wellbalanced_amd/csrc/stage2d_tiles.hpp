@@ -1,0 +1,233 @@
+// The 2D stage over a refined hierarchy's patches in square tiles: one workgroup of 256
+// threads per 16 x 16 cells of a patch, all tiles of all patches in one launch.
+//
+// The marching stage (stage2d.hpp) walks a strip of columns along x and keeps the
+// x-sweep in registers; on patches of 16..128 cells its strips are short and narrow, and
+// a hierarchy of a few hundred thousand cells gives too few waves to hide the latency of
+// each plane's chain of loads, barriers and face solves.  Here a workgroup does its tile
+// in three passes with no march:
+//   1. the primitive variables of the 20 x 20 cells around the tile (2-cell halo, the
+//      ghosts filled before the launch) into LDS, with each cell's low-face fields (the
+//      B_n of the faces the tile solves) and the cell E_z of the 18 x 18 cells the CT
+//      reads (`mhd_primitive_pyccel.py:51-86`);
+//   2. the 17 x 18 x-faces and 18 x 17 y-faces (`mhd_sweep_pyccel.py:254-319`): the x-
+//      and y-face tasks are padded to 320 each, so every wave solves faces of one kind;
+//   3. per thread one cell and its low-corner node: the CT-contact E_z of the node
+//      (`ct_contact_pyccel.py:35-61`), A_z -= dt E_z, the conservative update of the cell,
+//      the Heun average; the nodes on a patch's high faces in a short extra pass.
+// Every flux, E_z and update is the marching stage's expression on the same values, so
+// both give the same bits (the FP64 instance is the parity path).  Ideal MHD only: the
+// non-ideal face terms stay with the marching stage.
+#pragma once
+
+#include "mhd_device.hpp"
+#include "mhd_fast.hpp"
+#include "stage2d.hpp"
+#include "wbmhd_common.hpp"
+
+namespace wb {
+namespace tiles2d {
+
+constexpr int T = 16;               // cells per tile side
+constexpr int H = T + 4;            // cells loaded per side (2-cell halo)
+constexpr int NC = H * H;           // loaded cells
+constexpr int E = T + 2;            // cells with a cell E_z per side ([-1, T])
+constexpr int NXF = (T + 1) * (T + 2);  // x-faces: x-node [0, T] x cell [-1, T]
+constexpr int NYF = (T + 2) * (T + 1);  // y-faces: cell [-1, T] x y-node [0, T]
+constexpr int PAD = 320;            // face tasks of one kind, a whole number of waves
+constexpr int THREADS = T * T;
+static_assert(NXF <= PAD && NYF <= PAD && PAD % 64 == 0, "face tasks fit whole waves");
+
+template <typename P>
+struct Shared {
+  P prim[8][NC];    // rho, u, v, w, p, B_x, B_y, B_z of cell (i, j), local [-2, T+1]
+  P bxf[NC];        // B_n of the cell's low x-face
+  P byf[NC];        // B_n of the cell's low y-face
+  double ecc[E * E];  // cell E_z, local [-1, T]
+  P fx[7][NXF];     // x-face (x-node i, cell j): index i * (T + 2) + j + 1
+  P fy[7][NYF];     // y-face (cell i, y-node j): index (i + 1) * (T + 1) + j
+};
+
+__device__ __forceinline__ int cell_index(int i, int j) { return (i + 2) * H + (j + 2); }
+__device__ __forceinline__ int xface_index(int i, int j) { return i * (T + 2) + j + 1; }
+__device__ __forceinline__ int yface_index(int i, int j) { return (i + 1) * (T + 1) + j; }
+
+// Tile (tx, ty) of the patch whose stage arguments are `a` (in, out, nx, ny, X, Y, dx,
+// dy, walls set for that patch).
+template <typename P, int LIM>
+__device__ __forceinline__ void tile_body(const Stage2Args& a, int tx, int ty, Shared<P>& sh) {
+  const int tid = threadIdx.x;
+  const int X = a.X, Y = a.Y, YN = Y + 1;
+  const int i0 = 2 + tx * T, j0 = 2 + ty * T;  // haloed index of the tile's first cell
+  const int txv = min(T, a.nx + 2 - i0), tyv = min(T, a.ny + 2 - j0);
+  const bool last_x = i0 + T >= a.nx + 2, last_y = j0 + T >= a.ny + 2;
+  const size_t PL = (size_t)X * Y;
+  const double* az = a.in + 6 * PL;
+  auto AZ = [&](int i, int j) -> double { return az[(size_t)i * YN + j]; };
+
+  // 1. primitive variables of the loaded cells (mhd_primitive_pyccel.py:51-86)
+  for (int k = tid; k < NC; k += THREADS) {
+    const int li = k / H - 2, lj = k % H - 2;
+    const int i = min(max(i0 + li, 0), X - 1), j = min(max(j0 + lj, 0), Y - 1);
+    const size_t c = (size_t)i * Y + j;
+    const double bx0 = (AZ(i, j + 1) - AZ(i, j)) / a.dy + a.b0x;
+    const double bx1 = (AZ(i + 1, j + 1) - AZ(i + 1, j)) / a.dy + a.b0x;
+    const double by0 = -(AZ(i + 1, j) - AZ(i, j)) / a.dx + a.b0y;
+    const double by1 = -(AZ(i + 1, j + 1) - AZ(i, j + 1)) / a.dx + a.b0y;
+    const double r = a.in[c];
+    const double u = a.in[PL + c] / r, v = a.in[2 * PL + c] / r, w = a.in[3 * PL + c] / r;
+    const double bx = 0.5 * (bx0 + bx1);
+    const double by = 0.5 * (by0 + by1);
+    const double b = a.in[5 * PL + c];
+    const double p = (a.gamma - 1.0) * (a.in[4 * PL + c] - 0.5 * r * (u * u + v * v + w * w) -
+                                        0.5 * (bx * bx + by * by + b * b));
+    sh.prim[0][k] = P(r);
+    sh.prim[1][k] = P(u);
+    sh.prim[2][k] = P(v);
+    sh.prim[3][k] = P(w);
+    sh.prim[4][k] = P(p);
+    sh.prim[5][k] = P(bx);
+    sh.prim[6][k] = P(by);
+    sh.prim[7][k] = P(b);
+    sh.bxf[k] = P(bx0);
+    sh.byf[k] = P(by0);
+    if (li >= -1 && li <= T && lj >= -1 && lj <= T)
+      sh.ecc[(li + 1) * E + (lj + 1)] = bx * v - by * u;
+    if (li >= 0 && li < txv && lj >= 0 && lj < tyv) {
+      const bool is_bad = !(r > 0.0 && p > 0.0) ||
+                          !(finite(r) && finite(u) && finite(v) && finite(w) && finite(p) &&
+                            finite(bx) && finite(by) && finite(b));
+      if (is_bad) atomicAdd(a.bad, 1);
+    }
+  }
+  __syncthreads();
+
+  // 2. face fluxes; tasks [0, PAD) are x-faces, [PAD, 2 PAD) y-faces (wave-uniform kinds)
+  const P gam = P(a.gamma), inv_gm1 = P(1.0 / (a.gamma - 1.0));
+  for (int k = tid; k < 2 * PAD; k += THREADS) {
+    if (k < PAD) {
+      if (k < NXF) {
+        const int i = k / (T + 2), j = k % (T + 2) - 1;
+        constexpr int comp[7] = {0, 1, 2, 3, 4, 6, 7};
+        P q[7][4];
+#pragma unroll
+        for (int c = 0; c < 7; ++c)
+#pragma unroll
+          for (int s = 0; s < 4; ++s) q[c][s] = sh.prim[comp[c]][cell_index(i - 2 + s, j)];
+        P fl[7];
+        face_solve<P, LIM>(q, a.limiter, sh.bxf[cell_index(i, j)], gam, inv_gm1, fl);
+#pragma unroll
+        for (int r = 0; r < 7; ++r) sh.fx[r][k] = fl[r];
+      }
+    } else {
+      const int m = k - PAD;
+      if (m < NYF) {
+        const int i = m / (T + 1) - 1, j = m % (T + 1);
+        constexpr int comp[7] = {0, 2, 1, 3, 4, 5, 7};
+        P q[7][4];
+#pragma unroll
+        for (int c = 0; c < 7; ++c)
+#pragma unroll
+          for (int s = 0; s < 4; ++s) q[c][s] = sh.prim[comp[c]][cell_index(i, j - 2 + s)];
+        P fl[7];
+        face_solve<P, LIM>(q, a.limiter, sh.byf[cell_index(i, j)], gam, inv_gm1, fl);
+#pragma unroll
+        for (int r = 0; r < 7; ++r) sh.fy[r][m] = fl[r];
+      }
+    }
+  }
+  __syncthreads();
+
+  // 3. the CT at node (i, j) and the update of cell (i, j)
+  const double dt = *a.dt;
+  const double dtdx = dt / a.dx, dtdy = dt / a.dy;
+  const bool xlo = a.walls & 1, xhi = a.walls & 2, ylo = a.walls & 4, yhi = a.walls & 8;
+  auto node = [&](int i, int j) {
+    const int f = i0 + i, g = j0 + j;
+    double e = ct_node(double(sh.fx[5][xface_index(i, j - 1)]), double(sh.fx[5][xface_index(i, j)]),
+                       double(sh.fy[5][yface_index(i - 1, j)]), double(sh.fy[5][yface_index(i, j)]),
+                       double(sh.fx[0][xface_index(i, j - 1)]), double(sh.fx[0][xface_index(i, j)]),
+                       double(sh.fy[0][yface_index(i - 1, j)]), double(sh.fy[0][yface_index(i, j)]),
+                       sh.ecc[i * E + j], sh.ecc[(i + 1) * E + j], sh.ecc[i * E + j + 1],
+                       sh.ecc[(i + 1) * E + j + 1]);
+    if ((xlo && f == 2) || (xhi && f == a.nx + 2) || (ylo && g == 2) || (yhi && g == a.ny + 2))
+      e = 0.0;
+    const size_t o = 6 * PL + (size_t)f * YN + g;
+    double v = a.in[o] - dt * e;
+    if (a.heun) v = (v + a.out[o]) * 0.5;
+    a.out[o] = v;
+  };
+  const int ci = tid / T, cj = tid % T;
+  const bool own_x = ci < txv || (ci == txv && last_x), own_y = cj < tyv || (cj == tyv && last_y);
+  if (own_x && own_y) node(ci, cj);
+  if (ci < txv && cj < tyv) {
+    const int f = i0 + ci, g = j0 + cj;
+    constexpr int XR[6] = {0, 1, 2, 3, 4, 6};  // x-sweep rows of the 6 fields (mhd.py:134)
+    constexpr int YR[6] = {0, 2, 1, 3, 4, 6};
+    const bool wxl = (xlo && f == 2), wxh = (xhi && f + 1 == a.nx + 2);
+    const bool wyl = ylo && g == 2, wyh = yhi && g + 1 == a.ny + 2;
+    const size_t c = (size_t)f * Y + g;
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+      const int xr = XR[r], yr = YR[r];
+      const bool xz = xr != 1 && xr != 5, yz = yr != 1 && yr != 5;
+      const P flo = (wxl && xz) ? P(0) : sh.fx[xr][xface_index(ci, cj)];
+      const P fhi = (wxh && xz) ? P(0) : sh.fx[xr][xface_index(ci + 1, cj)];
+      const double dfx = double(fhi) - double(flo);
+      const double hi = (wyh && yz) ? 0.0 : double(sh.fy[yr][yface_index(ci, cj + 1)]);
+      const double lo = (wyl && yz) ? 0.0 : double(sh.fy[yr][yface_index(ci, cj)]);
+      const double dyy = dtdy * (hi - lo);
+      const size_t o = r * PL + c;
+      double v = a.in[o] - (dtdx * dfx + dyy);
+      if (a.heun) v = (v + a.out[o]) * 0.5;
+      a.out[o] = v;
+    }
+  }
+  // the nodes on the tile's high faces (i == T or j == T) where they are the patch's
+  if ((last_x && txv == T) || (last_y && tyv == T)) {
+    for (int k = tid; k < 2 * T + 1; k += THREADS) {
+      const int i = k <= T ? T : k - T - 1, j = k <= T ? k : T;
+      const bool ox = i < txv || (i == txv && last_x), oy = j < tyv || (j == tyv && last_y);
+      if (ox && oy) node(i, j);
+    }
+  }
+}
+
+}  // namespace tiles2d
+
+// One workgroup per item (patch, tile along y, tile along x: WorkItem's patch, tile, seg).
+// Workgroup w runs on XCD w % 8; each XCD takes a contiguous run of the item list, so the
+// tiles of a patch that share halo cells share an L2.
+template <typename P, int LIM>
+__global__ void __launch_bounds__(tiles2d::THREADS)
+k_stage2d_tiles(Stage2Args base, const PatchDesc* __restrict__ patches,
+                const WorkItem* __restrict__ items, int nitems, double* __restrict__ out_base) {
+  __shared__ tiles2d::Shared<P> sh;
+  const int n = gridDim.x, xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
+  const int per = n >> 3, rem = n & 7;
+  const int it = xcd * per + min(xcd, rem) + k;
+  if (it >= nitems) return;
+  const WorkItem w = items[it];
+  const PatchDesc pd = patches[w.patch];
+  Stage2Args a = base;
+  a.in = base.in + pd.off;
+  a.out = out_base + pd.off;
+  a.nx = pd.nx;
+  a.ny = pd.ny;
+  a.X = pd.nx + 4;
+  a.Y = pd.ny + 4;
+  a.dx = pd.dx;
+  a.dy = pd.dy;
+  a.walls = pd.walls;
+  tiles2d::tile_body<P, LIM>(a, w.seg, w.tile, sh);
+}
+
+template <typename P, int LIM>
+inline int launch_stage2d_tiles(const Stage2Args& a, const void* patches, const void* items,
+                                int nitems, double* out, hipStream_t s) {
+  hipLaunchKernelGGL((k_stage2d_tiles<P, LIM>), dim3((unsigned)nitems), dim3(tiles2d::THREADS), 0,
+                     s, a, (const PatchDesc*)patches, (const WorkItem*)items, nitems, out);
+  return wb_last_error();
+}
+
+}  // namespace wb

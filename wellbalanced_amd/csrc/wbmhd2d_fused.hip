@@ -1,6 +1,7 @@
 // The fused 2D stage in FP64 (the parity path, -ffp-contract=off): the same bits
 // as the split 2D kernels of wbmhd2d.hip.
 #include "stage2d.hpp"
+#include "stage2d_tiles.hpp"
 
 using namespace wb;
 
@@ -87,6 +88,20 @@ extern "C" int wb_amr2d_stage(const double* in, double* out, const void* patches
               : launch_batched<62, false>(a, patches, items, nitems, counter, out, grid, s);
   return ni ? launch_batched<126, true>(a, patches, items, nitems, counter, out, grid, s)
             : launch_batched<126, false>(a, patches, items, nitems, counter, out, grid, s);
+}
+
+// Every 16 x 16 tile of every patch in one launch, a workgroup per tile (ideal MHD).
+extern "C" int wb_amr2d_stage_tiles(const double* in, double* out, const void* patches,
+                                    const void* items, int nitems, double b0x, double b0y,
+                                    double gamma, int limiter, const double* dt, int heun,
+                                    int32_t* bad, void* stream) {
+  Stage2Args a;
+  const int rc = wb_fill_stage_args2d(&a, in, out, 2, 2, 1.0, 1.0, b0x, b0y, gamma, 0.0, 0.0,
+                                      0.0, limiter, dt, heun, 0, bad, 1);
+  if (rc) return rc;
+  if (nitems <= 0) return 0;
+  if (patches == nullptr || items == nullptr) return WB_EINVAL;
+  return launch_stage2d_tiles<double, -1>(a, patches, items, nitems, out, (hipStream_t)stream);
 }
 
 extern "C" int wb_amr2d_reflux(const double* src, double* dst, const void* patches,

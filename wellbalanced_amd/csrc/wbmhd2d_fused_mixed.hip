@@ -2,6 +2,7 @@
 // primitive conversion, the non-ideal face terms, the CT average, A_z and the update
 // stay FP64.
 #include "stage2d.hpp"
+#include "stage2d_tiles.hpp"
 
 using namespace wb;
 
@@ -71,6 +72,23 @@ extern "C" int wb_amr2d_stage_mixed(const double* in, double* out, const void* p
                 : launch_batched<62, L, false>(a, patches, items, nitems, counter, out, grid, s);
     return ni ? launch_batched<126, L, true>(a, patches, items, nitems, counter, out, grid, s)
               : launch_batched<126, L, false>(a, patches, items, nitems, counter, out, grid, s);
+  });
+}
+
+extern "C" int wb_amr2d_stage_tiles_mixed(const double* in, double* out, const void* patches,
+                                          const void* items, int nitems, double b0x, double b0y,
+                                          double gamma, int limiter, const double* dt, int heun,
+                                          int32_t* bad, void* stream) {
+  Stage2Args a;
+  const int rc = wb_fill_stage_args2d(&a, in, out, 2, 2, 1.0, 1.0, b0x, b0y, gamma, 0.0, 0.0,
+                                      0.0, limiter, dt, heun, 0, bad, 1);
+  if (rc) return rc;
+  if (nitems <= 0) return 0;
+  if (patches == nullptr || items == nullptr) return WB_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  return with_limiter(limiter, [&](auto lim) {
+    constexpr int L = decltype(lim)::value;
+    return launch_stage2d_tiles<float, L>(a, patches, items, nitems, out, s);
   });
 }
 
