@@ -385,6 +385,17 @@ int wb_amr2d_stage_mixed(const double* in, double* out, const void* patches, con
                          double eta, double kpar, double kperp, int limiter, const double* dt,
                          int heun, int32_t* bad, int nxs, int tile, int workgroups, void* stream);
 
+/* The uniform grid's fused stage with a workgroup per 16 x 16 tile and no march (ideal
+ * MHD; arguments as wb_mhd2d_stage without the diffusion coefficients and the launch
+ * shape): the same bits as wb_mhd2d_stage.  Replaces MHD2D._advance (mhd.py:889-927). */
+int wb_mhd2d_stage_tiles(const double* in, double* out, int64_t nx, int64_t ny, double dx,
+                         double dy, double b0x, double b0y, double gamma, int limiter,
+                         const double* dt, int heun, int wall_mask, int32_t* bad, void* stream);
+int wb_mhd2d_stage_tiles_mixed(const double* in, double* out, int64_t nx, int64_t ny, double dx,
+                               double dy, double b0x, double b0y, double gamma, int limiter,
+                               const double* dt, int heun, int wall_mask, int32_t* bad,
+                               void* stream);
+
 /* The same stage with a workgroup per 16 x 16 tile of a patch and no march: items are
  * int32 triples (patch, tile along y, tile along x), each tile inside its patch
  * (16 tile < ny, 16 x-tile < nx), one launch of nitems workgroups.  Ideal MHD only (the

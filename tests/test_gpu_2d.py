@@ -250,20 +250,58 @@ def test_fused_fp64_stage_is_bitwise_the_split_path(gpu, tile, nxs, limiter):
         np.testing.assert_array_equal(bb[2:-2, 2:-2], aa[2:-2, 2:-2])
 
 
+@pytest.mark.parametrize("limiter", ["minmod", "vanleer", "mc", "none"])
+def test_tiled_fp64_stage_is_bitwise_the_split_path(gpu, limiter):
+    """The tiled stage (a workgroup per 16 x 16 tile) on grids that are not multiples of
+    the tile: the same bits as the split kernels, walls included."""
+    for problem in (initial.orszag_tang(40, limiter=limiter),
+                    initial.harris_sheet(70, 36, length=3.2, noise=1e-2, seed=1, limiter=limiter)):
+        split = MHD2D.from_problem(problem)
+        tiled = MHD2D.from_problem(problem, kernels="fused")
+        tiled.kernel = "tiles"
+        for _ in range(4):
+            dt = split.stable_step()
+            split.step(dt)
+            tiled.step(dt)
+        a, aa = split.arrays()
+        b, bb = tiled.arrays()
+        np.testing.assert_array_equal(b[:, 2:-2, 2:-2], a[:, 2:-2, 2:-2])
+        np.testing.assert_array_equal(bb[2:-2, 2:-2], aa[2:-2, 2:-2])
+
+
+@pytest.mark.parametrize("limiter", ["minmod", "vanleer", "mc", "none"])
+def test_tiled_mixed_stage_is_bitwise_the_marching_one(gpu, limiter):
+    for problem in (initial.orszag_tang(40, limiter=limiter),
+                    initial.harris_sheet(70, 36, length=3.2, noise=1e-2, seed=1, limiter=limiter)):
+        march = MHD2D.from_problem(problem, kernels="fused", precision="mixed")
+        tiled = MHD2D.from_problem(problem, kernels="fused", precision="mixed")
+        tiled.kernel = "tiles"
+        for _ in range(4):
+            dt = march.stable_step()
+            march.step(dt)
+            tiled.step(dt)
+        a, aa = march.arrays()
+        b, bb = tiled.arrays()
+        np.testing.assert_array_equal(b, a)
+        np.testing.assert_array_equal(bb, aa)
+
+
 def test_fused_outflow_and_walls_match_the_oracle(gpu):
     h = AMRHierarchy((37, 29), ((0.0, 1.0), (0.0, 1.0)))
     problem = initial.Problem(h, {"limiter": "mc", "boundaries": {"x": "wall"}, "mean_field": (0.0, 0.1)}, dict(
         density=lambda x, y: 1.0 + 0.3 * np.exp(-((x - 0.4) ** 2 + (y - 0.6) ** 2) / 0.01),
         velocity=(0.3, -0.2, 0.1), pressure=1.0,
         vector_potential=lambda x, y: 0.05 * np.sin(3 * x) * np.cos(2 * y), bz=0.2))
-    model = MHD2D.from_problem(problem, kernels="fused")
-    model.nxs = 8
-    oracle = oracle2d(problem)
-    for _ in range(6):
-        dt = oracle.stable_step()
-        model.step(dt)
-        oracle.step(dt)
-    _compare(model, oracle)
+    for kernel in ("march", "tiles"):
+        model = MHD2D.from_problem(problem, kernels="fused")
+        model.nxs = 8
+        model.kernel = kernel
+        oracle = oracle2d(problem)
+        for _ in range(6):
+            dt = oracle.stable_step()
+            model.step(dt)
+            oracle.step(dt)
+        _compare(model, oracle)
 
 
 def test_mixed_2d_is_close_to_fp64(gpu):

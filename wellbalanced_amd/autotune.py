@@ -33,6 +33,14 @@ DEFAULTS_3D_MIXED = {"nxs": [8, 16, 32, 64], "waves": [2, 3]}
 DEFAULTS_AMR = {"tile": [30, 62, 126], "nxs": [4, 8, 16, 32], "workgroups": [512, 1024, 2048]}
 
 
+def _uniform2d_candidates(model) -> list[dict]:
+    """The tiled stage (ideal MHD only), then every marching launch shape."""
+    names = list(DEFAULTS_2D)
+    march = [dict(zip(names, v), kernel="march")
+             for v in itertools.product(*(DEFAULTS_2D[n] for n in names))]
+    return ([] if model._diffusive else [{"kernel": "tiles"}]) + march
+
+
 def _amr2d_candidates(model) -> list[dict]:
     """The tiled stage (ideal MHD only), then every marching launch shape."""
     names = list(DEFAULTS_AMR)
@@ -77,6 +85,8 @@ def tune(model, space: dict | list | None = None, steps: int = 3) -> dict:
             space = _amr2d_candidates(model)
         elif hasattr(model, "n"):
             space = DEFAULTS_3D if model.precision == "fp64" else DEFAULTS_3D_MIXED
+        elif getattr(model, "kernels", "fused") == "fused":
+            space = _uniform2d_candidates(model)
         else:
             space = DEFAULTS_2D
     if not space:

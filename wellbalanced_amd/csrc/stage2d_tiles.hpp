@@ -195,9 +195,10 @@ __device__ __forceinline__ void tile_body(const Stage2Args& a, int tx, int ty, S
 
 }  // namespace tiles2d
 
-// One workgroup per item (patch, tile along y, tile along x: WorkItem's patch, tile, seg).
-// Workgroup w runs on XCD w % 8; each XCD takes a contiguous run of the item list, so the
-// tiles of a patch that share halo cells share an L2.
+// One workgroup per item (patch, tile along y, tile along x: WorkItem's patch, tile, seg),
+// or, without a patch table (`patches` null), per tile of the one grid `base` describes
+// (tiles along y fastest).  Workgroup w runs on XCD w % 8; each XCD takes a contiguous
+// run of the items, so the tiles that share halo cells share an L2.
 template <typename P, int LIM>
 __global__ void __launch_bounds__(tiles2d::THREADS)
 k_stage2d_tiles(Stage2Args base, const PatchDesc* __restrict__ patches,
@@ -207,6 +208,11 @@ k_stage2d_tiles(Stage2Args base, const PatchDesc* __restrict__ patches,
   const int per = n >> 3, rem = n & 7;
   const int it = xcd * per + min(xcd, rem) + k;
   if (it >= nitems) return;
+  if (patches == nullptr) {
+    const int tiles_y = (base.ny + tiles2d::T - 1) / tiles2d::T;
+    tiles2d::tile_body<P, LIM>(base, it / tiles_y, it % tiles_y, sh);
+    return;
+  }
   const WorkItem w = items[it];
   const PatchDesc pd = patches[w.patch];
   Stage2Args a = base;
@@ -222,6 +228,7 @@ k_stage2d_tiles(Stage2Args base, const PatchDesc* __restrict__ patches,
   tiles2d::tile_body<P, LIM>(a, w.seg, w.tile, sh);
 }
 
+// patches == nullptr: every tile of the grid `a` (nitems is then the tile count)
 template <typename P, int LIM>
 inline int launch_stage2d_tiles(const Stage2Args& a, const void* patches, const void* items,
                                 int nitems, double* out, hipStream_t s) {

@@ -90,6 +90,20 @@ extern "C" int wb_amr2d_stage(const double* in, double* out, const void* patches
             : launch_batched<126, false>(a, patches, items, nitems, counter, out, grid, s);
 }
 
+// The uniform grid's stage with a workgroup per 16 x 16 tile (ideal MHD; the halo is
+// filled before the launch, as for wb_mhd2d_stage).
+extern "C" int wb_mhd2d_stage_tiles(const double* in, double* out, int64_t nx, int64_t ny,
+                                    double dx, double dy, double b0x, double b0y, double gamma,
+                                    int limiter, const double* dt, int heun, int wall_mask,
+                                    int32_t* bad, void* stream) {
+  Stage2Args a;
+  const int rc = wb_fill_stage_args2d(&a, in, out, nx, ny, dx, dy, b0x, b0y, gamma, 0.0, 0.0,
+                                      0.0, limiter, dt, heun, wall_mask, bad, 1);
+  if (rc) return rc;
+  const int tiles = (int)(((nx + 15) / 16) * ((ny + 15) / 16));
+  return launch_stage2d_tiles<double, -1>(a, nullptr, nullptr, tiles, out, (hipStream_t)stream);
+}
+
 // Every 16 x 16 tile of every patch in one launch, a workgroup per tile (ideal MHD).
 extern "C" int wb_amr2d_stage_tiles(const double* in, double* out, const void* patches,
                                     const void* items, int nitems, double b0x, double b0y,

@@ -75,6 +75,22 @@ extern "C" int wb_amr2d_stage_mixed(const double* in, double* out, const void* p
   });
 }
 
+extern "C" int wb_mhd2d_stage_tiles_mixed(const double* in, double* out, int64_t nx, int64_t ny,
+                                          double dx, double dy, double b0x, double b0y,
+                                          double gamma, int limiter, const double* dt, int heun,
+                                          int wall_mask, int32_t* bad, void* stream) {
+  Stage2Args a;
+  const int rc = wb_fill_stage_args2d(&a, in, out, nx, ny, dx, dy, b0x, b0y, gamma, 0.0, 0.0,
+                                      0.0, limiter, dt, heun, wall_mask, bad, 1);
+  if (rc) return rc;
+  const int tiles = (int)(((nx + 15) / 16) * ((ny + 15) / 16));
+  hipStream_t s = (hipStream_t)stream;
+  return with_limiter(limiter, [&](auto lim) {
+    constexpr int L = decltype(lim)::value;
+    return launch_stage2d_tiles<float, L>(a, nullptr, nullptr, tiles, out, s);
+  });
+}
+
 extern "C" int wb_amr2d_stage_tiles_mixed(const double* in, double* out, const void* patches,
                                           const void* items, int nitems, double b0x, double b0y,
                                           double gamma, int limiter, const double* dt, int heun,

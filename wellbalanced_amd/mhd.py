@@ -125,6 +125,10 @@ class MHD2D:
         self.kernels = kernels
         self.nxs = 32
         self.tile = 126
+        # the fused stage's kernel: "march" (strips of columns marched along x, the launch
+        # shape `tile` x `nxs`) or "tiles" (a workgroup per 16 x 16 tile; ideal MHD, a
+        # model with diffusion marches whatever this says); the results are the same
+        self.kernel = "march"
         self.device = _device(device)
         self.patch: Patch = hierarchy.patches[0]
         self.nx, self.ny = self.patch.shape
@@ -333,7 +337,14 @@ class MHD2D:
         pmask = int(self.periodic[0]) | (int(self.periodic[1]) << 1)
         wmask = int(self.walls[0]) | (int(self.walls[1]) << 1)
         _lib.call("wb_mhd2d_fill_halos", _lib.ptr(src), self.nx, self.ny, pmask, wmask, stream)
-        name = "wb_mhd2d_stage" if self.precision == "fp64" else "wb_mhd2d_stage_mixed"
+        suffix = "" if self.precision == "fp64" else "_mixed"
+        if self.kernel == "tiles" and not self._diffusive:
+            _lib.call("wb_mhd2d_stage_tiles" + suffix, _lib.ptr(src), _lib.ptr(dst), self.nx,
+                      self.ny, self.dx, self.dy, *self.mean_field, self.gamma,
+                      LIMITER_CODES.index(self.limiter), _lib.ptr(self._dt), int(heun), wmask,
+                      self._bad.data_ptr() + 4 * slot, stream)
+            return
+        name = "wb_mhd2d_stage" + suffix
         _lib.call(name, _lib.ptr(src), _lib.ptr(dst), self.nx, self.ny, self.dx, self.dy,
                   *self.mean_field, self.gamma, self.resistivity, *self.conduction,
                   LIMITER_CODES.index(self.limiter),
