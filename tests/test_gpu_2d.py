@@ -239,7 +239,7 @@ def test_fused_fp64_stage_is_bitwise_the_split_path(gpu, tile, nxs, limiter):
                     initial.harris_sheet(70, 36, length=3.2, noise=1e-2, seed=1, limiter=limiter)):
         split = MHD2D.from_problem(problem)
         fused = MHD2D.from_problem(problem, kernels="fused")
-        fused.tile, fused.nxs = tile, nxs
+        fused.kernel, fused.tile, fused.nxs = "march", tile, nxs
         for _ in range(4):
             dt = split.stable_step()
             split.step(dt)
@@ -274,6 +274,7 @@ def test_tiled_mixed_stage_is_bitwise_the_marching_one(gpu, limiter):
     for problem in (initial.orszag_tang(40, limiter=limiter),
                     initial.harris_sheet(70, 36, length=3.2, noise=1e-2, seed=1, limiter=limiter)):
         march = MHD2D.from_problem(problem, kernels="fused", precision="mixed")
+        march.kernel = "march"
         tiled = MHD2D.from_problem(problem, kernels="fused", precision="mixed")
         tiled.kernel = "tiles"
         for _ in range(4):

@@ -125,10 +125,10 @@ class MHD2D:
         self.kernels = kernels
         self.nxs = 32
         self.tile = 126
-        # the fused stage's kernel: "march" (strips of columns marched along x, the launch
-        # shape `tile` x `nxs`) or "tiles" (a workgroup per 16 x 16 tile; ideal MHD, a
-        # model with diffusion marches whatever this says); the results are the same
-        self.kernel = "march"
+        # the fused stage's kernel: "tiles" (a workgroup per 16 x 16 tile; ideal MHD, a
+        # model with diffusion marches whatever this says) or "march" (strips of columns
+        # marched along x, the launch shape `tile` x `nxs`); the results are the same
+        self.kernel = "tiles"
         self.device = _device(device)
         self.patch: Patch = hierarchy.patches[0]
         self.nx, self.ny = self.patch.shape
