@@ -404,11 +404,28 @@ int wb_mhd2d_stage_tiles_mixed(const double* in, double* out, int64_t nx, int64_
  * on every patch of a level (mhd.py:889-927). */
 int wb_amr2d_stage_tiles(const double* in, double* out, const void* patches, const void* items,
                          int nitems, double b0x, double b0y, double gamma, int limiter,
-                         const double* dt, int heun, int32_t* bad, void* stream);
+                         const double* dt, int heun, int32_t* bad, const int32_t* rec_first,
+                         const int32_t* rec_lists, double* rec, void* stream);
 int wb_amr2d_stage_tiles_mixed(const double* in, double* out, const void* patches,
                                const void* items, int nitems, double b0x, double b0y,
                                double gamma, int limiter, const double* dt, int heun,
-                               int32_t* bad, void* stream);
+                               int32_t* bad, const int32_t* rec_first, const int32_t* rec_lists,
+                               double* rec, void* stream);
+/* The fluxes the refluxing reads, recorded by the tiled stage (rec_first NULL: none):
+ * rec_first[item] is the start in rec_lists of the item's list (count, then count
+ * int32 pairs (kind | index << 1, slot); kind 0: x-face (x-node i, cell j) of the tile,
+ * index i * 18 + j + 1, local i in [0, 16], j in [-1, 16]; kind 1: y-face (cell i, y-node
+ * j), index (i + 1) * 17 + j), and rec[6 slot + f] receives the face's flux of cell field
+ * f (rho, m_x, m_y, m_z, E, B_z) in FP64.  Each face is listed by one item.  Then
+ * wb_amr2d_reflux_recorded[_mixed] is wb_amr2d_reflux on the recorded values: rows of 8
+ * int32 (coarse patch, cell a, cell b, axis, side, slot of the coarse face, slots of the
+ * two fine faces), the same groups, register and result; no face flux is recomputed. */
+int wb_amr2d_reflux_recorded(double* dst, const void* patches, const int32_t* rows,
+                             const int32_t* groups, int ngroups, const double* rec,
+                             const double* dt, int record, double* reg, void* stream);
+int wb_amr2d_reflux_recorded_mixed(double* dst, const void* patches, const int32_t* rows,
+                                   const int32_t* groups, int ngroups, const double* rec,
+                                   const double* dt, int record, double* reg, void* stream);
 
 /* The flux register and refluxing (FluxRegister.update with scale 0.5 dt and
  * FluxRegister.reflux, mhd.py:911-919,989-990).  For each group (first row, count) of

@@ -229,6 +229,7 @@ class AMRMHD2D:
                  for tx in range(-(-lay.nx // TILE)) for ty in range(-(-lay.ny // TILE))]
         self._tiles = torch.from_numpy(np.array(tiles, _ITEM).view(np.uint8).copy()).to(self.device)
         self.ntiles = len(tiles)
+        self._record_tables(rows_all)
         self._patches = torch.from_numpy(patches.view(np.uint8).copy()).to(self.device)
         self._items = torch.from_numpy(items.view(np.uint8).copy()).to(self.device)
         self._spatch = self._dev(spatch, torch.float64)
@@ -269,6 +270,45 @@ class AMRMHD2D:
         self.nrestrict = len(restrict)
         self._work = torch.zeros(_lib.call("wb_amr2d_totals_workspace"), dtype=torch.float64,
                                  device=self.device)
+
+    def _record_tables(self, rows: np.ndarray) -> None:
+        """The tiled stage records the fluxes of the refluxing's faces and the refluxing
+        reads them: a slot per distinct face (patch, axis, face a, face b) of the reflux
+        rows, each face listed by the tile that holds it (the tile of the cell on its
+        high side, the patch's last tile for a face on its high boundary)."""
+        keys = np.concatenate([rows[:, [0, 3, 5, 6]], rows[:, [7, 3, 8, 9]],
+                               rows[:, [10, 3, 11, 12]]]) if len(rows) else np.zeros((0, 4), np.int64)
+        faces, inverse = np.unique(keys, axis=0, return_inverse=True)
+        inverse = inverse.reshape(-1)
+        n = len(rows)
+        rows8 = np.zeros((n, 8), np.int32)
+        if n:
+            rows8[:, :5] = rows[:, :5]
+            rows8[:, 5], rows8[:, 6], rows8[:, 7] = inverse[:n], inverse[n:2 * n], inverse[2 * n:]
+        first_item = np.cumsum([0] + [-(-lay.nx // TILE) * -(-lay.ny // TILE)
+                                      for lay in self.layouts])
+        lists = [[] for _ in range(self.ntiles)]
+        for slot, (p, axis, fa, fb) in enumerate(faces):
+            lay = self.layouts[p]
+            ntx, nty = -(-lay.nx // TILE), -(-lay.ny // TILE)
+            tx, ty = (fa - HALO) // TILE, (fb - HALO) // TILE
+            if axis == 0:
+                tx = min(tx, ntx - 1)
+            else:
+                ty = min(ty, nty - 1)
+            i, j = fa - (HALO + TILE * tx), fb - (HALO + TILE * ty)
+            index = i * (TILE + 2) + j + 1 if axis == 0 else (i + 1) * (TILE + 1) + j
+            lists[first_item[p] + tx * nty + ty].append((int(axis) | (int(index) << 1), slot))
+        first, flat = np.zeros(self.ntiles, np.int32), []
+        for item, entries in enumerate(lists):
+            first[item] = len(flat)
+            flat.append(len(entries))
+            for code, slot in entries:
+                flat.extend((code, slot))
+        self._rows8 = self._dev(rows8.reshape(-1), torch.int32)
+        self._rec_first = self._dev(first, torch.int32)
+        self._rec_lists = self._dev(np.array(flat, np.int32), torch.int32)
+        self._rec = torch.zeros(6 * max(len(faces), 1), dtype=torch.float64, device=self.device)
 
     # -- state ---------------------------------------------------------------------------
     def set_state(self, density, velocity, pressure, *, vector_potential=0.0, bz=0.0) -> None:
@@ -359,23 +399,31 @@ class AMRMHD2D:
         suffix = "" if self.precision == "fp64" else "_mixed"
         code = LIMITER_CODES.index(self.limiter)
         diff = (self.resistivity, *self.conduction)
+        rows, groups, n = self._reflux
         if self.kernel == "tiles" and not self._diffusive:
+            # the stage records the fluxes of the refluxing's faces, which reads them
+            rec = ((_lib.ptr(self._rec_first), _lib.ptr(self._rec_lists), _lib.ptr(self._rec))
+                   if n else (None, None, None))
             _lib.call("wb_amr2d_stage_tiles" + suffix, _lib.ptr(src), _lib.ptr(dst),
                       _lib.ptr(self._patches), _lib.ptr(self._tiles), self.ntiles,
                       *self.mean_field, self.gamma, code, _lib.ptr(self._dt), int(heun),
-                      self._bad.data_ptr() + 4 * slot, stream)
+                      self._bad.data_ptr() + 4 * slot, *rec, stream)
+            if n:  # stage 1 records into the register, stage 2 refluxes (mhd.py:911-919,989)
+                _lib.call("wb_amr2d_reflux_recorded" + suffix, _lib.ptr(dst),
+                          _lib.ptr(self._patches), _lib.ptr(self._rows8), _lib.ptr(groups), n,
+                          _lib.ptr(self._rec), _lib.ptr(self._dt), int(not heun),
+                          _lib.ptr(self._reg), stream)
         else:
             _lib.call("wb_amr2d_stage" + suffix, _lib.ptr(src), _lib.ptr(dst),
                       _lib.ptr(self._patches), _lib.ptr(self._items), self.nitems,
                       _lib.ptr(self._counter), *self.mean_field, self.gamma, *diff, code,
                       _lib.ptr(self._dt), int(heun), self._bad.data_ptr() + 4 * slot,
                       int(self.nxs), int(self.tile), int(self.workgroups), stream)
-        rows, groups, n = self._reflux
-        if n:  # stage 1 records into the register, stage 2 refluxes (mhd.py:911-919,989)
-            _lib.call("wb_amr2d_reflux" + suffix, _lib.ptr(src), _lib.ptr(dst),
-                      _lib.ptr(self._patches), _lib.ptr(rows), _lib.ptr(groups), n,
-                      *self.mean_field, self.gamma, *diff, code, _lib.ptr(self._dt),
-                      int(not heun), _lib.ptr(self._reg), stream)
+            if n:  # stage 1 records into the register, stage 2 refluxes (mhd.py:911-919,989)
+                _lib.call("wb_amr2d_reflux" + suffix, _lib.ptr(src), _lib.ptr(dst),
+                          _lib.ptr(self._patches), _lib.ptr(rows), _lib.ptr(groups), n,
+                          *self.mean_field, self.gamma, *diff, code, _lib.ptr(self._dt),
+                          int(not heun), _lib.ptr(self._reg), stream)
         for L in range(self.nlev - 2, -1, -1):
             if heun:  # the synchronisation after the Heun average (mhd.py:989-991)
                 self._coarsen(dst, L, stream)

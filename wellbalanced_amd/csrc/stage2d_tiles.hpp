@@ -53,9 +53,14 @@ __device__ __forceinline__ int xface_index(int i, int j) { return i * (T + 2) + 
 __device__ __forceinline__ int yface_index(int i, int j) { return (i + 1) * (T + 1) + j; }
 
 // Tile (tx, ty) of the patch whose stage arguments are `a` (in, out, nx, ny, X, Y, dx,
-// dy, walls set for that patch).
+// dy, walls set for that patch).  `recs` (count, then count pairs (code, slot); null:
+// none): faces whose fluxes the refluxing reads, code = kind | index << 1 (kind 0: the
+// x-face fx index, 1: the y-face fy index); each is stored as the 6 fluxes of the cell
+// fields (x-rows 0, 1, 2, 3, 4, 6 or y-rows 0, 2, 1, 3, 4, 6), widened to FP64, at
+// rec + 6 slot.
 template <typename P, int LIM>
-__device__ __forceinline__ void tile_body(const Stage2Args& a, int tx, int ty, Shared<P>& sh) {
+__device__ __forceinline__ void tile_body(const Stage2Args& a, int tx, int ty, Shared<P>& sh,
+                                          const int* recs = nullptr, double* rec = nullptr) {
   const int tid = threadIdx.x;
   const int X = a.X, Y = a.Y, YN = Y + 1;
   const int i0 = 2 + tx * T, j0 = 2 + ty * T;  // haloed index of the tile's first cell
@@ -138,6 +143,25 @@ __device__ __forceinline__ void tile_body(const Stage2Args& a, int tx, int ty, S
   }
   __syncthreads();
 
+  // the fluxes the refluxing reads (the values the update below applies)
+  if (recs != nullptr) {
+    const int count = recs[0];
+    for (int k = tid; k < count; k += THREADS) {
+      const int code = recs[1 + 2 * k], slot = recs[2 + 2 * k];
+      const int idx = code >> 1;
+      double* o = rec + 6 * (size_t)slot;
+      if ((code & 1) == 0) {
+        constexpr int XR[6] = {0, 1, 2, 3, 4, 6};
+#pragma unroll
+        for (int r = 0; r < 6; ++r) o[r] = double(sh.fx[XR[r]][idx]);
+      } else {
+        constexpr int YR[6] = {0, 2, 1, 3, 4, 6};
+#pragma unroll
+        for (int r = 0; r < 6; ++r) o[r] = double(sh.fy[YR[r]][idx]);
+      }
+    }
+  }
+
   // 3. the CT at node (i, j) and the update of cell (i, j)
   const double dt = *a.dt;
   const double dtdx = dt / a.dx, dtdy = dt / a.dy;
@@ -199,10 +223,13 @@ __device__ __forceinline__ void tile_body(const Stage2Args& a, int tx, int ty, S
 // or, without a patch table (`patches` null), per tile of the one grid `base` describes
 // (tiles along y fastest).  Workgroup w runs on XCD w % 8; each XCD takes a contiguous
 // run of the items, so the tiles that share halo cells share an L2.
+// rec_first (null: no record): per item the start of its record list in rec_lists.
 template <typename P, int LIM>
 __global__ void __launch_bounds__(tiles2d::THREADS)
 k_stage2d_tiles(Stage2Args base, const PatchDesc* __restrict__ patches,
-                const WorkItem* __restrict__ items, int nitems, double* __restrict__ out_base) {
+                const WorkItem* __restrict__ items, int nitems, double* __restrict__ out_base,
+                const int* __restrict__ rec_first, const int* __restrict__ rec_lists,
+                double* __restrict__ rec) {
   __shared__ tiles2d::Shared<P> sh;
   const int n = gridDim.x, xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
   const int per = n >> 3, rem = n & 7;
@@ -225,15 +252,74 @@ k_stage2d_tiles(Stage2Args base, const PatchDesc* __restrict__ patches,
   a.dx = pd.dx;
   a.dy = pd.dy;
   a.walls = pd.walls;
-  tiles2d::tile_body<P, LIM>(a, w.seg, w.tile, sh);
+  const int* recs = rec_first ? rec_lists + rec_first[it] : nullptr;
+  tiles2d::tile_body<P, LIM>(a, w.seg, w.tile, sh, recs, rec);
 }
 
 // patches == nullptr: every tile of the grid `a` (nitems is then the tile count)
 template <typename P, int LIM>
 inline int launch_stage2d_tiles(const Stage2Args& a, const void* patches, const void* items,
-                                int nitems, double* out, hipStream_t s) {
+                                int nitems, double* out, hipStream_t s,
+                                const int32_t* rec_first = nullptr,
+                                const int32_t* rec_lists = nullptr, double* rec = nullptr) {
   hipLaunchKernelGGL((k_stage2d_tiles<P, LIM>), dim3((unsigned)nitems), dim3(tiles2d::THREADS), 0,
-                     s, a, (const PatchDesc*)patches, (const WorkItem*)items, nitems, out);
+                     s, a, (const PatchDesc*)patches, (const WorkItem*)items, nitems, out,
+                     (const int*)rec_first, (const int*)rec_lists, rec);
+  return wb_last_error();
+}
+
+// The flux register and refluxing from the fluxes the tiled stage recorded (the
+// arithmetic of k_reflux2d on the same values): per group (coarse cell) rows of 8 int32
+// (coarse patch, cell a, cell b, axis, side, slot of the coarse face, slots of the two
+// fine faces); record != 0: reg = 0.5 d; otherwise cell += reg + 0.5 d.  Internal linkage:
+// each translation unit compiles it with its own contraction rule, as it does the stage.
+static __global__ void __launch_bounds__(64)
+k_reflux2d_recorded(double* __restrict__ dst, const PatchDesc* __restrict__ patches,
+                    const int* __restrict__ rows, const int* __restrict__ groups, int ngroups,
+                    const double* __restrict__ rec, const double* __restrict__ dtp, int record,
+                    double* __restrict__ reg) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ngroups) return;
+  const double dt = *dtp;
+  const int first = groups[2 * g], count = groups[2 * g + 1];
+  double delta[6] = {0, 0, 0, 0, 0, 0};
+  int q = 0, ca = 0, cb = 0;
+  for (int k = 0; k < count; ++k) {
+    const int* r = rows + 8 * (size_t)(first + k);
+    q = r[0];
+    ca = r[1];
+    cb = r[2];
+    const int axis = r[3], side = r[4];
+    const PatchDesc cq = patches[q];
+    const double dtd = axis == 0 ? dt / cq.dx : dt / cq.dy;
+    const double* fc = rec + 6 * (size_t)r[5];
+    const double* f0 = rec + 6 * (size_t)r[6];
+    const double* f1 = rec + 6 * (size_t)r[7];
+    for (int f = 0; f < 6; ++f) {
+      const double mean = 0.5 * (f0[f] + f1[f]);
+      const double d = side == 0 ? dtd * (fc[f] - mean) : dtd * (mean - fc[f]);
+      delta[f] = delta[f] + d;
+    }
+  }
+  double* kept = reg + 6 * (size_t)g;
+  if (record) {
+    for (int f = 0; f < 6; ++f) kept[f] = 0.5 * delta[f];
+    return;
+  }
+  const PatchDesc cq = patches[q];
+  const size_t PL = (size_t)(cq.nx + 4) * (cq.ny + 4);
+  const size_t c = cq.off + (size_t)ca * (cq.ny + 4) + cb;
+  for (int f = 0; f < 6; ++f) dst[c + f * PL] = dst[c + f * PL] + (kept[f] + 0.5 * delta[f]);
+}
+
+inline int launch_reflux2d_recorded(double* dst, const void* patches, const int32_t* rows,
+                                    const int32_t* groups, int ngroups, const double* rec,
+                                    const double* dt, int record, double* reg, hipStream_t s) {
+  if (ngroups <= 0) return 0;
+  if (rows == nullptr || groups == nullptr || rec == nullptr || reg == nullptr) return WB_EINVAL;
+  hipLaunchKernelGGL(k_reflux2d_recorded, dim3((unsigned)((ngroups + 63) / 64)), dim3(64), 0, s,
+                     dst, (const PatchDesc*)patches, (const int*)rows, (const int*)groups,
+                     ngroups, rec, dt, record, reg);
   return wb_last_error();
 }
 

@@ -108,14 +108,25 @@ extern "C" int wb_mhd2d_stage_tiles(const double* in, double* out, int64_t nx, i
 extern "C" int wb_amr2d_stage_tiles(const double* in, double* out, const void* patches,
                                     const void* items, int nitems, double b0x, double b0y,
                                     double gamma, int limiter, const double* dt, int heun,
-                                    int32_t* bad, void* stream) {
+                                    int32_t* bad, const int32_t* rec_first,
+                                    const int32_t* rec_lists, double* rec, void* stream) {
   Stage2Args a;
   const int rc = wb_fill_stage_args2d(&a, in, out, 2, 2, 1.0, 1.0, b0x, b0y, gamma, 0.0, 0.0,
                                       0.0, limiter, dt, heun, 0, bad, 1);
   if (rc) return rc;
   if (nitems <= 0) return 0;
   if (patches == nullptr || items == nullptr) return WB_EINVAL;
-  return launch_stage2d_tiles<double, -1>(a, patches, items, nitems, out, (hipStream_t)stream);
+  if (rec_first != nullptr && (rec_lists == nullptr || rec == nullptr)) return WB_EINVAL;
+  return launch_stage2d_tiles<double, -1>(a, patches, items, nitems, out, (hipStream_t)stream,
+                                          rec_first, rec_lists, rec);
+}
+
+extern "C" int wb_amr2d_reflux_recorded(double* dst, const void* patches, const int32_t* rows,
+                                        const int32_t* groups, int ngroups, const double* rec,
+                                        const double* dt, int record, double* reg,
+                                        void* stream) {
+  return launch_reflux2d_recorded(dst, patches, rows, groups, ngroups, rec, dt, record, reg,
+                                  (hipStream_t)stream);
 }
 
 extern "C" int wb_amr2d_reflux(const double* src, double* dst, const void* patches,

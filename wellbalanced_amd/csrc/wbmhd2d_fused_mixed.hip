@@ -94,18 +94,29 @@ extern "C" int wb_mhd2d_stage_tiles_mixed(const double* in, double* out, int64_t
 extern "C" int wb_amr2d_stage_tiles_mixed(const double* in, double* out, const void* patches,
                                           const void* items, int nitems, double b0x, double b0y,
                                           double gamma, int limiter, const double* dt, int heun,
-                                          int32_t* bad, void* stream) {
+                                          int32_t* bad, const int32_t* rec_first,
+                                          const int32_t* rec_lists, double* rec, void* stream) {
   Stage2Args a;
   const int rc = wb_fill_stage_args2d(&a, in, out, 2, 2, 1.0, 1.0, b0x, b0y, gamma, 0.0, 0.0,
                                       0.0, limiter, dt, heun, 0, bad, 1);
   if (rc) return rc;
   if (nitems <= 0) return 0;
   if (patches == nullptr || items == nullptr) return WB_EINVAL;
+  if (rec_first != nullptr && (rec_lists == nullptr || rec == nullptr)) return WB_EINVAL;
   hipStream_t s = (hipStream_t)stream;
   return with_limiter(limiter, [&](auto lim) {
     constexpr int L = decltype(lim)::value;
-    return launch_stage2d_tiles<float, L>(a, patches, items, nitems, out, s);
+    return launch_stage2d_tiles<float, L>(a, patches, items, nitems, out, s, rec_first, rec_lists,
+                                          rec);
   });
+}
+
+extern "C" int wb_amr2d_reflux_recorded_mixed(double* dst, const void* patches,
+                                              const int32_t* rows, const int32_t* groups,
+                                              int ngroups, const double* rec, const double* dt,
+                                              int record, double* reg, void* stream) {
+  return launch_reflux2d_recorded(dst, patches, rows, groups, ngroups, rec, dt, record, reg,
+                                  (hipStream_t)stream);
 }
 
 extern "C" int wb_amr2d_reflux_mixed(const double* src, double* dst, const void* patches,
