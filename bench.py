@@ -228,6 +228,8 @@ def run_turbulence3d(args):
         "linf_density_vs_fp64": linf,
         "error_run": accuracy.get("note"),
         "hbm_fraction_algorithmic": value * 5 * STATE_BYTES_3D / (HBM_PEAK_GBS * 1e9),
+        # the compulsory lower bound: each step reads U^n and writes U^n+1 once (2S)
+        "hbm_fraction_compulsory": value * 2 * STATE_BYTES_3D / (HBM_PEAK_GBS * 1e9),
         "fp64_stage_ms": stage64,
         "roofline": {
             "bound": "hbm",
@@ -330,6 +332,7 @@ def _line(metric_value, rate64, steps, args, sec, workload, cells, precision, l1
         "l1_density_vs_fp64": l1,
         "linf_density_vs_fp64": linf,
         "hbm_fraction_algorithmic": metric_value * 5 * state_bytes / (HBM_PEAK_GBS * 1e9),
+        "hbm_fraction_compulsory": metric_value * 2 * state_bytes / (HBM_PEAK_GBS * 1e9),
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
