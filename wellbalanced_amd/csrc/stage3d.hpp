@@ -402,14 +402,16 @@ __device__ __forceinline__ void stage3d_body(const StageArgs& a, int ty_, int tz
     if (ly >= 0 && ly < tyv && lz >= 0 && lz < tzv && pl >= ix0 && pl < ix0 + nxv && c.bad())
       atomicAdd(a.bad, 1);
   };
-  // the outermost ring of primitive columns: one column per thread (tid < NH)
-  constexpr int NH = T::NP - T::NF;
+  // the outermost ring of primitive columns but its four corners, which no face stencil
+  // reads (the y-faces read columns 1..PZ-2, the z-faces rows 1..PY-2): one column per
+  // thread (tid < NH; 64 for the 12 x 16 tile, one wave)
+  constexpr int NH = 2 * (T::PZ - 2) + 2 * (T::PY - 2);
   static_assert(NH <= T::THREADS, "one ring column per thread");
   const bool ring = tid < NH;
   int rpy, rpz;
-  if (tid < T::PZ) { rpy = 0; rpz = tid; }
-  else if (tid < 2 * T::PZ) { rpy = T::PY - 1; rpz = tid - T::PZ; }
-  else { const int e = tid - 2 * T::PZ; rpy = 1 + e / 2; rpz = (e & 1) ? T::PZ - 1 : 0; }
+  if (tid < T::PZ - 2) { rpy = 0; rpz = tid + 1; }
+  else if (tid < 2 * (T::PZ - 2)) { rpy = T::PY - 1; rpz = tid - (T::PZ - 2) + 1; }
+  else { const int e = tid - 2 * (T::PZ - 2); rpy = 1 + e / 2; rpz = (e & 1) ? T::PZ - 1 : 0; }
   const int hj = jy0 + rpy - 2, hk = kz0 + rpz - 2;  // the ring column's haloed indices
   const int ryr = DIRECT ? rel(hj, g.ny) : 0, rzr = DIRECT ? rel(hk, g.nz) : 0;
   const int rj = read_index(hj, g.ny, g.Y, 2), rk = read_index(hk, g.nz, g.Z, 4);
