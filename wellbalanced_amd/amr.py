@@ -278,36 +278,42 @@ class AMRMHD2D:
         high side, the patch's last tile for a face on its high boundary)."""
         keys = np.concatenate([rows[:, [0, 3, 5, 6]], rows[:, [7, 3, 8, 9]],
                                rows[:, [10, 3, 11, 12]]]) if len(rows) else np.zeros((0, 4), np.int64)
-        faces, inverse = np.unique(keys, axis=0, return_inverse=True)
+        # one integer per face, ordered as the rows (patch, axis, face a, face b) are
+        packed = ((keys[:, 0] << 41) | (keys[:, 1] << 40) | (keys[:, 2] << 20) | keys[:, 3])
+        unique, inverse = np.unique(packed, return_inverse=True)
         inverse = inverse.reshape(-1)
+        faces = np.stack([unique >> 41, (unique >> 40) & 1, (unique >> 20) & 0xFFFFF,
+                          unique & 0xFFFFF], axis=1)
         n = len(rows)
         rows8 = np.zeros((n, 8), np.int32)
         if n:
             rows8[:, :5] = rows[:, :5]
             rows8[:, 5], rows8[:, 6], rows8[:, 7] = inverse[:n], inverse[n:2 * n], inverse[2 * n:]
-        first_item = np.cumsum([0] + [-(-lay.nx // TILE) * -(-lay.ny // TILE)
-                                      for lay in self.layouts])
-        lists = [[] for _ in range(self.ntiles)]
-        for slot, (p, axis, fa, fb) in enumerate(faces):
-            lay = self.layouts[p]
-            ntx, nty = -(-lay.nx // TILE), -(-lay.ny // TILE)
-            tx, ty = (fa - HALO) // TILE, (fb - HALO) // TILE
-            if axis == 0:
-                tx = min(tx, ntx - 1)
-            else:
-                ty = min(ty, nty - 1)
-            i, j = fa - (HALO + TILE * tx), fb - (HALO + TILE * ty)
-            index = i * (TILE + 2) + j + 1 if axis == 0 else (i + 1) * (TILE + 1) + j
-            lists[first_item[p] + tx * nty + ty].append((int(axis) | (int(index) << 1), slot))
-        first, flat = np.zeros(self.ntiles, np.int32), []
-        for item, entries in enumerate(lists):
-            first[item] = len(flat)
-            flat.append(len(entries))
-            for code, slot in entries:
-                flat.extend((code, slot))
+        nx = np.array([lay.nx for lay in self.layouts], np.int64)
+        ny = np.array([lay.ny for lay in self.layouts], np.int64)
+        ntx, nty = -(-nx // TILE), -(-ny // TILE)
+        first_item = np.concatenate([[0], np.cumsum(ntx * nty)])
+        p, axis, fa, fb = (faces[:, k] for k in range(4))
+        tx, ty = (fa - HALO) // TILE, (fb - HALO) // TILE
+        tx = np.where(axis == 0, np.minimum(tx, ntx[p] - 1), tx)
+        ty = np.where(axis == 1, np.minimum(ty, nty[p] - 1), ty)
+        i, j = fa - (HALO + TILE * tx), fb - (HALO + TILE * ty)
+        index = np.where(axis == 0, i * (TILE + 2) + j + 1, (i + 1) * (TILE + 1) + j)
+        item = first_item[p] + tx * nty[p] + ty
+        # per item: its count, then (code, slot) pairs
+        counts = np.bincount(item, minlength=self.ntiles)
+        first = np.concatenate([[0], np.cumsum(1 + 2 * counts)[:-1]])
+        flat = np.zeros(int(np.sum(1 + 2 * counts)), np.int32)
+        flat[first] = counts
+        order = np.argsort(item, kind="stable")
+        starts = np.concatenate([[0], np.cumsum(counts)[:-1]])
+        rank = np.arange(len(order)) - starts[item[order]]
+        at = first[item[order]] + 1 + 2 * rank
+        flat[at] = (axis[order] | (index[order] << 1)).astype(np.int32)
+        flat[at + 1] = order.astype(np.int32)
         self._rows8 = self._dev(rows8.reshape(-1), torch.int32)
         self._rec_first = self._dev(first, torch.int32)
-        self._rec_lists = self._dev(np.array(flat, np.int32), torch.int32)
+        self._rec_lists = self._dev(flat, torch.int32)
         self._rec = torch.zeros(6 * max(len(faces), 1), dtype=torch.float64, device=self.device)
 
     # -- state ---------------------------------------------------------------------------
