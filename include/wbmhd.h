@@ -385,32 +385,36 @@ int wb_amr2d_stage_mixed(const double* in, double* out, const void* patches, con
                          double eta, double kpar, double kperp, int limiter, const double* dt,
                          int heun, int32_t* bad, int nxs, int tile, int workgroups, void* stream);
 
-/* The uniform grid's fused stage with a workgroup per 16 x 16 tile and no march (ideal
- * MHD; arguments as wb_mhd2d_stage without the diffusion coefficients and the launch
- * shape): the same bits as wb_mhd2d_stage.  Replaces MHD2D._advance (mhd.py:889-927). */
+/* The uniform grid's fused stage with a workgroup per 16 x 16 tile and no march
+ * (arguments as wb_mhd2d_stage without the launch shape; the resistive and conduction
+ * terms as there): the same bits as wb_mhd2d_stage.  Replaces MHD2D._advance
+ * (mhd.py:889-927). */
 int wb_mhd2d_stage_tiles(const double* in, double* out, int64_t nx, int64_t ny, double dx,
-                         double dy, double b0x, double b0y, double gamma, int limiter,
-                         const double* dt, int heun, int wall_mask, int32_t* bad, void* stream);
+                         double dy, double b0x, double b0y, double gamma, double eta,
+                         double kpar, double kperp, int limiter, const double* dt, int heun,
+                         int wall_mask, int32_t* bad, void* stream);
 int wb_mhd2d_stage_tiles_mixed(const double* in, double* out, int64_t nx, int64_t ny, double dx,
-                               double dy, double b0x, double b0y, double gamma, int limiter,
-                               const double* dt, int heun, int wall_mask, int32_t* bad,
-                               void* stream);
+                               double dy, double b0x, double b0y, double gamma, double eta,
+                               double kpar, double kperp, int limiter, const double* dt,
+                               int heun, int wall_mask, int32_t* bad, void* stream);
 
 /* The same stage with a workgroup per 16 x 16 tile of a patch and no march: items are
  * int32 triples (patch, tile along y, tile along x), each tile inside its patch
- * (16 tile < ny, 16 x-tile < nx), one launch of nitems workgroups.  Ideal MHD only (the
- * non-ideal terms stay with wb_amr2d_stage); the same bits as wb_amr2d_stage (the
- * same expressions on the same values).  Replaces, like wb_amr2d_stage, MHD2D._advance
+ * (16 tile < ny, 16 x-tile < nx), one launch of nitems workgroups, the resistive and
+ * conduction terms as in wb_amr2d_stage; the same bits as wb_amr2d_stage (the same
+ * expressions on the same values).  Replaces, like wb_amr2d_stage, MHD2D._advance
  * on every patch of a level (mhd.py:889-927). */
 int wb_amr2d_stage_tiles(const double* in, double* out, const void* patches, const void* items,
-                         int nitems, double b0x, double b0y, double gamma, int limiter,
-                         const double* dt, int heun, int32_t* bad, const int32_t* rec_first,
-                         const int32_t* rec_lists, double* rec, void* stream);
+                         int nitems, double b0x, double b0y, double gamma, double eta,
+                         double kpar, double kperp, int limiter, const double* dt, int heun,
+                         int32_t* bad, const int32_t* rec_first, const int32_t* rec_lists,
+                         double* rec, void* stream);
 int wb_amr2d_stage_tiles_mixed(const double* in, double* out, const void* patches,
                                const void* items, int nitems, double b0x, double b0y,
-                               double gamma, int limiter, const double* dt, int heun,
-                               int32_t* bad, const int32_t* rec_first, const int32_t* rec_lists,
-                               double* rec, void* stream);
+                               double gamma, double eta, double kpar, double kperp, int limiter,
+                               const double* dt, int heun, int32_t* bad,
+                               const int32_t* rec_first, const int32_t* rec_lists, double* rec,
+                               void* stream);
 /* The fluxes the refluxing reads, recorded by the tiled stage (rec_first NULL: none):
  * rec_first[item] is the start in rec_lists of the item's list (count, then count
  * int32 pairs (kind | index << 1, slot); kind 0: x-face (x-node i, cell j) of the tile,

@@ -286,15 +286,15 @@ def _bitwise(model, oracle):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("kernels,tile,nxs", [("split", 126, 32), ("fused", 126, 32),
-                                              ("fused", 62, 5), ("fused", 254, 64),
-                                              ("fused", 30, 4)])
-def test_nonideal_steps_are_bitwise_the_oracle(gpu, kernels, tile, nxs):
+@pytest.mark.parametrize("kernels,kernel,tile,nxs", [
+    ("split", "march", 126, 32), ("fused", "march", 126, 32), ("fused", "march", 62, 5),
+    ("fused", "march", 254, 64), ("fused", "march", 30, 4), ("fused", "tiles", 126, 32)])
+def test_nonideal_steps_are_bitwise_the_oracle(gpu, kernels, kernel, tile, nxs):
     from wellbalanced_amd import MHD2D
 
     for problem in _nonideal_problems():
         model = MHD2D.from_problem(problem, kernels=kernels)
-        model.tile, model.nxs = tile, nxs
+        model.kernel, model.tile, model.nxs = kernel, tile, nxs
         oracle = oracle2d(problem)
         for _ in range(4):
             dt = oracle.stable_step()
@@ -358,8 +358,9 @@ def test_gpu_the_step_keeps_within_the_diffusion_limit(gpu):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("kernel", ["tiles", "march"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_gpu_hierarchy_with_diffusion_matches_the_oracle_and_conserves(gpu, precision):
+def test_gpu_hierarchy_with_diffusion_matches_the_oracle_and_conserves(gpu, precision, kernel):
     """Three levels with resistivity and conduction: the refluxing carries the diffusive
     fluxes, so the leaf totals are kept (to 1e-13 of the largest, mixed as FP64: the
     refluxing's recomputed FP32 face flux has the stage's bits);
@@ -372,7 +373,7 @@ def test_gpu_hierarchy_with_diffusion_matches_the_oracle_and_conserves(gpu, prec
     h = _refined(3)
     diff = dict(resistivity=2e-3, conduction=(0.03, 0.003))
     oracle = amr_oracle(h, _smooth_state(), limiter="minmod", **diff)
-    model = AMRMHD2D(h, limiter="minmod", precision=precision, **diff)
+    model = AMRMHD2D(h, limiter="minmod", precision=precision, kernel=kernel, **diff)
     model.set_state(**_smooth_state())
     before = model.leaf_totals()
     for _ in range(3):
@@ -563,3 +564,25 @@ def test_gpu_3d_resistivity_damps_a_force_free_field_at_the_diffusion_rate(gpu):
     ratio, want = _beltrami_ratio(before, float(after["magnetic_pressure"].mean()), 0.01, 0.5)
     assert ratio == pytest.approx(want, rel=2e-3)
     assert np.abs(after["velocity_x"]).max() < 1e-12
+
+
+@pytest.mark.gpu
+def test_gpu_tiled_nonideal_mixed_stage_is_bitwise_the_marching_one(gpu):
+    """With resistivity and conduction the tiled mixed stage keeps the marching one's
+    bits (both add the FP64 face terms to the same FP32 fluxes), walls included."""
+    from wellbalanced_amd import MHD2D
+
+    for problem in _nonideal_problems():
+        models = []
+        for kernel in ("march", "tiles"):
+            m = MHD2D.from_problem(problem, kernels="fused", precision="mixed")
+            m.kernel = kernel
+            models.append(m)
+        for _ in range(4):
+            dt = models[0].stable_step()
+            for m in models:
+                m.step(dt)
+        a, aa = models[0].arrays()
+        b, bb = models[1].arrays()
+        np.testing.assert_array_equal(b, a)
+        np.testing.assert_array_equal(bb, aa)

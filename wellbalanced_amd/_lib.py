@@ -91,10 +91,10 @@ SIGNATURES: dict[str, tuple[type, list]] = {
     "wb_amr2d_rate": (I, [P, P, P, I, D, D, D, D, D, P, P, P]),
     "wb_amr2d_stage": (I, [P, P, P, P, I, P] + [D] * 6 + [I, P, I, P, I, I, I, P]),
     "wb_amr2d_stage_mixed": (I, [P, P, P, P, I, P] + [D] * 6 + [I, P, I, P, I, I, I, P]),
-    "wb_mhd2d_stage_tiles": (I, [P, P, I64, I64] + [D] * 5 + [I, P, I, I, P, P]),
-    "wb_mhd2d_stage_tiles_mixed": (I, [P, P, I64, I64] + [D] * 5 + [I, P, I, I, P, P]),
-    "wb_amr2d_stage_tiles": (I, [P, P, P, P, I, D, D, D, I, P, I, P, P, P, P, P]),
-    "wb_amr2d_stage_tiles_mixed": (I, [P, P, P, P, I, D, D, D, I, P, I, P, P, P, P, P]),
+    "wb_mhd2d_stage_tiles": (I, [P, P, I64, I64] + [D] * 8 + [I, P, I, I, P, P]),
+    "wb_mhd2d_stage_tiles_mixed": (I, [P, P, I64, I64] + [D] * 8 + [I, P, I, I, P, P]),
+    "wb_amr2d_stage_tiles": (I, [P, P, P, P, I] + [D] * 6 + [I, P, I, P, P, P, P, P]),
+    "wb_amr2d_stage_tiles_mixed": (I, [P, P, P, P, I] + [D] * 6 + [I, P, I, P, P, P, P, P]),
     "wb_amr2d_reflux_recorded": (I, [P, P, P, P, I, P, P, I, P, P]),
     "wb_amr2d_reflux_recorded_mixed": (I, [P, P, P, P, I, P, P, I, P, P]),
     "wb_amr2d_reflux": (I, [P, P, P, P, P, I] + [D] * 6 + [I, P, I, P, P]),

@@ -12,10 +12,9 @@ ping-ponged as in `MHD2D`); a stage is
    non-periodic axis the outflow or wall ghosts of the level's patches on the domain
    faces (``mhd.py:748-777``), which the next level's interpolation may read;
 2. ONE launch over every tile of every patch, which updates every patch cell, covered
-   coarse cells included: a workgroup per 16 x 16 tile (`k_stage2d_tiles`, ideal MHD,
-   the default) or the persistent marching stage (`k_stage2d_batched`: workgroups
-   pulling (patch, tile, segment) items from a counter; with diffusion, or as
-   ``kernel="march"``);
+   coarse cells included: a workgroup per 16 x 16 tile (`k_stage2d_tiles`, the default)
+   or the persistent marching stage (`k_stage2d_batched`: workgroups pulling (patch,
+   tile, segment) items from a counter; ``kernel="march"``);
 3. the flux register of the coarse cells next to finer patches (one launch for every
    level pair): stage 1 records half its correction, stage 2 (whose output is the Heun
    average) adds the recorded half and its own;
@@ -135,8 +134,7 @@ class AMRMHD2D:
     def set_tiling(self, tile: int | None = None, nxs: int | None = None,
                    workgroups: int | None = None, kernel: str | None = None) -> None:
         """Change the launch shape (no effect on the results) and rebuild the work list:
-        ``kernel`` "tiles" (a workgroup per 16 x 16 tile of a patch; ideal MHD, a model
-        with diffusion runs the marching stage whatever this says) or "march" (the
+        ``kernel`` "tiles" (a workgroup per 16 x 16 tile of a patch) or "march" (the
         persistent marching stage with ``tile``, ``nxs`` and ``workgroups``)."""
         if kernel is not None:
             if kernel not in KERNELS:
@@ -406,13 +404,13 @@ class AMRMHD2D:
         code = LIMITER_CODES.index(self.limiter)
         diff = (self.resistivity, *self.conduction)
         rows, groups, n = self._reflux
-        if self.kernel == "tiles" and not self._diffusive:
+        if self.kernel == "tiles":
             # the stage records the fluxes of the refluxing's faces, which reads them
             rec = ((_lib.ptr(self._rec_first), _lib.ptr(self._rec_lists), _lib.ptr(self._rec))
                    if n else (None, None, None))
             _lib.call("wb_amr2d_stage_tiles" + suffix, _lib.ptr(src), _lib.ptr(dst),
                       _lib.ptr(self._patches), _lib.ptr(self._tiles), self.ntiles,
-                      *self.mean_field, self.gamma, code, _lib.ptr(self._dt), int(heun),
+                      *self.mean_field, self.gamma, *diff, code, _lib.ptr(self._dt), int(heun),
                       self._bad.data_ptr() + 4 * slot, *rec, stream)
             if n:  # stage 1 records into the register, stage 2 refluxes (mhd.py:911-919,989)
                 _lib.call("wb_amr2d_reflux_recorded" + suffix, _lib.ptr(dst),

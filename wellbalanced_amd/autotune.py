@@ -33,20 +33,11 @@ DEFAULTS_3D_MIXED = {"nxs": [8, 16, 32, 64], "waves": [2, 3]}
 DEFAULTS_AMR = {"tile": [30, 62, 126], "nxs": [4, 8, 16, 32], "workgroups": [512, 1024, 2048]}
 
 
-def _uniform2d_candidates(model) -> list[dict]:
-    """The tiled stage (ideal MHD only), then every marching launch shape."""
-    names = list(DEFAULTS_2D)
-    march = [dict(zip(names, v), kernel="march")
-             for v in itertools.product(*(DEFAULTS_2D[n] for n in names))]
-    return ([] if model._diffusive else [{"kernel": "tiles"}]) + march
-
-
-def _amr2d_candidates(model) -> list[dict]:
-    """The tiled stage (ideal MHD only), then every marching launch shape."""
-    names = list(DEFAULTS_AMR)
-    march = [dict(zip(names, v), kernel="march")
-             for v in itertools.product(*(DEFAULTS_AMR[n] for n in names))]
-    return ([] if model._diffusive else [{"kernel": "tiles"}]) + march
+def _candidates(defaults: dict) -> list[dict]:
+    """The tiled stage, then every marching launch shape."""
+    names = list(defaults)
+    return [{"kernel": "tiles"}] + [dict(zip(names, v), kernel="march")
+                                    for v in itertools.product(*(defaults[n] for n in names))]
 DEFAULTS_AMR3D_MIXED = {"waves": [2, 3], "direct": [True, False]}
 DEFAULTS_AMR3D = {"direct": [True, False]}
 
@@ -82,11 +73,11 @@ def tune(model, space: dict | list | None = None, steps: int = 3) -> dict:
         if hasattr(model, "nblocks"):
             space = DEFAULTS_AMR3D_MIXED if model.precision == "mixed" else DEFAULTS_AMR3D
         elif hasattr(model, "layouts"):
-            space = _amr2d_candidates(model)
+            space = _candidates(DEFAULTS_AMR)
         elif hasattr(model, "n"):
             space = DEFAULTS_3D if model.precision == "fp64" else DEFAULTS_3D_MIXED
         elif getattr(model, "kernels", "fused") == "fused":
-            space = _uniform2d_candidates(model)
+            space = _candidates(DEFAULTS_2D)
         else:
             space = DEFAULTS_2D
     if not space:

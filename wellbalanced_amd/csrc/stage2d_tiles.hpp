@@ -16,8 +16,10 @@
 //      (`ct_contact_pyccel.py:35-61`), A_z -= dt E_z, the conservative update of the cell,
 //      the Heun average; the nodes on a patch's high faces in a short extra pass.
 // Every flux, E_z and update is the marching stage's expression on the same values, so
-// both give the same bits (the FP64 instance is the parity path).  Ideal MHD only: the
-// non-ideal face terms stay with the marching stage.
+// both give the same bits (the FP64 instance is the parity path).  NI adds the resistive
+// and heat-conduction face terms (FP64) as the marching stage does: the first pass also
+// keeps T = p / rho and the cell field of the loaded cells and J_z at the 19 x 19 nodes
+// around the tile in LDS.
 #pragma once
 
 #include "mhd_device.hpp"
@@ -36,9 +38,10 @@ constexpr int NXF = (T + 1) * (T + 2);  // x-faces: x-node [0, T] x cell [-1, T]
 constexpr int NYF = (T + 2) * (T + 1);  // y-faces: cell [-1, T] x y-node [0, T]
 constexpr int PAD = 320;            // face tasks of one kind, a whole number of waves
 constexpr int THREADS = T * T;
+constexpr int N = T + 3;            // NI: nodes with a J_z per side ([-1, T + 1])
 static_assert(NXF <= PAD && NYF <= PAD && PAD % 64 == 0, "face tasks fit whole waves");
 
-template <typename P>
+template <typename P, bool NI = false>
 struct Shared {
   P prim[8][NC];    // rho, u, v, w, p, B_x, B_y, B_z of cell (i, j), local [-2, T+1]
   P bxf[NC];        // B_n of the cell's low x-face
@@ -46,11 +49,15 @@ struct Shared {
   double ecc[E * E];  // cell E_z, local [-1, T]
   P fx[7][NXF];     // x-face (x-node i, cell j): index i * (T + 2) + j + 1
   P fy[7][NYF];     // y-face (cell i, y-node j): index (i + 1) * (T + 1) + j
+  // NI: T = p / rho, B_x, B_y, B_z of the loaded cells; J_z at node (i, j), local [-1, T+1]
+  double cell[NI ? 4 : 1][NI ? NC : 1];
+  double jz[NI ? N * N : 1];
 };
 
 __device__ __forceinline__ int cell_index(int i, int j) { return (i + 2) * H + (j + 2); }
 __device__ __forceinline__ int xface_index(int i, int j) { return i * (T + 2) + j + 1; }
 __device__ __forceinline__ int yface_index(int i, int j) { return (i + 1) * (T + 1) + j; }
+__device__ __forceinline__ int node_index(int i, int j) { return (i + 1) * N + (j + 1); }
 
 // Tile (tx, ty) of the patch whose stage arguments are `a` (in, out, nx, ny, X, Y, dx,
 // dy, walls set for that patch).  `recs` (count, then count pairs (code, slot); null:
@@ -58,8 +65,8 @@ __device__ __forceinline__ int yface_index(int i, int j) { return (i + 1) * (T +
 // x-face fx index, 1: the y-face fy index); each is stored as the 6 fluxes of the cell
 // fields (x-rows 0, 1, 2, 3, 4, 6 or y-rows 0, 2, 1, 3, 4, 6), widened to FP64, at
 // rec + 6 slot.
-template <typename P, int LIM>
-__device__ __forceinline__ void tile_body(const Stage2Args& a, int tx, int ty, Shared<P>& sh,
+template <typename P, int LIM, bool NI = false>
+__device__ __forceinline__ void tile_body(const Stage2Args& a, int tx, int ty, Shared<P, NI>& sh,
                                           const int* recs = nullptr, double* rec = nullptr) {
   const int tid = threadIdx.x;
   const int X = a.X, Y = a.Y, YN = Y + 1;
@@ -98,11 +105,23 @@ __device__ __forceinline__ void tile_body(const Stage2Args& a, int tx, int ty, S
     sh.byf[k] = P(by0);
     if (li >= -1 && li <= T && lj >= -1 && lj <= T)
       sh.ecc[(li + 1) * E + (lj + 1)] = bx * v - by * u;
+    if constexpr (NI) {
+      sh.cell[0][k] = p / r;
+      sh.cell[1][k] = bx;
+      sh.cell[2][k] = by;
+      sh.cell[3][k] = b;
+    }
     if (li >= 0 && li < txv && lj >= 0 && lj < tyv) {
       const bool is_bad = !(r > 0.0 && p > 0.0) ||
                           !(finite(r) && finite(u) && finite(v) && finite(w) && finite(p) &&
                             finite(bx) && finite(by) && finite(b));
       if (is_bad) atomicAdd(a.bad, 1);
+    }
+  }
+  if constexpr (NI) {  // J_z at the nodes around the tile, inside the haloed patch
+    for (int k = tid; k < N * N; k += THREADS) {
+      const int I = min(max(i0 + k / N - 1, 1), X - 1), J = min(max(j0 + k % N - 1, 1), Y - 1);
+      sh.jz[k] = node_current_az(az, YN, I, J, a.dx, a.dy, a.b0x, a.b0y);
     }
   }
   __syncthreads();
@@ -121,6 +140,24 @@ __device__ __forceinline__ void tile_body(const Stage2Args& a, int tx, int ty, S
           for (int s = 0; s < 4; ++s) q[c][s] = sh.prim[comp[c]][cell_index(i - 2 + s, j)];
         P fl[7];
         face_solve<P, LIM>(q, a.limiter, sh.bxf[cell_index(i, j)], gam, inv_gm1, fl);
+        if constexpr (NI) {  // mhd.py:868-871, cells L = (i - 1, j) and R = (i, j)
+          const int l = cell_index(i - 1, j), c = cell_index(i, j);
+          const int fi = min(i0 + i, X - 1), jj = min(max(j0 + j, 0), Y - 1);
+          const double bnd = (AZ(fi, jj + 1) - AZ(fi, jj)) / a.dy + a.b0x;
+          double e4 = double(fl[4]), b6 = double(fl[6]);
+          if (a.diff.conduction())
+            conductive_face(bnd, sh.cell[2][l], sh.cell[2][c], sh.cell[3][l], sh.cell[3][c],
+                            sh.cell[0][l], sh.cell[0][c],
+                            limited_gradient(sh.cell[0][l - 1], sh.cell[0][l], sh.cell[0][l + 1], a.dy),
+                            limited_gradient(sh.cell[0][c - 1], sh.cell[0][c], sh.cell[0][c + 1], a.dy),
+                            a.dx, a.diff, e4);
+          if (a.diff.eta != 0.0)
+            resistive_xface(sh.cell[3][l], sh.cell[3][c], sh.cell[2][l], sh.cell[2][c],
+                            sh.jz[node_index(i, j)], sh.jz[node_index(i, j + 1)], a.dx,
+                            a.diff.eta, e4, b6);
+          fl[4] = P(e4);
+          fl[6] = P(b6);
+        }
 #pragma unroll
         for (int r = 0; r < 7; ++r) sh.fx[r][k] = fl[r];
       }
@@ -136,6 +173,24 @@ __device__ __forceinline__ void tile_body(const Stage2Args& a, int tx, int ty, S
           for (int s = 0; s < 4; ++s) q[c][s] = sh.prim[comp[c]][cell_index(i, j - 2 + s)];
         P fl[7];
         face_solve<P, LIM>(q, a.limiter, sh.byf[cell_index(i, j)], gam, inv_gm1, fl);
+        if constexpr (NI) {  // cells L = (i, j - 1) and R = (i, j)
+          const int l = cell_index(i, j - 1), c = cell_index(i, j);
+          const int ii = min(max(i0 + i, 0), X - 1), jj = min(max(j0 + j, 0), Y - 1);
+          const double bnd = -(AZ(ii + 1, jj) - AZ(ii, jj)) / a.dx + a.b0y;
+          double e4 = double(fl[4]), b6 = double(fl[6]);
+          if (a.diff.conduction())
+            conductive_face(bnd, sh.cell[1][l], sh.cell[1][c], sh.cell[3][l], sh.cell[3][c],
+                            sh.cell[0][l], sh.cell[0][c],
+                            limited_gradient(sh.cell[0][l - H], sh.cell[0][l], sh.cell[0][l + H], a.dx),
+                            limited_gradient(sh.cell[0][c - H], sh.cell[0][c], sh.cell[0][c + H], a.dx),
+                            a.dy, a.diff, e4);
+          if (a.diff.eta != 0.0)
+            resistive_yface(sh.cell[3][l], sh.cell[3][c], sh.cell[1][l], sh.cell[1][c],
+                            sh.jz[node_index(i, j)], sh.jz[node_index(i + 1, j)], a.dy,
+                            a.diff.eta, e4, b6);
+          fl[4] = P(e4);
+          fl[6] = P(b6);
+        }
 #pragma unroll
         for (int r = 0; r < 7; ++r) sh.fy[r][m] = fl[r];
       }
@@ -174,6 +229,9 @@ __device__ __forceinline__ void tile_body(const Stage2Args& a, int tx, int ty, S
                        double(sh.fy[0][yface_index(i - 1, j)]), double(sh.fy[0][yface_index(i, j)]),
                        sh.ecc[i * E + j], sh.ecc[(i + 1) * E + j], sh.ecc[i * E + j + 1],
                        sh.ecc[(i + 1) * E + j + 1]);
+    if constexpr (NI) {
+      if (a.diff.eta != 0.0) e = e + a.diff.eta * sh.jz[node_index(i, j)];
+    }
     if ((xlo && f == 2) || (xhi && f == a.nx + 2) || (ylo && g == 2) || (yhi && g == a.ny + 2))
       e = 0.0;
     const size_t o = 6 * PL + (size_t)f * YN + g;
@@ -224,20 +282,20 @@ __device__ __forceinline__ void tile_body(const Stage2Args& a, int tx, int ty, S
 // (tiles along y fastest).  Workgroup w runs on XCD w % 8; each XCD takes a contiguous
 // run of the items, so the tiles that share halo cells share an L2.
 // rec_first (null: no record): per item the start of its record list in rec_lists.
-template <typename P, int LIM>
+template <typename P, int LIM, bool NI>
 __global__ void __launch_bounds__(tiles2d::THREADS)
 k_stage2d_tiles(Stage2Args base, const PatchDesc* __restrict__ patches,
                 const WorkItem* __restrict__ items, int nitems, double* __restrict__ out_base,
                 const int* __restrict__ rec_first, const int* __restrict__ rec_lists,
                 double* __restrict__ rec) {
-  __shared__ tiles2d::Shared<P> sh;
+  __shared__ tiles2d::Shared<P, NI> sh;
   const int n = gridDim.x, xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
   const int per = n >> 3, rem = n & 7;
   const int it = xcd * per + min(xcd, rem) + k;
   if (it >= nitems) return;
   if (patches == nullptr) {
     const int tiles_y = (base.ny + tiles2d::T - 1) / tiles2d::T;
-    tiles2d::tile_body<P, LIM>(base, it / tiles_y, it % tiles_y, sh);
+    tiles2d::tile_body<P, LIM, NI>(base, it / tiles_y, it % tiles_y, sh);
     return;
   }
   const WorkItem w = items[it];
@@ -253,18 +311,26 @@ k_stage2d_tiles(Stage2Args base, const PatchDesc* __restrict__ patches,
   a.dy = pd.dy;
   a.walls = pd.walls;
   const int* recs = rec_first ? rec_lists + rec_first[it] : nullptr;
-  tiles2d::tile_body<P, LIM>(a, w.seg, w.tile, sh, recs, rec);
+  tiles2d::tile_body<P, LIM, NI>(a, w.seg, w.tile, sh, recs, rec);
 }
 
-// patches == nullptr: every tile of the grid `a` (nitems is then the tile count)
+// patches == nullptr: every tile of the grid `a` (nitems is then the tile count); the
+// non-ideal instance where a.diff has a resistivity or a conductivity
 template <typename P, int LIM>
 inline int launch_stage2d_tiles(const Stage2Args& a, const void* patches, const void* items,
                                 int nitems, double* out, hipStream_t s,
                                 const int32_t* rec_first = nullptr,
                                 const int32_t* rec_lists = nullptr, double* rec = nullptr) {
-  hipLaunchKernelGGL((k_stage2d_tiles<P, LIM>), dim3((unsigned)nitems), dim3(tiles2d::THREADS), 0,
-                     s, a, (const PatchDesc*)patches, (const WorkItem*)items, nitems, out,
-                     (const int*)rec_first, (const int*)rec_lists, rec);
+  if (a.diff.conduction() || a.diff.eta != 0.0)
+    hipLaunchKernelGGL((k_stage2d_tiles<P, LIM, true>), dim3((unsigned)nitems),
+                       dim3(tiles2d::THREADS), 0, s, a, (const PatchDesc*)patches,
+                       (const WorkItem*)items, nitems, out, (const int*)rec_first,
+                       (const int*)rec_lists, rec);
+  else
+    hipLaunchKernelGGL((k_stage2d_tiles<P, LIM, false>), dim3((unsigned)nitems),
+                       dim3(tiles2d::THREADS), 0, s, a, (const PatchDesc*)patches,
+                       (const WorkItem*)items, nitems, out, (const int*)rec_first,
+                       (const int*)rec_lists, rec);
   return wb_last_error();
 }
 

@@ -94,11 +94,12 @@ extern "C" int wb_amr2d_stage(const double* in, double* out, const void* patches
 // filled before the launch, as for wb_mhd2d_stage).
 extern "C" int wb_mhd2d_stage_tiles(const double* in, double* out, int64_t nx, int64_t ny,
                                     double dx, double dy, double b0x, double b0y, double gamma,
-                                    int limiter, const double* dt, int heun, int wall_mask,
-                                    int32_t* bad, void* stream) {
+                                    double eta, double kpar, double kperp, int limiter,
+                                    const double* dt, int heun, int wall_mask, int32_t* bad,
+                                    void* stream) {
   Stage2Args a;
-  const int rc = wb_fill_stage_args2d(&a, in, out, nx, ny, dx, dy, b0x, b0y, gamma, 0.0, 0.0,
-                                      0.0, limiter, dt, heun, wall_mask, bad, 1);
+  const int rc = wb_fill_stage_args2d(&a, in, out, nx, ny, dx, dy, b0x, b0y, gamma, eta, kpar,
+                                      kperp, limiter, dt, heun, wall_mask, bad, 1);
   if (rc) return rc;
   const int tiles = (int)(((nx + 15) / 16) * ((ny + 15) / 16));
   return launch_stage2d_tiles<double, -1>(a, nullptr, nullptr, tiles, out, (hipStream_t)stream);
@@ -107,12 +108,13 @@ extern "C" int wb_mhd2d_stage_tiles(const double* in, double* out, int64_t nx, i
 // Every 16 x 16 tile of every patch in one launch, a workgroup per tile (ideal MHD).
 extern "C" int wb_amr2d_stage_tiles(const double* in, double* out, const void* patches,
                                     const void* items, int nitems, double b0x, double b0y,
-                                    double gamma, int limiter, const double* dt, int heun,
-                                    int32_t* bad, const int32_t* rec_first,
-                                    const int32_t* rec_lists, double* rec, void* stream) {
+                                    double gamma, double eta, double kpar, double kperp,
+                                    int limiter, const double* dt, int heun, int32_t* bad,
+                                    const int32_t* rec_first, const int32_t* rec_lists,
+                                    double* rec, void* stream) {
   Stage2Args a;
-  const int rc = wb_fill_stage_args2d(&a, in, out, 2, 2, 1.0, 1.0, b0x, b0y, gamma, 0.0, 0.0,
-                                      0.0, limiter, dt, heun, 0, bad, 1);
+  const int rc = wb_fill_stage_args2d(&a, in, out, 2, 2, 1.0, 1.0, b0x, b0y, gamma, eta, kpar,
+                                      kperp, limiter, dt, heun, 0, bad, 1);
   if (rc) return rc;
   if (nitems <= 0) return 0;
   if (patches == nullptr || items == nullptr) return WB_EINVAL;

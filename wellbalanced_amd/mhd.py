@@ -125,9 +125,9 @@ class MHD2D:
         self.kernels = kernels
         self.nxs = 32
         self.tile = 126
-        # the fused stage's kernel: "tiles" (a workgroup per 16 x 16 tile; ideal MHD, a
-        # model with diffusion marches whatever this says) or "march" (strips of columns
-        # marched along x, the launch shape `tile` x `nxs`); the results are the same
+        # the fused stage's kernel: "tiles" (a workgroup per 16 x 16 tile) or "march"
+        # (strips of columns marched along x, the launch shape `tile` x `nxs`); the
+        # results are the same
         self.kernel = "tiles"
         self.device = _device(device)
         self.patch: Patch = hierarchy.patches[0]
@@ -338,11 +338,11 @@ class MHD2D:
         wmask = int(self.walls[0]) | (int(self.walls[1]) << 1)
         _lib.call("wb_mhd2d_fill_halos", _lib.ptr(src), self.nx, self.ny, pmask, wmask, stream)
         suffix = "" if self.precision == "fp64" else "_mixed"
-        if self.kernel == "tiles" and not self._diffusive:
+        if self.kernel == "tiles":
             _lib.call("wb_mhd2d_stage_tiles" + suffix, _lib.ptr(src), _lib.ptr(dst), self.nx,
-                      self.ny, self.dx, self.dy, *self.mean_field, self.gamma,
-                      LIMITER_CODES.index(self.limiter), _lib.ptr(self._dt), int(heun), wmask,
-                      self._bad.data_ptr() + 4 * slot, stream)
+                      self.ny, self.dx, self.dy, *self.mean_field, self.gamma, self.resistivity,
+                      *self.conduction, LIMITER_CODES.index(self.limiter), _lib.ptr(self._dt),
+                      int(heun), wmask, self._bad.data_ptr() + 4 * slot, stream)
             return
         name = "wb_mhd2d_stage" + suffix
         _lib.call(name, _lib.ptr(src), _lib.ptr(dst), self.nx, self.ny, self.dx, self.dy,
