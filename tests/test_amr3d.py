@@ -297,6 +297,50 @@ def test_gpu_region_fill_is_the_restricted_shell_fill(gpu, kind):
     assert torch.equal(c, b)
 
 
+def _block_views(buf, t, k, X=20):
+    """Block k's 5 cell fields (X^3 each) and its 3 potential components in a packed
+    hierarchy buffer."""
+    off = int(t.blocks["off"][k])
+    cells = buf[off:off + 5 * X**3].reshape(5, X, X, X)
+    o, pots = off + 5 * X**3, []
+    for shape in [(X, X + 1, X + 1), (X + 1, X, X + 1), (X + 1, X + 1, X)]:
+        m = int(np.prod(shape))
+        pots.append(buf[o:o + m].reshape(shape))
+        o += m
+    return cells, pots
+
+
+@pytest.mark.gpu
+def test_gpu_mixed_block_stage_shapes_keep_the_bits(gpu):
+    """The mixed block stage's launch shapes (whole blocks at 2 waves per SIMD, half
+    blocks at 3; direct reads or filled ghosts) leave the same cells and potentials,
+    bit for bit, over steps at one step size."""
+    from wellbalanced_amd.amr3d import AMRMHD3D
+
+    h = _hierarchy(3)
+    results = {}
+    for waves, direct in [(3, True), (3, False), (2, True), (2, False)]:
+        model = AMRMHD3D(h, mean_field=B0, limiter="minmod", precision="mixed")
+        model.waves, model.direct = waves, direct
+        model.set_state(**_blast_state())
+        dt = 0.5 * model.stable_step()
+        for _ in range(3):
+            model.step(dt)
+        model.check()
+        results[waves, direct] = (model, model.buffer())
+    ref_model, ref = results[3, True]
+    t = ref_model.tables
+    for key, (model, got) in results.items():
+        for k in range(len(t.blocks)):
+            gc, gp = _block_views(got, t, k)
+            wc, wp = _block_views(ref, t, k)
+            np.testing.assert_array_equal(gc[:, 2:18, 2:18, 2:18], wc[:, 2:18, 2:18, 2:18],
+                                          err_msg=str(key))
+            for c in range(3):
+                sl = tuple(slice(2, 18 + (a != c)) for a in range(3))
+                np.testing.assert_array_equal(gp[c][sl], wp[c][sl], err_msg=str(key))
+
+
 @pytest.mark.gpu
 def test_gpu_three_level_mixed_conserves(gpu):
     from wellbalanced_amd.amr3d import AMRMHD3D
