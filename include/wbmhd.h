@@ -421,15 +421,18 @@ int wb_amr2d_stage_tiles_mixed(const double* in, double* out, const void* patche
  * index i * 18 + j + 1, local i in [0, 16], j in [-1, 16]; kind 1: y-face (cell i, y-node
  * j), index (i + 1) * 17 + j), and rec[6 slot + f] receives the face's flux of cell field
  * f (rho, m_x, m_y, m_z, E, B_z) in FP64.  Each face is listed by one item.  Then
- * wb_amr2d_reflux_recorded[_mixed] is wb_amr2d_reflux on the recorded values: rows of 8
- * int32 (coarse patch, cell a, cell b, axis, side, slot of the coarse face, slots of the
- * two fine faces), the same groups, register and result; no face flux is recomputed. */
-int wb_amr2d_reflux_recorded(double* dst, const void* patches, const int32_t* rows,
-                             const int32_t* groups, int ngroups, const double* rec,
-                             const double* dt, int record, double* reg, void* stream);
-int wb_amr2d_reflux_recorded_mixed(double* dst, const void* patches, const int32_t* rows,
-                                   const int32_t* groups, int ngroups, const double* rec,
-                                   const double* dt, int record, double* reg, void* stream);
+ * wb_amr2d_reflux_both[_mixed] is wb_amr2d_reflux's two calls (record after stage 1,
+ * reflux after stage 2) in one launch after the Heun average, on the recorded values:
+ * rec1 holds stage 1's fluxes, rec2 stage 2's; rows of 8 int32 (coarse patch, cell a,
+ * cell b, axis, side, slot of the coarse face, slots of the two fine faces) and the same
+ * groups; the dst cell gains 0.5 d1 + 0.5 d2, the same result; no face flux is
+ * recomputed and no register is kept. */
+int wb_amr2d_reflux_both(double* dst, const void* patches, const int32_t* rows,
+                         const int32_t* groups, int ngroups, const double* rec1,
+                         const double* rec2, const double* dt, void* stream);
+int wb_amr2d_reflux_both_mixed(double* dst, const void* patches, const int32_t* rows,
+                               const int32_t* groups, int ngroups, const double* rec1,
+                               const double* rec2, const double* dt, void* stream);
 
 /* The flux register and refluxing (FluxRegister.update with scale 0.5 dt and
  * FluxRegister.reflux, mhd.py:911-919,989-990).  For each group (first row, count) of
@@ -540,7 +543,7 @@ int wb_amr3d_rate_mixed(const double* buf, const void* blocks, int nblocks, doub
  * rec_slots, rec (both or neither): per block 6 int32 slots (face 2 axis + side, side
  * 0 the low face; -1 none) and the record they index: the stage writes the fluxes it
  * solves on those faces there, 256 face cells (u, v in the two other axes, u slower) x
- * the 5 conserved fields (double) per slot, for wb_amr3d_reflux. */
+ * the 5 conserved fields (double) per slot, for wb_amr3d_reflux_both. */
 int wb_amr3d_stage(const double* in, double* out, const void* blocks, const int32_t* ids,
                    int nids, int direct, const int32_t* lookup, const void* levels, double b0x,
                    double b0y, double b0z, double gamma, int limiter, const double* dt, int heun,
@@ -552,21 +555,20 @@ int wb_amr3d_stage_mixed(const double* in, double* out, const void* blocks, cons
                          const int32_t* rec_slots, double* rec, void* stream);
 
 /* The flux register and refluxing, every coarse-fine block face of every level in one
- * launch (FluxRegister.update with scale 0.5 dt and FluxRegister.reflux,
- * mhd3d.py:650-655,745-750): per listed coarse cell t, d = the sum over its faces of
- * dt/h (coarse flux - mean of the four fine fluxes), signed by the side, from the
- * fluxes the last stage recorded in `rec` (slots as for wb_amr3d_stage: the coarse
- * block's face toward the fine block and the fine block's face; both precisions);
- * record != 0 (after stage 1): reg[5 t + f] = 0.5 d; record == 0 (after stage 2 and its
- * Heun average): the cell of `dst` gains reg + 0.5 d.  reg: 5 doubles per listed cell,
- * caller-owned.  items: {int32 fine, coarse, ci, cj, ck, dir} per coarse-fine block
- * face (dir = 2 axis + side; side 0: the coarse cells at the fine block's low face, 1:
- * at its high face); cells: {int32 n, item[3], lane[3], pad} per coarse cell with such
- * faces, its n faces as (item, lane = u * 8 + v inside the item's 8 x 8 face) in
- * increasing dir. */
-int wb_amr3d_reflux(double* dst, const void* blocks, const void* items, const void* cells,
-                    int ncells, const int32_t* rec_slots, const double* rec, const double* dt,
-                    int record, double* reg, void* stream);
+ * launch after the Heun average (FluxRegister.update with scale 0.5 dt in both stages
+ * and FluxRegister.reflux, mhd3d.py:650-655,745-750): per listed coarse cell, d_s = the
+ * sum over its faces of dt/h (coarse flux - mean of the four fine fluxes), signed by the
+ * side, from the fluxes stage s recorded (rec1: stage 1, rec2: stage 2; slots as for
+ * wb_amr3d_stage: the coarse block's face toward the fine block and the fine block's
+ * face; both precisions); the cell of `dst` gains 0.5 d1 + 0.5 d2 (the register of stage 1
+ * added to stage 2's half).  items: {int32 fine, coarse, ci, cj, ck, dir} per
+ * coarse-fine block face (dir = 2 axis + side; side 0: the coarse cells at the fine
+ * block's low face, 1: at its high face); cells: {int32 n, item[3], lane[3], pad} per
+ * coarse cell with such faces, its n faces as (item, lane = u * 8 + v inside the item's
+ * 8 x 8 face) in increasing dir. */
+int wb_amr3d_reflux_both(double* dst, const void* blocks, const void* items, const void* cells,
+                         int ncells, const int32_t* rec_slots, const double* rec1,
+                         const double* rec2, const double* dt, void* stream);
 
 /* ---- 3D hierarchies of patches of any size (MHD3D on patchwork patches) -------- */
 /* Each patch holds a 3D patch state as above, patch after patch in one buffer; the

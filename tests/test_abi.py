@@ -113,11 +113,19 @@ def test_new_entry_points_reject_bad_arguments_before_any_launch():
     assert lib.wb_amr2d_coarsen(None, None, 0, None, 0, None) == 0
     assert lib.wb_amr3d_fill_regions(None, None, None, None, -1, None) == E
     assert lib.wb_amr3d_fill_regions(None, None, None, None, 0, None) == 0
-    reflux = lib.wb_amr3d_reflux
-    assert reflux(None, None, None, None, -1, None, None, None, 0, None, None) == E
-    assert reflux(None, None, None, None, 0, None, None, None, 0, None, None) == 0
-    # a register and the face record are needed when there is a cell to reflux
-    assert reflux(None, None, None, None, 1, None, None, None, 1, None, None) == E
+    for name in ("wb_amr2d_reflux_both", "wb_amr2d_reflux_both_mixed"):
+        both2 = getattr(lib, name)
+        rec2d = ctypes.c_double(0.0)
+        assert both2(None, None, None, None, -1, None, None, None, None) == E
+        assert both2(None, None, None, None, 0, None, None, None, None) == 0
+        # rows, groups and both stages' records are needed when there is a group
+        assert both2(None, None, None, None, 1, ctypes.byref(rec2d), None, None, None) == E
+    both = lib.wb_amr3d_reflux_both
+    rec0 = ctypes.c_double(0.0)
+    assert both(None, None, None, None, -1, None, None, None, None, None) == E
+    assert both(None, None, None, None, 0, None, None, None, None, None) == 0
+    # the slots and both stages' face records are needed when there is a cell to reflux
+    assert both(None, None, None, None, 1, None, None, ctypes.byref(rec0), None, None) == E
     # the stage takes the face record's slots and the record together or neither
     rec = ctypes.c_double(0.0)
     assert lib.wb_amr3d_stage(None, None, None, None, 1, 0, None, None, 0.0, 0.0, 0.0, 5 / 3,

@@ -70,7 +70,7 @@ SIGNATURES: dict[str, tuple[type, list]] = {
     "wb_amr3d_rate_mixed": (I, [P, P, I, D, D, D, D, P, P]),
     "wb_amr3d_stage": (I, [P, P, P, P, I, I, P, P, D, D, D, D, I, P, I, P, P, P, P]),
     "wb_amr3d_stage_mixed": (I, [P, P, P, P, I, I, P, P, D, D, D, D, I, P, I, P, I, P, P, P]),
-    "wb_amr3d_reflux": (I, [P, P, P, P, I, P, P, P, I, P, P]),
+    "wb_amr3d_reflux_both": (I, [P, P, P, P, I, P, P, P, P, P]),
     "wb_cnn_prep": (I, [P, P, P]),
     "wb_cnn_moments": (I, [P, I64, I64, I64] + [D] * 7 + [P, I, P, P]),
     "wb_cnn_input": (I, [P, I64, I64, I64] + [D] * 7 + [P, P, P]),
@@ -95,8 +95,8 @@ SIGNATURES: dict[str, tuple[type, list]] = {
     "wb_mhd2d_stage_tiles_mixed": (I, [P, P, I64, I64] + [D] * 8 + [I, P, I, I, P, P]),
     "wb_amr2d_stage_tiles": (I, [P, P, P, P, I] + [D] * 6 + [I, P, I, P, P, P, P, P]),
     "wb_amr2d_stage_tiles_mixed": (I, [P, P, P, P, I] + [D] * 6 + [I, P, I, P, P, P, P, P]),
-    "wb_amr2d_reflux_recorded": (I, [P, P, P, P, I, P, P, I, P, P]),
-    "wb_amr2d_reflux_recorded_mixed": (I, [P, P, P, P, I, P, P, I, P, P]),
+    "wb_amr2d_reflux_both": (I, [P, P, P, P, I, P, P, P, P]),
+    "wb_amr2d_reflux_both_mixed": (I, [P, P, P, P, I, P, P, P, P]),
     "wb_amr2d_reflux": (I, [P, P, P, P, P, I] + [D] * 6 + [I, P, I, P, P]),
     "wb_amr2d_reflux_mixed": (I, [P, P, P, P, P, I] + [D] * 6 + [I, P, I, P, P]),
 }

@@ -312,6 +312,8 @@ class AMRMHD2D:
         self._rows8 = self._dev(rows8.reshape(-1), torch.int32)
         self._rec_first = self._dev(first, torch.int32)
         self._rec_lists = self._dev(flat, torch.int32)
+        # the recorded fluxes of stage 1 and stage 2
+        self._rec1 = torch.zeros(6 * max(len(faces), 1), dtype=torch.float64, device=self.device)
         self._rec = torch.zeros(6 * max(len(faces), 1), dtype=torch.float64, device=self.device)
 
     # -- state ---------------------------------------------------------------------------
@@ -405,18 +407,20 @@ class AMRMHD2D:
         diff = (self.resistivity, *self.conduction)
         rows, groups, n = self._reflux
         if self.kernel == "tiles":
-            # the stage records the fluxes of the refluxing's faces, which reads them
-            rec = ((_lib.ptr(self._rec_first), _lib.ptr(self._rec_lists), _lib.ptr(self._rec))
+            # the stage records the fluxes of the refluxing's faces (each stage in its
+            # own record), which the refluxing reads
+            record = self._rec if heun else self._rec1
+            rec = ((_lib.ptr(self._rec_first), _lib.ptr(self._rec_lists), _lib.ptr(record))
                    if n else (None, None, None))
             _lib.call("wb_amr2d_stage_tiles" + suffix, _lib.ptr(src), _lib.ptr(dst),
                       _lib.ptr(self._patches), _lib.ptr(self._tiles), self.ntiles,
                       *self.mean_field, self.gamma, *diff, code, _lib.ptr(self._dt), int(heun),
                       self._bad.data_ptr() + 4 * slot, *rec, stream)
-            if n:  # stage 1 records into the register, stage 2 refluxes (mhd.py:911-919,989)
-                _lib.call("wb_amr2d_reflux_recorded" + suffix, _lib.ptr(dst),
+            if n and heun:  # both stages' register and the reflux (mhd.py:911-919,989)
+                _lib.call("wb_amr2d_reflux_both" + suffix, _lib.ptr(dst),
                           _lib.ptr(self._patches), _lib.ptr(self._rows8), _lib.ptr(groups), n,
-                          _lib.ptr(self._rec), _lib.ptr(self._dt), int(not heun),
-                          _lib.ptr(self._reg), stream)
+                          _lib.ptr(self._rec1), _lib.ptr(self._rec), _lib.ptr(self._dt),
+                          stream)
         else:
             _lib.call("wb_amr2d_stage" + suffix, _lib.ptr(src), _lib.ptr(dst),
                       _lib.ptr(self._patches), _lib.ptr(self._items), self.nitems,

@@ -91,10 +91,11 @@ class AMRMHD3D:
         self._reflux_cells = self._bytes(t.reflux_cells) if len(t.reflux_cells) else self._blocks
         self.nreflux = len(t.reflux_cells)
         # the flux register: half the first stage's correction of each refluxed cell
-        self._reg = torch.zeros(5 * max(self.nreflux, 1), dtype=torch.float64, device=self.device)
         # the coarse-fine block faces whose fluxes the stage records for the refluxing
         slots, nslots = amr3d_maps.face_slots(self.nblocks, t.reflux_items)
         self._rec_slots = torch.as_tensor(slots.reshape(-1), device=self.device)
+        # the fluxes of the coarse-fine block faces each stage solves (stage 1, stage 2)
+        self._rec1 = torch.zeros(max(nslots, 1) * 256 * 5, dtype=torch.float64, device=self.device)
         self._rec = torch.zeros(max(nslots, 1) * 256 * 5, dtype=torch.float64, device=self.device)
         self._hoods = torch.as_tensor(t.hoods, device=self.device)
         # the blocks with a missing neighbour on their level: with direct reads, the only
@@ -222,18 +223,21 @@ class AMRMHD3D:
         if marks is not None:
             marks[0].record()
         extra = () if self.precision == "fp64" else (int(self.waves),)
-        # the stage records the fluxes of the coarse-fine block faces for the refluxing
-        rec = (_lib.ptr(self._rec_slots), _lib.ptr(self._rec)) if self.nreflux else (None, None)
+        # the stage records the fluxes of the coarse-fine block faces for the refluxing,
+        # each stage in its own record
+        record = self._rec if heun else self._rec1
+        rec = (_lib.ptr(self._rec_slots), _lib.ptr(record)) if self.nreflux else (None, None)
         _lib.call("wb_amr3d_stage" + suffix, _lib.ptr(src), _lib.ptr(dst), _lib.ptr(self._blocks),
                   None, self.nblocks, int(self.direct), _lib.ptr(self._lookup),
                   _lib.ptr(self._levels), *self.mean_field, self.gamma, code, _lib.ptr(self._dt),
                   int(heun), self._bad.data_ptr() + 4 * slot, *extra, *rec, stream)
         if marks is not None:
             marks[1].record()
-        if self.nreflux:  # stage 1 records into the register, stage 2 refluxes
-            _lib.call("wb_amr3d_reflux", _lib.ptr(dst), _lib.ptr(self._blocks),
+        if self.nreflux and heun:  # both stages' refluxing after the Heun average
+            _lib.call("wb_amr3d_reflux_both", _lib.ptr(dst), _lib.ptr(self._blocks),
                       _lib.ptr(self._reflux_items), _lib.ptr(self._reflux_cells), self.nreflux,
-                      *rec, _lib.ptr(self._dt), int(not heun), _lib.ptr(self._reg), stream)
+                      _lib.ptr(self._rec_slots), _lib.ptr(self._rec1), _lib.ptr(self._rec),
+                      _lib.ptr(self._dt), stream)
         # after stage 1 the potential is restricted (mhd3d.py:740-741), after the Heun
         # average the covered cells are averaged down and the potential restricted
         for L in range(self.nlev - 2, -1, -1):

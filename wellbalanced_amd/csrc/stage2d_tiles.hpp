@@ -337,18 +337,21 @@ inline int launch_stage2d_tiles(const Stage2Args& a, const void* patches, const 
 // The flux register and refluxing from the fluxes the tiled stage recorded (the
 // arithmetic of k_reflux2d on the same values): per group (coarse cell) rows of 8 int32
 // (coarse patch, cell a, cell b, axis, side, slot of the coarse face, slots of the two
-// fine faces); record != 0: reg = 0.5 d; otherwise cell += reg + 0.5 d.  Internal linkage:
-// each translation unit compiles it with its own contraction rule, as it does the stage.
+// fine faces), from the fluxes of stage 1 (rec1) and stage 2 (rec) after the Heun average:
+// cell += reg + 0.5 d2, reg = 0.5 d1 being what the register holds after stage 1 (both
+// stages' refluxing in one launch; the product by 0.5 is exact, so the sum rounds as
+// with a register kept in memory).  Internal linkage: each translation unit compiles it
+// with its own contraction rule, as it does the stage.
 static __global__ void __launch_bounds__(64)
 k_reflux2d_recorded(double* __restrict__ dst, const PatchDesc* __restrict__ patches,
                     const int* __restrict__ rows, const int* __restrict__ groups, int ngroups,
-                    const double* __restrict__ rec, const double* __restrict__ dtp, int record,
-                    double* __restrict__ reg) {
+                    const double* __restrict__ rec1, const double* __restrict__ rec,
+                    const double* __restrict__ dtp) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ngroups) return;
   const double dt = *dtp;
   const int first = groups[2 * g], count = groups[2 * g + 1];
-  double delta[6] = {0, 0, 0, 0, 0, 0};
+  double delta[6] = {0, 0, 0, 0, 0, 0}, delta1[6] = {0, 0, 0, 0, 0, 0};
   int q = 0, ca = 0, cb = 0;
   for (int k = 0; k < count; ++k) {
     const int* r = rows + 8 * (size_t)(first + k);
@@ -358,34 +361,38 @@ k_reflux2d_recorded(double* __restrict__ dst, const PatchDesc* __restrict__ patc
     const int axis = r[3], side = r[4];
     const PatchDesc cq = patches[q];
     const double dtd = axis == 0 ? dt / cq.dx : dt / cq.dy;
-    const double* fc = rec + 6 * (size_t)r[5];
-    const double* f0 = rec + 6 * (size_t)r[6];
-    const double* f1 = rec + 6 * (size_t)r[7];
-    for (int f = 0; f < 6; ++f) {
-      const double mean = 0.5 * (f0[f] + f1[f]);
-      const double d = side == 0 ? dtd * (fc[f] - mean) : dtd * (mean - fc[f]);
-      delta[f] = delta[f] + d;
-    }
-  }
-  double* kept = reg + 6 * (size_t)g;
-  if (record) {
-    for (int f = 0; f < 6; ++f) kept[f] = 0.5 * delta[f];
-    return;
+    auto add = [&](const double* fl, double* sum) {
+      const double* fc = fl + 6 * (size_t)r[5];
+      const double* f0 = fl + 6 * (size_t)r[6];
+      const double* f1 = fl + 6 * (size_t)r[7];
+      for (int f = 0; f < 6; ++f) {
+        const double mean = 0.5 * (f0[f] + f1[f]);
+        const double d = side == 0 ? dtd * (fc[f] - mean) : dtd * (mean - fc[f]);
+        sum[f] = sum[f] + d;
+      }
+    };
+    add(rec, delta);
+    add(rec1, delta1);
   }
   const PatchDesc cq = patches[q];
   const size_t PL = (size_t)(cq.nx + 4) * (cq.ny + 4);
   const size_t c = cq.off + (size_t)ca * (cq.ny + 4) + cb;
-  for (int f = 0; f < 6; ++f) dst[c + f * PL] = dst[c + f * PL] + (kept[f] + 0.5 * delta[f]);
+  for (int f = 0; f < 6; ++f) {
+    const double reg = 0.5 * delta1[f];
+    dst[c + f * PL] = dst[c + f * PL] + (reg + 0.5 * delta[f]);
+  }
 }
 
-inline int launch_reflux2d_recorded(double* dst, const void* patches, const int32_t* rows,
-                                    const int32_t* groups, int ngroups, const double* rec,
-                                    const double* dt, int record, double* reg, hipStream_t s) {
-  if (ngroups <= 0) return 0;
-  if (rows == nullptr || groups == nullptr || rec == nullptr || reg == nullptr) return WB_EINVAL;
+inline int launch_reflux2d_both(double* dst, const void* patches, const int32_t* rows,
+                                const int32_t* groups, int ngroups, const double* rec1,
+                                const double* rec2, const double* dt, hipStream_t s) {
+  if (ngroups < 0) return WB_EINVAL;
+  if (ngroups == 0) return 0;
+  if (rows == nullptr || groups == nullptr || rec1 == nullptr || rec2 == nullptr)
+    return WB_EINVAL;
   hipLaunchKernelGGL(k_reflux2d_recorded, dim3((unsigned)((ngroups + 63) / 64)), dim3(64), 0, s,
                      dst, (const PatchDesc*)patches, (const int*)rows, (const int*)groups,
-                     ngroups, rec, dt, record, reg);
+                     ngroups, rec1, rec2, dt);
   return wb_last_error();
 }
 

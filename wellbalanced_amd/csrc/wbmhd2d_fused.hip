@@ -123,12 +123,11 @@ extern "C" int wb_amr2d_stage_tiles(const double* in, double* out, const void* p
                                           rec_first, rec_lists, rec);
 }
 
-extern "C" int wb_amr2d_reflux_recorded(double* dst, const void* patches, const int32_t* rows,
-                                        const int32_t* groups, int ngroups, const double* rec,
-                                        const double* dt, int record, double* reg,
-                                        void* stream) {
-  return launch_reflux2d_recorded(dst, patches, rows, groups, ngroups, rec, dt, record, reg,
-                                  (hipStream_t)stream);
+extern "C" int wb_amr2d_reflux_both(double* dst, const void* patches, const int32_t* rows,
+                                    const int32_t* groups, int ngroups, const double* rec1,
+                                    const double* rec2, const double* dt, void* stream) {
+  return launch_reflux2d_both(dst, patches, rows, groups, ngroups, rec1, rec2, dt,
+                              (hipStream_t)stream);
 }
 
 extern "C" int wb_amr2d_reflux(const double* src, double* dst, const void* patches,

@@ -534,11 +534,12 @@ namespace {
 // The flux register and refluxing (FluxRegister.update with scale 0.5 dt and
 // FluxRegister.reflux as MHD3D.step calls them, mhd3d.py:650-655,745-750): per
 // coarse-fine block face, the 8 x 8 coarse cells across it trade the flux of their face
-// on the fine side for the mean of the four fine fluxes.  Both are the fluxes the stage
-// solved on those block faces (recorded by the stage, `rec` with the blocks' 6 face
-// slots `slots`): d = sum over the cell's faces of dt/h (F_c - mean) or dt/h (mean - F_c).
-// record (after stage 1): reg[5 t + f] = 0.5 d; otherwise (after stage 2 and its Heun
-// average) the cell gains reg + 0.5 d.  An item is one such face (pad = 2 axis + side;
+// on the fine side for the mean of the four fine fluxes.  Both are the fluxes a stage
+// solved on those block faces (recorded by the stage with the blocks' 6 face slots
+// `slots`: stage 1 in rec1, stage 2 in rec): d = sum over the cell's faces of dt/h (F_c -
+// mean) or dt/h (mean - F_c).  After stage 2 and its Heun average the cell gains
+// reg + 0.5 d2, reg = 0.5 d1 being what the register holds after stage 1: both stages'
+// refluxing in one launch.  An item is one such face (pad = 2 axis + side;
 // side 0: the coarse cells at the fine block's low face, their +axis face, the coarse
 // block's high face; side 1: at the fine block's high face, the coarse block's low face).
 // `cells` lists, per coarse cell, its n (1 to 3) faces as (item, lane) pairs in
@@ -549,15 +550,17 @@ __global__ void __launch_bounds__(256) k_reflux3d(const BlockDesc* __restrict__ 
                                                   const RefluxItem* __restrict__ items,
                                                   const RefluxCell* __restrict__ cells, int ncells,
                                                   const int* __restrict__ slots,
+                                                  const double* __restrict__ rec1,
                                                   const double* __restrict__ rec,
                                                   const double* __restrict__ dtp,
-                                                  double* __restrict__ dst, int record,
-                                                  double* __restrict__ reg) {
+                                                  double* __restrict__ dst) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ncells) return;
   const RefluxCell cl = cells[t];
   const double dt = *dtp;
-  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  // acc1, acc: the faces' corrections from the fluxes stage 1 (rec1) and stage 2 (rec)
+  // recorded, summed alike
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, acc1[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   int c[3] = {0, 0, 0};
   BlockDesc cd{};
   for (int f = 0; f < cl.n; ++f) {
@@ -575,28 +578,30 @@ __global__ void __launch_bounds__(256) k_reflux3d(const BlockDesc* __restrict__ 
     cd = blocks[item.coarse];
     const int cs = slots[6 * item.coarse + 2 * axis + (1 - side)];
     const int fs = slots[6 * item.fine + 2 * axis + side];
-    const double* fc = rec + ((size_t)cs * (B * B) + (c[a1] - 2) * B + (c[a2] - 2)) * 5;
-    const double* ff[4];
+    const size_t oc = ((size_t)cs * (B * B) + (c[a1] - 2) * B + (c[a2] - 2)) * 5;
+    size_t of[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-      ff[k] = rec + ((size_t)fs * (B * B) + (2 * u + (k >> 1)) * B + 2 * v + (k & 1)) * 5;
+      of[k] = ((size_t)fs * (B * B) + (2 * u + (k >> 1)) * B + 2 * v + (k & 1)) * 5;
     const double dtd = axis == 0 ? dt / cd.dx : (axis == 1 ? dt / cd.dy : dt / cd.dz);
+    auto add = [&](const double* r, double* sum) {
 #pragma unroll
-    for (int fld = 0; fld < 5; ++fld) {
-      const double fm = ((ff[0][fld] + ff[1][fld]) + (ff[2][fld] + ff[3][fld])) * 0.25;
-      const double d = side == 0 ? dtd * (fc[fld] - fm) : dtd * (fm - fc[fld]);
-      acc[fld] = acc[fld] + d;
-    }
+      for (int fld = 0; fld < 5; ++fld) {
+        const double fm =
+            ((r[of[0] + fld] + r[of[1] + fld]) + (r[of[2] + fld] + r[of[3] + fld])) * 0.25;
+        const double d = side == 0 ? dtd * (r[oc + fld] - fm) : dtd * (fm - r[oc + fld]);
+        sum[fld] = sum[fld] + d;
+      }
+    };
+    add(rec, acc);
+    add(rec1, acc1);
   }
-  double* kept = reg + 5 * (size_t)t;
-  if (record) {
-    for (int fld = 0; fld < 5; ++fld) kept[fld] = 0.5 * acc[fld];
-    return;
-  }
+  // the cell gains the register (stage 1's half correction) and stage 2's half
   double* out = dst + cd.off;
   for (int fld = 0; fld < 5; ++fld) {
     const int o = cell_at(fld, c[0], c[1], c[2]);
-    out[o] = out[o] + (kept[fld] + 0.5 * acc[fld]);
+    const double reg = 0.5 * acc1[fld];
+    out[o] = out[o] + (reg + 0.5 * acc[fld]);
   }
 }
 
@@ -724,16 +729,17 @@ int wb_amr3d_stage(const double* in, double* out, const void* blocks, const int3
   return wb_last_error();
 }
 
-int wb_amr3d_reflux(double* dst, const void* blocks, const void* items, const void* cells,
-                    int ncells, const int32_t* rec_slots, const double* rec, const double* dt,
-                    int record, double* reg, void* stream) {
-  if (ncells < 0 || (ncells > 0 && (reg == nullptr || rec_slots == nullptr || rec == nullptr)))
+int wb_amr3d_reflux_both(double* dst, const void* blocks, const void* items, const void* cells,
+                         int ncells, const int32_t* rec_slots, const double* rec1,
+                         const double* rec2, const double* dt, void* stream) {
+  if (ncells < 0 ||
+      (ncells > 0 && (rec_slots == nullptr || rec1 == nullptr || rec2 == nullptr)))
     return WB_EINVAL;
   if (ncells == 0) return 0;
   hipLaunchKernelGGL(k_reflux3d, dim3((unsigned)((ncells + 255) / 256)), dim3(256), 0,
                      (hipStream_t)stream, (const BlockDesc*)blocks, (const RefluxItem*)items,
-                     (const RefluxCell*)cells, ncells, (const int*)rec_slots, rec, dt, dst,
-                     record, reg);
+                     (const RefluxCell*)cells, ncells, (const int*)rec_slots, rec1, rec2, dt,
+                     dst);
   return wb_last_error();
 }
 

@@ -1,6 +1,6 @@
 // The refined 3D hierarchy's stage with the FP32 reconstruction and HLLD (the mixed
 // path, built with -ffp-contract=on); its refluxing reads the fluxes the stage recorded
-// (wb_amr3d_reflux, shared with the FP64 path).
+// (wb_amr3d_reflux_both, shared with the FP64 path).
 #include "amr3d.hpp"
 
 using namespace wb;
