@@ -188,3 +188,26 @@ def test_advance_keeps_dt_on_the_device(gpu):
     b.advance(3)
     assert b.time == a.time == sum([0.0] + times) or abs(b.time - a.time) == 0.0
     np.testing.assert_array_equal(a.arrays()[0], b.arrays()[0])
+
+
+@pytest.mark.parametrize("limiter", ["minmod", "mc"])
+def test_mixed_stage_launch_shapes_keep_the_bits(gpu, limiter):
+    """The mixed fused stage at 2 and 3 waves per SIMD (3: each Heun stage with its own
+    set of early loads) and with segments of 8 and 32 planes leaves the same cells and
+    potentials, bit for bit, over steps at one step size."""
+    problem = initial.turbulence_3d(32, seed=4, limiter=limiter)
+    results = {}
+    dt = None
+    for waves, nxs in [(3, 32), (3, 8), (2, 32), (2, 8)]:
+        model = MHD3D.from_problem(problem, kernels="fused", precision="mixed")
+        model.waves, model.nxs = waves, nxs
+        dt = dt or 0.5 * model.stable_step()
+        for _ in range(3):
+            model.step(dt)
+        model.check()
+        results[waves, nxs] = model.arrays()
+    cons0, pots0 = results[3, 32]
+    for key, (cons, pots) in results.items():
+        np.testing.assert_array_equal(cons, cons0, err_msg=str(key))
+        for c in range(3):
+            np.testing.assert_array_equal(pots[c], pots0[c], err_msg=str(key))

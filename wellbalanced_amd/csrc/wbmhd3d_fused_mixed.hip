@@ -24,7 +24,14 @@ extern "C" int wb_mhd3d_stage_mixed(const double* in, double* out, int64_t nx, i
   hipStream_t s = (hipStream_t)stream;
   return with_limiter(limiter, [&](auto lim) {
     constexpr int L = decltype(lim)::value;
-    if (waves == 3) return launch_stage3d<float, 12, 16, 3, L>(a, s);
+    // at 3 waves per SIMD each stage takes its own early loads: stage 1 (no `out` read)
+    // those of `in` only, stage 2 also the Heun average's cells through LDS and its edges
+    // early (256^3 turbulence: 1292 and 1487 us against 1309 and 1499 with all of them
+    // in registers)
+    constexpr int IN = EARLY_IN_CELLS | EARLY_IN_EDGES;
+    if (waves == 3 && heun)
+      return launch_stage3d<float, 12, 16, 3, L, IN | LDS_OUT_CELLS | EARLY_OUT_EDGES>(a, s);
+    if (waves == 3) return launch_stage3d<float, 12, 16, 3, L, IN>(a, s);
     return launch_stage3d<float, 12, 16, 2, L>(a, s);
   });
 }
