@@ -37,14 +37,15 @@ int wb_amr3d_stage_mixed(const double* in, double* out, const void* blocks, cons
     const dim3 halves(((nids + 7) / 8) * 16), tile_h(Tile<B / 2, B>::THREADS);
     const dim3 tile_b(Tile<B, B>::THREADS), blocks8(((nids + 7) / 8) * 8);
     // stage 2 takes the Heun average's cells through LDS and its edges early; stage 1,
-    // which reads no `out`, runs faster with no early loads (813 against 822 us)
+    // which reads no `out`, runs fastest with the edges of `in` early (806 us, against
+    // 814 with no early loads and 822 with stage 2's set)
     if (waves == 3 && direct && heun)
       hipLaunchKernelGGL((k_stage3d_halves<float, 3, L, true, LDS_OUT_CELLS | EARLY_OUT_EDGES>),
                          halves, tile_h, 0, s, a,
                          (const BlockDesc*)blocks, id, nids, lk, lv);
     else if (waves == 3 && direct)
-      hipLaunchKernelGGL((k_stage3d_halves<float, 3, L, true, 0>), halves, tile_h, 0, s, a,
-                         (const BlockDesc*)blocks, id, nids, lk, lv);
+      hipLaunchKernelGGL((k_stage3d_halves<float, 3, L, true, EARLY_IN_EDGES>), halves, tile_h,
+                         0, s, a, (const BlockDesc*)blocks, id, nids, lk, lv);
     else if (waves == 3)
       hipLaunchKernelGGL((k_stage3d_halves<float, 3, L, false>), halves, tile_h, 0, s, a,
                          (const BlockDesc*)blocks, id, nids, lk, lv);
