@@ -77,6 +77,26 @@ __device__ __forceinline__ void tile_body(const Stage2Args& a, int tx, int ty, S
   const double* az = a.in + 6 * PL;
   auto AZ = [&](int i, int j) -> double { return az[(size_t)i * YN + j]; };
 
+  // the stored values this thread's update starts from -- the 6 fields of cell (ci, cj)
+  // and the potential of its low-corner node, in `in` and (Heun average) in `out` --
+  // loaded before the passes so that their latency overlaps them; no other workgroup
+  // writes them
+  const int ci = tid / T, cj = tid % T;
+  const bool own_x = ci < txv || (ci == txv && last_x), own_y = cj < tyv || (cj == tyv && last_y);
+  const bool has_cell = ci < txv && cj < tyv, has_node = own_x && own_y;
+  double cin[6], cout[6], nin = 0.0, nout = 0.0;
+  {
+    const size_t c = has_cell ? (size_t)(i0 + ci) * Y + (j0 + cj) : 0;
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+      cin[r] = has_cell ? a.in[r * PL + c] : 0.0;
+      cout[r] = has_cell && a.heun ? a.out[r * PL + c] : 0.0;
+    }
+    const size_t o = has_node ? 6 * PL + (size_t)(i0 + ci) * YN + (j0 + cj) : 0;
+    nin = has_node ? a.in[o] : 0.0;
+    nout = has_node && a.heun ? a.out[o] : 0.0;
+  }
+
   // 1. primitive variables of the loaded cells (mhd_primitive_pyccel.py:51-86)
   for (int k = tid; k < NC; k += THREADS) {
     const int li = k / H - 2, lj = k % H - 2;
@@ -221,7 +241,8 @@ __device__ __forceinline__ void tile_body(const Stage2Args& a, int tx, int ty, S
   const double dt = *a.dt;
   const double dtdx = dt / a.dx, dtdy = dt / a.dy;
   const bool xlo = a.walls & 1, xhi = a.walls & 2, ylo = a.walls & 4, yhi = a.walls & 8;
-  auto node = [&](int i, int j) {
+  // node (i, j); `pre`: its stored values are vin, vout (loaded above)
+  auto node = [&](int i, int j, bool pre, double vin, double vout) {
     const int f = i0 + i, g = j0 + j;
     double e = ct_node(double(sh.fx[5][xface_index(i, j - 1)]), double(sh.fx[5][xface_index(i, j)]),
                        double(sh.fy[5][yface_index(i - 1, j)]), double(sh.fy[5][yface_index(i, j)]),
@@ -235,14 +256,12 @@ __device__ __forceinline__ void tile_body(const Stage2Args& a, int tx, int ty, S
     if ((xlo && f == 2) || (xhi && f == a.nx + 2) || (ylo && g == 2) || (yhi && g == a.ny + 2))
       e = 0.0;
     const size_t o = 6 * PL + (size_t)f * YN + g;
-    double v = a.in[o] - dt * e;
-    if (a.heun) v = (v + a.out[o]) * 0.5;
+    double v = (pre ? vin : a.in[o]) - dt * e;
+    if (a.heun) v = (v + (pre ? vout : a.out[o])) * 0.5;
     a.out[o] = v;
   };
-  const int ci = tid / T, cj = tid % T;
-  const bool own_x = ci < txv || (ci == txv && last_x), own_y = cj < tyv || (cj == tyv && last_y);
-  if (own_x && own_y) node(ci, cj);
-  if (ci < txv && cj < tyv) {
+  if (has_node) node(ci, cj, true, nin, nout);
+  if (has_cell) {
     const int f = i0 + ci, g = j0 + cj;
     constexpr int XR[6] = {0, 1, 2, 3, 4, 6};  // x-sweep rows of the 6 fields (mhd.py:134)
     constexpr int YR[6] = {0, 2, 1, 3, 4, 6};
@@ -260,8 +279,8 @@ __device__ __forceinline__ void tile_body(const Stage2Args& a, int tx, int ty, S
       const double lo = (wyl && yz) ? 0.0 : double(sh.fy[yr][yface_index(ci, cj)]);
       const double dyy = dtdy * (hi - lo);
       const size_t o = r * PL + c;
-      double v = a.in[o] - (dtdx * dfx + dyy);
-      if (a.heun) v = (v + a.out[o]) * 0.5;
+      double v = cin[r] - (dtdx * dfx + dyy);
+      if (a.heun) v = (v + cout[r]) * 0.5;
       a.out[o] = v;
     }
   }
@@ -270,7 +289,7 @@ __device__ __forceinline__ void tile_body(const Stage2Args& a, int tx, int ty, S
     for (int k = tid; k < 2 * T + 1; k += THREADS) {
       const int i = k <= T ? T : k - T - 1, j = k <= T ? k : T;
       const bool ox = i < txv || (i == txv && last_x), oy = j < tyv || (j == tyv && last_y);
-      if (ox && oy) node(i, j);
+      if (ox && oy) node(i, j, false, 0.0, 0.0);
     }
   }
 }
